@@ -1,0 +1,193 @@
+#!/usr/bin/env python3
+"""Louvain benchmark: TEPS of the full multi-level Louvain on one MI355X.
+
+One step = one complete run of the reference's pipeline (CSR build, init,
+local-moving sweeps, renumbering, coarsening at every level, flattening) on
+the R-MAT scale-22 graph (edge factor 16, a/b/c/d = .57/.19/.19/.05,
+duplicates merged into int32 weights, seed 1), with the edge list already
+resident in HBM.  value = TEPS = directed CSR entries of the input (2E) /
+seconds per step.
+
+Besides the JSON line's required fields it reports:
+  roofline     the dominant local-moving sweep kernel: algorithmic bytes
+               12*nnz + 24*N per launch (community-total reads P not counted)
+               / its mean HIP-event duration, against 8 TB/s
+  cpu_baseline the oracle (C restatement of the reference semantics, OpenMP)
+               on the same graph, timed on this host; its partition is also
+               the validation of the GPU result
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def host_threads():
+    n = os.cpu_count() or 1
+    try:
+        n = len(os.sched_getaffinity(0))
+    except Exception:
+        pass
+    return max(1, min(n, 16))
+
+
+def workload(name):
+    from louvaingpu_amd import generators
+    if name == "rmat22":
+        return "rmat22_ef16_int_seed1", generators.rmat22()
+    if name == "sbm10k":
+        return "sbm_10k_100blocks_seed0", generators.sbm_10k()
+    if name.startswith("rmat"):
+        sc = int(name[4:])
+        return "rmat%d_ef16_int_seed1" % sc, generators.rmat(sc, 16, 0.57, 0.19, 0.19, 1, "int")
+    raise SystemExit("unknown workload " + name)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="rmat22")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline / validation")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        return 2
+
+    from louvaingpu_amd import Engine
+    from louvaingpu_amd._lib import LV_SWEEP_TIERS
+
+    wname, (n, src, dst, w) = workload(args.workload)
+    e = len(src)
+    nnz0 = 2 * e
+    print("[bench] workload %s n=%d e=%d nnz=%d" % (wname, n, e, nnz0), file=sys.stderr, flush=True)
+
+    eng = Engine(timing=True)
+    t_load0 = time.perf_counter()
+    eng.load_edges(n, src, dst, w)
+    t_load = time.perf_counter() - t_load0
+    for i in range(args.warmup):
+        eng.run()
+        print("[bench] warmup %d: %.2f ms" % (i, eng.last_run_ms()), file=sys.stderr, flush=True)
+    eng.reset_stats()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        eng.run()
+    t1 = time.perf_counter()
+    ms = (t1 - t0) * 1000.0 / max(args.steps, 1)
+    st = eng.stats()
+    levels = eng.num_levels()
+    infos = [eng.level_info(l) for l in range(levels)]
+    final = eng.final_assignment()
+    q_gpu = eng.final_modularity()
+
+    # PCIe-inclusive: host edge list -> device -> dendrogram and final partition on the host
+    tp0 = time.perf_counter()
+    eng.load_edges(n, src, dst, w)
+    eng.run()
+    for l in range(levels):
+        eng.level_partition(l)
+    eng.final_assignment()
+    pcie_s = time.perf_counter() - tp0
+
+    # dominant sweep kernel (tier with the largest device time)
+    tier_ms = [st.tier_ms[t] for t in range(LV_SWEEP_TIERS)]
+    dom = int(np.argmax(tier_ms))
+    launches = max(1, st.tier_launches[dom])
+    avg_ms = st.tier_ms[dom] / launches
+    bytes_per_launch = st.tier_bytes[dom] / launches
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    sweep_gbs = st.sweep_bytes / (st.sweep_ms * 1e-3) / 1e9 if st.sweep_ms > 0 else 0.0
+    sweep_teps = st.sweep_entries / (st.sweep_ms * 1e-3) if st.sweep_ms > 0 else 0.0
+    steps = max(args.steps, 1)
+
+    out = {
+        "metric": "Louvain TEPS & end-to-end s at oracle modularity; batched vs per-graph",
+        "value": nnz0 / (ms * 1e-3),
+        "unit": "TEPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int64",
+        "data": "synthetic",
+        "config": {
+            "workload": wname,
+            "n": n, "e": e, "nnz": nnz0,
+            "levels": levels,
+            "passes": [i.passes for i in infos],
+            "level_n": [i.n for i in infos],
+            "level_nnz": [i.nnz for i in infos],
+            "parallelism": "single-gpu",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "sweep tier %d" % (dom + 1),
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "bytes_per_launch": bytes_per_launch,
+            "avg_launch_ms": avg_ms,
+        },
+        "e2e_s": ms * 1e-3,
+        "pcie_inclusive_s": pcie_s,
+        "load_s": t_load,
+        "phase_ms_per_step": {
+            "csr": st.csr_ms / steps, "init": st.init_ms / steps, "sweep": st.sweep_ms / steps,
+            "apply": st.apply_ms / steps, "renumber": st.renumber_ms / steps,
+            "coarsen": st.coarsen_ms / steps, "flatten": st.flatten_ms / steps,
+        },
+        "sweep_gbs": sweep_gbs,
+        "sweep_teps": sweep_teps,
+        "tier_ms_per_step": [t / steps for t in tier_ms],
+        "modularity": q_gpu,
+        "communities": int(final.max()) + 1,
+    }
+
+    if not args.no_cpu:
+        import oracle
+        threads = args.cpu_threads or host_threads()
+        c0 = time.perf_counter()
+        r = oracle.run(n, src, dst, w, threads=threads)
+        cpu_s = time.perf_counter() - c0
+        same = bool(np.array_equal(r.final_comm(), final))
+        q_or = r.final_modularity()
+        out["cpu_baseline"] = {
+            "value": nnz0 / cpu_s,
+            "unit": "TEPS",
+            "cores": threads,
+            "kind": "port",
+            "sample": "oracle (C restatement, OpenMP) full multi-level run on the same %s graph, "
+                      "%.1f s" % (wname, cpu_s),
+        }
+        out["validated"] = same and abs(q_or - q_gpu) <= 1e-9 * max(1.0, abs(q_or))
+        out["oracle_modularity"] = q_or
+        out["speedup_vs_cpu"] = cpu_s / (ms * 1e-3)
+    else:
+        out["cpu_baseline"] = None
+        out["validated"] = None
+    eng.close()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,183 @@
+/*
+ * louvain_amd.h - C ABI of the MI355X Louvain engine (liblouvain_amd.so).
+ *
+ * The reference drives its GPU pipeline through six extern C++ functions that
+ * share one mutable StatusCUDA/CUDAGraph pair (LouvainCUDA/LouvainCPP.cpp:129-134)
+ * plus AllocateMemory/DeAllocateMemory (LouvainCUDA/LouvainCPP.cpp:136-205).
+ * This header replaces that boundary with plain C: an opaque context owns one
+ * HBM arena, one HIP stream and the per-level device state; every function
+ * takes plain pointers and sizes and returns an lv_status code.  No exception,
+ * abort or host allocation crosses the ABI: host output buffers belong to the
+ * caller and the library copies into them.
+ *
+ * Semantics are the reference's (synchronous sweeps, singleton guard,
+ * one-pass-late convergence, merge-isolated post-step, first-occurrence
+ * renumbering, ordered-pair coarsening), with exact arithmetic: all weight
+ * sums are int64 in units of 2^-s (s = the smallest exponent that makes every
+ * input weight an integer), gains are FP64 evaluated as W - ((gamma*deg)*f).
+ *
+ * Threading: one context is used by one host thread at a time; contexts are
+ * independent (no globals).
+ */
+#ifndef LOUVAIN_AMD_H
+#define LOUVAIN_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct lv_ctx lv_ctx;
+
+/* Status codes.  The reference returns none (CUDA errors are printed only in
+ * AllocateMemory, LouvainCUDA/LouvainCPP.cpp:140-151). */
+enum {
+    LV_OK = 0,
+    LV_ERR_NOMEM = -1,         /* arena or device memory exhausted */
+    LV_ERR_INVALID_INPUT = -2, /* unknown/repeated node id, negative or non-finite weight */
+    LV_ERR_RANGE = -3,         /* weight sums exceed the exact int64 range */
+    LV_ERR_ARG = -4,           /* bad argument or call out of order */
+    LV_ERR_DEVICE = -5,        /* HIP runtime error */
+    LV_ERR_NO_TRACE = -6       /* trace requested without LV_FLAG_TRACE */
+};
+
+/* weight element types for lv_load_edges */
+enum { LV_WEIGHT_INT32 = 0, LV_WEIGHT_FLOAT32 = 1 };
+
+/* lv_config.flags */
+enum {
+    LV_FLAG_TRACE = 1,  /* keep every pass's next[] on the host (parity tests) */
+    LV_FLAG_TIMING = 2  /* record HIP events around every phase and sweep launch */
+};
+
+typedef struct {
+    double gamma;        /* resolution; the reference has none (gamma = 1) */
+    double min_gain;     /* convergence threshold, reference __MIN = 1e-7
+                            (LouvainCUDA/LouvainCPP.cpp:55) */
+    int32_t max_passes;  /* 0 = unbounded, the reference's __PASS_MAX -1 */
+    int32_t max_levels;  /* 0 = unbounded */
+    int32_t device;      /* HIP device ordinal (the reference uses device 0,
+                            LouvainCUDA/LouvainCPP.cpp:774) */
+    int32_t flags;       /* LV_FLAG_* */
+    uint64_t arena_bytes; /* 0 = sized from the graph at lv_load_edges */
+} lv_config;
+
+/* Per-level summary (the reference keeps only Partition{count,...}). */
+typedef struct {
+    int32_t n;           /* vertices of the level graph */
+    int32_t k;           /* communities after renumbering (0 before lv_renumber) */
+    int64_t nnz;         /* directed CSR entries the sweep reads */
+    int64_t m_units;     /* total weight m in units */
+    int32_t passes;      /* sweeps run */
+    int32_t scale;       /* weight unit exponent s */
+    double q;            /* value oneLevel returned */
+    double lower;        /* clamp passed to oneLevel */
+} lv_level_info;
+
+/* Timing counters, filled when LV_FLAG_TIMING is set (milliseconds of device
+ * time between HIP events on the context stream). */
+#define LV_SWEEP_TIERS 8
+typedef struct {
+    double csr_ms, init_ms, sweep_ms, apply_ms, renumber_ms, coarsen_ms, flatten_ms;
+    double tier_ms[LV_SWEEP_TIERS];
+    int64_t tier_launches[LV_SWEEP_TIERS];
+    double tier_bytes[LV_SWEEP_TIERS];     /* algorithmic bytes 12*nnz+24*N+8*P */
+    double sweep_bytes;                    /* sum over tiers */
+    int64_t sweep_entries;                 /* sum of nnz over passes */
+    int32_t levels, passes;
+} lv_stats;
+
+void lv_config_default(lv_config *cfg);
+const char *lv_strerror(int status);
+
+/* Context with its stream and arena.  Replaces AllocateMemory
+ * (LouvainCUDA/LouvainCPP.cpp:136-198) and the cached_allocator
+ * (LouvainCUDA/cached_allocator.h:12-108). */
+int lv_create(lv_ctx **out, const lv_config *cfg);
+/* Replaces DeAllocateMemory (LouvainCUDA/LouvainCPP.cpp:200-205). */
+int lv_destroy(lv_ctx *ctx);
+
+/* Input: node ids (NULL means 0..n-1) and one entry per edge (src id, dst id,
+ * weight).  Replaces Graph::setNodes/setEdges plus the H2D copies of
+ * LouvainCUDA/LouvainCPP.cpp:283-287.  Ids are mapped to positions on the
+ * device; an edge naming an unknown id, a repeated id, or a negative or
+ * non-finite weight gives LV_ERR_INVALID_INPUT.  The edge list stays resident
+ * in the arena; each lv_run / level-0 lv_neighbour_compute starts from it. */
+int lv_load_edges(lv_ctx *ctx, int32_t n, const int32_t *ids, int64_t e, const int32_t *src,
+                  const int32_t *dst, const void *w, int wtype);
+
+/* Phase entry points, in the reference's call order
+ * (LouvainCUDA/LouvainCPP.cpp:334-499). */
+
+/* CalculateNeighbour (LouvainCUDA/NeighbourCompute.cu:42-118): at level 0
+ * builds the symmetric CSR of the loaded edge list and m; at later levels the
+ * CSR was produced by lv_induce and this only checks the call order. */
+int lv_neighbour_compute(lv_ctx *ctx);
+/* initCUDA (LouvainCUDA/InitKernel.cu:128-222): degrees, self-loop weights,
+ * singleton communities, degree buckets for the sweep. */
+int lv_init(lv_ctx *ctx);
+/* modularityCUDA (LouvainCUDA/OneLevelKernel.cu:965-980): Q of the current
+ * community state. */
+int lv_modularity(lv_ctx *ctx, double *q);
+/* oneLevelCUDA (LouvainCUDA/OneLevelKernel.cu:518-954): the sweep loop,
+ * merge-isolated post-step; *q receives the level's returned modularity.
+ * `lower` is the clamp of LouvainCUDA/OneLevelKernel.cu:895. */
+int lv_one_level(lv_ctx *ctx, double lower, double *q, int32_t *passes);
+/* renumberCUDA (LouvainCUDA/RenumberKernel.cu:63-220): first-occurrence
+ * renumbering; appends the level to the device-resident dendrogram.  The host
+ * Partition arrays are fetched with lv_level_partition. */
+int lv_renumber(lv_ctx *ctx, int32_t *k);
+/* inducedGraphCUDA (LouvainCUDA/InducedGraphKernel.cu:50-113): coarsens the
+ * level graph into the next level's CSR (radix sort + segmented reduce). */
+int lv_induce(lv_ctx *ctx);
+
+/* generate_dendogram's level loop (LouvainCUDA/LouvainCPP.cpp:209-553) run
+ * entirely on the device, with the same termination rule; the dendrogram
+ * stays in HBM.  *levels receives its length. */
+int lv_run(lv_ctx *ctx, int32_t *levels);
+
+/* Results. */
+int lv_num_levels(lv_ctx *ctx, int32_t *levels);
+int lv_level_info_get(lv_ctx *ctx, int32_t level, lv_level_info *out);
+/* The level's Partition in the reference's layout: (node, community) listed in
+ * order of (old community, position) (LouvainCUDA/RenumberKernel.cu:121-133,
+ * 191,202-203).  node/comm hold info.n entries. */
+int lv_level_partition(lv_ctx *ctx, int32_t level, int32_t *node, int32_t *comm);
+/* Renumbered community of every position of the level (n entries). */
+int lv_level_assignment(lv_ctx *ctx, int32_t level, int32_t *comm);
+/* partitionAtLevel (LouvainCUDA/LouvainCPP.cpp:82-106): level-0 Partition
+ * order, communities mapped through levels 1..level. */
+int lv_flatten(lv_ctx *ctx, int32_t level, int32_t *node, int32_t *comm);
+/* Final community of each level-0 position, flattened at the last level. */
+int lv_final_assignment(lv_ctx *ctx, int32_t *comm);
+/* Newman modularity of the final partition on the input graph (exact integer
+ * aggregates, evaluated in FP64). */
+int lv_final_modularity(lv_ctx *ctx, double *q);
+
+/* Parity instrumentation. */
+/* next[] proposed by `pass` (1-based) of `level`, after the guard (and, on the
+ * level's last pass, after the merge-isolated step); needs LV_FLAG_TRACE. */
+int lv_trace_next(lv_ctx *ctx, int32_t level, int32_t pass, int32_t *next);
+/* Q measured at the start of `pass` (the value the break test compares). */
+int lv_trace_pass_q(lv_ctx *ctx, int32_t level, int32_t pass, double *q);
+/* Current level's CSR after lv_neighbour_compute/lv_induce: rowptr[n+1],
+ * col/w[nnz] (self entries are kept per vertex, not in the rows). */
+int lv_level_csr(lv_ctx *ctx, int64_t *rowptr, int32_t *col, int64_t *w);
+/* Current level's per-vertex degree k, last self-loop weight and sum of
+ * self-loop weights (w==0 counted as 1), in units; needs lv_init. */
+int lv_level_vertex(lv_ctx *ctx, int64_t *k, int64_t *loop, int64_t *selfsum);
+/* Current level's sizes before lv_init/lv_one_level. */
+int lv_current_level(lv_ctx *ctx, int32_t *level, int32_t *n, int64_t *nnz);
+
+int lv_get_stats(lv_ctx *ctx, lv_stats *out);
+int lv_reset_stats(lv_ctx *ctx);
+/* Wall time of the last lv_run in milliseconds (host clock, device synced). */
+int lv_last_run_ms(lv_ctx *ctx, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LOUVAIN_AMD_H */

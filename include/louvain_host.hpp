@@ -1,0 +1,137 @@
+/*
+ * louvain_host.hpp - C++ host API of liblouvain_host.so.
+ *
+ * Mirrors the reference's host types and level loop so that a caller of the
+ * reference can switch over unchanged:
+ *   Edge, EdgeContainer   LouvainCUDA/Edge.h:26-69
+ *   Graph                 LouvainCUDA/Graph.h:39-130 (storage and accessors; the
+ *                         visualisation fields x/y/cluster are not part of the path)
+ *   Community, Partition  LouvainCUDA/community.h:16-20, LouvainCUDA/partition.h:14-29
+ *   generate_dendogram    LouvainCUDA/LouvainCPP.cpp:209-553
+ *   partitionAtLevel, getPartitions, countPartitions
+ *                         LouvainCUDA/LouvainCPP.cpp:82-127
+ * generate_dendogram drives the device engine only through the C ABI in
+ * louvain_amd.h, phase by phase, in the reference's call order.
+ *
+ * Errors: the reference reports none; here a failing engine call throws
+ * louvain::Error (a std::runtime_error carrying the lv_status code), and
+ * getPartitions keeps the reference's `throw const char*` for a bad level.
+ */
+#ifndef LOUVAIN_HOST_HPP
+#define LOUVAIN_HOST_HPP
+
+#include <cstdint>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "louvain_amd.h"
+
+struct Edge {
+    int target, source;
+    float weight;
+};
+
+class EdgeContainer {
+public:
+    EdgeContainer() {}
+    ~EdgeContainer();
+    EdgeContainer(const EdgeContainer &) = delete;
+    EdgeContainer &operator=(const EdgeContainer &) = delete;
+    void allocate(int count);
+    int *target = nullptr;
+    int *source = nullptr;
+    float *weight = nullptr;
+    int count = 0;
+};
+
+class Graph {
+public:
+    Graph();
+    ~Graph();
+    Graph(const Graph &) = delete;
+    Graph &operator=(const Graph &) = delete;
+
+    void allocateNodes(int size);
+    void allocateEdges(int size);
+    void addNode(int node);
+    void addEdge(Edge edge);
+    void addEdgeFromVector(std::vector<Edge> edges);
+    int setNodes(int *nodes, int size);
+    int setEdges(Edge *edges, int size);
+
+    Edge getEdgeAt(int idx);
+    int getNodeAt(int idx);
+    int *getNodes();
+    EdgeContainer &getEdgeContainer();
+    int getNodeSize();
+    int getEdgeSize();
+    int getIdxOfNode(int node);
+
+private:
+    int *nodes = nullptr;
+    int nodeCapacity = 0;
+    int nodeSize = 0;
+    int edgeSize = 0;
+    EdgeContainer edges;
+    std::map<int, int> nodeMap;
+};
+
+struct Community {
+    int *node = nullptr;
+    int *community = nullptr;
+};
+
+struct Partition {
+    int count = 0;
+    Community partition;
+    void Deallocate();
+};
+
+namespace louvain {
+
+class Error : public std::runtime_error {
+public:
+    Error(int status, const std::string &where);
+    int status() const { return status_; }
+
+private:
+    int status_;
+};
+
+}  // namespace louvain
+
+/* generate_dendogram (LouvainCUDA/LouvainCPP.cpp:209-553).  part_init is accepted
+ * and, as in the reference, not used.  One Partition per level; the caller
+ * frees them with Partition::Deallocate.  The overload with an lv_config sets
+ * resolution, thresholds and the device. */
+std::vector<Partition> generate_dendogram(Graph &original_graph,
+                                          std::map<int, std::pair<int, float>> &part_init);
+std::vector<Partition> generate_dendogram(Graph &original_graph,
+                                          std::map<int, std::pair<int, float>> &part_init,
+                                          const lv_config &cfg);
+
+/* partitionAtLevel (LouvainCPP.cpp:82-106): level-0 order, communities mapped
+ * through levels 1..level.  Returns new arrays (the reference writes through
+ * the level-0 arrays). */
+Partition partitionAtLevel(std::vector<Partition> &dendogram, int level);
+
+/* getPartitions (LouvainCPP.cpp:108-114), including its `level || size-1`
+ * expression: the flattening is taken at level 1 whenever the dendrogram has
+ * two or more levels. */
+Partition getPartitions(std::vector<Partition> &dendogram, int level = 0);
+
+/* The flattening at the last level (the partition Louvain converged to). */
+Partition getFinalPartition(std::vector<Partition> &dendogram);
+
+/* countPartitions (LouvainCPP.cpp:116-127): 1 + the largest community id. */
+int countPartitions(Partition partitions);
+
+/* Text loaders of the reference CLI (LouvainCPP.cpp:660-737): a count, then the
+ * ids (nodes.txt) or `source target weight` triples (edges.txt). */
+std::vector<int> loadNodesFile(const std::string &path);
+std::vector<Edge> loadEdgesFile(const std::string &path);
+
+#endif /* LOUVAIN_HOST_HPP */

@@ -1,0 +1,55 @@
+/*
+ * louvain_host_c.h - flat C entry points of liblouvain_host.so, for callers
+ * that reach the host layer through an FFI (ctypes, cgo, JNI) instead of C++.
+ *
+ * lvh_*  wrap generate_dendogram and the dendrogram queries of the reference
+ *        (LouvainCUDA/LouvainCPP.cpp:82-127,209-553); they replace the
+ *        Boost.Python module (LouvainCUDA/LouvainCPP.cpp:943-951) as the
+ *        language boundary.
+ * lvg_*  deterministic synthetic graphs (the reference ships none): SBM and
+ *        R-MAT edge lists, canonical (u <= v) and lexicographically sorted.
+ */
+#ifndef LOUVAIN_HOST_C_H
+#define LOUVAIN_HOST_C_H
+
+#include <stdint.h>
+
+#include "louvain_amd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct lvh_dendrogram lvh_dendrogram;
+typedef struct lvg_graph lvg_graph;
+
+/* generate_dendogram over an edge list with float weights (ids NULL = 0..n-1). */
+int lvh_generate_dendrogram(int32_t n, const int32_t *ids, int64_t e, const int32_t *src,
+                            const int32_t *dst, const float *w, const lv_config *cfg,
+                            lvh_dendrogram **out);
+int lvh_num_levels(const lvh_dendrogram *d);
+int32_t lvh_level_count(const lvh_dendrogram *d, int level);
+/* Partition of one level, reference layout (node, community). */
+int lvh_level_partition(const lvh_dendrogram *d, int level, int32_t *node, int32_t *comm);
+/* partitionAtLevel (LouvainCUDA/LouvainCPP.cpp:82-106). */
+int lvh_partition_at_level(lvh_dendrogram *d, int level, int32_t *node, int32_t *comm);
+/* getPartitions (LouvainCUDA/LouvainCPP.cpp:108-114). */
+int lvh_get_partitions(lvh_dendrogram *d, int level, int32_t *node, int32_t *comm);
+/* countPartitions of partitionAtLevel(level) (LouvainCUDA/LouvainCPP.cpp:116-127). */
+int lvh_count_partitions(lvh_dendrogram *d, int level);
+void lvh_free(lvh_dendrogram *d);
+
+lvg_graph *lvg_sbm(int32_t n, int32_t block_size, double p_in, double p_out, uint64_t seed);
+lvg_graph *lvg_rmat(int32_t scale, int64_t edge_factor, double a, double b, double c,
+                    uint64_t seed, int wtype);
+int32_t lvg_num_nodes(const lvg_graph *g);
+int64_t lvg_num_edges(const lvg_graph *g);
+int lvg_wtype(const lvg_graph *g);
+int lvg_copy(const lvg_graph *g, int32_t *src, int32_t *dst, void *w);
+void lvg_free(lvg_graph *g);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LOUVAIN_HOST_C_H */

@@ -1,0 +1,1146 @@
+// api.hip - the C ABI of liblouvain_amd.so: context, phase entry points and
+// the device-resident level loop.  See include/louvain_amd.h.
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "engine.hpp"
+
+using namespace lv;
+
+namespace {
+
+enum State { ST_EMPTY = 0, ST_LOADED, ST_CSR, ST_INIT, ST_LEVEL_DONE, ST_RENUMBERED, ST_DONE };
+enum Cat { CAT_CSR = 0, CAT_INIT, CAT_SWEEP, CAT_APPLY, CAT_RENUMBER, CAT_COARSEN, CAT_FLATTEN };
+
+struct LevelRec {
+    int32_t n = 0, k = 0, passes = 0;
+    int64_t nnz = 0, m = 0;
+    double q = 0, lower = 0;
+    int32_t *cfinal = nullptr;  // assignment after the level (device)
+    int32_t *newc = nullptr;    // renumbered community of each position (device)
+    std::vector<double> pass_q;
+    std::vector<std::vector<int32_t>> trace;
+};
+
+struct PendingTime {
+    int cat, tier;
+    hipEvent_t a, b;
+    double bytes;
+    int64_t entries;
+};
+
+}  // namespace
+
+struct lv_ctx {
+    lv_config cfg;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Arena arena;
+    // small device/pinned scalars
+    DevFlags *d_flags = nullptr, *h_flags = nullptr;
+    QAcc *d_qacc = nullptr, *h_qacc = nullptr;
+    long long *d_small = nullptr, *h_small = nullptr;  // 64 slots
+    // input
+    bool loaded = false;
+    int32_t n0 = 0;
+    int64_t e0 = 0;
+    int scale = 0;
+    int64_t m0 = 0;
+    int32_t *d_ids = nullptr;  // null when ids are positions
+    int32_t *d_src = nullptr, *d_dst = nullptr;
+    int64_t *d_w = nullptr;
+    // current level
+    int state = ST_EMPTY;
+    int level = 0;
+    int32_t n = 0;
+    int64_t nnz = 0, m = 0;
+    int64_t *rowptr = nullptr;
+    int32_t *col = nullptr;
+    int64_t *cw = nullptr;
+    int64_t *kdeg = nullptr, *loopw = nullptr, *selfsum = nullptr;
+    int32_t *C = nullptr, *next = nullptr;
+    int64_t *deg = nullptr, *in = nullptr;
+    int32_t *size = nullptr, *dsize = nullptr;
+    ull *ddeg = nullptr, *din = nullptr;
+    int32_t *verts = nullptr;
+    int64_t tier_cnt[kTiers + 1] = {0}, tier_nnz[kTiers + 1] = {0}, tier_off[kTiers + 1] = {0};
+    int64_t *hub_off = nullptr;
+    int32_t *hub_log2 = nullptr;
+    int32_t *hkeys = nullptr;
+    ull *hvals = nullptr;
+    // pass results of the level being processed
+    double cur_q = 0, cur_lower = 0;
+    int32_t cur_passes = 0;
+    std::vector<double> cur_pass_q;
+    std::vector<std::vector<int32_t>> cur_trace;
+    // dendrogram
+    std::vector<LevelRec> levels;
+    int32_t *d_final = nullptr;
+    bool final_valid = false;
+    // timing
+    lv_stats stats;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<PendingTime> pending;
+    double last_run_ms = 0;
+};
+
+// ---------------------------------------------------------------------------
+// helpers
+
+namespace {
+
+bool timing(lv_ctx *c) { return (c->cfg.flags & LV_FLAG_TIMING) != 0; }
+
+hipEvent_t ev_get(lv_ctx *c)
+{
+    if (c->ev_pool.empty()) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        return e;
+    }
+    hipEvent_t e = c->ev_pool.back();
+    c->ev_pool.pop_back();
+    return e;
+}
+
+hipEvent_t t_begin(lv_ctx *c)
+{
+    if (!timing(c)) return nullptr;
+    hipEvent_t e = ev_get(c);
+    if (e) (void)hipEventRecord(e, c->stream);
+    return e;
+}
+
+void t_end(lv_ctx *c, hipEvent_t a, int cat, int tier = -1, double bytes = 0, int64_t entries = 0)
+{
+    if (!a) return;
+    hipEvent_t b = ev_get(c);
+    if (!b) return;
+    (void)hipEventRecord(b, c->stream);
+    c->pending.push_back({cat, tier, a, b, bytes, entries});
+}
+
+// Call only after the stream has been synchronised.
+void t_resolve(lv_ctx *c)
+{
+    for (auto &p : c->pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, p.a, p.b) != hipSuccess) ms = 0.f;
+        lv_stats &s = c->stats;
+        switch (p.cat) {
+        case CAT_CSR: s.csr_ms += ms; break;
+        case CAT_INIT: s.init_ms += ms; break;
+        case CAT_SWEEP:
+            s.sweep_ms += ms;
+            if (p.tier >= 1 && p.tier <= kTiers) {
+                s.tier_ms[p.tier - 1] += ms;
+                s.tier_launches[p.tier - 1] += 1;
+                s.tier_bytes[p.tier - 1] += p.bytes;
+            }
+            s.sweep_bytes += p.bytes;
+            s.sweep_entries += p.entries;
+            break;
+        case CAT_APPLY: s.apply_ms += ms; break;
+        case CAT_RENUMBER: s.renumber_ms += ms; break;
+        case CAT_COARSEN: s.coarsen_ms += ms; break;
+        case CAT_FLATTEN: s.flatten_ms += ms; break;
+        }
+        c->ev_pool.push_back(p.a);
+        c->ev_pool.push_back(p.b);
+    }
+    c->pending.clear();
+}
+
+int sync(lv_ctx *c)
+{
+    LV_HIP(hipStreamSynchronize(c->stream));
+    if (timing(c)) t_resolve(c);
+    return LV_OK;
+}
+
+template <typename T>
+int d2h(lv_ctx *c, T *host, const T *dev, int64_t count)
+{
+    if (count <= 0) return LV_OK;
+    LV_HIP(hipMemcpyAsync(host, dev, sizeof(T) * (size_t)count, hipMemcpyDeviceToHost, c->stream));
+    return LV_OK;
+}
+
+template <typename T>
+int h2d(lv_ctx *c, T *dev, const T *host, int64_t count)
+{
+    if (count <= 0) return LV_OK;
+    LV_HIP(hipMemcpyAsync(dev, host, sizeof(T) * (size_t)count, hipMemcpyHostToDevice, c->stream));
+    return LV_OK;
+}
+
+int zero_small(lv_ctx *c, int slots)
+{
+    fill_i64(c->stream, (int64_t *)c->d_small, slots, 0);
+    return LV_OK;
+}
+
+void free_level_arrays(lv_ctx *c)
+{
+    Arena &a = c->arena;
+    arena_free(a, c->rowptr);
+    arena_free(a, c->col);
+    arena_free(a, c->cw);
+    arena_free(a, c->kdeg);
+    arena_free(a, c->loopw);
+    arena_free(a, c->selfsum);
+    arena_free(a, c->C);
+    arena_free(a, c->next);
+    arena_free(a, c->deg);
+    arena_free(a, c->in);
+    arena_free(a, c->size);
+    arena_free(a, c->dsize);
+    arena_free(a, c->ddeg);
+    arena_free(a, c->din);
+    arena_free(a, c->verts);
+    arena_free(a, c->hub_off);
+    arena_free(a, c->hub_log2);
+    arena_free(a, c->hkeys);
+    arena_free(a, c->hvals);
+}
+
+void free_dendrogram(lv_ctx *c)
+{
+    for (auto &L : c->levels) {
+        arena_free(c->arena, L.cfinal);
+        arena_free(c->arena, L.newc);
+    }
+    c->levels.clear();
+    arena_free(c->arena, c->d_final);
+    c->final_valid = false;
+}
+
+void free_input(lv_ctx *c)
+{
+    arena_free(c->arena, c->d_ids);
+    arena_free(c->arena, c->d_src);
+    arena_free(c->arena, c->d_dst);
+    arena_free(c->arena, c->d_w);
+    c->loaded = false;
+}
+
+// Resets per-run state to "input loaded, level 0".
+void reset_run(lv_ctx *c)
+{
+    free_level_arrays(c);
+    free_dendrogram(c);
+    c->cur_pass_q.clear();
+    c->cur_trace.clear();
+    c->level = 0;
+    c->state = c->loaded ? ST_LOADED : ST_EMPTY;
+}
+
+double current_q(lv_ctx *c, int &rc)
+{
+    rc = LV_OK;
+    launch_zero_qacc(c->stream, c->d_qacc);
+    launch_qagg(c->stream, c->n, c->deg, c->in, c->d_qacc);
+    rc = d2h(c, c->h_qacc, c->d_qacc, 1);
+    if (rc == LV_OK) rc = sync(c);
+    if (rc != LV_OK) return 0.0;
+    return q_from_aggregates(c->h_qacc->sum_in, c->h_qacc->d2_lo, c->h_qacc->d2_hi, c->m,
+                             c->cfg.gamma);
+}
+
+// ---------------------------------------------------------------------------
+// phases
+
+// CalculateNeighbour at level 0: symmetric CSR of the loaded edge list.
+int build_level0_csr(lv_ctx *c)
+{
+    Arena &A = c->arena;
+    hipStream_t s = c->stream;
+    const int32_t n = c->n0;
+    const int64_t e = c->e0;
+    hipEvent_t t0 = t_begin(c);
+    int64_t *cnt = nullptr;
+    ull *lastloop = nullptr;
+    LV_TRY(arena_alloc(A, c->rowptr, (int64_t)n + 1));
+    LV_TRY(arena_alloc(A, cnt, (int64_t)n + 1));
+    LV_TRY(arena_alloc(A, c->kdeg, n));
+    LV_TRY(arena_alloc(A, c->selfsum, n));
+    LV_TRY(arena_alloc(A, c->loopw, n));
+    LV_TRY(arena_alloc(A, lastloop, n));
+    fill_i64(s, cnt, (int64_t)n + 1, 0);
+    fill_i64(s, c->kdeg, n, 0);
+    fill_i64(s, c->selfsum, n, 0);
+    fill_i64(s, (int64_t *)lastloop, n, 0);
+    launch_csr_count(s, c->d_src, c->d_dst, c->d_w, e, cnt, c->kdeg, c->selfsum, lastloop);
+    LV_TRY(scan_exclusive_i64(A, s, cnt, c->rowptr, n, c->rowptr + n));
+    LV_TRY(d2h(c, &c->h_small[0], (const long long *)(c->rowptr + n), 1));
+    LV_TRY(sync(c));
+    c->nnz = c->h_small[0];
+    LV_TRY(arena_alloc(A, c->col, c->nnz));
+    LV_TRY(arena_alloc(A, c->cw, c->nnz));
+    fill_i64(s, cnt, n, 0);
+    launch_csr_fill(s, c->d_src, c->d_dst, c->d_w, e, c->rowptr, cnt, c->col, c->cw);
+    launch_loop_gather(s, lastloop, c->d_w, n, c->loopw);
+    LV_HIP(hipGetLastError());
+    t_end(c, t0, CAT_CSR);
+    arena_free(A, cnt);
+    arena_free(A, lastloop);
+    c->n = n;
+    c->m = c->m0;
+    return LV_OK;
+}
+
+int init_level(lv_ctx *c)
+{
+    Arena &A = c->arena;
+    hipStream_t s = c->stream;
+    const int32_t n = c->n;
+    hipEvent_t t0 = t_begin(c);
+    LV_TRY(arena_alloc(A, c->deg, n));
+    LV_TRY(arena_alloc(A, c->in, n));
+    LV_TRY(arena_alloc(A, c->size, n));
+    LV_TRY(arena_alloc(A, c->dsize, n));
+    LV_TRY(arena_alloc(A, c->ddeg, n));
+    LV_TRY(arena_alloc(A, c->din, n));
+    LV_TRY(arena_alloc(A, c->C, n));
+    LV_TRY(arena_alloc(A, c->next, n));
+    uint32_t *tkey = nullptr, *tkey2 = nullptr;
+    int32_t *v1 = nullptr, *v2 = nullptr;
+    LV_TRY(arena_alloc(A, tkey, n));
+    LV_TRY(arena_alloc(A, tkey2, n));
+    LV_TRY(arena_alloc(A, v1, n));
+    LV_TRY(arena_alloc(A, v2, n));
+    LV_TRY(zero_small(c, 2 * (kTiers + 1)));
+    launch_init_vertex(s, n, c->rowptr, c->kdeg, c->loopw, c->selfsum, c->deg, c->in, c->size,
+                       c->ddeg, c->din, c->dsize, c->C, c->next, tkey, v1, c->d_small,
+                       c->d_small + (kTiers + 1));
+    uint32_t *ko = nullptr;
+    int32_t *vo = nullptr;
+    LV_TRY(radix_sort_u32_i32(A, s, tkey, v1, tkey2, v2, n, 4, &ko, &vo));
+    LV_TRY(d2h(c, c->h_small, c->d_small, 2 * (kTiers + 1)));
+    LV_TRY(sync(c));
+    // keep the sorted vertex list, release the other buffers
+    if (vo == v1) {
+        c->verts = v1;
+        arena_free(A, v2);
+    } else {
+        c->verts = v2;
+        arena_free(A, v1);
+    }
+    arena_free(A, tkey);
+    arena_free(A, tkey2);
+    int64_t off = 0;
+    for (int t = 0; t <= kTiers; ++t) {
+        c->tier_cnt[t] = c->h_small[t];
+        c->tier_nnz[t] = c->h_small[kTiers + 1 + t];
+        c->tier_off[t] = off;
+        off += c->tier_cnt[t];
+    }
+    // hub tier: one private hash table per vertex, capacity next_pow2(2q)
+    const int64_t nh = c->tier_cnt[kTiers];
+    if (nh > 0) {
+        int64_t *cap = nullptr;
+        LV_TRY(arena_alloc(A, cap, nh));
+        LV_TRY(arena_alloc(A, c->hub_off, nh));
+        LV_TRY(arena_alloc(A, c->hub_log2, nh));
+        launch_hub_caps(s, c->verts + c->tier_off[kTiers], (int32_t)nh, c->rowptr, c->selfsum, cap,
+                        c->hub_log2);
+        LV_TRY(scan_exclusive_i64(A, s, cap, c->hub_off, nh, (int64_t *)c->d_small));
+        LV_TRY(d2h(c, c->h_small, c->d_small, 1));
+        LV_TRY(sync(c));
+        arena_free(A, cap);
+        int64_t slots = c->h_small[0];
+        LV_TRY(arena_alloc(A, c->hkeys, slots));
+        LV_TRY(arena_alloc(A, c->hvals, slots));
+    }
+    LV_HIP(hipGetLastError());
+    t_end(c, t0, CAT_INIT);
+    return LV_OK;
+}
+
+int one_level(lv_ctx *c, double lower, double *q_out, int32_t *passes_out)
+{
+    hipStream_t s = c->stream;
+    int rc = LV_OK;
+    double q_prev = current_q(c, rc);  // Q of the state the first pass reads
+    if (rc != LV_OK) return rc;
+    c->cur_pass_q.clear();
+    c->cur_trace.clear();
+    const bool trace = (c->cfg.flags & LV_FLAG_TRACE) != 0;
+    SweepArgs a;
+    a.rowptr = c->rowptr;
+    a.col = c->col;
+    a.w = c->cw;
+    a.kdeg = c->kdeg;
+    a.loopw = c->loopw;
+    a.selfsum = c->selfsum;
+    a.deg = c->deg;
+    a.size = c->size;
+    a.ddeg = c->ddeg;
+    a.din = c->din;
+    a.dsize = c->dsize;
+    a.m2 = 2.0 * (double)c->m;
+    a.gamma = c->cfg.gamma;
+    a.hkeys = c->hkeys;
+    a.hvals = c->hvals;
+    a.hoff = c->hub_off;
+    a.hlog2 = c->hub_log2;
+    double curQ = -1.0;  // LouvainCUDA/OneLevelKernel.cu:528
+    int32_t pass = 0;
+    double q_level = 0.0;
+    for (;;) {
+        ++pass;
+        // The break test of pass p compares Q of the state before p's deltas
+        // (OneLevelKernel.cu:875-892), known before the sweep runs.
+        const bool last = (q_prev - curQ) < c->cfg.min_gain ||
+                          (c->cfg.max_passes > 0 && pass >= c->cfg.max_passes);
+        a.C = c->C;
+        a.next = c->next;
+        a.last = last ? 1 : 0;
+        for (int t = 1; t <= kTiers; ++t) {
+            const int64_t nv = c->tier_cnt[t];
+            if (nv == 0) continue;
+            hipEvent_t t0 = t_begin(c);
+            launch_sweep_tier(s, t, a, c->verts + c->tier_off[t], (int32_t)nv);
+            t_end(c, t0, CAT_SWEEP, t, 12.0 * (double)c->tier_nnz[t] + 24.0 * (double)nv,
+                  c->tier_nnz[t]);
+        }
+        LV_HIP(hipGetLastError());
+        if (trace) {
+            std::vector<int32_t> nx((size_t)c->n);
+            LV_TRY(d2h(c, nx.data(), c->next, c->n));
+            LV_TRY(sync(c));
+            c->cur_trace.push_back(std::move(nx));
+        }
+        hipEvent_t t1 = t_begin(c);
+        launch_zero_qacc(s, c->d_qacc);
+        launch_apply(s, c->n, c->deg, c->in, c->size, c->ddeg, c->din, c->dsize, c->d_qacc);
+        t_end(c, t1, CAT_APPLY);
+        LV_HIP(hipGetLastError());
+        LV_TRY(d2h(c, c->h_qacc, c->d_qacc, 1));
+        LV_TRY(sync(c));
+        const double q_new = q_from_aggregates(c->h_qacc->sum_in, c->h_qacc->d2_lo,
+                                               c->h_qacc->d2_hi, c->m, c->cfg.gamma);
+        c->cur_pass_q.push_back(q_prev);
+        std::swap(c->C, c->next);
+        if (last) {
+            q_level = q_new;  // Q after the merge-isolated deltas, :941-953
+            break;
+        }
+        curQ = q_prev;
+        if (curQ < lower) curQ = lower;  // :894-896
+        q_prev = q_new;
+    }
+    c->cur_q = q_level;
+    c->cur_lower = lower;
+    c->cur_passes = pass;
+    c->stats.passes += pass;
+    if (q_out) *q_out = q_level;
+    if (passes_out) *passes_out = pass;
+    return LV_OK;
+}
+
+int renumber(lv_ctx *c, int32_t *k_out)
+{
+    Arena &A = c->arena;
+    hipStream_t s = c->stream;
+    const int32_t n = c->n;
+    hipEvent_t t0 = t_begin(c);
+    int32_t *leader = nullptr, *flag = nullptr, *rank = nullptr;
+    LevelRec L;
+    LV_TRY(arena_alloc(A, leader, n));
+    LV_TRY(arena_alloc(A, flag, n));
+    LV_TRY(arena_alloc(A, rank, n));
+    LV_TRY(arena_alloc(A, L.cfinal, n));
+    LV_TRY(arena_alloc(A, L.newc, n));
+    fill_i32(s, leader, n, INT_MAX);
+    launch_leader(s, c->C, n, leader);
+    launch_leader_flag(s, c->C, leader, n, flag);
+    LV_TRY(scan_exclusive_i32(A, s, flag, rank, n, (int32_t *)c->d_small));
+    launch_newc(s, c->C, leader, rank, n, L.newc);
+    copy_i32(s, L.cfinal, c->C, n);
+    LV_HIP(hipGetLastError());
+    t_end(c, t0, CAT_RENUMBER);
+    int32_t kk = 0;
+    LV_TRY(d2h(c, &kk, (const int32_t *)c->d_small, 1));
+    LV_TRY(sync(c));
+    arena_free(A, leader);
+    arena_free(A, flag);
+    arena_free(A, rank);
+    L.n = n;
+    L.k = kk;
+    L.nnz = c->nnz;
+    L.m = c->m;
+    L.passes = c->cur_passes;
+    L.q = c->cur_q;
+    L.lower = c->cur_lower;
+    L.pass_q = std::move(c->cur_pass_q);
+    L.trace = std::move(c->cur_trace);
+    c->cur_pass_q.clear();
+    c->cur_trace.clear();
+    c->levels.push_back(std::move(L));
+    if (k_out) *k_out = kk;
+    return LV_OK;
+}
+
+int induce(lv_ctx *c)
+{
+    Arena &A = c->arena;
+    hipStream_t s = c->stream;
+    const LevelRec &L = c->levels.back();
+    const int32_t n = c->n;
+    const uint64_t K = (uint64_t)L.k;
+    const int64_t total = c->nnz + n;
+    hipEvent_t t0 = t_begin(c);
+    uint64_t *keys = nullptr, *ktmp = nullptr;
+    int64_t *vals = nullptr, *vtmp = nullptr;
+    LV_TRY(arena_alloc(A, keys, total));
+    LV_TRY(arena_alloc(A, vals, total));
+    LV_TRY(arena_alloc(A, ktmp, total));
+    LV_TRY(arena_alloc(A, vtmp, total));
+    launch_coarsen_keys(s, c->rowptr, c->col, c->cw, n, c->nnz, c->selfsum, L.newc, K, keys, vals);
+    uint64_t *sk = nullptr;
+    int64_t *sv = nullptr;
+    const int bits = bits_for(K * K - 1);
+    LV_TRY(radix_sort_u64_i64(A, s, keys, vals, ktmp, vtmp, total, bits, &sk, &sv));
+    if (sk == keys) arena_free(A, ktmp); else arena_free(A, keys);
+    if (sv == vals) arena_free(A, vtmp); else arena_free(A, vals);
+    int64_t *head = nullptr, *hscan = nullptr;
+    LV_TRY(arena_alloc(A, head, total));
+    LV_TRY(arena_alloc(A, hscan, total));
+    launch_seg_heads(s, sk, total, head);
+    LV_TRY(scan_exclusive_i64(A, s, head, hscan, total, (int64_t *)c->d_small));
+    LV_TRY(d2h(c, c->h_small, c->d_small, 1));
+    LV_TRY(sync(c));
+    const int64_t U = c->h_small[0];
+    uint64_t *ukey = nullptr;
+    ull *usum = nullptr;
+    LV_TRY(arena_alloc(A, ukey, U));
+    LV_TRY(arena_alloc(A, usum, U));
+    fill_i64(s, (int64_t *)usum, U, 0);
+    launch_seg_sum(s, sk, sv, hscan, head, total, ukey, usum);
+    arena_free(A, sk);
+    arena_free(A, sv);
+    arena_free(A, head);
+    arena_free(A, hscan);
+    // next level
+    const int32_t n2 = (int32_t)K;
+    int64_t *rowcnt = nullptr, *nsflag = nullptr, *pos = nullptr;
+    int64_t *rowptr2 = nullptr, *kdeg2 = nullptr, *loop2 = nullptr, *self2 = nullptr, *cw2 = nullptr;
+    int32_t *col2 = nullptr;
+    LV_TRY(arena_alloc(A, rowcnt, (int64_t)n2 + 1));
+    LV_TRY(arena_alloc(A, nsflag, U));
+    LV_TRY(arena_alloc(A, pos, U));
+    LV_TRY(arena_alloc(A, rowptr2, (int64_t)n2 + 1));
+    LV_TRY(arena_alloc(A, kdeg2, n2));
+    LV_TRY(arena_alloc(A, loop2, n2));
+    LV_TRY(arena_alloc(A, self2, n2));
+    fill_i64(s, rowcnt, (int64_t)n2 + 1, 0);
+    fill_i64(s, kdeg2, n2, 0);
+    fill_i64(s, loop2, n2, 0);
+    fill_i64(s, self2, n2, 0);
+    launch_coarse_split(s, ukey, usum, U, K, nsflag, loop2, self2, rowcnt, kdeg2);
+    LV_TRY(scan_exclusive_i64(A, s, nsflag, pos, U, (int64_t *)c->d_small));
+    LV_TRY(scan_exclusive_i64(A, s, rowcnt, rowptr2, n2, rowptr2 + n2));
+    fill_i64(s, (int64_t *)c->d_small + 1, 1, 0);
+    launch_sum_ull(s, usum, U, c->d_small + 1);
+    LV_TRY(d2h(c, c->h_small, c->d_small, 2));
+    LV_TRY(sync(c));
+    const int64_t nnz2 = c->h_small[0];
+    const int64_t m2 = c->h_small[1] / 2;
+    LV_TRY(arena_alloc(A, col2, nnz2));
+    LV_TRY(arena_alloc(A, cw2, nnz2));
+    launch_coarse_fill(s, ukey, usum, nsflag, pos, U, K, col2, cw2, kdeg2);
+    LV_HIP(hipGetLastError());
+    t_end(c, t0, CAT_COARSEN);
+    arena_free(A, ukey);
+    arena_free(A, usum);
+    arena_free(A, rowcnt);
+    arena_free(A, nsflag);
+    arena_free(A, pos);
+    free_level_arrays(c);
+    c->rowptr = rowptr2;
+    c->col = col2;
+    c->cw = cw2;
+    c->kdeg = kdeg2;
+    c->loopw = loop2;
+    c->selfsum = self2;
+    c->n = n2;
+    c->nnz = nnz2;
+    c->m = m2;
+    c->level += 1;
+    return LV_OK;
+}
+
+int flatten_final(lv_ctx *c)
+{
+    hipEvent_t t0 = t_begin(c);
+    arena_free(c->arena, c->d_final);
+    LV_TRY(arena_alloc(c->arena, c->d_final, c->n0));
+    copy_i32(c->stream, c->d_final, c->levels[0].newc, c->n0);
+    for (size_t l = 1; l < c->levels.size(); ++l)
+        launch_compose(c->stream, c->d_final, c->n0, c->levels[l].newc);
+    LV_HIP(hipGetLastError());
+    t_end(c, t0, CAT_FLATTEN);
+    c->final_valid = true;
+    return LV_OK;
+}
+
+// Positions of a level sorted by (final community, position): the order of
+// the reference's Partition (stable radix sort of positions by community).
+int partition_perm(lv_ctx *c, const LevelRec &L, int32_t **perm_out, int32_t **spare_out)
+{
+    Arena &A = c->arena;
+    uint32_t *k1 = nullptr, *k2 = nullptr;
+    int32_t *v1 = nullptr, *v2 = nullptr;
+    LV_TRY(arena_alloc(A, k1, L.n));
+    LV_TRY(arena_alloc(A, k2, L.n));
+    LV_TRY(arena_alloc(A, v1, L.n));
+    LV_TRY(arena_alloc(A, v2, L.n));
+    copy_i32(c->stream, (int32_t *)k1, L.cfinal, L.n);
+    iota_i32(c->stream, v1, L.n);
+    uint32_t *ko = nullptr;
+    int32_t *vo = nullptr;
+    LV_TRY(radix_sort_u32_i32(A, c->stream, k1, v1, k2, v2, L.n,
+                              bits_for(L.n > 0 ? (uint64_t)(L.n - 1) : 0), &ko, &vo));
+    arena_free(A, k1);
+    arena_free(A, k2);
+    *perm_out = vo;
+    *spare_out = (vo == v1) ? v2 : v1;
+    return LV_OK;
+}
+
+int check_ctx(lv_ctx *c) { return c ? LV_OK : LV_ERR_ARG; }
+
+int set_device(lv_ctx *c)
+{
+    LV_HIP(hipSetDevice(c->device));
+    return LV_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+
+extern "C" {
+
+void lv_config_default(lv_config *cfg)
+{
+    if (!cfg) return;
+    memset(cfg, 0, sizeof(*cfg));
+    cfg->gamma = 1.0;
+    cfg->min_gain = 1e-7;
+    cfg->max_passes = 0;
+    cfg->max_levels = 0;
+    cfg->device = 0;
+    cfg->flags = 0;
+    cfg->arena_bytes = 0;
+}
+
+const char *lv_strerror(int status)
+{
+    switch (status) {
+    case LV_OK: return "ok";
+    case LV_ERR_NOMEM: return "out of device memory (arena exhausted)";
+    case LV_ERR_INVALID_INPUT: return "invalid input (unknown or repeated node id, negative or non-finite weight)";
+    case LV_ERR_RANGE: return "weight sums exceed the exact int64 range";
+    case LV_ERR_ARG: return "invalid argument or call out of order";
+    case LV_ERR_DEVICE: return "HIP runtime error";
+    case LV_ERR_NO_TRACE: return "trace not recorded (create the context with LV_FLAG_TRACE)";
+    default: return "unknown status";
+    }
+}
+
+int lv_create(lv_ctx **out, const lv_config *cfg)
+{
+    if (!out) return LV_ERR_ARG;
+    *out = nullptr;
+    lv_ctx *c = new (std::nothrow) lv_ctx();
+    if (!c) return LV_ERR_NOMEM;
+    if (cfg)
+        c->cfg = *cfg;
+    else
+        lv_config_default(&c->cfg);
+    memset(&c->stats, 0, sizeof(c->stats));
+    if (!(c->cfg.gamma >= 0.0) || !(c->cfg.min_gain == c->cfg.min_gain) || c->cfg.max_passes < 0 ||
+        c->cfg.max_levels < 0) {
+        delete c;
+        return LV_ERR_ARG;
+    }
+    c->device = c->cfg.device;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= c->device || c->device < 0) {
+        (void)hipGetLastError();
+        delete c;
+        return LV_ERR_DEVICE;
+    }
+    int rc = LV_OK;
+    do {
+        if (hipSetDevice(c->device) != hipSuccess) { rc = LV_ERR_DEVICE; break; }
+        if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { rc = LV_ERR_DEVICE; break; }
+        if (hipMalloc(&c->d_flags, sizeof(DevFlags)) != hipSuccess) { rc = LV_ERR_NOMEM; break; }
+        if (hipMalloc(&c->d_qacc, sizeof(QAcc)) != hipSuccess) { rc = LV_ERR_NOMEM; break; }
+        if (hipMalloc(&c->d_small, 64 * sizeof(long long)) != hipSuccess) { rc = LV_ERR_NOMEM; break; }
+        if (hipHostMalloc(&c->h_flags, sizeof(DevFlags), hipHostMallocDefault) != hipSuccess) { rc = LV_ERR_NOMEM; break; }
+        if (hipHostMalloc(&c->h_qacc, sizeof(QAcc), hipHostMallocDefault) != hipSuccess) { rc = LV_ERR_NOMEM; break; }
+        if (hipHostMalloc(&c->h_small, 64 * sizeof(long long), hipHostMallocDefault) != hipSuccess) { rc = LV_ERR_NOMEM; break; }
+        if (c->cfg.arena_bytes) rc = c->arena.reserve(c->cfg.arena_bytes);
+    } while (0);
+    if (rc != LV_OK) {
+        (void)hipGetLastError();
+        lv_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return LV_OK;
+}
+
+int lv_destroy(lv_ctx *c)
+{
+    if (!c) return LV_ERR_ARG;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto &p : c->pending) {
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
+    for (auto e : c->ev_pool) (void)hipEventDestroy(e);
+    if (c->d_flags) (void)hipFree(c->d_flags);
+    if (c->d_qacc) (void)hipFree(c->d_qacc);
+    if (c->d_small) (void)hipFree(c->d_small);
+    if (c->h_flags) (void)hipHostFree(c->h_flags);
+    if (c->h_qacc) (void)hipHostFree(c->h_qacc);
+    if (c->h_small) (void)hipHostFree(c->h_small);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;  // the arena destructor frees the HBM block
+    return LV_OK;
+}
+
+int lv_load_edges(lv_ctx *c, int32_t n, const int32_t *ids, int64_t e, const int32_t *src,
+                  const int32_t *dst, const void *w, int wtype)
+{
+    LV_TRY(check_ctx(c));
+    if (n < 0 || e < 0 || (e > 0 && (!src || !dst || !w)) ||
+        (wtype != LV_WEIGHT_INT32 && wtype != LV_WEIGHT_FLOAT32))
+        return LV_ERR_ARG;
+    LV_TRY(set_device(c));
+    LV_HIP(hipStreamSynchronize(c->stream));
+    reset_run(c);
+    free_input(c);
+    c->state = ST_EMPTY;
+    Arena &A = c->arena;
+    hipStream_t s = c->stream;
+    // arena: sized for the largest level-0 phase unless the caller fixed it
+    if (!c->cfg.arena_bytes) {
+        size_t want = (size_t)180 * (size_t)e + (size_t)256 * (size_t)n + ((size_t)192 << 20);
+        size_t freeb = 0, totalb = 0;
+        if (hipMemGetInfo(&freeb, &totalb) == hipSuccess) {
+            size_t avail = (size_t)((double)(freeb + A.capacity()) * 0.92);
+            if (want > avail) want = avail;
+        }
+        if (A.capacity() < want) {
+            A.release_all();
+            int rc = A.reserve(want);
+            if (rc != LV_OK) return rc;
+        }
+    }
+    bool iota_ids = true;
+    if (ids)
+        for (int32_t i = 0; i < n && iota_ids; ++i) iota_ids = (ids[i] == i);
+    LV_TRY(arena_alloc(A, c->d_src, e));
+    LV_TRY(arena_alloc(A, c->d_dst, e));
+    LV_TRY(arena_alloc(A, c->d_w, e));
+    uint32_t *wraw = nullptr;
+    LV_TRY(arena_alloc(A, wraw, e));
+    LV_TRY(h2d(c, c->d_src, src, e));
+    LV_TRY(h2d(c, c->d_dst, dst, e));
+    LV_TRY(h2d(c, wraw, (const uint32_t *)w, e));
+    LV_HIP(hipMemsetAsync(c->d_flags, 0, sizeof(DevFlags), s));
+    if (!iota_ids) {
+        LV_TRY(arena_alloc(A, c->d_ids, n));
+        LV_TRY(h2d(c, c->d_ids, ids, n));
+        uint32_t *k1 = nullptr, *k2 = nullptr;
+        int32_t *v1 = nullptr, *v2 = nullptr;
+        LV_TRY(arena_alloc(A, k1, n));
+        LV_TRY(arena_alloc(A, k2, n));
+        LV_TRY(arena_alloc(A, v1, n));
+        LV_TRY(arena_alloc(A, v2, n));
+        launch_ids_to_keys(s, c->d_ids, k1, n);
+        iota_i32(s, v1, n);
+        uint32_t *ko = nullptr;
+        int32_t *vo = nullptr;
+        LV_TRY(radix_sort_u32_i32(A, s, k1, v1, k2, v2, n, 32, &ko, &vo));
+        launch_check_sorted_unique(s, ko, n, c->d_flags);
+        launch_map_endpoints(s, ko, vo, n, c->d_src, e, c->d_flags);
+        launch_map_endpoints(s, ko, vo, n, c->d_dst, e, c->d_flags);
+        LV_TRY(sync(c));
+        arena_free(A, k1);
+        arena_free(A, k2);
+        arena_free(A, v1);
+        arena_free(A, v2);
+    } else {
+        launch_check_endpoints(s, c->d_src, c->d_dst, e, n, c->d_flags);
+    }
+    launch_weight_exponent(s, wraw, wtype, e, c->d_flags);
+    LV_HIP(hipGetLastError());
+    LV_TRY(d2h(c, c->h_flags, c->d_flags, 1));
+    LV_TRY(sync(c));
+    if (c->h_flags->err) {
+        arena_free(A, wraw);
+        free_input(c);
+        return (c->h_flags->err & 1) ? LV_ERR_INVALID_INPUT : LV_ERR_RANGE;
+    }
+    const int scale = c->h_flags->scale;
+    launch_weight_units(s, wraw, wtype, e, scale, c->d_w, c->d_flags);
+    LV_HIP(hipGetLastError());
+    LV_TRY(d2h(c, c->h_flags, c->d_flags, 1));
+    LV_TRY(sync(c));
+    arena_free(A, wraw);
+    if (c->h_flags->err) {
+        free_input(c);
+        return (c->h_flags->err & 1) ? LV_ERR_INVALID_INPUT : LV_ERR_RANGE;
+    }
+    c->scale = scale;
+    c->m0 = c->h_flags->m_units;
+    c->n0 = n;
+    c->e0 = e;
+    c->loaded = true;
+    c->level = 0;
+    c->state = ST_LOADED;
+    return LV_OK;
+}
+
+int lv_neighbour_compute(lv_ctx *c)
+{
+    LV_TRY(check_ctx(c));
+    LV_TRY(set_device(c));
+    if (c->state == ST_LOADED && c->level == 0) {
+        if (c->e0 == 0) return LV_ERR_ARG;  // no edges: use lv_run (identity partition)
+        LV_TRY(build_level0_csr(c));
+        c->state = ST_CSR;
+        return LV_OK;
+    }
+    if (c->state == ST_CSR) return LV_OK;  // lv_induce already produced this level's CSR
+    return LV_ERR_ARG;
+}
+
+int lv_init(lv_ctx *c)
+{
+    LV_TRY(check_ctx(c));
+    LV_TRY(set_device(c));
+    if (c->state != ST_CSR) return LV_ERR_ARG;
+    LV_TRY(init_level(c));
+    c->state = ST_INIT;
+    return LV_OK;
+}
+
+int lv_modularity(lv_ctx *c, double *q)
+{
+    LV_TRY(check_ctx(c));
+    LV_TRY(set_device(c));
+    if (c->state < ST_INIT || !q) return LV_ERR_ARG;
+    int rc = LV_OK;
+    *q = current_q(c, rc);
+    return rc;
+}
+
+int lv_one_level(lv_ctx *c, double lower, double *q, int32_t *passes)
+{
+    LV_TRY(check_ctx(c));
+    LV_TRY(set_device(c));
+    if (c->state != ST_INIT) return LV_ERR_ARG;
+    LV_TRY(one_level(c, lower, q, passes));
+    c->state = ST_LEVEL_DONE;
+    return LV_OK;
+}
+
+int lv_renumber(lv_ctx *c, int32_t *k)
+{
+    LV_TRY(check_ctx(c));
+    LV_TRY(set_device(c));
+    if (c->state != ST_LEVEL_DONE) return LV_ERR_ARG;
+    LV_TRY(renumber(c, k));
+    c->state = ST_RENUMBERED;
+    LV_TRY(flatten_final(c));
+    return sync(c);
+}
+
+int lv_induce(lv_ctx *c)
+{
+    LV_TRY(check_ctx(c));
+    LV_TRY(set_device(c));
+    if (c->state != ST_RENUMBERED) return LV_ERR_ARG;
+    LV_TRY(induce(c));
+    c->state = ST_CSR;
+    return LV_OK;
+}
+
+int lv_run(lv_ctx *c, int32_t *levels_out)
+{
+    LV_TRY(check_ctx(c));
+    if (!c->loaded) return LV_ERR_ARG;
+    LV_TRY(set_device(c));
+    auto t_start = std::chrono::steady_clock::now();
+    LV_HIP(hipStreamSynchronize(c->stream));
+    reset_run(c);
+    if (c->e0 == 0) {
+        // identity partition (the reference leaves it undefined,
+        // LouvainCUDA/LouvainCPP.cpp:211-230)
+        LevelRec L;
+        L.n = c->n0;
+        L.k = c->n0;
+        LV_TRY(arena_alloc(c->arena, L.cfinal, c->n0));
+        LV_TRY(arena_alloc(c->arena, L.newc, c->n0));
+        iota_i32(c->stream, L.cfinal, c->n0);
+        iota_i32(c->stream, L.newc, c->n0);
+        c->levels.push_back(std::move(L));
+        LV_TRY(flatten_final(c));
+        LV_TRY(sync(c));
+        c->state = ST_RENUMBERED;
+        c->stats.levels += 1;
+        if (levels_out) *levels_out = 1;
+        c->last_run_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+        return LV_OK;
+    }
+    // level 0, LouvainCUDA/LouvainCPP.cpp:334-401
+    LV_TRY(build_level0_csr(c));
+    c->state = ST_CSR;
+    LV_TRY(init_level(c));
+    c->state = ST_INIT;
+    int rc = LV_OK;
+    double mod = current_q(c, rc);
+    if (rc != LV_OK) return rc;
+    double q = 0;
+    LV_TRY(one_level(c, mod, &q, nullptr));
+    LV_TRY(renumber(c, nullptr));
+    c->state = ST_RENUMBERED;
+    mod = q;
+    // later levels, :420-507
+    for (;;) {
+        if (c->cfg.max_levels > 0 && (int32_t)c->levels.size() >= c->cfg.max_levels) break;
+        LV_TRY(induce(c));
+        c->state = ST_CSR;
+        LV_TRY(init_level(c));
+        c->state = ST_INIT;
+        LV_TRY(one_level(c, mod, &q, nullptr));
+        LV_TRY(renumber(c, nullptr));
+        c->state = ST_RENUMBERED;
+        if (q - mod < c->cfg.min_gain) break;
+        mod = q;
+    }
+    LV_TRY(flatten_final(c));
+    LV_TRY(sync(c));
+    // the last level's working arrays are not part of the result
+    free_level_arrays(c);
+    c->state = ST_DONE;  // finished run: only result queries are valid
+    c->stats.levels += (int32_t)c->levels.size();
+    if (levels_out) *levels_out = (int32_t)c->levels.size();
+    c->last_run_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    return LV_OK;
+}
+
+int lv_num_levels(lv_ctx *c, int32_t *levels)
+{
+    LV_TRY(check_ctx(c));
+    if (!levels) return LV_ERR_ARG;
+    *levels = (int32_t)c->levels.size();
+    return LV_OK;
+}
+
+int lv_level_info_get(lv_ctx *c, int32_t level, lv_level_info *out)
+{
+    LV_TRY(check_ctx(c));
+    if (!out || level < 0 || level >= (int32_t)c->levels.size()) return LV_ERR_ARG;
+    const LevelRec &L = c->levels[level];
+    out->n = L.n;
+    out->k = L.k;
+    out->nnz = L.nnz;
+    out->m_units = L.m;
+    out->passes = L.passes;
+    out->scale = c->scale;
+    out->q = L.q;
+    out->lower = L.lower;
+    return LV_OK;
+}
+
+int lv_level_partition(lv_ctx *c, int32_t level, int32_t *node, int32_t *comm)
+{
+    LV_TRY(check_ctx(c));
+    if (!node || !comm || level < 0 || level >= (int32_t)c->levels.size()) return LV_ERR_ARG;
+    LV_TRY(set_device(c));
+    const LevelRec &L = c->levels[level];
+    if (L.n == 0) return LV_OK;
+    int32_t *perm = nullptr, *spare = nullptr, *dn = nullptr, *dc = nullptr;
+    LV_TRY(partition_perm(c, L, &perm, &spare));
+    LV_TRY(arena_alloc(c->arena, dn, L.n));
+    LV_TRY(arena_alloc(c->arena, dc, L.n));
+    launch_gather_partition(c->stream, perm, level == 0 ? c->d_ids : nullptr, L.newc, L.n, dn, dc);
+    LV_HIP(hipGetLastError());
+    LV_TRY(d2h(c, node, dn, L.n));
+    LV_TRY(d2h(c, comm, dc, L.n));
+    LV_TRY(sync(c));
+    arena_free(c->arena, perm);
+    arena_free(c->arena, spare);
+    arena_free(c->arena, dn);
+    arena_free(c->arena, dc);
+    return LV_OK;
+}
+
+int lv_level_assignment(lv_ctx *c, int32_t level, int32_t *comm)
+{
+    LV_TRY(check_ctx(c));
+    if (!comm || level < 0 || level >= (int32_t)c->levels.size()) return LV_ERR_ARG;
+    LV_TRY(set_device(c));
+    LV_TRY(d2h(c, comm, c->levels[level].newc, c->levels[level].n));
+    return sync(c);
+}
+
+int lv_flatten(lv_ctx *c, int32_t level, int32_t *node, int32_t *comm)
+{
+    LV_TRY(check_ctx(c));
+    if (!node || !comm || level < 0 || level >= (int32_t)c->levels.size()) return LV_ERR_ARG;
+    LV_TRY(set_device(c));
+    const LevelRec &L0 = c->levels[0];
+    if (L0.n == 0) return LV_OK;
+    int32_t *comp = nullptr, *perm = nullptr, *spare = nullptr, *dn = nullptr, *dc = nullptr;
+    LV_TRY(arena_alloc(c->arena, comp, L0.n));
+    copy_i32(c->stream, comp, L0.newc, L0.n);
+    for (int32_t l = 1; l <= level; ++l) launch_compose(c->stream, comp, L0.n, c->levels[l].newc);
+    LV_TRY(partition_perm(c, L0, &perm, &spare));
+    LV_TRY(arena_alloc(c->arena, dn, L0.n));
+    LV_TRY(arena_alloc(c->arena, dc, L0.n));
+    launch_gather_partition(c->stream, perm, c->d_ids, comp, L0.n, dn, dc);
+    LV_HIP(hipGetLastError());
+    LV_TRY(d2h(c, node, dn, L0.n));
+    LV_TRY(d2h(c, comm, dc, L0.n));
+    LV_TRY(sync(c));
+    arena_free(c->arena, comp);
+    arena_free(c->arena, perm);
+    arena_free(c->arena, spare);
+    arena_free(c->arena, dn);
+    arena_free(c->arena, dc);
+    return LV_OK;
+}
+
+int lv_final_assignment(lv_ctx *c, int32_t *comm)
+{
+    LV_TRY(check_ctx(c));
+    if (!comm || !c->final_valid) return LV_ERR_ARG;
+    LV_TRY(set_device(c));
+    LV_TRY(d2h(c, comm, c->d_final, c->n0));
+    return sync(c);
+}
+
+int lv_final_modularity(lv_ctx *c, double *q)
+{
+    LV_TRY(check_ctx(c));
+    if (!q || !c->final_valid) return LV_ERR_ARG;
+    LV_TRY(set_device(c));
+    if (c->e0 == 0 || c->m0 == 0) {
+        *q = 0.0;
+        return LV_OK;
+    }
+    ull *lc = nullptr, *dc = nullptr;
+    LV_TRY(arena_alloc(c->arena, lc, c->n0));
+    LV_TRY(arena_alloc(c->arena, dc, c->n0));
+    fill_i64(c->stream, (int64_t *)lc, c->n0, 0);
+    fill_i64(c->stream, (int64_t *)dc, c->n0, 0);
+    launch_final_q(c->stream, c->d_src, c->d_dst, c->d_w, c->e0, c->d_final, lc, dc);
+    launch_zero_qacc(c->stream, c->d_qacc);
+    launch_qagg_ull(c->stream, c->n0, dc, lc, c->d_qacc);
+    LV_HIP(hipGetLastError());
+    LV_TRY(d2h(c, c->h_qacc, c->d_qacc, 1));
+    LV_TRY(sync(c));
+    arena_free(c->arena, lc);
+    arena_free(c->arena, dc);
+    *q = q_from_aggregates(c->h_qacc->sum_in, c->h_qacc->d2_lo, c->h_qacc->d2_hi, c->m0,
+                           c->cfg.gamma);
+    return LV_OK;
+}
+
+int lv_trace_next(lv_ctx *c, int32_t level, int32_t pass, int32_t *next)
+{
+    LV_TRY(check_ctx(c));
+    if (!next || level < 0 || level >= (int32_t)c->levels.size()) return LV_ERR_ARG;
+    if (!(c->cfg.flags & LV_FLAG_TRACE)) return LV_ERR_NO_TRACE;
+    const LevelRec &L = c->levels[level];
+    if (pass < 1 || pass > (int32_t)L.trace.size()) return LV_ERR_ARG;
+    memcpy(next, L.trace[pass - 1].data(), sizeof(int32_t) * (size_t)L.n);
+    return LV_OK;
+}
+
+int lv_trace_pass_q(lv_ctx *c, int32_t level, int32_t pass, double *q)
+{
+    LV_TRY(check_ctx(c));
+    if (!q || level < 0 || level >= (int32_t)c->levels.size()) return LV_ERR_ARG;
+    const LevelRec &L = c->levels[level];
+    if (pass < 1 || pass > (int32_t)L.pass_q.size()) return LV_ERR_ARG;
+    *q = L.pass_q[pass - 1];
+    return LV_OK;
+}
+
+int lv_level_csr(lv_ctx *c, int64_t *rowptr, int32_t *col, int64_t *w)
+{
+    LV_TRY(check_ctx(c));
+    if (c->state < ST_CSR || c->state > ST_RENUMBERED || !c->rowptr) return LV_ERR_ARG;
+    if (!rowptr || (c->nnz > 0 && (!col || !w))) return LV_ERR_ARG;
+    LV_TRY(set_device(c));
+    LV_TRY(d2h(c, rowptr, c->rowptr, (int64_t)c->n + 1));
+    LV_TRY(d2h(c, col, c->col, c->nnz));
+    LV_TRY(d2h(c, w, c->cw, c->nnz));
+    return sync(c);
+}
+
+int lv_level_vertex(lv_ctx *c, int64_t *k, int64_t *loop, int64_t *selfsum)
+{
+    LV_TRY(check_ctx(c));
+    if (c->state < ST_CSR || c->state > ST_RENUMBERED || !c->kdeg) return LV_ERR_ARG;
+    if (!k || !loop || !selfsum) return LV_ERR_ARG;
+    LV_TRY(set_device(c));
+    LV_TRY(d2h(c, k, c->kdeg, c->n));
+    LV_TRY(d2h(c, loop, c->loopw, c->n));
+    LV_TRY(d2h(c, selfsum, c->selfsum, c->n));
+    return sync(c);
+}
+
+int lv_current_level(lv_ctx *c, int32_t *level, int32_t *n, int64_t *nnz)
+{
+    LV_TRY(check_ctx(c));
+    if (level) *level = c->level;
+    if (n) *n = (c->state >= ST_CSR && c->state <= ST_RENUMBERED) ? c->n : c->n0;
+    if (nnz) *nnz = (c->state >= ST_CSR && c->state <= ST_RENUMBERED) ? c->nnz : 0;
+    return LV_OK;
+}
+
+int lv_get_stats(lv_ctx *c, lv_stats *out)
+{
+    LV_TRY(check_ctx(c));
+    if (!out) return LV_ERR_ARG;
+    LV_TRY(set_device(c));
+    LV_TRY(sync(c));
+    *out = c->stats;
+    return LV_OK;
+}
+
+int lv_reset_stats(lv_ctx *c)
+{
+    LV_TRY(check_ctx(c));
+    LV_TRY(sync(c));
+    memset(&c->stats, 0, sizeof(c->stats));
+    return LV_OK;
+}
+
+int lv_last_run_ms(lv_ctx *c, double *ms)
+{
+    LV_TRY(check_ctx(c));
+    if (!ms) return LV_ERR_ARG;
+    *ms = c->last_run_ms;
+    return LV_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,212 @@
+// engine.hpp - internal declarations of the MI355X Louvain engine.
+//
+// Layout in HBM (one arena per context, first-fit with coalescing):
+//   input   : ids[n0] (only when ids are not 0..n0-1), el_src/el_dst[e0] positions,
+//             el_w[e0] int64 units                      (persistent until the next load)
+//   level l : CSR rowptr[n+1] int64, col[nnz] int32, w[nnz] int64 (non-self entries);
+//             per vertex k, loop, selfsum (int64), C/next (int32 ping-pong),
+//             per community deg/in (int64), size (int32) and their deltas;
+//             vertex lists bucketed by row length for the sweep tiers
+//   dendro  : per level cfinal[n] and newc[n] (int32), final[n0]   (persistent per run)
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <map>
+#include <vector>
+
+#include "../../../include/louvain_amd.h"
+
+namespace lv {
+
+typedef unsigned long long ull;
+
+// Weight sums stay below this many units so that 4*m and sum(deg) fit in int64.
+constexpr int64_t kUnitLimit = int64_t(1) << 60;
+
+// Sweep tiers by q = row length + (1 if the vertex has a self-loop).
+constexpr int kTiers = LV_SWEEP_TIERS;  // tiers 1..7 use LDS tables, tier 8 global
+constexpr int kTierMaxQ[kTiers] = {4, 8, 16, 32, 64, 512, 2048, 0x7fffffff};
+
+#define LV_TRY(expr)                                \
+    do {                                            \
+        int lv_rc_ = (expr);                        \
+        if (lv_rc_ != LV_OK) return lv_rc_;         \
+    } while (0)
+
+#define LV_HIP(expr)                                \
+    do {                                            \
+        hipError_t lv_e_ = (expr);                  \
+        if (lv_e_ != hipSuccess) return ::lv::hip_fail(lv_e_, #expr, __FILE__, __LINE__); \
+    } while (0)
+
+int hip_fail(hipError_t e, const char *what, const char *file, int line);
+
+// ---------------------------------------------------------------------------
+// HBM arena: one hipMalloc, host-side first-fit bookkeeping.
+class Arena {
+public:
+    ~Arena();
+    int reserve(size_t bytes);  // (re)allocates only when larger than the current block
+    void *alloc(size_t bytes);
+    void release(void *p);
+    void release_all();
+    size_t capacity() const { return cap_; }
+    size_t in_use() const { return used_bytes_; }
+    size_t high_water() const { return high_; }
+
+private:
+    char *base_ = nullptr;
+    size_t cap_ = 0;
+    size_t used_bytes_ = 0, high_ = 0;
+    std::map<size_t, size_t> free_;  // offset -> size
+    std::map<size_t, size_t> used_;  // offset -> size
+};
+
+// Typed arena allocation helper.
+template <typename T>
+int arena_alloc(Arena &a, T *&p, int64_t count)
+{
+    size_t bytes = sizeof(T) * (size_t)(count > 0 ? count : 1);
+    p = (T *)a.alloc(bytes);
+    return p ? LV_OK : LV_ERR_NOMEM;
+}
+
+template <typename T>
+void arena_free(Arena &a, T *&p)
+{
+    if (p) a.release((void *)p);
+    p = nullptr;
+}
+
+// ---------------------------------------------------------------------------
+// Exact modularity aggregates, filled by device reductions.
+struct QAcc {
+    long long sum_in;  // sum over communities of internals
+    ull d2_lo, d2_hi;  // sum over communities of deg^2 (128-bit)
+    long long pad;
+};
+
+double q_from_aggregates(int64_t sum_in, uint64_t d2_lo, uint64_t d2_hi, int64_t m, double gamma);
+
+// Per-sweep kernel arguments.
+struct SweepArgs {
+    const int64_t *rowptr;
+    const int32_t *col;
+    const int64_t *w;
+    const int64_t *kdeg;
+    const int64_t *loopw;
+    const int64_t *selfsum;
+    const int32_t *C;
+    int32_t *next;
+    const int64_t *deg;
+    const int32_t *size;
+    ull *ddeg;
+    ull *din;
+    int32_t *dsize;
+    double m2;     // 2.0 * (double)m
+    double gamma;
+    int last;      // last pass: fuse merge-isolated, drop move deltas
+    // global tables of the hub tier
+    int32_t *hkeys;
+    ull *hvals;
+    const int64_t *hoff;
+    const int32_t *hlog2;
+};
+
+// ---------------------------------------------------------------------------
+// primitives.hip
+int scan_exclusive_i64(Arena &a, hipStream_t s, const int64_t *in, int64_t *out, int64_t n,
+                       int64_t *total_dev);
+int scan_exclusive_i32(Arena &a, hipStream_t s, const int32_t *in, int32_t *out, int64_t n,
+                       int32_t *total_dev);
+// Stable LSD radix sort over bits [0, bits).  The sorted data ends in
+// (*keys_out, *vals_out), which is either the input pair or the tmp pair.
+int radix_sort_u64_i64(Arena &a, hipStream_t s, uint64_t *keys, int64_t *vals, uint64_t *ktmp,
+                       int64_t *vtmp, int64_t n, int bits, uint64_t **keys_out,
+                       int64_t **vals_out);
+int radix_sort_u32_i32(Arena &a, hipStream_t s, uint32_t *keys, int32_t *vals, uint32_t *ktmp,
+                       int32_t *vtmp, int64_t n, int bits, uint32_t **keys_out,
+                       int32_t **vals_out);
+void fill_i32(hipStream_t s, int32_t *p, int64_t n, int32_t v);
+void fill_i64(hipStream_t s, int64_t *p, int64_t n, int64_t v);
+void iota_i32(hipStream_t s, int32_t *p, int64_t n);
+void copy_i32(hipStream_t s, int32_t *dst, const int32_t *src, int64_t n);
+void sum_i64(hipStream_t s, const int64_t *p, int64_t n, int64_t *out_dev);  // *out += sum
+
+int bits_for(uint64_t maxval);  // bits needed to hold values 0..maxval
+
+// ---------------------------------------------------------------------------
+// louvain_kernels.hip (host launchers)
+struct DevFlags {  // error flags and small scalars written by kernels
+    int32_t err;              // bit 1: invalid input, bit 2: out of exact range
+    int32_t scale;            // weight unit exponent s
+    int32_t pad0, pad1;
+    long long total_units;    // sum of (w == 0 ? 1 : w) in units
+    long long m_units;        // sum of w in units
+};
+
+void launch_weight_exponent(hipStream_t s, const void *w, int wtype, int64_t e, DevFlags *f);
+void launch_weight_units(hipStream_t s, const void *w, int wtype, int64_t e, int scale,
+                         int64_t *wu, DevFlags *f);
+void launch_check_endpoints(hipStream_t s, const int32_t *src, const int32_t *dst, int64_t e,
+                            int32_t n, DevFlags *f);
+void launch_check_sorted_unique(hipStream_t s, const uint32_t *keys, int64_t n, DevFlags *f);
+void launch_map_endpoints(hipStream_t s, const uint32_t *sorted_ids, const int32_t *sorted_pos,
+                          int32_t n, int32_t *ep, int64_t e, DevFlags *f);
+void launch_ids_to_keys(hipStream_t s, const int32_t *ids, uint32_t *keys, int32_t n);
+
+void launch_csr_count(hipStream_t s, const int32_t *src, const int32_t *dst, const int64_t *w,
+                      int64_t e, int64_t *cnt, int64_t *kdeg, int64_t *selfsum,
+                      ull *lastloop);
+void launch_csr_fill(hipStream_t s, const int32_t *src, const int32_t *dst, const int64_t *w,
+                     int64_t e, const int64_t *rowptr, int64_t *cursor, int32_t *col,
+                     int64_t *cw);
+void launch_loop_gather(hipStream_t s, const ull *lastloop, const int64_t *w, int32_t n,
+                        int64_t *loopw);
+
+void launch_init_vertex(hipStream_t s, int32_t n, const int64_t *rowptr, const int64_t *kdeg,
+                        const int64_t *loopw, const int64_t *selfsum, int64_t *deg, int64_t *in,
+                        int32_t *size, ull *ddeg, ull *din, int32_t *dsize, int32_t *C,
+                        int32_t *next, uint32_t *tier_key, int32_t *verts,
+                        long long *tier_cnt, long long *tier_nnz);
+void launch_hub_caps(hipStream_t s, const int32_t *hub_verts, int32_t nh, const int64_t *rowptr,
+                     const int64_t *selfsum, int64_t *cap, int32_t *log2cap);
+void launch_qagg(hipStream_t s, int32_t n, const int64_t *deg, const int64_t *in, QAcc *acc);
+void launch_zero_qacc(hipStream_t s, QAcc *acc);
+void launch_apply(hipStream_t s, int32_t n, int64_t *deg, int64_t *in, int32_t *size, ull *ddeg,
+                  ull *din, int32_t *dsize, QAcc *acc);
+void launch_sweep_tier(hipStream_t s, int tier, const SweepArgs &a, const int32_t *verts,
+                       int32_t nv);
+
+void launch_leader(hipStream_t s, const int32_t *C, int32_t n, int32_t *leader);
+void launch_leader_flag(hipStream_t s, const int32_t *C, const int32_t *leader, int32_t n,
+                        int32_t *flag);
+void launch_newc(hipStream_t s, const int32_t *C, const int32_t *leader, const int32_t *rank,
+                 int32_t n, int32_t *newc);
+
+void launch_coarsen_keys(hipStream_t s, const int64_t *rowptr, const int32_t *col,
+                         const int64_t *w, int32_t n, int64_t nnz, const int64_t *selfsum,
+                         const int32_t *newc, uint64_t K, uint64_t *keys, int64_t *vals);
+void launch_seg_heads(hipStream_t s, const uint64_t *keys, int64_t n, int64_t *head);
+void launch_seg_sum(hipStream_t s, const uint64_t *keys, const int64_t *vals, const int64_t *hscan,
+                    const int64_t *head, int64_t n, uint64_t *ukey, ull *usum);
+void launch_coarse_split(hipStream_t s, const uint64_t *ukey, const ull *usum, int64_t u,
+                         uint64_t K, int64_t *nsflag, int64_t *loopw, int64_t *selfsum,
+                         int64_t *rowcnt, int64_t *kdeg);
+void launch_coarse_fill(hipStream_t s, const uint64_t *ukey, const ull *usum, const int64_t *nsflag,
+                        const int64_t *pos, int64_t u, uint64_t K, int32_t *col, int64_t *cw,
+                        int64_t *kdeg);
+void launch_sum_ull(hipStream_t s, const ull *p, int64_t n, long long *out);
+
+void launch_compose(hipStream_t s, int32_t *f, int32_t n, const int32_t *map);
+void launch_gather_partition(hipStream_t s, const int32_t *perm, const int32_t *ids,
+                             const int32_t *comm_by_pos, int32_t n, int32_t *node,
+                             int32_t *comm);
+void launch_final_q(hipStream_t s, const int32_t *src, const int32_t *dst, const int64_t *w,
+                    int64_t e, const int32_t *comm, ull *lc, ull *dc);
+void launch_qagg_ull(hipStream_t s, int32_t n, const ull *dc, const ull *lc, QAcc *acc);
+
+}  // namespace lv

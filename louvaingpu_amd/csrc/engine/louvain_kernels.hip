@@ -1,0 +1,1083 @@
+// louvain_kernels.hip - the Louvain kernels for gfx950 (64-lane wavefronts).
+//
+// Every weight sum is an exact int64 count of units 2^-s; gains and Q are
+// formed in FP64 from exact integers (build flag -ffp-contract=off keeps the
+// multiply and subtract of W - ((gamma*deg)*f) separate).
+#include <climits>
+
+#include "engine.hpp"
+
+namespace lv {
+
+static inline int grid_for(int64_t n, int block, int64_t cap = 1 << 16)
+{
+    int64_t g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (int)g;
+}
+
+#define GRID_STRIDE(i, n)                                                          \
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < (n); \
+         i += (int64_t)gridDim.x * blockDim.x)
+
+__device__ __forceinline__ long long wave_sum(long long v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ int wave_max_i(int v)
+{
+    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// 128-bit accumulate of deg^2 as (lo, hi).
+__device__ __forceinline__ void acc_sq(ull &lo, ull &hi, ull d)
+{
+    ull plo = d * d, phi = __umul64hi(d, d);
+    lo += plo;
+    hi += phi + (lo < plo ? 1ull : 0ull);
+}
+
+__device__ __forceinline__ void wave_sum_u128(ull &lo, ull &hi)
+{
+    for (int off = 32; off > 0; off >>= 1) {
+        ull olo = __shfl_xor(lo, off, 64), ohi = __shfl_xor(hi, off, 64);
+        lo += olo;
+        hi += ohi + (lo < olo ? 1ull : 0ull);
+    }
+}
+
+__device__ __forceinline__ void atomic_add_u128(ull *lo_p, ull *hi_p, ull lo, ull hi)
+{
+    ull old = atomicAdd(lo_p, lo);
+    ull carry = (old + lo < old) ? 1ull : 0ull;
+    if (hi + carry) atomicAdd(hi_p, hi + carry);
+}
+
+// Adds val into base[key] for keys sorted within the wave: lanes holding the
+// same key (contiguous runs) are combined first, one atomic per run.
+__device__ __forceinline__ void seg_atomic_add(bool valid, long long key, ull val, ull *base)
+{
+    const int lane = threadIdx.x & 63;
+    long long k = valid ? key : LLONG_MIN;
+    ull v = valid ? val : 0ull;
+    for (int off = 1; off < 64; off <<= 1) {
+        ull ov = __shfl_up(v, off, 64);
+        long long ok = __shfl_up(k, off, 64);
+        if (lane >= off && ok == k) v += ov;
+    }
+    long long nk = __shfl_down(k, 1, 64);
+    bool tail = (lane == 63) || (nk != k);
+    if (valid && tail && v != 0) atomicAdd(&base[k], v);
+}
+
+// ---------------------------------------------------------------------------
+// Input validation and weight units (the reference reads float weights and
+// truncates sums to int, LouvainCUDA/InitKernel.cu:119, StatusCUDA.h:31-41).
+
+// Smallest s >= 0 with w * 2^s integral, from the float32 bit pattern; -1 for
+// a negative, infinite or NaN weight.
+__device__ __forceinline__ int f32_unit_exponent(uint32_t x)
+{
+    if ((x & 0x7fffffffu) == 0) return 0;  // +0 or -0
+    if (x >> 31) return -1;
+    uint32_t ex = (x >> 23) & 0xffu, mant = x & 0x7fffffu;
+    if (ex == 0xffu) return -1;
+    uint32_t M = ex ? (mant | 0x800000u) : mant;
+    int e2 = ex ? (int)ex - 150 : -149;  // w = M * 2^e2
+    int s = -e2 - __builtin_ctz(M);
+    return s > 0 ? s : 0;
+}
+
+__global__ void k_weight_exponent(const void *w, int wtype, int64_t e, DevFlags *f)
+{
+    int smax = 0;
+    int bad = 0;
+    GRID_STRIDE(i, e)
+    {
+        if (wtype == LV_WEIGHT_INT32) {
+            if (((const int32_t *)w)[i] < 0) bad = 1;
+        } else {
+            int s = f32_unit_exponent(((const uint32_t *)w)[i]);
+            if (s < 0)
+                bad = 1;
+            else
+                smax = max(smax, s);
+        }
+    }
+    smax = wave_max_i(smax);
+    bad = wave_max_i(bad);
+    if ((threadIdx.x & 63) == 0) {
+        if (smax) atomicMax(&f->scale, smax);
+        if (bad) atomicOr(&f->err, 1);
+    }
+}
+
+__global__ void k_weight_units(const void *w, int wtype, int64_t e, int scale, int64_t *wu,
+                               DevFlags *f)
+{
+    ull tot = 0, msum = 0;
+    int bad = 0;
+    GRID_STRIDE(i, e)
+    {
+        int64_t u;
+        if (wtype == LV_WEIGHT_INT32) {
+            u = ((const int32_t *)w)[i];
+        } else {
+            uint32_t x = ((const uint32_t *)w)[i];
+            if ((x & 0x7fffffffu) == 0) {
+                u = 0;
+            } else {
+                uint32_t ex = (x >> 23) & 0xffu, mant = x & 0x7fffffu;
+                uint64_t M = ex ? (mant | 0x800000u) : mant;
+                int sh = (ex ? (int)ex - 150 : -149) + scale;
+                if (sh > 36) {
+                    bad = 2;
+                    u = 0;
+                } else {
+                    u = (int64_t)(sh >= 0 ? (M << sh) : (M >> (-sh)));
+                }
+            }
+        }
+        if (u >= kUnitLimit) bad = 2;
+        wu[i] = u;
+        tot += (ull)(u == 0 ? 1 : u);
+        msum += (ull)u;
+        if (tot >= (ull)kUnitLimit) bad = 2;
+    }
+    tot = (ull)wave_sum((long long)tot);  // each lane < 2^60: a wave sum cannot wrap
+    msum = (ull)wave_sum((long long)msum);
+    bad = wave_max_i(bad);
+    if ((threadIdx.x & 63) == 0) {
+        if (tot) {
+            ull old = atomicAdd((ull *)&f->total_units, tot);
+            if (old + tot >= (ull)kUnitLimit || tot >= (ull)kUnitLimit) bad = 2;
+        }
+        if (msum) atomicAdd((ull *)&f->m_units, msum);
+        if (bad) atomicOr(&f->err, bad);
+    }
+}
+
+__global__ void k_check_endpoints(const int32_t *src, const int32_t *dst, int64_t e, int32_t n,
+                                  DevFlags *f)
+{
+    int bad = 0;
+    GRID_STRIDE(i, e)
+    {
+        int32_t a = src[i], b = dst[i];
+        if (a < 0 || a >= n || b < 0 || b >= n) bad = 1;
+    }
+    bad = wave_max_i(bad);
+    if ((threadIdx.x & 63) == 0 && bad) atomicOr(&f->err, 1);
+}
+
+__global__ void k_ids_to_keys(const int32_t *ids, uint32_t *keys, int32_t n)
+{
+    GRID_STRIDE(i, n) keys[i] = (uint32_t)ids[i] ^ 0x80000000u;  // signed order
+}
+
+__global__ void k_check_sorted_unique(const uint32_t *keys, int64_t n, DevFlags *f)
+{
+    int bad = 0;
+    GRID_STRIDE(i, n)
+    {
+        if (i > 0 && keys[i] == keys[i - 1]) bad = 1;
+    }
+    bad = wave_max_i(bad);
+    if ((threadIdx.x & 63) == 0 && bad) atomicOr(&f->err, 1);
+}
+
+__global__ void k_map_endpoints(const uint32_t *skeys, const int32_t *spos, int32_t n, int32_t *ep,
+                                int64_t e, DevFlags *f)
+{
+    int bad = 0;
+    GRID_STRIDE(i, e)
+    {
+        uint32_t key = (uint32_t)ep[i] ^ 0x80000000u;
+        int32_t lo = 0, hi = n - 1, found = -1;
+        while (lo <= hi) {
+            int32_t mid = lo + ((hi - lo) >> 1);
+            uint32_t k = skeys[mid];
+            if (k == key) {
+                found = spos[mid];
+                break;
+            }
+            if (k < key)
+                lo = mid + 1;
+            else
+                hi = mid - 1;
+        }
+        if (found < 0) {
+            bad = 1;
+            found = 0;
+        }
+        ep[i] = found;
+    }
+    bad = wave_max_i(bad);
+    if ((threadIdx.x & 63) == 0 && bad) atomicOr(&f->err, 1);
+}
+
+void launch_weight_exponent(hipStream_t s, const void *w, int wtype, int64_t e, DevFlags *f)
+{
+    if (e > 0) hipLaunchKernelGGL(k_weight_exponent, grid_for(e, 256, 8192), 256, 0, s, w, wtype, e, f);
+}
+
+void launch_weight_units(hipStream_t s, const void *w, int wtype, int64_t e, int scale,
+                         int64_t *wu, DevFlags *f)
+{
+    if (e > 0)
+        hipLaunchKernelGGL(k_weight_units, grid_for(e, 256, 8192), 256, 0, s, w, wtype, e, scale,
+                           wu, f);
+}
+
+void launch_check_endpoints(hipStream_t s, const int32_t *src, const int32_t *dst, int64_t e,
+                            int32_t n, DevFlags *f)
+{
+    if (e > 0) hipLaunchKernelGGL(k_check_endpoints, grid_for(e, 256, 8192), 256, 0, s, src, dst, e, n, f);
+}
+
+void launch_check_sorted_unique(hipStream_t s, const uint32_t *keys, int64_t n, DevFlags *f)
+{
+    if (n > 1) hipLaunchKernelGGL(k_check_sorted_unique, grid_for(n, 256), 256, 0, s, keys, n, f);
+}
+
+void launch_map_endpoints(hipStream_t s, const uint32_t *sorted_ids, const int32_t *sorted_pos,
+                          int32_t n, int32_t *ep, int64_t e, DevFlags *f)
+{
+    if (e > 0)
+        hipLaunchKernelGGL(k_map_endpoints, grid_for(e, 256), 256, 0, s, sorted_ids, sorted_pos, n,
+                           ep, e, f);
+}
+
+void launch_ids_to_keys(hipStream_t s, const int32_t *ids, uint32_t *keys, int32_t n)
+{
+    if (n > 0) hipLaunchKernelGGL(k_ids_to_keys, grid_for(n, 256), 256, 0, s, ids, keys, n);
+}
+
+// ---------------------------------------------------------------------------
+// CSR build (CalculateNeighbour, LouvainCUDA/NeighbourCompute.cu:42-118, plus the
+// degree and self-loop part of initValuesKernel, InitKernel.cu:37-77,101-126).
+// Non-self edges become two row entries; self-loops stay out of the rows:
+// selfsum[i] = sum of (w==0?1:w) over the vertex's loops, k_i gains 4x that,
+// loop_i = the weight of the vertex's last loop in input order.
+
+__global__ void k_csr_count(const int32_t *src, const int32_t *dst, const int64_t *w, int64_t e,
+                            int64_t *cnt, int64_t *kdeg, int64_t *selfsum, ull *lastloop)
+{
+    GRID_STRIDE(i, e)
+    {
+        int32_t a = src[i], b = dst[i];
+        int64_t u = w[i];
+        ull wp = (ull)(u == 0 ? 1 : u);
+        if (a != b) {
+            atomicAdd((ull *)&cnt[a], 1ull);
+            atomicAdd((ull *)&cnt[b], 1ull);
+            atomicAdd((ull *)&kdeg[a], wp);
+            atomicAdd((ull *)&kdeg[b], wp);
+        } else {
+            atomicAdd((ull *)&selfsum[a], wp);
+            atomicAdd((ull *)&kdeg[a], 4ull * wp);
+            atomicMax(&lastloop[a], (ull)(i + 1));
+        }
+    }
+}
+
+__global__ void k_csr_fill(const int32_t *src, const int32_t *dst, const int64_t *w, int64_t e,
+                           const int64_t *rowptr, int64_t *cursor, int32_t *col, int64_t *cw)
+{
+    GRID_STRIDE(i, e)
+    {
+        int32_t a = src[i], b = dst[i];
+        if (a == b) continue;
+        int64_t u = w[i];
+        int64_t p = rowptr[a] + (int64_t)atomicAdd((ull *)&cursor[a], 1ull);
+        col[p] = b;
+        cw[p] = u;
+        int64_t q = rowptr[b] + (int64_t)atomicAdd((ull *)&cursor[b], 1ull);
+        col[q] = a;
+        cw[q] = u;
+    }
+}
+
+__global__ void k_loop_gather(const ull *lastloop, const int64_t *w, int32_t n, int64_t *loopw)
+{
+    GRID_STRIDE(i, n)
+    {
+        ull j = lastloop[i];
+        loopw[i] = j ? w[j - 1] : 0;
+    }
+}
+
+void launch_csr_count(hipStream_t s, const int32_t *src, const int32_t *dst, const int64_t *w,
+                      int64_t e, int64_t *cnt, int64_t *kdeg, int64_t *selfsum, ull *lastloop)
+{
+    if (e > 0)
+        hipLaunchKernelGGL(k_csr_count, grid_for(e, 256), 256, 0, s, src, dst, w, e, cnt, kdeg,
+                           selfsum, lastloop);
+}
+
+void launch_csr_fill(hipStream_t s, const int32_t *src, const int32_t *dst, const int64_t *w,
+                     int64_t e, const int64_t *rowptr, int64_t *cursor, int32_t *col, int64_t *cw)
+{
+    if (e > 0)
+        hipLaunchKernelGGL(k_csr_fill, grid_for(e, 256), 256, 0, s, src, dst, w, e, rowptr, cursor,
+                           col, cw);
+}
+
+void launch_loop_gather(hipStream_t s, const ull *lastloop, const int64_t *w, int32_t n,
+                        int64_t *loopw)
+{
+    if (n > 0) hipLaunchKernelGGL(k_loop_gather, grid_for(n, 256), 256, 0, s, lastloop, w, n, loopw);
+}
+
+// ---------------------------------------------------------------------------
+// Per-level init (initCUDA, LouvainCUDA/InitKernel.cu:79-126; communityInitKernel,
+// OneLevelKernel.cu:80-91) and the degree buckets of the sweep.
+
+__device__ __forceinline__ int tier_of(int64_t d, int64_t q)
+{
+    if (d == 0) return 0;  // no neighbours: the vertex can never move
+#pragma unroll
+    for (int t = 0; t < kTiers - 1; ++t)
+        if (q <= kTierMaxQ[t]) return t + 1;
+    return kTiers;
+}
+
+__global__ __launch_bounds__(256) void k_init_vertex(
+    int32_t n, const int64_t *rowptr, const int64_t *kdeg, const int64_t *loopw,
+    const int64_t *selfsum, int64_t *deg, int64_t *in, int32_t *size, ull *ddeg, ull *din,
+    int32_t *dsize, int32_t *C, int32_t *next, uint32_t *tier_key, int32_t *verts,
+    long long *tier_cnt, long long *tier_nnz)
+{
+    __shared__ long long s_cnt[kTiers + 1], s_nnz[kTiers + 1];
+    if (threadIdx.x <= kTiers) {
+        s_cnt[threadIdx.x] = 0;
+        s_nnz[threadIdx.x] = 0;
+    }
+    __syncthreads();
+    GRID_STRIDE(i, n)
+    {
+        deg[i] = kdeg[i];
+        in[i] = loopw[i];
+        size[i] = 1;
+        ddeg[i] = 0;
+        din[i] = 0;
+        dsize[i] = 0;
+        C[i] = (int32_t)i;
+        next[i] = (int32_t)i;
+        int64_t d = rowptr[i + 1] - rowptr[i];
+        int t = tier_of(d, d + (selfsum[i] > 0 ? 1 : 0));
+        tier_key[i] = (uint32_t)t;
+        verts[i] = (int32_t)i;
+        atomicAdd((ull *)&s_cnt[t], 1ull);
+        atomicAdd((ull *)&s_nnz[t], (ull)d);
+    }
+    __syncthreads();
+    if (threadIdx.x <= kTiers) {
+        if (s_cnt[threadIdx.x]) atomicAdd((ull *)&tier_cnt[threadIdx.x], (ull)s_cnt[threadIdx.x]);
+        if (s_nnz[threadIdx.x]) atomicAdd((ull *)&tier_nnz[threadIdx.x], (ull)s_nnz[threadIdx.x]);
+    }
+}
+
+__global__ void k_hub_caps(const int32_t *hv, int32_t nh, const int64_t *rowptr,
+                           const int64_t *selfsum, int64_t *cap, int32_t *log2cap)
+{
+    GRID_STRIDE(j, nh)
+    {
+        int32_t v = hv[j];
+        int64_t q = rowptr[v + 1] - rowptr[v] + (selfsum[v] > 0 ? 1 : 0);
+        int l = 1;
+        while ((int64_t(1) << l) < 2 * q) ++l;
+        cap[j] = int64_t(1) << l;
+        log2cap[j] = l;
+    }
+}
+
+void launch_init_vertex(hipStream_t s, int32_t n, const int64_t *rowptr, const int64_t *kdeg,
+                        const int64_t *loopw, const int64_t *selfsum, int64_t *deg, int64_t *in,
+                        int32_t *size, ull *ddeg, ull *din, int32_t *dsize, int32_t *C,
+                        int32_t *next, uint32_t *tier_key, int32_t *verts, long long *tier_cnt,
+                        long long *tier_nnz)
+{
+    if (n > 0)
+        hipLaunchKernelGGL(k_init_vertex, grid_for(n, 256, 8192), 256, 0, s, n, rowptr, kdeg,
+                           loopw, selfsum, deg, in, size, ddeg, din, dsize, C, next, tier_key,
+                           verts, tier_cnt, tier_nnz);
+}
+
+void launch_hub_caps(hipStream_t s, const int32_t *hub_verts, int32_t nh, const int64_t *rowptr,
+                     const int64_t *selfsum, int64_t *cap, int32_t *log2cap)
+{
+    if (nh > 0)
+        hipLaunchKernelGGL(k_hub_caps, grid_for(nh, 256), 256, 0, s, hub_verts, nh, rowptr,
+                           selfsum, cap, log2cap);
+}
+
+// ---------------------------------------------------------------------------
+// Exact Q aggregates (modularity kernel + thrust::reduce,
+// LouvainCUDA/OneLevelKernel.cu:427-451,875-880) and the Jacobi apply step
+// (communitySetKernel/mergeSetKernel, OneLevelKernel.cu:93-136).
+
+__device__ __forceinline__ void block_qacc(long long sin, ull lo, ull hi, QAcc *acc)
+{
+    sin = wave_sum(sin);
+    wave_sum_u128(lo, hi);
+    if ((threadIdx.x & 63) == 0) {
+        if (sin) atomicAdd((ull *)&acc->sum_in, (ull)sin);
+        if (lo | hi) atomic_add_u128(&acc->d2_lo, &acc->d2_hi, lo, hi);
+    }
+}
+
+__global__ void k_qagg(int32_t n, const int64_t *deg, const int64_t *in, QAcc *acc)
+{
+    long long sin = 0;
+    ull lo = 0, hi = 0;
+    GRID_STRIDE(c, n)
+    {
+        sin += in[c];
+        acc_sq(lo, hi, (ull)deg[c]);
+    }
+    block_qacc(sin, lo, hi, acc);
+}
+
+__global__ void k_qagg_ull(int32_t n, const ull *dc, const ull *lc, QAcc *acc)
+{
+    long long sin = 0;
+    ull lo = 0, hi = 0;
+    GRID_STRIDE(c, n)
+    {
+        sin += (long long)lc[c];
+        acc_sq(lo, hi, dc[c]);
+    }
+    block_qacc(sin, lo, hi, acc);
+}
+
+__global__ void k_zero_qacc(QAcc *acc)
+{
+    acc->sum_in = 0;
+    acc->d2_lo = 0;
+    acc->d2_hi = 0;
+    acc->pad = 0;
+}
+
+__global__ void k_apply(int32_t n, int64_t *deg, int64_t *in, int32_t *size, ull *ddeg, ull *din,
+                        int32_t *dsize, QAcc *acc)
+{
+    long long sin = 0;
+    ull lo = 0, hi = 0;
+    GRID_STRIDE(c, n)
+    {
+        int64_t dv = deg[c], iv = in[c];
+        ull dd = ddeg[c], di = din[c];
+        int32_t ds = dsize[c];
+        if (dd | di | (ull)(uint32_t)ds) {
+            dv += (int64_t)dd;
+            iv += (int64_t)di;
+            deg[c] = dv;
+            in[c] = iv;
+            size[c] += ds;
+            ddeg[c] = 0;
+            din[c] = 0;
+            dsize[c] = 0;
+        }
+        sin += iv;
+        acc_sq(lo, hi, (ull)dv);
+    }
+    block_qacc(sin, lo, hi, acc);
+}
+
+void launch_qagg(hipStream_t s, int32_t n, const int64_t *deg, const int64_t *in, QAcc *acc)
+{
+    if (n > 0) hipLaunchKernelGGL(k_qagg, grid_for(n, 256, 4096), 256, 0, s, n, deg, in, acc);
+}
+
+void launch_qagg_ull(hipStream_t s, int32_t n, const ull *dc, const ull *lc, QAcc *acc)
+{
+    if (n > 0) hipLaunchKernelGGL(k_qagg_ull, grid_for(n, 256, 4096), 256, 0, s, n, dc, lc, acc);
+}
+
+void launch_zero_qacc(hipStream_t s, QAcc *acc) { hipLaunchKernelGGL(k_zero_qacc, 1, 1, 0, s, acc); }
+
+void launch_apply(hipStream_t s, int32_t n, int64_t *deg, int64_t *in, int32_t *size, ull *ddeg,
+                  ull *din, int32_t *dsize, QAcc *acc)
+{
+    if (n > 0)
+        hipLaunchKernelGGL(k_apply, grid_for(n, 256, 4096), 256, 0, s, n, deg, in, size, ddeg,
+                           din, dsize, acc);
+}
+
+// ---------------------------------------------------------------------------
+// The local-moving sweep (neighbourCommunitySetKernel + sort/reduce_by_key +
+// SetNewCommunityWarpShuffle + oneLevelKernel, LouvainCUDA/OneLevelKernel.cu:28-78,
+// 248-422, and on the last pass mergeIsolatedNodeKernel :138-171).
+//
+// For each vertex v the distinct communities of its neighbours are gathered
+// into a hash table keyed by community id (LDS for q <= 2048, a global table
+// for hubs), summing the weights W_vc; a self-loop adds the own community
+// with W = 0.  Every occupied slot counts towards n_v; slots with W != 0 give
+// g = W - ((gamma*deg[c])*f), f = k_v/(2m).  The choice is the lowest c among
+// the maximal g when that maximum is > 0, otherwise the own community - the
+// outcome of the reference's ascending scan with its replace rule.
+
+__device__ __forceinline__ uint32_t hslot(int32_t c, int log2cap)
+{
+    return ((uint32_t)c * 0x9E3779B1u) >> (32 - log2cap);
+}
+
+__device__ __forceinline__ void lds_insert(int32_t *key, ull *val, int log2cap, int32_t c,
+                                           int64_t w)
+{
+    const uint32_t mask = (1u << log2cap) - 1u;
+    uint32_t s = hslot(c, log2cap);
+    for (;;) {
+        int32_t k = ((volatile int32_t *)key)[s];
+        if (k == c) break;
+        if (k == -1) {
+            int32_t prev = atomicCAS(&key[s], -1, c);
+            if (prev == -1 || prev == c) break;
+        }
+        s = (s + 1) & mask;
+    }
+    if (w != 0) atomicAdd(&val[s], (ull)w);
+}
+
+__device__ __forceinline__ void glb_insert(int32_t *key, ull *val, int log2cap, int32_t c,
+                                           int64_t w)
+{
+    const uint32_t mask = (1u << log2cap) - 1u;
+    uint32_t s = hslot(c, log2cap);
+    for (;;) {
+        int32_t k = __hip_atomic_load(&key[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (k == c) break;
+        if (k == -1) {
+            int32_t prev = atomicCAS(&key[s], -1, c);
+            if (prev == -1 || prev == c) break;
+        }
+        s = (s + 1) & mask;
+    }
+    if (w != 0) atomicAdd(&val[s], (ull)w);
+}
+
+__device__ __forceinline__ bool better(double g1, int32_t c1, double g2, int32_t c2)
+{
+    return g1 > g2 || (g1 == g2 && c1 < c2);
+}
+
+__device__ __forceinline__ void consider(int32_t c, int64_t W, const SweepArgs &a, double f,
+                                         int &cnt, double &bg, int32_t &bc, int64_t &bw)
+{
+    ++cnt;
+    if (W != 0) {
+        double g = (double)W - ((a.gamma * (double)a.deg[c]) * f);
+        if (g > 0.0 && better(g, c, bg, bc)) {
+            bg = g;
+            bc = c;
+            bw = W;
+        }
+    }
+}
+
+template <int W>
+__device__ __forceinline__ void group_reduce(int &cnt, double &bg, int32_t &bc, int64_t &bw)
+{
+#pragma unroll
+    for (int off = W / 2; off > 0; off >>= 1) {
+        cnt += __shfl_xor(cnt, off, W);
+        double og = __shfl_xor(bg, off, W);
+        int32_t oc = __shfl_xor(bc, off, W);
+        long long ow = __shfl_xor((long long)bw, off, W);
+        if (better(og, oc, bg, bc)) {
+            bg = og;
+            bc = oc;
+            bw = ow;
+        }
+    }
+}
+
+// Guard, move and deltas of one vertex (oneLevelKernel + remove/insert,
+// OneLevelKernel.cu:52-78,400-421); on the last pass the deltas are the
+// merge-isolated ones only (:109-171,921-927).
+__device__ __forceinline__ void finalize_vertex(const SweepArgs &a, int32_t v, int32_t own,
+                                                int n, int32_t bc, int64_t bw)
+{
+    const bool has = bc != INT_MAX;
+    const int32_t c0 = has ? bc : own;
+    const int64_t w0 = has ? bw : 0;
+    const int32_t so = a.size[own];
+    int32_t choice = c0;
+    if (so == 1 && a.size[c0] == 1 && c0 > own) choice = own;
+    const int64_t k = a.kdeg[v], lp = a.loopw[v];
+    if (!a.last) {
+        a.next[v] = choice;
+        if (choice != own) {
+            atomicAdd(&a.ddeg[own], (ull)(-k));
+            atomicAdd(&a.ddeg[choice], (ull)k);
+            atomicAdd(&a.din[own], (ull)(-(w0 + lp)));
+            atomicAdd(&a.din[choice], (ull)(w0 + lp));
+            atomicAdd(&a.dsize[own], -1);
+            atomicAdd(&a.dsize[choice], 1);
+        }
+    } else {
+        int32_t nv = choice;
+        if (n == 1 && so == 1 && c0 != own && own < c0) {
+            nv = c0;
+            atomicAdd(&a.ddeg[own], (ull)(-k));
+            atomicAdd(&a.ddeg[c0], (ull)k);
+            atomicAdd(&a.din[own], (ull)(-lp));
+            atomicAdd(&a.din[c0], (ull)lp);
+            atomicAdd(&a.dsize[own], -1);
+            atomicAdd(&a.dsize[c0], 1);
+        }
+        a.next[v] = nv;
+    }
+}
+
+// G lanes per vertex, 256/G vertices per block, LDS table of 2^LOG2CAP slots
+// per vertex (capacity >= 2q).
+template <int G, int LOG2CAP>
+__global__ __launch_bounds__(256) void k_sweep_group(SweepArgs a, const int32_t *__restrict__ verts,
+                                                    int32_t nv)
+{
+    constexpr int CAP = 1 << LOG2CAP;
+    constexpr int GPB = 256 / G;
+    __shared__ int32_t s_key[GPB * CAP];
+    __shared__ ull s_val[GPB * CAP];
+    const int lg = threadIdx.x % G, grp = threadIdx.x / G;
+    int32_t *key = s_key + grp * CAP;
+    ull *val = s_val + grp * CAP;
+    for (int64_t base = (int64_t)blockIdx.x * GPB; base < nv; base += (int64_t)gridDim.x * GPB) {
+        const int64_t vi = base + grp;
+        const bool active = vi < nv;
+#pragma unroll
+        for (int s = lg; s < CAP; s += G) {
+            key[s] = -1;
+            val[s] = 0;
+        }
+        __syncthreads();
+        int32_t v = 0, own = 0;
+        if (active) {
+            v = verts[vi];
+            own = a.C[v];
+            const int64_t beg = a.rowptr[v], end = a.rowptr[v + 1];
+            for (int64_t p = beg + lg; p < end; p += G) lds_insert(key, val, LOG2CAP, a.C[a.col[p]], a.w[p]);
+            if (lg == 0 && a.selfsum[v] > 0) lds_insert(key, val, LOG2CAP, own, 0);
+        }
+        __syncthreads();
+        int cnt = 0;
+        double bg = 0.0;
+        int32_t bc = INT_MAX;
+        int64_t bw = 0;
+        if (active) {
+            const double f = (double)a.kdeg[v] / a.m2;
+#pragma unroll
+            for (int s = lg; s < CAP; s += G) {
+                int32_t c = key[s];
+                if (c >= 0) consider(c, (int64_t)val[s], a, f, cnt, bg, bc, bw);
+            }
+        }
+        group_reduce<G>(cnt, bg, bc, bw);
+        if (active && lg == 0) finalize_vertex(a, v, own, cnt, bc, bw);
+        __syncthreads();
+    }
+}
+
+// One vertex per block of BLOCK threads; LDS table of 2^LOG2CAP slots, or for
+// hubs (GLOBAL) a private table of 2^hlog2[j] slots at hoff[j] in HBM.
+template <int BLOCK, int LOG2CAP, bool GLOBAL>
+__global__ __launch_bounds__(BLOCK) void k_sweep_block(SweepArgs a, const int32_t *__restrict__ verts,
+                                                      int32_t nv)
+{
+    constexpr int LCAP = GLOBAL ? 1 : (1 << LOG2CAP);
+    constexpr int NW = BLOCK / 64;
+    __shared__ int32_t s_key[LCAP];
+    __shared__ ull s_val[LCAP];
+    __shared__ int r_cnt[NW];
+    __shared__ double r_g[NW];
+    __shared__ int32_t r_c[NW];
+    __shared__ long long r_w[NW];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int64_t vi = blockIdx.x; vi < nv; vi += gridDim.x) {
+        const int32_t v = verts[vi];
+        const int32_t own = a.C[v];
+        int32_t *key;
+        ull *val;
+        int l2;
+        if (GLOBAL) {
+            key = a.hkeys + a.hoff[vi];
+            val = a.hvals + a.hoff[vi];
+            l2 = a.hlog2[vi];
+        } else {
+            key = s_key;
+            val = s_val;
+            l2 = LOG2CAP;
+        }
+        const int64_t cap = int64_t(1) << l2;
+        for (int64_t s = tid; s < cap; s += BLOCK) {
+            key[s] = -1;
+            val[s] = 0;
+        }
+        if (GLOBAL) __threadfence();
+        __syncthreads();
+        const int64_t beg = a.rowptr[v], end = a.rowptr[v + 1];
+        for (int64_t p = beg + tid; p < end; p += BLOCK) {
+            if (GLOBAL)
+                glb_insert(key, val, l2, a.C[a.col[p]], a.w[p]);
+            else
+                lds_insert(key, val, l2, a.C[a.col[p]], a.w[p]);
+        }
+        if (tid == 0 && a.selfsum[v] > 0) {
+            if (GLOBAL)
+                glb_insert(key, val, l2, own, 0);
+            else
+                lds_insert(key, val, l2, own, 0);
+        }
+        if (GLOBAL) __threadfence();
+        __syncthreads();
+        int cnt = 0;
+        double bg = 0.0;
+        int32_t bc = INT_MAX;
+        int64_t bw = 0;
+        const double f = (double)a.kdeg[v] / a.m2;
+        for (int64_t s = tid; s < cap; s += BLOCK) {
+            int32_t c;
+            int64_t W;
+            if (GLOBAL) {
+                c = __hip_atomic_load(&key[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                W = (int64_t)__hip_atomic_load(&val[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+                c = key[s];
+                W = (int64_t)val[s];
+            }
+            if (c >= 0) consider(c, W, a, f, cnt, bg, bc, bw);
+        }
+        group_reduce<64>(cnt, bg, bc, bw);
+        if (lane == 0) {
+            r_cnt[wave] = cnt;
+            r_g[wave] = bg;
+            r_c[wave] = bc;
+            r_w[wave] = bw;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            int tc = r_cnt[0];
+            double g0 = r_g[0];
+            int32_t c0 = r_c[0];
+            int64_t w0 = r_w[0];
+            for (int w = 1; w < NW; ++w) {
+                tc += r_cnt[w];
+                if (better(r_g[w], r_c[w], g0, c0)) {
+                    g0 = r_g[w];
+                    c0 = r_c[w];
+                    w0 = r_w[w];
+                }
+            }
+            finalize_vertex(a, v, own, tc, c0, w0);
+        }
+        __syncthreads();
+    }
+}
+
+template <typename KernelT>
+static void launch_grid(KernelT k, int64_t blocks, int block, hipStream_t s, const SweepArgs &a,
+                        const int32_t *verts, int32_t nv)
+{
+    int g = (int)(blocks > (1 << 20) ? (1 << 20) : (blocks < 1 ? 1 : blocks));
+    hipLaunchKernelGGL(k, g, block, 0, s, a, verts, nv);
+}
+
+void launch_sweep_tier(hipStream_t s, int tier, const SweepArgs &a, const int32_t *verts, int32_t nv)
+{
+    if (nv <= 0) return;
+    switch (tier) {
+    case 1: launch_grid(k_sweep_group<4, 3>, (nv + 63) / 64, 256, s, a, verts, nv); break;
+    case 2: launch_grid(k_sweep_group<8, 4>, (nv + 31) / 32, 256, s, a, verts, nv); break;
+    case 3: launch_grid(k_sweep_group<16, 5>, (nv + 15) / 16, 256, s, a, verts, nv); break;
+    case 4: launch_grid(k_sweep_group<32, 6>, (nv + 7) / 8, 256, s, a, verts, nv); break;
+    case 5: launch_grid(k_sweep_group<64, 7>, (nv + 3) / 4, 256, s, a, verts, nv); break;
+    case 6: launch_grid(k_sweep_group<64, 10>, (nv + 3) / 4, 256, s, a, verts, nv); break;
+    case 7: launch_grid(k_sweep_block<256, 12, false>, nv, 256, s, a, verts, nv); break;
+    case 8: launch_grid(k_sweep_block<1024, 0, true>, nv, 1024, s, a, verts, nv); break;
+    default: break;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Renumbering (renumberCUDA, LouvainCUDA/RenumberKernel.cu:63-220): the new id of
+// a community is the rank of its smallest member position.
+
+__global__ void k_leader(const int32_t *C, int32_t n, int32_t *leader)
+{
+    GRID_STRIDE(i, n) atomicMin(&leader[C[i]], (int32_t)i);
+}
+
+__global__ void k_leader_flag(const int32_t *C, const int32_t *leader, int32_t n, int32_t *flag)
+{
+    GRID_STRIDE(i, n) flag[i] = (leader[C[i]] == (int32_t)i) ? 1 : 0;
+}
+
+__global__ void k_newc(const int32_t *C, const int32_t *leader, const int32_t *rank, int32_t n,
+                       int32_t *newc)
+{
+    GRID_STRIDE(i, n) newc[i] = rank[leader[C[i]]];
+}
+
+void launch_leader(hipStream_t s, const int32_t *C, int32_t n, int32_t *leader)
+{
+    if (n > 0) hipLaunchKernelGGL(k_leader, grid_for(n, 256), 256, 0, s, C, n, leader);
+}
+
+void launch_leader_flag(hipStream_t s, const int32_t *C, const int32_t *leader, int32_t n,
+                        int32_t *flag)
+{
+    if (n > 0) hipLaunchKernelGGL(k_leader_flag, grid_for(n, 256), 256, 0, s, C, leader, n, flag);
+}
+
+void launch_newc(hipStream_t s, const int32_t *C, const int32_t *leader, const int32_t *rank,
+                 int32_t n, int32_t *newc)
+{
+    if (n > 0) hipLaunchKernelGGL(k_newc, grid_for(n, 256), 256, 0, s, C, leader, rank, n, newc);
+}
+
+// ---------------------------------------------------------------------------
+// Coarsening (inducedGraphCUDA, LouvainCUDA/InducedGraphKernel.cu:28-113).
+// Keys (newc[a]*K + newc[b]) over every row entry a->b with value w'=(w==0?1:w),
+// plus one key (newc[s], newc[s]) per vertex with value 2*selfsum[s].  After
+// the radix sort and segmented sum, a key (x,y), x != y, carries
+// E'(x,y) + E'(y,x) - the next level's row entry - and (x,x) carries 2*E'(x,x).
+
+__global__ __launch_bounds__(256) void k_coarsen_entry_keys(const int64_t *rowptr, const int32_t *col,
+                                                           const int64_t *w, int32_t n, int64_t nnz,
+                                                           const int32_t *newc, uint64_t K,
+                                                           uint64_t *keys, int64_t *vals)
+{
+    __shared__ int32_t s_lo, s_hi;
+    for (int64_t base = (int64_t)blockIdx.x * 256; base < nnz; base += (int64_t)gridDim.x * 256) {
+        if (threadIdx.x == 0) {
+            // rows containing the first and last entry of this chunk
+            int64_t last = base + 255 < nnz ? base + 255 : nnz - 1;
+            int32_t lo = 0, hi = n;
+            while (hi - lo > 1) {
+                int32_t mid = (lo + hi) >> 1;
+                if (rowptr[mid] <= base) lo = mid; else hi = mid;
+            }
+            s_lo = lo;
+            int32_t lo2 = lo;
+            hi = n;
+            while (hi - lo2 > 1) {
+                int32_t mid = (lo2 + hi) >> 1;
+                if (rowptr[mid] <= last) lo2 = mid; else hi = mid;
+            }
+            s_hi = lo2;
+        }
+        __syncthreads();
+        int64_t p = base + threadIdx.x;
+        if (p < nnz) {
+            int32_t lo = s_lo, hi = s_hi + 1;
+            while (hi - lo > 1) {
+                int32_t mid = (lo + hi) >> 1;
+                if (rowptr[mid] <= p) lo = mid; else hi = mid;
+            }
+            int64_t u = w[p];
+            keys[p] = (uint64_t)newc[lo] * K + (uint64_t)newc[col[p]];
+            vals[p] = u == 0 ? 1 : u;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_coarsen_self_keys(int32_t n, int64_t nnz, const int64_t *selfsum,
+                                    const int32_t *newc, uint64_t K, uint64_t *keys, int64_t *vals)
+{
+    GRID_STRIDE(i, n)
+    {
+        uint64_t c = (uint64_t)newc[i];
+        keys[nnz + i] = c * K + c;
+        vals[nnz + i] = 2 * selfsum[i];
+    }
+}
+
+void launch_coarsen_keys(hipStream_t s, const int64_t *rowptr, const int32_t *col,
+                         const int64_t *w, int32_t n, int64_t nnz, const int64_t *selfsum,
+                         const int32_t *newc, uint64_t K, uint64_t *keys, int64_t *vals)
+{
+    if (nnz > 0)
+        hipLaunchKernelGGL(k_coarsen_entry_keys, grid_for(nnz, 256, 1 << 20), 256, 0, s, rowptr,
+                           col, w, n, nnz, newc, K, keys, vals);
+    if (n > 0)
+        hipLaunchKernelGGL(k_coarsen_self_keys, grid_for(n, 256), 256, 0, s, n, nnz, selfsum, newc,
+                           K, keys, vals);
+}
+
+__global__ void k_seg_heads(const uint64_t *keys, int64_t n, int64_t *head)
+{
+    GRID_STRIDE(i, n) head[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1 : 0;
+}
+
+// hscan = exclusive scan of head: element i belongs to unique index
+// hscan[i] + head[i] - 1.  Runs are combined within the wave before the atomic.
+__global__ void k_seg_sum(const uint64_t *keys, const int64_t *vals, const int64_t *hscan,
+                          const int64_t *head, int64_t n, uint64_t *ukey, ull *usum)
+{
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n;
+         base += (int64_t)gridDim.x * blockDim.x) {
+        int64_t i = base + threadIdx.x;
+        bool valid = i < n;
+        long long u = 0;
+        ull v = 0;
+        if (valid) {
+            u = hscan[i] + head[i] - 1;
+            v = (ull)vals[i];
+            if (head[i]) ukey[u] = keys[i];
+        }
+        seg_atomic_add(valid, u, v, usum);
+    }
+}
+
+void launch_seg_heads(hipStream_t s, const uint64_t *keys, int64_t n, int64_t *head)
+{
+    if (n > 0) hipLaunchKernelGGL(k_seg_heads, grid_for(n, 256, 1 << 20), 256, 0, s, keys, n, head);
+}
+
+void launch_seg_sum(hipStream_t s, const uint64_t *keys, const int64_t *vals, const int64_t *hscan,
+                    const int64_t *head, int64_t n, uint64_t *ukey, ull *usum)
+{
+    if (n > 0)
+        hipLaunchKernelGGL(k_seg_sum, grid_for(n, 256, 1 << 20), 256, 0, s, keys, vals, hscan,
+                           head, n, ukey, usum);
+}
+
+__global__ void k_coarse_split(const uint64_t *ukey, const ull *usum, int64_t U, uint64_t K,
+                               int64_t *nsflag, int64_t *loopw, int64_t *selfsum, int64_t *rowcnt,
+                               int64_t *kdeg)
+{
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < U;
+         base += (int64_t)gridDim.x * blockDim.x) {
+        int64_t u = base + threadIdx.x;
+        bool valid = u < U;
+        long long a = 0;
+        ull kadd = 0, cadd = 0;
+        if (valid) {
+            uint64_t key = ukey[u];
+            a = (long long)(key / K);
+            long long b = (long long)(key % K);
+            ull sv = usum[u];
+            if (a == b) {
+                nsflag[u] = 0;
+                if (sv > 0) {
+                    loopw[a] = (int64_t)(sv / 2);
+                    selfsum[a] = (int64_t)(sv / 2);
+                    kadd = 4 * (sv / 2);
+                }
+            } else {
+                nsflag[u] = 1;
+                kadd = sv;
+                cadd = 1;
+            }
+        }
+        seg_atomic_add(valid, a, kadd, (ull *)kdeg);
+        seg_atomic_add(valid, a, cadd, (ull *)rowcnt);
+    }
+}
+
+__global__ void k_coarse_fill(const uint64_t *ukey, const ull *usum, const int64_t *nsflag,
+                              const int64_t *pos, int64_t U, uint64_t K, int32_t *col, int64_t *cw)
+{
+    GRID_STRIDE(u, U)
+    {
+        if (nsflag[u]) {
+            int64_t p = pos[u];
+            col[p] = (int32_t)(ukey[u] % K);
+            cw[p] = (int64_t)usum[u];
+        }
+    }
+}
+
+__global__ void k_sum_ull(const ull *p, int64_t n, long long *out)
+{
+    long long acc = 0;
+    GRID_STRIDE(i, n) acc += (long long)p[i];
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0 && acc) atomicAdd((ull *)out, (ull)acc);
+}
+
+void launch_coarse_split(hipStream_t s, const uint64_t *ukey, const ull *usum, int64_t u,
+                         uint64_t K, int64_t *nsflag, int64_t *loopw, int64_t *selfsum,
+                         int64_t *rowcnt, int64_t *kdeg)
+{
+    if (u > 0)
+        hipLaunchKernelGGL(k_coarse_split, grid_for(u, 256, 1 << 20), 256, 0, s, ukey, usum, u, K,
+                           nsflag, loopw, selfsum, rowcnt, kdeg);
+}
+
+void launch_coarse_fill(hipStream_t s, const uint64_t *ukey, const ull *usum, const int64_t *nsflag,
+                        const int64_t *pos, int64_t u, uint64_t K, int32_t *col, int64_t *cw,
+                        int64_t *kdeg)
+{
+    (void)kdeg;
+    if (u > 0)
+        hipLaunchKernelGGL(k_coarse_fill, grid_for(u, 256, 1 << 20), 256, 0, s, ukey, usum, nsflag,
+                           pos, u, K, col, cw);
+}
+
+void launch_sum_ull(hipStream_t s, const ull *p, int64_t n, long long *out)
+{
+    if (n > 0) hipLaunchKernelGGL(k_sum_ull, grid_for(n, 256, 4096), 256, 0, s, p, n, out);
+}
+
+// ---------------------------------------------------------------------------
+// Dendrogram flattening and the final modularity.
+
+__global__ void k_compose(int32_t *f, int32_t n, const int32_t *map)
+{
+    GRID_STRIDE(i, n) f[i] = map[f[i]];
+}
+
+__global__ void k_gather_partition(const int32_t *perm, const int32_t *ids, const int32_t *cbp,
+                                   int32_t n, int32_t *node, int32_t *comm)
+{
+    GRID_STRIDE(j, n)
+    {
+        int32_t p = perm[j];
+        node[j] = ids ? ids[p] : p;
+        comm[j] = cbp[p];
+    }
+}
+
+__global__ void k_final_q(const int32_t *src, const int32_t *dst, const int64_t *w, int64_t e,
+                          const int32_t *comm, ull *lc, ull *dc)
+{
+    GRID_STRIDE(i, e)
+    {
+        int32_t a = comm[src[i]], b = comm[dst[i]];
+        ull u = (ull)w[i];
+        if (u == 0) continue;
+        atomicAdd(&dc[a], u);
+        atomicAdd(&dc[b], u);
+        if (a == b) atomicAdd(&lc[a], u);
+    }
+}
+
+void launch_compose(hipStream_t s, int32_t *f, int32_t n, const int32_t *map)
+{
+    if (n > 0) hipLaunchKernelGGL(k_compose, grid_for(n, 256), 256, 0, s, f, n, map);
+}
+
+void launch_gather_partition(hipStream_t s, const int32_t *perm, const int32_t *ids,
+                             const int32_t *comm_by_pos, int32_t n, int32_t *node, int32_t *comm)
+{
+    if (n > 0)
+        hipLaunchKernelGGL(k_gather_partition, grid_for(n, 256), 256, 0, s, perm, ids, comm_by_pos,
+                           n, node, comm);
+}
+
+void launch_final_q(hipStream_t s, const int32_t *src, const int32_t *dst, const int64_t *w,
+                    int64_t e, const int32_t *comm, ull *lc, ull *dc)
+{
+    if (e > 0)
+        hipLaunchKernelGGL(k_final_q, grid_for(e, 256), 256, 0, s, src, dst, w, e, comm, lc, dc);
+}
+
+}  // namespace lv

@@ -246,7 +246,7 @@ static int cmp_group(const void *a, const void *b)
  * OneLevelKernel.cu:43-49 (self weight := 0), :619,644 (sort + reduce_by_key),
  * :265-270,303-316,330,358 (initial best, gain, replace rule). */
 static void row_best(const orc_level *L, int32_t i, const int32_t *C, const int64_t *deg,
-                     double gamma, int64_t *acc, int32_t *stamp, orc_group *gbuf,
+                     double gamma, int64_t *acc, int64_t *stamp, int64_t token, orc_group *gbuf,
                      int32_t *out_c, int64_t *out_w, int32_t *out_n)
 {
     int32_t ng = 0;
@@ -254,8 +254,8 @@ static void row_best(const orc_level *L, int32_t i, const int32_t *C, const int6
         int32_t j = L->col[p];
         int32_t c = C[j];
         int64_t w = (j == i) ? 0 : L->cw[p];
-        if (stamp[c] != i + 1) {
-            stamp[c] = i + 1;
+        if (stamp[c] != token) {   /* first entry of community c in this row */
+            stamp[c] = token;
             acc[c] = 0;
             gbuf[ng++].c = c;
         }
@@ -324,12 +324,18 @@ static int one_level(orc_run *R, orc_level *L)
     nthreads = R->threads > 0 ? R->threads : 1;
 #endif
     int64_t **accs = (int64_t **)xcalloc((size_t)nthreads, sizeof(int64_t *));
-    int32_t **stamps = (int32_t **)xcalloc((size_t)nthreads, sizeof(int32_t *));
+    int64_t **stamps = (int64_t **)xcalloc((size_t)nthreads, sizeof(int64_t *));
+    /* one token per (pass, row): stamps never repeat within a level */
+    int64_t *tokens = (int64_t *)xcalloc((size_t)nthreads, sizeof(int64_t));
     orc_group **gbufs = (orc_group **)xcalloc((size_t)nthreads, sizeof(orc_group *));
-    if (!accs || !stamps || !gbufs) { free(accs); free(stamps); free(gbufs); rc = ORC_ERR_NOMEM; goto out; }
+    if (!accs || !stamps || !gbufs || !tokens) {
+        free(accs); free(stamps); free(gbufs); free(tokens);
+        rc = ORC_ERR_NOMEM;
+        goto out;
+    }
     for (int t = 0; t < nthreads; ++t) {
         accs[t] = (int64_t *)xmalloc((size_t)n * sizeof(int64_t));
-        stamps[t] = (int32_t *)xcalloc((size_t)n, sizeof(int32_t));
+        stamps[t] = (int64_t *)xcalloc((size_t)n, sizeof(int64_t));
         gbufs[t] = (orc_group *)xmalloc((size_t)(maxrow + 1) * sizeof(orc_group));
         if (!accs[t] || !stamps[t] || !gbufs[t]) rc = ORC_ERR_NOMEM;
     }
@@ -356,7 +362,7 @@ static int one_level(orc_run *R, orc_level *L)
             int32_t bc;
             int64_t bw;
             int32_t ng;
-            row_best(L, i, C, deg, gamma, accs[t], stamps[t], gbufs[t], &bc, &bw, &ng);
+            row_best(L, i, C, deg, gamma, accs[t], stamps[t], ++tokens[t], gbufs[t], &bc, &bw, &ng);
             best_c[i] = bc;
             best_w[i] = bw;
             ngr[i] = ng;
@@ -456,7 +462,7 @@ out_scratch:
         if (stamps) free(stamps[t]);
         if (gbufs) free(gbufs[t]);
     }
-    free(accs); free(stamps); free(gbufs);
+    free(accs); free(stamps); free(gbufs); free(tokens);
 out:
     free(C); free(next); free(deg); free(in); free(size);
     free(ddeg); free(din); free(dsize);

@@ -1,0 +1,55 @@
+"""The C++ host layer (Graph / generate_dendogram / partitionAtLevel /
+getPartitions / countPartitions) and the start() dict API, on the GPU,
+against the oracle."""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import gpu_case
+from louvaingpu_amd import count_partitions, generate_dendogram, generators, start
+
+pytestmark = pytest.mark.gpu
+
+
+def test_generate_dendogram_matches_oracle():
+    gpu_case("host generate_dendogram sbm_10k")
+    n, src, dst, w = generators.sbm_10k()
+    r = oracle.run(n, src, dst, w)
+    d = generate_dendogram(n, src, dst, w.astype(np.float32))
+    try:
+        assert len(d) == r.num_levels
+        for l in range(len(d)):
+            node, comm = d.levels[l]
+            onode, ocomm = r.partition(l)
+            assert np.array_equal(node, onode) and np.array_equal(comm, ocomm)
+        for l in range(len(d)):
+            node, comm = d.partition_at_level(l)
+            onode, ocomm = r.flatten(l)
+            assert np.array_equal(node, onode) and np.array_equal(comm, ocomm)
+        node, comm = d.get_partitions()        # reference: level `0 || len-1` -> 1
+        onode, ocomm = r.flatten(1)
+        assert np.array_equal(comm, ocomm)
+        node, comm = d.final_partition()
+        onode, ocomm = r.flatten(r.num_levels - 1)
+        assert np.array_equal(comm, ocomm)
+    finally:
+        d.close()
+
+
+def test_start_dict_api():
+    gpu_case("start() dict api")
+    rng = np.random.Generator(np.random.PCG64(3))
+    n = 300
+    ids = (np.arange(n) * 3 + 17).astype(np.int32)
+    n_, s, t, w = generators.sbm(n, 30, 0.3, 0.01, 9)
+    nodes = [{"id": int(i)} for i in ids]
+    edges = [{"source": int(ids[a]), "target": int(ids[b]), "weight": float(x)}
+             for a, b, x in zip(s, t, w)]
+    res = start(nodes, edges, [])
+    r = oracle.run(n, ids[s], ids[t], w.astype(np.float32), ids=ids)
+    onode, ocomm = r.flatten(1)
+    assert res["size"] == count_partitions(ocomm)
+    for a, c in zip(onode.tolist(), ocomm.tolist()):
+        assert res[a] == c
+    for nd in nodes:
+        assert nd["cluster"] == res[nd["id"]]

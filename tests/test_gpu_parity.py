@@ -1,0 +1,254 @@
+"""Parity of the HIP engine (through its C ABI) with the oracle.
+
+Integer/index results are compared bit for bit: every pass's next[] of every
+level, the Q each break test compares (FP64, bit-equal: both sides evaluate
+the same expression on the same exact integers), the level's returned Q,
+renumbered assignments, Partition arrays, coarsened CSR rows and the final
+flattened partition.  The final Newman modularity, which the engine forms
+from exact int64 aggregates and the oracle sums naively in FP64, must agree
+within 1e-9 relative.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import gpu_case
+from louvaingpu_amd import Engine, generate_dendogram_phases, generators
+from louvaingpu_amd._lib import LouvainError, LV_ERR_INVALID_INPUT, LV_ERR_RANGE
+from test_oracle_pins import _random_graph
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+Q_REL = 1e-9
+
+
+def merged_rows(n, rp, col, w):
+    """(row, col, sum w) over non-self entries, sorted."""
+    rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(rp))
+    col = col.astype(np.int64)
+    keep = rows != col
+    key = rows[keep] * n + col[keep]
+    ww = w[keep]
+    order = np.argsort(key, kind="stable")
+    key, ww = key[order], ww[order]
+    if len(key) == 0:
+        return key, ww
+    heads = np.flatnonzero(np.r_[True, key[1:] != key[:-1]])
+    return key[heads], np.add.reduceat(ww, heads)
+
+
+def assert_run_matches(eng, r, n, trace=True):
+    L = eng.num_levels()
+    assert L == r.num_levels
+    for l in range(L):
+        info = eng.level_info(l)
+        assert info.n == r.level_n(l), l
+        assert info.k == r.level_k(l), l
+        assert info.m_units == r.level_m(l), l
+        assert info.passes == r.level_passes(l), l
+        for p in range(1, info.passes + 1):
+            assert eng.trace_pass_q(l, p) == r.pass_q(l, p), (l, p)
+            if trace:
+                got = eng.trace_next(l, p)
+                exp = r.pass_next(l, p)
+                if not np.array_equal(got, exp):
+                    bad = np.flatnonzero(got != exp)
+                    raise AssertionError("level %d pass %d: %d of %d differ, first %s: %s vs %s"
+                                         % (l, p, len(bad), len(exp), bad[:5], got[bad[:5]],
+                                            exp[bad[:5]]))
+        assert info.q == r.level_q(l), l
+        assert info.lower == r.level_lower(l), l
+        assert np.array_equal(eng.level_assignment(l), r.newc(l)), l
+        node, comm = eng.level_partition(l)
+        onode, ocomm = r.partition(l)
+        assert np.array_equal(node, onode), l
+        assert np.array_equal(comm, ocomm), l
+    assert np.array_equal(eng.final_assignment(), r.final_comm())
+    qe = eng.final_modularity()
+    qo = r.final_modularity()
+    assert math.isclose(qe, qo, rel_tol=Q_REL, abs_tol=Q_REL), (qe, qo)
+
+
+def run_both(n, src, dst, w, ids=None, gamma=1.0, threads=8):
+    r = oracle.run(n, src, dst, w, ids=ids, gamma=gamma, trace=True, threads=threads)
+    eng = Engine(gamma=gamma, trace=True)
+    eng.load_edges(n, src, dst, w, ids=ids)
+    eng.run()
+    return eng, r
+
+
+def test_appendix_b_known_answer_on_gpu():
+    gpu_case("appendix_b")
+    with open(os.path.join(GOLDEN, "appendix_b.json")) as f:
+        g = json.load(f)
+    w = np.array(g["w"], np.int32)
+    eng, r = run_both(g["n"], g["src"], g["dst"], w)
+    with eng:
+        assert_run_matches(eng, r, g["n"])
+        for l, exp in enumerate(g["levels"]):
+            for p, ep in enumerate(exp["passes"], start=1):
+                assert eng.trace_next(l, p).tolist() == ep["next"]
+            node, comm = eng.level_partition(l)
+            assert node.tolist() == exp["partition_node"]
+            assert comm.tolist() == exp["partition_comm"]
+        node, comm = eng.flatten(1)
+        assert comm.tolist() == g["flatten_level1"]
+        assert math.isclose(eng.final_modularity(), 5 / 14, rel_tol=1e-12)
+
+
+@pytest.mark.parametrize("seed", range(0, 300, 3))
+def test_random_small_graphs(seed):
+    n, src, dst, w, ids, gamma = _random_graph(seed)
+    gpu_case("small seed=%d n=%d e=%d gamma=%s ids=%s" % (seed, n, len(src), gamma, ids is not None))
+    esrc = ids[src] if ids is not None else src
+    edst = ids[dst] if ids is not None else dst
+    r = oracle.run(n, esrc, edst, w, ids=ids, gamma=gamma, trace=True)
+    with Engine(gamma=gamma, trace=True) as eng:
+        eng.load_edges(n, esrc, edst, w, ids=ids)
+        eng.run()
+        if len(src) == 0:
+            assert eng.num_levels() == 1
+            assert eng.final_assignment().tolist() == list(range(n))
+            return
+        assert_run_matches(eng, r, n)
+
+
+def test_sbm_10k_every_pass_and_csr():
+    gpu_case("sbm_10k phases")
+    n, src, dst, w = generators.sbm_10k()
+    r = oracle.run(n, src, dst, w, trace=True)
+    with Engine(trace=True) as eng:
+        eng.load_edges(n, src, dst, w)
+        # phase by phase, checking the CSR the engine builds at each level
+        eng.CalculateNeighbour()
+        level = 0
+        mod = None
+        while True:
+            _, ln, nnz = eng.current_level()
+            rp, col, cw = eng.level_csr()
+            orp, ocol, ocw = r.csr(level)
+            gk, gw = merged_rows(ln, rp, col, cw)
+            ok, ow = merged_rows(ln, orp, ocol, ocw)
+            assert np.array_equal(gk, ok) and np.array_equal(gw, ow), level
+            eng.initCUDA()
+            k, loop, selfsum = eng.level_vertex()
+            ok_, oloop = r.vertex(level)
+            assert np.array_equal(k, ok_) and np.array_equal(loop, oloop), level
+            if mod is None:
+                mod = eng.modularityCUDA()
+                assert mod == r.level_q_init(0)
+            q, passes = eng.oneLevelCUDA(mod)
+            assert q == r.level_q(level)
+            eng.renumberCUDA()
+            if level > 0 and q - mod < 1e-7:
+                break
+            mod = q
+            eng.inducedGraphCUDA()
+            eng.CalculateNeighbour()
+            level += 1
+        assert_run_matches(eng, r, n)
+
+
+@pytest.mark.parametrize("scale,weights,seed", [(12, "int", 1), (14, "float", 2), (16, "int", 3)])
+def test_rmat_with_hubs(scale, weights, seed):
+    gpu_case("rmat scale=%d %s" % (scale, weights))
+    n, src, dst, w = generators.rmat(scale, 16, 0.57, 0.19, 0.19, seed, weights)
+    eng, r = run_both(n, src, dst, w)
+    with eng:
+        assert_run_matches(eng, r, n)
+
+
+def test_resolution_sweep_on_sbm():
+    n, src, dst, w = generators.sbm(3000, 100, 0.08, 0.0016, 11)
+    for gamma in generators.resolutions(16)[::3]:
+        gpu_case("sbm3000 gamma=%g" % gamma)
+        eng, r = run_both(n, src, dst, w, gamma=gamma)
+        with eng:
+            assert_run_matches(eng, r, n)
+
+
+def test_arbitrary_ids_float_weights_and_loops():
+    gpu_case("ids+float+loops")
+    rng = np.random.Generator(np.random.PCG64(99))
+    n = 500
+    ids = (rng.permutation(n) * 7 - 1000).astype(np.int32)
+    e = 3000
+    s = rng.integers(0, n, e)
+    t = rng.integers(0, n, e)
+    t[:50] = s[:50]                                  # self-loops, some repeated
+    w = (rng.integers(0, 40, e) * 0.125).astype(np.float32)  # zeros included
+    eng, r = run_both(n, ids[s], ids[t], w, ids=ids)
+    with eng:
+        assert_run_matches(eng, r, n)
+        node, comm = eng.flatten(eng.num_levels() - 1)
+        onode, ocomm = r.flatten(r.num_levels - 1)
+        assert np.array_equal(node, onode) and np.array_equal(comm, ocomm)
+
+
+def test_repeat_runs_are_identical():
+    gpu_case("repeat runs")
+    n, src, dst, w = generators.rmat(14, 16, 0.57, 0.19, 0.19, 7, "int")
+    with Engine() as eng:
+        eng.load_edges(n, src, dst, w)
+        eng.run()
+        a = eng.final_assignment()
+        qa = eng.final_modularity()
+        for _ in range(2):
+            eng.run()
+            assert np.array_equal(eng.final_assignment(), a)
+            assert eng.final_modularity() == qa
+
+
+def test_invalid_input_is_reported():
+    gpu_case("invalid inputs")
+    with Engine() as eng:
+        with pytest.raises(LouvainError) as ex:
+            eng.load_edges(3, [0, 1], [1, 3], np.ones(2, np.int32))
+        assert ex.value.status == LV_ERR_INVALID_INPUT
+        with pytest.raises(LouvainError) as ex:
+            eng.load_edges(3, [0, 1], [1, 2], np.array([1, -2], np.int32))
+        assert ex.value.status == LV_ERR_INVALID_INPUT
+        with pytest.raises(LouvainError) as ex:
+            eng.load_edges(3, [0, 1], [1, 2], np.array([1, np.inf], np.float32))
+        assert ex.value.status == LV_ERR_INVALID_INPUT
+        with pytest.raises(LouvainError) as ex:
+            eng.load_edges(3, [5, 6], [6, 7], np.ones(2, np.int32), ids=[5, 6, 6])
+        assert ex.value.status == LV_ERR_INVALID_INPUT
+        with pytest.raises(LouvainError) as ex:
+            eng.load_edges(3, [0, 1], [1, 2], np.array([1e30, 1e-30], np.float32))
+        assert ex.value.status == LV_ERR_RANGE
+        # a valid load afterwards works, and an edgeless graph is the identity
+        eng.load_edges(4, np.zeros(0, np.int32), np.zeros(0, np.int32), np.zeros(0, np.int32))
+        assert eng.run() == 1
+        assert eng.final_assignment().tolist() == [0, 1, 2, 3]
+
+
+def test_phase_api_matches_run():
+    gpu_case("phase api vs run")
+    n, src, dst, w = generators.sbm(2000, 100, 0.1, 0.002, 5)
+    r = oracle.run(n, src, dst, w)
+    with Engine() as eng:
+        eng.load_edges(n, src, dst, w)
+        dendro = generate_dendogram_phases(eng)
+        assert len(dendro) == r.num_levels
+        for l, (node, comm) in enumerate(dendro):
+            onode, ocomm = r.partition(l)
+            assert np.array_equal(node, onode) and np.array_equal(comm, ocomm)
+
+
+@pytest.mark.timeout(600)
+def test_rmat22_full_size():
+    """The R-MAT scale-22 benchmark graph end to end: final partition identical,
+    every level's pass Qs bit-equal, modularity within 1e-9."""
+    gpu_case("rmat22 full")
+    n, src, dst, w = generators.rmat22()
+    r = oracle.run(n, src, dst, w, trace=False, threads=16)
+    with Engine() as eng:
+        eng.load_edges(n, src, dst, w)
+        eng.run()
+        assert_run_matches(eng, r, n, trace=False)
