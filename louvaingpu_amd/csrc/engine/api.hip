@@ -4,6 +4,7 @@
 #include <chrono>
 #include <climits>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -69,10 +70,13 @@ struct lv_ctx {
     ull *ddeg = nullptr, *din = nullptr;
     int32_t *verts = nullptr;
     int64_t tier_cnt[kTiers + 1] = {0}, tier_nnz[kTiers + 1] = {0}, tier_off[kTiers + 1] = {0};
-    int64_t *hub_off = nullptr;
-    int32_t *hub_log2 = nullptr;
-    int32_t *hkeys = nullptr;
-    ull *hvals = nullptr;
+    // hub tier plan (rows with q > 2048), see HubArgs
+    HubArgs hub{};
+    int32_t n_hubs = 0;
+    int64_t hub_parts = 0, hub_chunks = 0, hub_entries = 0;
+    // expected entries per hub partition; the LDS table of a partition holds 4096
+    // distinct communities (more fall back to a global table)
+    int32_t hub_part_target = kHubPartTarget;
     // pass results of the level being processed
     double cur_q = 0, cur_lower = 0;
     int32_t cur_passes = 0;
@@ -203,10 +207,29 @@ void free_level_arrays(lv_ctx *c)
     arena_free(a, c->ddeg);
     arena_free(a, c->din);
     arena_free(a, c->verts);
-    arena_free(a, c->hub_off);
-    arena_free(a, c->hub_log2);
-    arena_free(a, c->hkeys);
-    arena_free(a, c->hvals);
+    HubArgs &h = c->hub;
+    a.release((void *)h.hq);
+    a.release((void *)h.hP);
+    a.release((void *)h.ebase);
+    a.release((void *)h.pbase);
+    a.release((void *)h.part_hub);
+    a.release((void *)h.chunk_hub);
+    a.release((void *)h.chunk_k);
+    a.release((void *)h.tmp_c);
+    a.release((void *)h.buf_c);
+    a.release((void *)h.buf_w);
+    a.release((void *)h.pcount);
+    a.release((void *)h.poff);
+    a.release((void *)h.pcur);
+    a.release((void *)h.fbk);
+    a.release((void *)h.fbv);
+    a.release((void *)h.pg);
+    a.release((void *)h.pc);
+    a.release((void *)h.pw);
+    a.release((void *)h.pn);
+    c->hub = HubArgs{};
+    c->n_hubs = 0;
+    c->hub_parts = c->hub_chunks = c->hub_entries = 0;
 }
 
 void free_dendrogram(lv_ctx *c)
@@ -262,35 +285,112 @@ int build_level0_csr(lv_ctx *c)
     hipStream_t s = c->stream;
     const int32_t n = c->n0;
     const int64_t e = c->e0;
+    const int64_t total = 2 * e;
     hipEvent_t t0 = t_begin(c);
-    int64_t *cnt = nullptr;
     ull *lastloop = nullptr;
+    uint32_t *k1 = nullptr, *k2 = nullptr, *v1 = nullptr, *v2 = nullptr;
     LV_TRY(arena_alloc(A, c->rowptr, (int64_t)n + 1));
-    LV_TRY(arena_alloc(A, cnt, (int64_t)n + 1));
     LV_TRY(arena_alloc(A, c->kdeg, n));
     LV_TRY(arena_alloc(A, c->selfsum, n));
     LV_TRY(arena_alloc(A, c->loopw, n));
     LV_TRY(arena_alloc(A, lastloop, n));
-    fill_i64(s, cnt, (int64_t)n + 1, 0);
+    LV_TRY(arena_alloc(A, k1, total));
+    LV_TRY(arena_alloc(A, v1, total));
+    LV_TRY(arena_alloc(A, k2, total));
+    LV_TRY(arena_alloc(A, v2, total));
     fill_i64(s, c->kdeg, n, 0);
     fill_i64(s, c->selfsum, n, 0);
     fill_i64(s, (int64_t *)lastloop, n, 0);
-    launch_csr_count(s, c->d_src, c->d_dst, c->d_w, e, cnt, c->kdeg, c->selfsum, lastloop);
-    LV_TRY(scan_exclusive_i64(A, s, cnt, c->rowptr, n, c->rowptr + n));
+    launch_csr_entries(s, c->d_src, c->d_dst, e, n, k1, v1);
+    launch_self_loops(s, c->d_src, c->d_dst, c->d_w, e, c->kdeg, c->selfsum, lastloop);
+    uint32_t *sk = nullptr, *sv = nullptr;
+    LV_TRY(radix_sort_u32_u32(A, s, k1, v1, k2, v2, total, bits_for((uint64_t)n), &sk, &sv));
+    launch_rowptr_search(s, sk, total, n, c->rowptr);
     LV_TRY(d2h(c, &c->h_small[0], (const long long *)(c->rowptr + n), 1));
     LV_TRY(sync(c));
     c->nnz = c->h_small[0];
     LV_TRY(arena_alloc(A, c->col, c->nnz));
     LV_TRY(arena_alloc(A, c->cw, c->nnz));
-    fill_i64(s, cnt, n, 0);
-    launch_csr_fill(s, c->d_src, c->d_dst, c->d_w, e, c->rowptr, cnt, c->col, c->cw);
+    launch_csr_gather(s, sk, sv, c->nnz, c->d_src, c->d_dst, c->d_w, c->col, c->cw, c->kdeg);
     launch_loop_gather(s, lastloop, c->d_w, n, c->loopw);
     LV_HIP(hipGetLastError());
     t_end(c, t0, CAT_CSR);
-    arena_free(A, cnt);
     arena_free(A, lastloop);
+    arena_free(A, k1);
+    arena_free(A, k2);
+    arena_free(A, v1);
+    arena_free(A, v2);
     c->n = n;
     c->m = c->m0;
+    return LV_OK;
+}
+
+template <typename T>
+int alloc_into(Arena &A, const T *&p, int64_t count)
+{
+    T *q = nullptr;
+    LV_TRY(arena_alloc(A, q, count));
+    p = q;
+    return LV_OK;
+}
+
+int plan_hubs(lv_ctx *c, int32_t nh)
+{
+    Arena &A = c->arena;
+    hipStream_t s = c->stream;
+    HubArgs &h = c->hub;
+    const int32_t *hv = c->verts + c->tier_off[kTiers];
+    int64_t *hq = nullptr, *hP64 = nullptr, *hch = nullptr, *ebase = nullptr, *pbase = nullptr,
+            *cbase = nullptr;
+    int32_t *hP = nullptr;
+    LV_TRY(arena_alloc(A, hq, nh));
+    LV_TRY(arena_alloc(A, hP, nh));
+    LV_TRY(arena_alloc(A, hP64, nh));
+    LV_TRY(arena_alloc(A, hch, nh));
+    LV_TRY(arena_alloc(A, ebase, (int64_t)nh + 1));
+    LV_TRY(arena_alloc(A, pbase, (int64_t)nh + 1));
+    LV_TRY(arena_alloc(A, cbase, (int64_t)nh + 1));
+    launch_hub_setup(s, hv, nh, c->rowptr, c->selfsum, c->hub_part_target, hq, hP, hP64, hch);
+    LV_TRY(scan_exclusive_i64(A, s, hq, ebase, nh, ebase + nh));
+    LV_TRY(scan_exclusive_i64(A, s, hP64, pbase, nh, pbase + nh));
+    LV_TRY(scan_exclusive_i64(A, s, hch, cbase, nh, cbase + nh));
+    LV_TRY(d2h(c, c->h_small, (const long long *)(ebase + nh), 1));
+    LV_TRY(d2h(c, c->h_small + 1, (const long long *)(pbase + nh), 1));
+    LV_TRY(d2h(c, c->h_small + 2, (const long long *)(cbase + nh), 1));
+    LV_TRY(sync(c));
+    c->hub_entries = c->h_small[0];
+    c->hub_parts = c->h_small[1];
+    c->hub_chunks = c->h_small[2];
+    c->n_hubs = nh;
+    h.hv = hv;
+    h.hq = hq;
+    h.hP = hP;
+    h.ebase = ebase;
+    h.pbase = pbase;
+    int32_t *part_hub = nullptr, *chunk_hub = nullptr, *chunk_k = nullptr;
+    LV_TRY(arena_alloc(A, part_hub, c->hub_parts));
+    LV_TRY(arena_alloc(A, chunk_hub, c->hub_chunks));
+    LV_TRY(arena_alloc(A, chunk_k, c->hub_chunks));
+    launch_hub_maps(s, nh, hP, pbase, hch, cbase, part_hub, chunk_hub, chunk_k);
+    h.part_hub = part_hub;
+    h.chunk_hub = chunk_hub;
+    h.chunk_k = chunk_k;
+    arena_free(A, hP64);
+    arena_free(A, hch);
+    arena_free(A, cbase);
+    const int64_t E = c->hub_entries, P = c->hub_parts;
+    LV_TRY(arena_alloc(A, h.tmp_c, E));
+    LV_TRY(arena_alloc(A, h.buf_c, E));
+    LV_TRY(arena_alloc(A, h.buf_w, E));
+    LV_TRY(arena_alloc(A, h.fbk, 2 * E));
+    LV_TRY(arena_alloc(A, h.fbv, 2 * E));
+    LV_TRY(arena_alloc(A, h.pcount, P + 1));
+    LV_TRY(alloc_into(A, h.poff, P + 1));
+    LV_TRY(arena_alloc(A, h.pcur, P));
+    LV_TRY(arena_alloc(A, h.pg, P));
+    LV_TRY(arena_alloc(A, h.pc, P));
+    LV_TRY(arena_alloc(A, h.pw, P));
+    LV_TRY(arena_alloc(A, h.pn, P));
     return LV_OK;
 }
 
@@ -340,23 +440,9 @@ int init_level(lv_ctx *c)
         c->tier_off[t] = off;
         off += c->tier_cnt[t];
     }
-    // hub tier: one private hash table per vertex, capacity next_pow2(2q)
+    // hub tier: chunks and partitions per hub (see HubArgs)
     const int64_t nh = c->tier_cnt[kTiers];
-    if (nh > 0) {
-        int64_t *cap = nullptr;
-        LV_TRY(arena_alloc(A, cap, nh));
-        LV_TRY(arena_alloc(A, c->hub_off, nh));
-        LV_TRY(arena_alloc(A, c->hub_log2, nh));
-        launch_hub_caps(s, c->verts + c->tier_off[kTiers], (int32_t)nh, c->rowptr, c->selfsum, cap,
-                        c->hub_log2);
-        LV_TRY(scan_exclusive_i64(A, s, cap, c->hub_off, nh, (int64_t *)c->d_small));
-        LV_TRY(d2h(c, c->h_small, c->d_small, 1));
-        LV_TRY(sync(c));
-        arena_free(A, cap);
-        int64_t slots = c->h_small[0];
-        LV_TRY(arena_alloc(A, c->hkeys, slots));
-        LV_TRY(arena_alloc(A, c->hvals, slots));
-    }
+    if (nh > 0) LV_TRY(plan_hubs(c, (int32_t)nh));
     LV_HIP(hipGetLastError());
     t_end(c, t0, CAT_INIT);
     return LV_OK;
@@ -385,10 +471,6 @@ int one_level(lv_ctx *c, double lower, double *q_out, int32_t *passes_out)
     a.dsize = c->dsize;
     a.m2 = 2.0 * (double)c->m;
     a.gamma = c->cfg.gamma;
-    a.hkeys = c->hkeys;
-    a.hvals = c->hvals;
-    a.hoff = c->hub_off;
-    a.hlog2 = c->hub_log2;
     double curQ = -1.0;  // LouvainCUDA/OneLevelKernel.cu:528
     int32_t pass = 0;
     double q_level = 0.0;
@@ -405,7 +487,11 @@ int one_level(lv_ctx *c, double lower, double *q_out, int32_t *passes_out)
             const int64_t nv = c->tier_cnt[t];
             if (nv == 0) continue;
             hipEvent_t t0 = t_begin(c);
-            launch_sweep_tier(s, t, a, c->verts + c->tier_off[t], (int32_t)nv);
+            if (t < kTiers)
+                launch_sweep_tier(s, t, a, c->verts + c->tier_off[t], (int32_t)nv);
+            else
+                LV_TRY(run_hub_sweep(c->arena, s, a, c->hub, c->n_hubs, c->hub_parts,
+                                     c->hub_chunks));
             t_end(c, t0, CAT_SWEEP, t, 12.0 * (double)c->tier_nnz[t] + 24.0 * (double)nv,
                   c->tier_nnz[t]);
         }
@@ -673,6 +759,10 @@ int lv_create(lv_ctx **out, const lv_config *cfg)
         return LV_ERR_ARG;
     }
     c->device = c->cfg.device;
+    if (const char *e = getenv("LV_HUB_PART_TARGET")) {
+        long t = strtol(e, nullptr, 10);
+        if (t >= 1 && t <= (1 << 30)) c->hub_part_target = (int32_t)t;
+    }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= c->device || c->device < 0) {
         (void)hipGetLastError();

@@ -27,7 +27,7 @@ typedef unsigned long long ull;
 constexpr int64_t kUnitLimit = int64_t(1) << 60;
 
 // Sweep tiers by q = row length + (1 if the vertex has a self-loop).
-constexpr int kTiers = LV_SWEEP_TIERS;  // tiers 1..7 use LDS tables, tier 8 global
+constexpr int kTiers = LV_SWEEP_TIERS;  // tiers 1..7: LDS table per row; 8: hub partitions
 constexpr int kTierMaxQ[kTiers] = {4, 8, 16, 32, 64, 512, 2048, 0x7fffffff};
 
 #define LV_TRY(expr)                                \
@@ -109,11 +109,38 @@ struct SweepArgs {
     double m2;     // 2.0 * (double)m
     double gamma;
     int last;      // last pass: fuse merge-isolated, drop move deltas
-    // global tables of the hub tier
-    int32_t *hkeys;
-    ull *hvals;
-    const int64_t *hoff;
-    const int32_t *hlog2;
+};
+
+// Hub tier plan (rows with q > 2048): each hub's entries (plus a virtual
+// own-community entry when it has a self-loop) are cut into chunks of
+// kHubChunk; every pass buckets them by a hash of their community into
+// P_j = ceil(q_j / kHubPartTarget) partitions, aggregates each partition in an
+// LDS table, then reduces the partitions of a hub.
+constexpr int kHubChunk = 4096;
+constexpr int kHubPartTarget = 1024;  // default; LV_HUB_PART_TARGET overrides
+constexpr int kHubMaxParts = 2048;
+
+struct HubArgs {
+    const int32_t *hv;        // hub vertex of hub j
+    const int64_t *hq;        // q_j
+    const int32_t *hP;        // partitions of hub j
+    const int64_t *ebase;     // first entry slot of hub j (exclusive scan of q)
+    const int64_t *pbase;     // first partition of hub j (exclusive scan of P)
+    const int32_t *part_hub;  // hub of each partition
+    const int32_t *chunk_hub; // hub of each chunk
+    const int32_t *chunk_k;   // chunk number within its hub
+    int32_t *tmp_c;           // community of each hub entry (entry order)
+    int32_t *buf_c;           // partitioned communities
+    int64_t *buf_w;           // partitioned weights
+    int64_t *pcount;          // entries per partition
+    const int64_t *poff;      // partition offsets (exclusive scan of pcount)
+    int64_t *pcur;            // scatter cursors
+    int32_t *fbk;             // fallback global tables, 2 slots per entry
+    ull *fbv;
+    double *pg;               // per-partition best gain, community, weight, groups
+    int32_t *pc;
+    int64_t *pw;
+    int32_t *pn;
 };
 
 // ---------------------------------------------------------------------------
@@ -130,6 +157,9 @@ int radix_sort_u64_i64(Arena &a, hipStream_t s, uint64_t *keys, int64_t *vals, u
 int radix_sort_u32_i32(Arena &a, hipStream_t s, uint32_t *keys, int32_t *vals, uint32_t *ktmp,
                        int32_t *vtmp, int64_t n, int bits, uint32_t **keys_out,
                        int32_t **vals_out);
+int radix_sort_u32_u32(Arena &a, hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *ktmp,
+                       uint32_t *vtmp, int64_t n, int bits, uint32_t **keys_out,
+                       uint32_t **vals_out);
 void fill_i32(hipStream_t s, int32_t *p, int64_t n, int32_t v);
 void fill_i64(hipStream_t s, int64_t *p, int64_t n, int64_t v);
 void iota_i32(hipStream_t s, int32_t *p, int64_t n);
@@ -158,12 +188,15 @@ void launch_map_endpoints(hipStream_t s, const uint32_t *sorted_ids, const int32
                           int32_t n, int32_t *ep, int64_t e, DevFlags *f);
 void launch_ids_to_keys(hipStream_t s, const int32_t *ids, uint32_t *keys, int32_t n);
 
-void launch_csr_count(hipStream_t s, const int32_t *src, const int32_t *dst, const int64_t *w,
-                      int64_t e, int64_t *cnt, int64_t *kdeg, int64_t *selfsum,
-                      ull *lastloop);
-void launch_csr_fill(hipStream_t s, const int32_t *src, const int32_t *dst, const int64_t *w,
-                     int64_t e, const int64_t *rowptr, int64_t *cursor, int32_t *col,
-                     int64_t *cw);
+void launch_csr_entries(hipStream_t s, const int32_t *src, const int32_t *dst, int64_t e,
+                        int32_t n, uint32_t *keys, uint32_t *codes);
+void launch_self_loops(hipStream_t s, const int32_t *src, const int32_t *dst, const int64_t *w,
+                       int64_t e, int64_t *kdeg, int64_t *selfsum, ull *lastloop);
+void launch_rowptr_search(hipStream_t s, const uint32_t *skeys, int64_t total, int32_t n,
+                          int64_t *rowptr);
+void launch_csr_gather(hipStream_t s, const uint32_t *skeys, const uint32_t *scodes, int64_t nnz,
+                       const int32_t *src, const int32_t *dst, const int64_t *w, int32_t *col,
+                       int64_t *cw, int64_t *kdeg);
 void launch_loop_gather(hipStream_t s, const ull *lastloop, const int64_t *w, int32_t n,
                         int64_t *loopw);
 
@@ -172,14 +205,20 @@ void launch_init_vertex(hipStream_t s, int32_t n, const int64_t *rowptr, const i
                         int32_t *size, ull *ddeg, ull *din, int32_t *dsize, int32_t *C,
                         int32_t *next, uint32_t *tier_key, int32_t *verts,
                         long long *tier_cnt, long long *tier_nnz);
-void launch_hub_caps(hipStream_t s, const int32_t *hub_verts, int32_t nh, const int64_t *rowptr,
-                     const int64_t *selfsum, int64_t *cap, int32_t *log2cap);
 void launch_qagg(hipStream_t s, int32_t n, const int64_t *deg, const int64_t *in, QAcc *acc);
 void launch_zero_qacc(hipStream_t s, QAcc *acc);
 void launch_apply(hipStream_t s, int32_t n, int64_t *deg, int64_t *in, int32_t *size, ull *ddeg,
                   ull *din, int32_t *dsize, QAcc *acc);
 void launch_sweep_tier(hipStream_t s, int tier, const SweepArgs &a, const int32_t *verts,
                        int32_t nv);
+void launch_hub_setup(hipStream_t s, const int32_t *hv, int32_t nh, const int64_t *rowptr,
+                      const int64_t *selfsum, int32_t target, int64_t *hq, int32_t *hP,
+                      int64_t *hP64, int64_t *hch);
+void launch_hub_maps(hipStream_t s, int32_t nh, const int32_t *hP, const int64_t *pbase,
+                     const int64_t *hch, const int64_t *cbase, int32_t *part_hub,
+                     int32_t *chunk_hub, int32_t *chunk_k);
+int run_hub_sweep(Arena &A, hipStream_t s, const SweepArgs &a, const HubArgs &h, int32_t nh,
+                  int64_t nparts, int64_t nchunks);
 
 void launch_leader(hipStream_t s, const int32_t *C, int32_t n, int32_t *leader);
 void launch_leader_flag(hipStream_t s, const int32_t *C, const int32_t *leader, int32_t n,

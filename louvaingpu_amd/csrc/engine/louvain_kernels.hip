@@ -260,45 +260,75 @@ void launch_ids_to_keys(hipStream_t s, const int32_t *ids, uint32_t *keys, int32
 // ---------------------------------------------------------------------------
 // CSR build (CalculateNeighbour, LouvainCUDA/NeighbourCompute.cu:42-118, plus the
 // degree and self-loop part of initValuesKernel, InitKernel.cu:37-77,101-126).
-// Non-self edges become two row entries; self-loops stay out of the rows:
-// selfsum[i] = sum of (w==0?1:w) over the vertex's loops, k_i gains 4x that,
-// loop_i = the weight of the vertex's last loop in input order.
+// Every non-self edge e gives two directed entries, coded 2e (row src, col dst)
+// and 2e+1 (row dst, col src); a stable radix sort by row groups them (no
+// atomics on the counters of high-degree rows).  Self-loops stay out of the
+// rows: selfsum[i] = sum of (w==0?1:w) over the vertex's loops, k_i gains 4x
+// that, loop_i = the weight of the vertex's last loop in input order.
 
-__global__ void k_csr_count(const int32_t *src, const int32_t *dst, const int64_t *w, int64_t e,
-                            int64_t *cnt, int64_t *kdeg, int64_t *selfsum, ull *lastloop)
+__global__ void k_csr_entries(const int32_t *src, const int32_t *dst, int64_t e, int32_t n,
+                              uint32_t *keys, uint32_t *codes)
 {
     GRID_STRIDE(i, e)
     {
         int32_t a = src[i], b = dst[i];
-        int64_t u = w[i];
-        ull wp = (ull)(u == 0 ? 1 : u);
-        if (a != b) {
-            atomicAdd((ull *)&cnt[a], 1ull);
-            atomicAdd((ull *)&cnt[b], 1ull);
-            atomicAdd((ull *)&kdeg[a], wp);
-            atomicAdd((ull *)&kdeg[b], wp);
-        } else {
-            atomicAdd((ull *)&selfsum[a], wp);
-            atomicAdd((ull *)&kdeg[a], 4ull * wp);
-            atomicMax(&lastloop[a], (ull)(i + 1));
-        }
+        bool self = (a == b);
+        keys[2 * i] = self ? (uint32_t)n : (uint32_t)a;  // self-loops sort past every row
+        keys[2 * i + 1] = self ? (uint32_t)n : (uint32_t)b;
+        codes[2 * i] = (uint32_t)(2 * i);
+        codes[2 * i + 1] = (uint32_t)(2 * i + 1);
     }
 }
 
-__global__ void k_csr_fill(const int32_t *src, const int32_t *dst, const int64_t *w, int64_t e,
-                           const int64_t *rowptr, int64_t *cursor, int32_t *col, int64_t *cw)
+__global__ void k_self_loops(const int32_t *src, const int32_t *dst, const int64_t *w, int64_t e,
+                             int64_t *kdeg, int64_t *selfsum, ull *lastloop)
 {
     GRID_STRIDE(i, e)
     {
-        int32_t a = src[i], b = dst[i];
-        if (a == b) continue;
+        int32_t a = src[i];
+        if (a != dst[i]) continue;
         int64_t u = w[i];
-        int64_t p = rowptr[a] + (int64_t)atomicAdd((ull *)&cursor[a], 1ull);
-        col[p] = b;
-        cw[p] = u;
-        int64_t q = rowptr[b] + (int64_t)atomicAdd((ull *)&cursor[b], 1ull);
-        col[q] = a;
-        cw[q] = u;
+        ull wp = (ull)(u == 0 ? 1 : u);
+        atomicAdd((ull *)&selfsum[a], wp);
+        atomicAdd((ull *)&kdeg[a], 4ull * wp);
+        atomicMax(&lastloop[a], (ull)(i + 1));
+    }
+}
+
+// rowptr[v] = first position of a key >= v in the sorted keys (v in [0, n]).
+__global__ void k_rowptr_search(const uint32_t *skeys, int64_t total, int32_t n, int64_t *rowptr)
+{
+    GRID_STRIDE(v, (int64_t)n + 1)
+    {
+        int64_t lo = 0, hi = total;
+        while (lo < hi) {
+            int64_t mid = (lo + hi) >> 1;
+            if (skeys[mid] < (uint32_t)v) lo = mid + 1; else hi = mid;
+        }
+        rowptr[v] = lo;
+    }
+}
+
+__global__ void k_csr_gather(const uint32_t *skeys, const uint32_t *scodes, int64_t nnz,
+                             const int32_t *src, const int32_t *dst, const int64_t *w, int32_t *col,
+                             int64_t *cw, int64_t *kdeg)
+{
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < nnz;
+         base += (int64_t)gridDim.x * blockDim.x) {
+        int64_t p = base + threadIdx.x;
+        bool valid = p < nnz;
+        long long row = 0;
+        ull wp = 0;
+        if (valid) {
+            uint32_t code = scodes[p];
+            int64_t ei = (int64_t)(code >> 1);
+            col[p] = (code & 1u) ? src[ei] : dst[ei];
+            int64_t u = w[ei];
+            cw[p] = u;
+            row = (long long)skeys[p];
+            wp = (ull)(u == 0 ? 1 : u);
+        }
+        seg_atomic_add(valid, row, wp, (ull *)kdeg);
     }
 }
 
@@ -311,20 +341,36 @@ __global__ void k_loop_gather(const ull *lastloop, const int64_t *w, int32_t n, 
     }
 }
 
-void launch_csr_count(hipStream_t s, const int32_t *src, const int32_t *dst, const int64_t *w,
-                      int64_t e, int64_t *cnt, int64_t *kdeg, int64_t *selfsum, ull *lastloop)
+void launch_csr_entries(hipStream_t s, const int32_t *src, const int32_t *dst, int64_t e,
+                        int32_t n, uint32_t *keys, uint32_t *codes)
 {
     if (e > 0)
-        hipLaunchKernelGGL(k_csr_count, grid_for(e, 256), 256, 0, s, src, dst, w, e, cnt, kdeg,
+        hipLaunchKernelGGL(k_csr_entries, grid_for(e, 256, 1 << 20), 256, 0, s, src, dst, e, n,
+                           keys, codes);
+}
+
+void launch_self_loops(hipStream_t s, const int32_t *src, const int32_t *dst, const int64_t *w,
+                       int64_t e, int64_t *kdeg, int64_t *selfsum, ull *lastloop)
+{
+    if (e > 0)
+        hipLaunchKernelGGL(k_self_loops, grid_for(e, 256, 1 << 20), 256, 0, s, src, dst, w, e, kdeg,
                            selfsum, lastloop);
 }
 
-void launch_csr_fill(hipStream_t s, const int32_t *src, const int32_t *dst, const int64_t *w,
-                     int64_t e, const int64_t *rowptr, int64_t *cursor, int32_t *col, int64_t *cw)
+void launch_rowptr_search(hipStream_t s, const uint32_t *skeys, int64_t total, int32_t n,
+                          int64_t *rowptr)
 {
-    if (e > 0)
-        hipLaunchKernelGGL(k_csr_fill, grid_for(e, 256), 256, 0, s, src, dst, w, e, rowptr, cursor,
-                           col, cw);
+    hipLaunchKernelGGL(k_rowptr_search, grid_for((int64_t)n + 1, 256, 1 << 20), 256, 0, s, skeys,
+                       total, n, rowptr);
+}
+
+void launch_csr_gather(hipStream_t s, const uint32_t *skeys, const uint32_t *scodes, int64_t nnz,
+                       const int32_t *src, const int32_t *dst, const int64_t *w, int32_t *col,
+                       int64_t *cw, int64_t *kdeg)
+{
+    if (nnz > 0)
+        hipLaunchKernelGGL(k_csr_gather, grid_for(nnz, 256, 1 << 20), 256, 0, s, skeys, scodes,
+                           nnz, src, dst, w, col, cw, kdeg);
 }
 
 void launch_loop_gather(hipStream_t s, const ull *lastloop, const int64_t *w, int32_t n,
@@ -382,20 +428,6 @@ __global__ __launch_bounds__(256) void k_init_vertex(
     }
 }
 
-__global__ void k_hub_caps(const int32_t *hv, int32_t nh, const int64_t *rowptr,
-                           const int64_t *selfsum, int64_t *cap, int32_t *log2cap)
-{
-    GRID_STRIDE(j, nh)
-    {
-        int32_t v = hv[j];
-        int64_t q = rowptr[v + 1] - rowptr[v] + (selfsum[v] > 0 ? 1 : 0);
-        int l = 1;
-        while ((int64_t(1) << l) < 2 * q) ++l;
-        cap[j] = int64_t(1) << l;
-        log2cap[j] = l;
-    }
-}
-
 void launch_init_vertex(hipStream_t s, int32_t n, const int64_t *rowptr, const int64_t *kdeg,
                         const int64_t *loopw, const int64_t *selfsum, int64_t *deg, int64_t *in,
                         int32_t *size, ull *ddeg, ull *din, int32_t *dsize, int32_t *C,
@@ -406,14 +438,6 @@ void launch_init_vertex(hipStream_t s, int32_t n, const int64_t *rowptr, const i
         hipLaunchKernelGGL(k_init_vertex, grid_for(n, 256, 8192), 256, 0, s, n, rowptr, kdeg,
                            loopw, selfsum, deg, in, size, ddeg, din, dsize, C, next, tier_key,
                            verts, tier_cnt, tier_nnz);
-}
-
-void launch_hub_caps(hipStream_t s, const int32_t *hub_verts, int32_t nh, const int64_t *rowptr,
-                     const int64_t *selfsum, int64_t *cap, int32_t *log2cap)
-{
-    if (nh > 0)
-        hipLaunchKernelGGL(k_hub_caps, grid_for(nh, 256), 256, 0, s, hub_verts, nh, rowptr,
-                           selfsum, cap, log2cap);
 }
 
 // ---------------------------------------------------------------------------
@@ -544,23 +568,6 @@ __device__ __forceinline__ void lds_insert(int32_t *key, ull *val, int log2cap, 
     if (w != 0) atomicAdd(&val[s], (ull)w);
 }
 
-__device__ __forceinline__ void glb_insert(int32_t *key, ull *val, int log2cap, int32_t c,
-                                           int64_t w)
-{
-    const uint32_t mask = (1u << log2cap) - 1u;
-    uint32_t s = hslot(c, log2cap);
-    for (;;) {
-        int32_t k = __hip_atomic_load(&key[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (k == c) break;
-        if (k == -1) {
-            int32_t prev = atomicCAS(&key[s], -1, c);
-            if (prev == -1 || prev == c) break;
-        }
-        s = (s + 1) & mask;
-    }
-    if (w != 0) atomicAdd(&val[s], (ull)w);
-}
-
 __device__ __forceinline__ bool better(double g1, int32_t c1, double g2, int32_t c2)
 {
     return g1 > g2 || (g1 == g2 && c1 < c2);
@@ -635,9 +642,18 @@ __device__ __forceinline__ void finalize_vertex(const SweepArgs &a, int32_t v, i
     }
 }
 
-// G lanes per vertex, 256/G vertices per block, LDS table of 2^LOG2CAP slots
-// per vertex (capacity >= 2q).
-template <int G, int LOG2CAP>
+// Table size for a row of q inserts: the smallest power of two >= 2q, at least
+// 2^lo (never above the kernel's LDS capacity, which the tier bound ensures).
+__device__ __forceinline__ int table_log2(int64_t q, int lo)
+{
+    int l = lo;
+    while ((int64_t(1) << l) < 2 * q) ++l;
+    return l;
+}
+
+// G lanes per vertex, 256/G vertices per block, an LDS table of up to
+// 2^LOG2CAP slots per vertex (2^MINLOG2 .. 2^LOG2CAP sized to the row).
+template <int G, int LOG2CAP, int MINLOG2>
 __global__ __launch_bounds__(256) void k_sweep_group(SweepArgs a, const int32_t *__restrict__ verts,
                                                     int32_t nv)
 {
@@ -651,19 +667,25 @@ __global__ __launch_bounds__(256) void k_sweep_group(SweepArgs a, const int32_t 
     for (int64_t base = (int64_t)blockIdx.x * GPB; base < nv; base += (int64_t)gridDim.x * GPB) {
         const int64_t vi = base + grp;
         const bool active = vi < nv;
-#pragma unroll
-        for (int s = lg; s < CAP; s += G) {
+        int32_t v = 0, own = 0;
+        int64_t beg = 0, end = 0;
+        int l2 = MINLOG2;
+        if (active) {
+            v = verts[vi];
+            own = a.C[v];
+            beg = a.rowptr[v];
+            end = a.rowptr[v + 1];
+            if (MINLOG2 < LOG2CAP) l2 = table_log2(end - beg + (a.selfsum[v] > 0 ? 1 : 0), MINLOG2);
+        }
+        const int cap = 1 << l2;
+        for (int s = lg; s < cap; s += G) {
             key[s] = -1;
             val[s] = 0;
         }
         __syncthreads();
-        int32_t v = 0, own = 0;
         if (active) {
-            v = verts[vi];
-            own = a.C[v];
-            const int64_t beg = a.rowptr[v], end = a.rowptr[v + 1];
-            for (int64_t p = beg + lg; p < end; p += G) lds_insert(key, val, LOG2CAP, a.C[a.col[p]], a.w[p]);
-            if (lg == 0 && a.selfsum[v] > 0) lds_insert(key, val, LOG2CAP, own, 0);
+            for (int64_t p = beg + lg; p < end; p += G) lds_insert(key, val, l2, a.C[a.col[p]], a.w[p]);
+            if (lg == 0 && a.selfsum[v] > 0) lds_insert(key, val, l2, own, 0);
         }
         __syncthreads();
         int cnt = 0;
@@ -672,8 +694,7 @@ __global__ __launch_bounds__(256) void k_sweep_group(SweepArgs a, const int32_t 
         int64_t bw = 0;
         if (active) {
             const double f = (double)a.kdeg[v] / a.m2;
-#pragma unroll
-            for (int s = lg; s < CAP; s += G) {
+            for (int s = lg; s < cap; s += G) {
                 int32_t c = key[s];
                 if (c >= 0) consider(c, (int64_t)val[s], a, f, cnt, bg, bc, bw);
             }
@@ -684,16 +705,16 @@ __global__ __launch_bounds__(256) void k_sweep_group(SweepArgs a, const int32_t 
     }
 }
 
-// One vertex per block of BLOCK threads; LDS table of 2^LOG2CAP slots, or for
-// hubs (GLOBAL) a private table of 2^hlog2[j] slots at hoff[j] in HBM.
-template <int BLOCK, int LOG2CAP, bool GLOBAL>
+// One vertex per block of BLOCK threads with an LDS table of up to
+// 2^LOG2CAP slots sized to the row.
+template <int BLOCK, int LOG2CAP, int MINLOG2>
 __global__ __launch_bounds__(BLOCK) void k_sweep_block(SweepArgs a, const int32_t *__restrict__ verts,
                                                       int32_t nv)
 {
-    constexpr int LCAP = GLOBAL ? 1 : (1 << LOG2CAP);
+    constexpr int CAP = 1 << LOG2CAP;
     constexpr int NW = BLOCK / 64;
-    __shared__ int32_t s_key[LCAP];
-    __shared__ ull s_val[LCAP];
+    __shared__ int32_t s_key[CAP];
+    __shared__ ull s_val[CAP];
     __shared__ int r_cnt[NW];
     __shared__ double r_g[NW];
     __shared__ int32_t r_c[NW];
@@ -702,56 +723,25 @@ __global__ __launch_bounds__(BLOCK) void k_sweep_block(SweepArgs a, const int32_
     for (int64_t vi = blockIdx.x; vi < nv; vi += gridDim.x) {
         const int32_t v = verts[vi];
         const int32_t own = a.C[v];
-        int32_t *key;
-        ull *val;
-        int l2;
-        if (GLOBAL) {
-            key = a.hkeys + a.hoff[vi];
-            val = a.hvals + a.hoff[vi];
-            l2 = a.hlog2[vi];
-        } else {
-            key = s_key;
-            val = s_val;
-            l2 = LOG2CAP;
-        }
-        const int64_t cap = int64_t(1) << l2;
-        for (int64_t s = tid; s < cap; s += BLOCK) {
-            key[s] = -1;
-            val[s] = 0;
-        }
-        if (GLOBAL) __threadfence();
-        __syncthreads();
         const int64_t beg = a.rowptr[v], end = a.rowptr[v + 1];
-        for (int64_t p = beg + tid; p < end; p += BLOCK) {
-            if (GLOBAL)
-                glb_insert(key, val, l2, a.C[a.col[p]], a.w[p]);
-            else
-                lds_insert(key, val, l2, a.C[a.col[p]], a.w[p]);
+        const int l2 = table_log2(end - beg + (a.selfsum[v] > 0 ? 1 : 0), MINLOG2);
+        const int cap = 1 << l2;
+        for (int s = tid; s < cap; s += BLOCK) {
+            s_key[s] = -1;
+            s_val[s] = 0;
         }
-        if (tid == 0 && a.selfsum[v] > 0) {
-            if (GLOBAL)
-                glb_insert(key, val, l2, own, 0);
-            else
-                lds_insert(key, val, l2, own, 0);
-        }
-        if (GLOBAL) __threadfence();
+        __syncthreads();
+        for (int64_t p = beg + tid; p < end; p += BLOCK) lds_insert(s_key, s_val, l2, a.C[a.col[p]], a.w[p]);
+        if (tid == 0 && a.selfsum[v] > 0) lds_insert(s_key, s_val, l2, own, 0);
         __syncthreads();
         int cnt = 0;
         double bg = 0.0;
         int32_t bc = INT_MAX;
         int64_t bw = 0;
         const double f = (double)a.kdeg[v] / a.m2;
-        for (int64_t s = tid; s < cap; s += BLOCK) {
-            int32_t c;
-            int64_t W;
-            if (GLOBAL) {
-                c = __hip_atomic_load(&key[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                W = (int64_t)__hip_atomic_load(&val[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            } else {
-                c = key[s];
-                W = (int64_t)val[s];
-            }
-            if (c >= 0) consider(c, W, a, f, cnt, bg, bc, bw);
+        for (int s = tid; s < cap; s += BLOCK) {
+            int32_t c = s_key[s];
+            if (c >= 0) consider(c, (int64_t)s_val[s], a, f, cnt, bg, bc, bw);
         }
         group_reduce<64>(cnt, bg, bc, bw);
         if (lane == 0) {
@@ -788,20 +778,317 @@ static void launch_grid(KernelT k, int64_t blocks, int block, hipStream_t s, con
     hipLaunchKernelGGL(k, g, block, 0, s, a, verts, nv);
 }
 
+// Tiers 1..7 (tier 8, the hubs, runs through run_hub_sweep).
 void launch_sweep_tier(hipStream_t s, int tier, const SweepArgs &a, const int32_t *verts, int32_t nv)
 {
     if (nv <= 0) return;
     switch (tier) {
-    case 1: launch_grid(k_sweep_group<4, 3>, (nv + 63) / 64, 256, s, a, verts, nv); break;
-    case 2: launch_grid(k_sweep_group<8, 4>, (nv + 31) / 32, 256, s, a, verts, nv); break;
-    case 3: launch_grid(k_sweep_group<16, 5>, (nv + 15) / 16, 256, s, a, verts, nv); break;
-    case 4: launch_grid(k_sweep_group<32, 6>, (nv + 7) / 8, 256, s, a, verts, nv); break;
-    case 5: launch_grid(k_sweep_group<64, 7>, (nv + 3) / 4, 256, s, a, verts, nv); break;
-    case 6: launch_grid(k_sweep_group<64, 10>, (nv + 3) / 4, 256, s, a, verts, nv); break;
-    case 7: launch_grid(k_sweep_block<256, 12, false>, nv, 256, s, a, verts, nv); break;
-    case 8: launch_grid(k_sweep_block<1024, 0, true>, nv, 1024, s, a, verts, nv); break;
+    case 1: launch_grid(k_sweep_group<4, 3, 3>, (nv + 63) / 64, 256, s, a, verts, nv); break;
+    case 2: launch_grid(k_sweep_group<8, 4, 4>, (nv + 31) / 32, 256, s, a, verts, nv); break;
+    case 3: launch_grid(k_sweep_group<16, 5, 5>, (nv + 15) / 16, 256, s, a, verts, nv); break;
+    case 4: launch_grid(k_sweep_group<32, 6, 6>, (nv + 7) / 8, 256, s, a, verts, nv); break;
+    case 5: launch_grid(k_sweep_group<64, 7, 7>, (nv + 3) / 4, 256, s, a, verts, nv); break;
+    case 6: launch_grid(k_sweep_group<64, 10, 8>, (nv + 3) / 4, 256, s, a, verts, nv); break;
+    case 7: launch_grid(k_sweep_block<256, 12, 11>, nv, 256, s, a, verts, nv); break;
     default: break;
     }
+}
+
+// ---------------------------------------------------------------------------
+// Hub tier (q > 2048): partition + LDS aggregation instead of one global hash
+// table per hub, so the work is streaming reads/writes plus LDS atomics.
+
+__device__ __forceinline__ uint32_t fmix32(uint32_t h)
+{
+    h ^= h >> 16;
+    h *= 0x85ebca6bu;
+    h ^= h >> 13;
+    h *= 0xc2b2ae35u;
+    h ^= h >> 16;
+    return h;
+}
+
+// Partition of community c among P; independent of hslot's multiplicative hash
+// so the keys of one partition spread over its table.
+__device__ __forceinline__ int hub_part(int32_t c, int P)
+{
+    return (int)(((uint64_t)fmix32((uint32_t)c) * (uint64_t)P) >> 32);
+}
+
+__global__ void k_hub_setup(const int32_t *hv, int32_t nh, const int64_t *rowptr,
+                            const int64_t *selfsum, int32_t target, int64_t *hq, int32_t *hP,
+                            int64_t *hP64, int64_t *hch)
+{
+    GRID_STRIDE(j, nh)
+    {
+        int32_t v = hv[j];
+        int64_t q = rowptr[v + 1] - rowptr[v] + (selfsum[v] > 0 ? 1 : 0);
+        int64_t P = (q + target - 1) / target;
+        if (P < 1) P = 1;
+        if (P > kHubMaxParts) P = kHubMaxParts;
+        hq[j] = q;
+        hP[j] = (int32_t)P;
+        hP64[j] = P;
+        hch[j] = (q + kHubChunk - 1) / kHubChunk;
+    }
+}
+
+__global__ void k_hub_maps(int32_t nh, const int32_t *hP, const int64_t *pbase, const int64_t *hch,
+                           const int64_t *cbase, int32_t *part_hub, int32_t *chunk_hub,
+                           int32_t *chunk_k)
+{
+    GRID_STRIDE(j, nh)
+    {
+        for (int32_t p = 0; p < hP[j]; ++p) part_hub[pbase[j] + p] = (int32_t)j;
+        for (int64_t k = 0; k < hch[j]; ++k) {
+            chunk_hub[cbase[j] + k] = (int32_t)j;
+            chunk_k[cbase[j] + k] = (int32_t)k;
+        }
+    }
+}
+
+void launch_hub_setup(hipStream_t s, const int32_t *hv, int32_t nh, const int64_t *rowptr,
+                      const int64_t *selfsum, int32_t target, int64_t *hq, int32_t *hP,
+                      int64_t *hP64, int64_t *hch)
+{
+    if (nh > 0)
+        hipLaunchKernelGGL(k_hub_setup, grid_for(nh, 256), 256, 0, s, hv, nh, rowptr, selfsum,
+                           target, hq, hP, hP64, hch);
+}
+
+void launch_hub_maps(hipStream_t s, int32_t nh, const int32_t *hP, const int64_t *pbase,
+                     const int64_t *hch, const int64_t *cbase, int32_t *part_hub,
+                     int32_t *chunk_hub, int32_t *chunk_k)
+{
+    if (nh > 0)
+        hipLaunchKernelGGL(k_hub_maps, grid_for(nh, 64), 64, 0, s, nh, hP, pbase, hch, cbase,
+                           part_hub, chunk_hub, chunk_k);
+}
+
+// Entry t of hub j: t < d is row entry rowptr[v]+t, t == d the virtual
+// own-community entry (weight 0) of a hub with a self-loop.
+__global__ __launch_bounds__(256) void k_hub_count(SweepArgs a, HubArgs h)
+{
+    __shared__ int hist[kHubMaxParts];
+    const int32_t ci = blockIdx.x;
+    const int32_t j = h.chunk_hub[ci];
+    const int32_t v = h.hv[j];
+    const int64_t q = h.hq[j];
+    const int P = h.hP[j];
+    const int64_t rb = a.rowptr[v], d = a.rowptr[v + 1] - rb;
+    const int32_t own = a.C[v];
+    for (int t = threadIdx.x; t < P; t += 256) hist[t] = 0;
+    __syncthreads();
+    const int64_t t0 = (int64_t)h.chunk_k[ci] * kHubChunk;
+    const int64_t t1 = min(q, t0 + kHubChunk);
+    int32_t *tc = h.tmp_c + h.ebase[j];
+    for (int64_t t = t0 + threadIdx.x; t < t1; t += 256) {
+        int32_t c = (t < d) ? a.C[a.col[rb + t]] : own;
+        tc[t] = c;
+        atomicAdd(&hist[hub_part(c, P)], 1);
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < P; t += 256)
+        if (hist[t]) atomicAdd((ull *)&h.pcount[h.pbase[j] + t], (ull)hist[t]);
+}
+
+__global__ __launch_bounds__(256) void k_hub_scatter(SweepArgs a, HubArgs h)
+{
+    __shared__ int hist[kHubMaxParts];
+    __shared__ long long lbase[kHubMaxParts];
+    const int32_t ci = blockIdx.x;
+    const int32_t j = h.chunk_hub[ci];
+    const int32_t v = h.hv[j];
+    const int64_t q = h.hq[j];
+    const int P = h.hP[j];
+    const int64_t rb = a.rowptr[v], d = a.rowptr[v + 1] - rb;
+    for (int t = threadIdx.x; t < P; t += 256) hist[t] = 0;
+    __syncthreads();
+    const int64_t t0 = (int64_t)h.chunk_k[ci] * kHubChunk;
+    const int64_t t1 = min(q, t0 + kHubChunk);
+    const int32_t *tc = h.tmp_c + h.ebase[j];
+    for (int64_t t = t0 + threadIdx.x; t < t1; t += 256) atomicAdd(&hist[hub_part(tc[t], P)], 1);
+    __syncthreads();
+    for (int t = threadIdx.x; t < P; t += 256) {
+        if (hist[t]) lbase[t] = (long long)atomicAdd((ull *)&h.pcur[h.pbase[j] + t], (ull)hist[t]);
+        hist[t] = 0;
+    }
+    __syncthreads();
+    for (int64_t t = t0 + threadIdx.x; t < t1; t += 256) {
+        int32_t c = tc[t];
+        int part = hub_part(c, P);
+        int64_t pos = lbase[part] + atomicAdd(&hist[part], 1);
+        h.buf_c[pos] = c;
+        h.buf_w[pos] = (t < d) ? a.w[rb + t] : 0;
+    }
+}
+
+// Insert with at most 2^log2cap probes; false when the table is full.
+__device__ __forceinline__ bool lds_insert_bounded(int32_t *key, ull *val, int log2cap, int32_t c,
+                                                   int64_t w)
+{
+    const uint32_t mask = (1u << log2cap) - 1u;
+    uint32_t s = hslot(c, log2cap);
+    for (uint32_t probe = 0; probe <= mask; ++probe) {
+        int32_t k = ((volatile int32_t *)key)[s];
+        if (k == -1) {
+            int32_t prev = atomicCAS(&key[s], -1, c);
+            if (prev == -1 || prev == c) {
+                if (w != 0) atomicAdd(&val[s], (ull)w);
+                return true;
+            }
+        } else if (k == c) {
+            if (w != 0) atomicAdd(&val[s], (ull)w);
+            return true;
+        }
+        s = (s + 1) & mask;
+    }
+    return false;
+}
+
+// Global open-addressing table of `cap` slots (cap > distinct keys).
+__device__ __forceinline__ void glb_insert_mod(int32_t *key, ull *val, int64_t cap, int32_t c,
+                                               int64_t w)
+{
+    int64_t s = (int64_t)(fmix32((uint32_t)c ^ 0x5bd1e995u) % (uint64_t)cap);
+    for (;;) {
+        int32_t k = __hip_atomic_load(&key[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (k == c) break;
+        if (k == -1) {
+            int32_t prev = atomicCAS(&key[s], -1, c);
+            if (prev == -1 || prev == c) break;
+        }
+        s = (s + 1 == cap) ? 0 : s + 1;
+    }
+    if (w != 0) atomicAdd(&val[s], (ull)w);
+}
+
+constexpr int kHubAggLog2 = 12;  // 4096 LDS slots per partition block
+
+__global__ __launch_bounds__(256) void k_hub_agg(SweepArgs a, HubArgs h)
+{
+    constexpr int CAP = 1 << kHubAggLog2;
+    __shared__ int32_t s_key[CAP];
+    __shared__ ull s_val[CAP];
+    __shared__ int s_over;
+    __shared__ int r_cnt[4];
+    __shared__ double r_g[4];
+    __shared__ int32_t r_c[4];
+    __shared__ long long r_w[4];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t pi = blockIdx.x;
+    const int64_t beg = h.poff[pi], end = h.poff[pi + 1];
+    const int32_t j = h.part_hub[pi];
+    const int32_t v = h.hv[j];
+    const double f = (double)a.kdeg[v] / a.m2;
+    for (int s = tid; s < CAP; s += 256) {
+        s_key[s] = -1;
+        s_val[s] = 0;
+    }
+    if (tid == 0) s_over = 0;
+    __syncthreads();
+    for (int64_t e = beg + tid; e < end; e += 256)
+        if (!lds_insert_bounded(s_key, s_val, kHubAggLog2, h.buf_c[e], h.buf_w[e])) s_over = 1;
+    __syncthreads();
+    int cnt = 0;
+    double bg = 0.0;
+    int32_t bc = INT_MAX;
+    int64_t bw = 0;
+    if (!s_over) {
+        for (int s = tid; s < CAP; s += 256) {
+            int32_t c = s_key[s];
+            if (c >= 0) consider(c, (int64_t)s_val[s], a, f, cnt, bg, bc, bw);
+        }
+    } else {
+        // more than CAP distinct keys: redo the partition in its global region
+        const int64_t cap = 2 * (end - beg);
+        int32_t *gk = h.fbk + 2 * beg;
+        ull *gv = h.fbv + 2 * beg;
+        for (int64_t s = tid; s < cap; s += 256) {
+            gk[s] = -1;
+            gv[s] = 0;
+        }
+        __threadfence();
+        __syncthreads();
+        for (int64_t e = beg + tid; e < end; e += 256) glb_insert_mod(gk, gv, cap, h.buf_c[e], h.buf_w[e]);
+        __threadfence();
+        __syncthreads();
+        for (int64_t s = tid; s < cap; s += 256) {
+            int32_t c = __hip_atomic_load(&gk[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (c >= 0)
+                consider(c, (int64_t)__hip_atomic_load(&gv[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                         a, f, cnt, bg, bc, bw);
+        }
+    }
+    group_reduce<64>(cnt, bg, bc, bw);
+    if (lane == 0) {
+        r_cnt[wave] = cnt;
+        r_g[wave] = bg;
+        r_c[wave] = bc;
+        r_w[wave] = bw;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int tc = r_cnt[0];
+        double g0 = r_g[0];
+        int32_t c0 = r_c[0];
+        int64_t w0 = r_w[0];
+        for (int w = 1; w < 4; ++w) {
+            tc += r_cnt[w];
+            if (better(r_g[w], r_c[w], g0, c0)) {
+                g0 = r_g[w];
+                c0 = r_c[w];
+                w0 = r_w[w];
+            }
+        }
+        h.pn[pi] = tc;
+        h.pg[pi] = g0;
+        h.pc[pi] = c0;
+        h.pw[pi] = w0;
+    }
+}
+
+// One wavefront per hub: combine its partitions, then guard/move as usual.
+__global__ __launch_bounds__(256) void k_hub_final(SweepArgs a, HubArgs h, int32_t nh)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t j = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    int cnt = 0;
+    double bg = 0.0;
+    int32_t bc = INT_MAX;
+    int64_t bw = 0;
+    if (j < nh) {
+        const int64_t base = h.pbase[j];
+        for (int p = lane; p < h.hP[j]; p += 64) {
+            cnt += h.pn[base + p];
+            if (better(h.pg[base + p], h.pc[base + p], bg, bc)) {
+                bg = h.pg[base + p];
+                bc = h.pc[base + p];
+                bw = h.pw[base + p];
+            }
+        }
+    }
+    group_reduce<64>(cnt, bg, bc, bw);
+    if (j < nh && lane == 0) {
+        const int32_t v = h.hv[j];
+        finalize_vertex(a, v, a.C[v], cnt, bc, bw);
+    }
+}
+
+int run_hub_sweep(Arena &A, hipStream_t s, const SweepArgs &a, const HubArgs &h, int32_t nh,
+                  int64_t nparts, int64_t nchunks)
+{
+    if (nh <= 0) return LV_OK;
+    fill_i64(s, h.pcount, nparts + 1, 0);
+    hipLaunchKernelGGL(k_hub_count, (int)nchunks, 256, 0, s, a, h);
+    LV_TRY(scan_exclusive_i64(A, s, h.pcount, (int64_t *)h.poff, nparts, (int64_t *)h.poff + nparts));
+    LV_HIP(hipMemcpyAsync(h.pcur, h.poff, sizeof(int64_t) * (size_t)nparts,
+                          hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(k_hub_scatter, (int)nchunks, 256, 0, s, a, h);
+    hipLaunchKernelGGL(k_hub_agg, (int)nparts, 256, 0, s, a, h);
+    hipLaunchKernelGGL(k_hub_final, (int)((nh + 3) / 4), 256, 0, s, a, h, nh);
+    LV_HIP(hipGetLastError());
+    return LV_OK;
 }
 
 // ---------------------------------------------------------------------------

@@ -458,4 +458,12 @@ int radix_sort_u32_i32(Arena &a, hipStream_t s, uint32_t *keys, int32_t *vals, u
                                          vals_out);
 }
 
+int radix_sort_u32_u32(Arena &a, hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *ktmp,
+                       uint32_t *vtmp, int64_t n, int bits, uint32_t **keys_out,
+                       uint32_t **vals_out)
+{
+    return radix_sort<uint32_t, uint32_t>(a, s, keys, vals, ktmp, vtmp, n, bits, keys_out,
+                                          vals_out);
+}
+
 }  // namespace lv

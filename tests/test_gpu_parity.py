@@ -252,3 +252,38 @@ def test_rmat22_full_size():
         eng.load_edges(n, src, dst, w)
         eng.run()
         assert_run_matches(eng, r, n, trace=False)
+
+
+def _hub_graph(seed, hubs=3, hub_deg=9000, n=30000):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    src, dst = [], []
+    for h in range(hubs):
+        nb = rng.choice(np.arange(hubs, n), hub_deg, replace=False)
+        src += [h] * hub_deg
+        dst += nb.tolist()
+    e = 3 * n
+    src += rng.integers(hubs, n, e).tolist()
+    dst += rng.integers(hubs, n, e).tolist()
+    src += [0, 1]            # self-loops on two hubs: the virtual own-community entry
+    dst += [0, 1]
+    w = rng.integers(1, 5, len(src)).astype(np.int32)
+    return n, np.array(src, np.int32), np.array(dst, np.int32), w
+
+
+def test_hub_partitions():
+    gpu_case("hub partitions")
+    n, src, dst, w = _hub_graph(5)
+    eng, r = run_both(n, src, dst, w)
+    with eng:
+        assert_run_matches(eng, r, n)
+
+
+def test_hub_global_fallback(monkeypatch):
+    """One partition per hub holds ~9000 distinct communities, more than the
+    4096-slot LDS table: the partition is redone in its global table."""
+    gpu_case("hub global fallback")
+    monkeypatch.setenv("LV_HUB_PART_TARGET", "1000000")
+    n, src, dst, w = _hub_graph(6)
+    eng, r = run_both(n, src, dst, w)
+    with eng:
+        assert_run_matches(eng, r, n)
