@@ -22,7 +22,7 @@ struct LevelRec {
     int32_t n = 0, k = 0, passes = 0;
     int64_t nnz = 0, m = 0;
     double q = 0, lower = 0;
-    int32_t *cfinal = nullptr;  // assignment after the level (device)
+    int32_t *perm = nullptr;    // positions sorted by (community, position) (device)
     int32_t *newc = nullptr;    // renumbered community of each position (device)
     std::vector<double> pass_q;
     std::vector<std::vector<int32_t>> trace;
@@ -77,6 +77,9 @@ struct lv_ctx {
     // expected entries per hub partition; the LDS table of a partition holds 4096
     // distinct communities (more fall back to a global table)
     int32_t hub_part_target = kHubPartTarget;
+    // groups of the last renumbering (sorted order): head flags, their exclusive
+    // scan, first sorted index of each group, new id of each group
+    int32_t *g_head = nullptr, *g_hscan = nullptr, *g_start = nullptr, *g_newid = nullptr;
     // pass results of the level being processed
     double cur_q = 0, cur_lower = 0;
     int32_t cur_passes = 0;
@@ -207,6 +210,10 @@ void free_level_arrays(lv_ctx *c)
     arena_free(a, c->ddeg);
     arena_free(a, c->din);
     arena_free(a, c->verts);
+    arena_free(a, c->g_head);
+    arena_free(a, c->g_hscan);
+    arena_free(a, c->g_start);
+    arena_free(a, c->g_newid);
     HubArgs &h = c->hub;
     a.release((void *)h.hq);
     a.release((void *)h.hP);
@@ -235,7 +242,7 @@ void free_level_arrays(lv_ctx *c)
 void free_dendrogram(lv_ctx *c)
 {
     for (auto &L : c->levels) {
-        arena_free(c->arena, L.cfinal);
+        arena_free(c->arena, L.perm);
         arena_free(c->arena, L.newc);
     }
     c->levels.clear();
@@ -530,32 +537,51 @@ int one_level(lv_ctx *c, double lower, double *q_out, int32_t *passes_out)
     return LV_OK;
 }
 
+// renumberCUDA: stable sort of positions by community; group heads are the
+// smallest members; new id = rank of the head among heads in position order.
 int renumber(lv_ctx *c, int32_t *k_out)
 {
     Arena &A = c->arena;
     hipStream_t s = c->stream;
     const int32_t n = c->n;
     hipEvent_t t0 = t_begin(c);
-    int32_t *leader = nullptr, *flag = nullptr, *rank = nullptr;
     LevelRec L;
-    LV_TRY(arena_alloc(A, leader, n));
-    LV_TRY(arena_alloc(A, flag, n));
+    uint32_t *k1 = nullptr, *k2 = nullptr;
+    int32_t *v1 = nullptr, *v2 = nullptr, *lflag = nullptr, *rank = nullptr;
+    LV_TRY(arena_alloc(A, k1, n));
+    LV_TRY(arena_alloc(A, k2, n));
+    LV_TRY(arena_alloc(A, v1, n));
+    LV_TRY(arena_alloc(A, v2, n));
+    copy_i32(s, (int32_t *)k1, c->C, n);
+    iota_i32(s, v1, n);
+    uint32_t *Cs = nullptr;
+    int32_t *perm = nullptr;
+    LV_TRY(radix_sort_u32_i32(A, s, k1, v1, k2, v2, n, bits_for(n > 0 ? (uint64_t)(n - 1) : 0),
+                              &Cs, &perm));
+    L.perm = perm;
+    arena_free(A, perm == v1 ? v2 : v1);
+    LV_TRY(arena_alloc(A, c->g_head, n));
+    LV_TRY(arena_alloc(A, c->g_hscan, n));
+    LV_TRY(arena_alloc(A, c->g_start, (int64_t)n + 1));
+    LV_TRY(arena_alloc(A, lflag, n));
     LV_TRY(arena_alloc(A, rank, n));
-    LV_TRY(arena_alloc(A, L.cfinal, n));
-    LV_TRY(arena_alloc(A, L.newc, n));
-    fill_i32(s, leader, n, INT_MAX);
-    launch_leader(s, c->C, n, leader);
-    launch_leader_flag(s, c->C, leader, n, flag);
-    LV_TRY(scan_exclusive_i32(A, s, flag, rank, n, (int32_t *)c->d_small));
-    launch_newc(s, c->C, leader, rank, n, L.newc);
-    copy_i32(s, L.cfinal, c->C, n);
-    LV_HIP(hipGetLastError());
-    t_end(c, t0, CAT_RENUMBER);
+    launch_cr_heads(s, Cs, n, c->g_head);
+    LV_TRY(scan_exclusive_i32(A, s, c->g_head, c->g_hscan, n, (int32_t *)c->d_small));
+    fill_i32(s, lflag, n, 0);
+    launch_cr_groups(s, c->g_head, c->g_hscan, perm, n, c->g_start, lflag);
+    LV_TRY(scan_exclusive_i32(A, s, lflag, rank, n, nullptr));
     int32_t kk = 0;
     LV_TRY(d2h(c, &kk, (const int32_t *)c->d_small, 1));
     LV_TRY(sync(c));
-    arena_free(A, leader);
-    arena_free(A, flag);
+    LV_TRY(arena_alloc(A, c->g_newid, kk));
+    LV_TRY(arena_alloc(A, L.newc, n));
+    launch_cr_newid(s, c->g_start, perm, rank, kk, c->g_newid);
+    launch_cr_newc(s, c->g_head, c->g_hscan, perm, c->g_newid, n, L.newc);
+    LV_HIP(hipGetLastError());
+    t_end(c, t0, CAT_RENUMBER);
+    arena_free(A, k1);
+    arena_free(A, k2);
+    arena_free(A, lflag);
     arena_free(A, rank);
     L.n = n;
     L.k = kk;
@@ -573,81 +599,152 @@ int renumber(lv_ctx *c, int32_t *k_out)
     return LV_OK;
 }
 
+// inducedGraphCUDA: gather each community's entries in member order, aggregate
+// every community segment by neighbour community, copy the unique entries to
+// row newid(community) of the next level.
 int induce(lv_ctx *c)
 {
     Arena &A = c->arena;
     hipStream_t s = c->stream;
     const LevelRec &L = c->levels.back();
     const int32_t n = c->n;
-    const uint64_t K = (uint64_t)L.k;
-    const int64_t total = c->nnz + n;
+    const int32_t K = L.k;
+    const int64_t nnz = c->nnz;
     hipEvent_t t0 = t_begin(c);
-    uint64_t *keys = nullptr, *ktmp = nullptr;
-    int64_t *vals = nullptr, *vtmp = nullptr;
-    LV_TRY(arena_alloc(A, keys, total));
-    LV_TRY(arena_alloc(A, vals, total));
-    LV_TRY(arena_alloc(A, ktmp, total));
-    LV_TRY(arena_alloc(A, vtmp, total));
-    launch_coarsen_keys(s, c->rowptr, c->col, c->cw, n, c->nnz, c->selfsum, L.newc, K, keys, vals);
-    uint64_t *sk = nullptr;
-    int64_t *sv = nullptr;
-    const int bits = bits_for(K * K - 1);
-    LV_TRY(radix_sort_u64_i64(A, s, keys, vals, ktmp, vtmp, total, bits, &sk, &sv));
-    if (sk == keys) arena_free(A, ktmp); else arena_free(A, keys);
-    if (sv == vals) arena_free(A, vtmp); else arena_free(A, vals);
-    int64_t *head = nullptr, *hscan = nullptr;
-    LV_TRY(arena_alloc(A, head, total));
-    LV_TRY(arena_alloc(A, hscan, total));
-    launch_seg_heads(s, sk, total, head);
-    LV_TRY(scan_exclusive_i64(A, s, head, hscan, total, (int64_t *)c->d_small));
-    LV_TRY(d2h(c, c->h_small, c->d_small, 1));
+    // member order entry starts and the member-order gather
+    int64_t *mdeg = nullptr, *estart = nullptr;
+    ull *selfl = nullptr, *ssum = nullptr, *internal = nullptr;
+    int32_t *ey = nullptr, *bound = nullptr, *uy = nullptr, *nuniq = nullptr;
+    int64_t *ew = nullptr, *uw = nullptr;
+    LV_TRY(arena_alloc(A, mdeg, n));
+    LV_TRY(arena_alloc(A, estart, (int64_t)n + 1));
+    LV_TRY(arena_alloc(A, selfl, K));
+    LV_TRY(arena_alloc(A, ssum, K));
+    LV_TRY(arena_alloc(A, internal, K));
+    LV_TRY(arena_alloc(A, nuniq, K));
+    LV_TRY(arena_alloc(A, ey, nnz));
+    LV_TRY(arena_alloc(A, ew, nnz));
+    LV_TRY(arena_alloc(A, uy, nnz));
+    LV_TRY(arena_alloc(A, uw, nnz));
+    LV_TRY(arena_alloc(A, bound, nnz / 1024 + 2));
+    launch_cr_memdeg(s, L.perm, c->rowptr, n, mdeg);
+    LV_TRY(scan_exclusive_i64(A, s, mdeg, estart, n, estart + n));
+    fill_i64(s, (int64_t *)selfl, K, 0);
+    fill_i64(s, (int64_t *)ssum, K, 0);
+    fill_i64(s, (int64_t *)internal, K, 0);
+    fill_i32(s, nuniq, K, 0);
+    launch_cr_selfl(s, L.perm, c->selfsum, c->g_head, c->g_hscan, n, selfl);
+    launch_cr_gather(s, estart, bound, L.perm, c->rowptr, c->col, c->cw, L.newc, c->g_head,
+                     c->g_hscan, n, nnz, ey, ew, ssum);
+    // segments by length
+    int64_t *sb = nullptr, *slen = nullptr;
+    uint32_t *tk1 = nullptr, *tk2 = nullptr;
+    int32_t *sid1 = nullptr, *sid2 = nullptr;
+    LV_TRY(arena_alloc(A, sb, K));
+    LV_TRY(arena_alloc(A, slen, K));
+    LV_TRY(arena_alloc(A, tk1, K));
+    LV_TRY(arena_alloc(A, tk2, K));
+    LV_TRY(arena_alloc(A, sid1, K));
+    LV_TRY(arena_alloc(A, sid2, K));
+    LV_TRY(zero_small(c, kTiers + 1));
+    launch_cr_seg_tier(s, c->g_start, estart, K, sb, slen, tk1, sid1, c->d_small);
+    uint32_t *tko = nullptr;
+    int32_t *segs = nullptr;
+    LV_TRY(radix_sort_u32_i32(A, s, tk1, sid1, tk2, sid2, K, 4, &tko, &segs));
+    LV_TRY(d2h(c, c->h_small, c->d_small, kTiers + 1));
     LV_TRY(sync(c));
-    const int64_t U = c->h_small[0];
-    uint64_t *ukey = nullptr;
-    ull *usum = nullptr;
-    LV_TRY(arena_alloc(A, ukey, U));
-    LV_TRY(arena_alloc(A, usum, U));
-    fill_i64(s, (int64_t *)usum, U, 0);
-    launch_seg_sum(s, sk, sv, hscan, head, total, ukey, usum);
-    arena_free(A, sk);
-    arena_free(A, sv);
-    arena_free(A, head);
-    arena_free(A, hscan);
-    // next level
-    const int32_t n2 = (int32_t)K;
-    int64_t *rowcnt = nullptr, *nsflag = nullptr, *pos = nullptr;
-    int64_t *rowptr2 = nullptr, *kdeg2 = nullptr, *loop2 = nullptr, *self2 = nullptr, *cw2 = nullptr;
+    int64_t tcnt[kTiers + 1], toff[kTiers + 1], off = 0;
+    for (int t = 0; t <= kTiers; ++t) {
+        tcnt[t] = c->h_small[t];
+        toff[t] = off;
+        off += tcnt[t];
+    }
+    CoarsenArgs ca;
+    ca.ey = ey;
+    ca.ew = ew;
+    ca.sb = sb;
+    ca.slen = slen;
+    ca.newid_g = c->g_newid;
+    ca.uy = uy;
+    ca.uw = uw;
+    ca.nuniq = nuniq;
+    ca.internal = internal;
+    for (int t = 1; t < kTiers; ++t) launch_cr_agg_tier(s, t, ca, segs + toff[t], (int32_t)tcnt[t]);
+    // segments above 2048 entries: hash partitions
+    BigSegArgs B{};
+    const int32_t nbs = (int32_t)tcnt[kTiers];
+    int64_t nparts = 0, nchunks = 0, blen = 0;
+    int32_t *bP = nullptr, *part_big = nullptr, *chunk_big = nullptr, *chunk_k = nullptr;
+    int64_t *bP64 = nullptr, *bch = nullptr, *pbase = nullptr, *cbase = nullptr;
+    if (nbs > 0) {
+        const int32_t *bsegs = segs + toff[kTiers];
+        LV_TRY(arena_alloc(A, bP, nbs));
+        LV_TRY(arena_alloc(A, bP64, nbs));
+        LV_TRY(arena_alloc(A, bch, nbs));
+        LV_TRY(arena_alloc(A, pbase, (int64_t)nbs + 1));
+        LV_TRY(arena_alloc(A, cbase, (int64_t)nbs + 1));
+        launch_cr_big_setup(s, bsegs, nbs, slen, c->hub_part_target, bP, bP64, bch);
+        LV_TRY(scan_exclusive_i64(A, s, bP64, pbase, nbs, pbase + nbs));
+        LV_TRY(scan_exclusive_i64(A, s, bch, cbase, nbs, cbase + nbs));
+        LV_TRY(d2h(c, c->h_small, (const long long *)(pbase + nbs), 1));
+        LV_TRY(d2h(c, c->h_small + 1, (const long long *)(cbase + nbs), 1));
+        LV_TRY(sync(c));
+        nparts = c->h_small[0];
+        nchunks = c->h_small[1];
+        blen = nchunks * (int64_t)kHubChunk;  // bound on the entries of big segments
+        if (blen > nnz) blen = nnz;
+        LV_TRY(arena_alloc(A, part_big, nparts));
+        LV_TRY(arena_alloc(A, chunk_big, nchunks));
+        LV_TRY(arena_alloc(A, chunk_k, nchunks));
+        launch_cr_big_maps(s, nbs, bP, pbase, bch, cbase, part_big, chunk_big, chunk_k);
+        B.bsegs = bsegs;
+        B.bP = bP;
+        B.pbase = pbase;
+        B.part_big = part_big;
+        B.chunk_big = chunk_big;
+        B.chunk_k = chunk_k;
+        LV_TRY(arena_alloc(A, B.pcount, nparts + 1));
+        LV_TRY(arena_alloc(A, B.poff, nparts + 1));
+        LV_TRY(arena_alloc(A, B.pcur, nparts));
+        LV_TRY(arena_alloc(A, B.py, blen));
+        LV_TRY(arena_alloc(A, B.pw, blen));
+        LV_TRY(arena_alloc(A, B.buy, blen));
+        LV_TRY(arena_alloc(A, B.buw, blen));
+        LV_TRY(arena_alloc(A, B.fbk, 2 * blen));
+        LV_TRY(arena_alloc(A, B.fbv, 2 * blen));
+        LV_TRY(arena_alloc(A, B.ucnt, nparts));
+        LV_TRY(arena_alloc(A, B.pre, nparts));
+        LV_TRY(run_cr_big(A, s, ca, B, nbs, nparts, nchunks));
+    }
+    // next level rows
+    int64_t *rowcnt = nullptr, *rowptr2 = nullptr, *kdeg2 = nullptr, *loop2 = nullptr,
+            *self2 = nullptr, *cw2 = nullptr;
     int32_t *col2 = nullptr;
-    LV_TRY(arena_alloc(A, rowcnt, (int64_t)n2 + 1));
-    LV_TRY(arena_alloc(A, nsflag, U));
-    LV_TRY(arena_alloc(A, pos, U));
-    LV_TRY(arena_alloc(A, rowptr2, (int64_t)n2 + 1));
-    LV_TRY(arena_alloc(A, kdeg2, n2));
-    LV_TRY(arena_alloc(A, loop2, n2));
-    LV_TRY(arena_alloc(A, self2, n2));
-    fill_i64(s, rowcnt, (int64_t)n2 + 1, 0);
-    fill_i64(s, kdeg2, n2, 0);
-    fill_i64(s, loop2, n2, 0);
-    fill_i64(s, self2, n2, 0);
-    launch_coarse_split(s, ukey, usum, U, K, nsflag, loop2, self2, rowcnt, kdeg2);
-    LV_TRY(scan_exclusive_i64(A, s, nsflag, pos, U, (int64_t *)c->d_small));
-    LV_TRY(scan_exclusive_i64(A, s, rowcnt, rowptr2, n2, rowptr2 + n2));
+    LV_TRY(arena_alloc(A, rowcnt, (int64_t)K + 1));
+    LV_TRY(arena_alloc(A, rowptr2, (int64_t)K + 1));
+    LV_TRY(arena_alloc(A, kdeg2, K));
+    LV_TRY(arena_alloc(A, loop2, K));
+    LV_TRY(arena_alloc(A, self2, K));
+    launch_cr_rowcnt(s, c->g_newid, nuniq, K, rowcnt);
+    LV_TRY(scan_exclusive_i64(A, s, rowcnt, rowptr2, K, rowptr2 + K));
     fill_i64(s, (int64_t *)c->d_small + 1, 1, 0);
-    launch_sum_ull(s, usum, U, c->d_small + 1);
-    LV_TRY(d2h(c, c->h_small, c->d_small, 2));
+    launch_cr_vertex(s, c->g_newid, ssum, internal, selfl, K, loop2, self2, kdeg2, c->d_small + 1);
+    LV_TRY(d2h(c, c->h_small, (const long long *)(rowptr2 + K), 1));
+    LV_TRY(d2h(c, c->h_small + 1, c->d_small + 1, 1));
     LV_TRY(sync(c));
     const int64_t nnz2 = c->h_small[0];
     const int64_t m2 = c->h_small[1] / 2;
     LV_TRY(arena_alloc(A, col2, nnz2));
     LV_TRY(arena_alloc(A, cw2, nnz2));
-    launch_coarse_fill(s, ukey, usum, nsflag, pos, U, K, col2, cw2, kdeg2);
+    launch_cr_copy(s, ca, B, K, nparts, rowptr2, col2, cw2);
     LV_HIP(hipGetLastError());
     t_end(c, t0, CAT_COARSEN);
-    arena_free(A, ukey);
-    arena_free(A, usum);
-    arena_free(A, rowcnt);
-    arena_free(A, nsflag);
-    arena_free(A, pos);
+    // release the transient buffers (later launches on this stream may reuse them)
+    void *tmp[] = {mdeg, estart, selfl, ssum, internal, nuniq, ey, ew, uy, uw, bound, sb, slen,
+                   tk1, tk2, sid1, sid2, bP, bP64, bch, pbase, cbase, part_big, chunk_big, chunk_k,
+                   B.pcount, B.poff, B.pcur, B.py, B.pw, B.buy, B.buw, B.fbk, B.fbv, B.ucnt, B.pre,
+                   rowcnt};
+    for (void *p : tmp) A.release(p);
     free_level_arrays(c);
     c->rowptr = rowptr2;
     c->col = col2;
@@ -655,7 +752,7 @@ int induce(lv_ctx *c)
     c->kdeg = kdeg2;
     c->loopw = loop2;
     c->selfsum = self2;
-    c->n = n2;
+    c->n = K;
     c->nnz = nnz2;
     c->m = m2;
     c->level += 1;
@@ -673,30 +770,6 @@ int flatten_final(lv_ctx *c)
     LV_HIP(hipGetLastError());
     t_end(c, t0, CAT_FLATTEN);
     c->final_valid = true;
-    return LV_OK;
-}
-
-// Positions of a level sorted by (final community, position): the order of
-// the reference's Partition (stable radix sort of positions by community).
-int partition_perm(lv_ctx *c, const LevelRec &L, int32_t **perm_out, int32_t **spare_out)
-{
-    Arena &A = c->arena;
-    uint32_t *k1 = nullptr, *k2 = nullptr;
-    int32_t *v1 = nullptr, *v2 = nullptr;
-    LV_TRY(arena_alloc(A, k1, L.n));
-    LV_TRY(arena_alloc(A, k2, L.n));
-    LV_TRY(arena_alloc(A, v1, L.n));
-    LV_TRY(arena_alloc(A, v2, L.n));
-    copy_i32(c->stream, (int32_t *)k1, L.cfinal, L.n);
-    iota_i32(c->stream, v1, L.n);
-    uint32_t *ko = nullptr;
-    int32_t *vo = nullptr;
-    LV_TRY(radix_sort_u32_i32(A, c->stream, k1, v1, k2, v2, L.n,
-                              bits_for(L.n > 0 ? (uint64_t)(L.n - 1) : 0), &ko, &vo));
-    arena_free(A, k1);
-    arena_free(A, k2);
-    *perm_out = vo;
-    *spare_out = (vo == v1) ? v2 : v1;
     return LV_OK;
 }
 
@@ -984,9 +1057,9 @@ int lv_run(lv_ctx *c, int32_t *levels_out)
         LevelRec L;
         L.n = c->n0;
         L.k = c->n0;
-        LV_TRY(arena_alloc(c->arena, L.cfinal, c->n0));
+        LV_TRY(arena_alloc(c->arena, L.perm, c->n0));
         LV_TRY(arena_alloc(c->arena, L.newc, c->n0));
-        iota_i32(c->stream, L.cfinal, c->n0);
+        iota_i32(c->stream, L.perm, c->n0);
         iota_i32(c->stream, L.newc, c->n0);
         c->levels.push_back(std::move(L));
         LV_TRY(flatten_final(c));
@@ -1065,17 +1138,14 @@ int lv_level_partition(lv_ctx *c, int32_t level, int32_t *node, int32_t *comm)
     LV_TRY(set_device(c));
     const LevelRec &L = c->levels[level];
     if (L.n == 0) return LV_OK;
-    int32_t *perm = nullptr, *spare = nullptr, *dn = nullptr, *dc = nullptr;
-    LV_TRY(partition_perm(c, L, &perm, &spare));
+    int32_t *dn = nullptr, *dc = nullptr;
     LV_TRY(arena_alloc(c->arena, dn, L.n));
     LV_TRY(arena_alloc(c->arena, dc, L.n));
-    launch_gather_partition(c->stream, perm, level == 0 ? c->d_ids : nullptr, L.newc, L.n, dn, dc);
+    launch_gather_partition(c->stream, L.perm, level == 0 ? c->d_ids : nullptr, L.newc, L.n, dn, dc);
     LV_HIP(hipGetLastError());
     LV_TRY(d2h(c, node, dn, L.n));
     LV_TRY(d2h(c, comm, dc, L.n));
     LV_TRY(sync(c));
-    arena_free(c->arena, perm);
-    arena_free(c->arena, spare);
     arena_free(c->arena, dn);
     arena_free(c->arena, dc);
     return LV_OK;
@@ -1097,21 +1167,18 @@ int lv_flatten(lv_ctx *c, int32_t level, int32_t *node, int32_t *comm)
     LV_TRY(set_device(c));
     const LevelRec &L0 = c->levels[0];
     if (L0.n == 0) return LV_OK;
-    int32_t *comp = nullptr, *perm = nullptr, *spare = nullptr, *dn = nullptr, *dc = nullptr;
+    int32_t *comp = nullptr, *dn = nullptr, *dc = nullptr;
     LV_TRY(arena_alloc(c->arena, comp, L0.n));
     copy_i32(c->stream, comp, L0.newc, L0.n);
     for (int32_t l = 1; l <= level; ++l) launch_compose(c->stream, comp, L0.n, c->levels[l].newc);
-    LV_TRY(partition_perm(c, L0, &perm, &spare));
     LV_TRY(arena_alloc(c->arena, dn, L0.n));
     LV_TRY(arena_alloc(c->arena, dc, L0.n));
-    launch_gather_partition(c->stream, perm, c->d_ids, comp, L0.n, dn, dc);
+    launch_gather_partition(c->stream, L0.perm, c->d_ids, comp, L0.n, dn, dc);
     LV_HIP(hipGetLastError());
     LV_TRY(d2h(c, node, dn, L0.n));
     LV_TRY(d2h(c, comm, dc, L0.n));
     LV_TRY(sync(c));
     arena_free(c->arena, comp);
-    arena_free(c->arena, perm);
-    arena_free(c->arena, spare);
     arena_free(c->arena, dn);
     arena_free(c->arena, dc);
     return LV_OK;

@@ -1,0 +1,663 @@
+// coarsen.hip - renumbering and coarsening by communities (gfx950).
+//
+// Renumbering (renumberCUDA, LouvainCUDA/RenumberKernel.cu:63-220): positions are
+// stably radix-sorted by their community; the head of each group is the
+// community's smallest member, and the new id of a community is the rank of that
+// head among all heads in position order - the reference's first-occurrence
+// rule.  The sorted positions are also the order of the reference's Partition
+// arrays (:121-133,191,202-203) and the member lists coarsening walks.
+//
+// Coarsening (inducedGraphCUDA, LouvainCUDA/InducedGraphKernel.cu:28-113): every
+// community's row entries are gathered contiguously in member order as
+// (newc[target], w' = w==0?1:w); each community segment is then aggregated by
+// neighbour community - in one LDS table for segments of up to 2048 entries, or
+// by hash partitions of about 1024 entries for larger ones - which gives row x
+// of the next level: one entry (y, E'(x,y) + E'(y,x)) per neighbour community
+// y != x.  Entries with y == x and the members' self-loops form the next
+// level's self-loop E'(x,x) = internal/2 + sum(selfsum).
+#include "device.hpp"
+
+namespace lv {
+
+// ---------------------------------------------------------------------------
+// renumbering
+
+__global__ void k_cr_heads(const uint32_t *Cs, int32_t n, int32_t *head)
+{
+    GRID_STRIDE(j, n) head[j] = (j == 0 || Cs[j] != Cs[j - 1]) ? 1 : 0;
+}
+
+// hscan = exclusive scan of head; group of sorted element j = hscan[j]+head[j]-1.
+__global__ void k_cr_groups(const int32_t *head, const int32_t *hscan, const int32_t *perm,
+                            int32_t n, int32_t *gstart, int32_t *lflag)
+{
+    GRID_STRIDE(j, n)
+    {
+        if (head[j]) {
+            gstart[hscan[j]] = (int32_t)j;
+            lflag[perm[j]] = 1;
+        }
+        if (j == n - 1) gstart[hscan[j] + head[j]] = n;
+    }
+}
+
+__global__ void k_cr_newid(const int32_t *gstart, const int32_t *perm, const int32_t *rank,
+                           int32_t K, int32_t *newid_g)
+{
+    GRID_STRIDE(g, K) newid_g[g] = rank[perm[gstart[g]]];
+}
+
+__global__ void k_cr_newc(const int32_t *head, const int32_t *hscan, const int32_t *perm,
+                          const int32_t *newid_g, int32_t n, int32_t *newc)
+{
+    GRID_STRIDE(j, n) newc[perm[j]] = newid_g[hscan[j] + head[j] - 1];
+}
+
+void launch_cr_heads(hipStream_t s, const uint32_t *Cs, int32_t n, int32_t *head)
+{
+    if (n > 0) hipLaunchKernelGGL(k_cr_heads, grid_for(n, 256), 256, 0, s, Cs, n, head);
+}
+
+void launch_cr_groups(hipStream_t s, const int32_t *head, const int32_t *hscan, const int32_t *perm,
+                      int32_t n, int32_t *gstart, int32_t *lflag)
+{
+    if (n > 0)
+        hipLaunchKernelGGL(k_cr_groups, grid_for(n, 256), 256, 0, s, head, hscan, perm, n, gstart,
+                           lflag);
+}
+
+void launch_cr_newid(hipStream_t s, const int32_t *gstart, const int32_t *perm, const int32_t *rank,
+                     int32_t K, int32_t *newid_g)
+{
+    if (K > 0)
+        hipLaunchKernelGGL(k_cr_newid, grid_for(K, 256), 256, 0, s, gstart, perm, rank, K, newid_g);
+}
+
+void launch_cr_newc(hipStream_t s, const int32_t *head, const int32_t *hscan, const int32_t *perm,
+                    const int32_t *newid_g, int32_t n, int32_t *newc)
+{
+    if (n > 0)
+        hipLaunchKernelGGL(k_cr_newc, grid_for(n, 256), 256, 0, s, head, hscan, perm, newid_g, n,
+                           newc);
+}
+
+// ---------------------------------------------------------------------------
+// member-order gather
+
+__global__ void k_cr_memdeg(const int32_t *perm, const int64_t *rowptr, int32_t n, int64_t *mdeg)
+{
+    GRID_STRIDE(j, n)
+    {
+        int32_t a = perm[j];
+        mdeg[j] = rowptr[a + 1] - rowptr[a];
+    }
+}
+
+// selfl[g] = sum of selfsum over the members of group g (members are sorted by group).
+__global__ void k_cr_selfl(const int32_t *perm, const int64_t *selfsum, const int32_t *head,
+                           const int32_t *hscan, int32_t n, ull *selfl)
+{
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n;
+         base += (int64_t)gridDim.x * blockDim.x) {
+        int64_t j = base + threadIdx.x;
+        bool valid = j < n;
+        long long g = 0;
+        ull v = 0;
+        if (valid) {
+            g = hscan[j] + head[j] - 1;
+            v = (ull)selfsum[perm[j]];
+        }
+        seg_atomic_add(valid, g, v, selfl);
+    }
+}
+
+// bound[b] = the last member j with estart[j] <= min(b*T, total-1).
+__global__ void k_lbs_bounds(const int64_t *estart, int32_t n, int64_t total, int T, int64_t nb,
+                             int32_t *bound)
+{
+    GRID_STRIDE(b, nb + 1)
+    {
+        int64_t o = b * (int64_t)T;
+        if (o > total - 1) o = total - 1;
+        int32_t lo = 0, hi = n;  // estart[lo] <= o < estart[hi]
+        while (hi - lo > 1) {
+            int32_t mid = (lo + hi) >> 1;
+            if (estart[mid] <= o) lo = mid; else hi = mid;
+        }
+        bound[b] = lo;
+    }
+}
+
+constexpr int kGatherT = 1024;  // output entries per block (4 per thread)
+
+// ey/ew[o] for every output slot o of member order: the o-th entry of the
+// member rows, as (newc[target], w'); ssum[g] accumulates w' per group.
+__global__ __launch_bounds__(256) void k_cr_gather(const int64_t *estart, const int32_t *bound,
+                                                  const int32_t *perm, const int64_t *rowptr,
+                                                  const int32_t *col, const int64_t *cw,
+                                                  const int32_t *newc, const int32_t *head,
+                                                  const int32_t *hscan, int64_t total,
+                                                  int32_t *ey, int64_t *ew, ull *ssum)
+{
+    const int64_t b = blockIdx.x;
+    const int32_t lo0 = bound[b], hi0 = bound[b + 1];
+#pragma unroll
+    for (int r = 0; r < kGatherT / 256; ++r) {
+        const int64_t o = b * kGatherT + r * 256 + threadIdx.x;
+        const bool valid = o < total;
+        long long g = 0;
+        ull wv = 0;
+        if (valid) {
+            int32_t lo = lo0, hi = hi0 + 1;  // last j in [lo0, hi0] with estart[j] <= o
+            while (hi - lo > 1) {
+                int32_t mid = (lo + hi) >> 1;
+                if (estart[mid] <= o) lo = mid; else hi = mid;
+            }
+            const int32_t a = perm[lo];
+            const int64_t p = rowptr[a] + (o - estart[lo]);
+            const int64_t u = cw[p];
+            ey[o] = newc[col[p]];
+            wv = (ull)(u == 0 ? 1 : u);
+            ew[o] = (int64_t)wv;
+            g = hscan[lo] + head[lo] - 1;
+        }
+        seg_atomic_add(valid, g, wv, ssum);
+    }
+}
+
+void launch_cr_memdeg(hipStream_t s, const int32_t *perm, const int64_t *rowptr, int32_t n,
+                      int64_t *mdeg)
+{
+    if (n > 0) hipLaunchKernelGGL(k_cr_memdeg, grid_for(n, 256), 256, 0, s, perm, rowptr, n, mdeg);
+}
+
+void launch_cr_selfl(hipStream_t s, const int32_t *perm, const int64_t *selfsum,
+                     const int32_t *head, const int32_t *hscan, int32_t n, ull *selfl)
+{
+    if (n > 0)
+        hipLaunchKernelGGL(k_cr_selfl, grid_for(n, 256, 1 << 20), 256, 0, s, perm, selfsum, head,
+                           hscan, n, selfl);
+}
+
+void launch_cr_gather(hipStream_t s, const int64_t *estart, int32_t *bound, const int32_t *perm,
+                      const int64_t *rowptr, const int32_t *col, const int64_t *cw,
+                      const int32_t *newc, const int32_t *head, const int32_t *hscan, int32_t n,
+                      int64_t total, int32_t *ey, int64_t *ew, ull *ssum)
+{
+    if (total <= 0) return;
+    const int64_t nb = (total + kGatherT - 1) / kGatherT;
+    hipLaunchKernelGGL(k_lbs_bounds, grid_for(nb + 1, 256), 256, 0, s, estart, n, total, kGatherT,
+                       nb, bound);
+    hipLaunchKernelGGL(k_cr_gather, (int)nb, 256, 0, s, estart, bound, perm, rowptr, col, cw, newc,
+                       head, hscan, total, ey, ew, ssum);
+}
+
+// ---------------------------------------------------------------------------
+// segments: group g's entries are [sb[g], sb[g] + slen[g]) of ey/ew
+
+__global__ __launch_bounds__(256) void k_cr_seg_tier(const int32_t *gstart, const int64_t *estart,
+                                                    int32_t K, int64_t *sb, int64_t *slen,
+                                                    uint32_t *tkey, int32_t *sid,
+                                                    long long *tier_cnt)
+{
+    __shared__ long long s_cnt[kTiers + 1];
+    if (threadIdx.x <= kTiers) s_cnt[threadIdx.x] = 0;
+    __syncthreads();
+    GRID_STRIDE(g, K)
+    {
+        int64_t b = estart[gstart[g]], e = estart[gstart[g + 1]];
+        sb[g] = b;
+        slen[g] = e - b;
+        int t = tier_of(e - b, e - b);
+        tkey[g] = (uint32_t)t;
+        sid[g] = (int32_t)g;
+        atomicAdd((ull *)&s_cnt[t], 1ull);
+    }
+    __syncthreads();
+    if (threadIdx.x <= kTiers && s_cnt[threadIdx.x])
+        atomicAdd((ull *)&tier_cnt[threadIdx.x], (ull)s_cnt[threadIdx.x]);
+}
+
+void launch_cr_seg_tier(hipStream_t s, const int32_t *gstart, const int64_t *estart, int32_t K,
+                        int64_t *sb, int64_t *slen, uint32_t *tkey, int32_t *sid,
+                        long long *tier_cnt)
+{
+    if (K > 0)
+        hipLaunchKernelGGL(k_cr_seg_tier, grid_for(K, 256, 8192), 256, 0, s, gstart, estart, K, sb,
+                           slen, tkey, sid, tier_cnt);
+}
+
+// Segments of up to 2048 entries: G lanes (or a block) per segment, LDS table
+// sized to the segment; uniques are written at sb[g] in ey's index space.
+template <int G, int LOG2CAP, int MINLOG2>
+__global__ __launch_bounds__(256) void k_cr_agg_group(CoarsenArgs a, const int32_t *__restrict__ segs,
+                                                     int32_t ns)
+{
+    constexpr int CAP = 1 << LOG2CAP;
+    constexpr int GPB = 256 / G;
+    __shared__ int32_t s_key[GPB * CAP];
+    __shared__ ull s_val[GPB * CAP];
+    __shared__ int s_cnt[GPB];
+    const int lg = threadIdx.x % G, grp = threadIdx.x / G;
+    int32_t *key = s_key + grp * CAP;
+    ull *val = s_val + grp * CAP;
+    for (int64_t base = (int64_t)blockIdx.x * GPB; base < ns; base += (int64_t)gridDim.x * GPB) {
+        const int64_t vi = base + grp;
+        const bool active = vi < ns;
+        int32_t g = 0, x = 0;
+        int64_t b = 0, L = 0;
+        int l2 = MINLOG2;
+        if (active) {
+            g = segs[vi];
+            x = a.newid_g[g];
+            b = a.sb[g];
+            L = a.slen[g];
+            if (MINLOG2 < LOG2CAP) l2 = table_log2(L, MINLOG2);
+        }
+        const int cap = 1 << l2;
+        for (int s = lg; s < cap; s += G) {
+            key[s] = -1;
+            val[s] = 0;
+        }
+        if (lg == 0) s_cnt[grp] = 0;
+        __syncthreads();
+        ull inter = 0;
+        if (active) {
+            for (int64_t e = b + lg; e < b + L; e += G) {
+                int32_t y = a.ey[e];
+                int64_t w = a.ew[e];
+                if (y == x)
+                    inter += (ull)w;
+                else
+                    lds_insert(key, val, l2, y, w);
+            }
+        }
+        __syncthreads();
+        if (active) {
+            for (int s = lg; s < cap; s += G) {
+                int32_t y = key[s];
+                if (y >= 0) {
+                    int pos = atomicAdd(&s_cnt[grp], 1);
+                    a.uy[b + pos] = y;
+                    a.uw[b + pos] = (int64_t)val[s];
+                }
+            }
+        }
+#pragma unroll
+        for (int off = G / 2; off > 0; off >>= 1) inter += __shfl_xor(inter, off, G);
+        __syncthreads();
+        if (active && lg == 0) {
+            a.nuniq[g] = s_cnt[grp];
+            a.internal[g] = inter;
+        }
+        __syncthreads();
+    }
+}
+
+template <int LOG2CAP, int MINLOG2>
+__global__ __launch_bounds__(256) void k_cr_agg_block(CoarsenArgs a, const int32_t *__restrict__ segs,
+                                                     int32_t ns)
+{
+    constexpr int CAP = 1 << LOG2CAP;
+    __shared__ int32_t s_key[CAP];
+    __shared__ ull s_val[CAP];
+    __shared__ int s_cnt;
+    __shared__ ull s_inter[4];
+    const int tid = threadIdx.x;
+    for (int64_t vi = blockIdx.x; vi < ns; vi += gridDim.x) {
+        const int32_t g = segs[vi];
+        const int32_t x = a.newid_g[g];
+        const int64_t b = a.sb[g], L = a.slen[g];
+        const int l2 = table_log2(L, MINLOG2);
+        const int cap = 1 << l2;
+        for (int s = tid; s < cap; s += 256) {
+            s_key[s] = -1;
+            s_val[s] = 0;
+        }
+        if (tid == 0) s_cnt = 0;
+        __syncthreads();
+        ull inter = 0;
+        for (int64_t e = b + tid; e < b + L; e += 256) {
+            int32_t y = a.ey[e];
+            int64_t w = a.ew[e];
+            if (y == x)
+                inter += (ull)w;
+            else
+                lds_insert(s_key, s_val, l2, y, w);
+        }
+        __syncthreads();
+        for (int s = tid; s < cap; s += 256) {
+            int32_t y = s_key[s];
+            if (y >= 0) {
+                int pos = atomicAdd(&s_cnt, 1);
+                a.uy[b + pos] = y;
+                a.uw[b + pos] = (int64_t)s_val[s];
+            }
+        }
+        inter = (ull)wave_sum((long long)inter);
+        if ((tid & 63) == 0) s_inter[tid >> 6] = inter;
+        __syncthreads();
+        if (tid == 0) {
+            a.nuniq[g] = s_cnt;
+            a.internal[g] = s_inter[0] + s_inter[1] + s_inter[2] + s_inter[3];
+        }
+        __syncthreads();
+    }
+}
+
+template <typename KernelT>
+static void launch_seg(KernelT k, int64_t blocks, hipStream_t s, const CoarsenArgs &a,
+                       const int32_t *segs, int32_t ns)
+{
+    int g = (int)(blocks > (1 << 20) ? (1 << 20) : (blocks < 1 ? 1 : blocks));
+    hipLaunchKernelGGL(k, g, 256, 0, s, a, segs, ns);
+}
+
+void launch_cr_agg_tier(hipStream_t s, int tier, const CoarsenArgs &a, const int32_t *segs, int32_t ns)
+{
+    if (ns <= 0) return;
+    switch (tier) {
+    case 1: launch_seg(k_cr_agg_group<4, 3, 3>, (ns + 63) / 64, s, a, segs, ns); break;
+    case 2: launch_seg(k_cr_agg_group<8, 4, 4>, (ns + 31) / 32, s, a, segs, ns); break;
+    case 3: launch_seg(k_cr_agg_group<16, 5, 5>, (ns + 15) / 16, s, a, segs, ns); break;
+    case 4: launch_seg(k_cr_agg_group<32, 6, 6>, (ns + 7) / 8, s, a, segs, ns); break;
+    case 5: launch_seg(k_cr_agg_group<64, 7, 7>, (ns + 3) / 4, s, a, segs, ns); break;
+    case 6: launch_seg(k_cr_agg_group<64, 10, 8>, (ns + 3) / 4, s, a, segs, ns); break;
+    case 7: launch_seg(k_cr_agg_block<12, 11>, ns, s, a, segs, ns); break;
+    default: break;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// segments of more than 2048 entries: hash partitions of about kHubPartTarget
+// entries, each aggregated in LDS (global table when it has more than 4096
+// distinct keys); uniques land at poff[p] of buy/buw.
+
+__global__ void k_cr_big_setup(const int32_t *bsegs, int32_t nbs, const int64_t *slen,
+                               int32_t target, int32_t *bP, int64_t *bP64, int64_t *bch)
+{
+    GRID_STRIDE(j, nbs)
+    {
+        int64_t L = slen[bsegs[j]];
+        int64_t P = (L + target - 1) / target;
+        if (P < 1) P = 1;
+        if (P > kHubMaxParts) P = kHubMaxParts;
+        bP[j] = (int32_t)P;
+        bP64[j] = P;
+        bch[j] = (L + kHubChunk - 1) / kHubChunk;
+    }
+}
+
+__global__ void k_cr_big_maps(int32_t nbs, const int32_t *bP, const int64_t *pbase,
+                              const int64_t *bch, const int64_t *cbase, int32_t *part_big,
+                              int32_t *chunk_big, int32_t *chunk_k)
+{
+    GRID_STRIDE(j, nbs)
+    {
+        for (int32_t p = 0; p < bP[j]; ++p) part_big[pbase[j] + p] = (int32_t)j;
+        for (int64_t k = 0; k < bch[j]; ++k) {
+            chunk_big[cbase[j] + k] = (int32_t)j;
+            chunk_k[cbase[j] + k] = (int32_t)k;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_cr_big_count(CoarsenArgs a, BigSegArgs B)
+{
+    __shared__ int hist[kHubMaxParts];
+    __shared__ ull s_inter[4];
+    const int32_t ci = blockIdx.x;
+    const int32_t j = B.chunk_big[ci];
+    const int32_t g = B.bsegs[j];
+    const int32_t x = a.newid_g[g];
+    const int P = B.bP[j];
+    const int64_t b = a.sb[g], L = a.slen[g];
+    for (int t = threadIdx.x; t < P; t += 256) hist[t] = 0;
+    __syncthreads();
+    const int64_t t0 = (int64_t)B.chunk_k[ci] * kHubChunk;
+    const int64_t t1 = min(L, t0 + kHubChunk);
+    ull inter = 0;
+    for (int64_t t = t0 + threadIdx.x; t < t1; t += 256) {
+        int32_t y = a.ey[b + t];
+        if (y == x)
+            inter += (ull)a.ew[b + t];
+        else
+            atomicAdd(&hist[hub_part(y, P)], 1);
+    }
+    inter = (ull)wave_sum((long long)inter);
+    if ((threadIdx.x & 63) == 0) s_inter[threadIdx.x >> 6] = inter;
+    __syncthreads();
+    for (int t = threadIdx.x; t < P; t += 256)
+        if (hist[t]) atomicAdd((ull *)&B.pcount[B.pbase[j] + t], (ull)hist[t]);
+    if (threadIdx.x == 0) {
+        ull tot = s_inter[0] + s_inter[1] + s_inter[2] + s_inter[3];
+        if (tot) atomicAdd(&a.internal[g], tot);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_cr_big_scatter(CoarsenArgs a, BigSegArgs B)
+{
+    __shared__ int hist[kHubMaxParts];
+    __shared__ long long lbase[kHubMaxParts];
+    const int32_t ci = blockIdx.x;
+    const int32_t j = B.chunk_big[ci];
+    const int32_t g = B.bsegs[j];
+    const int32_t x = a.newid_g[g];
+    const int P = B.bP[j];
+    const int64_t b = a.sb[g], L = a.slen[g];
+    for (int t = threadIdx.x; t < P; t += 256) hist[t] = 0;
+    __syncthreads();
+    const int64_t t0 = (int64_t)B.chunk_k[ci] * kHubChunk;
+    const int64_t t1 = min(L, t0 + kHubChunk);
+    for (int64_t t = t0 + threadIdx.x; t < t1; t += 256) {
+        int32_t y = a.ey[b + t];
+        if (y != x) atomicAdd(&hist[hub_part(y, P)], 1);
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < P; t += 256) {
+        if (hist[t]) lbase[t] = (long long)atomicAdd((ull *)&B.pcur[B.pbase[j] + t], (ull)hist[t]);
+        hist[t] = 0;
+    }
+    __syncthreads();
+    for (int64_t t = t0 + threadIdx.x; t < t1; t += 256) {
+        int32_t y = a.ey[b + t];
+        if (y == x) continue;
+        int part = hub_part(y, P);
+        int64_t pos = lbase[part] + atomicAdd(&hist[part], 1);
+        B.py[pos] = y;
+        B.pw[pos] = a.ew[b + t];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_cr_big_agg(BigSegArgs B)
+{
+    constexpr int CAP = 1 << kHubAggLog2;
+    __shared__ int32_t s_key[CAP];
+    __shared__ ull s_val[CAP];
+    __shared__ int s_over;
+    __shared__ int s_cnt;
+    const int tid = threadIdx.x;
+    const int64_t pi = blockIdx.x;
+    const int64_t beg = B.poff[pi], end = B.poff[pi + 1];
+    for (int s = tid; s < CAP; s += 256) {
+        s_key[s] = -1;
+        s_val[s] = 0;
+    }
+    if (tid == 0) {
+        s_over = 0;
+        s_cnt = 0;
+    }
+    __syncthreads();
+    for (int64_t e = beg + tid; e < end; e += 256)
+        if (!lds_insert_bounded(s_key, s_val, kHubAggLog2, B.py[e], B.pw[e])) s_over = 1;
+    __syncthreads();
+    if (!s_over) {
+        for (int s = tid; s < CAP; s += 256) {
+            int32_t y = s_key[s];
+            if (y >= 0) {
+                int pos = atomicAdd(&s_cnt, 1);
+                B.buy[beg + pos] = y;
+                B.buw[beg + pos] = (int64_t)s_val[s];
+            }
+        }
+    } else {
+        const int64_t cap = 2 * (end - beg);
+        int32_t *gk = B.fbk + 2 * beg;
+        ull *gv = B.fbv + 2 * beg;
+        for (int64_t s = tid; s < cap; s += 256) {
+            gk[s] = -1;
+            gv[s] = 0;
+        }
+        __threadfence();
+        __syncthreads();
+        for (int64_t e = beg + tid; e < end; e += 256) glb_insert_mod(gk, gv, cap, B.py[e], B.pw[e]);
+        __threadfence();
+        __syncthreads();
+        for (int64_t s = tid; s < cap; s += 256) {
+            int32_t y = __hip_atomic_load(&gk[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (y >= 0) {
+                int pos = atomicAdd(&s_cnt, 1);
+                B.buy[beg + pos] = y;
+                B.buw[beg + pos] =
+                    (int64_t)__hip_atomic_load(&gv[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+    }
+    __syncthreads();
+    if (tid == 0) B.ucnt[pi] = s_cnt;
+}
+
+// per big segment: prefix of its partitions' unique counts and the total
+__global__ void k_cr_big_final(CoarsenArgs a, BigSegArgs B, int32_t nbs)
+{
+    GRID_STRIDE(j, nbs)
+    {
+        const int64_t base = B.pbase[j];
+        int64_t run = 0;
+        for (int32_t p = 0; p < B.bP[j]; ++p) {
+            B.pre[base + p] = run;
+            run += B.ucnt[base + p];
+        }
+        a.nuniq[B.bsegs[j]] = (int32_t)run;
+    }
+}
+
+void launch_cr_big_setup(hipStream_t s, const int32_t *bsegs, int32_t nbs, const int64_t *slen,
+                         int32_t target, int32_t *bP, int64_t *bP64, int64_t *bch)
+{
+    if (nbs > 0)
+        hipLaunchKernelGGL(k_cr_big_setup, grid_for(nbs, 256), 256, 0, s, bsegs, nbs, slen, target,
+                           bP, bP64, bch);
+}
+
+void launch_cr_big_maps(hipStream_t s, int32_t nbs, const int32_t *bP, const int64_t *pbase,
+                        const int64_t *bch, const int64_t *cbase, int32_t *part_big,
+                        int32_t *chunk_big, int32_t *chunk_k)
+{
+    if (nbs > 0)
+        hipLaunchKernelGGL(k_cr_big_maps, grid_for(nbs, 64), 64, 0, s, nbs, bP, pbase, bch, cbase,
+                           part_big, chunk_big, chunk_k);
+}
+
+int run_cr_big(Arena &A, hipStream_t s, const CoarsenArgs &a, const BigSegArgs &B, int32_t nbs,
+               int64_t nparts, int64_t nchunks)
+{
+    if (nbs <= 0) return LV_OK;
+    fill_i64(s, B.pcount, nparts + 1, 0);
+    hipLaunchKernelGGL(k_cr_big_count, (int)nchunks, 256, 0, s, a, B);
+    LV_TRY(scan_exclusive_i64(A, s, B.pcount, B.poff, nparts, B.poff + nparts));
+    LV_HIP(hipMemcpyAsync(B.pcur, B.poff, sizeof(int64_t) * (size_t)nparts,
+                          hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(k_cr_big_scatter, (int)nchunks, 256, 0, s, a, B);
+    hipLaunchKernelGGL(k_cr_big_agg, (int)nparts, 256, 0, s, B);
+    hipLaunchKernelGGL(k_cr_big_final, grid_for(nbs, 256), 256, 0, s, a, B, nbs);
+    LV_HIP(hipGetLastError());
+    return LV_OK;
+}
+
+// ---------------------------------------------------------------------------
+// next level: row counts, row copies, per-vertex k / loop / selfsum
+
+__global__ void k_cr_rowcnt(const int32_t *newid_g, const int32_t *nuniq, int32_t K, int64_t *rowcnt)
+{
+    GRID_STRIDE(g, K) rowcnt[newid_g[g]] = nuniq[g];
+}
+
+// 16 lanes per run.
+__global__ void k_cr_copy_small(CoarsenArgs a, const int64_t *rowptr2, int32_t K, int32_t *col2,
+                                int64_t *cw2)
+{
+    const int lg = threadIdx.x & 15;
+    for (int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4; g < K;
+         g += ((int64_t)gridDim.x * blockDim.x) >> 4) {
+        if (a.slen[g] > kTierMaxQ[kTiers - 2]) continue;  // big segments copy per partition
+        const int64_t src = a.sb[g], dst = rowptr2[a.newid_g[g]];
+        const int32_t len = a.nuniq[g];
+        for (int32_t t = lg; t < len; t += 16) {
+            col2[dst + t] = a.uy[src + t];
+            cw2[dst + t] = a.uw[src + t];
+        }
+    }
+}
+
+__global__ void k_cr_copy_big(CoarsenArgs a, BigSegArgs B, const int64_t *rowptr2, int64_t nparts,
+                              int32_t *col2, int64_t *cw2)
+{
+    const int lg = threadIdx.x & 15;
+    for (int64_t p = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4; p < nparts;
+         p += ((int64_t)gridDim.x * blockDim.x) >> 4) {
+        const int32_t g = B.bsegs[B.part_big[p]];
+        const int64_t src = B.poff[p], dst = rowptr2[a.newid_g[g]] + B.pre[p];
+        const int32_t len = B.ucnt[p];
+        for (int32_t t = lg; t < len; t += 16) {
+            col2[dst + t] = B.buy[src + t];
+            cw2[dst + t] = B.buw[src + t];
+        }
+    }
+}
+
+__global__ void k_cr_vertex(const int32_t *newid_g, const ull *ssum, const ull *internal,
+                            const ull *selfl, int32_t K, int64_t *loop2, int64_t *self2,
+                            int64_t *kdeg2, long long *msum)
+{
+    long long acc = 0;
+    GRID_STRIDE(g, K)
+    {
+        const int32_t x = newid_g[g];
+        const int64_t lp = (int64_t)((internal[g] + 2 * selfl[g]) / 2);
+        loop2[x] = lp;
+        self2[x] = lp;
+        kdeg2[x] = (int64_t)(ssum[g] - internal[g]) + 4 * lp;
+        acc += (long long)(ssum[g] + 2 * selfl[g]);
+    }
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0 && acc) atomicAdd((ull *)msum, (ull)acc);
+}
+
+void launch_cr_rowcnt(hipStream_t s, const int32_t *newid_g, const int32_t *nuniq, int32_t K,
+                      int64_t *rowcnt)
+{
+    if (K > 0) hipLaunchKernelGGL(k_cr_rowcnt, grid_for(K, 256), 256, 0, s, newid_g, nuniq, K, rowcnt);
+}
+
+void launch_cr_copy(hipStream_t s, const CoarsenArgs &a, const BigSegArgs &B, int32_t K,
+                    int64_t nparts, const int64_t *rowptr2, int32_t *col2, int64_t *cw2)
+{
+    if (K > 0)
+        hipLaunchKernelGGL(k_cr_copy_small, grid_for((int64_t)K * 16, 256, 1 << 18), 256, 0, s, a,
+                           rowptr2, K, col2, cw2);
+    if (nparts > 0)
+        hipLaunchKernelGGL(k_cr_copy_big, grid_for(nparts * 16, 256, 1 << 18), 256, 0, s, a, B,
+                           rowptr2, nparts, col2, cw2);
+}
+
+void launch_cr_vertex(hipStream_t s, const int32_t *newid_g, const ull *ssum, const ull *internal,
+                      const ull *selfl, int32_t K, int64_t *loop2, int64_t *self2, int64_t *kdeg2,
+                      long long *msum)
+{
+    if (K > 0)
+        hipLaunchKernelGGL(k_cr_vertex, grid_for(K, 256, 4096), 256, 0, s, newid_g, ssum, internal,
+                           selfl, K, loop2, self2, kdeg2, msum);
+}
+
+}  // namespace lv

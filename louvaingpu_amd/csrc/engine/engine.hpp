@@ -119,6 +119,7 @@ struct SweepArgs {
 constexpr int kHubChunk = 4096;
 constexpr int kHubPartTarget = 1024;  // default; LV_HUB_PART_TARGET overrides
 constexpr int kHubMaxParts = 2048;
+constexpr int kHubAggLog2 = 12;  // 4096 LDS slots per partition block
 
 struct HubArgs {
     const int32_t *hv;        // hub vertex of hub j
@@ -141,6 +142,40 @@ struct HubArgs {
     int32_t *pc;
     int64_t *pw;
     int32_t *pn;
+};
+
+// Coarsening: community segments of the member-order entry list.
+struct CoarsenArgs {
+    const int32_t *ey;      // newc[target] of every entry, member order
+    const int64_t *ew;      // w' of every entry
+    const int64_t *sb;      // first entry of group g
+    const int64_t *slen;    // entries of group g
+    const int32_t *newid_g; // new id of group g
+    int32_t *uy;            // uniques of segments up to 2048 entries, at sb[g]
+    int64_t *uw;
+    int32_t *nuniq;         // uniques per group
+    ull *internal;          // sum of w' over entries inside the group
+};
+
+// Segments of more than 2048 entries, partitioned like the hub rows.
+struct BigSegArgs {
+    const int32_t *bsegs;   // group of big segment j
+    const int32_t *bP;      // partitions of big segment j
+    const int64_t *pbase;   // first partition of j
+    const int32_t *part_big;
+    const int32_t *chunk_big;
+    const int32_t *chunk_k;
+    int64_t *pcount;
+    int64_t *poff;
+    int64_t *pcur;
+    int32_t *py;            // partitioned entries
+    int64_t *pw;
+    int32_t *buy;           // uniques per partition, at poff[p]
+    int64_t *buw;
+    int32_t *fbk;           // fallback tables
+    ull *fbv;
+    int32_t *ucnt;          // uniques per partition
+    int64_t *pre;           // prefix of ucnt within the segment
 };
 
 // ---------------------------------------------------------------------------
@@ -219,6 +254,41 @@ void launch_hub_maps(hipStream_t s, int32_t nh, const int32_t *hP, const int64_t
                      int32_t *chunk_hub, int32_t *chunk_k);
 int run_hub_sweep(Arena &A, hipStream_t s, const SweepArgs &a, const HubArgs &h, int32_t nh,
                   int64_t nparts, int64_t nchunks);
+
+// coarsen.hip
+void launch_cr_heads(hipStream_t s, const uint32_t *Cs, int32_t n, int32_t *head);
+void launch_cr_groups(hipStream_t s, const int32_t *head, const int32_t *hscan, const int32_t *perm,
+                      int32_t n, int32_t *gstart, int32_t *lflag);
+void launch_cr_newid(hipStream_t s, const int32_t *gstart, const int32_t *perm, const int32_t *rank,
+                     int32_t K, int32_t *newid_g);
+void launch_cr_newc(hipStream_t s, const int32_t *head, const int32_t *hscan, const int32_t *perm,
+                    const int32_t *newid_g, int32_t n, int32_t *newc);
+void launch_cr_memdeg(hipStream_t s, const int32_t *perm, const int64_t *rowptr, int32_t n,
+                      int64_t *mdeg);
+void launch_cr_selfl(hipStream_t s, const int32_t *perm, const int64_t *selfsum,
+                     const int32_t *head, const int32_t *hscan, int32_t n, ull *selfl);
+void launch_cr_gather(hipStream_t s, const int64_t *estart, int32_t *bound, const int32_t *perm,
+                      const int64_t *rowptr, const int32_t *col, const int64_t *cw,
+                      const int32_t *newc, const int32_t *head, const int32_t *hscan, int32_t n,
+                      int64_t total, int32_t *ey, int64_t *ew, ull *ssum);
+void launch_cr_seg_tier(hipStream_t s, const int32_t *gstart, const int64_t *estart, int32_t K,
+                        int64_t *sb, int64_t *slen, uint32_t *tkey, int32_t *sid,
+                        long long *tier_cnt);
+void launch_cr_agg_tier(hipStream_t s, int tier, const CoarsenArgs &a, const int32_t *segs, int32_t ns);
+void launch_cr_big_setup(hipStream_t s, const int32_t *bsegs, int32_t nbs, const int64_t *slen,
+                         int32_t target, int32_t *bP, int64_t *bP64, int64_t *bch);
+void launch_cr_big_maps(hipStream_t s, int32_t nbs, const int32_t *bP, const int64_t *pbase,
+                        const int64_t *bch, const int64_t *cbase, int32_t *part_big,
+                        int32_t *chunk_big, int32_t *chunk_k);
+int run_cr_big(Arena &A, hipStream_t s, const CoarsenArgs &a, const BigSegArgs &B, int32_t nbs,
+               int64_t nparts, int64_t nchunks);
+void launch_cr_rowcnt(hipStream_t s, const int32_t *newid_g, const int32_t *nuniq, int32_t K,
+                      int64_t *rowcnt);
+void launch_cr_copy(hipStream_t s, const CoarsenArgs &a, const BigSegArgs &B, int32_t K,
+                    int64_t nparts, const int64_t *rowptr2, int32_t *col2, int64_t *cw2);
+void launch_cr_vertex(hipStream_t s, const int32_t *newid_g, const ull *ssum, const ull *internal,
+                      const ull *selfl, int32_t K, int64_t *loop2, int64_t *self2, int64_t *kdeg2,
+                      long long *msum);
 
 void launch_leader(hipStream_t s, const int32_t *C, int32_t n, int32_t *leader);
 void launch_leader_flag(hipStream_t s, const int32_t *C, const int32_t *leader, int32_t n,

@@ -5,74 +5,9 @@
 // multiply and subtract of W - ((gamma*deg)*f) separate).
 #include <climits>
 
-#include "engine.hpp"
+#include "device.hpp"
 
 namespace lv {
-
-static inline int grid_for(int64_t n, int block, int64_t cap = 1 << 16)
-{
-    int64_t g = (n + block - 1) / block;
-    if (g < 1) g = 1;
-    if (g > cap) g = cap;
-    return (int)g;
-}
-
-#define GRID_STRIDE(i, n)                                                          \
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < (n); \
-         i += (int64_t)gridDim.x * blockDim.x)
-
-__device__ __forceinline__ long long wave_sum(long long v)
-{
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
-__device__ __forceinline__ int wave_max_i(int v)
-{
-    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
-    return v;
-}
-
-// 128-bit accumulate of deg^2 as (lo, hi).
-__device__ __forceinline__ void acc_sq(ull &lo, ull &hi, ull d)
-{
-    ull plo = d * d, phi = __umul64hi(d, d);
-    lo += plo;
-    hi += phi + (lo < plo ? 1ull : 0ull);
-}
-
-__device__ __forceinline__ void wave_sum_u128(ull &lo, ull &hi)
-{
-    for (int off = 32; off > 0; off >>= 1) {
-        ull olo = __shfl_xor(lo, off, 64), ohi = __shfl_xor(hi, off, 64);
-        lo += olo;
-        hi += ohi + (lo < olo ? 1ull : 0ull);
-    }
-}
-
-__device__ __forceinline__ void atomic_add_u128(ull *lo_p, ull *hi_p, ull lo, ull hi)
-{
-    ull old = atomicAdd(lo_p, lo);
-    ull carry = (old + lo < old) ? 1ull : 0ull;
-    if (hi + carry) atomicAdd(hi_p, hi + carry);
-}
-
-// Adds val into base[key] for keys sorted within the wave: lanes holding the
-// same key (contiguous runs) are combined first, one atomic per run.
-__device__ __forceinline__ void seg_atomic_add(bool valid, long long key, ull val, ull *base)
-{
-    const int lane = threadIdx.x & 63;
-    long long k = valid ? key : LLONG_MIN;
-    ull v = valid ? val : 0ull;
-    for (int off = 1; off < 64; off <<= 1) {
-        ull ov = __shfl_up(v, off, 64);
-        long long ok = __shfl_up(k, off, 64);
-        if (lane >= off && ok == k) v += ov;
-    }
-    long long nk = __shfl_down(k, 1, 64);
-    bool tail = (lane == 63) || (nk != k);
-    if (valid && tail && v != 0) atomicAdd(&base[k], v);
-}
 
 // ---------------------------------------------------------------------------
 // Input validation and weight units (the reference reads float weights and
@@ -383,15 +318,6 @@ void launch_loop_gather(hipStream_t s, const ull *lastloop, const int64_t *w, in
 // Per-level init (initCUDA, LouvainCUDA/InitKernel.cu:79-126; communityInitKernel,
 // OneLevelKernel.cu:80-91) and the degree buckets of the sweep.
 
-__device__ __forceinline__ int tier_of(int64_t d, int64_t q)
-{
-    if (d == 0) return 0;  // no neighbours: the vertex can never move
-#pragma unroll
-    for (int t = 0; t < kTiers - 1; ++t)
-        if (q <= kTierMaxQ[t]) return t + 1;
-    return kTiers;
-}
-
 __global__ __launch_bounds__(256) void k_init_vertex(
     int32_t n, const int64_t *rowptr, const int64_t *kdeg, const int64_t *loopw,
     const int64_t *selfsum, int64_t *deg, int64_t *in, int32_t *size, ull *ddeg, ull *din,
@@ -546,33 +472,6 @@ void launch_apply(hipStream_t s, int32_t n, int64_t *deg, int64_t *in, int32_t *
 // the maximal g when that maximum is > 0, otherwise the own community - the
 // outcome of the reference's ascending scan with its replace rule.
 
-__device__ __forceinline__ uint32_t hslot(int32_t c, int log2cap)
-{
-    return ((uint32_t)c * 0x9E3779B1u) >> (32 - log2cap);
-}
-
-__device__ __forceinline__ void lds_insert(int32_t *key, ull *val, int log2cap, int32_t c,
-                                           int64_t w)
-{
-    const uint32_t mask = (1u << log2cap) - 1u;
-    uint32_t s = hslot(c, log2cap);
-    for (;;) {
-        int32_t k = ((volatile int32_t *)key)[s];
-        if (k == c) break;
-        if (k == -1) {
-            int32_t prev = atomicCAS(&key[s], -1, c);
-            if (prev == -1 || prev == c) break;
-        }
-        s = (s + 1) & mask;
-    }
-    if (w != 0) atomicAdd(&val[s], (ull)w);
-}
-
-__device__ __forceinline__ bool better(double g1, int32_t c1, double g2, int32_t c2)
-{
-    return g1 > g2 || (g1 == g2 && c1 < c2);
-}
-
 __device__ __forceinline__ void consider(int32_t c, int64_t W, const SweepArgs &a, double f,
                                          int &cnt, double &bg, int32_t &bc, int64_t &bw)
 {
@@ -640,15 +539,6 @@ __device__ __forceinline__ void finalize_vertex(const SweepArgs &a, int32_t v, i
         }
         a.next[v] = nv;
     }
-}
-
-// Table size for a row of q inserts: the smallest power of two >= 2q, at least
-// 2^lo (never above the kernel's LDS capacity, which the tier bound ensures).
-__device__ __forceinline__ int table_log2(int64_t q, int lo)
-{
-    int l = lo;
-    while ((int64_t(1) << l) < 2 * q) ++l;
-    return l;
 }
 
 // G lanes per vertex, 256/G vertices per block, an LDS table of up to
@@ -798,23 +688,6 @@ void launch_sweep_tier(hipStream_t s, int tier, const SweepArgs &a, const int32_
 // Hub tier (q > 2048): partition + LDS aggregation instead of one global hash
 // table per hub, so the work is streaming reads/writes plus LDS atomics.
 
-__device__ __forceinline__ uint32_t fmix32(uint32_t h)
-{
-    h ^= h >> 16;
-    h *= 0x85ebca6bu;
-    h ^= h >> 13;
-    h *= 0xc2b2ae35u;
-    h ^= h >> 16;
-    return h;
-}
-
-// Partition of community c among P; independent of hslot's multiplicative hash
-// so the keys of one partition spread over its table.
-__device__ __forceinline__ int hub_part(int32_t c, int P)
-{
-    return (int)(((uint64_t)fmix32((uint32_t)c) * (uint64_t)P) >> 32);
-}
-
 __global__ void k_hub_setup(const int32_t *hv, int32_t nh, const int64_t *rowptr,
                             const int64_t *selfsum, int32_t target, int64_t *hq, int32_t *hP,
                             int64_t *hP64, int64_t *hch)
@@ -923,47 +796,6 @@ __global__ __launch_bounds__(256) void k_hub_scatter(SweepArgs a, HubArgs h)
     }
 }
 
-// Insert with at most 2^log2cap probes; false when the table is full.
-__device__ __forceinline__ bool lds_insert_bounded(int32_t *key, ull *val, int log2cap, int32_t c,
-                                                   int64_t w)
-{
-    const uint32_t mask = (1u << log2cap) - 1u;
-    uint32_t s = hslot(c, log2cap);
-    for (uint32_t probe = 0; probe <= mask; ++probe) {
-        int32_t k = ((volatile int32_t *)key)[s];
-        if (k == -1) {
-            int32_t prev = atomicCAS(&key[s], -1, c);
-            if (prev == -1 || prev == c) {
-                if (w != 0) atomicAdd(&val[s], (ull)w);
-                return true;
-            }
-        } else if (k == c) {
-            if (w != 0) atomicAdd(&val[s], (ull)w);
-            return true;
-        }
-        s = (s + 1) & mask;
-    }
-    return false;
-}
-
-// Global open-addressing table of `cap` slots (cap > distinct keys).
-__device__ __forceinline__ void glb_insert_mod(int32_t *key, ull *val, int64_t cap, int32_t c,
-                                               int64_t w)
-{
-    int64_t s = (int64_t)(fmix32((uint32_t)c ^ 0x5bd1e995u) % (uint64_t)cap);
-    for (;;) {
-        int32_t k = __hip_atomic_load(&key[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (k == c) break;
-        if (k == -1) {
-            int32_t prev = atomicCAS(&key[s], -1, c);
-            if (prev == -1 || prev == c) break;
-        }
-        s = (s + 1 == cap) ? 0 : s + 1;
-    }
-    if (w != 0) atomicAdd(&val[s], (ull)w);
-}
-
-constexpr int kHubAggLog2 = 12;  // 4096 LDS slots per partition block
 
 __global__ __launch_bounds__(256) void k_hub_agg(SweepArgs a, HubArgs h)
 {
