@@ -158,6 +158,8 @@ def main():
         "sweep_gbs": sweep_gbs,
         "sweep_teps": sweep_teps,
         "tier_ms_per_step": [t / steps for t in tier_ms],
+        "arena_gb": st.arena_capacity / 1e9,
+        "arena_high_water_gb": st.arena_high_water / 1e9,
         "modularity": q_gpu,
         "communities": int(final.max()) + 1,
     }

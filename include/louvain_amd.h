@@ -87,6 +87,8 @@ typedef struct {
     double sweep_bytes;                    /* sum over tiers */
     int64_t sweep_entries;                 /* sum of nnz over passes */
     int32_t levels, passes;
+    uint64_t arena_capacity;               /* bytes of the HBM arena */
+    uint64_t arena_high_water;             /* most bytes allocated at once */
 } lv_stats;
 
 void lv_config_default(lv_config *cfg);
@@ -119,7 +121,8 @@ int lv_neighbour_compute(lv_ctx *ctx);
  * singleton communities, degree buckets for the sweep. */
 int lv_init(lv_ctx *ctx);
 /* modularityCUDA (LouvainCUDA/OneLevelKernel.cu:965-980): Q of the current
- * community state. */
+ * community state; valid between lv_init and lv_renumber (renumbering
+ * releases the level's community totals). */
 int lv_modularity(lv_ctx *ctx, double *q);
 /* oneLevelCUDA (LouvainCUDA/OneLevelKernel.cu:518-954): the sweep loop,
  * merge-isolated post-step; *q receives the level's returned modularity.
@@ -130,7 +133,9 @@ int lv_one_level(lv_ctx *ctx, double lower, double *q, int32_t *passes);
  * Partition arrays are fetched with lv_level_partition. */
 int lv_renumber(lv_ctx *ctx, int32_t *k);
 /* inducedGraphCUDA (LouvainCUDA/InducedGraphKernel.cu:50-113): coarsens the
- * level graph into the next level's CSR (radix sort + segmented reduce). */
+ * level graph into the next level's CSR: each community's entries are
+ * gathered in member order and aggregated by neighbour community (LDS hash
+ * tables; hash partitions for communities with more than 2048 entries). */
 int lv_induce(lv_ctx *ctx);
 
 /* generate_dendogram's level loop (LouvainCUDA/LouvainCPP.cpp:209-553) run

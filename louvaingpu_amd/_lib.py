@@ -87,6 +87,8 @@ class LvStats(ct.Structure):
         ("sweep_entries", ct.c_int64),
         ("levels", ct.c_int32),
         ("passes", ct.c_int32),
+        ("arena_capacity", ct.c_uint64),
+        ("arena_high_water", ct.c_uint64),
     ]
 
 

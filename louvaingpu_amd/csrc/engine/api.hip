@@ -192,15 +192,10 @@ int zero_small(lv_ctx *c, int slots)
     return LV_OK;
 }
 
-void free_level_arrays(lv_ctx *c)
+// Community state of the sweeps (not needed once the level is renumbered).
+void free_sweep_state(lv_ctx *c)
 {
     Arena &a = c->arena;
-    arena_free(a, c->rowptr);
-    arena_free(a, c->col);
-    arena_free(a, c->cw);
-    arena_free(a, c->kdeg);
-    arena_free(a, c->loopw);
-    arena_free(a, c->selfsum);
     arena_free(a, c->C);
     arena_free(a, c->next);
     arena_free(a, c->deg);
@@ -210,10 +205,6 @@ void free_level_arrays(lv_ctx *c)
     arena_free(a, c->ddeg);
     arena_free(a, c->din);
     arena_free(a, c->verts);
-    arena_free(a, c->g_head);
-    arena_free(a, c->g_hscan);
-    arena_free(a, c->g_start);
-    arena_free(a, c->g_newid);
     HubArgs &h = c->hub;
     a.release((void *)h.hq);
     a.release((void *)h.hP);
@@ -237,6 +228,22 @@ void free_level_arrays(lv_ctx *c)
     c->hub = HubArgs{};
     c->n_hubs = 0;
     c->hub_parts = c->hub_chunks = c->hub_entries = 0;
+}
+
+void free_level_arrays(lv_ctx *c)
+{
+    Arena &a = c->arena;
+    free_sweep_state(c);
+    arena_free(a, c->rowptr);
+    arena_free(a, c->col);
+    arena_free(a, c->cw);
+    arena_free(a, c->kdeg);
+    arena_free(a, c->loopw);
+    arena_free(a, c->selfsum);
+    arena_free(a, c->g_head);
+    arena_free(a, c->g_hscan);
+    arena_free(a, c->g_start);
+    arena_free(a, c->g_newid);
 }
 
 void free_dendrogram(lv_ctx *c)
@@ -583,6 +590,7 @@ int renumber(lv_ctx *c, int32_t *k_out)
     arena_free(A, k2);
     arena_free(A, lflag);
     arena_free(A, rank);
+    free_sweep_state(c);  // the level's communities are final
     L.n = n;
     L.k = kk;
     L.nnz = c->nnz;
@@ -900,7 +908,7 @@ int lv_load_edges(lv_ctx *c, int32_t n, const int32_t *ids, int64_t e, const int
     hipStream_t s = c->stream;
     // arena: sized for the largest level-0 phase unless the caller fixed it
     if (!c->cfg.arena_bytes) {
-        size_t want = (size_t)180 * (size_t)e + (size_t)256 * (size_t)n + ((size_t)192 << 20);
+        size_t want = (size_t)200 * (size_t)e + (size_t)320 * (size_t)n + ((size_t)256 << 20);
         size_t freeb = 0, totalb = 0;
         if (hipMemGetInfo(&freeb, &totalb) == hipSuccess) {
             size_t avail = (size_t)((double)(freeb + A.capacity()) * 0.92);
@@ -1281,6 +1289,8 @@ int lv_get_stats(lv_ctx *c, lv_stats *out)
     LV_TRY(set_device(c));
     LV_TRY(sync(c));
     *out = c->stats;
+    out->arena_capacity = (uint64_t)c->arena.capacity();
+    out->arena_high_water = (uint64_t)c->arena.high_water();
     return LV_OK;
 }
 
