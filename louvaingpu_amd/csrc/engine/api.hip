@@ -385,7 +385,8 @@ int plan_hubs(lv_ctx *c, int32_t nh)
     LV_TRY(arena_alloc(A, part_hub, c->hub_parts));
     LV_TRY(arena_alloc(A, chunk_hub, c->hub_chunks));
     LV_TRY(arena_alloc(A, chunk_k, c->hub_chunks));
-    launch_hub_maps(s, nh, hP, pbase, hch, cbase, part_hub, chunk_hub, chunk_k);
+    launch_expand_owner(s, pbase, nh, c->hub_parts, part_hub, nullptr);
+    launch_expand_owner(s, cbase, nh, c->hub_chunks, chunk_hub, chunk_k);
     h.part_hub = part_hub;
     h.chunk_hub = chunk_hub;
     h.chunk_k = chunk_k;
@@ -704,7 +705,8 @@ int induce(lv_ctx *c)
         LV_TRY(arena_alloc(A, part_big, nparts));
         LV_TRY(arena_alloc(A, chunk_big, nchunks));
         LV_TRY(arena_alloc(A, chunk_k, nchunks));
-        launch_cr_big_maps(s, nbs, bP, pbase, bch, cbase, part_big, chunk_big, chunk_k);
+        launch_expand_owner(s, pbase, nbs, nparts, part_big, nullptr);
+        launch_expand_owner(s, cbase, nbs, nchunks, chunk_big, chunk_k);
         B.bsegs = bsegs;
         B.bP = bP;
         B.pbase = pbase;

@@ -129,9 +129,11 @@ __global__ void k_lbs_bounds(const int64_t *estart, int32_t n, int64_t total, in
 }
 
 constexpr int kGatherT = 1024;  // output entries per block (4 per thread)
+constexpr int kGatherLds = 2048; // member offsets a block can stage in LDS
 
 // ey/ew[o] for every output slot o of member order: the o-th entry of the
-// member rows, as (newc[target], w'); ssum[g] accumulates w' per group.
+// member rows, as (newc[target], w'); ssum[g] accumulates w' per group.  The
+// block's member offsets estart[lo0..hi0+1] are staged in LDS when they fit.
 __global__ __launch_bounds__(256) void k_cr_gather(const int64_t *estart, const int32_t *bound,
                                                   const int32_t *perm, const int64_t *rowptr,
                                                   const int32_t *col, const int64_t *cw,
@@ -139,8 +141,15 @@ __global__ __launch_bounds__(256) void k_cr_gather(const int64_t *estart, const 
                                                   const int32_t *hscan, int64_t total,
                                                   int32_t *ey, int64_t *ew, ull *ssum)
 {
+    __shared__ int64_t s_est[kGatherLds];
     const int64_t b = blockIdx.x;
     const int32_t lo0 = bound[b], hi0 = bound[b + 1];
+    const int32_t span = hi0 - lo0 + 2;  // estart[lo0 .. hi0+1]
+    const bool staged = span <= kGatherLds;
+    if (staged) {
+        for (int i = threadIdx.x; i < span; i += 256) s_est[i] = estart[lo0 + i];
+    }
+    __syncthreads();
 #pragma unroll
     for (int r = 0; r < kGatherT / 256; ++r) {
         const int64_t o = b * kGatherT + r * 256 + threadIdx.x;
@@ -148,13 +157,25 @@ __global__ __launch_bounds__(256) void k_cr_gather(const int64_t *estart, const 
         long long g = 0;
         ull wv = 0;
         if (valid) {
-            int32_t lo = lo0, hi = hi0 + 1;  // last j in [lo0, hi0] with estart[j] <= o
-            while (hi - lo > 1) {
-                int32_t mid = (lo + hi) >> 1;
-                if (estart[mid] <= o) lo = mid; else hi = mid;
+            int32_t lo;
+            if (staged) {
+                int32_t l = 0, h = span - 1;  // s_est[l] <= o < s_est[h]
+                while (h - l > 1) {
+                    int32_t mid = (l + h) >> 1;
+                    if (s_est[mid] <= o) l = mid; else h = mid;
+                }
+                lo = lo0 + l;
+            } else {
+                int32_t l = lo0, h = hi0 + 1;
+                while (h - l > 1) {
+                    int32_t mid = (l + h) >> 1;
+                    if (estart[mid] <= o) l = mid; else h = mid;
+                }
+                lo = l;
             }
+            const int64_t first = staged ? s_est[lo - lo0] : estart[lo];
             const int32_t a = perm[lo];
-            const int64_t p = rowptr[a] + (o - estart[lo]);
+            const int64_t p = rowptr[a] + (o - first);
             const int64_t u = cw[p];
             ey[o] = newc[col[p]];
             wv = (ull)(u == 0 ? 1 : u);
@@ -388,20 +409,6 @@ __global__ void k_cr_big_setup(const int32_t *bsegs, int32_t nbs, const int64_t 
     }
 }
 
-__global__ void k_cr_big_maps(int32_t nbs, const int32_t *bP, const int64_t *pbase,
-                              const int64_t *bch, const int64_t *cbase, int32_t *part_big,
-                              int32_t *chunk_big, int32_t *chunk_k)
-{
-    GRID_STRIDE(j, nbs)
-    {
-        for (int32_t p = 0; p < bP[j]; ++p) part_big[pbase[j] + p] = (int32_t)j;
-        for (int64_t k = 0; k < bch[j]; ++k) {
-            chunk_big[cbase[j] + k] = (int32_t)j;
-            chunk_k[cbase[j] + k] = (int32_t)k;
-        }
-    }
-}
-
 __global__ __launch_bounds__(256) void k_cr_big_count(CoarsenArgs a, BigSegArgs B)
 {
     __shared__ int hist[kHubMaxParts];
@@ -527,19 +534,13 @@ __global__ __launch_bounds__(256) void k_cr_big_agg(BigSegArgs B)
     if (tid == 0) B.ucnt[pi] = s_cnt;
 }
 
-// per big segment: prefix of its partitions' unique counts and the total
-__global__ void k_cr_big_final(CoarsenArgs a, BigSegArgs B, int32_t nbs)
+// ucscan = exclusive scan of ucnt over all partitions (segment-major):
+// pre[p] = its offset within its segment, nuniq = the segment's total.
+__global__ void k_cr_big_prefix(CoarsenArgs a, BigSegArgs B, const int64_t *ucscan, int64_t nparts,
+                                int32_t nbs)
 {
-    GRID_STRIDE(j, nbs)
-    {
-        const int64_t base = B.pbase[j];
-        int64_t run = 0;
-        for (int32_t p = 0; p < B.bP[j]; ++p) {
-            B.pre[base + p] = run;
-            run += B.ucnt[base + p];
-        }
-        a.nuniq[B.bsegs[j]] = (int32_t)run;
-    }
+    GRID_STRIDE(p, nparts) B.pre[p] = ucscan[p] - ucscan[B.pbase[B.part_big[p]]];
+    GRID_STRIDE(j, nbs) a.nuniq[B.bsegs[j]] = (int32_t)(ucscan[B.pbase[j + 1]] - ucscan[B.pbase[j]]);
 }
 
 void launch_cr_big_setup(hipStream_t s, const int32_t *bsegs, int32_t nbs, const int64_t *slen,
@@ -548,15 +549,6 @@ void launch_cr_big_setup(hipStream_t s, const int32_t *bsegs, int32_t nbs, const
     if (nbs > 0)
         hipLaunchKernelGGL(k_cr_big_setup, grid_for(nbs, 256), 256, 0, s, bsegs, nbs, slen, target,
                            bP, bP64, bch);
-}
-
-void launch_cr_big_maps(hipStream_t s, int32_t nbs, const int32_t *bP, const int64_t *pbase,
-                        const int64_t *bch, const int64_t *cbase, int32_t *part_big,
-                        int32_t *chunk_big, int32_t *chunk_k)
-{
-    if (nbs > 0)
-        hipLaunchKernelGGL(k_cr_big_maps, grid_for(nbs, 64), 64, 0, s, nbs, bP, pbase, bch, cbase,
-                           part_big, chunk_big, chunk_k);
 }
 
 int run_cr_big(Arena &A, hipStream_t s, const CoarsenArgs &a, const BigSegArgs &B, int32_t nbs,
@@ -570,8 +562,16 @@ int run_cr_big(Arena &A, hipStream_t s, const CoarsenArgs &a, const BigSegArgs &
                           hipMemcpyDeviceToDevice, s));
     hipLaunchKernelGGL(k_cr_big_scatter, (int)nchunks, 256, 0, s, a, B);
     hipLaunchKernelGGL(k_cr_big_agg, (int)nparts, 256, 0, s, B);
-    hipLaunchKernelGGL(k_cr_big_final, grid_for(nbs, 256), 256, 0, s, a, B, nbs);
+    int64_t *ucscan = nullptr, *uc64 = nullptr;
+    LV_TRY(arena_alloc(A, uc64, nparts));
+    LV_TRY(arena_alloc(A, ucscan, nparts + 1));
+    launch_widen_i32(s, B.ucnt, nparts, uc64);
+    LV_TRY(scan_exclusive_i64(A, s, uc64, ucscan, nparts, ucscan + nparts));
+    hipLaunchKernelGGL(k_cr_big_prefix, grid_for(nparts > nbs ? nparts : nbs, 256), 256, 0, s, a,
+                       B, ucscan, nparts, nbs);
     LV_HIP(hipGetLastError());
+    arena_free(A, uc64);
+    arena_free(A, ucscan);
     return LV_OK;
 }
 

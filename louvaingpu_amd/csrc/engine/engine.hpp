@@ -249,9 +249,11 @@ void launch_sweep_tier(hipStream_t s, int tier, const SweepArgs &a, const int32_
 void launch_hub_setup(hipStream_t s, const int32_t *hv, int32_t nh, const int64_t *rowptr,
                       const int64_t *selfsum, int32_t target, int64_t *hq, int32_t *hP,
                       int64_t *hP64, int64_t *hch);
-void launch_hub_maps(hipStream_t s, int32_t nh, const int32_t *hP, const int64_t *pbase,
-                     const int64_t *hch, const int64_t *cbase, int32_t *part_hub,
-                     int32_t *chunk_hub, int32_t *chunk_k);
+// owner[i] = segment j of i under exclusive offsets base[0..nseg]; rank[i] =
+// i - base[j] (rank may be null).
+void launch_expand_owner(hipStream_t s, const int64_t *base, int32_t nseg, int64_t total,
+                         int32_t *owner, int32_t *rank);
+void launch_widen_i32(hipStream_t s, const int32_t *in, int64_t n, int64_t *out);
 int run_hub_sweep(Arena &A, hipStream_t s, const SweepArgs &a, const HubArgs &h, int32_t nh,
                   int64_t nparts, int64_t nchunks);
 
@@ -277,9 +279,6 @@ void launch_cr_seg_tier(hipStream_t s, const int32_t *gstart, const int64_t *est
 void launch_cr_agg_tier(hipStream_t s, int tier, const CoarsenArgs &a, const int32_t *segs, int32_t ns);
 void launch_cr_big_setup(hipStream_t s, const int32_t *bsegs, int32_t nbs, const int64_t *slen,
                          int32_t target, int32_t *bP, int64_t *bP64, int64_t *bch);
-void launch_cr_big_maps(hipStream_t s, int32_t nbs, const int32_t *bP, const int64_t *pbase,
-                        const int64_t *bch, const int64_t *cbase, int32_t *part_big,
-                        int32_t *chunk_big, int32_t *chunk_k);
 int run_cr_big(Arena &A, hipStream_t s, const CoarsenArgs &a, const BigSegArgs &B, int32_t nbs,
                int64_t nparts, int64_t nchunks);
 void launch_cr_rowcnt(hipStream_t s, const int32_t *newid_g, const int32_t *nuniq, int32_t K,
@@ -289,26 +288,6 @@ void launch_cr_copy(hipStream_t s, const CoarsenArgs &a, const BigSegArgs &B, in
 void launch_cr_vertex(hipStream_t s, const int32_t *newid_g, const ull *ssum, const ull *internal,
                       const ull *selfl, int32_t K, int64_t *loop2, int64_t *self2, int64_t *kdeg2,
                       long long *msum);
-
-void launch_leader(hipStream_t s, const int32_t *C, int32_t n, int32_t *leader);
-void launch_leader_flag(hipStream_t s, const int32_t *C, const int32_t *leader, int32_t n,
-                        int32_t *flag);
-void launch_newc(hipStream_t s, const int32_t *C, const int32_t *leader, const int32_t *rank,
-                 int32_t n, int32_t *newc);
-
-void launch_coarsen_keys(hipStream_t s, const int64_t *rowptr, const int32_t *col,
-                         const int64_t *w, int32_t n, int64_t nnz, const int64_t *selfsum,
-                         const int32_t *newc, uint64_t K, uint64_t *keys, int64_t *vals);
-void launch_seg_heads(hipStream_t s, const uint64_t *keys, int64_t n, int64_t *head);
-void launch_seg_sum(hipStream_t s, const uint64_t *keys, const int64_t *vals, const int64_t *hscan,
-                    const int64_t *head, int64_t n, uint64_t *ukey, ull *usum);
-void launch_coarse_split(hipStream_t s, const uint64_t *ukey, const ull *usum, int64_t u,
-                         uint64_t K, int64_t *nsflag, int64_t *loopw, int64_t *selfsum,
-                         int64_t *rowcnt, int64_t *kdeg);
-void launch_coarse_fill(hipStream_t s, const uint64_t *ukey, const ull *usum, const int64_t *nsflag,
-                        const int64_t *pos, int64_t u, uint64_t K, int32_t *col, int64_t *cw,
-                        int64_t *kdeg);
-void launch_sum_ull(hipStream_t s, const ull *p, int64_t n, long long *out);
 
 void launch_compose(hipStream_t s, int32_t *f, int32_t n, const int32_t *map);
 void launch_gather_partition(hipStream_t s, const int32_t *perm, const int32_t *ids,

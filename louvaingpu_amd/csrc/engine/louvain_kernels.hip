@@ -706,18 +706,38 @@ __global__ void k_hub_setup(const int32_t *hv, int32_t nh, const int64_t *rowptr
     }
 }
 
-__global__ void k_hub_maps(int32_t nh, const int32_t *hP, const int64_t *pbase, const int64_t *hch,
-                           const int64_t *cbase, int32_t *part_hub, int32_t *chunk_hub,
-                           int32_t *chunk_k)
+// owner[i] = the j with base[j] <= i < base[j+1]; rank[i] = i - base[j].
+__global__ void k_expand_owner(const int64_t *base, int32_t nseg, int64_t total, int32_t *owner,
+                               int32_t *rank)
 {
-    GRID_STRIDE(j, nh)
+    GRID_STRIDE(i, total)
     {
-        for (int32_t p = 0; p < hP[j]; ++p) part_hub[pbase[j] + p] = (int32_t)j;
-        for (int64_t k = 0; k < hch[j]; ++k) {
-            chunk_hub[cbase[j] + k] = (int32_t)j;
-            chunk_k[cbase[j] + k] = (int32_t)k;
+        int32_t lo = 0, hi = nseg;  // base[lo] <= i < base[hi]
+        while (hi - lo > 1) {
+            int32_t mid = (lo + hi) >> 1;
+            if (base[mid] <= i) lo = mid; else hi = mid;
         }
+        owner[i] = lo;
+        if (rank) rank[i] = (int32_t)(i - base[lo]);
     }
+}
+
+__global__ void k_widen_i32(const int32_t *in, int64_t n, int64_t *out)
+{
+    GRID_STRIDE(i, n) out[i] = in[i];
+}
+
+void launch_expand_owner(hipStream_t s, const int64_t *base, int32_t nseg, int64_t total,
+                         int32_t *owner, int32_t *rank)
+{
+    if (total > 0)
+        hipLaunchKernelGGL(k_expand_owner, grid_for(total, 256, 1 << 20), 256, 0, s, base, nseg,
+                           total, owner, rank);
+}
+
+void launch_widen_i32(hipStream_t s, const int32_t *in, int64_t n, int64_t *out)
+{
+    if (n > 0) hipLaunchKernelGGL(k_widen_i32, grid_for(n, 256), 256, 0, s, in, n, out);
 }
 
 void launch_hub_setup(hipStream_t s, const int32_t *hv, int32_t nh, const int64_t *rowptr,
@@ -727,15 +747,6 @@ void launch_hub_setup(hipStream_t s, const int32_t *hv, int32_t nh, const int64_
     if (nh > 0)
         hipLaunchKernelGGL(k_hub_setup, grid_for(nh, 256), 256, 0, s, hv, nh, rowptr, selfsum,
                            target, hq, hP, hP64, hch);
-}
-
-void launch_hub_maps(hipStream_t s, int32_t nh, const int32_t *hP, const int64_t *pbase,
-                     const int64_t *hch, const int64_t *cbase, int32_t *part_hub,
-                     int32_t *chunk_hub, int32_t *chunk_k)
-{
-    if (nh > 0)
-        hipLaunchKernelGGL(k_hub_maps, grid_for(nh, 64), 64, 0, s, nh, hP, pbase, hch, cbase,
-                           part_hub, chunk_hub, chunk_k);
 }
 
 // Entry t of hub j: t < d is row entry rowptr[v]+t, t == d the virtual
@@ -921,229 +932,6 @@ int run_hub_sweep(Arena &A, hipStream_t s, const SweepArgs &a, const HubArgs &h,
     hipLaunchKernelGGL(k_hub_final, (int)((nh + 3) / 4), 256, 0, s, a, h, nh);
     LV_HIP(hipGetLastError());
     return LV_OK;
-}
-
-// ---------------------------------------------------------------------------
-// Renumbering (renumberCUDA, LouvainCUDA/RenumberKernel.cu:63-220): the new id of
-// a community is the rank of its smallest member position.
-
-__global__ void k_leader(const int32_t *C, int32_t n, int32_t *leader)
-{
-    GRID_STRIDE(i, n) atomicMin(&leader[C[i]], (int32_t)i);
-}
-
-__global__ void k_leader_flag(const int32_t *C, const int32_t *leader, int32_t n, int32_t *flag)
-{
-    GRID_STRIDE(i, n) flag[i] = (leader[C[i]] == (int32_t)i) ? 1 : 0;
-}
-
-__global__ void k_newc(const int32_t *C, const int32_t *leader, const int32_t *rank, int32_t n,
-                       int32_t *newc)
-{
-    GRID_STRIDE(i, n) newc[i] = rank[leader[C[i]]];
-}
-
-void launch_leader(hipStream_t s, const int32_t *C, int32_t n, int32_t *leader)
-{
-    if (n > 0) hipLaunchKernelGGL(k_leader, grid_for(n, 256), 256, 0, s, C, n, leader);
-}
-
-void launch_leader_flag(hipStream_t s, const int32_t *C, const int32_t *leader, int32_t n,
-                        int32_t *flag)
-{
-    if (n > 0) hipLaunchKernelGGL(k_leader_flag, grid_for(n, 256), 256, 0, s, C, leader, n, flag);
-}
-
-void launch_newc(hipStream_t s, const int32_t *C, const int32_t *leader, const int32_t *rank,
-                 int32_t n, int32_t *newc)
-{
-    if (n > 0) hipLaunchKernelGGL(k_newc, grid_for(n, 256), 256, 0, s, C, leader, rank, n, newc);
-}
-
-// ---------------------------------------------------------------------------
-// Coarsening (inducedGraphCUDA, LouvainCUDA/InducedGraphKernel.cu:28-113).
-// Keys (newc[a]*K + newc[b]) over every row entry a->b with value w'=(w==0?1:w),
-// plus one key (newc[s], newc[s]) per vertex with value 2*selfsum[s].  After
-// the radix sort and segmented sum, a key (x,y), x != y, carries
-// E'(x,y) + E'(y,x) - the next level's row entry - and (x,x) carries 2*E'(x,x).
-
-__global__ __launch_bounds__(256) void k_coarsen_entry_keys(const int64_t *rowptr, const int32_t *col,
-                                                           const int64_t *w, int32_t n, int64_t nnz,
-                                                           const int32_t *newc, uint64_t K,
-                                                           uint64_t *keys, int64_t *vals)
-{
-    __shared__ int32_t s_lo, s_hi;
-    for (int64_t base = (int64_t)blockIdx.x * 256; base < nnz; base += (int64_t)gridDim.x * 256) {
-        if (threadIdx.x == 0) {
-            // rows containing the first and last entry of this chunk
-            int64_t last = base + 255 < nnz ? base + 255 : nnz - 1;
-            int32_t lo = 0, hi = n;
-            while (hi - lo > 1) {
-                int32_t mid = (lo + hi) >> 1;
-                if (rowptr[mid] <= base) lo = mid; else hi = mid;
-            }
-            s_lo = lo;
-            int32_t lo2 = lo;
-            hi = n;
-            while (hi - lo2 > 1) {
-                int32_t mid = (lo2 + hi) >> 1;
-                if (rowptr[mid] <= last) lo2 = mid; else hi = mid;
-            }
-            s_hi = lo2;
-        }
-        __syncthreads();
-        int64_t p = base + threadIdx.x;
-        if (p < nnz) {
-            int32_t lo = s_lo, hi = s_hi + 1;
-            while (hi - lo > 1) {
-                int32_t mid = (lo + hi) >> 1;
-                if (rowptr[mid] <= p) lo = mid; else hi = mid;
-            }
-            int64_t u = w[p];
-            keys[p] = (uint64_t)newc[lo] * K + (uint64_t)newc[col[p]];
-            vals[p] = u == 0 ? 1 : u;
-        }
-        __syncthreads();
-    }
-}
-
-__global__ void k_coarsen_self_keys(int32_t n, int64_t nnz, const int64_t *selfsum,
-                                    const int32_t *newc, uint64_t K, uint64_t *keys, int64_t *vals)
-{
-    GRID_STRIDE(i, n)
-    {
-        uint64_t c = (uint64_t)newc[i];
-        keys[nnz + i] = c * K + c;
-        vals[nnz + i] = 2 * selfsum[i];
-    }
-}
-
-void launch_coarsen_keys(hipStream_t s, const int64_t *rowptr, const int32_t *col,
-                         const int64_t *w, int32_t n, int64_t nnz, const int64_t *selfsum,
-                         const int32_t *newc, uint64_t K, uint64_t *keys, int64_t *vals)
-{
-    if (nnz > 0)
-        hipLaunchKernelGGL(k_coarsen_entry_keys, grid_for(nnz, 256, 1 << 20), 256, 0, s, rowptr,
-                           col, w, n, nnz, newc, K, keys, vals);
-    if (n > 0)
-        hipLaunchKernelGGL(k_coarsen_self_keys, grid_for(n, 256), 256, 0, s, n, nnz, selfsum, newc,
-                           K, keys, vals);
-}
-
-__global__ void k_seg_heads(const uint64_t *keys, int64_t n, int64_t *head)
-{
-    GRID_STRIDE(i, n) head[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1 : 0;
-}
-
-// hscan = exclusive scan of head: element i belongs to unique index
-// hscan[i] + head[i] - 1.  Runs are combined within the wave before the atomic.
-__global__ void k_seg_sum(const uint64_t *keys, const int64_t *vals, const int64_t *hscan,
-                          const int64_t *head, int64_t n, uint64_t *ukey, ull *usum)
-{
-    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n;
-         base += (int64_t)gridDim.x * blockDim.x) {
-        int64_t i = base + threadIdx.x;
-        bool valid = i < n;
-        long long u = 0;
-        ull v = 0;
-        if (valid) {
-            u = hscan[i] + head[i] - 1;
-            v = (ull)vals[i];
-            if (head[i]) ukey[u] = keys[i];
-        }
-        seg_atomic_add(valid, u, v, usum);
-    }
-}
-
-void launch_seg_heads(hipStream_t s, const uint64_t *keys, int64_t n, int64_t *head)
-{
-    if (n > 0) hipLaunchKernelGGL(k_seg_heads, grid_for(n, 256, 1 << 20), 256, 0, s, keys, n, head);
-}
-
-void launch_seg_sum(hipStream_t s, const uint64_t *keys, const int64_t *vals, const int64_t *hscan,
-                    const int64_t *head, int64_t n, uint64_t *ukey, ull *usum)
-{
-    if (n > 0)
-        hipLaunchKernelGGL(k_seg_sum, grid_for(n, 256, 1 << 20), 256, 0, s, keys, vals, hscan,
-                           head, n, ukey, usum);
-}
-
-__global__ void k_coarse_split(const uint64_t *ukey, const ull *usum, int64_t U, uint64_t K,
-                               int64_t *nsflag, int64_t *loopw, int64_t *selfsum, int64_t *rowcnt,
-                               int64_t *kdeg)
-{
-    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < U;
-         base += (int64_t)gridDim.x * blockDim.x) {
-        int64_t u = base + threadIdx.x;
-        bool valid = u < U;
-        long long a = 0;
-        ull kadd = 0, cadd = 0;
-        if (valid) {
-            uint64_t key = ukey[u];
-            a = (long long)(key / K);
-            long long b = (long long)(key % K);
-            ull sv = usum[u];
-            if (a == b) {
-                nsflag[u] = 0;
-                if (sv > 0) {
-                    loopw[a] = (int64_t)(sv / 2);
-                    selfsum[a] = (int64_t)(sv / 2);
-                    kadd = 4 * (sv / 2);
-                }
-            } else {
-                nsflag[u] = 1;
-                kadd = sv;
-                cadd = 1;
-            }
-        }
-        seg_atomic_add(valid, a, kadd, (ull *)kdeg);
-        seg_atomic_add(valid, a, cadd, (ull *)rowcnt);
-    }
-}
-
-__global__ void k_coarse_fill(const uint64_t *ukey, const ull *usum, const int64_t *nsflag,
-                              const int64_t *pos, int64_t U, uint64_t K, int32_t *col, int64_t *cw)
-{
-    GRID_STRIDE(u, U)
-    {
-        if (nsflag[u]) {
-            int64_t p = pos[u];
-            col[p] = (int32_t)(ukey[u] % K);
-            cw[p] = (int64_t)usum[u];
-        }
-    }
-}
-
-__global__ void k_sum_ull(const ull *p, int64_t n, long long *out)
-{
-    long long acc = 0;
-    GRID_STRIDE(i, n) acc += (long long)p[i];
-    acc = wave_sum(acc);
-    if ((threadIdx.x & 63) == 0 && acc) atomicAdd((ull *)out, (ull)acc);
-}
-
-void launch_coarse_split(hipStream_t s, const uint64_t *ukey, const ull *usum, int64_t u,
-                         uint64_t K, int64_t *nsflag, int64_t *loopw, int64_t *selfsum,
-                         int64_t *rowcnt, int64_t *kdeg)
-{
-    if (u > 0)
-        hipLaunchKernelGGL(k_coarse_split, grid_for(u, 256, 1 << 20), 256, 0, s, ukey, usum, u, K,
-                           nsflag, loopw, selfsum, rowcnt, kdeg);
-}
-
-void launch_coarse_fill(hipStream_t s, const uint64_t *ukey, const ull *usum, const int64_t *nsflag,
-                        const int64_t *pos, int64_t u, uint64_t K, int32_t *col, int64_t *cw,
-                        int64_t *kdeg)
-{
-    (void)kdeg;
-    if (u > 0)
-        hipLaunchKernelGGL(k_coarse_fill, grid_for(u, 256, 1 << 20), 256, 0, s, ukey, usum, nsflag,
-                           pos, u, K, col, cw);
-}
-
-void launch_sum_ull(hipStream_t s, const ull *p, int64_t n, long long *out)
-{
-    if (n > 0) hipLaunchKernelGGL(k_sum_ull, grid_for(n, 256, 4096), 256, 0, s, p, n, out);
 }
 
 // ---------------------------------------------------------------------------
