@@ -383,8 +383,10 @@ void launch_cr_agg_tier(hipStream_t s, int tier, const CoarsenArgs &a, const int
     case 3: launch_seg(k_cr_agg_group<16, 5, 5>, (ns + 15) / 16, s, a, segs, ns); break;
     case 4: launch_seg(k_cr_agg_group<32, 6, 6>, (ns + 7) / 8, s, a, segs, ns); break;
     case 5: launch_seg(k_cr_agg_group<64, 7, 7>, (ns + 3) / 4, s, a, segs, ns); break;
-    case 6: launch_seg(k_cr_agg_group<64, 10, 8>, (ns + 3) / 4, s, a, segs, ns); break;
-    case 7: launch_seg(k_cr_agg_block<12, 11>, ns, s, a, segs, ns); break;
+    case 6: launch_seg(k_cr_agg_group<64, 8, 8>, (ns + 3) / 4, s, a, segs, ns); break;
+    case 7: launch_seg(k_cr_agg_group<64, 9, 9>, (ns + 3) / 4, s, a, segs, ns); break;
+    case 8: launch_seg(k_cr_agg_group<64, 10, 10>, (ns + 3) / 4, s, a, segs, ns); break;
+    case 9: launch_seg(k_cr_agg_block<12, 11>, ns, s, a, segs, ns); break;
     default: break;
     }
 }
@@ -486,7 +488,9 @@ __global__ __launch_bounds__(256) void k_cr_big_agg(BigSegArgs B)
     const int tid = threadIdx.x;
     const int64_t pi = blockIdx.x;
     const int64_t beg = B.poff[pi], end = B.poff[pi + 1];
-    for (int s = tid; s < CAP; s += 256) {
+    const int l2 = (end - beg) * 2 >= CAP ? kHubAggLog2 : table_log2(end - beg, 6);
+    const int cap = 1 << l2;
+    for (int s = tid; s < cap; s += 256) {
         s_key[s] = -1;
         s_val[s] = 0;
     }
@@ -496,10 +500,10 @@ __global__ __launch_bounds__(256) void k_cr_big_agg(BigSegArgs B)
     }
     __syncthreads();
     for (int64_t e = beg + tid; e < end; e += 256)
-        if (!lds_insert_bounded(s_key, s_val, kHubAggLog2, B.py[e], B.pw[e])) s_over = 1;
+        if (!lds_insert_bounded(s_key, s_val, l2, B.py[e], B.pw[e])) s_over = 1;
     __syncthreads();
     if (!s_over) {
-        for (int s = tid; s < CAP; s += 256) {
+        for (int s = tid; s < cap; s += 256) {
             int32_t y = s_key[s];
             if (y >= 0) {
                 int pos = atomicAdd(&s_cnt, 1);

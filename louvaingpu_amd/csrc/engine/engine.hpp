@@ -27,8 +27,8 @@ typedef unsigned long long ull;
 constexpr int64_t kUnitLimit = int64_t(1) << 60;
 
 // Sweep tiers by q = row length + (1 if the vertex has a self-loop).
-constexpr int kTiers = LV_SWEEP_TIERS;  // tiers 1..7: LDS table per row; 8: hub partitions
-constexpr int kTierMaxQ[kTiers] = {4, 8, 16, 32, 64, 512, 2048, 0x7fffffff};
+constexpr int kTiers = LV_SWEEP_TIERS;  // tiers 1..9: LDS table per row; 10: hub partitions
+constexpr int kTierMaxQ[kTiers] = {4, 8, 16, 32, 64, 128, 256, 512, 2048, 0x7fffffff};
 
 #define LV_TRY(expr)                                \
     do {                                            \
@@ -117,7 +117,7 @@ struct SweepArgs {
 // P_j = ceil(q_j / kHubPartTarget) partitions, aggregates each partition in an
 // LDS table, then reduces the partitions of a hub.
 constexpr int kHubChunk = 4096;
-constexpr int kHubPartTarget = 1024;  // default; LV_HUB_PART_TARGET overrides
+constexpr int kHubPartTarget = 512;  // default; LV_HUB_PART_TARGET overrides
 constexpr int kHubMaxParts = 2048;
 constexpr int kHubAggLog2 = 12;  // 4096 LDS slots per partition block
 

@@ -541,6 +541,30 @@ __device__ __forceinline__ void finalize_vertex(const SweepArgs &a, int32_t v, i
     }
 }
 
+// Inserts row entries [beg, end) with stride S starting at beg+lane: four
+// independent row loads, then four independent C gathers, then the inserts.
+template <int S>
+__device__ __forceinline__ void insert_row(const SweepArgs &a, int32_t *key, ull *val, int l2,
+                                           int64_t beg, int64_t end, int lane)
+{
+    for (int64_t p0 = beg + lane; p0 < end; p0 += 4 * S) {
+        int32_t c[4];
+        int64_t w[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int64_t p = p0 + (int64_t)u * S;
+            c[u] = (p < end) ? a.col[p] : -1;
+            w[u] = (p < end) ? a.w[p] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (c[u] >= 0) c[u] = a.C[c[u]];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (p0 + (int64_t)u * S < end) lds_insert(key, val, l2, c[u], w[u]);
+    }
+}
+
 // G lanes per vertex, 256/G vertices per block, an LDS table of up to
 // 2^LOG2CAP slots per vertex (2^MINLOG2 .. 2^LOG2CAP sized to the row).
 template <int G, int LOG2CAP, int MINLOG2>
@@ -574,7 +598,7 @@ __global__ __launch_bounds__(256) void k_sweep_group(SweepArgs a, const int32_t 
         }
         __syncthreads();
         if (active) {
-            for (int64_t p = beg + lg; p < end; p += G) lds_insert(key, val, l2, a.C[a.col[p]], a.w[p]);
+            insert_row<G>(a, key, val, l2, beg, end, lg);
             if (lg == 0 && a.selfsum[v] > 0) lds_insert(key, val, l2, own, 0);
         }
         __syncthreads();
@@ -621,7 +645,7 @@ __global__ __launch_bounds__(BLOCK) void k_sweep_block(SweepArgs a, const int32_
             s_val[s] = 0;
         }
         __syncthreads();
-        for (int64_t p = beg + tid; p < end; p += BLOCK) lds_insert(s_key, s_val, l2, a.C[a.col[p]], a.w[p]);
+        insert_row<BLOCK>(a, s_key, s_val, l2, beg, end, tid);
         if (tid == 0 && a.selfsum[v] > 0) lds_insert(s_key, s_val, l2, own, 0);
         __syncthreads();
         int cnt = 0;
@@ -668,7 +692,7 @@ static void launch_grid(KernelT k, int64_t blocks, int block, hipStream_t s, con
     hipLaunchKernelGGL(k, g, block, 0, s, a, verts, nv);
 }
 
-// Tiers 1..7 (tier 8, the hubs, runs through run_hub_sweep).
+// Tiers 1..9 (tier 10, the hubs, runs through run_hub_sweep).
 void launch_sweep_tier(hipStream_t s, int tier, const SweepArgs &a, const int32_t *verts, int32_t nv)
 {
     if (nv <= 0) return;
@@ -678,8 +702,10 @@ void launch_sweep_tier(hipStream_t s, int tier, const SweepArgs &a, const int32_
     case 3: launch_grid(k_sweep_group<16, 5, 5>, (nv + 15) / 16, 256, s, a, verts, nv); break;
     case 4: launch_grid(k_sweep_group<32, 6, 6>, (nv + 7) / 8, 256, s, a, verts, nv); break;
     case 5: launch_grid(k_sweep_group<64, 7, 7>, (nv + 3) / 4, 256, s, a, verts, nv); break;
-    case 6: launch_grid(k_sweep_group<64, 10, 8>, (nv + 3) / 4, 256, s, a, verts, nv); break;
-    case 7: launch_grid(k_sweep_block<256, 12, 11>, nv, 256, s, a, verts, nv); break;
+    case 6: launch_grid(k_sweep_group<64, 8, 8>, (nv + 3) / 4, 256, s, a, verts, nv); break;
+    case 7: launch_grid(k_sweep_group<64, 9, 9>, (nv + 3) / 4, 256, s, a, verts, nv); break;
+    case 8: launch_grid(k_sweep_group<64, 10, 10>, (nv + 3) / 4, 256, s, a, verts, nv); break;
+    case 9: launch_grid(k_sweep_block<256, 12, 11>, nv, 256, s, a, verts, nv); break;
     default: break;
     }
 }
@@ -824,21 +850,25 @@ __global__ __launch_bounds__(256) void k_hub_agg(SweepArgs a, HubArgs h)
     const int32_t j = h.part_hub[pi];
     const int32_t v = h.hv[j];
     const double f = (double)a.kdeg[v] / a.m2;
-    for (int s = tid; s < CAP; s += 256) {
+    // 2 slots per entry up to the LDS capacity; a full table (possible only
+    // with more than CAP/2 entries) sends the partition to its global table
+    const int l2 = (end - beg) * 2 >= CAP ? kHubAggLog2 : table_log2(end - beg, 6);
+    const int cap = 1 << l2;
+    for (int s = tid; s < cap; s += 256) {
         s_key[s] = -1;
         s_val[s] = 0;
     }
     if (tid == 0) s_over = 0;
     __syncthreads();
     for (int64_t e = beg + tid; e < end; e += 256)
-        if (!lds_insert_bounded(s_key, s_val, kHubAggLog2, h.buf_c[e], h.buf_w[e])) s_over = 1;
+        if (!lds_insert_bounded(s_key, s_val, l2, h.buf_c[e], h.buf_w[e])) s_over = 1;
     __syncthreads();
     int cnt = 0;
     double bg = 0.0;
     int32_t bc = INT_MAX;
     int64_t bw = 0;
     if (!s_over) {
-        for (int s = tid; s < CAP; s += 256) {
+        for (int s = tid; s < cap; s += 256) {
             int32_t c = s_key[s];
             if (c >= 0) consider(c, (int64_t)s_val[s], a, f, cnt, bg, bc, bw);
         }
