@@ -45,6 +45,7 @@ struct lv_ctx {
     // small device/pinned scalars
     DevFlags *d_flags = nullptr, *h_flags = nullptr;
     QAcc *d_qacc = nullptr, *h_qacc = nullptr;
+    QAcc *d_qpart = nullptr;  // per-block partials of the Q reductions
     long long *d_small = nullptr, *h_small = nullptr;  // 64 slots
     // input
     bool loaded = false;
@@ -280,8 +281,7 @@ void reset_run(lv_ctx *c)
 double current_q(lv_ctx *c, int &rc)
 {
     rc = LV_OK;
-    launch_zero_qacc(c->stream, c->d_qacc);
-    launch_qagg(c->stream, c->n, c->deg, c->in, c->d_qacc);
+    launch_qagg(c->stream, c->n, c->deg, c->in, c->d_qpart, c->d_qacc);
     rc = d2h(c, c->h_qacc, c->d_qacc, 1);
     if (rc == LV_OK) rc = sync(c);
     if (rc != LV_OK) return 0.0;
@@ -518,8 +518,8 @@ int one_level(lv_ctx *c, double lower, double *q_out, int32_t *passes_out)
             c->cur_trace.push_back(std::move(nx));
         }
         hipEvent_t t1 = t_begin(c);
-        launch_zero_qacc(s, c->d_qacc);
-        launch_apply(s, c->n, c->deg, c->in, c->size, c->ddeg, c->din, c->dsize, c->d_qacc);
+        launch_apply(s, c->n, c->deg, c->in, c->size, c->ddeg, c->din, c->dsize, c->d_qpart,
+                     c->d_qacc);
         t_end(c, t1, CAT_APPLY);
         LV_HIP(hipGetLastError());
         LV_TRY(d2h(c, c->h_qacc, c->d_qacc, 1));
@@ -858,6 +858,7 @@ int lv_create(lv_ctx **out, const lv_config *cfg)
         if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { rc = LV_ERR_DEVICE; break; }
         if (hipMalloc(&c->d_flags, sizeof(DevFlags)) != hipSuccess) { rc = LV_ERR_NOMEM; break; }
         if (hipMalloc(&c->d_qacc, sizeof(QAcc)) != hipSuccess) { rc = LV_ERR_NOMEM; break; }
+        if (hipMalloc(&c->d_qpart, sizeof(QAcc) * kQParts) != hipSuccess) { rc = LV_ERR_NOMEM; break; }
         if (hipMalloc(&c->d_small, 64 * sizeof(long long)) != hipSuccess) { rc = LV_ERR_NOMEM; break; }
         if (hipHostMalloc(&c->h_flags, sizeof(DevFlags), hipHostMallocDefault) != hipSuccess) { rc = LV_ERR_NOMEM; break; }
         if (hipHostMalloc(&c->h_qacc, sizeof(QAcc), hipHostMallocDefault) != hipSuccess) { rc = LV_ERR_NOMEM; break; }
@@ -885,6 +886,7 @@ int lv_destroy(lv_ctx *c)
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
     if (c->d_flags) (void)hipFree(c->d_flags);
     if (c->d_qacc) (void)hipFree(c->d_qacc);
+    if (c->d_qpart) (void)hipFree(c->d_qpart);
     if (c->d_small) (void)hipFree(c->d_small);
     if (c->h_flags) (void)hipHostFree(c->h_flags);
     if (c->h_qacc) (void)hipHostFree(c->h_qacc);
@@ -1218,8 +1220,7 @@ int lv_final_modularity(lv_ctx *c, double *q)
     fill_i64(c->stream, (int64_t *)lc, c->n0, 0);
     fill_i64(c->stream, (int64_t *)dc, c->n0, 0);
     launch_final_q(c->stream, c->d_src, c->d_dst, c->d_w, c->e0, c->d_final, lc, dc);
-    launch_zero_qacc(c->stream, c->d_qacc);
-    launch_qagg_ull(c->stream, c->n0, dc, lc, c->d_qacc);
+    launch_qagg_ull(c->stream, c->n0, dc, lc, c->d_qpart, c->d_qacc);
     LV_HIP(hipGetLastError());
     LV_TRY(d2h(c, c->h_qacc, c->d_qacc, 1));
     LV_TRY(sync(c));

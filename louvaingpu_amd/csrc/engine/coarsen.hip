@@ -150,14 +150,17 @@ __global__ __launch_bounds__(256) void k_cr_gather(const int64_t *estart, const 
         for (int i = threadIdx.x; i < span; i += 256) s_est[i] = estart[lo0 + i];
     }
     __syncthreads();
+    constexpr int R = kGatherT / 256;
+    int32_t jj[R];
+    int64_t pp[R];
 #pragma unroll
-    for (int r = 0; r < kGatherT / 256; ++r) {
+    for (int r = 0; r < R; ++r) {
         const int64_t o = b * kGatherT + r * 256 + threadIdx.x;
-        const bool valid = o < total;
-        long long g = 0;
-        ull wv = 0;
-        if (valid) {
+        jj[r] = -1;
+        pp[r] = 0;
+        if (o < total) {
             int32_t lo;
+            int64_t first;
             if (staged) {
                 int32_t l = 0, h = span - 1;  // s_est[l] <= o < s_est[h]
                 while (h - l > 1) {
@@ -165,6 +168,7 @@ __global__ __launch_bounds__(256) void k_cr_gather(const int64_t *estart, const 
                     if (s_est[mid] <= o) l = mid; else h = mid;
                 }
                 lo = lo0 + l;
+                first = s_est[l];
             } else {
                 int32_t l = lo0, h = hi0 + 1;
                 while (h - l > 1) {
@@ -172,15 +176,37 @@ __global__ __launch_bounds__(256) void k_cr_gather(const int64_t *estart, const 
                     if (estart[mid] <= o) l = mid; else h = mid;
                 }
                 lo = l;
+                first = estart[l];
             }
-            const int64_t first = staged ? s_est[lo - lo0] : estart[lo];
-            const int32_t a = perm[lo];
-            const int64_t p = rowptr[a] + (o - first);
-            const int64_t u = cw[p];
-            ey[o] = newc[col[p]];
-            wv = (ull)(u == 0 ? 1 : u);
+            jj[r] = lo;
+            pp[r] = o - first;
+        }
+    }
+    // independent loads of the four entries before any atomics
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+        if (jj[r] >= 0) pp[r] += rowptr[perm[jj[r]]];
+    int32_t cc[R];
+    int64_t uu[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        cc[r] = (jj[r] >= 0) ? col[pp[r]] : 0;
+        uu[r] = (jj[r] >= 0) ? cw[pp[r]] : 0;
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+        if (jj[r] >= 0) cc[r] = newc[cc[r]];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int64_t o = b * kGatherT + r * 256 + threadIdx.x;
+        const bool valid = jj[r] >= 0;
+        long long g = 0;
+        ull wv = 0;
+        if (valid) {
+            wv = (ull)(uu[r] == 0 ? 1 : uu[r]);
+            ey[o] = cc[r];
             ew[o] = (int64_t)wv;
-            g = hscan[lo] + head[lo] - 1;
+            g = hscan[jj[r]] + head[jj[r]] - 1;
         }
         seg_atomic_add(valid, g, wv, ssum);
     }

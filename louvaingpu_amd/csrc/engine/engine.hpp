@@ -117,7 +117,7 @@ struct SweepArgs {
 // P_j = ceil(q_j / kHubPartTarget) partitions, aggregates each partition in an
 // LDS table, then reduces the partitions of a hub.
 constexpr int kHubChunk = 4096;
-constexpr int kHubPartTarget = 512;  // default; LV_HUB_PART_TARGET overrides
+constexpr int kHubPartTarget = 1024;  // default; LV_HUB_PART_TARGET overrides
 constexpr int kHubMaxParts = 2048;
 constexpr int kHubAggLog2 = 12;  // 4096 LDS slots per partition block
 
@@ -240,10 +240,12 @@ void launch_init_vertex(hipStream_t s, int32_t n, const int64_t *rowptr, const i
                         int32_t *size, ull *ddeg, ull *din, int32_t *dsize, int32_t *C,
                         int32_t *next, uint32_t *tier_key, int32_t *verts,
                         long long *tier_cnt, long long *tier_nnz);
-void launch_qagg(hipStream_t s, int32_t n, const int64_t *deg, const int64_t *in, QAcc *acc);
-void launch_zero_qacc(hipStream_t s, QAcc *acc);
+// Q aggregates: per-block partials in part[kQParts], summed into *acc.
+constexpr int kQParts = 1024;
+void launch_qagg(hipStream_t s, int32_t n, const int64_t *deg, const int64_t *in, QAcc *part,
+                 QAcc *acc);
 void launch_apply(hipStream_t s, int32_t n, int64_t *deg, int64_t *in, int32_t *size, ull *ddeg,
-                  ull *din, int32_t *dsize, QAcc *acc);
+                  ull *din, int32_t *dsize, QAcc *part, QAcc *acc);
 void launch_sweep_tier(hipStream_t s, int tier, const SweepArgs &a, const int32_t *verts,
                        int32_t nv);
 void launch_hub_setup(hipStream_t s, const int32_t *hv, int32_t nh, const int64_t *rowptr,
@@ -295,6 +297,6 @@ void launch_gather_partition(hipStream_t s, const int32_t *perm, const int32_t *
                              int32_t *comm);
 void launch_final_q(hipStream_t s, const int32_t *src, const int32_t *dst, const int64_t *w,
                     int64_t e, const int32_t *comm, ull *lc, ull *dc);
-void launch_qagg_ull(hipStream_t s, int32_t n, const ull *dc, const ull *lc, QAcc *acc);
+void launch_qagg_ull(hipStream_t s, int32_t n, const ull *dc, const ull *lc, QAcc *part, QAcc *acc);
 
 }  // namespace lv

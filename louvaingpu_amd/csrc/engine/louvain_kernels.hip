@@ -244,26 +244,47 @@ __global__ void k_rowptr_search(const uint32_t *skeys, int64_t total, int32_t n,
     }
 }
 
-__global__ void k_csr_gather(const uint32_t *skeys, const uint32_t *scodes, int64_t nnz,
-                             const int32_t *src, const int32_t *dst, const int64_t *w, int32_t *col,
-                             int64_t *cw, int64_t *kdeg)
+__global__ __launch_bounds__(256) void k_csr_gather(const uint32_t *skeys, const uint32_t *scodes,
+                                                   int64_t nnz, const int32_t *src,
+                                                   const int32_t *dst, const int64_t *w,
+                                                   int32_t *col, int64_t *cw, int64_t *kdeg)
 {
-    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < nnz;
-         base += (int64_t)gridDim.x * blockDim.x) {
-        int64_t p = base + threadIdx.x;
-        bool valid = p < nnz;
-        long long row = 0;
-        ull wp = 0;
-        if (valid) {
-            uint32_t code = scodes[p];
-            int64_t ei = (int64_t)(code >> 1);
-            col[p] = (code & 1u) ? src[ei] : dst[ei];
-            int64_t u = w[ei];
-            cw[p] = u;
-            row = (long long)skeys[p];
-            wp = (ull)(u == 0 ? 1 : u);
+    // four entries per thread, 256 apart, so each wave covers 64 consecutive
+    // entries (rows sorted within the wave for the segmented atomics)
+    for (int64_t base = (int64_t)blockIdx.x * 1024; base < nnz; base += (int64_t)gridDim.x * 1024) {
+        int64_t ei[4];
+        uint32_t code[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int64_t p = base + r * 256 + threadIdx.x;
+            code[r] = (p < nnz) ? scodes[p] : 0u;
+            ei[r] = (int64_t)(code[r] >> 1);
         }
-        seg_atomic_add(valid, row, wp, (ull *)kdeg);
+        int32_t cv[4];
+        int64_t u[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int64_t p = base + r * 256 + threadIdx.x;
+            if (p < nnz) {
+                cv[r] = (code[r] & 1u) ? src[ei[r]] : dst[ei[r]];
+                u[r] = w[ei[r]];
+            } else {
+                cv[r] = 0;
+                u[r] = 0;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int64_t p = base + r * 256 + threadIdx.x;
+            const bool valid = p < nnz;
+            long long row = 0;
+            if (valid) {
+                col[p] = cv[r];
+                cw[p] = u[r];
+                row = (long long)skeys[p];
+            }
+            seg_atomic_add(valid, row, (ull)(u[r] == 0 ? 1 : u[r]), (ull *)kdeg);
+        }
     }
 }
 
@@ -304,7 +325,7 @@ void launch_csr_gather(hipStream_t s, const uint32_t *skeys, const uint32_t *sco
                        int64_t *cw, int64_t *kdeg)
 {
     if (nnz > 0)
-        hipLaunchKernelGGL(k_csr_gather, grid_for(nnz, 256, 1 << 20), 256, 0, s, skeys, scodes,
+        hipLaunchKernelGGL(k_csr_gather, grid_for(nnz, 1024, 1 << 20), 256, 0, s, skeys, scodes,
                            nnz, src, dst, w, col, cw, kdeg);
 }
 
@@ -371,17 +392,51 @@ void launch_init_vertex(hipStream_t s, int32_t n, const int64_t *rowptr, const i
 // LouvainCUDA/OneLevelKernel.cu:427-451,875-880) and the Jacobi apply step
 // (communitySetKernel/mergeSetKernel, OneLevelKernel.cu:93-136).
 
-__device__ __forceinline__ void block_qacc(long long sin, ull lo, ull hi, QAcc *acc)
+// Per-block partial of (sum_in, sum deg^2) written to part[blockIdx.x]; the
+// partials are summed by k_qfinal (no atomics on shared accumulators).
+__device__ __forceinline__ void block_qpartial(long long sin, ull lo, ull hi, QAcc *part)
 {
+    __shared__ long long s_in[4];
+    __shared__ ull s_lo[4], s_hi[4];
     sin = wave_sum(sin);
     wave_sum_u128(lo, hi);
     if ((threadIdx.x & 63) == 0) {
-        if (sin) atomicAdd((ull *)&acc->sum_in, (ull)sin);
-        if (lo | hi) atomic_add_u128(&acc->d2_lo, &acc->d2_hi, lo, hi);
+        s_in[threadIdx.x >> 6] = sin;
+        s_lo[threadIdx.x >> 6] = lo;
+        s_hi[threadIdx.x >> 6] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long ti = 0;
+        ull tl = 0, th = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+            ti += s_in[w];
+            tl += s_lo[w];
+            th += s_hi[w] + (tl < s_lo[w] ? 1ull : 0ull);
+        }
+        part[blockIdx.x].sum_in = ti;
+        part[blockIdx.x].d2_lo = tl;
+        part[blockIdx.x].d2_hi = th;
+        part[blockIdx.x].pad = 0;
     }
 }
 
-__global__ void k_qagg(int32_t n, const int64_t *deg, const int64_t *in, QAcc *acc)
+__global__ __launch_bounds__(256) void k_qfinal(const QAcc *part, int nb, QAcc *acc)
+{
+    long long sin = 0;
+    ull lo = 0, hi = 0;
+    for (int b = threadIdx.x; b < nb; b += 256) {
+        sin += part[b].sum_in;
+        lo += part[b].d2_lo;
+        hi += part[b].d2_hi + (lo < part[b].d2_lo ? 1ull : 0ull);
+    }
+    __shared__ QAcc one[1];
+    block_qpartial(sin, lo, hi, one);
+    __syncthreads();
+    if (threadIdx.x == 0) *acc = one[0];
+}
+
+__global__ __launch_bounds__(256) void k_qagg(int32_t n, const int64_t *deg, const int64_t *in, QAcc *part)
 {
     long long sin = 0;
     ull lo = 0, hi = 0;
@@ -390,10 +445,10 @@ __global__ void k_qagg(int32_t n, const int64_t *deg, const int64_t *in, QAcc *a
         sin += in[c];
         acc_sq(lo, hi, (ull)deg[c]);
     }
-    block_qacc(sin, lo, hi, acc);
+    block_qpartial(sin, lo, hi, part);
 }
 
-__global__ void k_qagg_ull(int32_t n, const ull *dc, const ull *lc, QAcc *acc)
+__global__ __launch_bounds__(256) void k_qagg_ull(int32_t n, const ull *dc, const ull *lc, QAcc *part)
 {
     long long sin = 0;
     ull lo = 0, hi = 0;
@@ -402,19 +457,11 @@ __global__ void k_qagg_ull(int32_t n, const ull *dc, const ull *lc, QAcc *acc)
         sin += (long long)lc[c];
         acc_sq(lo, hi, dc[c]);
     }
-    block_qacc(sin, lo, hi, acc);
+    block_qpartial(sin, lo, hi, part);
 }
 
-__global__ void k_zero_qacc(QAcc *acc)
-{
-    acc->sum_in = 0;
-    acc->d2_lo = 0;
-    acc->d2_hi = 0;
-    acc->pad = 0;
-}
-
-__global__ void k_apply(int32_t n, int64_t *deg, int64_t *in, int32_t *size, ull *ddeg, ull *din,
-                        int32_t *dsize, QAcc *acc)
+__global__ __launch_bounds__(256) void k_apply(int32_t n, int64_t *deg, int64_t *in, int32_t *size,
+                                              ull *ddeg, ull *din, int32_t *dsize, QAcc *part)
 {
     long long sin = 0;
     ull lo = 0, hi = 0;
@@ -436,27 +483,32 @@ __global__ void k_apply(int32_t n, int64_t *deg, int64_t *in, int32_t *size, ull
         sin += iv;
         acc_sq(lo, hi, (ull)dv);
     }
-    block_qacc(sin, lo, hi, acc);
+    block_qpartial(sin, lo, hi, part);
 }
 
-void launch_qagg(hipStream_t s, int32_t n, const int64_t *deg, const int64_t *in, QAcc *acc)
+static int qblocks(int32_t n) { return grid_for(n, 256, kQParts); }
+
+void launch_qagg(hipStream_t s, int32_t n, const int64_t *deg, const int64_t *in, QAcc *part,
+                 QAcc *acc)
 {
-    if (n > 0) hipLaunchKernelGGL(k_qagg, grid_for(n, 256, 4096), 256, 0, s, n, deg, in, acc);
+    int nb = qblocks(n);
+    hipLaunchKernelGGL(k_qagg, nb, 256, 0, s, n, deg, in, part);
+    hipLaunchKernelGGL(k_qfinal, 1, 256, 0, s, part, nb, acc);
 }
 
-void launch_qagg_ull(hipStream_t s, int32_t n, const ull *dc, const ull *lc, QAcc *acc)
+void launch_qagg_ull(hipStream_t s, int32_t n, const ull *dc, const ull *lc, QAcc *part, QAcc *acc)
 {
-    if (n > 0) hipLaunchKernelGGL(k_qagg_ull, grid_for(n, 256, 4096), 256, 0, s, n, dc, lc, acc);
+    int nb = qblocks(n);
+    hipLaunchKernelGGL(k_qagg_ull, nb, 256, 0, s, n, dc, lc, part);
+    hipLaunchKernelGGL(k_qfinal, 1, 256, 0, s, part, nb, acc);
 }
-
-void launch_zero_qacc(hipStream_t s, QAcc *acc) { hipLaunchKernelGGL(k_zero_qacc, 1, 1, 0, s, acc); }
 
 void launch_apply(hipStream_t s, int32_t n, int64_t *deg, int64_t *in, int32_t *size, ull *ddeg,
-                  ull *din, int32_t *dsize, QAcc *acc)
+                  ull *din, int32_t *dsize, QAcc *part, QAcc *acc)
 {
-    if (n > 0)
-        hipLaunchKernelGGL(k_apply, grid_for(n, 256, 4096), 256, 0, s, n, deg, in, size, ddeg,
-                           din, dsize, acc);
+    int nb = qblocks(n);
+    hipLaunchKernelGGL(k_apply, nb, 256, 0, s, n, deg, in, size, ddeg, din, dsize, part);
+    hipLaunchKernelGGL(k_qfinal, 1, 256, 0, s, part, nb, acc);
 }
 
 // ---------------------------------------------------------------------------
@@ -983,17 +1035,51 @@ __global__ void k_gather_partition(const int32_t *perm, const int32_t *ids, cons
     }
 }
 
-__global__ void k_final_q(const int32_t *src, const int32_t *dst, const int64_t *w, int64_t e,
-                          const int32_t *comm, ull *lc, ull *dc)
+// Final modularity aggregates: d_c += w at both ends, l_c += w when both ends
+// are in c.  Each block combines a chunk of edges in an LDS table first (big
+// communities would otherwise serialise on their counters).
+constexpr int kFqLog2 = 12;
+constexpr int kFqChunk = 2048;
+
+__global__ __launch_bounds__(256) void k_final_q(const int32_t *src, const int32_t *dst,
+                                                const int64_t *w, int64_t e, const int32_t *comm,
+                                                ull *lc, ull *dc)
 {
-    GRID_STRIDE(i, e)
-    {
-        int32_t a = comm[src[i]], b = comm[dst[i]];
-        ull u = (ull)w[i];
-        if (u == 0) continue;
-        atomicAdd(&dc[a], u);
-        atomicAdd(&dc[b], u);
-        if (a == b) atomicAdd(&lc[a], u);
+    constexpr int CAP = 1 << kFqLog2;
+    __shared__ int32_t s_key[CAP];
+    __shared__ ull s_d[CAP], s_l[CAP];
+    for (int64_t base = (int64_t)blockIdx.x * kFqChunk; base < e;
+         base += (int64_t)gridDim.x * kFqChunk) {
+        for (int s = threadIdx.x; s < CAP; s += 256) {
+            s_key[s] = -1;
+            s_d[s] = 0;
+            s_l[s] = 0;
+        }
+        __syncthreads();
+        for (int64_t i = base + threadIdx.x; i < e && i < base + kFqChunk; i += 256) {
+            int32_t a = comm[src[i]], b = comm[dst[i]];
+            ull u = (ull)w[i];
+            if (u == 0) continue;
+            if (!lds_insert_bounded(s_key, s_d, kFqLog2, a, (int64_t)u)) {
+                atomicAdd(&dc[a], u);
+                if (a == b) atomicAdd(&lc[a], u);
+            } else if (a == b) {
+                // same slot: add the internal weight too
+                uint32_t mask = CAP - 1, sl = hslot(a, kFqLog2);
+                while (((volatile int32_t *)s_key)[sl] != a) sl = (sl + 1) & mask;
+                atomicAdd(&s_l[sl], u);
+            }
+            if (!lds_insert_bounded(s_key, s_d, kFqLog2, b, (int64_t)u)) atomicAdd(&dc[b], u);
+        }
+        __syncthreads();
+        for (int s = threadIdx.x; s < CAP; s += 256) {
+            int32_t k = s_key[s];
+            if (k >= 0) {
+                if (s_d[s]) atomicAdd(&dc[k], s_d[s]);
+                if (s_l[s]) atomicAdd(&lc[k], s_l[s]);
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -1014,7 +1100,8 @@ void launch_final_q(hipStream_t s, const int32_t *src, const int32_t *dst, const
                     int64_t e, const int32_t *comm, ull *lc, ull *dc)
 {
     if (e > 0)
-        hipLaunchKernelGGL(k_final_q, grid_for(e, 256), 256, 0, s, src, dst, w, e, comm, lc, dc);
+        hipLaunchKernelGGL(k_final_q, grid_for(e, kFqChunk, 4096), 256, 0, s, src, dst, w, e, comm,
+                           lc, dc);
 }
 
 }  // namespace lv
