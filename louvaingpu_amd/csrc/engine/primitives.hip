@@ -350,55 +350,75 @@ __global__ __launch_bounds__(kRadixBlock) void k_radix_hist(const K *keys, int64
     hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = s_h[threadIdx.x];
 }
 
+// Scatter of one digit pass.  A block round covers 1024 consecutive items:
+// wave w owns [w*256, w*256+256) of the round as four sub-rounds of 64 lanes.
+// Each sub-round ranks its items among equal digits with eight ballots and the
+// wave's running per-digit counters in LDS, so item order (round, wave,
+// sub-round, lane) is preserved: the sort is stable.
+constexpr int kRadixSub = 4;  // sub-rounds of 64 items per wave per round
+
 template <typename K, typename V>
 __global__ __launch_bounds__(kRadixBlock) void k_radix_scatter(const K *kin, const V *vin, K *kout,
                                                               V *vout, int64_t n, int shift,
                                                               int rounds, const int64_t *hscan,
                                                               int64_t nblocks)
 {
+    constexpr int NW = kRadixBlock / 64;
+    constexpr int PER_ROUND = kRadixBlock * kRadixSub;
     __shared__ int64_t s_base[256];
-    __shared__ int s_cnt[kRadixBlock / 64][256];
-    __shared__ int64_t s_off[kRadixBlock / 64][256];
+    __shared__ int s_cnt[NW][256];
+    __shared__ int64_t s_off[NW][256];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     s_base[tid] = hscan[(int64_t)tid * nblocks + blockIdx.x];
     const ull lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
-    int64_t base = (int64_t)blockIdx.x * rounds * kRadixBlock;
-    for (int r = 0; r < rounds; ++r) {
-        int64_t i = base + (int64_t)r * kRadixBlock + tid;
-        bool valid = i < n;
-        K k = 0;
-        V v = 0;
-        int d = 0;
-        if (valid) {
-            k = kin[i];
-            v = vin[i];
-            d = (int)((k >> shift) & 255);
-        }
+    const int64_t base = (int64_t)blockIdx.x * rounds * kRadixBlock;
+    const int64_t end = min(n, base + (int64_t)rounds * kRadixBlock);
+    for (int64_t r0 = base; r0 < end; r0 += PER_ROUND) {
 #pragma unroll
-        for (int w = 0; w < kRadixBlock / 64; ++w) s_cnt[w][tid] = 0;
+        for (int w = 0; w < NW; ++w) s_cnt[w][tid] = 0;
         __syncthreads();
-        ull mask = __ballot(valid);
+        K k[kRadixSub];
+        V v[kRadixSub];
+        int d[kRadixSub], rank[kRadixSub];
+        bool valid[kRadixSub];
 #pragma unroll
-        for (int b = 0; b < 8; ++b) {
-            int bit = (d >> b) & 1;
-            ull bb = __ballot(bit);
-            mask &= bit ? bb : ~bb;
+        for (int u = 0; u < kRadixSub; ++u) {
+            const int64_t i = r0 + wave * 64 * kRadixSub + u * 64 + lane;
+            valid[u] = i < end;
+            k[u] = valid[u] ? kin[i] : (K)0;
+            v[u] = valid[u] ? vin[i] : (V)0;
+            d[u] = (int)((k[u] >> shift) & 255);
         }
-        int rank = __popcll(mask & lt);
-        if (valid && rank == 0) s_cnt[wave][d] = __popcll(mask);
+#pragma unroll
+        for (int u = 0; u < kRadixSub; ++u) {
+            ull mask = __ballot(valid[u]);
+#pragma unroll
+            for (int b = 0; b < 8; ++b) {
+                int bit = (d[u] >> b) & 1;
+                ull bb = __ballot(bit);
+                mask &= bit ? bb : ~bb;
+            }
+            const int before = ((volatile int *)s_cnt[wave])[d[u]];
+            rank[u] = before + __popcll(mask & lt);
+            if (valid[u] && __popcll(mask & lt) == 0) s_cnt[wave][d[u]] = before + __popcll(mask);
+            __builtin_amdgcn_wave_barrier();
+        }
         __syncthreads();
         int64_t run = s_base[tid];
 #pragma unroll
-        for (int w = 0; w < kRadixBlock / 64; ++w) {
+        for (int w = 0; w < NW; ++w) {
             s_off[w][tid] = run;
             run += s_cnt[w][tid];
         }
         s_base[tid] = run;
         __syncthreads();
-        if (valid) {
-            int64_t dst = s_off[wave][d] + rank;
-            kout[dst] = k;
-            vout[dst] = v;
+#pragma unroll
+        for (int u = 0; u < kRadixSub; ++u) {
+            if (valid[u]) {
+                const int64_t dst = s_off[wave][d[u]] + rank[u];
+                kout[dst] = k[u];
+                vout[dst] = v[u];
+            }
         }
     }
 }
