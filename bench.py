@@ -41,6 +41,22 @@ def host_threads():
     return max(1, min(n, 16))
 
 
+def copy_probe(nbytes):
+    """One hipMemcpy device-to-device of nbytes (a plain copy kernel)."""
+    import ctypes as ct
+    hip = ct.CDLL("libamdhip64.so")
+    a, b = ct.c_void_p(), ct.c_void_p()
+    assert hip.hipMalloc(ct.byref(a), ct.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ct.byref(b), ct.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, ct.c_size_t(nbytes)) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    assert hip.hipMemcpy(b, a, ct.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    return nbytes
+
+
 def workload(name):
     from louvaingpu_amd import generators
     if name == "rmat22":
@@ -61,6 +77,9 @@ def main():
     ap.add_argument("--workload", default="rmat22")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline / validation")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="also run one device-to-device copy of this many MiB (calibrates "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE against a known byte count)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -186,6 +205,8 @@ def main():
     else:
         out["cpu_baseline"] = None
         out["validated"] = None
+    if args.copy_probe_mb:
+        out["copy_probe_bytes"] = copy_probe(args.copy_probe_mb << 20)
     eng.close()
     print(json.dumps(out), flush=True)
     return 0

@@ -53,6 +53,7 @@ struct lv_ctx {
     int64_t e0 = 0;
     int scale = 0;
     int64_t m0 = 0;
+    uint64_t max_units0 = 0;  // largest input weight in units
     int32_t *d_ids = nullptr;  // null when ids are positions
     int32_t *d_src = nullptr, *d_dst = nullptr;
     int64_t *d_w = nullptr;
@@ -292,7 +293,12 @@ double current_q(lv_ctx *c, int &rc)
 // ---------------------------------------------------------------------------
 // phases
 
-// CalculateNeighbour at level 0: symmetric CSR of the loaded edge list.
+// CalculateNeighbour at level 0: symmetric CSR of the loaded edge list.  Each
+// edge gives two entries keyed by row (self-loops keyed past the last row); a
+// stable radix sort by row keeps input order within a row.  When every weight
+// fits in 32 bits the entry carries (col, weight) through the sort and is
+// unpacked in one streaming pass; otherwise it carries its edge code and the
+// columns and weights are gathered from the edge list afterwards.
 int build_level0_csr(lv_ctx *c)
 {
     Arena &A = c->arena;
@@ -300,40 +306,62 @@ int build_level0_csr(lv_ctx *c)
     const int32_t n = c->n0;
     const int64_t e = c->e0;
     const int64_t total = 2 * e;
+    const int bits = bits_for((uint64_t)n);
+    const bool packed = c->max_units0 <= 0xffffffffull;
     hipEvent_t t0 = t_begin(c);
     ull *lastloop = nullptr;
     uint32_t *k1 = nullptr, *k2 = nullptr, *v1 = nullptr, *v2 = nullptr;
+    uint64_t *p1 = nullptr, *p2 = nullptr;
     LV_TRY(arena_alloc(A, c->rowptr, (int64_t)n + 1));
     LV_TRY(arena_alloc(A, c->kdeg, n));
     LV_TRY(arena_alloc(A, c->selfsum, n));
     LV_TRY(arena_alloc(A, c->loopw, n));
     LV_TRY(arena_alloc(A, lastloop, n));
     LV_TRY(arena_alloc(A, k1, total));
-    LV_TRY(arena_alloc(A, v1, total));
     LV_TRY(arena_alloc(A, k2, total));
-    LV_TRY(arena_alloc(A, v2, total));
+    if (packed) {
+        LV_TRY(arena_alloc(A, p1, total));
+        LV_TRY(arena_alloc(A, p2, total));
+    } else {
+        LV_TRY(arena_alloc(A, v1, total));
+        LV_TRY(arena_alloc(A, v2, total));
+    }
     fill_i64(s, c->kdeg, n, 0);
     fill_i64(s, c->selfsum, n, 0);
     fill_i64(s, (int64_t *)lastloop, n, 0);
-    launch_csr_entries(s, c->d_src, c->d_dst, e, n, k1, v1);
     launch_self_loops(s, c->d_src, c->d_dst, c->d_w, e, c->kdeg, c->selfsum, lastloop);
     uint32_t *sk = nullptr, *sv = nullptr;
-    LV_TRY(radix_sort_u32_u32(A, s, k1, v1, k2, v2, total, bits_for((uint64_t)n), &sk, &sv));
+    uint64_t *sp = nullptr;
+    if (packed) {
+        launch_csr_entries_packed(s, c->d_src, c->d_dst, c->d_w, e, n, k1, p1);
+        LV_TRY(radix_sort_u32_u64(A, s, k1, p1, k2, p2, total, bits, &sk, &sp));
+    } else {
+        launch_csr_entries(s, c->d_src, c->d_dst, e, n, k1, v1);
+        LV_TRY(radix_sort_u32_u32(A, s, k1, v1, k2, v2, total, bits, &sk, &sv));
+    }
     launch_rowptr_search(s, sk, total, n, c->rowptr);
     LV_TRY(d2h(c, &c->h_small[0], (const long long *)(c->rowptr + n), 1));
     LV_TRY(sync(c));
     c->nnz = c->h_small[0];
     LV_TRY(arena_alloc(A, c->col, c->nnz));
     LV_TRY(arena_alloc(A, c->cw, c->nnz));
-    launch_csr_gather(s, sk, sv, c->nnz, c->d_src, c->d_dst, c->d_w, c->col, c->cw, c->kdeg);
+    if (packed)
+        launch_csr_unpack(s, sk, sp, c->nnz, c->col, c->cw, c->kdeg);
+    else
+        launch_csr_gather(s, sk, sv, c->nnz, c->d_src, c->d_dst, c->d_w, c->col, c->cw, c->kdeg);
     launch_loop_gather(s, lastloop, c->d_w, n, c->loopw);
     LV_HIP(hipGetLastError());
     t_end(c, t0, CAT_CSR);
     arena_free(A, lastloop);
     arena_free(A, k1);
     arena_free(A, k2);
-    arena_free(A, v1);
-    arena_free(A, v2);
+    if (packed) {
+        arena_free(A, p1);
+        arena_free(A, p2);
+    } else {
+        arena_free(A, v1);
+        arena_free(A, v2);
+    }
     c->n = n;
     c->m = c->m0;
     return LV_OK;
@@ -982,6 +1010,7 @@ int lv_load_edges(lv_ctx *c, int32_t n, const int32_t *ids, int64_t e, const int
     }
     c->scale = scale;
     c->m0 = c->h_flags->m_units;
+    c->max_units0 = c->h_flags->max_units;
     c->n0 = n;
     c->e0 = e;
     c->loaded = true;

@@ -195,6 +195,9 @@ int radix_sort_u32_i32(Arena &a, hipStream_t s, uint32_t *keys, int32_t *vals, u
 int radix_sort_u32_u32(Arena &a, hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *ktmp,
                        uint32_t *vtmp, int64_t n, int bits, uint32_t **keys_out,
                        uint32_t **vals_out);
+int radix_sort_u32_u64(Arena &a, hipStream_t s, uint32_t *keys, uint64_t *vals, uint32_t *ktmp,
+                       uint64_t *vtmp, int64_t n, int bits, uint32_t **keys_out,
+                       uint64_t **vals_out);
 void fill_i32(hipStream_t s, int32_t *p, int64_t n, int32_t v);
 void fill_i64(hipStream_t s, int64_t *p, int64_t n, int64_t v);
 void iota_i32(hipStream_t s, int32_t *p, int64_t n);
@@ -211,6 +214,8 @@ struct DevFlags {  // error flags and small scalars written by kernels
     int32_t pad0, pad1;
     long long total_units;    // sum of (w == 0 ? 1 : w) in units
     long long m_units;        // sum of w in units
+    unsigned long long max_units;  // largest weight in units
+    long long pad2;
 };
 
 void launch_weight_exponent(hipStream_t s, const void *w, int wtype, int64_t e, DevFlags *f);
@@ -225,6 +230,12 @@ void launch_ids_to_keys(hipStream_t s, const int32_t *ids, uint32_t *keys, int32
 
 void launch_csr_entries(hipStream_t s, const int32_t *src, const int32_t *dst, int64_t e,
                         int32_t n, uint32_t *keys, uint32_t *codes);
+// Entries with the payload (col << 32 | w) when every weight fits in 32 bits.
+void launch_csr_entries_packed(hipStream_t s, const int32_t *src, const int32_t *dst,
+                               const int64_t *w, int64_t e, int32_t n, uint32_t *keys,
+                               uint64_t *payload);
+void launch_csr_unpack(hipStream_t s, const uint32_t *skeys, const uint64_t *spay, int64_t nnz,
+                       int32_t *col, int64_t *cw, int64_t *kdeg);
 void launch_self_loops(hipStream_t s, const int32_t *src, const int32_t *dst, const int64_t *w,
                        int64_t e, int64_t *kdeg, int64_t *selfsum, ull *lastloop);
 void launch_rowptr_search(hipStream_t s, const uint32_t *skeys, int64_t total, int32_t n,

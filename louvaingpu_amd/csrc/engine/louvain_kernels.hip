@@ -54,7 +54,7 @@ __global__ void k_weight_exponent(const void *w, int wtype, int64_t e, DevFlags 
 __global__ void k_weight_units(const void *w, int wtype, int64_t e, int scale, int64_t *wu,
                                DevFlags *f)
 {
-    ull tot = 0, msum = 0;
+    ull tot = 0, msum = 0, umax = 0;
     int bad = 0;
     GRID_STRIDE(i, e)
     {
@@ -79,6 +79,7 @@ __global__ void k_weight_units(const void *w, int wtype, int64_t e, int scale, i
         }
         if (u >= kUnitLimit) bad = 2;
         wu[i] = u;
+        umax = max(umax, (ull)u);
         tot += (ull)(u == 0 ? 1 : u);
         msum += (ull)u;
         if (tot >= (ull)kUnitLimit) bad = 2;
@@ -94,6 +95,8 @@ __global__ void k_weight_units(const void *w, int wtype, int64_t e, int scale, i
         if (msum) atomicAdd((ull *)&f->m_units, msum);
         if (bad) atomicOr(&f->err, bad);
     }
+    for (int off = 32; off > 0; off >>= 1) umax = max(umax, __shfl_xor(umax, off, 64));
+    if ((threadIdx.x & 63) == 0 && umax) atomicMax(&f->max_units, umax);
 }
 
 __global__ void k_check_endpoints(const int32_t *src, const int32_t *dst, int64_t e, int32_t n,
@@ -213,6 +216,60 @@ __global__ void k_csr_entries(const int32_t *src, const int32_t *dst, int64_t e,
         codes[2 * i] = (uint32_t)(2 * i);
         codes[2 * i + 1] = (uint32_t)(2 * i + 1);
     }
+}
+
+__global__ void k_csr_entries_packed(const int32_t *src, const int32_t *dst, const int64_t *w,
+                                     int64_t e, int32_t n, uint32_t *keys, uint64_t *payload)
+{
+    GRID_STRIDE(i, e)
+    {
+        int32_t a = src[i], b = dst[i];
+        uint64_t u = (uint64_t)w[i];
+        bool self = (a == b);
+        keys[2 * i] = self ? (uint32_t)n : (uint32_t)a;
+        keys[2 * i + 1] = self ? (uint32_t)n : (uint32_t)b;
+        payload[2 * i] = ((uint64_t)(uint32_t)b << 32) | u;
+        payload[2 * i + 1] = ((uint64_t)(uint32_t)a << 32) | u;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_csr_unpack(const uint32_t *skeys, const uint64_t *spay,
+                                                   int64_t nnz, int32_t *col, int64_t *cw,
+                                                   int64_t *kdeg)
+{
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < nnz;
+         base += (int64_t)gridDim.x * blockDim.x) {
+        int64_t p = base + threadIdx.x;
+        bool valid = p < nnz;
+        long long row = 0;
+        ull wp = 0;
+        if (valid) {
+            uint64_t x = spay[p];
+            int64_t u = (int64_t)(x & 0xffffffffull);
+            col[p] = (int32_t)(x >> 32);
+            cw[p] = u;
+            row = (long long)skeys[p];
+            wp = (ull)(u == 0 ? 1 : u);
+        }
+        seg_atomic_add(valid, row, wp, (ull *)kdeg);
+    }
+}
+
+void launch_csr_entries_packed(hipStream_t s, const int32_t *src, const int32_t *dst,
+                               const int64_t *w, int64_t e, int32_t n, uint32_t *keys,
+                               uint64_t *payload)
+{
+    if (e > 0)
+        hipLaunchKernelGGL(k_csr_entries_packed, grid_for(e, 256, 1 << 20), 256, 0, s, src, dst, w,
+                           e, n, keys, payload);
+}
+
+void launch_csr_unpack(hipStream_t s, const uint32_t *skeys, const uint64_t *spay, int64_t nnz,
+                       int32_t *col, int64_t *cw, int64_t *kdeg)
+{
+    if (nnz > 0)
+        hipLaunchKernelGGL(k_csr_unpack, grid_for(nnz, 256, 1 << 20), 256, 0, s, skeys, spay, nnz,
+                           col, cw, kdeg);
 }
 
 __global__ void k_self_loops(const int32_t *src, const int32_t *dst, const int64_t *w, int64_t e,

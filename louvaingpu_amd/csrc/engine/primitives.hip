@@ -478,6 +478,14 @@ int radix_sort_u32_i32(Arena &a, hipStream_t s, uint32_t *keys, int32_t *vals, u
                                          vals_out);
 }
 
+int radix_sort_u32_u64(Arena &a, hipStream_t s, uint32_t *keys, uint64_t *vals, uint32_t *ktmp,
+                       uint64_t *vtmp, int64_t n, int bits, uint32_t **keys_out,
+                       uint64_t **vals_out)
+{
+    return radix_sort<uint32_t, uint64_t>(a, s, keys, vals, ktmp, vtmp, n, bits, keys_out,
+                                          vals_out);
+}
+
 int radix_sort_u32_u32(Arena &a, hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *ktmp,
                        uint32_t *vtmp, int64_t n, int bits, uint32_t **keys_out,
                        uint32_t **vals_out)
