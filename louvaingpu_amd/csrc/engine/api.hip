@@ -67,6 +67,7 @@ struct lv_ctx {
     int64_t *cw = nullptr;
     int64_t *kdeg = nullptr, *loopw = nullptr, *selfsum = nullptr;
     int32_t *C = nullptr, *next = nullptr;
+    Lab *lab = nullptr;
     int64_t *deg = nullptr, *in = nullptr;
     int32_t *size = nullptr, *dsize = nullptr;
     ull *ddeg = nullptr, *din = nullptr;
@@ -200,6 +201,7 @@ void free_sweep_state(lv_ctx *c)
     Arena &a = c->arena;
     arena_free(a, c->C);
     arena_free(a, c->next);
+    arena_free(a, c->lab);
     arena_free(a, c->deg);
     arena_free(a, c->in);
     arena_free(a, c->size);
@@ -451,6 +453,7 @@ int init_level(lv_ctx *c)
     LV_TRY(arena_alloc(A, c->din, n));
     LV_TRY(arena_alloc(A, c->C, n));
     LV_TRY(arena_alloc(A, c->next, n));
+    LV_TRY(arena_alloc(A, c->lab, n));
     uint32_t *tkey = nullptr, *tkey2 = nullptr;
     int32_t *v1 = nullptr, *v2 = nullptr;
     LV_TRY(arena_alloc(A, tkey, n));
@@ -501,6 +504,7 @@ int one_level(lv_ctx *c, double lower, double *q_out, int32_t *passes_out)
     c->cur_trace.clear();
     const bool trace = (c->cfg.flags & LV_FLAG_TRACE) != 0;
     SweepArgs a;
+    a.lab = c->lab;
     a.rowptr = c->rowptr;
     a.col = c->col;
     a.w = c->cw;
@@ -526,6 +530,9 @@ int one_level(lv_ctx *c, double lower, double *q_out, int32_t *passes_out)
         a.C = c->C;
         a.next = c->next;
         a.last = last ? 1 : 0;
+        hipEvent_t tl = t_begin(c);
+        launch_lab(s, c->n, c->C, c->size, c->deg, c->lab);
+        t_end(c, tl, CAT_APPLY);
         for (int t = 1; t <= kTiers; ++t) {
             const int64_t nv = c->tier_cnt[t];
             if (nv == 0) continue;

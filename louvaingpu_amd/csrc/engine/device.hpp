@@ -104,6 +104,29 @@ __device__ __forceinline__ void lds_insert(int32_t *key, ull *val, int log2cap, 
     if (w != 0) atomicAdd(&val[s], (ull)w);
 }
 
+// As lds_insert; the lane that creates a slot also stores the key's
+// community total d in dg[slot].
+__device__ __forceinline__ void lds_insert_d(int32_t *key, ull *val, int64_t *dg, int log2cap,
+                                             int32_t c, int64_t w, int64_t d)
+{
+    const uint32_t mask = (1u << log2cap) - 1u;
+    uint32_t s = hslot(c, log2cap);
+    for (;;) {
+        int32_t k = ((volatile int32_t *)key)[s];
+        if (k == c) break;
+        if (k == -1) {
+            int32_t prev = atomicCAS(&key[s], -1, c);
+            if (prev == -1) {
+                dg[s] = d;
+                break;
+            }
+            if (prev == c) break;
+        }
+        s = (s + 1) & mask;
+    }
+    if (w != 0) atomicAdd(&val[s], (ull)w);
+}
+
 __device__ __forceinline__ bool better(double g1, int32_t c1, double g2, int32_t c2)
 {
     return g1 > g2 || (g1 == g2 && c1 < c2);

@@ -92,7 +92,17 @@ struct QAcc {
 double q_from_aggregates(int64_t sum_in, uint64_t d2_lo, uint64_t d2_hi, int64_t m, double gamma);
 
 // Per-sweep kernel arguments.
+// A vertex's view of its community at the start of a pass: the community id
+// and that community's size and total degree, in one 16-byte record, so a
+// neighbour's community and its total arrive with one gather.
+struct alignas(16) Lab {
+    int32_t c;
+    int32_t sz;
+    int64_t deg;
+};
+
 struct SweepArgs {
+    const Lab *lab;
     const int64_t *rowptr;
     const int32_t *col;
     const int64_t *w;
@@ -255,6 +265,8 @@ void launch_init_vertex(hipStream_t s, int32_t n, const int64_t *rowptr, const i
 constexpr int kQParts = 1024;
 void launch_qagg(hipStream_t s, int32_t n, const int64_t *deg, const int64_t *in, QAcc *part,
                  QAcc *acc);
+void launch_lab(hipStream_t s, int32_t n, const int32_t *C, const int32_t *size,
+                const int64_t *deg, Lab *lab);
 void launch_apply(hipStream_t s, int32_t n, int64_t *deg, int64_t *in, int32_t *size, ull *ddeg,
                   ull *din, int32_t *dsize, QAcc *part, QAcc *acc);
 void launch_sweep_tier(hipStream_t s, int tier, const SweepArgs &a, const int32_t *verts,

@@ -543,6 +543,25 @@ __global__ __launch_bounds__(256) void k_apply(int32_t n, int64_t *deg, int64_t 
     block_qpartial(sin, lo, hi, part);
 }
 
+__global__ void k_lab(int32_t n, const int32_t *C, const int32_t *size, const int64_t *deg, Lab *lab)
+{
+    GRID_STRIDE(v, n)
+    {
+        const int32_t c = C[v];
+        Lab L;
+        L.c = c;
+        L.sz = size[c];
+        L.deg = deg[c];
+        lab[v] = L;
+    }
+}
+
+void launch_lab(hipStream_t s, int32_t n, const int32_t *C, const int32_t *size,
+                const int64_t *deg, Lab *lab)
+{
+    if (n > 0) hipLaunchKernelGGL(k_lab, grid_for(n, 256, 1 << 20), 256, 0, s, n, C, size, deg, lab);
+}
+
 static int qblocks(int32_t n) { return grid_for(n, 256, kQParts); }
 
 void launch_qagg(hipStream_t s, int32_t n, const int64_t *deg, const int64_t *in, QAcc *part,
@@ -581,12 +600,12 @@ void launch_apply(hipStream_t s, int32_t n, int64_t *deg, int64_t *in, int32_t *
 // the maximal g when that maximum is > 0, otherwise the own community - the
 // outcome of the reference's ascending scan with its replace rule.
 
-__device__ __forceinline__ void consider(int32_t c, int64_t W, const SweepArgs &a, double f,
-                                         int &cnt, double &bg, int32_t &bc, int64_t &bw)
+__device__ __forceinline__ void consider(int32_t c, int64_t W, int64_t degc, const SweepArgs &a,
+                                         double f, int &cnt, double &bg, int32_t &bc, int64_t &bw)
 {
     ++cnt;
     if (W != 0) {
-        double g = (double)W - ((a.gamma * (double)a.deg[c]) * f);
+        double g = (double)W - ((a.gamma * (double)degc) * f);
         if (g > 0.0 && better(g, c, bg, bc)) {
             bg = g;
             bc = c;
@@ -616,12 +635,11 @@ __device__ __forceinline__ void group_reduce(int &cnt, double &bg, int32_t &bc, 
 // OneLevelKernel.cu:52-78,400-421); on the last pass the deltas are the
 // merge-isolated ones only (:109-171,921-927).
 __device__ __forceinline__ void finalize_vertex(const SweepArgs &a, int32_t v, int32_t own,
-                                                int n, int32_t bc, int64_t bw)
+                                                int32_t so, int n, int32_t bc, int64_t bw)
 {
     const bool has = bc != INT_MAX;
     const int32_t c0 = has ? bc : own;
     const int64_t w0 = has ? bw : 0;
-    const int32_t so = a.size[own];
     int32_t choice = c0;
     if (so == 1 && a.size[c0] == 1 && c0 > own) choice = own;
     const int64_t k = a.kdeg[v], lp = a.loopw[v];
@@ -651,32 +669,46 @@ __device__ __forceinline__ void finalize_vertex(const SweepArgs &a, int32_t v, i
 }
 
 // Inserts row entries [beg, end) with stride S starting at beg+lane: four
-// independent row loads, then four independent C gathers, then the inserts.
-template <int S>
-__device__ __forceinline__ void insert_row(const SweepArgs &a, int32_t *key, ull *val, int l2,
-                                           int64_t beg, int64_t end, int lane)
+// independent row loads, then four independent gathers, then the inserts.
+// With DL the gather is the neighbour's 16-byte label record and the
+// community total goes into the table (dg); otherwise only C[target] is
+// gathered and the totals are read per candidate later.
+template <int S, bool DL>
+__device__ __forceinline__ void insert_row(const SweepArgs &a, int32_t *key, ull *val, int64_t *dg,
+                                           int l2, int64_t beg, int64_t end, int lane)
 {
     for (int64_t p0 = beg + lane; p0 < end; p0 += 4 * S) {
-        int32_t c[4];
+        int32_t j[4];
         int64_t w[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int64_t p = p0 + (int64_t)u * S;
-            c[u] = (p < end) ? a.col[p] : -1;
+            j[u] = (p < end) ? a.col[p] : -1;
             w[u] = (p < end) ? a.w[p] : 0;
         }
+        if constexpr (DL) {
+            Lab L[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
-            if (c[u] >= 0) c[u] = a.C[c[u]];
+            for (int u = 0; u < 4; ++u)
+                if (j[u] >= 0) L[u] = a.lab[j[u]];
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
-            if (p0 + (int64_t)u * S < end) lds_insert(key, val, l2, c[u], w[u]);
+            for (int u = 0; u < 4; ++u)
+                if (j[u] >= 0) lds_insert_d(key, val, dg, l2, L[u].c, w[u], L[u].deg);
+        } else {
+            int32_t c[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (j[u] >= 0) c[u] = a.C[j[u]];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (j[u] >= 0) lds_insert(key, val, l2, c[u], w[u]);
+        }
     }
 }
 
 // G lanes per vertex, 256/G vertices per block, an LDS table of up to
 // 2^LOG2CAP slots per vertex (2^MINLOG2 .. 2^LOG2CAP sized to the row).
-template <int G, int LOG2CAP, int MINLOG2>
+template <int G, int LOG2CAP, int MINLOG2, bool DL>
 __global__ __launch_bounds__(256) void k_sweep_group(SweepArgs a, const int32_t *__restrict__ verts,
                                                     int32_t nv)
 {
@@ -684,21 +716,26 @@ __global__ __launch_bounds__(256) void k_sweep_group(SweepArgs a, const int32_t 
     constexpr int GPB = 256 / G;
     __shared__ int32_t s_key[GPB * CAP];
     __shared__ ull s_val[GPB * CAP];
+    __shared__ int64_t s_dg[DL ? GPB * CAP : 1];
     const int lg = threadIdx.x % G, grp = threadIdx.x / G;
     int32_t *key = s_key + grp * CAP;
     ull *val = s_val + grp * CAP;
+    int64_t *dg = s_dg + (DL ? grp * CAP : 0);
     for (int64_t base = (int64_t)blockIdx.x * GPB; base < nv; base += (int64_t)gridDim.x * GPB) {
         const int64_t vi = base + grp;
         const bool active = vi < nv;
-        int32_t v = 0, own = 0;
+        int32_t v = 0;
+        Lab me{0, 0, 0};
         int64_t beg = 0, end = 0;
+        bool loop = false;
         int l2 = MINLOG2;
         if (active) {
             v = verts[vi];
-            own = a.C[v];
+            me = a.lab[v];
             beg = a.rowptr[v];
             end = a.rowptr[v + 1];
-            if (MINLOG2 < LOG2CAP) l2 = table_log2(end - beg + (a.selfsum[v] > 0 ? 1 : 0), MINLOG2);
+            loop = a.selfsum[v] > 0;
+            if (MINLOG2 < LOG2CAP) l2 = table_log2(end - beg + (loop ? 1 : 0), MINLOG2);
         }
         const int cap = 1 << l2;
         for (int s = lg; s < cap; s += G) {
@@ -707,8 +744,13 @@ __global__ __launch_bounds__(256) void k_sweep_group(SweepArgs a, const int32_t 
         }
         __syncthreads();
         if (active) {
-            insert_row<G>(a, key, val, l2, beg, end, lg);
-            if (lg == 0 && a.selfsum[v] > 0) lds_insert(key, val, l2, own, 0);
+            insert_row<G, DL>(a, key, val, dg, l2, beg, end, lg);
+            if (lg == 0 && loop) {
+                if constexpr (DL)
+                    lds_insert_d(key, val, dg, l2, me.c, 0, me.deg);
+                else
+                    lds_insert(key, val, l2, me.c, 0);
+            }
         }
         __syncthreads();
         int cnt = 0;
@@ -719,11 +761,11 @@ __global__ __launch_bounds__(256) void k_sweep_group(SweepArgs a, const int32_t 
             const double f = (double)a.kdeg[v] / a.m2;
             for (int s = lg; s < cap; s += G) {
                 int32_t c = key[s];
-                if (c >= 0) consider(c, (int64_t)val[s], a, f, cnt, bg, bc, bw);
+                if (c >= 0) consider(c, (int64_t)val[s], DL ? dg[s] : a.deg[c], a, f, cnt, bg, bc, bw);
             }
         }
         group_reduce<G>(cnt, bg, bc, bw);
-        if (active && lg == 0) finalize_vertex(a, v, own, cnt, bc, bw);
+        if (active && lg == 0) finalize_vertex(a, v, me.c, me.sz, cnt, bc, bw);
         __syncthreads();
     }
 }
@@ -745,17 +787,18 @@ __global__ __launch_bounds__(BLOCK) void k_sweep_block(SweepArgs a, const int32_
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int64_t vi = blockIdx.x; vi < nv; vi += gridDim.x) {
         const int32_t v = verts[vi];
-        const int32_t own = a.C[v];
+        const Lab me = a.lab[v];
         const int64_t beg = a.rowptr[v], end = a.rowptr[v + 1];
-        const int l2 = table_log2(end - beg + (a.selfsum[v] > 0 ? 1 : 0), MINLOG2);
+        const bool loop = a.selfsum[v] > 0;
+        const int l2 = table_log2(end - beg + (loop ? 1 : 0), MINLOG2);
         const int cap = 1 << l2;
         for (int s = tid; s < cap; s += BLOCK) {
             s_key[s] = -1;
             s_val[s] = 0;
         }
         __syncthreads();
-        insert_row<BLOCK>(a, s_key, s_val, l2, beg, end, tid);
-        if (tid == 0 && a.selfsum[v] > 0) lds_insert(s_key, s_val, l2, own, 0);
+        insert_row<BLOCK, false>(a, s_key, s_val, nullptr, l2, beg, end, tid);
+        if (tid == 0 && loop) lds_insert(s_key, s_val, l2, me.c, 0);
         __syncthreads();
         int cnt = 0;
         double bg = 0.0;
@@ -764,7 +807,7 @@ __global__ __launch_bounds__(BLOCK) void k_sweep_block(SweepArgs a, const int32_
         const double f = (double)a.kdeg[v] / a.m2;
         for (int s = tid; s < cap; s += BLOCK) {
             int32_t c = s_key[s];
-            if (c >= 0) consider(c, (int64_t)s_val[s], a, f, cnt, bg, bc, bw);
+            if (c >= 0) consider(c, (int64_t)s_val[s], a.deg[c], a, f, cnt, bg, bc, bw);
         }
         group_reduce<64>(cnt, bg, bc, bw);
         if (lane == 0) {
@@ -787,7 +830,7 @@ __global__ __launch_bounds__(BLOCK) void k_sweep_block(SweepArgs a, const int32_
                     w0 = r_w[w];
                 }
             }
-            finalize_vertex(a, v, own, tc, c0, w0);
+            finalize_vertex(a, v, me.c, me.sz, tc, c0, w0);
         }
         __syncthreads();
     }
@@ -806,14 +849,14 @@ void launch_sweep_tier(hipStream_t s, int tier, const SweepArgs &a, const int32_
 {
     if (nv <= 0) return;
     switch (tier) {
-    case 1: launch_grid(k_sweep_group<4, 3, 3>, (nv + 63) / 64, 256, s, a, verts, nv); break;
-    case 2: launch_grid(k_sweep_group<8, 4, 4>, (nv + 31) / 32, 256, s, a, verts, nv); break;
-    case 3: launch_grid(k_sweep_group<16, 5, 5>, (nv + 15) / 16, 256, s, a, verts, nv); break;
-    case 4: launch_grid(k_sweep_group<32, 6, 6>, (nv + 7) / 8, 256, s, a, verts, nv); break;
-    case 5: launch_grid(k_sweep_group<64, 7, 7>, (nv + 3) / 4, 256, s, a, verts, nv); break;
-    case 6: launch_grid(k_sweep_group<64, 8, 8>, (nv + 3) / 4, 256, s, a, verts, nv); break;
-    case 7: launch_grid(k_sweep_group<64, 9, 9>, (nv + 3) / 4, 256, s, a, verts, nv); break;
-    case 8: launch_grid(k_sweep_group<64, 10, 10>, (nv + 3) / 4, 256, s, a, verts, nv); break;
+    case 1: launch_grid(k_sweep_group<4, 3, 3, true>, (nv + 63) / 64, 256, s, a, verts, nv); break;
+    case 2: launch_grid(k_sweep_group<8, 4, 4, true>, (nv + 31) / 32, 256, s, a, verts, nv); break;
+    case 3: launch_grid(k_sweep_group<16, 5, 5, true>, (nv + 15) / 16, 256, s, a, verts, nv); break;
+    case 4: launch_grid(k_sweep_group<32, 6, 6, true>, (nv + 7) / 8, 256, s, a, verts, nv); break;
+    case 5: launch_grid(k_sweep_group<64, 7, 7, true>, (nv + 3) / 4, 256, s, a, verts, nv); break;
+    case 6: launch_grid(k_sweep_group<64, 8, 8, true>, (nv + 3) / 4, 256, s, a, verts, nv); break;
+    case 7: launch_grid(k_sweep_group<64, 9, 9, true>, (nv + 3) / 4, 256, s, a, verts, nv); break;
+    case 8: launch_grid(k_sweep_group<64, 10, 10, false>, (nv + 3) / 4, 256, s, a, verts, nv); break;
     case 9: launch_grid(k_sweep_block<256, 12, 11>, nv, 256, s, a, verts, nv); break;
     default: break;
     }
@@ -979,7 +1022,7 @@ __global__ __launch_bounds__(256) void k_hub_agg(SweepArgs a, HubArgs h)
     if (!s_over) {
         for (int s = tid; s < cap; s += 256) {
             int32_t c = s_key[s];
-            if (c >= 0) consider(c, (int64_t)s_val[s], a, f, cnt, bg, bc, bw);
+            if (c >= 0) consider(c, (int64_t)s_val[s], a.deg[c], a, f, cnt, bg, bc, bw);
         }
     } else {
         // more than CAP distinct keys: redo the partition in its global region
@@ -999,7 +1042,7 @@ __global__ __launch_bounds__(256) void k_hub_agg(SweepArgs a, HubArgs h)
             int32_t c = __hip_atomic_load(&gk[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (c >= 0)
                 consider(c, (int64_t)__hip_atomic_load(&gv[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
-                         a, f, cnt, bg, bc, bw);
+                         a.deg[c], a, f, cnt, bg, bc, bw);
         }
     }
     group_reduce<64>(cnt, bg, bc, bw);
@@ -1053,7 +1096,8 @@ __global__ __launch_bounds__(256) void k_hub_final(SweepArgs a, HubArgs h, int32
     group_reduce<64>(cnt, bg, bc, bw);
     if (j < nh && lane == 0) {
         const int32_t v = h.hv[j];
-        finalize_vertex(a, v, a.C[v], cnt, bc, bw);
+        const Lab me = a.lab[v];
+        finalize_vertex(a, v, me.c, me.sz, cnt, bc, bw);
     }
 }
 
