@@ -660,7 +660,9 @@ int induce(lv_ctx *c)
     ull *selfl = nullptr, *ssum = nullptr, *internal = nullptr;
     int32_t *ey = nullptr, *bound = nullptr, *uy = nullptr, *nuniq = nullptr;
     int64_t *ew = nullptr, *uw = nullptr;
+    int64_t *mstart = nullptr;
     LV_TRY(arena_alloc(A, mdeg, n));
+    LV_TRY(arena_alloc(A, mstart, n));
     LV_TRY(arena_alloc(A, estart, (int64_t)n + 1));
     LV_TRY(arena_alloc(A, selfl, K));
     LV_TRY(arena_alloc(A, ssum, K));
@@ -671,15 +673,14 @@ int induce(lv_ctx *c)
     LV_TRY(arena_alloc(A, uy, nnz));
     LV_TRY(arena_alloc(A, uw, nnz));
     LV_TRY(arena_alloc(A, bound, nnz / 1024 + 2));
-    launch_cr_memdeg(s, L.perm, c->rowptr, n, mdeg);
+    launch_cr_memdeg(s, L.perm, c->rowptr, n, mdeg, mstart);
     LV_TRY(scan_exclusive_i64(A, s, mdeg, estart, n, estart + n));
     fill_i64(s, (int64_t *)selfl, K, 0);
     fill_i64(s, (int64_t *)ssum, K, 0);
     fill_i64(s, (int64_t *)internal, K, 0);
     fill_i32(s, nuniq, K, 0);
     launch_cr_selfl(s, L.perm, c->selfsum, c->g_head, c->g_hscan, n, selfl);
-    launch_cr_gather(s, estart, bound, L.perm, c->rowptr, c->col, c->cw, L.newc, c->g_head,
-                     c->g_hscan, n, nnz, ey, ew, ssum);
+    launch_cr_gather(s, estart, bound, mstart, c->col, c->cw, L.newc, n, nnz, ey, ew);
     // segments by length
     int64_t *sb = nullptr, *slen = nullptr;
     uint32_t *tk1 = nullptr, *tk2 = nullptr;
@@ -713,6 +714,7 @@ int induce(lv_ctx *c)
     ca.uw = uw;
     ca.nuniq = nuniq;
     ca.internal = internal;
+    ca.ssum = ssum;
     for (int t = 1; t < kTiers; ++t) launch_cr_agg_tier(s, t, ca, segs + toff[t], (int32_t)tcnt[t]);
     // segments above 2048 entries: hash partitions
     BigSegArgs B{};
@@ -722,21 +724,27 @@ int induce(lv_ctx *c)
     int64_t *bP64 = nullptr, *bch = nullptr, *pbase = nullptr, *cbase = nullptr;
     if (nbs > 0) {
         const int32_t *bsegs = segs + toff[kTiers];
+        int64_t *blen64 = nullptr, *blscan = nullptr;
         LV_TRY(arena_alloc(A, bP, nbs));
         LV_TRY(arena_alloc(A, bP64, nbs));
         LV_TRY(arena_alloc(A, bch, nbs));
+        LV_TRY(arena_alloc(A, blen64, nbs));
+        LV_TRY(arena_alloc(A, blscan, (int64_t)nbs + 1));
         LV_TRY(arena_alloc(A, pbase, (int64_t)nbs + 1));
         LV_TRY(arena_alloc(A, cbase, (int64_t)nbs + 1));
-        launch_cr_big_setup(s, bsegs, nbs, slen, c->hub_part_target, bP, bP64, bch);
+        launch_cr_big_setup(s, bsegs, nbs, slen, c->hub_part_target, bP, bP64, bch, blen64);
         LV_TRY(scan_exclusive_i64(A, s, bP64, pbase, nbs, pbase + nbs));
         LV_TRY(scan_exclusive_i64(A, s, bch, cbase, nbs, cbase + nbs));
+        LV_TRY(scan_exclusive_i64(A, s, blen64, blscan, nbs, blscan + nbs));
         LV_TRY(d2h(c, c->h_small, (const long long *)(pbase + nbs), 1));
         LV_TRY(d2h(c, c->h_small + 1, (const long long *)(cbase + nbs), 1));
+        LV_TRY(d2h(c, c->h_small + 2, (const long long *)(blscan + nbs), 1));
         LV_TRY(sync(c));
         nparts = c->h_small[0];
         nchunks = c->h_small[1];
-        blen = nchunks * (int64_t)kHubChunk;  // bound on the entries of big segments
-        if (blen > nnz) blen = nnz;
+        blen = c->h_small[2];  // entries of the big segments
+        arena_free(A, blen64);
+        arena_free(A, blscan);
         LV_TRY(arena_alloc(A, part_big, nparts));
         LV_TRY(arena_alloc(A, chunk_big, nchunks));
         LV_TRY(arena_alloc(A, chunk_k, nchunks));
@@ -785,7 +793,7 @@ int induce(lv_ctx *c)
     LV_HIP(hipGetLastError());
     t_end(c, t0, CAT_COARSEN);
     // release the transient buffers (later launches on this stream may reuse them)
-    void *tmp[] = {mdeg, estart, selfl, ssum, internal, nuniq, ey, ew, uy, uw, bound, sb, slen,
+    void *tmp[] = {mdeg, mstart, estart, selfl, ssum, internal, nuniq, ey, ew, uy, uw, bound, sb, slen,
                    tk1, tk2, sid1, sid2, bP, bP64, bch, pbase, cbase, part_big, chunk_big, chunk_k,
                    B.pcount, B.poff, B.pcur, B.py, B.pw, B.buy, B.buw, B.fbk, B.fbv, B.ucnt, B.pre,
                    rowcnt};

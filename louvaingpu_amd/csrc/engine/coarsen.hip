@@ -84,12 +84,17 @@ void launch_cr_newc(hipStream_t s, const int32_t *head, const int32_t *hscan, co
 // ---------------------------------------------------------------------------
 // member-order gather
 
-__global__ void k_cr_memdeg(const int32_t *perm, const int64_t *rowptr, int32_t n, int64_t *mdeg)
+// Per member j (member order): its row length and row start, so the gather
+// below reaches a member's entries without dependent lookups.
+__global__ void k_cr_memdeg(const int32_t *perm, const int64_t *rowptr, int32_t n, int64_t *mdeg,
+                            int64_t *mstart)
 {
     GRID_STRIDE(j, n)
     {
         int32_t a = perm[j];
-        mdeg[j] = rowptr[a + 1] - rowptr[a];
+        const int64_t b = rowptr[a];
+        mdeg[j] = rowptr[a + 1] - b;
+        mstart[j] = b;
     }
 }
 
@@ -132,14 +137,14 @@ constexpr int kGatherT = 1024;  // output entries per block (4 per thread)
 constexpr int kGatherLds = 2048; // member offsets a block can stage in LDS
 
 // ey/ew[o] for every output slot o of member order: the o-th entry of the
-// member rows, as (newc[target], w'); ssum[g] accumulates w' per group.  The
-// block's member offsets estart[lo0..hi0+1] are staged in LDS when they fit.
+// member rows, as (newc[target], w').  The
+// block's member offsets estart[lo0..hi0+1] are staged in LDS when they fit;
+// a member's row start comes precomputed (mstart), so the chain
+// of dependent global loads per entry is: member record, row entry, newc.
 __global__ __launch_bounds__(256) void k_cr_gather(const int64_t *estart, const int32_t *bound,
-                                                  const int32_t *perm, const int64_t *rowptr,
-                                                  const int32_t *col, const int64_t *cw,
-                                                  const int32_t *newc, const int32_t *head,
-                                                  const int32_t *hscan, int64_t total,
-                                                  int32_t *ey, int64_t *ew, ull *ssum)
+                                                  const int64_t *mstart, const int32_t *col,
+                                                  const int64_t *cw, const int32_t *newc,
+                                                  int64_t total, int32_t *ey, int64_t *ew)
 {
     __shared__ int64_t s_est[kGatherLds];
     const int64_t b = blockIdx.x;
@@ -152,70 +157,62 @@ __global__ __launch_bounds__(256) void k_cr_gather(const int64_t *estart, const 
     __syncthreads();
     constexpr int R = kGatherT / 256;
     int32_t jj[R];
-    int64_t pp[R];
+    int64_t off[R];
+    bool ok[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int64_t o = b * kGatherT + r * 256 + threadIdx.x;
-        jj[r] = -1;
-        pp[r] = 0;
-        if (o < total) {
-            int32_t lo;
-            int64_t first;
+        ok[r] = o < total;
+        jj[r] = 0;
+        off[r] = 0;
+        if (ok[r]) {
             if (staged) {
                 int32_t l = 0, h = span - 1;  // s_est[l] <= o < s_est[h]
                 while (h - l > 1) {
                     int32_t mid = (l + h) >> 1;
                     if (s_est[mid] <= o) l = mid; else h = mid;
                 }
-                lo = lo0 + l;
-                first = s_est[l];
+                jj[r] = lo0 + l;
+                off[r] = o - s_est[l];
             } else {
                 int32_t l = lo0, h = hi0 + 1;
                 while (h - l > 1) {
                     int32_t mid = (l + h) >> 1;
                     if (estart[mid] <= o) l = mid; else h = mid;
                 }
-                lo = l;
-                first = estart[l];
+                jj[r] = l;
+                off[r] = o - estart[l];
             }
-            jj[r] = lo;
-            pp[r] = o - first;
         }
     }
-    // independent loads of the four entries before any atomics
+    int64_t pp[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r)
-        if (jj[r] >= 0) pp[r] += rowptr[perm[jj[r]]];
+    for (int r = 0; r < R; ++r) pp[r] = ok[r] ? mstart[jj[r]] + off[r] : 0;
     int32_t cc[R];
     int64_t uu[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        cc[r] = (jj[r] >= 0) ? col[pp[r]] : 0;
-        uu[r] = (jj[r] >= 0) ? cw[pp[r]] : 0;
+        cc[r] = ok[r] ? col[pp[r]] : 0;
+        uu[r] = ok[r] ? cw[pp[r]] : 0;
     }
 #pragma unroll
     for (int r = 0; r < R; ++r)
-        if (jj[r] >= 0) cc[r] = newc[cc[r]];
+        if (ok[r]) cc[r] = newc[cc[r]];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int64_t o = b * kGatherT + r * 256 + threadIdx.x;
-        const bool valid = jj[r] >= 0;
-        long long g = 0;
-        ull wv = 0;
-        if (valid) {
-            wv = (ull)(uu[r] == 0 ? 1 : uu[r]);
+        if (ok[r]) {
             ey[o] = cc[r];
-            ew[o] = (int64_t)wv;
-            g = hscan[jj[r]] + head[jj[r]] - 1;
+            ew[o] = uu[r] == 0 ? 1 : uu[r];
         }
-        seg_atomic_add(valid, g, wv, ssum);
     }
 }
 
 void launch_cr_memdeg(hipStream_t s, const int32_t *perm, const int64_t *rowptr, int32_t n,
-                      int64_t *mdeg)
+                      int64_t *mdeg, int64_t *mstart)
 {
-    if (n > 0) hipLaunchKernelGGL(k_cr_memdeg, grid_for(n, 256), 256, 0, s, perm, rowptr, n, mdeg);
+    if (n > 0)
+        hipLaunchKernelGGL(k_cr_memdeg, grid_for(n, 256), 256, 0, s, perm, rowptr, n, mdeg, mstart);
 }
 
 void launch_cr_selfl(hipStream_t s, const int32_t *perm, const int64_t *selfsum,
@@ -226,17 +223,16 @@ void launch_cr_selfl(hipStream_t s, const int32_t *perm, const int64_t *selfsum,
                            hscan, n, selfl);
 }
 
-void launch_cr_gather(hipStream_t s, const int64_t *estart, int32_t *bound, const int32_t *perm,
-                      const int64_t *rowptr, const int32_t *col, const int64_t *cw,
-                      const int32_t *newc, const int32_t *head, const int32_t *hscan, int32_t n,
-                      int64_t total, int32_t *ey, int64_t *ew, ull *ssum)
+void launch_cr_gather(hipStream_t s, const int64_t *estart, int32_t *bound, const int64_t *mstart,
+                      const int32_t *col, const int64_t *cw, const int32_t *newc, int32_t n,
+                      int64_t total, int32_t *ey, int64_t *ew)
 {
     if (total <= 0) return;
     const int64_t nb = (total + kGatherT - 1) / kGatherT;
     hipLaunchKernelGGL(k_lbs_bounds, grid_for(nb + 1, 256), 256, 0, s, estart, n, total, kGatherT,
                        nb, bound);
-    hipLaunchKernelGGL(k_cr_gather, (int)nb, 256, 0, s, estart, bound, perm, rowptr, col, cw, newc,
-                       head, hscan, total, ey, ew, ssum);
+    hipLaunchKernelGGL(k_cr_gather, (int)nb, 256, 0, s, estart, bound, mstart, col, cw, newc,
+                       total, ey, ew);
 }
 
 // ---------------------------------------------------------------------------
@@ -308,11 +304,12 @@ __global__ __launch_bounds__(256) void k_cr_agg_group(CoarsenArgs a, const int32
         }
         if (lg == 0) s_cnt[grp] = 0;
         __syncthreads();
-        ull inter = 0;
+        ull inter = 0, tot = 0;
         if (active) {
             for (int64_t e = b + lg; e < b + L; e += G) {
                 int32_t y = a.ey[e];
                 int64_t w = a.ew[e];
+                tot += (ull)w;
                 if (y == x)
                     inter += (ull)w;
                 else
@@ -331,11 +328,15 @@ __global__ __launch_bounds__(256) void k_cr_agg_group(CoarsenArgs a, const int32
             }
         }
 #pragma unroll
-        for (int off = G / 2; off > 0; off >>= 1) inter += __shfl_xor(inter, off, G);
+        for (int off = G / 2; off > 0; off >>= 1) {
+            inter += __shfl_xor(inter, off, G);
+            tot += __shfl_xor(tot, off, G);
+        }
         __syncthreads();
         if (active && lg == 0) {
             a.nuniq[g] = s_cnt[grp];
             a.internal[g] = inter;
+            a.ssum[g] = tot;
         }
         __syncthreads();
     }
@@ -349,7 +350,7 @@ __global__ __launch_bounds__(256) void k_cr_agg_block(CoarsenArgs a, const int32
     __shared__ int32_t s_key[CAP];
     __shared__ ull s_val[CAP];
     __shared__ int s_cnt;
-    __shared__ ull s_inter[4];
+    __shared__ ull s_inter[4], s_tot[4];
     const int tid = threadIdx.x;
     for (int64_t vi = blockIdx.x; vi < ns; vi += gridDim.x) {
         const int32_t g = segs[vi];
@@ -363,10 +364,11 @@ __global__ __launch_bounds__(256) void k_cr_agg_block(CoarsenArgs a, const int32
         }
         if (tid == 0) s_cnt = 0;
         __syncthreads();
-        ull inter = 0;
+        ull inter = 0, tot = 0;
         for (int64_t e = b + tid; e < b + L; e += 256) {
             int32_t y = a.ey[e];
             int64_t w = a.ew[e];
+            tot += (ull)w;
             if (y == x)
                 inter += (ull)w;
             else
@@ -382,11 +384,16 @@ __global__ __launch_bounds__(256) void k_cr_agg_block(CoarsenArgs a, const int32
             }
         }
         inter = (ull)wave_sum((long long)inter);
-        if ((tid & 63) == 0) s_inter[tid >> 6] = inter;
+        tot = (ull)wave_sum((long long)tot);
+        if ((tid & 63) == 0) {
+            s_inter[tid >> 6] = inter;
+            s_tot[tid >> 6] = tot;
+        }
         __syncthreads();
         if (tid == 0) {
             a.nuniq[g] = s_cnt;
             a.internal[g] = s_inter[0] + s_inter[1] + s_inter[2] + s_inter[3];
+            a.ssum[g] = s_tot[0] + s_tot[1] + s_tot[2] + s_tot[3];
         }
         __syncthreads();
     }
@@ -423,7 +430,8 @@ void launch_cr_agg_tier(hipStream_t s, int tier, const CoarsenArgs &a, const int
 // distinct keys); uniques land at poff[p] of buy/buw.
 
 __global__ void k_cr_big_setup(const int32_t *bsegs, int32_t nbs, const int64_t *slen,
-                               int32_t target, int32_t *bP, int64_t *bP64, int64_t *bch)
+                               int32_t target, int32_t *bP, int64_t *bP64, int64_t *bch,
+                               int64_t *blen)
 {
     GRID_STRIDE(j, nbs)
     {
@@ -431,16 +439,18 @@ __global__ void k_cr_big_setup(const int32_t *bsegs, int32_t nbs, const int64_t 
         int64_t P = (L + target - 1) / target;
         if (P < 1) P = 1;
         if (P > kHubMaxParts) P = kHubMaxParts;
+        const int64_t cs = chunk_span((int)P);
         bP[j] = (int32_t)P;
         bP64[j] = P;
-        bch[j] = (L + kHubChunk - 1) / kHubChunk;
+        bch[j] = (L + cs - 1) / cs;
+        blen[j] = L;
     }
 }
 
 __global__ __launch_bounds__(256) void k_cr_big_count(CoarsenArgs a, BigSegArgs B)
 {
     __shared__ int hist[kHubMaxParts];
-    __shared__ ull s_inter[4];
+    __shared__ ull s_inter[4], s_tot[4];
     const int32_t ci = blockIdx.x;
     const int32_t j = B.chunk_big[ci];
     const int32_t g = B.bsegs[j];
@@ -449,24 +459,33 @@ __global__ __launch_bounds__(256) void k_cr_big_count(CoarsenArgs a, BigSegArgs 
     const int64_t b = a.sb[g], L = a.slen[g];
     for (int t = threadIdx.x; t < P; t += 256) hist[t] = 0;
     __syncthreads();
-    const int64_t t0 = (int64_t)B.chunk_k[ci] * kHubChunk;
-    const int64_t t1 = min(L, t0 + kHubChunk);
-    ull inter = 0;
+    const int64_t cs = chunk_span(P);
+    const int64_t t0 = (int64_t)B.chunk_k[ci] * cs;
+    const int64_t t1 = min(L, t0 + cs);
+    ull inter = 0, tot = 0;
     for (int64_t t = t0 + threadIdx.x; t < t1; t += 256) {
         int32_t y = a.ey[b + t];
+        const ull w = (ull)a.ew[b + t];
+        tot += w;
         if (y == x)
-            inter += (ull)a.ew[b + t];
+            inter += w;
         else
             atomicAdd(&hist[hub_part(y, P)], 1);
     }
     inter = (ull)wave_sum((long long)inter);
-    if ((threadIdx.x & 63) == 0) s_inter[threadIdx.x >> 6] = inter;
+    tot = (ull)wave_sum((long long)tot);
+    if ((threadIdx.x & 63) == 0) {
+        s_inter[threadIdx.x >> 6] = inter;
+        s_tot[threadIdx.x >> 6] = tot;
+    }
     __syncthreads();
     for (int t = threadIdx.x; t < P; t += 256)
         if (hist[t]) atomicAdd((ull *)&B.pcount[B.pbase[j] + t], (ull)hist[t]);
     if (threadIdx.x == 0) {
-        ull tot = s_inter[0] + s_inter[1] + s_inter[2] + s_inter[3];
-        if (tot) atomicAdd(&a.internal[g], tot);
+        ull it = s_inter[0] + s_inter[1] + s_inter[2] + s_inter[3];
+        if (it) atomicAdd(&a.internal[g], it);
+        ull tt = s_tot[0] + s_tot[1] + s_tot[2] + s_tot[3];
+        if (tt) atomicAdd(&a.ssum[g], tt);
     }
 }
 
@@ -482,8 +501,9 @@ __global__ __launch_bounds__(256) void k_cr_big_scatter(CoarsenArgs a, BigSegArg
     const int64_t b = a.sb[g], L = a.slen[g];
     for (int t = threadIdx.x; t < P; t += 256) hist[t] = 0;
     __syncthreads();
-    const int64_t t0 = (int64_t)B.chunk_k[ci] * kHubChunk;
-    const int64_t t1 = min(L, t0 + kHubChunk);
+    const int64_t cs = chunk_span(P);
+    const int64_t t0 = (int64_t)B.chunk_k[ci] * cs;
+    const int64_t t1 = min(L, t0 + cs);
     for (int64_t t = t0 + threadIdx.x; t < t1; t += 256) {
         int32_t y = a.ey[b + t];
         if (y != x) atomicAdd(&hist[hub_part(y, P)], 1);
@@ -574,11 +594,11 @@ __global__ void k_cr_big_prefix(CoarsenArgs a, BigSegArgs B, const int64_t *ucsc
 }
 
 void launch_cr_big_setup(hipStream_t s, const int32_t *bsegs, int32_t nbs, const int64_t *slen,
-                         int32_t target, int32_t *bP, int64_t *bP64, int64_t *bch)
+                         int32_t target, int32_t *bP, int64_t *bP64, int64_t *bch, int64_t *blen)
 {
     if (nbs > 0)
         hipLaunchKernelGGL(k_cr_big_setup, grid_for(nbs, 256), 256, 0, s, bsegs, nbs, slen, target,
-                           bP, bP64, bch);
+                           bP, bP64, bch, blen);
 }
 
 int run_cr_big(Arena &A, hipStream_t s, const CoarsenArgs &a, const BigSegArgs &B, int32_t nbs,

@@ -123,7 +123,7 @@ struct SweepArgs {
 
 // Hub tier plan (rows with q > 2048): each hub's entries (plus a virtual
 // own-community entry when it has a self-loop) are cut into chunks of
-// kHubChunk; every pass buckets them by a hash of their community into
+// chunk_span(P) entries (kHubChunk or more); every pass buckets them by a hash of their community into
 // P_j = ceil(q_j / kHubPartTarget) partitions, aggregates each partition in an
 // LDS table, then reduces the partitions of a hub.
 constexpr int kHubChunk = 4096;
@@ -165,6 +165,7 @@ struct CoarsenArgs {
     int64_t *uw;
     int32_t *nuniq;         // uniques per group
     ull *internal;          // sum of w' over entries inside the group
+    ull *ssum;              // sum of w' over all entries of the group
 };
 
 // Segments of more than 2048 entries, partitioned like the hub rows.
@@ -291,19 +292,18 @@ void launch_cr_newid(hipStream_t s, const int32_t *gstart, const int32_t *perm, 
 void launch_cr_newc(hipStream_t s, const int32_t *head, const int32_t *hscan, const int32_t *perm,
                     const int32_t *newid_g, int32_t n, int32_t *newc);
 void launch_cr_memdeg(hipStream_t s, const int32_t *perm, const int64_t *rowptr, int32_t n,
-                      int64_t *mdeg);
+                      int64_t *mdeg, int64_t *mstart);
 void launch_cr_selfl(hipStream_t s, const int32_t *perm, const int64_t *selfsum,
                      const int32_t *head, const int32_t *hscan, int32_t n, ull *selfl);
-void launch_cr_gather(hipStream_t s, const int64_t *estart, int32_t *bound, const int32_t *perm,
-                      const int64_t *rowptr, const int32_t *col, const int64_t *cw,
-                      const int32_t *newc, const int32_t *head, const int32_t *hscan, int32_t n,
-                      int64_t total, int32_t *ey, int64_t *ew, ull *ssum);
+void launch_cr_gather(hipStream_t s, const int64_t *estart, int32_t *bound, const int64_t *mstart,
+                      const int32_t *col, const int64_t *cw, const int32_t *newc, int32_t n,
+                      int64_t total, int32_t *ey, int64_t *ew);
 void launch_cr_seg_tier(hipStream_t s, const int32_t *gstart, const int64_t *estart, int32_t K,
                         int64_t *sb, int64_t *slen, uint32_t *tkey, int32_t *sid,
                         long long *tier_cnt);
 void launch_cr_agg_tier(hipStream_t s, int tier, const CoarsenArgs &a, const int32_t *segs, int32_t ns);
 void launch_cr_big_setup(hipStream_t s, const int32_t *bsegs, int32_t nbs, const int64_t *slen,
-                         int32_t target, int32_t *bP, int64_t *bP64, int64_t *bch);
+                         int32_t target, int32_t *bP, int64_t *bP64, int64_t *bch, int64_t *blen);
 int run_cr_big(Arena &A, hipStream_t s, const CoarsenArgs &a, const BigSegArgs &B, int32_t nbs,
                int64_t nparts, int64_t nchunks);
 void launch_cr_rowcnt(hipStream_t s, const int32_t *newid_g, const int32_t *nuniq, int32_t K,

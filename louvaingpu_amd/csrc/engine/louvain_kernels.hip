@@ -880,7 +880,8 @@ __global__ void k_hub_setup(const int32_t *hv, int32_t nh, const int64_t *rowptr
         hq[j] = q;
         hP[j] = (int32_t)P;
         hP64[j] = P;
-        hch[j] = (q + kHubChunk - 1) / kHubChunk;
+        const int64_t cs = chunk_span((int)P);
+        hch[j] = (q + cs - 1) / cs;
     }
 }
 
@@ -941,8 +942,9 @@ __global__ __launch_bounds__(256) void k_hub_count(SweepArgs a, HubArgs h)
     const int32_t own = a.C[v];
     for (int t = threadIdx.x; t < P; t += 256) hist[t] = 0;
     __syncthreads();
-    const int64_t t0 = (int64_t)h.chunk_k[ci] * kHubChunk;
-    const int64_t t1 = min(q, t0 + kHubChunk);
+    const int64_t cs = chunk_span(P);
+    const int64_t t0 = (int64_t)h.chunk_k[ci] * cs;
+    const int64_t t1 = min(q, t0 + cs);
     int32_t *tc = h.tmp_c + h.ebase[j];
     for (int64_t t = t0 + threadIdx.x; t < t1; t += 256) {
         int32_t c = (t < d) ? a.C[a.col[rb + t]] : own;
@@ -966,8 +968,9 @@ __global__ __launch_bounds__(256) void k_hub_scatter(SweepArgs a, HubArgs h)
     const int64_t rb = a.rowptr[v], d = a.rowptr[v + 1] - rb;
     for (int t = threadIdx.x; t < P; t += 256) hist[t] = 0;
     __syncthreads();
-    const int64_t t0 = (int64_t)h.chunk_k[ci] * kHubChunk;
-    const int64_t t1 = min(q, t0 + kHubChunk);
+    const int64_t cs = chunk_span(P);
+    const int64_t t0 = (int64_t)h.chunk_k[ci] * cs;
+    const int64_t t1 = min(q, t0 + cs);
     const int32_t *tc = h.tmp_c + h.ebase[j];
     for (int64_t t = t0 + threadIdx.x; t < t1; t += 256) atomicAdd(&hist[hub_part(tc[t], P)], 1);
     __syncthreads();
