@@ -9,9 +9,11 @@ resident in HBM.  value = TEPS = directed CSR entries of the input (2E) /
 seconds per step.
 
 Besides the JSON line's required fields it reports:
-  roofline     the dominant local-moving sweep kernel: algorithmic bytes
-               12*nnz + 24*N per launch (community-total reads P not counted)
-               / its mean HIP-event duration, against 8 TB/s
+  roofline     the dominant local-moving sweep tier (one kernel, or the four
+               kernels of the hub pipeline): algorithmic bytes 12*nnz + 24*N per
+               launch (community-total reads P not counted) / its mean HIP-event
+               duration, against 8 TB/s; traffic = measured HBM bytes per launch
+               from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
   cpu_baseline the oracle (C restatement of the reference semantics, OpenMP)
                on the same graph, timed on this host; its partition is also
                the validation of the GPU result
@@ -30,6 +32,17 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+
+# Kernels of each sweep tier, as named in rocprofv3 summaries.
+TIER_KERNELS = {
+    1: ["k_sweep_group<4, 3, 3,"], 2: ["k_sweep_group<8, 4, 4,"],
+    3: ["k_sweep_group<16, 5, 5,"], 4: ["k_sweep_group<32, 6, 6,"],
+    5: ["k_sweep_group<64, 7, 7,"], 6: ["k_sweep_group<64, 8, 8,"],
+    7: ["k_sweep_group<64, 9, 9,"], 8: ["k_sweep_group<64, 10, 10,"],
+    9: ["k_sweep_block<256, 12, 11>"],
+    10: ["k_hub_count", "k_hub_scatter", "k_hub_agg", "k_hub_final"],
+}
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "louvain_%s_hbm_traffic.txt")
 
 
 def host_threads():
@@ -55,6 +68,31 @@ def copy_probe(nbytes):
     hip.hipFree(a)
     hip.hipFree(b)
     return nbytes
+
+
+def measured_traffic(workload_key, tier):
+    """HBM bytes per launch of a sweep tier (sum over its kernels) from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary of this workload, or None."""
+    path = TRAFFIC_PROFILE % workload_key
+    if not os.path.exists(path):
+        return None
+    per_kernel = {}
+    with open(path) as f:
+        for line in f:
+            if line.startswith("#") or not line.strip():
+                continue
+            parts = line.split(None, 3)
+            if len(parts) < 4 or not parts[0].isdigit():
+                continue
+            per_kernel[parts[3].strip()] = (float(parts[1]) + float(parts[2])) * 1e6
+    total, found = 0.0, 0
+    for pat in TIER_KERNELS[tier]:
+        for name, b in per_kernel.items():
+            if pat in name:
+                total += b
+                found += 1
+                break
+    return total if found == len(TIER_KERNELS[tier]) else None
 
 
 def workload(name):
@@ -157,12 +195,13 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "sweep tier %d" % (dom + 1),
+            "kernel": "sweep tier %d (%s)" % (dom + 1, " + ".join(
+                k[:-1] + ", ..>" if k.endswith(",") else k for k in TIER_KERNELS[dom + 1])),
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(args.workload, dom + 1),
             "bytes_per_launch": bytes_per_launch,
             "avg_launch_ms": avg_ms,
         },
