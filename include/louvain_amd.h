@@ -176,6 +176,29 @@ int lv_level_vertex(lv_ctx *ctx, int64_t *k, int64_t *loop, int64_t *selfsum);
 /* Current level's sizes before lv_init/lv_one_level. */
 int lv_current_level(lv_ctx *ctx, int32_t *level, int32_t *n, int64_t *nnz);
 
+/* Graph sets (batching).  The reference runs one graph per generate_dendogram
+ * call (LouvainCUDA/LouvainCPP.cpp:209-553); these run many independent graphs,
+ * each at nres resolutions, as the components of one union graph, so every
+ * phase is one launch for the whole set.  Component c is graph c / nres at
+ * resolution gamma[c % nres]; each component follows the single-graph
+ * semantics exactly (its own m, break test, lower clamp, level loop and
+ * termination), and its results equal an lv_run of that graph alone.
+ * Edges use local ids 0..n[g]-1 and are listed graph after graph (e[g] each);
+ * cfg.gamma is ignored.  Node ids are positions (no id mapping). */
+int lv_batch_load(lv_ctx *ctx, int32_t ngraphs, const int32_t *n, const int64_t *e,
+                  const int32_t *src, const int32_t *dst, const void *w, int wtype, int32_t nres,
+                  const double *gamma);
+int lv_batch_run(lv_ctx *ctx);
+int lv_batch_num_components(lv_ctx *ctx, int32_t *ncomp);
+int lv_batch_num_levels(lv_ctx *ctx, int32_t comp, int32_t *levels);
+/* Per-component level summary (nnz is not tracked per component: 0). */
+int lv_batch_level_info(lv_ctx *ctx, int32_t comp, int32_t level, lv_level_info *out);
+/* Renumbered community of every vertex of the component at that level. */
+int lv_batch_level_assignment(lv_ctx *ctx, int32_t comp, int32_t level, int32_t *comm);
+/* Final community of every level-0 vertex of the component (comp = -1: all
+ * components, concatenated in component order). */
+int lv_batch_final_assignment(lv_ctx *ctx, int32_t comp, int32_t *comm);
+
 int lv_get_stats(lv_ctx *ctx, lv_stats *out);
 int lv_reset_stats(lv_ctx *ctx);
 /* Wall time of the last lv_run in milliseconds (host clock, device synced). */

@@ -6,8 +6,9 @@ reference's Graph/Partition API and level loop.  Python reaches both through
 ctypes; there is no host fallback for any phase.
 """
 from .engine import Engine, generate_dendogram_phases, make_config  # noqa: F401
+from .batch import GraphSet  # noqa: F401
 from .louvain import Dendrogram, count_partitions, generate_dendogram, start  # noqa: F401
 from . import generators  # noqa: F401
 
-__all__ = ["Engine", "generate_dendogram_phases", "make_config", "Dendrogram",
+__all__ = ["Engine", "GraphSet", "generate_dendogram_phases", "make_config", "Dendrogram",
            "count_partitions", "generate_dendogram", "start", "generators"]

@@ -38,6 +38,8 @@ ENGINE_SYMBOLS = (
     "lv_level_assignment", "lv_flatten", "lv_final_assignment", "lv_final_modularity",
     "lv_trace_next", "lv_trace_pass_q", "lv_level_csr", "lv_level_vertex", "lv_current_level",
     "lv_get_stats", "lv_reset_stats", "lv_last_run_ms",
+    "lv_batch_load", "lv_batch_run", "lv_batch_num_components", "lv_batch_num_levels",
+    "lv_batch_level_info", "lv_batch_level_assignment", "lv_batch_final_assignment",
 )
 HOST_SYMBOLS = (
     "lvh_generate_dendrogram", "lvh_num_levels", "lvh_level_count", "lvh_level_partition",
@@ -149,6 +151,14 @@ def engine():
         "lv_get_stats": (ct.c_int, [ctx, ct.POINTER(LvStats)]),
         "lv_reset_stats": (ct.c_int, [ctx]),
         "lv_last_run_ms": (ct.c_int, [ctx, _DP]),
+        "lv_batch_load": (ct.c_int, [ctx, ct.c_int32, _I32P, _I64P, _I32P, _I32P, _VP, ct.c_int,
+                                     ct.c_int32, _DP]),
+        "lv_batch_run": (ct.c_int, [ctx]),
+        "lv_batch_num_components": (ct.c_int, [ctx, _I32P]),
+        "lv_batch_num_levels": (ct.c_int, [ctx, ct.c_int32, _I32P]),
+        "lv_batch_level_info": (ct.c_int, [ctx, ct.c_int32, ct.c_int32, ct.POINTER(LvLevelInfo)]),
+        "lv_batch_level_assignment": (ct.c_int, [ctx, ct.c_int32, ct.c_int32, _I32P]),
+        "lv_batch_final_assignment": (ct.c_int, [ctx, ct.c_int32, _I32P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

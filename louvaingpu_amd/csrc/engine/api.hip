@@ -24,6 +24,10 @@ struct LevelRec {
     double q = 0, lower = 0;
     int32_t *perm = nullptr;    // positions sorted by (community, position) (device)
     int32_t *newc = nullptr;    // renumbered community of each position (device)
+    // graph sets: first community id of each component (ncomp+1) and first
+    // vertex of each component at the next level, -1 when it stops (ncomp)
+    int64_t *koff = nullptr, *vnext = nullptr;
+    std::vector<int64_t> koff_h, voff_h;  // host copies; voff_h = this level's vertex offsets
     std::vector<double> pass_q;
     std::vector<std::vector<int32_t>> trace;
 };
@@ -92,6 +96,27 @@ struct lv_ctx {
     std::vector<LevelRec> levels;
     int32_t *d_final = nullptr;
     bool final_valid = false;
+    // graph set (lv_batch_load): ncomp = graphs x resolutions components of one
+    // union graph; component c is graph c / nres at resolution c % nres
+    bool batch = false;
+    int32_t ncomp = 0, nres = 1;
+    std::vector<int32_t> comp_n0;           // level-0 vertices per component
+    std::vector<int64_t> comp_e0;           // input edges per component
+    std::vector<double> comp_gamma;
+    std::vector<int64_t> comp_m0;           // m (units) at level 0
+    std::vector<int> comp_scale;            // unit exponent of each component's graph
+    std::vector<int64_t> comp_m;            // m (units) at the current level
+    std::vector<int64_t> comp_voff;         // current level: first vertex of each component
+    std::vector<std::vector<lv_level_info>> comp_rec;  // per component, per level
+    int32_t *vcomp = nullptr;               // component of each vertex, current level
+    int32_t *d_vcomp0 = nullptr;            // component of each level-0 vertex
+    int32_t *d_cmode = nullptr;
+    double *d_cm2 = nullptr, *d_cgam = nullptr;
+    QAcc *d_cq = nullptr;
+    ull *d_ccnt = nullptr;
+    QAcc *h_cq = nullptr;                   // pinned, ncomp
+    ull *h_ccnt = nullptr;                  // pinned, ncomp
+    int32_t h_cap = 0;                      // capacity of the pinned arrays
     // timing
     lv_stats stats;
     std::vector<hipEvent_t> ev_pool;
@@ -248,6 +273,7 @@ void free_level_arrays(lv_ctx *c)
     arena_free(a, c->g_hscan);
     arena_free(a, c->g_start);
     arena_free(a, c->g_newid);
+    arena_free(a, c->vcomp);
 }
 
 void free_dendrogram(lv_ctx *c)
@@ -255,6 +281,8 @@ void free_dendrogram(lv_ctx *c)
     for (auto &L : c->levels) {
         arena_free(c->arena, L.perm);
         arena_free(c->arena, L.newc);
+        arena_free(c->arena, L.koff);
+        arena_free(c->arena, L.vnext);
     }
     c->levels.clear();
     arena_free(c->arena, c->d_final);
@@ -268,6 +296,25 @@ void free_input(lv_ctx *c)
     arena_free(c->arena, c->d_dst);
     arena_free(c->arena, c->d_w);
     c->loaded = false;
+    c->batch = false;
+    c->ncomp = 0;
+    c->comp_n0.clear();
+    c->comp_e0.clear();
+    c->comp_gamma.clear();
+    c->comp_m0.clear();
+    c->comp_scale.clear();
+}
+
+void free_batch_run(lv_ctx *c)
+{
+    Arena &a = c->arena;
+    arena_free(a, c->d_vcomp0);
+    arena_free(a, c->d_cmode);
+    arena_free(a, c->d_cm2);
+    arena_free(a, c->d_cgam);
+    arena_free(a, c->d_cq);
+    arena_free(a, c->d_ccnt);
+    c->comp_rec.clear();
 }
 
 // Resets per-run state to "input loaded, level 0".
@@ -275,6 +322,7 @@ void reset_run(lv_ctx *c)
 {
     free_level_arrays(c);
     free_dendrogram(c);
+    free_batch_run(c);
     c->cur_pass_q.clear();
     c->cur_trace.clear();
     c->level = 0;
@@ -503,7 +551,7 @@ int one_level(lv_ctx *c, double lower, double *q_out, int32_t *passes_out)
     c->cur_pass_q.clear();
     c->cur_trace.clear();
     const bool trace = (c->cfg.flags & LV_FLAG_TRACE) != 0;
-    SweepArgs a;
+    SweepArgs a{};
     a.lab = c->lab;
     a.rowptr = c->rowptr;
     a.col = c->col;
@@ -538,10 +586,10 @@ int one_level(lv_ctx *c, double lower, double *q_out, int32_t *passes_out)
             if (nv == 0) continue;
             hipEvent_t t0 = t_begin(c);
             if (t < kTiers)
-                launch_sweep_tier(s, t, a, c->verts + c->tier_off[t], (int32_t)nv);
+                launch_sweep_tier(s, t, a, c->verts + c->tier_off[t], (int32_t)nv, false);
             else
                 LV_TRY(run_hub_sweep(c->arena, s, a, c->hub, c->n_hubs, c->hub_parts,
-                                     c->hub_chunks));
+                                     c->hub_chunks, false));
             t_end(c, t0, CAT_SWEEP, t, 12.0 * (double)c->tier_nnz[t] + 24.0 * (double)nv,
                   c->tier_nnz[t]);
         }
@@ -646,7 +694,9 @@ int renumber(lv_ctx *c, int32_t *k_out)
 // inducedGraphCUDA: gather each community's entries in member order, aggregate
 // every community segment by neighbour community, copy the unique entries to
 // row newid(community) of the next level.
-int induce(lv_ctx *c)
+// newc/newid_g: next-level vertex id of every position / group (-1 for the
+// groups of a graph set's stopped components), Kn: vertices of the next level.
+int induce(lv_ctx *c, const int32_t *newc, const int32_t *newid_g, int32_t Kn)
 {
     Arena &A = c->arena;
     hipStream_t s = c->stream;
@@ -680,7 +730,7 @@ int induce(lv_ctx *c)
     fill_i64(s, (int64_t *)internal, K, 0);
     fill_i32(s, nuniq, K, 0);
     launch_cr_selfl(s, L.perm, c->selfsum, c->g_head, c->g_hscan, n, selfl);
-    launch_cr_gather(s, estart, bound, mstart, c->col, c->cw, L.newc, n, nnz, ey, ew);
+    launch_cr_gather(s, estart, bound, mstart, c->col, c->cw, newc, n, nnz, ey, ew);
     // segments by length
     int64_t *sb = nullptr, *slen = nullptr;
     uint32_t *tk1 = nullptr, *tk2 = nullptr;
@@ -692,7 +742,7 @@ int induce(lv_ctx *c)
     LV_TRY(arena_alloc(A, sid1, K));
     LV_TRY(arena_alloc(A, sid2, K));
     LV_TRY(zero_small(c, kTiers + 1));
-    launch_cr_seg_tier(s, c->g_start, estart, K, sb, slen, tk1, sid1, c->d_small);
+    launch_cr_seg_tier(s, c->g_start, estart, newid_g, K, sb, slen, tk1, sid1, c->d_small);
     uint32_t *tko = nullptr;
     int32_t *segs = nullptr;
     LV_TRY(radix_sort_u32_i32(A, s, tk1, sid1, tk2, sid2, K, 4, &tko, &segs));
@@ -709,7 +759,7 @@ int induce(lv_ctx *c)
     ca.ew = ew;
     ca.sb = sb;
     ca.slen = slen;
-    ca.newid_g = c->g_newid;
+    ca.newid_g = newid_g;
     ca.uy = uy;
     ca.uw = uw;
     ca.nuniq = nuniq;
@@ -773,16 +823,16 @@ int induce(lv_ctx *c)
     int64_t *rowcnt = nullptr, *rowptr2 = nullptr, *kdeg2 = nullptr, *loop2 = nullptr,
             *self2 = nullptr, *cw2 = nullptr;
     int32_t *col2 = nullptr;
-    LV_TRY(arena_alloc(A, rowcnt, (int64_t)K + 1));
-    LV_TRY(arena_alloc(A, rowptr2, (int64_t)K + 1));
-    LV_TRY(arena_alloc(A, kdeg2, K));
-    LV_TRY(arena_alloc(A, loop2, K));
-    LV_TRY(arena_alloc(A, self2, K));
-    launch_cr_rowcnt(s, c->g_newid, nuniq, K, rowcnt);
-    LV_TRY(scan_exclusive_i64(A, s, rowcnt, rowptr2, K, rowptr2 + K));
+    LV_TRY(arena_alloc(A, rowcnt, (int64_t)Kn + 1));
+    LV_TRY(arena_alloc(A, rowptr2, (int64_t)Kn + 1));
+    LV_TRY(arena_alloc(A, kdeg2, Kn));
+    LV_TRY(arena_alloc(A, loop2, Kn));
+    LV_TRY(arena_alloc(A, self2, Kn));
+    launch_cr_rowcnt(s, newid_g, nuniq, K, rowcnt);
+    LV_TRY(scan_exclusive_i64(A, s, rowcnt, rowptr2, Kn, rowptr2 + Kn));
     fill_i64(s, (int64_t *)c->d_small + 1, 1, 0);
-    launch_cr_vertex(s, c->g_newid, ssum, internal, selfl, K, loop2, self2, kdeg2, c->d_small + 1);
-    LV_TRY(d2h(c, c->h_small, (const long long *)(rowptr2 + K), 1));
+    launch_cr_vertex(s, newid_g, ssum, internal, selfl, K, loop2, self2, kdeg2, c->d_small + 1);
+    LV_TRY(d2h(c, c->h_small, (const long long *)(rowptr2 + Kn), 1));
     LV_TRY(d2h(c, c->h_small + 1, c->d_small + 1, 1));
     LV_TRY(sync(c));
     const int64_t nnz2 = c->h_small[0];
@@ -805,7 +855,7 @@ int induce(lv_ctx *c)
     c->kdeg = kdeg2;
     c->loopw = loop2;
     c->selfsum = self2;
-    c->n = K;
+    c->n = Kn;
     c->nnz = nnz2;
     c->m = m2;
     c->level += 1;
@@ -823,6 +873,295 @@ int flatten_final(lv_ctx *c)
     LV_HIP(hipGetLastError());
     t_end(c, t0, CAT_FLATTEN);
     c->final_valid = true;
+    return LV_OK;
+}
+
+// ---------------------------------------------------------------------------
+// graph sets: the level loop of every component, run on the union graph
+
+int pinned_cap(lv_ctx *c, int32_t need)
+{
+    if (c->h_cap >= need) return LV_OK;
+    if (c->h_cq) (void)hipHostFree(c->h_cq);
+    if (c->h_ccnt) (void)hipHostFree(c->h_ccnt);
+    c->h_cq = nullptr;
+    c->h_ccnt = nullptr;
+    c->h_cap = 0;
+    if (hipHostMalloc(&c->h_cq, sizeof(QAcc) * (size_t)need, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc(&c->h_ccnt, sizeof(ull) * (size_t)need, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        return LV_ERR_NOMEM;
+    }
+    c->h_cap = need;
+    return LV_OK;
+}
+
+// Component of every vertex of the current level from comp_voff, each
+// component's m at levels above 0 (from the coarsened k and loops), and the
+// per-component 2m / resolution tables.
+int batch_level_setup(lv_ctx *c, int32_t level)
+{
+    Arena &A = c->arena;
+    hipStream_t s = c->stream;
+    const int32_t nc = c->ncomp;
+    int64_t *dvoff = nullptr;
+    LV_TRY(arena_alloc(A, dvoff, (int64_t)nc + 1));
+    LV_TRY(h2d(c, dvoff, (const int64_t *)c->comp_voff.data(), (int64_t)nc + 1));
+    LV_TRY(arena_alloc(A, c->vcomp, c->n));
+    launch_expand_owner(s, dvoff, nc, c->n, c->vcomp, nullptr);
+    if (level > 0) {
+        fill_i64(s, (int64_t *)c->d_ccnt, nc, 0);
+        launch_b_vertex_sum(s, c->kdeg, c->loopw, c->vcomp, c->n, c->d_ccnt);
+        LV_HIP(hipGetLastError());
+        LV_TRY(d2h(c, c->h_ccnt, c->d_ccnt, nc));
+        LV_TRY(sync(c));
+        for (int32_t k = 0; k < nc; ++k)
+            if (c->comp_voff[k + 1] > c->comp_voff[k]) c->comp_m[k] = (int64_t)(c->h_ccnt[k] / 2);
+    }
+    std::vector<double> m2((size_t)nc), gam((size_t)nc);
+    for (int32_t k = 0; k < nc; ++k) {
+        m2[k] = 2.0 * (double)c->comp_m[k];
+        gam[k] = c->comp_gamma[k];
+    }
+    LV_TRY(h2d(c, c->d_cm2, (const double *)m2.data(), nc));
+    LV_TRY(h2d(c, c->d_cgam, (const double *)gam.data(), nc));
+    LV_TRY(sync(c));  // the host vectors go out of scope
+    arena_free(A, dvoff);
+    return LV_OK;
+}
+
+// Exact Q aggregates per component (apply: add the pass deltas first).
+int batch_qagg(lv_ctx *c, bool apply)
+{
+    fill_i64(c->stream, (int64_t *)c->d_cq, 4 * (int64_t)c->ncomp, 0);
+    launch_b_qagg(c->stream, c->n, c->deg, c->in, c->size, c->ddeg, c->din, c->dsize, c->vcomp,
+                  apply, c->d_cq);
+    LV_HIP(hipGetLastError());
+    LV_TRY(d2h(c, c->h_cq, c->d_cq, c->ncomp));
+    return sync(c);
+}
+
+double batch_q(const lv_ctx *c, int32_t k)
+{
+    const QAcc &a = c->h_cq[k];
+    return q_from_aggregates(a.sum_in, a.d2_lo, a.d2_hi, c->comp_m[k], c->comp_gamma[k]);
+}
+
+// oneLevel for every present component: the same pass rule as one_level, with
+// each component's break decided from its own Q; components that have
+// finished (or have no edges) are frozen while the others sweep.
+int batch_one_level(lv_ctx *c, const std::vector<double> &lower, const std::vector<uint8_t> &present,
+                    std::vector<double> &q_level, std::vector<int32_t> &passes)
+{
+    hipStream_t s = c->stream;
+    const int32_t nc = c->ncomp;
+    LV_TRY(batch_qagg(c, false));
+    std::vector<double> q_prev((size_t)nc, 0.0), curQ((size_t)nc, -1.0);
+    std::vector<uint8_t> done((size_t)nc, 1);
+    std::vector<int32_t> mode((size_t)nc, 0);
+    q_level.assign((size_t)nc, 0.0);
+    passes.assign((size_t)nc, 0);
+    for (int32_t k = 0; k < nc; ++k) {
+        if (!present[k] || c->comp_e0[k] == 0) continue;
+        done[k] = 0;
+        q_prev[k] = batch_q(c, k);
+    }
+    SweepArgs a{};
+    a.lab = c->lab;
+    a.rowptr = c->rowptr;
+    a.col = c->col;
+    a.w = c->cw;
+    a.kdeg = c->kdeg;
+    a.loopw = c->loopw;
+    a.selfsum = c->selfsum;
+    a.deg = c->deg;
+    a.size = c->size;
+    a.ddeg = c->ddeg;
+    a.din = c->din;
+    a.dsize = c->dsize;
+    a.vcomp = c->vcomp;
+    a.cmode = c->d_cmode;
+    a.cm2 = c->d_cm2;
+    a.cgam = c->d_cgam;
+    int32_t pass = 0;
+    for (;;) {
+        bool any = false;
+        for (int32_t k = 0; k < nc; ++k) {
+            if (done[k]) {
+                mode[k] = 0;
+                continue;
+            }
+            any = true;
+            passes[k] += 1;
+            const bool last = (q_prev[k] - curQ[k]) < c->cfg.min_gain ||
+                              (c->cfg.max_passes > 0 && passes[k] >= c->cfg.max_passes);
+            mode[k] = last ? 2 : 1;
+        }
+        if (!any) break;
+        ++pass;
+        LV_TRY(h2d(c, c->d_cmode, (const int32_t *)mode.data(), nc));
+        a.C = c->C;
+        a.next = c->next;
+        hipEvent_t tl = t_begin(c);
+        launch_lab(s, c->n, c->C, c->size, c->deg, c->lab);
+        t_end(c, tl, CAT_APPLY);
+        for (int t = 1; t <= kTiers; ++t) {
+            const int64_t nv = c->tier_cnt[t];
+            if (nv == 0) continue;
+            hipEvent_t t0 = t_begin(c);
+            if (t < kTiers)
+                launch_sweep_tier(s, t, a, c->verts + c->tier_off[t], (int32_t)nv, true);
+            else
+                LV_TRY(run_hub_sweep(c->arena, s, a, c->hub, c->n_hubs, c->hub_parts,
+                                     c->hub_chunks, true));
+            t_end(c, t0, CAT_SWEEP, t, 12.0 * (double)c->tier_nnz[t] + 24.0 * (double)nv,
+                  c->tier_nnz[t]);
+        }
+        LV_HIP(hipGetLastError());
+        hipEvent_t t1 = t_begin(c);
+        LV_TRY(batch_qagg(c, true));  // synchronises: mode[] may be rewritten below
+        t_end(c, t1, CAT_APPLY);
+        std::swap(c->C, c->next);
+        for (int32_t k = 0; k < nc; ++k) {
+            if (mode[k] == 0) continue;
+            const double q_new = batch_q(c, k);
+            if (mode[k] == 2) {
+                q_level[k] = q_new;
+                done[k] = 1;
+                continue;
+            }
+            curQ[k] = q_prev[k];
+            if (curQ[k] < lower[k]) curQ[k] = lower[k];
+            q_prev[k] = q_new;
+        }
+    }
+    c->stats.passes += pass;
+    return LV_OK;
+}
+
+// generate_dendogram for every component: the same level loop as lv_run
+// (lower, the level-0 exemption from the break test, max_levels), each
+// component deciding for itself; components that stop drop out of the union
+// at the next coarsening.
+int batch_run(lv_ctx *c)
+{
+    Arena &A = c->arena;
+    hipStream_t s = c->stream;
+    const int32_t nc = c->ncomp;
+    LV_TRY(pinned_cap(c, nc));
+    LV_TRY(arena_alloc(A, c->d_cmode, nc));
+    LV_TRY(arena_alloc(A, c->d_cm2, nc));
+    LV_TRY(arena_alloc(A, c->d_cgam, nc));
+    LV_TRY(arena_alloc(A, c->d_cq, nc));
+    LV_TRY(arena_alloc(A, c->d_ccnt, nc));
+    c->comp_rec.assign((size_t)nc, {});
+    c->comp_voff.assign((size_t)nc + 1, 0);
+    for (int32_t k = 0; k < nc; ++k) c->comp_voff[k + 1] = c->comp_voff[k] + c->comp_n0[k];
+    c->comp_m = c->comp_m0;
+    std::vector<uint8_t> present((size_t)nc, 1);
+    std::vector<double> mod((size_t)nc, 0.0), lower((size_t)nc, 0.0), q;
+    std::vector<int32_t> passes;
+    std::vector<int64_t> K((size_t)nc, 0);
+    LV_TRY(build_level0_csr(c));
+    for (int32_t level = 0;; ++level) {
+        LV_TRY(batch_level_setup(c, level));
+        if (level == 0) {
+            LV_TRY(arena_alloc(A, c->d_vcomp0, c->n));
+            copy_i32(s, c->d_vcomp0, c->vcomp, c->n);
+        }
+        LV_TRY(init_level(c));
+        if (level == 0) {
+            // lower of level 0 = Q of its initial state (LouvainCPP.cpp:356-366)
+            LV_TRY(batch_qagg(c, false));
+            for (int32_t k = 0; k < nc; ++k)
+                if (present[k] && c->comp_e0[k] > 0) lower[k] = batch_q(c, k);
+        } else {
+            lower = mod;
+        }
+        LV_TRY(batch_one_level(c, lower, present, q, passes));
+        LV_TRY(renumber(c, nullptr));
+        LevelRec &L = c->levels.back();
+        fill_i64(s, (int64_t *)c->d_ccnt, nc, 0);
+        launch_b_groups(s, c->g_start, L.perm, c->vcomp, L.k, c->d_ccnt);
+        LV_HIP(hipGetLastError());
+        LV_TRY(d2h(c, c->h_ccnt, c->d_ccnt, nc));
+        LV_TRY(sync(c));
+        L.voff_h = c->comp_voff;
+        L.koff_h.assign((size_t)nc + 1, 0);
+        for (int32_t k = 0; k < nc; ++k) {
+            K[k] = (int64_t)c->h_ccnt[k];
+            L.koff_h[k + 1] = L.koff_h[k] + K[k];
+        }
+        // per-component records and the level-loop decision
+        std::vector<int64_t> vnext((size_t)nc, -1), voff_next((size_t)nc + 1, 0);
+        int64_t kn = 0;
+        for (int32_t k = 0; k < nc; ++k) {
+            voff_next[k] = kn;
+            if (!present[k]) continue;
+            const bool edges = c->comp_e0[k] > 0;
+            lv_level_info r{};
+            r.n = (int32_t)(c->comp_voff[k + 1] - c->comp_voff[k]);
+            r.k = (int32_t)K[k];
+            r.nnz = 0;
+            r.m_units = edges ? c->comp_m[k] : 0;
+            r.passes = edges ? passes[k] : 0;
+            r.scale = c->comp_scale[k];
+            r.q = edges ? q[k] : 0.0;
+            r.lower = edges ? lower[k] : 0.0;
+            c->comp_rec[k].push_back(r);
+            bool cont = edges;
+            if (cont && level > 0 && q[k] - mod[k] < c->cfg.min_gain) cont = false;
+            if (cont) mod[k] = q[k];
+            if (c->cfg.max_levels > 0 && (int32_t)c->comp_rec[k].size() >= c->cfg.max_levels)
+                cont = false;
+            present[k] = cont ? 1 : 0;
+            if (cont) {
+                vnext[k] = kn;
+                kn += K[k];
+            }
+        }
+        voff_next[nc] = kn;
+        LV_TRY(arena_alloc(A, L.koff, (int64_t)nc + 1));
+        LV_TRY(arena_alloc(A, L.vnext, nc));
+        LV_TRY(h2d(c, L.koff, (const int64_t *)L.koff_h.data(), (int64_t)nc + 1));
+        LV_TRY(h2d(c, L.vnext, (const int64_t *)vnext.data(), nc));
+        if (kn == 0) {
+            LV_TRY(sync(c));
+            break;
+        }
+        if (kn > INT32_MAX) return LV_ERR_RANGE;
+        int32_t *newid_next = nullptr, *newc_next = nullptr;
+        LV_TRY(arena_alloc(A, newid_next, L.k));
+        LV_TRY(arena_alloc(A, newc_next, L.n));
+        launch_b_next_ids(s, c->g_start, L.perm, c->vcomp, c->g_newid, L.k, L.koff, L.vnext,
+                          newid_next);
+        launch_cr_newc(s, c->g_head, c->g_hscan, L.perm, newid_next, L.n, newc_next);
+        LV_TRY(induce(c, newc_next, newid_next, (int32_t)kn));
+        arena_free(A, newid_next);
+        arena_free(A, newc_next);
+        c->comp_voff = voff_next;
+        LV_TRY(sync(c));  // vnext / voff_next host buffers
+    }
+    // flatten every component through its own levels
+    hipEvent_t t0 = t_begin(c);
+    std::vector<int32_t> clev((size_t)nc);
+    for (int32_t k = 0; k < nc; ++k) clev[k] = (int32_t)c->comp_rec[k].size();
+    int32_t *dclev = nullptr;
+    LV_TRY(arena_alloc(A, dclev, nc));
+    LV_TRY(h2d(c, dclev, (const int32_t *)clev.data(), nc));
+    arena_free(A, c->d_final);
+    LV_TRY(arena_alloc(A, c->d_final, c->n0));
+    iota_i32(s, c->d_final, c->n0);
+    for (size_t l = 0; l < c->levels.size(); ++l)
+        launch_b_flatten(s, c->d_final, c->n0, c->d_vcomp0, dclev, (int32_t)l, c->levels[l].newc,
+                         c->levels[l].koff, c->levels[l].vnext);
+    LV_HIP(hipGetLastError());
+    t_end(c, t0, CAT_FLATTEN);
+    LV_TRY(sync(c));
+    arena_free(A, dclev);
+    c->final_valid = true;
+    free_level_arrays(c);
+    c->stats.levels += (int32_t)c->levels.size();
     return LV_OK;
 }
 
@@ -934,6 +1273,8 @@ int lv_destroy(lv_ctx *c)
     if (c->h_flags) (void)hipHostFree(c->h_flags);
     if (c->h_qacc) (void)hipHostFree(c->h_qacc);
     if (c->h_small) (void)hipHostFree(c->h_small);
+    if (c->h_cq) (void)hipHostFree(c->h_cq);
+    if (c->h_ccnt) (void)hipHostFree(c->h_ccnt);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;  // the arena destructor frees the HBM block
     return LV_OK;
@@ -1038,6 +1379,7 @@ int lv_neighbour_compute(lv_ctx *c)
 {
     LV_TRY(check_ctx(c));
     LV_TRY(set_device(c));
+    if (c->batch) return LV_ERR_ARG;  // graph sets run through lv_batch_run
     if (c->state == ST_LOADED && c->level == 0) {
         if (c->e0 == 0) return LV_ERR_ARG;  // no edges: use lv_run (identity partition)
         LV_TRY(build_level0_csr(c));
@@ -1093,8 +1435,8 @@ int lv_induce(lv_ctx *c)
 {
     LV_TRY(check_ctx(c));
     LV_TRY(set_device(c));
-    if (c->state != ST_RENUMBERED) return LV_ERR_ARG;
-    LV_TRY(induce(c));
+    if (c->state != ST_RENUMBERED || c->batch) return LV_ERR_ARG;
+    LV_TRY(induce(c, c->levels.back().newc, c->g_newid, c->levels.back().k));
     c->state = ST_CSR;
     return LV_OK;
 }
@@ -1102,7 +1444,7 @@ int lv_induce(lv_ctx *c)
 int lv_run(lv_ctx *c, int32_t *levels_out)
 {
     LV_TRY(check_ctx(c));
-    if (!c->loaded) return LV_ERR_ARG;
+    if (!c->loaded || c->batch) return LV_ERR_ARG;
     LV_TRY(set_device(c));
     auto t_start = std::chrono::steady_clock::now();
     LV_HIP(hipStreamSynchronize(c->stream));
@@ -1142,7 +1484,7 @@ int lv_run(lv_ctx *c, int32_t *levels_out)
     // later levels, :420-507
     for (;;) {
         if (c->cfg.max_levels > 0 && (int32_t)c->levels.size() >= c->cfg.max_levels) break;
-        LV_TRY(induce(c));
+        LV_TRY(induce(c, c->levels.back().newc, c->g_newid, c->levels.back().k));
         c->state = ST_CSR;
         LV_TRY(init_level(c));
         c->state = ST_INIT;
@@ -1355,6 +1697,239 @@ int lv_last_run_ms(lv_ctx *c, double *ms)
     if (!ms) return LV_ERR_ARG;
     *ms = c->last_run_ms;
     return LV_OK;
+}
+
+// ---------------------------------------------------------------------------
+// graph sets
+
+int lv_batch_load(lv_ctx *c, int32_t ngraphs, const int32_t *n, const int64_t *e,
+                  const int32_t *src, const int32_t *dst, const void *w, int wtype, int32_t nres,
+                  const double *gamma)
+{
+    LV_TRY(check_ctx(c));
+    if (ngraphs < 1 || nres < 1 || !n || !e || !gamma ||
+        (wtype != LV_WEIGHT_INT32 && wtype != LV_WEIGHT_FLOAT32))
+        return LV_ERR_ARG;
+    if ((int64_t)ngraphs * nres > INT32_MAX) return LV_ERR_ARG;
+    int64_t nb = 0, eb = 0;
+    for (int32_t g = 0; g < ngraphs; ++g) {
+        if (n[g] < 0 || e[g] < 0) return LV_ERR_ARG;
+        nb += n[g];
+        eb += e[g];
+    }
+    for (int32_t r = 0; r < nres; ++r)
+        if (!(gamma[r] >= 0.0)) return LV_ERR_ARG;
+    if (eb > 0 && (!src || !dst || !w)) return LV_ERR_ARG;
+    const int64_t nu = nb * nres, eu = eb * nres;
+    if (nu > INT32_MAX) return LV_ERR_RANGE;
+    LV_TRY(set_device(c));
+    LV_HIP(hipStreamSynchronize(c->stream));
+    reset_run(c);
+    free_input(c);
+    c->state = ST_EMPTY;
+    Arena &A = c->arena;
+    hipStream_t s = c->stream;
+    const int32_t nc = ngraphs * nres;
+    if (!c->cfg.arena_bytes) {
+        size_t want = (size_t)200 * (size_t)eu + (size_t)320 * (size_t)nu + (size_t)16 * (size_t)eb +
+                      (size_t)96 * (size_t)nc + ((size_t)256 << 20);
+        size_t freeb = 0, totalb = 0;
+        if (hipMemGetInfo(&freeb, &totalb) == hipSuccess) {
+            size_t avail = (size_t)((double)(freeb + A.capacity()) * 0.92);
+            if (want > avail) want = avail;
+        }
+        if (A.capacity() < want) {
+            A.release_all();
+            int rc = A.reserve(want);
+            if (rc != LV_OK) return rc;
+        }
+    }
+    // offsets of the graphs and of the union's components
+    std::vector<int64_t> beoff((size_t)ngraphs + 1, 0), ueoff((size_t)nc + 1, 0),
+        uvoff((size_t)nc + 1, 0);
+    for (int32_t g = 0; g < ngraphs; ++g) beoff[g + 1] = beoff[g] + e[g];
+    c->comp_n0.assign((size_t)nc, 0);
+    c->comp_e0.assign((size_t)nc, 0);
+    c->comp_gamma.assign((size_t)nc, 1.0);
+    for (int32_t k = 0; k < nc; ++k) {
+        const int32_t g = k / nres;
+        c->comp_n0[k] = n[g];
+        c->comp_e0[k] = e[g];
+        c->comp_gamma[k] = gamma[k % nres];
+        ueoff[k + 1] = ueoff[k] + e[g];
+        uvoff[k + 1] = uvoff[k] + n[g];
+    }
+    int32_t *bsrc = nullptr, *bdst = nullptr, *bn = nullptr;
+    uint32_t *bw = nullptr;
+    int *gscale = nullptr;
+    int64_t *dbeoff = nullptr, *dueoff = nullptr, *duvoff = nullptr;
+    LV_TRY(arena_alloc(A, bsrc, eb));
+    LV_TRY(arena_alloc(A, bdst, eb));
+    LV_TRY(arena_alloc(A, bw, eb));
+    LV_TRY(arena_alloc(A, bn, ngraphs));
+    LV_TRY(arena_alloc(A, gscale, ngraphs));
+    LV_TRY(arena_alloc(A, dbeoff, (int64_t)ngraphs + 1));
+    LV_TRY(arena_alloc(A, dueoff, (int64_t)nc + 1));
+    LV_TRY(arena_alloc(A, duvoff, (int64_t)nc + 1));
+    LV_TRY(arena_alloc(A, c->d_src, eu));
+    LV_TRY(arena_alloc(A, c->d_dst, eu));
+    LV_TRY(arena_alloc(A, c->d_w, eu));
+    LV_TRY(h2d(c, bsrc, src, eb));
+    LV_TRY(h2d(c, bdst, dst, eb));
+    LV_TRY(h2d(c, bw, (const uint32_t *)w, eb));
+    LV_TRY(h2d(c, bn, n, ngraphs));
+    LV_TRY(h2d(c, dbeoff, (const int64_t *)beoff.data(), (int64_t)ngraphs + 1));
+    LV_TRY(h2d(c, dueoff, (const int64_t *)ueoff.data(), (int64_t)nc + 1));
+    LV_TRY(h2d(c, duvoff, (const int64_t *)uvoff.data(), (int64_t)nc + 1));
+    LV_HIP(hipMemsetAsync(c->d_flags, 0, sizeof(DevFlags), s));
+    fill_i32(s, gscale, ngraphs, 0);
+    launch_b_graph_exponent(s, bw, wtype, dbeoff, ngraphs, eb, gscale, c->d_flags);
+    launch_b_union_edges(s, bsrc, bdst, bw, wtype, gscale, dbeoff, bn, dueoff, duvoff, nc, nres, eu,
+                         c->d_src, c->d_dst, c->d_w, c->d_flags);
+    LV_HIP(hipGetLastError());
+    std::vector<int> gs((size_t)ngraphs, 0);
+    LV_TRY(d2h(c, gs.data(), gscale, ngraphs));
+    LV_TRY(d2h(c, c->h_flags, c->d_flags, 1));
+    LV_TRY(sync(c));  // the host offset vectors are read by the copies above
+    arena_free(A, bsrc);
+    arena_free(A, bdst);
+    arena_free(A, bw);
+    arena_free(A, bn);
+    arena_free(A, gscale);
+    arena_free(A, dbeoff);
+    if (c->h_flags->err) {
+        arena_free(A, dueoff);
+        arena_free(A, duvoff);
+        free_input(c);
+        return (c->h_flags->err & 1) ? LV_ERR_INVALID_INPUT : LV_ERR_RANGE;
+    }
+    int scale = 0;
+    c->comp_scale.assign((size_t)nc, 0);
+    for (int32_t k = 0; k < nc; ++k) {
+        c->comp_scale[k] = gs[k / nres];
+        scale = std::max(scale, c->comp_scale[k]);
+    }
+    // m of every component (exact units)
+    LV_TRY(pinned_cap(c, nc));
+    ull *dcm = nullptr;
+    LV_TRY(arena_alloc(A, dcm, nc));
+    fill_i64(s, (int64_t *)dcm, nc, 0);
+    launch_b_edge_sum(s, c->d_w, dueoff, nc, eu, dcm);
+    LV_HIP(hipGetLastError());
+    LV_TRY(d2h(c, c->h_ccnt, dcm, nc));
+    LV_TRY(sync(c));
+    c->comp_m0.assign((size_t)nc, 0);
+    for (int32_t k = 0; k < nc; ++k) c->comp_m0[k] = (int64_t)c->h_ccnt[k];
+    arena_free(A, dcm);
+    arena_free(A, dueoff);
+    arena_free(A, duvoff);
+    c->scale = scale;
+    c->m0 = 0;
+    for (int32_t k = 0; k < nc; ++k) c->m0 += c->comp_m0[k];
+    c->max_units0 = c->h_flags->max_units;
+    c->n0 = (int32_t)nu;
+    c->e0 = eu;
+    c->ncomp = nc;
+    c->nres = nres;
+    c->batch = true;
+    c->loaded = true;
+    c->level = 0;
+    c->state = ST_LOADED;
+    return LV_OK;
+}
+
+int lv_batch_run(lv_ctx *c)
+{
+    LV_TRY(check_ctx(c));
+    if (!c->loaded || !c->batch) return LV_ERR_ARG;
+    LV_TRY(set_device(c));
+    auto t_start = std::chrono::steady_clock::now();
+    LV_HIP(hipStreamSynchronize(c->stream));
+    reset_run(c);
+    if (c->e0 == 0) {
+        // no component has edges: every one is its identity partition
+        const int32_t nc = c->ncomp;
+        c->comp_rec.assign((size_t)nc, {});
+        std::vector<int32_t> fin((size_t)c->n0);
+        int64_t off = 0;
+        for (int32_t k = 0; k < nc; ++k) {
+            lv_level_info r{};
+            r.n = r.k = c->comp_n0[k];
+            r.scale = c->comp_scale[k];
+            c->comp_rec[k].push_back(r);
+            for (int32_t i = 0; i < c->comp_n0[k]; ++i) fin[(size_t)(off + i)] = i;
+            off += c->comp_n0[k];
+        }
+        LV_TRY(arena_alloc(c->arena, c->d_final, c->n0));
+        LV_TRY(h2d(c, c->d_final, (const int32_t *)fin.data(), c->n0));
+        LV_TRY(sync(c));
+        c->final_valid = true;
+    } else {
+        LV_TRY(batch_run(c));
+    }
+    c->state = ST_DONE;
+    c->last_run_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    return LV_OK;
+}
+
+int lv_batch_num_components(lv_ctx *c, int32_t *ncomp)
+{
+    LV_TRY(check_ctx(c));
+    if (!ncomp || !c->batch) return LV_ERR_ARG;
+    *ncomp = c->ncomp;
+    return LV_OK;
+}
+
+int lv_batch_num_levels(lv_ctx *c, int32_t comp, int32_t *levels)
+{
+    LV_TRY(check_ctx(c));
+    if (!levels || !c->batch || !c->final_valid || comp < 0 || comp >= c->ncomp) return LV_ERR_ARG;
+    *levels = (int32_t)c->comp_rec[comp].size();
+    return LV_OK;
+}
+
+int lv_batch_level_info(lv_ctx *c, int32_t comp, int32_t level, lv_level_info *out)
+{
+    LV_TRY(check_ctx(c));
+    if (!out || !c->batch || !c->final_valid || comp < 0 || comp >= c->ncomp) return LV_ERR_ARG;
+    if (level < 0 || level >= (int32_t)c->comp_rec[comp].size()) return LV_ERR_ARG;
+    *out = c->comp_rec[comp][level];
+    return LV_OK;
+}
+
+int lv_batch_level_assignment(lv_ctx *c, int32_t comp, int32_t level, int32_t *comm)
+{
+    LV_TRY(check_ctx(c));
+    if (!comm || !c->batch || !c->final_valid || comp < 0 || comp >= c->ncomp) return LV_ERR_ARG;
+    if (level < 0 || level >= (int32_t)c->comp_rec[comp].size()) return LV_ERR_ARG;
+    LV_TRY(set_device(c));
+    const int32_t nv = c->comp_rec[comp][level].n;
+    if (nv == 0) return LV_OK;
+    if (c->levels.empty()) {  // edgeless set: identity
+        for (int32_t i = 0; i < nv; ++i) comm[i] = i;
+        return LV_OK;
+    }
+    const LevelRec &L = c->levels[level];
+    LV_TRY(d2h(c, comm, L.newc + L.voff_h[comp], nv));
+    LV_TRY(sync(c));
+    const int32_t ko = (int32_t)L.koff_h[comp];
+    for (int32_t i = 0; i < nv; ++i) comm[i] -= ko;
+    return LV_OK;
+}
+
+int lv_batch_final_assignment(lv_ctx *c, int32_t comp, int32_t *comm)
+{
+    LV_TRY(check_ctx(c));
+    if (!comm || !c->batch || !c->final_valid || comp < -1 || comp >= c->ncomp) return LV_ERR_ARG;
+    LV_TRY(set_device(c));
+    if (comp < 0) {
+        LV_TRY(d2h(c, comm, c->d_final, c->n0));
+        return sync(c);
+    }
+    int64_t off = 0;
+    for (int32_t k = 0; k < comp; ++k) off += c->comp_n0[k];
+    LV_TRY(d2h(c, comm, c->d_final + off, c->comp_n0[comp]));
+    return sync(c);
 }
 
 }  // extern "C"

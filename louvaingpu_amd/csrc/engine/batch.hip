@@ -1,0 +1,331 @@
+// batch.hip - graph sets (gfx950): many independent graphs, each possibly at
+// several resolutions, run as the components of one union graph.
+//
+// Component c is graph c / nres at resolution c % nres.  Its vertices form one
+// contiguous id range of every level's union graph and no edge crosses two
+// components, so the sweep, renumbering and coarsening kernels of the single
+// graph serve the set unchanged; what is per component is 2m, the resolution,
+// the pass mode (sweep / last pass / frozen), the exact Q aggregates and the
+// level-loop decision.  Renumbering the union by first occurrence numbers each
+// component's communities consecutively, in component order, which is every
+// component's own first-occurrence numbering shifted by an offset (koff).
+#include "device.hpp"
+
+namespace lv {
+
+// Unit exponent of every graph of the set: gscale[g] = the smallest s making
+// all of graph g's weights integral in units of 2^-s (so every component
+// computes in the units a single-graph run of its graph would use); negative,
+// infinite or NaN weights flag invalid input.
+__global__ void k_b_graph_exponent(const uint32_t *bw, int wtype, const int64_t *beoff,
+                                   int32_t ngraphs, int64_t eb, int *gscale, DevFlags *f)
+{
+    int bad = 0;
+    const int lane = threadIdx.x & 63;
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < eb;
+         base += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = base + threadIdx.x;
+        const bool valid = i < eb;
+        int32_t g = INT_MIN;
+        int sc = 0;
+        if (valid) {
+            int32_t lo = 0, hi = ngraphs;  // beoff[lo] <= i < beoff[hi]
+            while (hi - lo > 1) {
+                const int32_t mid = (lo + hi) >> 1;
+                if (beoff[mid] <= i) lo = mid; else hi = mid;
+            }
+            g = lo;
+            if (wtype == LV_WEIGHT_INT32) {
+                if ((int32_t)bw[i] < 0) bad = 1;
+            } else {
+                sc = f32_unit_exponent(bw[i]);
+                if (sc < 0) {
+                    bad = 1;
+                    sc = 0;
+                }
+            }
+        }
+        // runs of equal graphs (edges are grouped by graph): max, one atomic per run
+        for (int off = 1; off < 64; off <<= 1) {
+            const int os = __shfl_up(sc, off, 64);
+            const int32_t og = __shfl_up(g, off, 64);
+            if (lane >= off && og == g) sc = max(sc, os);
+        }
+        const int32_t ng = __shfl_down(g, 1, 64);
+        if (valid && (lane == 63 || ng != g) && sc > 0) atomicMax(&gscale[g], sc);
+    }
+    bad = wave_max_i(bad);
+    if (lane == 0 && bad) atomicOr(&f->err, 1);
+}
+
+// Union edge list: union edge j of component c is edge j - ueoff[c] of graph
+// g = c / nres, with endpoints shifted by uvoff[c] and its weight in graph g's
+// units; endpoints outside [0, n_g) flag invalid input, weights beyond the
+// exact range flag a range error.
+__global__ void k_b_union_edges(const int32_t *bsrc, const int32_t *bdst, const uint32_t *bw,
+                                int wtype, const int *gscale, const int64_t *beoff,
+                                const int32_t *bn, const int64_t *ueoff, const int64_t *uvoff,
+                                int32_t ncomp, int32_t nres, int64_t etot, int32_t *usrc,
+                                int32_t *udst, int64_t *uw, DevFlags *f)
+{
+    int bad = 0;
+    ull tot = 0, umax = 0;
+    GRID_STRIDE(j, etot)
+    {
+        int32_t lo = 0, hi = ncomp;  // ueoff[lo] <= j < ueoff[hi]
+        while (hi - lo > 1) {
+            const int32_t mid = (lo + hi) >> 1;
+            if (ueoff[mid] <= j) lo = mid; else hi = mid;
+        }
+        const int32_t g = lo / nres;
+        const int64_t i = beoff[g] + (j - ueoff[lo]);
+        const int32_t a = bsrc[i], b = bdst[i], ng = bn[g];
+        if (a < 0 || a >= ng || b < 0 || b >= ng) bad = 1;
+        usrc[j] = a + (int32_t)uvoff[lo];
+        udst[j] = b + (int32_t)uvoff[lo];
+        const int64_t u = weight_units(bw[i], wtype, gscale[g], bad);
+        uw[j] = u;
+        umax = max(umax, (ull)u);
+        tot += (ull)(u == 0 ? 1 : u);
+        if (tot >= (ull)kUnitLimit) bad = 2;
+    }
+    tot = (ull)wave_sum((long long)tot);
+    bad = wave_max_i(bad);
+    for (int off = 32; off > 0; off >>= 1) umax = max(umax, __shfl_xor(umax, off, 64));
+    if ((threadIdx.x & 63) == 0) {
+        if (tot) {
+            ull old = atomicAdd((ull *)&f->total_units, tot);
+            if (old + tot >= (ull)kUnitLimit || tot >= (ull)kUnitLimit) bad |= 2;
+        }
+        if (umax) atomicMax(&f->max_units, umax);
+        if (bad) atomicOr(&f->err, bad);
+    }
+}
+
+// cm[c] += sum of w over the union edges of component c (edge ranges ueoff).
+__global__ void k_b_edge_sum(const int64_t *w, const int64_t *ueoff, int32_t ncomp, int64_t etot,
+                             ull *cm)
+{
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < etot;
+         base += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = base + threadIdx.x;
+        const bool valid = j < etot;
+        long long key = 0;
+        ull v = 0;
+        if (valid) {
+            int32_t lo = 0, hi = ncomp;
+            while (hi - lo > 1) {
+                const int32_t mid = (lo + hi) >> 1;
+                if (ueoff[mid] <= j) lo = mid; else hi = mid;
+            }
+            key = lo;
+            v = (ull)w[j];
+        }
+        seg_atomic_add(valid, key, v, cm);
+    }
+}
+
+// Wave-segmented sum of (sin, 128-bit d2) over lanes with equal, sorted keys;
+// the last lane of each run adds the run into cq[key].
+__device__ __forceinline__ void seg_qacc_add(bool valid, int32_t key, long long sin, ull lo, ull hi,
+                                             QAcc *cq)
+{
+    const int lane = threadIdx.x & 63;
+    const int32_t k = valid ? key : INT_MIN;
+    if (!valid) {
+        sin = 0;
+        lo = 0;
+        hi = 0;
+    }
+    for (int off = 1; off < 64; off <<= 1) {
+        const long long os = __shfl_up(sin, off, 64);
+        const ull ol = __shfl_up(lo, off, 64), oh = __shfl_up(hi, off, 64);
+        const int32_t ok = __shfl_up(k, off, 64);
+        if (lane >= off && ok == k) {
+            sin += os;
+            lo += ol;
+            hi += oh + (lo < ol ? 1ull : 0ull);
+        }
+    }
+    const int32_t nk = __shfl_down(k, 1, 64);
+    const bool tail = (lane == 63) || (nk != k);
+    if (valid && tail) {
+        if (sin) atomicAdd((ull *)&cq[k].sum_in, (ull)sin);
+        if (lo | hi) atomic_add_u128(&cq[k].d2_lo, &cq[k].d2_hi, lo, hi);
+    }
+}
+
+// Per component Q aggregates of the community state (sum of internals, sum of
+// deg^2); with `apply` the pass deltas are added first (k_apply per component).
+__global__ __launch_bounds__(256) void k_b_qagg(int32_t n, int64_t *deg, int64_t *in, int32_t *size,
+                                               ull *ddeg, ull *din, int32_t *dsize,
+                                               const int32_t *vcomp, int apply, QAcc *cq)
+{
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n;
+         base += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t c = base + threadIdx.x;
+        const bool valid = c < n;
+        int32_t key = 0;
+        long long sin = 0;
+        ull lo = 0, hi = 0;
+        if (valid) {
+            int64_t dv = deg[c], iv = in[c];
+            if (apply) {
+                const ull dd = ddeg[c], di = din[c];
+                const int32_t ds = dsize[c];
+                if (dd | di | (ull)(uint32_t)ds) {
+                    dv += (int64_t)dd;
+                    iv += (int64_t)di;
+                    deg[c] = dv;
+                    in[c] = iv;
+                    size[c] += ds;
+                    ddeg[c] = 0;
+                    din[c] = 0;
+                    dsize[c] = 0;
+                }
+            }
+            key = vcomp[c];
+            sin = iv;
+            acc_sq(lo, hi, (ull)dv);
+        }
+        seg_qacc_add(valid, key, sin, lo, hi, cq);
+    }
+}
+
+// Communities (renumbering groups) per component: group g's component is its
+// head's; groups are in community order, so components ascend.
+__global__ void k_b_groups(const int32_t *gstart, const int32_t *perm, const int32_t *vcomp,
+                           int32_t K, ull *kcnt)
+{
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < K;
+         base += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t g = base + threadIdx.x;
+        const bool valid = g < K;
+        const long long key = valid ? vcomp[perm[gstart[g]]] : 0;
+        seg_atomic_add(valid, key, 1ull, kcnt);
+    }
+}
+
+// Next-level vertex id of every group: its community's number within its
+// component plus the component's first vertex at the next level, or -1 when
+// the component stops at this level.
+__global__ void k_b_next_ids(const int32_t *gstart, const int32_t *perm, const int32_t *vcomp,
+                             const int32_t *newid_g, int32_t K, const int64_t *koff,
+                             const int64_t *vnext, int32_t *newid_next)
+{
+    GRID_STRIDE(g, K)
+    {
+        const int32_t comp = vcomp[perm[gstart[g]]];
+        const int64_t vn = vnext[comp];
+        newid_next[g] = vn < 0 ? -1 : (int32_t)(newid_g[g] - koff[comp] + vn);
+    }
+}
+
+// cm[comp] += kdeg - 2*loop over the vertices of each component (= sum over the
+// component's communities of their entry weights plus twice their members'
+// self-loops, i.e. 2m of the coarsened component).
+__global__ void k_b_vertex_sum(const int64_t *kdeg, const int64_t *loopw, const int32_t *vcomp,
+                               int32_t n, ull *cm)
+{
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n;
+         base += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t x = base + threadIdx.x;
+        const bool valid = x < n;
+        long long key = 0;
+        ull v = 0;
+        if (valid) {
+            key = vcomp[x];
+            v = (ull)(kdeg[x] - 2 * loopw[x]);
+        }
+        seg_atomic_add(valid, key, v, cm);
+    }
+}
+
+// One level of the flattening: fin holds each level-0 vertex's id at level l
+// (a union vertex id); components with more than l+1 levels move on to their
+// next-level vertex, the others keep their community number at level l.
+__global__ void k_b_flatten(int32_t *fin, int32_t n0, const int32_t *vcomp0, const int32_t *clev,
+                            int32_t l, const int32_t *newc_l, const int64_t *koff_l,
+                            const int64_t *vnext_l)
+{
+    GRID_STRIDE(v, n0)
+    {
+        const int32_t comp = vcomp0[v];
+        const int32_t L = clev[comp];
+        if (L <= l) continue;
+        const int64_t local = (int64_t)newc_l[fin[v]] - koff_l[comp];
+        fin[v] = (int32_t)(L > l + 1 ? local + vnext_l[comp] : local);
+    }
+}
+
+void launch_b_graph_exponent(hipStream_t s, const uint32_t *bw, int wtype, const int64_t *beoff,
+                             int32_t ngraphs, int64_t eb, int *gscale, DevFlags *f)
+{
+    if (eb > 0)
+        hipLaunchKernelGGL(k_b_graph_exponent, grid_for(eb, 256, 1 << 20), 256, 0, s, bw, wtype,
+                           beoff, ngraphs, eb, gscale, f);
+}
+
+void launch_b_union_edges(hipStream_t s, const int32_t *bsrc, const int32_t *bdst,
+                          const uint32_t *bw, int wtype, const int *gscale, const int64_t *beoff,
+                          const int32_t *bn, const int64_t *ueoff, const int64_t *uvoff,
+                          int32_t ncomp, int32_t nres, int64_t etot, int32_t *usrc, int32_t *udst,
+                          int64_t *uw, DevFlags *f)
+{
+    if (etot > 0)
+        hipLaunchKernelGGL(k_b_union_edges, grid_for(etot, 256, 1 << 20), 256, 0, s, bsrc, bdst, bw,
+                           wtype, gscale, beoff, bn, ueoff, uvoff, ncomp, nres, etot, usrc, udst,
+                           uw, f);
+}
+
+void launch_b_edge_sum(hipStream_t s, const int64_t *w, const int64_t *ueoff, int32_t ncomp,
+                       int64_t etot, ull *cm)
+{
+    if (etot > 0)
+        hipLaunchKernelGGL(k_b_edge_sum, grid_for(etot, 256, 1 << 20), 256, 0, s, w, ueoff, ncomp,
+                           etot, cm);
+}
+
+void launch_b_qagg(hipStream_t s, int32_t n, int64_t *deg, int64_t *in, int32_t *size, ull *ddeg,
+                   ull *din, int32_t *dsize, const int32_t *vcomp, bool apply, QAcc *cq)
+{
+    if (n > 0)
+        hipLaunchKernelGGL(k_b_qagg, grid_for(n, 256, 1 << 20), 256, 0, s, n, deg, in, size, ddeg,
+                           din, dsize, vcomp, apply ? 1 : 0, cq);
+}
+
+void launch_b_groups(hipStream_t s, const int32_t *gstart, const int32_t *perm,
+                     const int32_t *vcomp, int32_t K, ull *kcnt)
+{
+    if (K > 0)
+        hipLaunchKernelGGL(k_b_groups, grid_for(K, 256, 1 << 20), 256, 0, s, gstart, perm, vcomp,
+                           K, kcnt);
+}
+
+void launch_b_next_ids(hipStream_t s, const int32_t *gstart, const int32_t *perm,
+                       const int32_t *vcomp, const int32_t *newid_g, int32_t K,
+                       const int64_t *koff, const int64_t *vnext, int32_t *newid_next)
+{
+    if (K > 0)
+        hipLaunchKernelGGL(k_b_next_ids, grid_for(K, 256), 256, 0, s, gstart, perm, vcomp, newid_g,
+                           K, koff, vnext, newid_next);
+}
+
+void launch_b_vertex_sum(hipStream_t s, const int64_t *kdeg, const int64_t *loopw,
+                         const int32_t *vcomp, int32_t n, ull *cm)
+{
+    if (n > 0)
+        hipLaunchKernelGGL(k_b_vertex_sum, grid_for(n, 256, 1 << 20), 256, 0, s, kdeg, loopw, vcomp,
+                           n, cm);
+}
+
+void launch_b_flatten(hipStream_t s, int32_t *fin, int32_t n0, const int32_t *vcomp0,
+                      const int32_t *clev, int32_t l, const int32_t *newc_l, const int64_t *koff_l,
+                      const int64_t *vnext_l)
+{
+    if (n0 > 0)
+        hipLaunchKernelGGL(k_b_flatten, grid_for(n0, 256), 256, 0, s, fin, n0, vcomp0, clev, l,
+                           newc_l, koff_l, vnext_l);
+}
+
+}  // namespace lv

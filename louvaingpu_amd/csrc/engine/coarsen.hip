@@ -238,9 +238,11 @@ void launch_cr_gather(hipStream_t s, const int64_t *estart, int32_t *bound, cons
 // ---------------------------------------------------------------------------
 // segments: group g's entries are [sb[g], sb[g] + slen[g]) of ey/ew
 
+// Groups whose new id is -1 (components of a graph set that stop at this
+// level) are put in tier 0: they produce no next-level row.
 __global__ __launch_bounds__(256) void k_cr_seg_tier(const int32_t *gstart, const int64_t *estart,
-                                                    int32_t K, int64_t *sb, int64_t *slen,
-                                                    uint32_t *tkey, int32_t *sid,
+                                                    const int32_t *newid_g, int32_t K, int64_t *sb,
+                                                    int64_t *slen, uint32_t *tkey, int32_t *sid,
                                                     long long *tier_cnt)
 {
     __shared__ long long s_cnt[kTiers + 1];
@@ -251,7 +253,7 @@ __global__ __launch_bounds__(256) void k_cr_seg_tier(const int32_t *gstart, cons
         int64_t b = estart[gstart[g]], e = estart[gstart[g + 1]];
         sb[g] = b;
         slen[g] = e - b;
-        int t = tier_of(e - b, e - b);
+        int t = newid_g[g] < 0 ? 0 : tier_of(e - b, e - b);
         tkey[g] = (uint32_t)t;
         sid[g] = (int32_t)g;
         atomicAdd((ull *)&s_cnt[t], 1ull);
@@ -261,13 +263,13 @@ __global__ __launch_bounds__(256) void k_cr_seg_tier(const int32_t *gstart, cons
         atomicAdd((ull *)&tier_cnt[threadIdx.x], (ull)s_cnt[threadIdx.x]);
 }
 
-void launch_cr_seg_tier(hipStream_t s, const int32_t *gstart, const int64_t *estart, int32_t K,
-                        int64_t *sb, int64_t *slen, uint32_t *tkey, int32_t *sid,
-                        long long *tier_cnt)
+void launch_cr_seg_tier(hipStream_t s, const int32_t *gstart, const int64_t *estart,
+                        const int32_t *newid_g, int32_t K, int64_t *sb, int64_t *slen,
+                        uint32_t *tkey, int32_t *sid, long long *tier_cnt)
 {
     if (K > 0)
-        hipLaunchKernelGGL(k_cr_seg_tier, grid_for(K, 256, 8192), 256, 0, s, gstart, estart, K, sb,
-                           slen, tkey, sid, tier_cnt);
+        hipLaunchKernelGGL(k_cr_seg_tier, grid_for(K, 256, 8192), 256, 0, s, gstart, estart,
+                           newid_g, K, sb, slen, tkey, sid, tier_cnt);
 }
 
 // Segments of up to 2048 entries: G lanes (or a block) per segment, LDS table
@@ -630,7 +632,10 @@ int run_cr_big(Arena &A, hipStream_t s, const CoarsenArgs &a, const BigSegArgs &
 
 __global__ void k_cr_rowcnt(const int32_t *newid_g, const int32_t *nuniq, int32_t K, int64_t *rowcnt)
 {
-    GRID_STRIDE(g, K) rowcnt[newid_g[g]] = nuniq[g];
+    GRID_STRIDE(g, K)
+    {
+        if (newid_g[g] >= 0) rowcnt[newid_g[g]] = nuniq[g];
+    }
 }
 
 // 16 lanes per run.
@@ -641,6 +646,7 @@ __global__ void k_cr_copy_small(CoarsenArgs a, const int64_t *rowptr2, int32_t K
     for (int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4; g < K;
          g += ((int64_t)gridDim.x * blockDim.x) >> 4) {
         if (a.slen[g] > kTierMaxQ[kTiers - 2]) continue;  // big segments copy per partition
+        if (a.newid_g[g] < 0) continue;                     // stopped component
         const int64_t src = a.sb[g], dst = rowptr2[a.newid_g[g]];
         const int32_t len = a.nuniq[g];
         for (int32_t t = lg; t < len; t += 16) {
@@ -656,7 +662,7 @@ __global__ void k_cr_copy_big(CoarsenArgs a, BigSegArgs B, const int64_t *rowptr
     const int lg = threadIdx.x & 15;
     for (int64_t p = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4; p < nparts;
          p += ((int64_t)gridDim.x * blockDim.x) >> 4) {
-        const int32_t g = B.bsegs[B.part_big[p]];
+        const int32_t g = B.bsegs[B.part_big[p]];  // (big segments are never stopped groups)
         const int64_t src = B.poff[p], dst = rowptr2[a.newid_g[g]] + B.pre[p];
         const int32_t len = B.ucnt[p];
         for (int32_t t = lg; t < len; t += 16) {
@@ -674,6 +680,7 @@ __global__ void k_cr_vertex(const int32_t *newid_g, const ull *ssum, const ull *
     GRID_STRIDE(g, K)
     {
         const int32_t x = newid_g[g];
+        if (x < 0) continue;  // stopped component
         const int64_t lp = (int64_t)((internal[g] + 2 * selfl[g]) / 2);
         loop2[x] = lp;
         self2[x] = lp;

@@ -20,6 +20,44 @@ static inline int grid_for(int64_t n, int block, int64_t cap = 1 << 16)
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < (n); \
          i += (int64_t)gridDim.x * blockDim.x)
 
+// Smallest s >= 0 with w * 2^s integral, from the float32 bit pattern; -1 for
+// a negative, infinite or NaN weight.
+__device__ __forceinline__ int f32_unit_exponent(uint32_t x)
+{
+    if ((x & 0x7fffffffu) == 0) return 0;  // +0 or -0
+    if (x >> 31) return -1;
+    uint32_t ex = (x >> 23) & 0xffu, mant = x & 0x7fffffu;
+    if (ex == 0xffu) return -1;
+    uint32_t M = ex ? (mant | 0x800000u) : mant;
+    int e2 = ex ? (int)ex - 150 : -149;  // w = M * 2^e2
+    int s = -e2 - __builtin_ctz(M);
+    return s > 0 ? s : 0;
+}
+
+// Weight in units of 2^-scale from its raw 32-bit pattern (int32 or float32);
+// sets bad = 2 when it reaches the exact range limit.
+__device__ __forceinline__ int64_t weight_units(uint32_t x, int wtype, int scale, int &bad)
+{
+    int64_t u;
+    if (wtype == LV_WEIGHT_INT32) {
+        u = (int32_t)x;
+    } else if ((x & 0x7fffffffu) == 0) {
+        u = 0;
+    } else {
+        uint32_t ex = (x >> 23) & 0xffu, mant = x & 0x7fffffu;
+        uint64_t M = ex ? (mant | 0x800000u) : mant;
+        int sh = (ex ? (int)ex - 150 : -149) + scale;
+        if (sh > 36) {
+            bad = 2;
+            u = 0;
+        } else {
+            u = (int64_t)(sh >= 0 ? (M << sh) : (M >> (-sh)));
+        }
+    }
+    if (u >= kUnitLimit) bad = 2;
+    return u;
+}
+
 __device__ __forceinline__ long long wave_sum(long long v)
 {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);

@@ -119,6 +119,13 @@ struct SweepArgs {
     double m2;     // 2.0 * (double)m
     double gamma;
     int last;      // last pass: fuse merge-isolated, drop move deltas
+    // Graph sets (batched kernels only): component of each vertex, and per
+    // component its mode (0 frozen: next = C; 1 sweep; 2 last pass), 2m and
+    // resolution.
+    const int32_t *vcomp;
+    const int32_t *cmode;
+    const double *cm2;
+    const double *cgam;
 };
 
 // Hub tier plan (rows with q > 2048): each hub's entries (plus a virtual
@@ -271,7 +278,7 @@ void launch_lab(hipStream_t s, int32_t n, const int32_t *C, const int32_t *size,
 void launch_apply(hipStream_t s, int32_t n, int64_t *deg, int64_t *in, int32_t *size, ull *ddeg,
                   ull *din, int32_t *dsize, QAcc *part, QAcc *acc);
 void launch_sweep_tier(hipStream_t s, int tier, const SweepArgs &a, const int32_t *verts,
-                       int32_t nv);
+                       int32_t nv, bool batch);
 void launch_hub_setup(hipStream_t s, const int32_t *hv, int32_t nh, const int64_t *rowptr,
                       const int64_t *selfsum, int32_t target, int64_t *hq, int32_t *hP,
                       int64_t *hP64, int64_t *hch);
@@ -281,7 +288,30 @@ void launch_expand_owner(hipStream_t s, const int64_t *base, int32_t nseg, int64
                          int32_t *owner, int32_t *rank);
 void launch_widen_i32(hipStream_t s, const int32_t *in, int64_t n, int64_t *out);
 int run_hub_sweep(Arena &A, hipStream_t s, const SweepArgs &a, const HubArgs &h, int32_t nh,
-                  int64_t nparts, int64_t nchunks);
+                  int64_t nparts, int64_t nchunks, bool batch);
+
+// batch.hip (graph sets)
+void launch_b_graph_exponent(hipStream_t s, const uint32_t *bw, int wtype, const int64_t *beoff,
+                             int32_t ngraphs, int64_t eb, int *gscale, DevFlags *f);
+void launch_b_union_edges(hipStream_t s, const int32_t *bsrc, const int32_t *bdst,
+                          const uint32_t *bw, int wtype, const int *gscale, const int64_t *beoff,
+                          const int32_t *bn, const int64_t *ueoff, const int64_t *uvoff,
+                          int32_t ncomp, int32_t nres, int64_t etot, int32_t *usrc, int32_t *udst,
+                          int64_t *uw, DevFlags *f);
+void launch_b_edge_sum(hipStream_t s, const int64_t *w, const int64_t *ueoff, int32_t ncomp,
+                       int64_t etot, ull *cm);
+void launch_b_qagg(hipStream_t s, int32_t n, int64_t *deg, int64_t *in, int32_t *size, ull *ddeg,
+                   ull *din, int32_t *dsize, const int32_t *vcomp, bool apply, QAcc *cq);
+void launch_b_groups(hipStream_t s, const int32_t *gstart, const int32_t *perm,
+                     const int32_t *vcomp, int32_t K, ull *kcnt);
+void launch_b_next_ids(hipStream_t s, const int32_t *gstart, const int32_t *perm,
+                       const int32_t *vcomp, const int32_t *newid_g, int32_t K,
+                       const int64_t *koff, const int64_t *vnext, int32_t *newid_next);
+void launch_b_vertex_sum(hipStream_t s, const int64_t *kdeg, const int64_t *loopw,
+                         const int32_t *vcomp, int32_t n, ull *cm);
+void launch_b_flatten(hipStream_t s, int32_t *fin, int32_t n0, const int32_t *vcomp0,
+                      const int32_t *clev, int32_t l, const int32_t *newc_l, const int64_t *koff_l,
+                      const int64_t *vnext_l);
 
 // coarsen.hip
 void launch_cr_heads(hipStream_t s, const uint32_t *Cs, int32_t n, int32_t *head);
@@ -298,9 +328,9 @@ void launch_cr_selfl(hipStream_t s, const int32_t *perm, const int64_t *selfsum,
 void launch_cr_gather(hipStream_t s, const int64_t *estart, int32_t *bound, const int64_t *mstart,
                       const int32_t *col, const int64_t *cw, const int32_t *newc, int32_t n,
                       int64_t total, int32_t *ey, int64_t *ew);
-void launch_cr_seg_tier(hipStream_t s, const int32_t *gstart, const int64_t *estart, int32_t K,
-                        int64_t *sb, int64_t *slen, uint32_t *tkey, int32_t *sid,
-                        long long *tier_cnt);
+void launch_cr_seg_tier(hipStream_t s, const int32_t *gstart, const int64_t *estart,
+                        const int32_t *newid_g, int32_t K, int64_t *sb, int64_t *slen,
+                        uint32_t *tkey, int32_t *sid, long long *tier_cnt);
 void launch_cr_agg_tier(hipStream_t s, int tier, const CoarsenArgs &a, const int32_t *segs, int32_t ns);
 void launch_cr_big_setup(hipStream_t s, const int32_t *bsegs, int32_t nbs, const int64_t *slen,
                          int32_t target, int32_t *bP, int64_t *bP64, int64_t *bch, int64_t *blen);

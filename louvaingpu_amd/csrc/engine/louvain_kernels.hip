@@ -13,20 +13,6 @@ namespace lv {
 // Input validation and weight units (the reference reads float weights and
 // truncates sums to int, LouvainCUDA/InitKernel.cu:119, StatusCUDA.h:31-41).
 
-// Smallest s >= 0 with w * 2^s integral, from the float32 bit pattern; -1 for
-// a negative, infinite or NaN weight.
-__device__ __forceinline__ int f32_unit_exponent(uint32_t x)
-{
-    if ((x & 0x7fffffffu) == 0) return 0;  // +0 or -0
-    if (x >> 31) return -1;
-    uint32_t ex = (x >> 23) & 0xffu, mant = x & 0x7fffffu;
-    if (ex == 0xffu) return -1;
-    uint32_t M = ex ? (mant | 0x800000u) : mant;
-    int e2 = ex ? (int)ex - 150 : -149;  // w = M * 2^e2
-    int s = -e2 - __builtin_ctz(M);
-    return s > 0 ? s : 0;
-}
-
 __global__ void k_weight_exponent(const void *w, int wtype, int64_t e, DevFlags *f)
 {
     int smax = 0;
@@ -58,26 +44,7 @@ __global__ void k_weight_units(const void *w, int wtype, int64_t e, int scale, i
     int bad = 0;
     GRID_STRIDE(i, e)
     {
-        int64_t u;
-        if (wtype == LV_WEIGHT_INT32) {
-            u = ((const int32_t *)w)[i];
-        } else {
-            uint32_t x = ((const uint32_t *)w)[i];
-            if ((x & 0x7fffffffu) == 0) {
-                u = 0;
-            } else {
-                uint32_t ex = (x >> 23) & 0xffu, mant = x & 0x7fffffu;
-                uint64_t M = ex ? (mant | 0x800000u) : mant;
-                int sh = (ex ? (int)ex - 150 : -149) + scale;
-                if (sh > 36) {
-                    bad = 2;
-                    u = 0;
-                } else {
-                    u = (int64_t)(sh >= 0 ? (M << sh) : (M >> (-sh)));
-                }
-            }
-        }
-        if (u >= kUnitLimit) bad = 2;
+        const int64_t u = weight_units(((const uint32_t *)w)[i], wtype, scale, bad);
         wu[i] = u;
         umax = max(umax, (ull)u);
         tot += (ull)(u == 0 ? 1 : u);
@@ -600,12 +567,30 @@ void launch_apply(hipStream_t s, int32_t n, int64_t *deg, int64_t *in, int32_t *
 // the maximal g when that maximum is > 0, otherwise the own community - the
 // outcome of the reference's ascending scan with its replace rule.
 
-__device__ __forceinline__ void consider(int32_t c, int64_t W, int64_t degc, const SweepArgs &a,
+// Per-vertex parameters: 2m, resolution and pass mode (0 frozen, 1 sweep,
+// 2 last pass) - of the single graph, or of the vertex's component in a set.
+struct VParam {
+    double m2, gamma;
+    int mode;
+};
+
+template <bool B>
+__device__ __forceinline__ VParam vparam(const SweepArgs &a, int32_t v)
+{
+    if constexpr (B) {
+        const int32_t c = a.vcomp[v];
+        return {a.cm2[c], a.cgam[c], a.cmode[c]};
+    } else {
+        return {a.m2, a.gamma, a.last ? 2 : 1};
+    }
+}
+
+__device__ __forceinline__ void consider(int32_t c, int64_t W, int64_t degc, double gamma,
                                          double f, int &cnt, double &bg, int32_t &bc, int64_t &bw)
 {
     ++cnt;
     if (W != 0) {
-        double g = (double)W - ((a.gamma * (double)degc) * f);
+        double g = (double)W - ((gamma * (double)degc) * f);
         if (g > 0.0 && better(g, c, bg, bc)) {
             bg = g;
             bc = c;
@@ -635,15 +620,19 @@ __device__ __forceinline__ void group_reduce(int &cnt, double &bg, int32_t &bc, 
 // OneLevelKernel.cu:52-78,400-421); on the last pass the deltas are the
 // merge-isolated ones only (:109-171,921-927).
 __device__ __forceinline__ void finalize_vertex(const SweepArgs &a, int32_t v, int32_t own,
-                                                int32_t so, int n, int32_t bc, int64_t bw)
+                                                int32_t so, int n, int32_t bc, int64_t bw, int mode)
 {
+    if (mode == 0) {  // frozen component: keeps its communities
+        a.next[v] = own;
+        return;
+    }
     const bool has = bc != INT_MAX;
     const int32_t c0 = has ? bc : own;
     const int64_t w0 = has ? bw : 0;
     int32_t choice = c0;
     if (so == 1 && a.size[c0] == 1 && c0 > own) choice = own;
     const int64_t k = a.kdeg[v], lp = a.loopw[v];
-    if (!a.last) {
+    if (mode == 1) {
         a.next[v] = choice;
         if (choice != own) {
             atomicAdd(&a.ddeg[own], (ull)(-k));
@@ -708,7 +697,7 @@ __device__ __forceinline__ void insert_row(const SweepArgs &a, int32_t *key, ull
 
 // G lanes per vertex, 256/G vertices per block, an LDS table of up to
 // 2^LOG2CAP slots per vertex (2^MINLOG2 .. 2^LOG2CAP sized to the row).
-template <int G, int LOG2CAP, int MINLOG2, bool DL>
+template <int G, int LOG2CAP, int MINLOG2, bool DL, bool B>
 __global__ __launch_bounds__(256) void k_sweep_group(SweepArgs a, const int32_t *__restrict__ verts,
                                                     int32_t nv)
 {
@@ -726,15 +715,18 @@ __global__ __launch_bounds__(256) void k_sweep_group(SweepArgs a, const int32_t 
         const bool active = vi < nv;
         int32_t v = 0;
         Lab me{0, 0, 0};
+        VParam vp{1.0, 1.0, 0};
         int64_t beg = 0, end = 0;
         bool loop = false;
         int l2 = MINLOG2;
         if (active) {
             v = verts[vi];
+            vp = vparam<B>(a, v);
             me = a.lab[v];
             beg = a.rowptr[v];
             end = a.rowptr[v + 1];
             loop = a.selfsum[v] > 0;
+            if (vp.mode == 0) end = beg, loop = false;  // frozen: nothing to gather
             if (MINLOG2 < LOG2CAP) l2 = table_log2(end - beg + (loop ? 1 : 0), MINLOG2);
         }
         const int cap = 1 << l2;
@@ -758,21 +750,22 @@ __global__ __launch_bounds__(256) void k_sweep_group(SweepArgs a, const int32_t 
         int32_t bc = INT_MAX;
         int64_t bw = 0;
         if (active) {
-            const double f = (double)a.kdeg[v] / a.m2;
+            const double f = (double)a.kdeg[v] / vp.m2;
             for (int s = lg; s < cap; s += G) {
                 int32_t c = key[s];
-                if (c >= 0) consider(c, (int64_t)val[s], DL ? dg[s] : a.deg[c], a, f, cnt, bg, bc, bw);
+                if (c >= 0)
+                    consider(c, (int64_t)val[s], DL ? dg[s] : a.deg[c], vp.gamma, f, cnt, bg, bc, bw);
             }
         }
         group_reduce<G>(cnt, bg, bc, bw);
-        if (active && lg == 0) finalize_vertex(a, v, me.c, me.sz, cnt, bc, bw);
+        if (active && lg == 0) finalize_vertex(a, v, me.c, me.sz, cnt, bc, bw, vp.mode);
         __syncthreads();
     }
 }
 
 // One vertex per block of BLOCK threads with an LDS table of up to
 // 2^LOG2CAP slots sized to the row.
-template <int BLOCK, int LOG2CAP, int MINLOG2>
+template <int BLOCK, int LOG2CAP, int MINLOG2, bool B>
 __global__ __launch_bounds__(BLOCK) void k_sweep_block(SweepArgs a, const int32_t *__restrict__ verts,
                                                       int32_t nv)
 {
@@ -787,9 +780,11 @@ __global__ __launch_bounds__(BLOCK) void k_sweep_block(SweepArgs a, const int32_
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int64_t vi = blockIdx.x; vi < nv; vi += gridDim.x) {
         const int32_t v = verts[vi];
+        const VParam vp = vparam<B>(a, v);
         const Lab me = a.lab[v];
-        const int64_t beg = a.rowptr[v], end = a.rowptr[v + 1];
-        const bool loop = a.selfsum[v] > 0;
+        const int64_t beg = a.rowptr[v];
+        const int64_t end = vp.mode == 0 ? beg : a.rowptr[v + 1];  // frozen: nothing to gather
+        const bool loop = vp.mode != 0 && a.selfsum[v] > 0;
         const int l2 = table_log2(end - beg + (loop ? 1 : 0), MINLOG2);
         const int cap = 1 << l2;
         for (int s = tid; s < cap; s += BLOCK) {
@@ -804,10 +799,10 @@ __global__ __launch_bounds__(BLOCK) void k_sweep_block(SweepArgs a, const int32_
         double bg = 0.0;
         int32_t bc = INT_MAX;
         int64_t bw = 0;
-        const double f = (double)a.kdeg[v] / a.m2;
+        const double f = (double)a.kdeg[v] / vp.m2;
         for (int s = tid; s < cap; s += BLOCK) {
             int32_t c = s_key[s];
-            if (c >= 0) consider(c, (int64_t)s_val[s], a.deg[c], a, f, cnt, bg, bc, bw);
+            if (c >= 0) consider(c, (int64_t)s_val[s], a.deg[c], vp.gamma, f, cnt, bg, bc, bw);
         }
         group_reduce<64>(cnt, bg, bc, bw);
         if (lane == 0) {
@@ -830,7 +825,7 @@ __global__ __launch_bounds__(BLOCK) void k_sweep_block(SweepArgs a, const int32_
                     w0 = r_w[w];
                 }
             }
-            finalize_vertex(a, v, me.c, me.sz, tc, c0, w0);
+            finalize_vertex(a, v, me.c, me.sz, tc, c0, w0, vp.mode);
         }
         __syncthreads();
     }
@@ -844,22 +839,33 @@ static void launch_grid(KernelT k, int64_t blocks, int block, hipStream_t s, con
     hipLaunchKernelGGL(k, g, block, 0, s, a, verts, nv);
 }
 
-// Tiers 1..9 (tier 10, the hubs, runs through run_hub_sweep).
-void launch_sweep_tier(hipStream_t s, int tier, const SweepArgs &a, const int32_t *verts, int32_t nv)
+template <bool B>
+static void launch_sweep_tier_t(hipStream_t s, int tier, const SweepArgs &a, const int32_t *verts,
+                                int32_t nv)
 {
-    if (nv <= 0) return;
     switch (tier) {
-    case 1: launch_grid(k_sweep_group<4, 3, 3, true>, (nv + 63) / 64, 256, s, a, verts, nv); break;
-    case 2: launch_grid(k_sweep_group<8, 4, 4, true>, (nv + 31) / 32, 256, s, a, verts, nv); break;
-    case 3: launch_grid(k_sweep_group<16, 5, 5, true>, (nv + 15) / 16, 256, s, a, verts, nv); break;
-    case 4: launch_grid(k_sweep_group<32, 6, 6, true>, (nv + 7) / 8, 256, s, a, verts, nv); break;
-    case 5: launch_grid(k_sweep_group<64, 7, 7, true>, (nv + 3) / 4, 256, s, a, verts, nv); break;
-    case 6: launch_grid(k_sweep_group<64, 8, 8, true>, (nv + 3) / 4, 256, s, a, verts, nv); break;
-    case 7: launch_grid(k_sweep_group<64, 9, 9, true>, (nv + 3) / 4, 256, s, a, verts, nv); break;
-    case 8: launch_grid(k_sweep_group<64, 10, 10, false>, (nv + 3) / 4, 256, s, a, verts, nv); break;
-    case 9: launch_grid(k_sweep_block<256, 12, 11>, nv, 256, s, a, verts, nv); break;
+    case 1: launch_grid(k_sweep_group<4, 3, 3, true, B>, (nv + 63) / 64, 256, s, a, verts, nv); break;
+    case 2: launch_grid(k_sweep_group<8, 4, 4, true, B>, (nv + 31) / 32, 256, s, a, verts, nv); break;
+    case 3: launch_grid(k_sweep_group<16, 5, 5, true, B>, (nv + 15) / 16, 256, s, a, verts, nv); break;
+    case 4: launch_grid(k_sweep_group<32, 6, 6, true, B>, (nv + 7) / 8, 256, s, a, verts, nv); break;
+    case 5: launch_grid(k_sweep_group<64, 7, 7, true, B>, (nv + 3) / 4, 256, s, a, verts, nv); break;
+    case 6: launch_grid(k_sweep_group<64, 8, 8, true, B>, (nv + 3) / 4, 256, s, a, verts, nv); break;
+    case 7: launch_grid(k_sweep_group<64, 9, 9, true, B>, (nv + 3) / 4, 256, s, a, verts, nv); break;
+    case 8: launch_grid(k_sweep_group<64, 10, 10, false, B>, (nv + 3) / 4, 256, s, a, verts, nv); break;
+    case 9: launch_grid(k_sweep_block<256, 12, 11, B>, nv, 256, s, a, verts, nv); break;
     default: break;
     }
+}
+
+// Tiers 1..9 (tier 10, the hubs, runs through run_hub_sweep).
+void launch_sweep_tier(hipStream_t s, int tier, const SweepArgs &a, const int32_t *verts, int32_t nv,
+                       bool batch)
+{
+    if (nv <= 0) return;
+    if (batch)
+        launch_sweep_tier_t<true>(s, tier, a, verts, nv);
+    else
+        launch_sweep_tier_t<false>(s, tier, a, verts, nv);
 }
 
 // ---------------------------------------------------------------------------
@@ -989,6 +995,7 @@ __global__ __launch_bounds__(256) void k_hub_scatter(SweepArgs a, HubArgs h)
 }
 
 
+template <bool B>
 __global__ __launch_bounds__(256) void k_hub_agg(SweepArgs a, HubArgs h)
 {
     constexpr int CAP = 1 << kHubAggLog2;
@@ -1004,7 +1011,8 @@ __global__ __launch_bounds__(256) void k_hub_agg(SweepArgs a, HubArgs h)
     const int64_t beg = h.poff[pi], end = h.poff[pi + 1];
     const int32_t j = h.part_hub[pi];
     const int32_t v = h.hv[j];
-    const double f = (double)a.kdeg[v] / a.m2;
+    const VParam vp = vparam<B>(a, v);
+    const double f = (double)a.kdeg[v] / vp.m2;
     // 2 slots per entry up to the LDS capacity; a full table (possible only
     // with more than CAP/2 entries) sends the partition to its global table
     const int l2 = (end - beg) * 2 >= CAP ? kHubAggLog2 : table_log2(end - beg, 6);
@@ -1025,7 +1033,7 @@ __global__ __launch_bounds__(256) void k_hub_agg(SweepArgs a, HubArgs h)
     if (!s_over) {
         for (int s = tid; s < cap; s += 256) {
             int32_t c = s_key[s];
-            if (c >= 0) consider(c, (int64_t)s_val[s], a.deg[c], a, f, cnt, bg, bc, bw);
+            if (c >= 0) consider(c, (int64_t)s_val[s], a.deg[c], vp.gamma, f, cnt, bg, bc, bw);
         }
     } else {
         // more than CAP distinct keys: redo the partition in its global region
@@ -1045,7 +1053,7 @@ __global__ __launch_bounds__(256) void k_hub_agg(SweepArgs a, HubArgs h)
             int32_t c = __hip_atomic_load(&gk[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (c >= 0)
                 consider(c, (int64_t)__hip_atomic_load(&gv[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
-                         a.deg[c], a, f, cnt, bg, bc, bw);
+                         a.deg[c], vp.gamma, f, cnt, bg, bc, bw);
         }
     }
     group_reduce<64>(cnt, bg, bc, bw);
@@ -1077,6 +1085,7 @@ __global__ __launch_bounds__(256) void k_hub_agg(SweepArgs a, HubArgs h)
 }
 
 // One wavefront per hub: combine its partitions, then guard/move as usual.
+template <bool B>
 __global__ __launch_bounds__(256) void k_hub_final(SweepArgs a, HubArgs h, int32_t nh)
 {
     const int lane = threadIdx.x & 63;
@@ -1100,12 +1109,12 @@ __global__ __launch_bounds__(256) void k_hub_final(SweepArgs a, HubArgs h, int32
     if (j < nh && lane == 0) {
         const int32_t v = h.hv[j];
         const Lab me = a.lab[v];
-        finalize_vertex(a, v, me.c, me.sz, cnt, bc, bw);
+        finalize_vertex(a, v, me.c, me.sz, cnt, bc, bw, vparam<B>(a, v).mode);
     }
 }
 
 int run_hub_sweep(Arena &A, hipStream_t s, const SweepArgs &a, const HubArgs &h, int32_t nh,
-                  int64_t nparts, int64_t nchunks)
+                  int64_t nparts, int64_t nchunks, bool batch)
 {
     if (nh <= 0) return LV_OK;
     fill_i64(s, h.pcount, nparts + 1, 0);
@@ -1114,8 +1123,13 @@ int run_hub_sweep(Arena &A, hipStream_t s, const SweepArgs &a, const HubArgs &h,
     LV_HIP(hipMemcpyAsync(h.pcur, h.poff, sizeof(int64_t) * (size_t)nparts,
                           hipMemcpyDeviceToDevice, s));
     hipLaunchKernelGGL(k_hub_scatter, (int)nchunks, 256, 0, s, a, h);
-    hipLaunchKernelGGL(k_hub_agg, (int)nparts, 256, 0, s, a, h);
-    hipLaunchKernelGGL(k_hub_final, (int)((nh + 3) / 4), 256, 0, s, a, h, nh);
+    if (batch) {
+        hipLaunchKernelGGL(k_hub_agg<true>, (int)nparts, 256, 0, s, a, h);
+        hipLaunchKernelGGL(k_hub_final<true>, (int)((nh + 3) / 4), 256, 0, s, a, h, nh);
+    } else {
+        hipLaunchKernelGGL(k_hub_agg<false>, (int)nparts, 256, 0, s, a, h);
+        hipLaunchKernelGGL(k_hub_final<false>, (int)((nh + 3) / 4), 256, 0, s, a, h, nh);
+    }
     LV_HIP(hipGetLastError());
     return LV_OK;
 }
