@@ -17,6 +17,12 @@ Besides the JSON line's required fields it reports:
   cpu_baseline the oracle (C restatement of the reference semantics, OpenMP)
                on the same graph, timed on this host; its partition is also
                the validation of the GPU result
+  batched      the batching workload: 4096 SBM graphs (n log-uniform in
+               [100, 50000], blocks of 100, p_in ~ U[0.02, 0.1], p_out =
+               p_in/50, seed 3) x 16 resolutions 0.25..4.0 as graph sets, in
+               graph-runs/s, against the plain per-graph launch loop of the same
+               engine on every 64th graph (all resolutions), with identical
+               partitions required (--workload batch prints it as its own line)
 """
 from __future__ import annotations
 
@@ -95,6 +101,117 @@ def measured_traffic(workload_key, tier):
     return total if found == len(TIER_KERNELS[tier]) else None
 
 
+def gen_batch_graphs(count, seed, threads):
+    """The batching workload's SBM graphs (generated in parallel; the C++
+    generator releases the GIL)."""
+    from concurrent.futures import ThreadPoolExecutor
+    from louvaingpu_amd import generators
+    params = generators.batch_params(count, seed=seed)
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        return list(ex.map(lambda p: generators.sbm(p[0], 100, p[1], p[1] / 50, p[2]), params))
+
+
+def run_batched(args):
+    """4096 SBM graphs x 16 resolutions as graph sets (packed launches) against
+    the plain per-graph launch loop of the same engine on a sample."""
+    import oracle
+    from louvaingpu_amd import Engine, GraphSet, generators
+    count = args.batch_graphs
+    gammas = generators.resolutions(16)
+    nr = len(gammas)
+    t0 = time.perf_counter()
+    graphs = gen_batch_graphs(count, 3, host_threads())
+    gen_s = time.perf_counter() - t0
+    e_tot = sum(len(g[1]) for g in graphs)
+    n_tot = sum(g[0] for g in graphs)
+    print("[bench] batch: %d graphs, n=%d, e=%d, generated in %.1f s" % (count, n_tot, e_tot, gen_s),
+          file=sys.stderr, flush=True)
+    # chunks of consecutive graphs, bounded by union edges (graph edges x resolutions)
+    budget = args.batch_chunk_edges
+    chunks, cur, ce = [], [], 0
+    for i, g in enumerate(graphs):
+        if cur and ce + len(g[1]) * nr > budget:
+            chunks.append(cur)
+            cur, ce = [], 0
+        cur.append(i)
+        ce += len(g[1]) * nr
+    if cur:
+        chunks.append(cur)
+    arena = int(budget * 200 + 400 * sum(graphs[i][0] for i in max(chunks, key=len)) * nr
+                + (2 << 30))
+    run_ms, load_s = 0.0, 0.0
+    finals = {}
+    sample = list(range(0, count, max(1, count // args.batch_sample)))
+    sample_set = set(sample)
+    with GraphSet(arena_bytes=arena) as gs:
+        for ci, ch in enumerate(chunks):
+            tl = time.perf_counter()
+            gs.load([graphs[i] for i in ch], gammas)
+            load_s += time.perf_counter() - tl
+            ms = gs.run()
+            run_ms += ms
+            for j, gi in enumerate(ch):
+                if gi in sample_set:
+                    finals[gi] = [gs.final(j * nr + r) for r in range(nr)]
+            print("[bench] batch chunk %d/%d: %d graphs, %.1f ms" % (ci + 1, len(chunks), len(ch), ms),
+                  file=sys.stderr, flush=True)
+    runs = count * nr
+    # plain per-graph loop over the sample: one context per resolution, input resident
+    engines = [Engine(gamma=g) for g in gammas]
+    loop_ms, same = 0.0, True
+    try:
+        for gi in sample:
+            g = graphs[gi]
+            for r, eng in enumerate(engines):
+                eng.load_edges(*g)
+                tr = time.perf_counter()
+                eng.run()
+                loop_ms += (time.perf_counter() - tr) * 1e3
+                same = same and bool(np.array_equal(eng.final_assignment(), finals[gi][r]))
+    finally:
+        for eng in engines:
+            eng.close()
+    sample_e = sum(len(graphs[i][1]) for i in sample)
+    loop_est_s = loop_ms * 1e-3 * (e_tot / max(sample_e, 1))  # scaled by edges
+    loop_est_runs_s = loop_ms * 1e-3 * (count / len(sample))
+    # oracle on a few sampled graphs and resolutions
+    oracle_ok = True
+    for gi in sample[:4]:
+        for r in (0, nr // 2, nr - 1):
+            ro = oracle.run(*graphs[gi], gamma=gammas[r], threads=host_threads())
+            oracle_ok = oracle_ok and bool(np.array_equal(ro.final_comm(), finals[gi][r]))
+    batched_s = run_ms * 1e-3
+    out = {
+        "metric": "Louvain TEPS & end-to-end s at oracle modularity; batched vs per-graph",
+        "value": runs / batched_s,
+        "unit": "graph-runs/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 0,
+        "ms_per_step": run_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int64",
+        "data": "synthetic",
+        "config": {"workload": "sbm_batch_%dx%dgamma_seed3" % (count, nr), "graphs": count,
+                   "resolutions": nr, "n_total": n_tot, "e_total": e_tot, "chunks": len(chunks),
+                   "chunk_union_edges": budget, "parallelism": "single-gpu"},
+        "batched_s": batched_s,
+        "batched_load_s": load_s,
+        "per_graph_loop_s_est": loop_est_runs_s,
+        "per_graph_loop_s_est_by_edges": loop_est_s,
+        "batch_vs_loop": loop_est_runs_s / batched_s,
+        "loop_sample": "%d graphs x %d resolutions (every %d-th graph), %.1f s" % (
+            len(sample), nr, max(1, count // args.batch_sample), loop_ms * 1e-3),
+        "validated": same and oracle_ok,
+        "identical_partitions_sample": same,
+        "oracle_sample_ok": oracle_ok,
+        "generation_s": gen_s,
+    }
+    return out
+
+
 def workload(name):
     from louvaingpu_amd import generators
     if name == "rmat22":
@@ -118,12 +235,22 @@ def main():
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (calibrates "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE against a known byte count)")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the batched-vs-per-graph measurement of the default run")
+    ap.add_argument("--batch-graphs", type=int, default=4096)
+    ap.add_argument("--batch-sample", type=int, default=64,
+                    help="graphs of the per-graph loop sample (each at all resolutions)")
+    ap.add_argument("--batch-chunk-edges", type=int, default=300_000_000,
+                    help="union edges (graph edges x resolutions) per graph set")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
         return 2
 
+    if args.workload == "batch":
+        print(json.dumps(run_batched(args)), flush=True)
+        return 0
     from louvaingpu_amd import Engine
     from louvaingpu_amd._lib import LV_SWEEP_TIERS
 
@@ -247,6 +374,11 @@ def main():
     if args.copy_probe_mb:
         out["copy_probe_bytes"] = copy_probe(args.copy_probe_mb << 20)
     eng.close()
+    if not args.no_batch:
+        b = run_batched(args)
+        out["batched"] = {k: b[k] for k in (
+            "value", "unit", "config", "batched_s", "batched_load_s", "per_graph_loop_s_est",
+            "batch_vs_loop", "loop_sample", "validated")}
     print(json.dumps(out), flush=True)
     return 0
 
