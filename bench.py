@@ -218,6 +218,8 @@ def workload(name):
         return "rmat22_ef16_int_seed1", generators.rmat22()
     if name == "sbm10k":
         return "sbm_10k_100blocks_seed0", generators.sbm_10k()
+    if name == "rmat26f":
+        return "rmat26_ef16_float_seed2", generators.rmat26_float()
     if name.startswith("rmat"):
         sc = int(name[4:])
         return "rmat%d_ef16_int_seed1" % sc, generators.rmat(sc, 16, 0.57, 0.19, 0.19, 1, "int")

@@ -345,8 +345,8 @@ double current_q(lv_ctx *c, int &rc)
 
 // CalculateNeighbour at level 0: symmetric CSR of the loaded edge list.  Each
 // edge gives two entries keyed by row (self-loops keyed past the last row); a
-// stable radix sort by row keeps input order within a row.  When every weight
-// fits in 32 bits the entry carries (col, weight) through the sort and is
+// stable radix sort by row keeps input order within a row.  When the columns
+// and weights fit in 64 bits together the entry carries (col, weight) through the sort and is
 // unpacked in one streaming pass; otherwise it carries its edge code and the
 // columns and weights are gathered from the edge list afterwards.
 int build_level0_csr(lv_ctx *c)
@@ -357,7 +357,9 @@ int build_level0_csr(lv_ctx *c)
     const int64_t e = c->e0;
     const int64_t total = 2 * e;
     const int bits = bits_for((uint64_t)n);
-    const bool packed = c->max_units0 <= 0xffffffffull;
+    // (col, weight) share 64 bits when the widest column and weight fit together
+    const int wbits = std::min(63, 64 - bits_for(n > 0 ? (uint64_t)(n - 1) : 0));
+    const bool packed = c->max_units0 < (1ull << wbits);
     hipEvent_t t0 = t_begin(c);
     ull *lastloop = nullptr;
     uint32_t *k1 = nullptr, *k2 = nullptr, *v1 = nullptr, *v2 = nullptr;
@@ -383,7 +385,7 @@ int build_level0_csr(lv_ctx *c)
     uint32_t *sk = nullptr, *sv = nullptr;
     uint64_t *sp = nullptr;
     if (packed) {
-        launch_csr_entries_packed(s, c->d_src, c->d_dst, c->d_w, e, n, k1, p1);
+        launch_csr_entries_packed(s, c->d_src, c->d_dst, c->d_w, e, n, wbits, k1, p1);
         LV_TRY(radix_sort_u32_u64(A, s, k1, p1, k2, p2, total, bits, &sk, &sp));
     } else {
         launch_csr_entries(s, c->d_src, c->d_dst, e, n, k1, v1);
@@ -396,7 +398,7 @@ int build_level0_csr(lv_ctx *c)
     LV_TRY(arena_alloc(A, c->col, c->nnz));
     LV_TRY(arena_alloc(A, c->cw, c->nnz));
     if (packed)
-        launch_csr_unpack(s, sk, sp, c->nnz, c->col, c->cw, c->kdeg);
+        launch_csr_unpack(s, sk, sp, c->nnz, wbits, c->col, c->cw, c->kdeg);
     else
         launch_csr_gather(s, sk, sv, c->nnz, c->d_src, c->d_dst, c->d_w, c->col, c->cw, c->kdeg);
     launch_loop_gather(s, lastloop, c->d_w, n, c->loopw);

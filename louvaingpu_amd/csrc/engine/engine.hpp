@@ -248,12 +248,13 @@ void launch_ids_to_keys(hipStream_t s, const int32_t *ids, uint32_t *keys, int32
 
 void launch_csr_entries(hipStream_t s, const int32_t *src, const int32_t *dst, int64_t e,
                         int32_t n, uint32_t *keys, uint32_t *codes);
-// Entries with the payload (col << 32 | w) when every weight fits in 32 bits.
+// Entries with the payload (col << wbits | w) when every weight fits in wbits
+// bits and every column in 64 - wbits.
 void launch_csr_entries_packed(hipStream_t s, const int32_t *src, const int32_t *dst,
-                               const int64_t *w, int64_t e, int32_t n, uint32_t *keys,
+                               const int64_t *w, int64_t e, int32_t n, int wbits, uint32_t *keys,
                                uint64_t *payload);
 void launch_csr_unpack(hipStream_t s, const uint32_t *skeys, const uint64_t *spay, int64_t nnz,
-                       int32_t *col, int64_t *cw, int64_t *kdeg);
+                       int wbits, int32_t *col, int64_t *cw, int64_t *kdeg);
 void launch_self_loops(hipStream_t s, const int32_t *src, const int32_t *dst, const int64_t *w,
                        int64_t e, int64_t *kdeg, int64_t *selfsum, ull *lastloop);
 void launch_rowptr_search(hipStream_t s, const uint32_t *skeys, int64_t total, int32_t n,

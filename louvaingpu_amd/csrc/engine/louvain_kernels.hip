@@ -185,8 +185,10 @@ __global__ void k_csr_entries(const int32_t *src, const int32_t *dst, int64_t e,
     }
 }
 
+// payload = col << wbits | w (the host checks that w < 2^wbits and col < 2^(64-wbits)).
 __global__ void k_csr_entries_packed(const int32_t *src, const int32_t *dst, const int64_t *w,
-                                     int64_t e, int32_t n, uint32_t *keys, uint64_t *payload)
+                                     int64_t e, int32_t n, int wbits, uint32_t *keys,
+                                     uint64_t *payload)
 {
     GRID_STRIDE(i, e)
     {
@@ -195,14 +197,14 @@ __global__ void k_csr_entries_packed(const int32_t *src, const int32_t *dst, con
         bool self = (a == b);
         keys[2 * i] = self ? (uint32_t)n : (uint32_t)a;
         keys[2 * i + 1] = self ? (uint32_t)n : (uint32_t)b;
-        payload[2 * i] = ((uint64_t)(uint32_t)b << 32) | u;
-        payload[2 * i + 1] = ((uint64_t)(uint32_t)a << 32) | u;
+        payload[2 * i] = ((uint64_t)(uint32_t)b << wbits) | u;
+        payload[2 * i + 1] = ((uint64_t)(uint32_t)a << wbits) | u;
     }
 }
 
 __global__ __launch_bounds__(256) void k_csr_unpack(const uint32_t *skeys, const uint64_t *spay,
-                                                   int64_t nnz, int32_t *col, int64_t *cw,
-                                                   int64_t *kdeg)
+                                                   int64_t nnz, int wbits, int32_t *col,
+                                                   int64_t *cw, int64_t *kdeg)
 {
     for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < nnz;
          base += (int64_t)gridDim.x * blockDim.x) {
@@ -212,8 +214,8 @@ __global__ __launch_bounds__(256) void k_csr_unpack(const uint32_t *skeys, const
         ull wp = 0;
         if (valid) {
             uint64_t x = spay[p];
-            int64_t u = (int64_t)(x & 0xffffffffull);
-            col[p] = (int32_t)(x >> 32);
+            int64_t u = (int64_t)(x & ((1ull << wbits) - 1ull));
+            col[p] = (int32_t)(x >> wbits);
             cw[p] = u;
             row = (long long)skeys[p];
             wp = (ull)(u == 0 ? 1 : u);
@@ -223,20 +225,20 @@ __global__ __launch_bounds__(256) void k_csr_unpack(const uint32_t *skeys, const
 }
 
 void launch_csr_entries_packed(hipStream_t s, const int32_t *src, const int32_t *dst,
-                               const int64_t *w, int64_t e, int32_t n, uint32_t *keys,
+                               const int64_t *w, int64_t e, int32_t n, int wbits, uint32_t *keys,
                                uint64_t *payload)
 {
     if (e > 0)
         hipLaunchKernelGGL(k_csr_entries_packed, grid_for(e, 256, 1 << 20), 256, 0, s, src, dst, w,
-                           e, n, keys, payload);
+                           e, n, wbits, keys, payload);
 }
 
 void launch_csr_unpack(hipStream_t s, const uint32_t *skeys, const uint64_t *spay, int64_t nnz,
-                       int32_t *col, int64_t *cw, int64_t *kdeg)
+                       int wbits, int32_t *col, int64_t *cw, int64_t *kdeg)
 {
     if (nnz > 0)
         hipLaunchKernelGGL(k_csr_unpack, grid_for(nnz, 256, 1 << 20), 256, 0, s, skeys, spay, nnz,
-                           col, cw, kdeg);
+                           wbits, col, cw, kdeg);
 }
 
 __global__ void k_self_loops(const int32_t *src, const int32_t *dst, const int64_t *w, int64_t e,

@@ -37,6 +37,7 @@
  */
 #define _GNU_SOURCE
 #include <stdint.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #include <math.h>
@@ -98,6 +99,21 @@ typedef struct {
 /* ------------------------------------------------------------------------- */
 
 static void *xcalloc(size_t n, size_t sz) { return calloc(n ? n : 1, sz); }
+
+/* ORACLE_PROGRESS=1: one stderr line per stage and pass, so long runs report
+ * as they go. */
+static int progress_on(void)
+{
+    const char *p = getenv("ORACLE_PROGRESS");
+    return p && p[0] == '1';
+}
+
+static void progress(const char *stage, int level, long long a)
+{
+    if (!progress_on()) return;
+    fprintf(stderr, "[oracle] level %d %s %lld\n", level, stage, a);
+    fflush(stderr);
+}
 static void *xmalloc(size_t n) { return malloc(n ? n : 1); }
 
 /* u128 -> double as hi*2^64 + lo, each conversion rounded to nearest. */
@@ -345,6 +361,10 @@ static int one_level(orc_run *R, orc_level *L)
     int32_t pass = 0;
     for (;;) {
         ++pass;
+        if (progress_on()) {
+            fprintf(stderr, "[oracle] pass %d of a level with n %d\n", (int)pass, (int)n);
+            fflush(stderr);
+        }
         /* steps 1-5: every row reads the same state S (Jacobi) */
 #ifdef _OPENMP
 #pragma omp parallel for schedule(dynamic, 1024) num_threads(nthreads)
@@ -643,6 +663,7 @@ orc_run *orc_run_louvain(int32_t n, const int32_t *ids, int64_t e, const int32_t
             if (wi[j] < 0) { R->status = ORC_ERR_INPUT; return R; }
     }
     R->scale = s;
+    progress("weights checked, e", 0, e);
 
     if (grow_levels(R) != ORC_OK) { R->status = ORC_ERR_NOMEM; return R; }
     orc_level *L0 = &R->lv[0];
@@ -688,9 +709,11 @@ orc_run *orc_run_louvain(int32_t n, const int32_t *ids, int64_t e, const int32_t
     int level = 0;
     for (;;) {
         orc_level *L = &R->lv[level];
+        progress("csr", level, L->e);
         rc = build_csr(L);
         if (rc == ORC_OK) rc = init_vertices(L);
         if (rc != ORC_OK) { R->status = rc; return R; }
+        progress("init done, n", level, L->n);
         {
             /* initial Q of this level: internals = loops, degrees = k */
             L->q_init = q_of_state(L->n, L->loop, L->k, L->m, gamma);
@@ -700,6 +723,7 @@ orc_run *orc_run_louvain(int32_t n, const int32_t *ids, int64_t e, const int32_t
         rc = one_level(R, L);
         if (rc == ORC_OK) rc = renumber(R, L, level);
         if (rc != ORC_OK) { R->status = rc; return R; }
+        progress("renumbered, k", level, L->kcomm);
         R->nlevels = level + 1;             /* push, :384 / :464 */
         double q = L->q;
         if (level > 0) {
@@ -710,6 +734,7 @@ orc_run *orc_run_louvain(int32_t n, const int32_t *ids, int64_t e, const int32_t
         L = &R->lv[level];
         rc = coarsen(L, &R->lv[level + 1]);
         if (rc != ORC_OK) { R->status = rc; return R; }
+        progress("coarsened, e'", level, R->lv[level + 1].e);
         ++level;
     }
 
