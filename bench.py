@@ -23,6 +23,11 @@ Besides the JSON line's required fields it reports:
                graph-runs/s, against the plain per-graph launch loop of the same
                engine on every 64th graph (all resolutions), with identical
                partitions required (--workload batch prints it as its own line)
+  serving      Poisson arrivals at 200 requests/s of SBMs drawn as above plus
+               2% R-MAT scale-20 float graphs, continuous batching on two
+               engine contexts: p50/p99 latency, graphs/s, saturated
+               throughput against the per-graph loop, and each result equal
+               to the request alone (--workload serve)
 """
 from __future__ import annotations
 
@@ -212,6 +217,92 @@ def run_batched(args):
     return out
 
 
+def run_serving(args):
+    """Poisson stream (200 req/s) of SBM graphs drawn as in the batching
+    workload plus 2% R-MAT scale-20 float graphs, served with continuous
+    batching; p50/p99 latency, graphs/s, saturated throughput against the
+    plain per-graph loop, and results checked against the request alone."""
+    from concurrent.futures import ThreadPoolExecutor
+    from louvaingpu_amd import Engine, Server, generators
+    nreq = args.serve_requests
+    rate = args.serve_rate
+    rng = np.random.Generator(np.random.PCG64(4))
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
+    is_rmat = rng.uniform(size=nreq) < 0.02
+    params = generators.batch_params(nreq, seed=4)
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=host_threads()) as ex:
+        pool = list(ex.map(lambda sd: generators.rmat(20, 16, 0.57, 0.19, 0.19, sd, "float"),
+                           range(4, 8)))
+        sbms = list(ex.map(lambda p: generators.sbm(p[0], 100, p[1], p[1] / 50, p[2]), params))
+    reqs = [pool[i % len(pool)] if is_rmat[i] else sbms[i] for i in range(nreq)]
+    gen_s = time.perf_counter() - t0
+    print("[bench] serve: %d requests (%d R-MAT 20), generated in %.1f s"
+          % (nreq, int(is_rmat.sum()), gen_s), file=sys.stderr, flush=True)
+    budget = args.serve_budget_gb << 30
+
+    def stream(srv, times):
+        ids = [None] * nreq
+        start = time.perf_counter()
+        for i in range(nreq):
+            if times is not None:
+                d = times[i] - (time.perf_counter() - start)
+                if d > 0:
+                    time.sleep(d)
+            ids[i] = srv.submit(*reqs[i])
+        return ids
+
+    out = {}
+    with Server(workers=args.serve_workers, budget_bytes=budget) as srv:
+        srv.wait(srv.submit(*reqs[0]))  # warm the contexts
+        ids = stream(srv, arrivals)
+        res = [srv.wait(r) for r in ids]
+        lat = np.array([r[1].t_done_ms - r[1].t_submit_ms for r in res])
+        span = (max(r[1].t_done_ms for r in res) - min(r[1].t_submit_ms for r in res)) * 1e-3
+        st = srv.stats()
+        out.update({"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                    "mean_ms": float(lat.mean()), "graphs_per_s": nreq / span,
+                    "sets": int(st.sets) - 1, "max_set": int(st.max_set)})
+        lat_r = lat[is_rmat]
+        if len(lat_r):
+            out["p50_ms_rmat20"] = float(np.percentile(lat_r, 50))
+        print("[bench] serve: p50 %.1f ms p99 %.1f ms" % (out["p50_ms"], out["p99_ms"]),
+              file=sys.stderr, flush=True)
+        # saturated: every request queued at once
+        ts = time.perf_counter()
+        ids = stream(srv, None)
+        sat = [srv.wait(r) for r in ids]
+        sat_s = time.perf_counter() - ts
+        out["saturated_graphs_per_s"] = nreq / sat_s
+        same_mix = all(np.array_equal(a[0], b[0]) for a, b in zip(res, sat))
+    # plain per-graph loop (one context; each request uploaded, run, fetched) on a
+    # sample, and alone == mix
+    sample = list(range(0, nreq, max(1, nreq // args.serve_sample)))
+    loop_s, alone_ok = 0.0, True
+    with Engine() as eng:
+        for i in sample:
+            tr = time.perf_counter()  # a request end to end: upload, run, partition back
+            eng.load_edges(*reqs[i])
+            eng.run()
+            fin = eng.final_assignment()
+            loop_s += time.perf_counter() - tr
+            alone_ok = alone_ok and bool(np.array_equal(fin, res[i][0]))
+    loop_rate = len(sample) / loop_s
+    out.update({
+        "metric": "Louvain TEPS & end-to-end s at oracle modularity; batched vs per-graph",
+        "value": out["graphs_per_s"], "unit": "graphs/s",
+        "config": {"workload": "serve_poisson%d_sbm98_rmat20f2_seed4" % int(rate),
+                   "requests": nreq, "rate_per_s": rate, "workers": args.serve_workers,
+                   "budget_gb_per_worker": args.serve_budget_gb},
+        "per_graph_loop_graphs_per_s": loop_rate,
+        "saturated_vs_loop": out["saturated_graphs_per_s"] / loop_rate,
+        "loop_sample": "%d requests" % len(sample),
+        "alone_equals_mix": alone_ok and same_mix,
+        "generation_s": gen_s,
+    })
+    return out
+
+
 def workload(name):
     from louvaingpu_amd import generators
     if name == "rmat22":
@@ -240,6 +331,13 @@ def main():
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the batched-vs-per-graph measurement of the default run")
     ap.add_argument("--batch-graphs", type=int, default=4096)
+    ap.add_argument("--no-serve", action="store_true",
+                    help="skip the serving measurement of the default run")
+    ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-workers", type=int, default=2)
+    ap.add_argument("--serve-budget-gb", type=int, default=24)
+    ap.add_argument("--serve-sample", type=int, default=200)
     ap.add_argument("--batch-sample", type=int, default=64,
                     help="graphs of the per-graph loop sample (each at all resolutions)")
     ap.add_argument("--batch-chunk-edges", type=int, default=300_000_000,
@@ -252,6 +350,9 @@ def main():
 
     if args.workload == "batch":
         print(json.dumps(run_batched(args)), flush=True)
+        return 0
+    if args.workload == "serve":
+        print(json.dumps(run_serving(args)), flush=True)
         return 0
     from louvaingpu_amd import Engine
     from louvaingpu_amd._lib import LV_SWEEP_TIERS
@@ -381,6 +482,11 @@ def main():
         out["batched"] = {k: b[k] for k in (
             "value", "unit", "config", "batched_s", "batched_load_s", "per_graph_loop_s_est",
             "batch_vs_loop", "loop_sample", "validated")}
+    if not args.no_serve:
+        v = run_serving(args)
+        out["serving"] = {k: v[k] for k in (
+            "config", "p50_ms", "p99_ms", "graphs_per_s", "saturated_graphs_per_s",
+            "per_graph_loop_graphs_per_s", "saturated_vs_loop", "alone_equals_mix", "max_set")}
     print(json.dumps(out), flush=True)
     return 0
 

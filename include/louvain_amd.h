@@ -188,6 +188,11 @@ int lv_current_level(lv_ctx *ctx, int32_t *level, int32_t *n, int64_t *nnz);
 int lv_batch_load(lv_ctx *ctx, int32_t ngraphs, const int32_t *n, const int64_t *e,
                   const int32_t *src, const int32_t *dst, const void *w, int wtype, int32_t nres,
                   const double *gamma);
+/* The same with one pointer per graph for each array (no concatenation by the
+ * caller); the arrays are read before the call returns. */
+int lv_batch_load_graphs(lv_ctx *ctx, int32_t ngraphs, const int32_t *n, const int64_t *e,
+                         const int32_t *const *src, const int32_t *const *dst,
+                         const void *const *w, int wtype, int32_t nres, const double *gamma);
 int lv_batch_run(lv_ctx *ctx);
 int lv_batch_num_components(lv_ctx *ctx, int32_t *ncomp);
 int lv_batch_num_levels(lv_ctx *ctx, int32_t comp, int32_t *levels);

@@ -7,8 +7,9 @@ ctypes; there is no host fallback for any phase.
 """
 from .engine import Engine, generate_dendogram_phases, make_config  # noqa: F401
 from .batch import GraphSet  # noqa: F401
+from .serve import Server  # noqa: F401
 from .louvain import Dendrogram, count_partitions, generate_dendogram, start  # noqa: F401
 from . import generators  # noqa: F401
 
-__all__ = ["Engine", "GraphSet", "generate_dendogram_phases", "make_config", "Dendrogram",
+__all__ = ["Engine", "GraphSet", "Server", "generate_dendogram_phases", "make_config", "Dendrogram",
            "count_partitions", "generate_dendogram", "start", "generators"]

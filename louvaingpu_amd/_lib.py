@@ -38,13 +38,16 @@ ENGINE_SYMBOLS = (
     "lv_level_assignment", "lv_flatten", "lv_final_assignment", "lv_final_modularity",
     "lv_trace_next", "lv_trace_pass_q", "lv_level_csr", "lv_level_vertex", "lv_current_level",
     "lv_get_stats", "lv_reset_stats", "lv_last_run_ms",
-    "lv_batch_load", "lv_batch_run", "lv_batch_num_components", "lv_batch_num_levels",
+    "lv_batch_load", "lv_batch_load_graphs", "lv_batch_run", "lv_batch_num_components", "lv_batch_num_levels",
     "lv_batch_level_info", "lv_batch_level_assignment", "lv_batch_final_assignment",
 )
 HOST_SYMBOLS = (
     "lvh_generate_dendrogram", "lvh_num_levels", "lvh_level_count", "lvh_level_partition",
     "lvh_partition_at_level", "lvh_get_partitions", "lvh_count_partitions", "lvh_free",
     "lvg_sbm", "lvg_rmat", "lvg_num_nodes", "lvg_num_edges", "lvg_wtype", "lvg_copy", "lvg_free",
+)
+SERVE_SYMBOLS = (
+    "lvs_create", "lvs_submit", "lvs_wait", "lvs_get_stats", "lvs_now_ms", "lvs_destroy",
 )
 
 
@@ -70,6 +73,29 @@ class LvLevelInfo(ct.Structure):
         ("scale", ct.c_int32),
         ("q", ct.c_double),
         ("lower", ct.c_double),
+    ]
+
+
+class LvsResult(ct.Structure):
+    _fields_ = [
+        ("status", ct.c_int32),
+        ("levels", ct.c_int32),
+        ("communities", ct.c_int32),
+        ("batch_size", ct.c_int32),
+        ("q", ct.c_double),
+        ("t_submit_ms", ct.c_double),
+        ("t_start_ms", ct.c_double),
+        ("t_done_ms", ct.c_double),
+    ]
+
+
+class LvsStats(ct.Structure):
+    _fields_ = [
+        ("submitted", ct.c_int64),
+        ("completed", ct.c_int64),
+        ("sets", ct.c_int64),
+        ("max_set", ct.c_int32),
+        ("workers", ct.c_int32),
     ]
 
 
@@ -153,6 +179,9 @@ def engine():
         "lv_last_run_ms": (ct.c_int, [ctx, _DP]),
         "lv_batch_load": (ct.c_int, [ctx, ct.c_int32, _I32P, _I64P, _I32P, _I32P, _VP, ct.c_int,
                                      ct.c_int32, _DP]),
+        "lv_batch_load_graphs": (ct.c_int, [ctx, ct.c_int32, _I32P, _I64P, ct.POINTER(ct.c_void_p),
+                                            ct.POINTER(ct.c_void_p), ct.POINTER(ct.c_void_p),
+                                            ct.c_int, ct.c_int32, _DP]),
         "lv_batch_run": (ct.c_int, [ctx]),
         "lv_batch_num_components": (ct.c_int, [ctx, _I32P]),
         "lv_batch_num_levels": (ct.c_int, [ctx, ct.c_int32, _I32P]),
@@ -195,6 +224,13 @@ def host():
         "lvg_wtype": (ct.c_int, [d]),
         "lvg_copy": (ct.c_int, [d, _I32P, _I32P, ct.c_void_p]),
         "lvg_free": (None, [d]),
+        "lvs_create": (ct.c_int, [ct.POINTER(d), ct.POINTER(LvConfig), ct.c_int32, ct.c_uint64]),
+        "lvs_submit": (ct.c_int, [d, ct.c_int32, ct.c_int64, _I32P, _I32P, ct.c_void_p, ct.c_int,
+                                  ct.c_double, _I64P]),
+        "lvs_wait": (ct.c_int, [d, ct.c_int64, _I32P, ct.POINTER(LvsResult)]),
+        "lvs_get_stats": (ct.c_int, [d, ct.POINTER(LvsStats)]),
+        "lvs_now_ms": (ct.c_double, [d]),
+        "lvs_destroy": (ct.c_int, [d]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

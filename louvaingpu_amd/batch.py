@@ -53,18 +53,24 @@ class GraphSet:
             raise ValueError("empty graph list")
         ns = np.array([int(g[0]) for g in graphs], np.int32)
         es = np.array([len(g[1]) for g in graphs], np.int64)
-        src = np.ascontiguousarray(np.concatenate([np.asarray(g[1], np.int32) for g in graphs]))
-        dst = np.ascontiguousarray(np.concatenate([np.asarray(g[2], np.int32) for g in graphs]))
-        ws = [np.asarray(g[3]) for g in graphs]
+        srcs = [np.ascontiguousarray(g[1], np.int32) for g in graphs]
+        dsts = [np.ascontiguousarray(g[2], np.int32) for g in graphs]
+        ws = [np.ascontiguousarray(g[3]) for g in graphs]
+        if any(not (len(a) == len(b) == len(c)) for a, b, c in zip(srcs, dsts, ws)):
+            raise ValueError("src, dst and w of a graph must have the same length")
         dt = ws[0].dtype
         if any(w.dtype != dt for w in ws):
             raise TypeError("all graphs must share one weight dtype")
-        w, wtype = weights_array(np.concatenate(ws).astype(dt, copy=False))
+        _, wtype = weights_array(ws[0])
+        vp = ct.c_void_p * len(graphs)
+        ps = vp(*[a.ctypes.data for a in srcs])
+        pd = vp(*[a.ctypes.data for a in dsts])
+        pw = vp(*[a.ctypes.data for a in ws])
         gam = np.ascontiguousarray(gammas, np.float64)
-        check(self._lib.lv_batch_load(self._h, len(graphs), _p32(ns), _p64(es), _p32(src),
-                                      _p32(dst), w.ctypes.data_as(ct.c_void_p), wtype, len(gam),
-                                      gam.ctypes.data_as(ct.POINTER(ct.c_double))),
-              "lv_batch_load")
+        check(self._lib.lv_batch_load_graphs(self._h, len(graphs), _p32(ns), _p64(es), ps, pd, pw,
+                                             wtype, len(gam),
+                                             gam.ctypes.data_as(ct.POINTER(ct.c_double))),
+              "lv_batch_load_graphs")
         self.ngraphs, self.nres, self.n = len(graphs), len(gam), ns
         return self
 

@@ -114,6 +114,8 @@ struct lv_ctx {
     double *d_cm2 = nullptr, *d_cgam = nullptr;
     QAcc *d_cq = nullptr;
     ull *d_ccnt = nullptr;
+    void *h_stage = nullptr;                // pinned staging of small graphs' edges
+    size_t stage_cap = 0;
     QAcc *h_cq = nullptr;                   // pinned, ncomp
     ull *h_ccnt = nullptr;                  // pinned, ncomp
     int32_t h_cap = 0;                      // capacity of the pinned arrays
@@ -881,20 +883,36 @@ int flatten_final(lv_ctx *c)
 // ---------------------------------------------------------------------------
 // graph sets: the level loop of every component, run on the union graph
 
+// Frees pinned host memory; a failure is reported and cleared here so that it
+// cannot surface later as the error of an unrelated call.
+void pinned_free(void *&p)
+{
+    if (!p) return;
+    const hipError_t e = hipHostFree(p);
+    if (e != hipSuccess) {
+        (void)hip_fail(e, "hipHostFree", __FILE__, __LINE__);
+        (void)hipGetLastError();
+    }
+    p = nullptr;
+}
+
+// Pinned per-component buffers, grown geometrically (rarely reallocated).
 int pinned_cap(lv_ctx *c, int32_t need)
 {
     if (c->h_cap >= need) return LV_OK;
-    if (c->h_cq) (void)hipHostFree(c->h_cq);
-    if (c->h_ccnt) (void)hipHostFree(c->h_ccnt);
+    const int32_t cap = std::max(need, std::max(4096, 2 * c->h_cap));
+    void *a = c->h_cq, *b = c->h_ccnt;
+    pinned_free(a);
+    pinned_free(b);
     c->h_cq = nullptr;
     c->h_ccnt = nullptr;
     c->h_cap = 0;
-    if (hipHostMalloc(&c->h_cq, sizeof(QAcc) * (size_t)need, hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc(&c->h_ccnt, sizeof(ull) * (size_t)need, hipHostMallocDefault) != hipSuccess) {
+    if (hipHostMalloc(&c->h_cq, sizeof(QAcc) * (size_t)cap, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc(&c->h_ccnt, sizeof(ull) * (size_t)cap, hipHostMallocDefault) != hipSuccess) {
         (void)hipGetLastError();
         return LV_ERR_NOMEM;
     }
-    c->h_cap = need;
+    c->h_cap = cap;
     return LV_OK;
 }
 
@@ -1277,6 +1295,7 @@ int lv_destroy(lv_ctx *c)
     if (c->h_small) (void)hipHostFree(c->h_small);
     if (c->h_cq) (void)hipHostFree(c->h_cq);
     if (c->h_ccnt) (void)hipHostFree(c->h_ccnt);
+    if (c->h_stage) (void)hipHostFree(c->h_stage);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;  // the arena destructor frees the HBM block
     return LV_OK;
@@ -1704,9 +1723,76 @@ int lv_last_run_ms(lv_ctx *c, double *ms)
 // ---------------------------------------------------------------------------
 // graph sets
 
-int lv_batch_load(lv_ctx *c, int32_t ngraphs, const int32_t *n, const int64_t *e,
-                  const int32_t *src, const int32_t *dst, const void *w, int wtype, int32_t nres,
-                  const double *gamma)
+}  // extern "C"
+
+namespace {
+
+// Uploads the edge lists of graphs given by per-graph pointers into the base
+// arrays at beoff: graphs of at least kDirect edges are copied straight from
+// the caller's memory; runs of smaller graphs are packed into a pinned staging
+// buffer first, so a set of many small graphs costs a few DMA transfers.
+int upload_graphs(lv_ctx *c, int32_t ng, const int64_t *e, const std::vector<int64_t> &beoff,
+                  const int32_t *const *srcs, const int32_t *const *dsts, const void *const *ws,
+                  int32_t *bsrc, int32_t *bdst, uint32_t *bw)
+{
+    constexpr int64_t kDirect = 1 << 16;
+    int64_t small = 0;
+    for (int32_t g = 0; g < ng; ++g)
+        if (e[g] < kDirect) small += e[g];
+    const size_t need = (size_t)small * 12;
+    if (need > c->stage_cap) {
+        const size_t cap = std::max(need, std::max((size_t)64 << 20, 2 * c->stage_cap));
+        pinned_free(c->h_stage);
+        c->stage_cap = 0;
+        if (hipHostMalloc(&c->h_stage, cap, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            return LV_ERR_NOMEM;
+        }
+        c->stage_cap = cap;
+    }
+    int32_t *ss = (int32_t *)c->h_stage;
+    int32_t *sd = ss + small;
+    uint32_t *sw = (uint32_t *)(sd + small);
+    int64_t run_dev = 0, run_stage = 0, run_len = 0, soff = 0;
+    auto flush = [&]() -> int {
+        if (run_len) {
+            LV_TRY(h2d(c, bsrc + run_dev, (const int32_t *)(ss + run_stage), run_len));
+            LV_TRY(h2d(c, bdst + run_dev, (const int32_t *)(sd + run_stage), run_len));
+            LV_TRY(h2d(c, bw + run_dev, (const uint32_t *)(sw + run_stage), run_len));
+        }
+        run_len = 0;
+        return LV_OK;
+    };
+    for (int32_t g = 0; g < ng; ++g) {
+        const int64_t eg = e[g];
+        if (eg == 0) continue;
+        if (!srcs[g] || !dsts[g] || !ws[g]) return LV_ERR_ARG;
+        if (eg >= kDirect) {
+            LV_TRY(flush());
+            LV_TRY(h2d(c, bsrc + beoff[g], srcs[g], eg));
+            LV_TRY(h2d(c, bdst + beoff[g], dsts[g], eg));
+            LV_TRY(h2d(c, bw + beoff[g], (const uint32_t *)ws[g], eg));
+            continue;
+        }
+        if (run_len == 0) {
+            run_dev = beoff[g];
+            run_stage = soff;
+        }
+        memcpy(ss + soff, srcs[g], sizeof(int32_t) * (size_t)eg);
+        memcpy(sd + soff, dsts[g], sizeof(int32_t) * (size_t)eg);
+        memcpy(sw + soff, ws[g], sizeof(uint32_t) * (size_t)eg);
+        soff += eg;
+        run_len += eg;
+    }
+    return flush();
+}
+
+// lv_batch_load / lv_batch_load_graphs: either one contiguous edge list
+// (src/dst/w) or per-graph pointers (srcs/dsts/ws).
+int batch_load(lv_ctx *c, int32_t ngraphs, const int32_t *n, const int64_t *e, const int32_t *src,
+               const int32_t *dst, const void *w, const int32_t *const *srcs,
+               const int32_t *const *dsts, const void *const *ws, int wtype, int32_t nres,
+               const double *gamma)
 {
     LV_TRY(check_ctx(c));
     if (ngraphs < 1 || nres < 1 || !n || !e || !gamma ||
@@ -1721,7 +1807,8 @@ int lv_batch_load(lv_ctx *c, int32_t ngraphs, const int32_t *n, const int64_t *e
     }
     for (int32_t r = 0; r < nres; ++r)
         if (!(gamma[r] >= 0.0)) return LV_ERR_ARG;
-    if (eb > 0 && (!src || !dst || !w)) return LV_ERR_ARG;
+    if (eb > 0 && !srcs && (!src || !dst || !w)) return LV_ERR_ARG;
+    if (eb > 0 && srcs && (!dsts || !ws)) return LV_ERR_ARG;
     const int64_t nu = nb * nres, eu = eb * nres;
     if (nu > INT32_MAX) return LV_ERR_RANGE;
     LV_TRY(set_device(c));
@@ -1776,9 +1863,13 @@ int lv_batch_load(lv_ctx *c, int32_t ngraphs, const int32_t *n, const int64_t *e
     LV_TRY(arena_alloc(A, c->d_src, eu));
     LV_TRY(arena_alloc(A, c->d_dst, eu));
     LV_TRY(arena_alloc(A, c->d_w, eu));
-    LV_TRY(h2d(c, bsrc, src, eb));
-    LV_TRY(h2d(c, bdst, dst, eb));
-    LV_TRY(h2d(c, bw, (const uint32_t *)w, eb));
+    if (srcs) {
+        LV_TRY(upload_graphs(c, ngraphs, e, beoff, srcs, dsts, ws, bsrc, bdst, bw));
+    } else {
+        LV_TRY(h2d(c, bsrc, src, eb));
+        LV_TRY(h2d(c, bdst, dst, eb));
+        LV_TRY(h2d(c, bw, (const uint32_t *)w, eb));
+    }
     LV_TRY(h2d(c, bn, n, ngraphs));
     LV_TRY(h2d(c, dbeoff, (const int64_t *)beoff.data(), (int64_t)ngraphs + 1));
     LV_TRY(h2d(c, dueoff, (const int64_t *)ueoff.data(), (int64_t)nc + 1));
@@ -1838,6 +1929,27 @@ int lv_batch_load(lv_ctx *c, int32_t ngraphs, const int32_t *n, const int64_t *e
     c->level = 0;
     c->state = ST_LOADED;
     return LV_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lv_batch_load(lv_ctx *c, int32_t ngraphs, const int32_t *n, const int64_t *e,
+                  const int32_t *src, const int32_t *dst, const void *w, int wtype, int32_t nres,
+                  const double *gamma)
+{
+    return batch_load(c, ngraphs, n, e, src, dst, w, nullptr, nullptr, nullptr, wtype, nres,
+                      gamma);
+}
+
+int lv_batch_load_graphs(lv_ctx *c, int32_t ngraphs, const int32_t *n, const int64_t *e,
+                         const int32_t *const *src, const int32_t *const *dst,
+                         const void *const *w, int wtype, int32_t nres, const double *gamma)
+{
+    if (!src) return LV_ERR_ARG;
+    return batch_load(c, ngraphs, n, e, nullptr, nullptr, nullptr, src, dst, w, wtype, nres,
+                      gamma);
 }
 
 int lv_batch_run(lv_ctx *c)

@@ -34,6 +34,24 @@ def test_host_exports_every_declared_symbol():
     assert set(names) == set(_lib.HOST_SYMBOLS)
 
 
+def test_server_exports_every_declared_symbol():
+    names = declared("louvain_serve.h", "lvs_")
+    lib = _lib.host()
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(_lib.SERVE_SYMBOLS)
+
+
+def test_server_argument_errors_without_device():
+    lib = _lib.host()
+    assert lib.lvs_destroy(None) == _lib.LV_ERR_ARG
+    assert lib.lvs_create(None, None, 1, 0) == _lib.LV_ERR_ARG
+    h = ct.c_void_p()
+    assert lib.lvs_create(ct.byref(h), None, 0, 0) == _lib.LV_ERR_ARG
+    rid = ct.c_int64()
+    assert lib.lvs_submit(None, 1, 0, None, None, None, 0, 1.0, ct.byref(rid)) == _lib.LV_ERR_ARG
+
+
 def test_strerror_and_defaults():
     assert "invalid input" in _lib.strerror(_lib.LV_ERR_INVALID_INPUT)
     assert _lib.strerror(12345) == "unknown status"

@@ -139,3 +139,23 @@ def test_set_edgeless_and_invalid():
         with pytest.raises(LouvainError) as ei:
             gs.load([bad], [1.0])
         assert ei.value.status == LV_ERR_INVALID_INPUT
+
+
+def test_reloading_one_context_with_growing_sets():
+    # later loads need more components and more staged edges than earlier ones,
+    # so the pinned buffers of the context grow between runs
+    sizes = [2, 30, 200]
+    params = generators.batch_params(max(sizes), seed=5)
+    graphs = [generators.sbm(min(n, 1500), 100, p, p / 50, sd) for n, p, sd in params]
+    gpu_case("batch reload growing sets %s" % sizes)
+    with GraphSet() as gs:
+        for k in sizes:
+            gs.load(graphs[:k], [1.0, 2.0])
+            gs.run()
+            for gi in (0, k // 2, k - 1):
+                for ri, gamma in enumerate([1.0, 2.0]):
+                    eng = _single(graphs[gi], gamma)
+                    try:
+                        assert np.array_equal(gs.final(gi * 2 + ri), eng.final_assignment()), (k, gi)
+                    finally:
+                        eng.close()
