@@ -161,6 +161,16 @@ int lv_final_assignment(lv_ctx *ctx, int32_t *comm);
  * aggregates, evaluated in FP64). */
 int lv_final_modularity(lv_ctx *ctx, double *q);
 
+/* calculateClusterSize + layoutProcess (LouvainCUDA/LouvainCPP.cpp:555-651) for a
+ * partition given per position (comm[i] in [0, ncomm)): cluster_end[c] = the
+ * reference's clusterSize[c] (inclusive prefix sum of cluster sizes),
+ * index_in_cluster[i] = earlier positions in the same cluster, and the circle
+ * layout x = 50 cos(2 pi s / n), y = 50 + 50 sin(2 pi s / n) at slot
+ * s = cluster_end[c-1] + index_in_cluster[i] (double, stored as float).
+ * Independent of the loaded graph; uses the context's arena. */
+int lv_cluster_layout(lv_ctx *ctx, int32_t n, const int32_t *comm, int32_t ncomm,
+                      int32_t *cluster_end, int32_t *index_in_cluster, float *x, float *y);
+
 /* Parity instrumentation. */
 /* next[] proposed by `pass` (1-based) of `level`, after the guard (and, on the
  * level's last pass, after the merge-isolated step); needs LV_FLAG_TRACE. */

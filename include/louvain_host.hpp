@@ -70,13 +70,28 @@ public:
     int getEdgeSize();
     int getIdxOfNode(int node);
 
+    /* Per-node cluster, index within the cluster and circle position, filled by
+     * calculateClusterSize and layoutProcess (the reference's Graph.h keeps
+     * them per node too). */
+    void setCluster(int idx, int cluster);
+    int getCluster(int idx);
+    void setIndexInCluster(int idx, int index);
+    int getIndexInCluster(int idx);
+    float getX(int idx);
+    float getY(int idx);
+    float *getXPtr(int idx);
+    float *getYPtr(int idx);
+
 private:
+    void ensureNodeAttrs();
     int *nodes = nullptr;
     int nodeCapacity = 0;
     int nodeSize = 0;
     int edgeSize = 0;
     EdgeContainer edges;
     std::map<int, int> nodeMap;
+    std::vector<int> cluster_, indexInCluster_;
+    std::vector<float> x_, y_;
 };
 
 struct Community {
@@ -128,6 +143,18 @@ Partition getFinalPartition(std::vector<Partition> &dendogram);
 
 /* countPartitions (LouvainCPP.cpp:116-127): 1 + the largest community id. */
 int countPartitions(Partition partitions);
+
+/* calculateClusterSize (LouvainCPP.cpp:555-583): each node's cluster (from the
+ * partition, by node id) and index among earlier nodes of its cluster; returns
+ * new int[numberOfClusters] with the inclusive prefix sums of the cluster sizes
+ * (delete[] by the caller).  A node missing from the partition throws
+ * louvain::Error (the reference reads an uninitialised cluster). */
+int *calculateClusterSize(Graph &graph, Partition partitions, int numberOfClusters);
+
+/* layoutProcess (LouvainCPP.cpp:585-651): node i of cluster c goes to slot
+ * clusterSize[c-1] + indexInCluster(i) of a circle of radius 50 centred at
+ * (0, 50): x = 50 cos(2 pi slot / n), y = 50 + 50 sin(2 pi slot / n). */
+void layoutProcess(int *clusterSize, Graph &graph, int nrPartitions);
 
 /* Text loaders of the reference CLI (LouvainCPP.cpp:660-737): a count, then the
  * ids (nodes.txt) or `source target weight` triples (edges.txt). */

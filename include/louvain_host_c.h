@@ -38,6 +38,13 @@ int lvh_get_partitions(lvh_dendrogram *d, int level, int32_t *node, int32_t *com
 /* countPartitions of partitionAtLevel(level) (LouvainCUDA/LouvainCPP.cpp:116-127). */
 int lvh_count_partitions(lvh_dendrogram *d, int level);
 void lvh_free(lvh_dendrogram *d);
+/* calculateClusterSize + layoutProcess (LouvainCUDA/LouvainCPP.cpp:555-651) for the
+ * graph with node ids ids[0..n-1] and a Partition (node, comm) of count entries:
+ * cluster_size[nclusters] (inclusive prefix sums), and per node (ids order) its
+ * cluster, index in cluster and circle position. */
+int lvh_cluster_layout(int32_t n, const int32_t *ids, int32_t count, const int32_t *node,
+                       const int32_t *comm, int32_t nclusters, int32_t *cluster_size,
+                       int32_t *cluster, int32_t *index_in_cluster, float *x, float *y);
 
 lvg_graph *lvg_sbm(int32_t n, int32_t block_size, double p_in, double p_out, uint64_t seed);
 lvg_graph *lvg_rmat(int32_t scale, int64_t edge_factor, double a, double b, double c,

@@ -36,7 +36,7 @@ ENGINE_SYMBOLS = (
     "lv_neighbour_compute", "lv_init", "lv_modularity", "lv_one_level", "lv_renumber",
     "lv_induce", "lv_run", "lv_num_levels", "lv_level_info_get", "lv_level_partition",
     "lv_level_assignment", "lv_flatten", "lv_final_assignment", "lv_final_modularity",
-    "lv_trace_next", "lv_trace_pass_q", "lv_level_csr", "lv_level_vertex", "lv_current_level",
+    "lv_cluster_layout", "lv_trace_next", "lv_trace_pass_q", "lv_level_csr", "lv_level_vertex", "lv_current_level",
     "lv_get_stats", "lv_reset_stats", "lv_last_run_ms",
     "lv_batch_load", "lv_batch_load_graphs", "lv_batch_run", "lv_batch_num_components", "lv_batch_num_levels",
     "lv_batch_level_info", "lv_batch_level_assignment", "lv_batch_final_assignment",
@@ -44,6 +44,7 @@ ENGINE_SYMBOLS = (
 HOST_SYMBOLS = (
     "lvh_generate_dendrogram", "lvh_num_levels", "lvh_level_count", "lvh_level_partition",
     "lvh_partition_at_level", "lvh_get_partitions", "lvh_count_partitions", "lvh_free",
+    "lvh_cluster_layout",
     "lvg_sbm", "lvg_rmat", "lvg_num_nodes", "lvg_num_edges", "lvg_wtype", "lvg_copy", "lvg_free",
 )
 SERVE_SYMBOLS = (
@@ -169,6 +170,8 @@ def engine():
         "lv_flatten": (ct.c_int, [ctx, ct.c_int32, _I32P, _I32P]),
         "lv_final_assignment": (ct.c_int, [ctx, _I32P]),
         "lv_final_modularity": (ct.c_int, [ctx, _DP]),
+        "lv_cluster_layout": (ct.c_int, [ctx, ct.c_int32, _I32P, ct.c_int32, _I32P, _I32P,
+                                         ct.POINTER(ct.c_float), ct.POINTER(ct.c_float)]),
         "lv_trace_next": (ct.c_int, [ctx, ct.c_int32, ct.c_int32, _I32P]),
         "lv_trace_pass_q": (ct.c_int, [ctx, ct.c_int32, ct.c_int32, _DP]),
         "lv_level_csr": (ct.c_int, [ctx, _I64P, _I32P, _I64P]),
@@ -216,6 +219,9 @@ def host():
         "lvh_get_partitions": (ct.c_int, [d, ct.c_int, _I32P, _I32P]),
         "lvh_count_partitions": (ct.c_int, [d, ct.c_int]),
         "lvh_free": (None, [d]),
+        "lvh_cluster_layout": (ct.c_int, [ct.c_int32, _I32P, ct.c_int32, _I32P, _I32P, ct.c_int32,
+                                          _I32P, _I32P, _I32P, ct.POINTER(ct.c_float),
+                                          ct.POINTER(ct.c_float)]),
         "lvg_sbm": (ct.c_void_p, [ct.c_int32, ct.c_int32, ct.c_double, ct.c_double, ct.c_uint64]),
         "lvg_rmat": (ct.c_void_p, [ct.c_int32, ct.c_int64, ct.c_double, ct.c_double,
                                    ct.c_double, ct.c_uint64, ct.c_int]),

@@ -1638,6 +1638,65 @@ int lv_final_modularity(lv_ctx *c, double *q)
     return LV_OK;
 }
 
+int lv_cluster_layout(lv_ctx *c, int32_t n, const int32_t *comm, int32_t ncomm,
+                      int32_t *cluster_end, int32_t *index_in_cluster, float *x, float *y)
+{
+    LV_TRY(check_ctx(c));
+    if (n < 0 || ncomm < 0 || (n > 0 && (!comm || !index_in_cluster || !x || !y)) ||
+        (ncomm > 0 && !cluster_end))
+        return LV_ERR_ARG;
+    LV_TRY(set_device(c));
+    if (n == 0) {
+        for (int32_t k = 0; k < ncomm; ++k) cluster_end[k] = 0;
+        return LV_OK;
+    }
+    if (ncomm == 0) return LV_ERR_INVALID_INPUT;
+    Arena &A = c->arena;
+    hipStream_t s = c->stream;
+    if (A.capacity() == 0) LV_TRY(A.reserve((size_t)64 * (size_t)n + (size_t)16 * (size_t)ncomm + (64u << 20)));
+    uint32_t *k1 = nullptr, *k2 = nullptr;
+    int32_t *v1 = nullptr, *v2 = nullptr, *first = nullptr, *count = nullptr, *idx = nullptr;
+    int64_t *cnt64 = nullptr, *end64 = nullptr;
+    float *dx = nullptr, *dy = nullptr;
+    LV_TRY(arena_alloc(A, k1, n));
+    LV_TRY(arena_alloc(A, k2, n));
+    LV_TRY(arena_alloc(A, v1, n));
+    LV_TRY(arena_alloc(A, v2, n));
+    LV_TRY(arena_alloc(A, first, ncomm));
+    LV_TRY(arena_alloc(A, count, ncomm));
+    LV_TRY(arena_alloc(A, idx, n));
+    LV_TRY(arena_alloc(A, cnt64, ncomm));
+    LV_TRY(arena_alloc(A, end64, (int64_t)ncomm + 1));
+    LV_TRY(arena_alloc(A, dx, n));
+    LV_TRY(arena_alloc(A, dy, n));
+    LV_TRY(h2d(c, (int32_t *)k1, comm, n));
+    iota_i32(s, v1, n);
+    fill_i32(s, count, ncomm, 0);
+    LV_HIP(hipMemsetAsync(c->d_flags, 0, sizeof(DevFlags), s));
+    uint32_t *sk = nullptr;
+    int32_t *perm = nullptr;
+    LV_TRY(radix_sort_u32_i32(A, s, k1, v1, k2, v2, n, 32, &sk, &perm));
+    launch_layout(s, sk, perm, n, ncomm, first, count, idx, dx, dy, c->d_flags);
+    launch_widen_i32(s, count, ncomm, cnt64);
+    LV_TRY(scan_exclusive_i64(A, s, cnt64, end64, ncomm, end64 + ncomm));
+    LV_HIP(hipGetLastError());
+    LV_TRY(d2h(c, c->h_flags, c->d_flags, 1));
+    LV_TRY(sync(c));
+    int rc = c->h_flags->err ? LV_ERR_INVALID_INPUT : LV_OK;
+    if (rc == LV_OK) {
+        std::vector<int64_t> e64((size_t)ncomm + 1);
+        LV_TRY(d2h(c, e64.data(), (const int64_t *)end64, (int64_t)ncomm + 1));
+        LV_TRY(d2h(c, index_in_cluster, idx, n));
+        LV_TRY(d2h(c, x, dx, n));
+        LV_TRY(d2h(c, y, dy, n));
+        LV_TRY(sync(c));
+        for (int32_t k = 0; k < ncomm; ++k) cluster_end[k] = (int32_t)e64[k + 1];  // inclusive
+    }
+    void *tmp[] = {k1, k2, v1, v2, first, count, idx, cnt64, end64, dx, dy};
+    for (void *p : tmp) A.release(p);
+    return rc;
+}
+
 int lv_trace_next(lv_ctx *c, int32_t level, int32_t pass, int32_t *next)
 {
     LV_TRY(check_ctx(c));

@@ -346,6 +346,10 @@ void launch_cr_vertex(hipStream_t s, const int32_t *newid_g, const ull *ssum, co
                       long long *msum);
 
 void launch_compose(hipStream_t s, int32_t *f, int32_t n, const int32_t *map);
+// skey/perm: cluster keys sorted stably and the positions they came from.
+void launch_layout(hipStream_t s, const uint32_t *skey, const int32_t *perm, int32_t n,
+                   int32_t ncomm, int32_t *first, int32_t *count, int32_t *index_in_cluster,
+                   float *x, float *y, DevFlags *f);
 void launch_gather_partition(hipStream_t s, const int32_t *perm, const int32_t *ids,
                              const int32_t *comm_by_pos, int32_t n, int32_t *node,
                              int32_t *comm);

@@ -1216,6 +1216,56 @@ void launch_gather_partition(hipStream_t s, const int32_t *perm, const int32_t *
                            n, node, comm);
 }
 
+// ---------------------------------------------------------------------------
+// Cluster sizes and circle layout of a partition (calculateClusterSize and
+// layoutProcess, LouvainCUDA/LouvainCPP.cpp:555-651).  With positions stably
+// sorted by cluster, position j of the sorted order is the reference's
+// clusterSize[c-1] + indexInCluster: the node's slot on the circle.
+
+__global__ void k_layout_first(const uint32_t *skey, int32_t n, int32_t ncomm, int32_t *first,
+                               DevFlags *f)
+{
+    int bad = 0;
+    GRID_STRIDE(j, n)
+    {
+        const uint32_t c = skey[j];
+        if (c >= (uint32_t)ncomm) {
+            bad = 1;
+            continue;
+        }
+        if (j == 0 || skey[j - 1] != c) first[c] = (int32_t)j;
+    }
+    bad = wave_max_i(bad);
+    if ((threadIdx.x & 63) == 0 && bad) atomicOr(&f->err, 1);
+}
+
+__global__ void k_layout_place(const uint32_t *skey, const int32_t *perm, int32_t n, int32_t ncomm,
+                               const int32_t *first, int32_t *count, int32_t *index_in_cluster,
+                               float *x, float *y)
+{
+    GRID_STRIDE(j, n)
+    {
+        const uint32_t c = skey[j];
+        if (c >= (uint32_t)ncomm) continue;
+        const int32_t v = perm[j];
+        index_in_cluster[v] = (int32_t)j - first[c];
+        if (j == n - 1 || skey[j + 1] != c) count[c] = (int32_t)j + 1 - first[c];
+        const int32_t i = (int32_t)j;
+        x[v] = (float)(50 * cos(2 * M_PI * i / n));
+        y[v] = (float)(50 + 50 * sin(2 * M_PI * i / n));
+    }
+}
+
+void launch_layout(hipStream_t s, const uint32_t *skey, const int32_t *perm, int32_t n,
+                   int32_t ncomm, int32_t *first, int32_t *count, int32_t *index_in_cluster,
+                   float *x, float *y, DevFlags *f)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_layout_first, grid_for(n, 256), 256, 0, s, skey, n, ncomm, first, f);
+    hipLaunchKernelGGL(k_layout_place, grid_for(n, 256), 256, 0, s, skey, perm, n, ncomm, first,
+                       count, index_in_cluster, x, y);
+}
+
 void launch_final_q(hipStream_t s, const int32_t *src, const int32_t *dst, const int64_t *w,
                     int64_t e, const int32_t *comm, ull *lc, ull *dc)
 {

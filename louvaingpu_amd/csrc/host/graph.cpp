@@ -114,6 +114,57 @@ int Graph::getIdxOfNode(int node)
     return it == nodeMap.end() ? -1 : it->second;
 }
 
+void Graph::ensureNodeAttrs()
+{
+    if ((int)cluster_.size() != nodeSize) {
+        cluster_.assign((size_t)nodeSize, 0);
+        indexInCluster_.assign((size_t)nodeSize, 0);
+        x_.assign((size_t)nodeSize, 0.f);
+        y_.assign((size_t)nodeSize, 0.f);
+    }
+}
+
+void Graph::setCluster(int idx, int cluster)
+{
+    ensureNodeAttrs();
+    cluster_.at((size_t)idx) = cluster;
+}
+int Graph::getCluster(int idx)
+{
+    ensureNodeAttrs();
+    return cluster_.at((size_t)idx);
+}
+void Graph::setIndexInCluster(int idx, int index)
+{
+    ensureNodeAttrs();
+    indexInCluster_.at((size_t)idx) = index;
+}
+int Graph::getIndexInCluster(int idx)
+{
+    ensureNodeAttrs();
+    return indexInCluster_.at((size_t)idx);
+}
+float Graph::getX(int idx)
+{
+    ensureNodeAttrs();
+    return x_.at((size_t)idx);
+}
+float Graph::getY(int idx)
+{
+    ensureNodeAttrs();
+    return y_.at((size_t)idx);
+}
+float *Graph::getXPtr(int idx)
+{
+    ensureNodeAttrs();
+    return &x_.at((size_t)idx);
+}
+float *Graph::getYPtr(int idx)
+{
+    ensureNodeAttrs();
+    return &y_.at((size_t)idx);
+}
+
 // The reference reads a count and then tokens until end of file; every token
 // after the count is taken here (the reference drops the last node id when the
 // file lacks trailing whitespace, LouvainCPP.cpp:676-692).

@@ -4,6 +4,8 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <cmath>
+#include <unordered_map>
 
 #include "louvain_host.hpp"
 
@@ -171,6 +173,44 @@ int countPartitions(Partition partitions)
     return 1 + mx;
 }
 
+int *calculateClusterSize(Graph &graph, Partition partitions, int numberOfClusters)
+{
+    if (numberOfClusters < 0) throw louvain::Error(LV_ERR_ARG, "calculateClusterSize");
+    std::unordered_map<int, int> comm;
+    comm.reserve((size_t)partitions.count * 2);
+    for (int j = 0; j < partitions.count; ++j)  // the first entry of a node wins, as the
+        comm.emplace(partitions.partition.node[j], partitions.partition.community[j]);  // scan does
+    int *clusterSize = new int[numberOfClusters > 0 ? numberOfClusters : 1]();
+    const int n = graph.getNodeSize();
+    for (int i = 0; i < n; ++i) {
+        auto it = comm.find(graph.getNodeAt(i));
+        if (it == comm.end() || it->second < 0 || it->second >= numberOfClusters) {
+            delete[] clusterSize;
+            throw louvain::Error(LV_ERR_INVALID_INPUT, "calculateClusterSize");
+        }
+        const int c = it->second;
+        graph.setCluster(i, c);
+        graph.setIndexInCluster(i, clusterSize[c]);
+        clusterSize[c] += 1;
+    }
+    for (int i = 1; i < numberOfClusters; ++i) clusterSize[i] += clusterSize[i - 1];
+    return clusterSize;
+}
+
+void layoutProcess(int *clusterSize, Graph &graph, int nrPartitions)
+{
+    const int n = graph.getNodeSize();
+    if (n > 0) graph.getXPtr(0);  // sizes the per-node arrays before the threads start
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i) {
+        const int c = graph.getCluster(i);
+        if (c < 0 || c >= nrPartitions) continue;
+        const int slot = (c > 0 ? clusterSize[c - 1] : 0) + graph.getIndexInCluster(i);
+        *graph.getXPtr(i) = (float)(50 * cos(2 * M_PI * slot / n));
+        *graph.getYPtr(i) = (float)(50 + 50 * sin(2 * M_PI * slot / n));
+    }
+}
+
 // ---------------------------------------------------------------------------
 // flat C API
 
@@ -265,5 +305,38 @@ int lvh_count_partitions(lvh_dendrogram *d, int level)
 }
 
 void lvh_free(lvh_dendrogram *d) { delete d; }
+
+int lvh_cluster_layout(int32_t n, const int32_t *ids, int32_t count, const int32_t *node,
+                       const int32_t *comm, int32_t nclusters, int32_t *cluster_size,
+                       int32_t *cluster, int32_t *index_in_cluster, float *x, float *y)
+{
+    if (n < 0 || count < 0 || nclusters < 0 || (n > 0 && (!ids || !cluster || !index_in_cluster ||
+                                                           !x || !y)) ||
+        (count > 0 && (!node || !comm)) || (nclusters > 0 && !cluster_size))
+        return LV_ERR_ARG;
+    try {
+        Graph g;
+        g.setNodes(const_cast<int *>(ids), n);
+        Partition p;
+        p.count = count;
+        p.partition.node = const_cast<int *>(node);
+        p.partition.community = const_cast<int *>(comm);
+        int *cs = calculateClusterSize(g, p, nclusters);
+        layoutProcess(cs, g, nclusters);
+        for (int32_t k = 0; k < nclusters; ++k) cluster_size[k] = cs[k];
+        delete[] cs;
+        for (int32_t i = 0; i < n; ++i) {
+            cluster[i] = g.getCluster(i);
+            index_in_cluster[i] = g.getIndexInCluster(i);
+            x[i] = g.getX(i);
+            y[i] = g.getY(i);
+        }
+    } catch (const louvain::Error &e) {
+        return e.status();
+    } catch (...) {
+        return LV_ERR_NOMEM;
+    }
+    return LV_OK;
+}
 
 }  // extern "C"

@@ -98,6 +98,24 @@ class Engine:
         self.n, self.e, self.ids = int(n), len(src), ids_arr
         return self
 
+    # post-processing --------------------------------------------------------
+    def cluster_layout(self, comm, ncomm=None):
+        """calculateClusterSize + layoutProcess (LouvainCUDA/LouvainCPP.cpp:555-651) on
+        the device for a partition given per position: (cluster_end,
+        index_in_cluster, x, y)."""
+        comm = np.ascontiguousarray(comm, np.int32)
+        n = len(comm)
+        k = int(comm.max()) + 1 if ncomm is None and n else int(ncomm or 0)
+        ce = np.empty(max(k, 1), np.int32)
+        ix = np.empty(n, np.int32)
+        x = np.empty(n, np.float32)
+        y = np.empty(n, np.float32)
+        fp = ct.POINTER(ct.c_float)
+        check(self._lib.lv_cluster_layout(self._h, n, _p32(comm), k, _p32(ce), _p32(ix),
+                                          x.ctypes.data_as(fp), y.ctypes.data_as(fp)),
+              "lv_cluster_layout")
+        return ce[:k], ix, x, y
+
     # phases (reference names) ---------------------------------------------
     def CalculateNeighbour(self):
         check(self._lib.lv_neighbour_compute(self._h), "lv_neighbour_compute")

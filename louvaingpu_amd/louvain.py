@@ -19,7 +19,8 @@ import ctypes as ct
 import numpy as np
 
 from . import _lib
-from .engine import make_config
+from ._lib import check
+from .engine import _p32, make_config
 
 
 class Dendrogram:
@@ -93,6 +94,27 @@ def generate_dendogram(n, src, dst, w, ids=None, **config) -> Dendrogram:
     return Dendrogram(h)
 
 
+def cluster_layout(ids, node, comm, nclusters):
+    """calculateClusterSize + layoutProcess (LouvainCUDA/LouvainCPP.cpp:555-651) on
+    the host library: (cluster_size, cluster, index_in_cluster, x, y) for the
+    nodes ``ids`` under the Partition (node, comm)."""
+    lib = _lib.host()
+    ids = np.ascontiguousarray(ids, np.int32)
+    node = np.ascontiguousarray(node, np.int32)
+    comm = np.ascontiguousarray(comm, np.int32)
+    n = len(ids)
+    cs = np.empty(max(int(nclusters), 1), np.int32)
+    cl = np.empty(n, np.int32)
+    ix = np.empty(n, np.int32)
+    x = np.empty(n, np.float32)
+    y = np.empty(n, np.float32)
+    fp = ct.POINTER(ct.c_float)
+    check(lib.lvh_cluster_layout(n, _p32(ids), len(node), _p32(node), _p32(comm), int(nclusters),
+                                 _p32(cs), _p32(cl), _p32(ix), x.ctypes.data_as(fp),
+                                 y.ctypes.data_as(fp)), "lvh_cluster_layout")
+    return cs[:int(nclusters)], cl, ix, x, y
+
+
 def start(nodeList, edgeList, part_init=None, **config):
     """Boost.Python ``start`` (LouvainCUDA/LouvainCPP.cpp:808-941)."""
     ids = np.array([int(d["id"]) for d in nodeList], np.int32)
@@ -104,11 +126,14 @@ def start(nodeList, edgeList, part_init=None, **config):
         node, comm = d.get_partitions()
     finally:
         d.close()
-    result = {"size": count_partitions(comm)}
+    size = count_partitions(comm)
+    result = {"size": size}
     for a, b in zip(node.tolist(), comm.tolist()):
         result[a] = b
-    for nd in nodeList:
-        c = result.get(int(nd["id"]))
-        nd["cluster"] = c
-        nd["my_community"] = c
+    _, cl, _, x, y = cluster_layout(ids, node, comm, size)
+    for i, nd in enumerate(nodeList):
+        nd["cluster"] = int(cl[i])
+        nd["my_community"] = int(cl[i])
+        nd["x"] = float(x[i])
+        nd["y"] = float(y[i])
     return result

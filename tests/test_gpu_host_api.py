@@ -53,3 +53,9 @@ def test_start_dict_api():
         assert res[a] == c
     for nd in nodes:
         assert nd["cluster"] == res[nd["id"]]
+    # circle layout of the reference's layoutProcess on the same partition
+    from test_layout import reference_layout
+    _, _, _, x, y = reference_layout(ids.tolist(), onode.tolist(), ocomm.tolist(),
+                                     count_partitions(ocomm))
+    assert [nd["x"] for nd in nodes] == [float(v) for v in x]
+    assert [nd["y"] for nd in nodes] == [float(v) for v in y]
