@@ -77,6 +77,10 @@ struct lv_ctx {
     ull *ddeg = nullptr, *din = nullptr;
     int32_t *verts = nullptr;
     int64_t tier_cnt[kTiers + 1] = {0}, tier_nnz[kTiers + 1] = {0}, tier_off[kTiers + 1] = {0};
+    // every row of the level lists each neighbour once (coarsened levels always;
+    // level 0 when the CSR rows are strictly increasing): the first pass of the
+    // level can then skip aggregation
+    bool rows_unique = false;
     // hub tier plan (rows with q > 2048), see HubArgs
     HubArgs hub{};
     int32_t n_hubs = 0;
@@ -256,6 +260,11 @@ void free_sweep_state(lv_ctx *c)
     a.release((void *)h.pc);
     a.release((void *)h.pw);
     a.release((void *)h.pn);
+    a.release((void *)h.cbase);
+    a.release((void *)h.cg);
+    a.release((void *)h.cc);
+    a.release((void *)h.cw);
+    a.release((void *)h.cn);
     c->hub = HubArgs{};
     c->n_hubs = 0;
     c->hub_parts = c->hub_chunks = c->hub_entries = 0;
@@ -404,7 +413,12 @@ int build_level0_csr(lv_ctx *c)
     else
         launch_csr_gather(s, sk, sv, c->nnz, c->d_src, c->d_dst, c->d_w, c->col, c->cw, c->kdeg);
     launch_loop_gather(s, lastloop, c->d_w, n, c->loopw);
+    LV_HIP(hipMemsetAsync(&c->d_flags->err, 0, sizeof(int), s));
+    launch_rows_check(s, sk, c->col, c->nnz, c->d_flags);
     LV_HIP(hipGetLastError());
+    LV_TRY(d2h(c, c->h_flags, c->d_flags, 1));
+    LV_TRY(sync(c));
+    c->rows_unique = (c->h_flags->err & 4) == 0;
     t_end(c, t0, CAT_CSR);
     arena_free(A, lastloop);
     arena_free(A, k1);
@@ -472,9 +486,9 @@ int plan_hubs(lv_ctx *c, int32_t nh)
     h.part_hub = part_hub;
     h.chunk_hub = chunk_hub;
     h.chunk_k = chunk_k;
+    h.cbase = cbase;
     arena_free(A, hP64);
     arena_free(A, hch);
-    arena_free(A, cbase);
     const int64_t E = c->hub_entries, P = c->hub_parts;
     LV_TRY(arena_alloc(A, h.tmp_c, E));
     LV_TRY(arena_alloc(A, h.buf_c, E));
@@ -488,6 +502,10 @@ int plan_hubs(lv_ctx *c, int32_t nh)
     LV_TRY(arena_alloc(A, h.pc, P));
     LV_TRY(arena_alloc(A, h.pw, P));
     LV_TRY(arena_alloc(A, h.pn, P));
+    LV_TRY(arena_alloc(A, h.cg, c->hub_chunks));
+    LV_TRY(arena_alloc(A, h.cc, c->hub_chunks));
+    LV_TRY(arena_alloc(A, h.cw, c->hub_chunks));
+    LV_TRY(arena_alloc(A, h.cn, c->hub_chunks));
     return LV_OK;
 }
 
@@ -582,15 +600,23 @@ int one_level(lv_ctx *c, double lower, double *q_out, int32_t *passes_out)
         a.C = c->C;
         a.next = c->next;
         a.last = last ? 1 : 0;
-        hipEvent_t tl = t_begin(c);
-        launch_lab(s, c->n, c->C, c->size, c->deg, c->lab);
-        t_end(c, tl, CAT_APPLY);
+        // first pass: every community is one vertex (no aggregation, no labels)
+        const bool singles = pass == 1 && c->rows_unique;
+        if (!singles) {
+            hipEvent_t tl = t_begin(c);
+            launch_lab(s, c->n, c->C, c->size, c->deg, c->lab);
+            t_end(c, tl, CAT_APPLY);
+        }
         for (int t = 1; t <= kTiers; ++t) {
             const int64_t nv = c->tier_cnt[t];
             if (nv == 0) continue;
             hipEvent_t t0 = t_begin(c);
-            if (t < kTiers)
+            if (t < kTiers && singles)
+                launch_sweep1_tier(s, t, a, c->verts + c->tier_off[t], (int32_t)nv, false);
+            else if (t < kTiers)
                 launch_sweep_tier(s, t, a, c->verts + c->tier_off[t], (int32_t)nv, false);
+            else if (singles)
+                LV_TRY(run_hub_sweep1(s, a, c->hub, c->n_hubs, c->hub_chunks, false));
             else
                 LV_TRY(run_hub_sweep(c->arena, s, a, c->hub, c->n_hubs, c->hub_parts,
                                      c->hub_chunks, false));
@@ -861,6 +887,7 @@ int induce(lv_ctx *c, const int32_t *newc, const int32_t *newid_g, int32_t Kn)
     c->selfsum = self2;
     c->n = Kn;
     c->nnz = nnz2;
+    c->rows_unique = true;  // one entry per neighbour community
     c->m = m2;
     c->level += 1;
     return LV_OK;
@@ -1022,15 +1049,22 @@ int batch_one_level(lv_ctx *c, const std::vector<double> &lower, const std::vect
         LV_TRY(h2d(c, c->d_cmode, (const int32_t *)mode.data(), nc));
         a.C = c->C;
         a.next = c->next;
-        hipEvent_t tl = t_begin(c);
-        launch_lab(s, c->n, c->C, c->size, c->deg, c->lab);
-        t_end(c, tl, CAT_APPLY);
+        const bool singles = pass == 1 && c->rows_unique;  // see one_level
+        if (!singles) {
+            hipEvent_t tl = t_begin(c);
+            launch_lab(s, c->n, c->C, c->size, c->deg, c->lab);
+            t_end(c, tl, CAT_APPLY);
+        }
         for (int t = 1; t <= kTiers; ++t) {
             const int64_t nv = c->tier_cnt[t];
             if (nv == 0) continue;
             hipEvent_t t0 = t_begin(c);
-            if (t < kTiers)
+            if (t < kTiers && singles)
+                launch_sweep1_tier(s, t, a, c->verts + c->tier_off[t], (int32_t)nv, true);
+            else if (t < kTiers)
                 launch_sweep_tier(s, t, a, c->verts + c->tier_off[t], (int32_t)nv, true);
+            else if (singles)
+                LV_TRY(run_hub_sweep1(s, a, c->hub, c->n_hubs, c->hub_chunks, true));
             else
                 LV_TRY(run_hub_sweep(c->arena, s, a, c->hub, c->n_hubs, c->hub_parts,
                                      c->hub_chunks, true));

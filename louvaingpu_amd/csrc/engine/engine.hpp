@@ -159,6 +159,11 @@ struct HubArgs {
     int32_t *pc;
     int64_t *pw;
     int32_t *pn;
+    const int64_t *cbase;     // first chunk of hub j (exclusive scan of chunks)
+    double *cg;               // per-chunk best of the first-pass sweep
+    int32_t *cc;
+    int64_t *cw;
+    int32_t *cn;
 };
 
 // Coarsening: community segments of the member-order entry list.
@@ -227,7 +232,8 @@ int bits_for(uint64_t maxval);  // bits needed to hold values 0..maxval
 // ---------------------------------------------------------------------------
 // louvain_kernels.hip (host launchers)
 struct DevFlags {  // error flags and small scalars written by kernels
-    int32_t err;              // bit 1: invalid input, bit 2: out of exact range
+    int32_t err;              // bit 1: invalid input, bit 2: out of exact range,
+                              // bit 4: CSR rows not strictly increasing
     int32_t scale;            // weight unit exponent s
     int32_t pad0, pad1;
     long long total_units;    // sum of (w == 0 ? 1 : w) in units
@@ -280,6 +286,16 @@ void launch_apply(hipStream_t s, int32_t n, int64_t *deg, int64_t *in, int32_t *
                   ull *din, int32_t *dsize, QAcc *part, QAcc *acc);
 void launch_sweep_tier(hipStream_t s, int tier, const SweepArgs &a, const int32_t *verts,
                        int32_t nv, bool batch);
+// First pass of a level on rows that hold each neighbour once: every community
+// is a single vertex, so no aggregation table is needed (tiers 1..9; hubs via
+// run_hub_sweep1).
+void launch_sweep1_tier(hipStream_t s, int tier, const SweepArgs &a, const int32_t *verts,
+                        int32_t nv, bool batch);
+int run_hub_sweep1(hipStream_t s, const SweepArgs &a, const HubArgs &h, int32_t nh,
+                   int64_t nchunks, bool batch);
+// flags->err |= 4 when some row's columns are not strictly increasing.
+void launch_rows_check(hipStream_t s, const uint32_t *skeys, const int32_t *col, int64_t nnz,
+                       DevFlags *f);
 void launch_hub_setup(hipStream_t s, const int32_t *hv, int32_t nh, const int64_t *rowptr,
                       const int64_t *selfsum, int32_t target, int64_t *hq, int32_t *hP,
                       int64_t *hP64, int64_t *hch);

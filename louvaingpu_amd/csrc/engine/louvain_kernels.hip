@@ -871,6 +871,281 @@ void launch_sweep_tier(hipStream_t s, int tier, const SweepArgs &a, const int32_
 }
 
 // ---------------------------------------------------------------------------
+// First pass of a level.  After init every community is one vertex (C = id,
+// size 1, deg = k), so on rows that list each neighbour once a neighbour j is
+// its own candidate with W = w(v, j): the argmax needs no aggregation table,
+// and the only gather per entry is deg[j].  n_v = row length + has_loop (the
+// own community enters through a self-loop with W = 0).
+
+template <int G, bool B>
+__global__ __launch_bounds__(256) void k_sweep1_group(SweepArgs a, const int32_t *__restrict__ verts,
+                                                     int32_t nv)
+{
+    constexpr int GPB = 256 / G;
+    const int lg = threadIdx.x % G, grp = threadIdx.x / G;
+    for (int64_t base = (int64_t)blockIdx.x * GPB; base < nv; base += (int64_t)gridDim.x * GPB) {
+        const int64_t vi = base + grp;
+        const bool active = vi < nv;
+        int cnt = 0;
+        double bg = 0.0;
+        int32_t bc = INT_MAX;
+        int64_t bw = 0;
+        int32_t v = 0;
+        VParam vp{1.0, 1.0, 0};
+        if (active) {
+            v = verts[vi];
+            vp = vparam<B>(a, v);
+            const int64_t beg = a.rowptr[v];
+            const int64_t end = vp.mode == 0 ? beg : a.rowptr[v + 1];
+            const double f = (double)a.kdeg[v] / vp.m2;
+            for (int64_t p0 = beg + lg; p0 < end; p0 += 4 * G) {
+                int32_t j[4];
+                int64_t w[4], d[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int64_t p = p0 + (int64_t)u * G;
+                    j[u] = (p < end) ? a.col[p] : -1;
+                    w[u] = (p < end) ? a.w[p] : 0;
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) d[u] = (j[u] >= 0) ? a.deg[j[u]] : 0;
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    if (j[u] >= 0) consider(j[u], w[u], d[u], vp.gamma, f, cnt, bg, bc, bw);
+            }
+        }
+        group_reduce<G>(cnt, bg, bc, bw);
+        if (active && lg == 0) {
+            if (vp.mode != 0 && a.selfsum[v] > 0) ++cnt;
+            finalize_vertex(a, v, v, 1, cnt, bc, bw, vp.mode);
+        }
+    }
+}
+
+template <int BLOCK, bool B>
+__global__ __launch_bounds__(BLOCK) void k_sweep1_block(SweepArgs a, const int32_t *__restrict__ verts,
+                                                       int32_t nv)
+{
+    constexpr int NW = BLOCK / 64;
+    __shared__ int r_cnt[NW];
+    __shared__ double r_g[NW];
+    __shared__ int32_t r_c[NW];
+    __shared__ long long r_w[NW];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int64_t vi = blockIdx.x; vi < nv; vi += gridDim.x) {
+        const int32_t v = verts[vi];
+        const VParam vp = vparam<B>(a, v);
+        const int64_t beg = a.rowptr[v];
+        const int64_t end = vp.mode == 0 ? beg : a.rowptr[v + 1];
+        const double f = (double)a.kdeg[v] / vp.m2;
+        int cnt = 0;
+        double bg = 0.0;
+        int32_t bc = INT_MAX;
+        int64_t bw = 0;
+        for (int64_t p0 = beg + tid; p0 < end; p0 += 4 * BLOCK) {
+            int32_t j[4];
+            int64_t w[4], d[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int64_t p = p0 + (int64_t)u * BLOCK;
+                j[u] = (p < end) ? a.col[p] : -1;
+                w[u] = (p < end) ? a.w[p] : 0;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) d[u] = (j[u] >= 0) ? a.deg[j[u]] : 0;
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (j[u] >= 0) consider(j[u], w[u], d[u], vp.gamma, f, cnt, bg, bc, bw);
+        }
+        group_reduce<64>(cnt, bg, bc, bw);
+        if (lane == 0) {
+            r_cnt[wave] = cnt;
+            r_g[wave] = bg;
+            r_c[wave] = bc;
+            r_w[wave] = bw;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            int tc = r_cnt[0];
+            double g0 = r_g[0];
+            int32_t c0 = r_c[0];
+            int64_t w0 = r_w[0];
+            for (int w = 1; w < NW; ++w) {
+                tc += r_cnt[w];
+                if (better(r_g[w], r_c[w], g0, c0)) {
+                    g0 = r_g[w];
+                    c0 = r_c[w];
+                    w0 = r_w[w];
+                }
+            }
+            if (vp.mode != 0 && a.selfsum[v] > 0) ++tc;
+            finalize_vertex(a, v, v, 1, tc, c0, w0, vp.mode);
+        }
+        __syncthreads();
+    }
+}
+
+template <bool B>
+static void launch_sweep1_tier_t(hipStream_t s, int tier, const SweepArgs &a, const int32_t *verts,
+                                 int32_t nv)
+{
+    switch (tier) {
+    case 1: launch_grid(k_sweep1_group<4, B>, (nv + 63) / 64, 256, s, a, verts, nv); break;
+    case 2: launch_grid(k_sweep1_group<8, B>, (nv + 31) / 32, 256, s, a, verts, nv); break;
+    case 3: launch_grid(k_sweep1_group<16, B>, (nv + 15) / 16, 256, s, a, verts, nv); break;
+    case 4: launch_grid(k_sweep1_group<32, B>, (nv + 7) / 8, 256, s, a, verts, nv); break;
+    case 5:
+    case 6:
+    case 7:
+    case 8: launch_grid(k_sweep1_group<64, B>, (nv + 3) / 4, 256, s, a, verts, nv); break;
+    case 9: launch_grid(k_sweep1_block<256, B>, nv, 256, s, a, verts, nv); break;
+    default: break;
+    }
+}
+
+void launch_sweep1_tier(hipStream_t s, int tier, const SweepArgs &a, const int32_t *verts,
+                        int32_t nv, bool batch)
+{
+    if (nv <= 0) return;
+    if (batch)
+        launch_sweep1_tier_t<true>(s, tier, a, verts, nv);
+    else
+        launch_sweep1_tier_t<false>(s, tier, a, verts, nv);
+}
+
+// Hubs: one block per chunk of a hub's entries (entry t == row length is the
+// self-loop's own community, W = 0), then one wavefront per hub over its chunks.
+template <bool B>
+__global__ __launch_bounds__(256) void k_hub1_chunk(SweepArgs a, HubArgs h)
+{
+    __shared__ int r_cnt[4];
+    __shared__ double r_g[4];
+    __shared__ int32_t r_c[4];
+    __shared__ long long r_w[4];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int32_t ci = blockIdx.x;
+    const int32_t j = h.chunk_hub[ci];
+    const int32_t v = h.hv[j];
+    const int64_t q = h.hq[j];
+    const int64_t rb = a.rowptr[v], d = a.rowptr[v + 1] - rb;
+    const VParam vp = vparam<B>(a, v);
+    const double f = (double)a.kdeg[v] / vp.m2;
+    const int64_t cs = chunk_span(h.hP[j]);
+    const int64_t t0 = (int64_t)h.chunk_k[ci] * cs;
+    const int64_t t1 = vp.mode == 0 ? t0 : min(q, t0 + cs);
+    int cnt = 0;
+    double bg = 0.0;
+    int32_t bc = INT_MAX;
+    int64_t bw = 0;
+    for (int64_t p0 = t0 + tid; p0 < t1; p0 += 4 * 256) {
+        int32_t jj[4];
+        int64_t w[4], dg[4];
+        bool own[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int64_t t = p0 + (int64_t)u * 256;
+            own[u] = (t == d);
+            jj[u] = (t < t1 && t < d) ? a.col[rb + t] : -1;
+            w[u] = (jj[u] >= 0) ? a.w[rb + t] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) dg[u] = (jj[u] >= 0) ? a.deg[jj[u]] : 0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (jj[u] >= 0)
+                consider(jj[u], w[u], dg[u], vp.gamma, f, cnt, bg, bc, bw);
+            else if (own[u] && p0 + (int64_t)u * 256 < t1)
+                ++cnt;
+        }
+    }
+    group_reduce<64>(cnt, bg, bc, bw);
+    if (lane == 0) {
+        r_cnt[wave] = cnt;
+        r_g[wave] = bg;
+        r_c[wave] = bc;
+        r_w[wave] = bw;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int tc = r_cnt[0];
+        double g0 = r_g[0];
+        int32_t c0 = r_c[0];
+        int64_t w0 = r_w[0];
+        for (int w = 1; w < 4; ++w) {
+            tc += r_cnt[w];
+            if (better(r_g[w], r_c[w], g0, c0)) {
+                g0 = r_g[w];
+                c0 = r_c[w];
+                w0 = r_w[w];
+            }
+        }
+        h.cn[ci] = tc;
+        h.cg[ci] = g0;
+        h.cc[ci] = c0;
+        h.cw[ci] = w0;
+    }
+}
+
+template <bool B>
+__global__ __launch_bounds__(256) void k_hub1_final(SweepArgs a, HubArgs h, int32_t nh)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t j = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    int cnt = 0;
+    double bg = 0.0;
+    int32_t bc = INT_MAX;
+    int64_t bw = 0;
+    if (j < nh) {
+        for (int64_t k = h.cbase[j] + lane; k < h.cbase[j + 1]; k += 64) {
+            cnt += h.cn[k];
+            if (better(h.cg[k], h.cc[k], bg, bc)) {
+                bg = h.cg[k];
+                bc = h.cc[k];
+                bw = h.cw[k];
+            }
+        }
+    }
+    group_reduce<64>(cnt, bg, bc, bw);
+    if (j < nh && lane == 0) {
+        const int32_t v = h.hv[j];
+        finalize_vertex(a, v, v, 1, cnt, bc, bw, vparam<B>(a, v).mode);
+    }
+}
+
+int run_hub_sweep1(hipStream_t s, const SweepArgs &a, const HubArgs &h, int32_t nh, int64_t nchunks,
+                   bool batch)
+{
+    if (nh <= 0) return LV_OK;
+    if (batch) {
+        hipLaunchKernelGGL(k_hub1_chunk<true>, (int)nchunks, 256, 0, s, a, h);
+        hipLaunchKernelGGL(k_hub1_final<true>, (int)((nh + 3) / 4), 256, 0, s, a, h, nh);
+    } else {
+        hipLaunchKernelGGL(k_hub1_chunk<false>, (int)nchunks, 256, 0, s, a, h);
+        hipLaunchKernelGGL(k_hub1_final<false>, (int)((nh + 3) / 4), 256, 0, s, a, h, nh);
+    }
+    LV_HIP(hipGetLastError());
+    return LV_OK;
+}
+
+__global__ void k_rows_check(const uint32_t *skeys, const int32_t *col, int64_t nnz, DevFlags *f)
+{
+    int bad = 0;
+    GRID_STRIDE(p, nnz)
+    {
+        if (p > 0 && skeys[p] == skeys[p - 1] && col[p] <= col[p - 1]) bad = 4;
+    }
+    bad = wave_max_i(bad);
+    if ((threadIdx.x & 63) == 0 && bad) atomicOr(&f->err, bad);
+}
+
+void launch_rows_check(hipStream_t s, const uint32_t *skeys, const int32_t *col, int64_t nnz,
+                       DevFlags *f)
+{
+    if (nnz > 1)
+        hipLaunchKernelGGL(k_rows_check, grid_for(nnz, 256, 1 << 20), 256, 0, s, skeys, col, nnz, f);
+}
+
+// ---------------------------------------------------------------------------
 // Hub tier (q > 2048): partition + LDS aggregation instead of one global hash
 // table per hub, so the work is streaming reads/writes plus LDS atomics.
 
