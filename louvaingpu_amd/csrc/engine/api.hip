@@ -354,6 +354,63 @@ double current_q(lv_ctx *c, int &rc)
 // ---------------------------------------------------------------------------
 // phases
 
+// Level-0 CSR for edges sorted by source (self[i] = edge i is a self-loop):
+// forward entries stay in edge order, reverse entries are radix-sorted by
+// destination, row v = its reverse entries then its forward entries.
+int build_level0_csr_sorted(lv_ctx *c, int32_t *self, int wbits, const ull *lastloop)
+{
+    Arena &A = c->arena;
+    hipStream_t s = c->stream;
+    const int32_t n = c->n0;
+    const int64_t e = c->e0;
+    int32_t *slscan = nullptr;
+    LV_TRY(arena_alloc(A, slscan, e + 1));
+    LV_TRY(scan_exclusive_i32(A, s, self, slscan, e, slscan + e));
+    int32_t nself = 0;
+    LV_TRY(d2h(c, &nself, (const int32_t *)(slscan + e), 1));
+    LV_TRY(sync(c));
+    const int64_t nr = e - nself;
+    int64_t *fs = nullptr, *rs = nullptr;
+    uint32_t *k1 = nullptr, *k2 = nullptr;
+    uint64_t *p1 = nullptr, *p2 = nullptr;
+    LV_TRY(arena_alloc(A, fs, (int64_t)n + 1));
+    LV_TRY(arena_alloc(A, rs, (int64_t)n + 1));
+    LV_TRY(arena_alloc(A, k1, nr));
+    LV_TRY(arena_alloc(A, k2, nr));
+    LV_TRY(arena_alloc(A, p1, nr));
+    LV_TRY(arena_alloc(A, p2, nr));
+    launch_csr_sorted(s, 0, c->d_src, c->d_dst, c->d_w, slscan, e, n, wbits, fs, k1, p1);
+    uint32_t *sk = nullptr;
+    uint64_t *sp = nullptr;
+    LV_TRY(radix_sort_u32_u64(A, s, k1, p1, k2, p2, nr, bits_for(n > 0 ? (uint64_t)(n - 1) : 0),
+                              &sk, &sp));
+    launch_rowptr_search(s, sk, nr, n, rs);
+    c->nnz = 2 * nr;
+    LV_TRY(arena_alloc(A, c->col, c->nnz));
+    LV_TRY(arena_alloc(A, c->cw, c->nnz));
+    launch_csr_sorted_fill(s, c->d_src, c->d_dst, c->d_w, slscan, e, n, wbits, fs, sk, sp, nr, rs,
+                           c->rowptr, c->col, c->cw, c->kdeg);
+    launch_loop_gather(s, lastloop, c->d_w, n, c->loopw);
+    arena_free(A, k1);
+    arena_free(A, k2);
+    arena_free(A, p1);
+    arena_free(A, p2);
+    uint8_t *rstart = nullptr;
+    LV_TRY(arena_alloc(A, rstart, c->nnz));
+    LV_HIP(hipMemsetAsync(rstart, 0, (size_t)c->nnz, s));
+    LV_HIP(hipMemsetAsync(&c->d_flags->err, 0, sizeof(int), s));
+    launch_rows_check_marked(s, c->rowptr, n, rstart, c->col, c->nnz, c->d_flags);
+    LV_HIP(hipGetLastError());
+    LV_TRY(d2h(c, c->h_flags, c->d_flags, 1));
+    LV_TRY(sync(c));
+    c->rows_unique = (c->h_flags->err & 4) == 0;
+    arena_free(A, rstart);
+    arena_free(A, slscan);
+    arena_free(A, fs);
+    arena_free(A, rs);
+    return LV_OK;
+}
+
 // CalculateNeighbour at level 0: symmetric CSR of the loaded edge list.  Each
 // edge gives two entries keyed by row (self-loops keyed past the last row); a
 // stable radix sort by row keeps input order within a row.  When the columns
@@ -380,6 +437,31 @@ int build_level0_csr(lv_ctx *c)
     LV_TRY(arena_alloc(A, c->selfsum, n));
     LV_TRY(arena_alloc(A, c->loopw, n));
     LV_TRY(arena_alloc(A, lastloop, n));
+    fill_i64(s, c->kdeg, n, 0);
+    fill_i64(s, c->selfsum, n, 0);
+    fill_i64(s, (int64_t *)lastloop, n, 0);
+    launch_self_loops(s, c->d_src, c->d_dst, c->d_w, e, c->kdeg, c->selfsum, lastloop);
+    if (packed && e < INT32_MAX) {
+        // edges sorted by source: only the reverse entries need the row sort
+        int32_t *self = nullptr;
+        LV_TRY(arena_alloc(A, self, e + 1));
+        LV_HIP(hipMemsetAsync(&c->d_flags->err, 0, sizeof(int), s));
+        launch_src_sorted(s, c->d_src, c->d_dst, e, self, c->d_flags);
+        LV_HIP(hipGetLastError());
+        LV_TRY(d2h(c, c->h_flags, c->d_flags, 1));
+        LV_TRY(sync(c));
+        if ((c->h_flags->err & 8) == 0) {
+            const int rc = build_level0_csr_sorted(c, self, wbits, lastloop);
+            arena_free(A, self);
+            arena_free(A, lastloop);
+            if (rc != LV_OK) return rc;
+            t_end(c, t0, CAT_CSR);
+            c->n = n;
+            c->m = c->m0;
+            return LV_OK;
+        }
+        arena_free(A, self);
+    }
     LV_TRY(arena_alloc(A, k1, total));
     LV_TRY(arena_alloc(A, k2, total));
     if (packed) {
@@ -389,10 +471,6 @@ int build_level0_csr(lv_ctx *c)
         LV_TRY(arena_alloc(A, v1, total));
         LV_TRY(arena_alloc(A, v2, total));
     }
-    fill_i64(s, c->kdeg, n, 0);
-    fill_i64(s, c->selfsum, n, 0);
-    fill_i64(s, (int64_t *)lastloop, n, 0);
-    launch_self_loops(s, c->d_src, c->d_dst, c->d_w, e, c->kdeg, c->selfsum, lastloop);
     uint32_t *sk = nullptr, *sv = nullptr;
     uint64_t *sp = nullptr;
     if (packed) {

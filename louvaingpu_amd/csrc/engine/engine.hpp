@@ -293,6 +293,19 @@ void launch_sweep1_tier(hipStream_t s, int tier, const SweepArgs &a, const int32
                         int32_t nv, bool batch);
 int run_hub_sweep1(hipStream_t s, const SweepArgs &a, const HubArgs &h, int32_t nh,
                    int64_t nchunks, bool batch);
+// CSR build for edges sorted by source (see louvain_kernels.hip).
+void launch_src_sorted(hipStream_t s, const int32_t *src, const int32_t *dst, int64_t e,
+                       int32_t *self, DevFlags *f);
+void launch_csr_sorted(hipStream_t s, int stage, const int32_t *src, const int32_t *dst,
+                       const int64_t *w, const int32_t *slscan, int64_t e, int32_t n, int wbits,
+                       int64_t *fs, uint32_t *keys, uint64_t *payload);
+void launch_csr_sorted_fill(hipStream_t s, const int32_t *src, const int32_t *dst,
+                            const int64_t *w, const int32_t *slscan, int64_t e, int32_t n,
+                            int wbits, const int64_t *fs, const uint32_t *skey,
+                            const uint64_t *spay, int64_t nr, const int64_t *rs, int64_t *rowptr,
+                            int32_t *col, int64_t *cw, int64_t *kdeg);
+void launch_rows_check_marked(hipStream_t s, const int64_t *rowptr, int32_t n, uint8_t *rstart,
+                              const int32_t *col, int64_t nnz, DevFlags *f);
 // flags->err |= 4 when some row's columns are not strictly increasing.
 void launch_rows_check(hipStream_t s, const uint32_t *skeys, const int32_t *col, int64_t nnz,
                        DevFlags *f);

@@ -256,6 +256,185 @@ __global__ void k_self_loops(const int32_t *src, const int32_t *dst, const int64
     }
 }
 
+// ---------------------------------------------------------------------------
+// CSR build for an edge list sorted by source (every generator and most files):
+// the entries (row = src) of non-self edges are already grouped by row, so only
+// the reverse entries (row = dst) go through the row sort, and row v is its
+// reverse entries (in edge order) followed by its forward ones.
+
+// bit 8 of err when src is not non-decreasing; self[i] = 1 for a self-loop edge.
+__global__ void k_src_sorted(const int32_t *src, const int32_t *dst, int64_t e, int32_t *self,
+                             DevFlags *f)
+{
+    int bad = 0;
+    GRID_STRIDE(i, e)
+    {
+        const int32_t a = src[i];
+        if (i > 0 && a < src[i - 1]) bad = 8;
+        self[i] = (a == dst[i]) ? 1 : 0;
+    }
+    bad = wave_max_i(bad);
+    if ((threadIdx.x & 63) == 0 && bad) atomicOr(&f->err, bad);
+}
+
+// fs[v] = first edge with src >= v (v in [0, n]).
+__global__ void k_src_starts(const int32_t *src, int64_t e, int32_t n, int64_t *fs)
+{
+    GRID_STRIDE(v, (int64_t)n + 1)
+    {
+        int64_t lo = 0, hi = e;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (src[mid] < (int32_t)v) lo = mid + 1; else hi = mid;
+        }
+        fs[v] = lo;
+    }
+}
+
+// Reverse entries of non-self edges, compacted: key dst, payload src << wbits | w.
+__global__ void k_rev_entries(const int32_t *src, const int32_t *dst, const int64_t *w,
+                              const int32_t *slscan, int64_t e, int wbits, uint32_t *keys,
+                              uint64_t *payload)
+{
+    GRID_STRIDE(i, e)
+    {
+        const int32_t a = src[i], b = dst[i];
+        if (a == b) continue;
+        const int64_t r = i - slscan[i];
+        keys[r] = (uint32_t)b;
+        payload[r] = ((uint64_t)(uint32_t)a << wbits) | (uint64_t)w[i];
+    }
+}
+
+// rowptr[v] = reverse entries of rows < v + forward (non-self) entries of rows < v.
+__global__ void k_rowptr_sum(const int64_t *rs, const int64_t *fs, const int32_t *slscan, int32_t n,
+                             int64_t *rowptr)
+{
+    GRID_STRIDE(v, (int64_t)n + 1) rowptr[v] = rs[v] + (fs[v] - slscan[fs[v]]);
+}
+
+__global__ __launch_bounds__(256) void k_fill_rev(const uint32_t *skey, const uint64_t *spay,
+                                                 int64_t nr, int wbits, const int64_t *rs,
+                                                 const int64_t *rowptr, int32_t *col, int64_t *cw,
+                                                 int64_t *kdeg)
+{
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < nr;
+         base += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = base + threadIdx.x;
+        const bool valid = r < nr;
+        long long row = 0;
+        ull wp = 0;
+        if (valid) {
+            const uint32_t v = skey[r];
+            const uint64_t x = spay[r];
+            const int64_t u = (int64_t)(x & ((1ull << wbits) - 1ull));
+            const int64_t pos = rowptr[v] + (r - rs[v]);
+            col[pos] = (int32_t)(x >> wbits);
+            cw[pos] = u;
+            row = v;
+            wp = (ull)(u == 0 ? 1 : u);
+        }
+        seg_atomic_add(valid, row, wp, (ull *)kdeg);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_fill_fwd(const int32_t *src, const int32_t *dst,
+                                                 const int64_t *w, const int32_t *slscan, int64_t e,
+                                                 const int64_t *fs, const int64_t *rs,
+                                                 const int64_t *rowptr, int32_t *col, int64_t *cw,
+                                                 int64_t *kdeg)
+{
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < e;
+         base += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = base + threadIdx.x;
+        bool valid = i < e;
+        long long row = 0;
+        ull wp = 0;
+        if (valid) {
+            const int32_t v = src[i], b = dst[i];
+            valid = v != b;
+            if (valid) {
+                const int64_t f0 = fs[v];
+                const int64_t rank = (i - f0) - (slscan[i] - slscan[f0]);
+                const int64_t pos = rowptr[v] + (rs[v + 1] - rs[v]) + rank;
+                const int64_t u = w[i];
+                col[pos] = b;
+                cw[pos] = u;
+                row = v;
+                wp = (ull)(u == 0 ? 1 : u);
+            }
+        }
+        seg_atomic_add(valid, row, wp, (ull *)kdeg);
+    }
+}
+
+// Strictly increasing columns within every row (rows are found by their start
+// marks): err bit 4 otherwise.
+__global__ void k_mark_rows(const int64_t *rowptr, int32_t n, uint8_t *rstart)
+{
+    GRID_STRIDE(v, n)
+    {
+        if (rowptr[v + 1] > rowptr[v]) rstart[rowptr[v]] = 1;
+    }
+}
+
+__global__ void k_rows_check_marked(const uint8_t *rstart, const int32_t *col, int64_t nnz,
+                                    DevFlags *f)
+{
+    int bad = 0;
+    GRID_STRIDE(p, nnz)
+    {
+        if (p > 0 && !rstart[p] && col[p] <= col[p - 1]) bad = 4;
+    }
+    bad = wave_max_i(bad);
+    if ((threadIdx.x & 63) == 0 && bad) atomicOr(&f->err, bad);
+}
+
+void launch_src_sorted(hipStream_t s, const int32_t *src, const int32_t *dst, int64_t e,
+                       int32_t *self, DevFlags *f)
+{
+    if (e > 0)
+        hipLaunchKernelGGL(k_src_sorted, grid_for(e, 256, 1 << 20), 256, 0, s, src, dst, e, self, f);
+}
+
+void launch_csr_sorted(hipStream_t s, int stage, const int32_t *src, const int32_t *dst,
+                       const int64_t *w, const int32_t *slscan, int64_t e, int32_t n, int wbits,
+                       int64_t *fs, uint32_t *keys, uint64_t *payload)
+{
+    if (stage == 0) {
+        hipLaunchKernelGGL(k_src_starts, grid_for((int64_t)n + 1, 256, 1 << 20), 256, 0, s, src, e,
+                           n, fs);
+        if (e > 0)
+            hipLaunchKernelGGL(k_rev_entries, grid_for(e, 256, 1 << 20), 256, 0, s, src, dst, w,
+                               slscan, e, wbits, keys, payload);
+    }
+}
+
+void launch_csr_sorted_fill(hipStream_t s, const int32_t *src, const int32_t *dst,
+                            const int64_t *w, const int32_t *slscan, int64_t e, int32_t n,
+                            int wbits, const int64_t *fs, const uint32_t *skey,
+                            const uint64_t *spay, int64_t nr, const int64_t *rs, int64_t *rowptr,
+                            int32_t *col, int64_t *cw, int64_t *kdeg)
+{
+    hipLaunchKernelGGL(k_rowptr_sum, grid_for((int64_t)n + 1, 256, 1 << 20), 256, 0, s, rs, fs,
+                       slscan, n, rowptr);
+    if (nr > 0)
+        hipLaunchKernelGGL(k_fill_rev, grid_for(nr, 256, 1 << 20), 256, 0, s, skey, spay, nr, wbits,
+                           rs, rowptr, col, cw, kdeg);
+    if (e > 0)
+        hipLaunchKernelGGL(k_fill_fwd, grid_for(e, 256, 1 << 20), 256, 0, s, src, dst, w, slscan,
+                           e, fs, rs, rowptr, col, cw, kdeg);
+}
+
+void launch_rows_check_marked(hipStream_t s, const int64_t *rowptr, int32_t n, uint8_t *rstart,
+                              const int32_t *col, int64_t nnz, DevFlags *f)
+{
+    if (nnz <= 1) return;
+    hipLaunchKernelGGL(k_mark_rows, grid_for(n, 256, 1 << 20), 256, 0, s, rowptr, n, rstart);
+    hipLaunchKernelGGL(k_rows_check_marked, grid_for(nnz, 256, 1 << 20), 256, 0, s, rstart, col,
+                       nnz, f);
+}
+
 // rowptr[v] = first position of a key >= v in the sorted keys (v in [0, n]).
 __global__ void k_rowptr_search(const uint32_t *skeys, int64_t total, int32_t n, int64_t *rowptr)
 {
