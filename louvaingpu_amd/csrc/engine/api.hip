@@ -249,8 +249,7 @@ void free_sweep_state(lv_ctx *c)
     a.release((void *)h.chunk_hub);
     a.release((void *)h.chunk_k);
     a.release((void *)h.tmp_c);
-    a.release((void *)h.buf_c);
-    a.release((void *)h.buf_w);
+    a.release((void *)h.buf);
     a.release((void *)h.pcount);
     a.release((void *)h.poff);
     a.release((void *)h.pcur);
@@ -569,8 +568,7 @@ int plan_hubs(lv_ctx *c, int32_t nh)
     arena_free(A, hch);
     const int64_t E = c->hub_entries, P = c->hub_parts;
     LV_TRY(arena_alloc(A, h.tmp_c, E));
-    LV_TRY(arena_alloc(A, h.buf_c, E));
-    LV_TRY(arena_alloc(A, h.buf_w, E));
+    LV_TRY(arena_alloc(A, h.buf, E));
     LV_TRY(arena_alloc(A, h.fbk, 2 * E));
     LV_TRY(arena_alloc(A, h.fbv, 2 * E));
     LV_TRY(arena_alloc(A, h.pcount, P + 1));
@@ -917,8 +915,7 @@ int induce(lv_ctx *c, const int32_t *newc, const int32_t *newid_g, int32_t Kn)
         LV_TRY(arena_alloc(A, B.pcount, nparts + 1));
         LV_TRY(arena_alloc(A, B.poff, nparts + 1));
         LV_TRY(arena_alloc(A, B.pcur, nparts));
-        LV_TRY(arena_alloc(A, B.py, blen));
-        LV_TRY(arena_alloc(A, B.pw, blen));
+        LV_TRY(arena_alloc(A, B.pe, blen));
         LV_TRY(arena_alloc(A, B.buy, blen));
         LV_TRY(arena_alloc(A, B.buw, blen));
         LV_TRY(arena_alloc(A, B.fbk, 2 * blen));
@@ -953,7 +950,7 @@ int induce(lv_ctx *c, const int32_t *newc, const int32_t *newid_g, int32_t Kn)
     // release the transient buffers (later launches on this stream may reuse them)
     void *tmp[] = {mdeg, mstart, estart, selfl, ssum, internal, nuniq, ey, ew, uy, uw, bound, sb, slen,
                    tk1, tk2, sid1, sid2, bP, bP64, bch, pbase, cbase, part_big, chunk_big, chunk_k,
-                   B.pcount, B.poff, B.pcur, B.py, B.pw, B.buy, B.buw, B.fbk, B.fbv, B.ucnt, B.pre,
+                   B.pcount, B.poff, B.pcur, B.pe, B.buy, B.buw, B.fbk, B.fbv, B.ucnt, B.pre,
                    rowcnt};
     for (void *p : tmp) A.release(p);
     free_level_arrays(c);

@@ -521,8 +521,11 @@ __global__ __launch_bounds__(256) void k_cr_big_scatter(CoarsenArgs a, BigSegArg
         if (y == x) continue;
         int part = hub_part(y, P);
         int64_t pos = lbase[part] + atomicAdd(&hist[part], 1);
-        B.py[pos] = y;
-        B.pw[pos] = a.ew[b + t];
+        PEntry en;
+        en.c = y;
+        en.pad = 0;
+        en.w = a.ew[b + t];
+        B.pe[pos] = en;
     }
 }
 
@@ -547,8 +550,10 @@ __global__ __launch_bounds__(256) void k_cr_big_agg(BigSegArgs B)
         s_cnt = 0;
     }
     __syncthreads();
-    for (int64_t e = beg + tid; e < end; e += 256)
-        if (!lds_insert_bounded(s_key, s_val, l2, B.py[e], B.pw[e])) s_over = 1;
+    for (int64_t e = beg + tid; e < end; e += 256) {
+        const PEntry en = B.pe[e];
+        if (!lds_insert_bounded(s_key, s_val, l2, en.c, en.w)) s_over = 1;
+    }
     __syncthreads();
     if (!s_over) {
         for (int s = tid; s < cap; s += 256) {
@@ -569,7 +574,7 @@ __global__ __launch_bounds__(256) void k_cr_big_agg(BigSegArgs B)
         }
         __threadfence();
         __syncthreads();
-        for (int64_t e = beg + tid; e < end; e += 256) glb_insert_mod(gk, gv, cap, B.py[e], B.pw[e]);
+        for (int64_t e = beg + tid; e < end; e += 256) glb_insert_mod(gk, gv, cap, B.pe[e].c, B.pe[e].w);
         __threadfence();
         __syncthreads();
         for (int64_t s = tid; s < cap; s += 256) {

@@ -101,6 +101,14 @@ struct alignas(16) Lab {
     int64_t deg;
 };
 
+// One (community, weight) entry of a hash partition: a single 16-byte store
+// per entry when partitions are scattered.
+struct alignas(16) PEntry {
+    int32_t c;
+    int32_t pad;
+    int64_t w;
+};
+
 struct SweepArgs {
     const Lab *lab;
     const int64_t *rowptr;
@@ -148,8 +156,7 @@ struct HubArgs {
     const int32_t *chunk_hub; // hub of each chunk
     const int32_t *chunk_k;   // chunk number within its hub
     int32_t *tmp_c;           // community of each hub entry (entry order)
-    int32_t *buf_c;           // partitioned communities
-    int64_t *buf_w;           // partitioned weights
+    PEntry *buf;              // partitioned (community, weight) entries
     int64_t *pcount;          // entries per partition
     const int64_t *poff;      // partition offsets (exclusive scan of pcount)
     int64_t *pcur;            // scatter cursors
@@ -191,8 +198,7 @@ struct BigSegArgs {
     int64_t *pcount;
     int64_t *poff;
     int64_t *pcur;
-    int32_t *py;            // partitioned entries
-    int64_t *pw;
+    PEntry *pe;             // partitioned entries
     int32_t *buy;           // uniques per partition, at poff[p]
     int64_t *buw;
     int32_t *fbk;           // fallback tables

@@ -1445,8 +1445,11 @@ __global__ __launch_bounds__(256) void k_hub_scatter(SweepArgs a, HubArgs h)
         int32_t c = tc[t];
         int part = hub_part(c, P);
         int64_t pos = lbase[part] + atomicAdd(&hist[part], 1);
-        h.buf_c[pos] = c;
-        h.buf_w[pos] = (t < d) ? a.w[rb + t] : 0;
+        PEntry en;
+        en.c = c;
+        en.pad = 0;
+        en.w = (t < d) ? a.w[rb + t] : 0;
+        h.buf[pos] = en;
     }
 }
 
@@ -1479,8 +1482,10 @@ __global__ __launch_bounds__(256) void k_hub_agg(SweepArgs a, HubArgs h)
     }
     if (tid == 0) s_over = 0;
     __syncthreads();
-    for (int64_t e = beg + tid; e < end; e += 256)
-        if (!lds_insert_bounded(s_key, s_val, l2, h.buf_c[e], h.buf_w[e])) s_over = 1;
+    for (int64_t e = beg + tid; e < end; e += 256) {
+        const PEntry en = h.buf[e];
+        if (!lds_insert_bounded(s_key, s_val, l2, en.c, en.w)) s_over = 1;
+    }
     __syncthreads();
     int cnt = 0;
     double bg = 0.0;
@@ -1502,7 +1507,7 @@ __global__ __launch_bounds__(256) void k_hub_agg(SweepArgs a, HubArgs h)
         }
         __threadfence();
         __syncthreads();
-        for (int64_t e = beg + tid; e < end; e += 256) glb_insert_mod(gk, gv, cap, h.buf_c[e], h.buf_w[e]);
+        for (int64_t e = beg + tid; e < end; e += 256) glb_insert_mod(gk, gv, cap, h.buf[e].c, h.buf[e].w);
         __threadfence();
         __syncthreads();
         for (int64_t s = tid; s < cap; s += 256) {
