@@ -387,23 +387,18 @@ int build_level0_csr_sorted(lv_ctx *c, int32_t *self, int wbits, const ull *last
     c->nnz = 2 * nr;
     LV_TRY(arena_alloc(A, c->col, c->nnz));
     LV_TRY(arena_alloc(A, c->cw, c->nnz));
-    launch_csr_sorted_fill(s, c->d_src, c->d_dst, c->d_w, slscan, e, n, wbits, fs, sk, sp, nr, rs,
-                           c->rowptr, c->col, c->cw, c->kdeg);
-    launch_loop_gather(s, lastloop, c->d_w, n, c->loopw);
-    arena_free(A, k1);
-    arena_free(A, k2);
-    arena_free(A, p1);
-    arena_free(A, p2);
-    uint8_t *rstart = nullptr;
-    LV_TRY(arena_alloc(A, rstart, c->nnz));
-    LV_HIP(hipMemsetAsync(rstart, 0, (size_t)c->nnz, s));
     LV_HIP(hipMemsetAsync(&c->d_flags->err, 0, sizeof(int), s));
-    launch_rows_check_marked(s, c->rowptr, n, rstart, c->col, c->nnz, c->d_flags);
+    launch_csr_sorted_fill(s, c->d_src, c->d_dst, c->d_w, slscan, e, n, wbits, fs, sk, sp, nr, rs,
+                           c->rowptr, c->col, c->cw, c->kdeg, c->d_flags);
+    launch_loop_gather(s, lastloop, c->d_w, n, c->loopw);
     LV_HIP(hipGetLastError());
     LV_TRY(d2h(c, c->h_flags, c->d_flags, 1));
     LV_TRY(sync(c));
     c->rows_unique = (c->h_flags->err & 4) == 0;
-    arena_free(A, rstart);
+    arena_free(A, k1);
+    arena_free(A, k2);
+    arena_free(A, p1);
+    arena_free(A, p2);
     arena_free(A, slscan);
     arena_free(A, fs);
     arena_free(A, rs);

@@ -62,7 +62,7 @@ __global__ void k_b_graph_exponent(const uint32_t *bw, int wtype, const int64_t 
 // g = c / nres, with endpoints shifted by uvoff[c] and its weight in graph g's
 // units; endpoints outside [0, n_g) flag invalid input, weights beyond the
 // exact range flag a range error.
-__global__ void k_b_union_edges(const int32_t *bsrc, const int32_t *bdst, const uint32_t *bw,
+__global__ __launch_bounds__(256) void k_b_union_edges(const int32_t *bsrc, const int32_t *bdst, const uint32_t *bw,
                                 int wtype, const int *gscale, const int64_t *beoff,
                                 const int32_t *bn, const int64_t *ueoff, const int64_t *uvoff,
                                 int32_t ncomp, int32_t nres, int64_t etot, int32_t *usrc,
@@ -86,20 +86,9 @@ __global__ void k_b_union_edges(const int32_t *bsrc, const int32_t *bdst, const 
         const int64_t u = weight_units(bw[i], wtype, gscale[g], bad);
         uw[j] = u;
         umax = max(umax, (ull)u);
-        tot += (ull)(u == 0 ? 1 : u);
-        if (tot >= (ull)kUnitLimit) bad = 2;
+        tot = sat_add(tot, (ull)(u == 0 ? 1 : u));
     }
-    tot = (ull)wave_sum((long long)tot);
-    bad = wave_max_i(bad);
-    for (int off = 32; off > 0; off >>= 1) umax = max(umax, __shfl_xor(umax, off, 64));
-    if ((threadIdx.x & 63) == 0) {
-        if (tot) {
-            ull old = atomicAdd((ull *)&f->total_units, tot);
-            if (old + tot >= (ull)kUnitLimit || tot >= (ull)kUnitLimit) bad |= 2;
-        }
-        if (umax) atomicMax(&f->max_units, umax);
-        if (bad) atomicOr(&f->err, bad);
-    }
+    block_units_flush(tot, 0, umax, bad, f);
 }
 
 // cm[c] += sum of w over the union edges of component c (edge ranges ueoff).
@@ -273,7 +262,7 @@ void launch_b_union_edges(hipStream_t s, const int32_t *bsrc, const int32_t *bds
                           int64_t *uw, DevFlags *f)
 {
     if (etot > 0)
-        hipLaunchKernelGGL(k_b_union_edges, grid_for(etot, 256, 1 << 20), 256, 0, s, bsrc, bdst, bw,
+        hipLaunchKernelGGL(k_b_union_edges, grid_for(etot, 256, 8192), 256, 0, s, bsrc, bdst, bw,
                            wtype, gscale, beoff, bn, ueoff, uvoff, ncomp, nres, etot, usrc, udst,
                            uw, f);
 }

@@ -677,10 +677,12 @@ __global__ void k_cr_copy_big(CoarsenArgs a, BigSegArgs B, const int64_t *rowptr
     }
 }
 
-__global__ void k_cr_vertex(const int32_t *newid_g, const ull *ssum, const ull *internal,
-                            const ull *selfl, int32_t K, int64_t *loop2, int64_t *self2,
-                            int64_t *kdeg2, long long *msum)
+__global__ __launch_bounds__(256) void k_cr_vertex(const int32_t *newid_g, const ull *ssum,
+                                                  const ull *internal, const ull *selfl, int32_t K,
+                                                  int64_t *loop2, int64_t *self2, int64_t *kdeg2,
+                                                  long long *msum)
 {
+    __shared__ long long s_acc[4];
     long long acc = 0;
     GRID_STRIDE(g, K)
     {
@@ -692,8 +694,13 @@ __global__ void k_cr_vertex(const int32_t *newid_g, const ull *ssum, const ull *
         kdeg2[x] = (int64_t)(ssum[g] - internal[g]) + 4 * lp;
         acc += (long long)(ssum[g] + 2 * selfl[g]);
     }
-    acc = wave_sum(acc);
-    if ((threadIdx.x & 63) == 0 && acc) atomicAdd((ull *)msum, (ull)acc);
+    acc = wave_sum(acc);  // the level's total is below 2^61: no wrap
+    if ((threadIdx.x & 63) == 0) s_acc[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const long long t = s_acc[0] + s_acc[1] + s_acc[2] + s_acc[3];
+        if (t) atomicAdd((ull *)msum, (ull)t);  // one atomic per block on the shared total
+    }
 }
 
 void launch_cr_rowcnt(hipStream_t s, const int32_t *newid_g, const int32_t *nuniq, int32_t K,

@@ -70,6 +70,58 @@ __device__ __forceinline__ int wave_max_i(int v)
     return v;
 }
 
+// Sums of weight units saturate at kUnitLimit (both operands are at most
+// kUnitLimit, so a + b never wraps): reaching the limit signals the range error.
+__device__ __forceinline__ ull sat_add(ull a, ull b)
+{
+    const ull s = a + b;
+    return s >= (ull)kUnitLimit ? (ull)kUnitLimit : s;
+}
+
+__device__ __forceinline__ ull wave_sum_sat(ull v)
+{
+    for (int off = 32; off > 0; off >>= 1) v = sat_add(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// End of a 256-thread block of a unit-summing kernel: the block's totals
+// reach the device flags with one atomic each (one address shared by the
+// whole grid, so per-wave atomics would serialise).
+__device__ __forceinline__ void block_units_flush(ull tot, ull msum, ull umax, int bad, DevFlags *f)
+{
+    __shared__ ull s_t[4], s_m[4], s_x[4];
+    __shared__ int s_b[4];
+    tot = wave_sum_sat(tot);
+    msum = wave_sum_sat(msum);
+    for (int off = 32; off > 0; off >>= 1) umax = max(umax, __shfl_xor(umax, off, 64));
+    bad = wave_max_i(bad);
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        s_t[wave] = tot;
+        s_m[wave] = msum;
+        s_x[wave] = umax;
+        s_b[wave] = bad;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        ull T = 0, M = 0, X = 0;
+        int B = 0;
+        for (int w = 0; w < 4; ++w) {
+            T = sat_add(T, s_t[w]);
+            M = sat_add(M, s_m[w]);
+            X = max(X, s_x[w]);
+            B |= s_b[w];
+        }
+        if (T) {
+            const ull old = atomicAdd((ull *)&f->total_units, T);
+            if (T >= (ull)kUnitLimit || old + T >= (ull)kUnitLimit) B |= 2;
+        }
+        if (M) atomicAdd((ull *)&f->m_units, M);
+        if (X) atomicMax(&f->max_units, X);
+        if (B) atomicOr(&f->err, B);
+    }
+}
+
 // 128-bit accumulate of deg^2 as (lo, hi).
 __device__ __forceinline__ void acc_sq(ull &lo, ull &hi, ull d)
 {

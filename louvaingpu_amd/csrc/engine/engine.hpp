@@ -309,9 +309,7 @@ void launch_csr_sorted_fill(hipStream_t s, const int32_t *src, const int32_t *ds
                             const int64_t *w, const int32_t *slscan, int64_t e, int32_t n,
                             int wbits, const int64_t *fs, const uint32_t *skey,
                             const uint64_t *spay, int64_t nr, const int64_t *rs, int64_t *rowptr,
-                            int32_t *col, int64_t *cw, int64_t *kdeg);
-void launch_rows_check_marked(hipStream_t s, const int64_t *rowptr, int32_t n, uint8_t *rstart,
-                              const int32_t *col, int64_t nnz, DevFlags *f);
+                            int32_t *col, int64_t *cw, int64_t *kdeg, DevFlags *f);
 // flags->err |= 4 when some row's columns are not strictly increasing.
 void launch_rows_check(hipStream_t s, const uint32_t *skeys, const int32_t *col, int64_t nnz,
                        DevFlags *f);

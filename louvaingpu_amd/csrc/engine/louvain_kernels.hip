@@ -37,8 +37,8 @@ __global__ void k_weight_exponent(const void *w, int wtype, int64_t e, DevFlags 
     }
 }
 
-__global__ void k_weight_units(const void *w, int wtype, int64_t e, int scale, int64_t *wu,
-                               DevFlags *f)
+__global__ __launch_bounds__(256) void k_weight_units(const void *w, int wtype, int64_t e, int scale,
+                                                     int64_t *wu, DevFlags *f)
 {
     ull tot = 0, msum = 0, umax = 0;
     int bad = 0;
@@ -47,23 +47,10 @@ __global__ void k_weight_units(const void *w, int wtype, int64_t e, int scale, i
         const int64_t u = weight_units(((const uint32_t *)w)[i], wtype, scale, bad);
         wu[i] = u;
         umax = max(umax, (ull)u);
-        tot += (ull)(u == 0 ? 1 : u);
-        msum += (ull)u;
-        if (tot >= (ull)kUnitLimit) bad = 2;
+        tot = sat_add(tot, (ull)(u == 0 ? 1 : u));
+        msum = sat_add(msum, (ull)u);
     }
-    tot = (ull)wave_sum((long long)tot);  // each lane < 2^60: a wave sum cannot wrap
-    msum = (ull)wave_sum((long long)msum);
-    bad = wave_max_i(bad);
-    if ((threadIdx.x & 63) == 0) {
-        if (tot) {
-            ull old = atomicAdd((ull *)&f->total_units, tot);
-            if (old + tot >= (ull)kUnitLimit || tot >= (ull)kUnitLimit) bad = 2;
-        }
-        if (msum) atomicAdd((ull *)&f->m_units, msum);
-        if (bad) atomicOr(&f->err, bad);
-    }
-    for (int off = 32; off > 0; off >>= 1) umax = max(umax, __shfl_xor(umax, off, 64));
-    if ((threadIdx.x & 63) == 0 && umax) atomicMax(&f->max_units, umax);
+    block_units_flush(tot, msum, umax, bad, f);
 }
 
 __global__ void k_check_endpoints(const int32_t *src, const int32_t *dst, int64_t e, int32_t n,
@@ -313,11 +300,15 @@ __global__ void k_rowptr_sum(const int64_t *rs, const int64_t *fs, const int32_t
     GRID_STRIDE(v, (int64_t)n + 1) rowptr[v] = rs[v] + (fs[v] - slscan[fs[v]]);
 }
 
+// The fills also check that every row's columns strictly increase (err bit
+// 4): within the reverse part, within the forward part (skipping self-loop
+// edges), and from the last reverse column to the first forward one.
 __global__ __launch_bounds__(256) void k_fill_rev(const uint32_t *skey, const uint64_t *spay,
                                                  int64_t nr, int wbits, const int64_t *rs,
                                                  const int64_t *rowptr, int32_t *col, int64_t *cw,
-                                                 int64_t *kdeg)
+                                                 int64_t *kdeg, DevFlags *f)
 {
+    int bad = 0;
     for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < nr;
          base += (int64_t)gridDim.x * blockDim.x) {
         const int64_t r = base + threadIdx.x;
@@ -329,21 +320,27 @@ __global__ __launch_bounds__(256) void k_fill_rev(const uint32_t *skey, const ui
             const uint64_t x = spay[r];
             const int64_t u = (int64_t)(x & ((1ull << wbits) - 1ull));
             const int64_t pos = rowptr[v] + (r - rs[v]);
-            col[pos] = (int32_t)(x >> wbits);
+            const int32_t cc = (int32_t)(x >> wbits);
+            col[pos] = cc;
             cw[pos] = u;
             row = v;
             wp = (ull)(u == 0 ? 1 : u);
+            if (r > 0 && skey[r - 1] == v && (int32_t)(spay[r - 1] >> wbits) >= cc) bad = 4;
         }
         seg_atomic_add(valid, row, wp, (ull *)kdeg);
     }
+    bad = wave_max_i(bad);
+    if ((threadIdx.x & 63) == 0 && bad) atomicOr(&f->err, bad);
 }
 
 __global__ __launch_bounds__(256) void k_fill_fwd(const int32_t *src, const int32_t *dst,
                                                  const int64_t *w, const int32_t *slscan, int64_t e,
                                                  const int64_t *fs, const int64_t *rs,
+                                                 const uint64_t *spay, int wbits,
                                                  const int64_t *rowptr, int32_t *col, int64_t *cw,
-                                                 int64_t *kdeg)
+                                                 int64_t *kdeg, DevFlags *f)
 {
+    int bad = 0;
     for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < e;
          base += (int64_t)gridDim.x * blockDim.x) {
         const int64_t i = base + threadIdx.x;
@@ -356,35 +353,23 @@ __global__ __launch_bounds__(256) void k_fill_fwd(const int32_t *src, const int3
             if (valid) {
                 const int64_t f0 = fs[v];
                 const int64_t rank = (i - f0) - (slscan[i] - slscan[f0]);
-                const int64_t pos = rowptr[v] + (rs[v + 1] - rs[v]) + rank;
+                const int64_t nrev = rs[v + 1] - rs[v];
+                const int64_t pos = rowptr[v] + nrev + rank;
                 const int64_t u = w[i];
                 col[pos] = b;
                 cw[pos] = u;
                 row = v;
                 wp = (ull)(u == 0 ? 1 : u);
+                if (rank > 0) {
+                    int64_t k = i - 1;  // the previous non-self edge of the row
+                    while (src[k] == dst[k]) --k;
+                    if (dst[k] >= b) bad = 4;
+                } else if (nrev > 0 && (int32_t)(spay[rs[v + 1] - 1] >> wbits) >= b) {
+                    bad = 4;
+                }
             }
         }
         seg_atomic_add(valid, row, wp, (ull *)kdeg);
-    }
-}
-
-// Strictly increasing columns within every row (rows are found by their start
-// marks): err bit 4 otherwise.
-__global__ void k_mark_rows(const int64_t *rowptr, int32_t n, uint8_t *rstart)
-{
-    GRID_STRIDE(v, n)
-    {
-        if (rowptr[v + 1] > rowptr[v]) rstart[rowptr[v]] = 1;
-    }
-}
-
-__global__ void k_rows_check_marked(const uint8_t *rstart, const int32_t *col, int64_t nnz,
-                                    DevFlags *f)
-{
-    int bad = 0;
-    GRID_STRIDE(p, nnz)
-    {
-        if (p > 0 && !rstart[p] && col[p] <= col[p - 1]) bad = 4;
     }
     bad = wave_max_i(bad);
     if ((threadIdx.x & 63) == 0 && bad) atomicOr(&f->err, bad);
@@ -414,25 +399,16 @@ void launch_csr_sorted_fill(hipStream_t s, const int32_t *src, const int32_t *ds
                             const int64_t *w, const int32_t *slscan, int64_t e, int32_t n,
                             int wbits, const int64_t *fs, const uint32_t *skey,
                             const uint64_t *spay, int64_t nr, const int64_t *rs, int64_t *rowptr,
-                            int32_t *col, int64_t *cw, int64_t *kdeg)
+                            int32_t *col, int64_t *cw, int64_t *kdeg, DevFlags *f)
 {
     hipLaunchKernelGGL(k_rowptr_sum, grid_for((int64_t)n + 1, 256, 1 << 20), 256, 0, s, rs, fs,
                        slscan, n, rowptr);
     if (nr > 0)
         hipLaunchKernelGGL(k_fill_rev, grid_for(nr, 256, 1 << 20), 256, 0, s, skey, spay, nr, wbits,
-                           rs, rowptr, col, cw, kdeg);
+                           rs, rowptr, col, cw, kdeg, f);
     if (e > 0)
         hipLaunchKernelGGL(k_fill_fwd, grid_for(e, 256, 1 << 20), 256, 0, s, src, dst, w, slscan,
-                           e, fs, rs, rowptr, col, cw, kdeg);
-}
-
-void launch_rows_check_marked(hipStream_t s, const int64_t *rowptr, int32_t n, uint8_t *rstart,
-                              const int32_t *col, int64_t nnz, DevFlags *f)
-{
-    if (nnz <= 1) return;
-    hipLaunchKernelGGL(k_mark_rows, grid_for(n, 256, 1 << 20), 256, 0, s, rowptr, n, rstart);
-    hipLaunchKernelGGL(k_rows_check_marked, grid_for(nnz, 256, 1 << 20), 256, 0, s, rstart, col,
-                       nnz, f);
+                           e, fs, rs, spay, wbits, rowptr, col, cw, kdeg, f);
 }
 
 // rowptr[v] = first position of a key >= v in the sorted keys (v in [0, n]).
