@@ -44,15 +44,27 @@ import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 
-# Kernels of each sweep tier, as named in rocprofv3 summaries.
+# Kernels of each sweep tier as named in rocprofv3 summaries: the first pass
+# of a level (no aggregation) and the later passes.  The last kernel of each
+# list runs once per launch of the tier.  Tiers 5-8 share their first-pass
+# kernel, so their traffic is not separable.
 TIER_KERNELS = {
-    1: ["k_sweep_group<4, 3, 3,"], 2: ["k_sweep_group<8, 4, 4,"],
-    3: ["k_sweep_group<16, 5, 5,"], 4: ["k_sweep_group<32, 6, 6,"],
-    5: ["k_sweep_group<64, 7, 7,"], 6: ["k_sweep_group<64, 8, 8,"],
-    7: ["k_sweep_group<64, 9, 9,"], 8: ["k_sweep_group<64, 10, 10,"],
-    9: ["k_sweep_block<256, 12, 11>"],
-    10: ["k_hub_count", "k_hub_scatter", "k_hub_agg", "k_hub_final"],
+    1: (["k_sweep1_group<4,", "k_sweep_group<4, 3, 3,"], None),
+    2: (["k_sweep1_group<8,", "k_sweep_group<8, 4, 4,"], None),
+    3: (["k_sweep1_group<16,", "k_sweep_group<16, 5, 5,"], None),
+    4: (["k_sweep1_group<32,", "k_sweep_group<32, 6, 6,"], None),
+    5: None, 6: None, 7: None, 8: None,
+    9: (["k_sweep1_block<256,", "k_sweep_block<256, 12, 11,"], None),
+    10: (["k_hub1_chunk", "k_hub1_final", "k_hub_count", "k_hub_scatter", "k_hub_agg",
+          "k_hub_final"], ["k_hub1_final", "k_hub_final"]),
 }
+TIER_NAMES = {1: "k_sweep1_group<4> / k_sweep_group<4,3,3>", 2: "k_sweep1_group<8> / k_sweep_group<8,4,4>",
+              3: "k_sweep1_group<16> / k_sweep_group<16,5,5>", 4: "k_sweep1_group<32> / k_sweep_group<32,6,6>",
+              5: "k_sweep1_group<64> / k_sweep_group<64,7,7>", 6: "k_sweep1_group<64> / k_sweep_group<64,8,8>",
+              7: "k_sweep1_group<64> / k_sweep_group<64,9,9>", 8: "k_sweep1_group<64> / k_sweep_group<64,10,10>",
+              9: "k_sweep1_block<256> / k_sweep_block<256,12,11>",
+              10: "first pass: k_hub1_chunk + k_hub1_final; later passes: k_hub_count + k_hub_scatter "
+                  "+ k_hub_agg + k_hub_final"}
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "louvain_%s_hbm_traffic.txt")
 
 
@@ -82,28 +94,35 @@ def copy_probe(nbytes):
 
 
 def measured_traffic(workload_key, tier):
-    """HBM bytes per launch of a sweep tier (sum over its kernels) from the committed
-    rocprofv3 FETCH_SIZE/WRITE_SIZE summary of this workload, or None."""
+    """HBM bytes per launch of a sweep tier from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary of this workload: the tier's kernels' bytes
+    (per-dispatch average x dispatches) over the tier's launches, or None."""
     path = TRAFFIC_PROFILE % workload_key
-    if not os.path.exists(path):
+    spec = TIER_KERNELS.get(tier)
+    if spec is None or not os.path.exists(path):
         return None
     per_kernel = {}
     with open(path) as f:
         for line in f:
-            if line.startswith("#") or not line.strip():
-                continue
             parts = line.split(None, 3)
-            if len(parts) < 4 or not parts[0].isdigit():
+            if line.startswith("#") or len(parts) < 4 or not parts[0].isdigit():
                 continue
-            per_kernel[parts[3].strip()] = (float(parts[1]) + float(parts[2])) * 1e6
-    total, found = 0.0, 0
-    for pat in TIER_KERNELS[tier]:
-        for name, b in per_kernel.items():
-            if pat in name:
-                total += b
-                found += 1
-                break
-    return total if found == len(TIER_KERNELS[tier]) else None
+            per_kernel[parts[3].strip()] = (int(parts[0]), (float(parts[1]) + float(parts[2])) * 1e6)
+    pats, markers = spec
+
+    def lookup(pat):
+        hits = [v for k, v in per_kernel.items() if pat in k]
+        return hits[0] if len(hits) == 1 else None
+
+    total, launches = 0.0, 0
+    for pat in pats:
+        hit = lookup(pat)
+        if hit is None:
+            return None
+        total += hit[0] * hit[1]
+    for pat in (markers or pats):
+        launches += lookup(pat)[0]
+    return total / launches if launches else None
 
 
 def gen_batch_graphs(count, seed, threads):
@@ -425,8 +444,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "sweep tier %d (%s)" % (dom + 1, " + ".join(
-                k[:-1] + ", ..>" if k.endswith(",") else k for k in TIER_KERNELS[dom + 1])),
+            "kernel": "sweep tier %d (%s)" % (dom + 1, TIER_NAMES[dom + 1]),
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

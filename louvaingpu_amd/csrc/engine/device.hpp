@@ -147,7 +147,9 @@ __device__ __forceinline__ void atomic_add_u128(ull *lo_p, ull *hi_p, ull lo, ul
 }
 
 // Adds val into base[key] for keys sorted within the wave: lanes holding the
-// same key (contiguous runs) are combined first, one atomic per run.
+// same key (contiguous runs) are combined first, one atomic per run.  Invalid
+// lanes may only follow the valid ones (a lane that must not add in the middle
+// of a run keeps the run's key with val 0).
 __device__ __forceinline__ void seg_atomic_add(bool valid, long long key, ull val, ull *base)
 {
     const int lane = threadIdx.x & 63;

@@ -344,13 +344,14 @@ __global__ __launch_bounds__(256) void k_fill_fwd(const int32_t *src, const int3
     for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < e;
          base += (int64_t)gridDim.x * blockDim.x) {
         const int64_t i = base + threadIdx.x;
-        bool valid = i < e;
+        const bool valid = i < e;
         long long row = 0;
         ull wp = 0;
         if (valid) {
             const int32_t v = src[i], b = dst[i];
-            valid = v != b;
-            if (valid) {
+            row = v;  // a self-loop edge keeps its row's key with weight 0, so the
+                      // runs of equal keys stay contiguous for the segmented sum
+            if (v != b) {
                 const int64_t f0 = fs[v];
                 const int64_t rank = (i - f0) - (slscan[i] - slscan[f0]);
                 const int64_t nrev = rs[v + 1] - rs[v];
@@ -358,7 +359,6 @@ __global__ __launch_bounds__(256) void k_fill_fwd(const int32_t *src, const int3
                 const int64_t u = w[i];
                 col[pos] = b;
                 cw[pos] = u;
-                row = v;
                 wp = (ull)(u == 0 ? 1 : u);
                 if (rank > 0) {
                     int64_t k = i - 1;  // the previous non-self edge of the row

@@ -287,3 +287,39 @@ def test_hub_global_fallback(monkeypatch):
     eng, r = run_both(n, src, dst, w)
     with eng:
         assert_run_matches(eng, r, n)
+
+
+@pytest.mark.parametrize("variant", ["shuffled", "reversed_pairs", "dups_loops_sorted"])
+def test_csr_paths_agree(variant):
+    """The CSR of an edge list sorted by source (forward entries kept in place,
+    only reverse entries row-sorted) and of any other order (full row sort)
+    give the oracle's results; rows with repeated neighbours take the
+    aggregating first pass."""
+    n, src, dst, w = generators.rmat(13, 16, 0.57, 0.19, 0.19, 21, "int")
+    rng = np.random.Generator(np.random.PCG64(5))
+    if variant == "shuffled":
+        p = rng.permutation(len(src))
+        src, dst, w = src[p], dst[p], w[p]
+    elif variant == "reversed_pairs":
+        # sorted by source but not canonical: some edges stored as (max, min)
+        flip = rng.uniform(size=len(src)) < 0.5
+        s2 = np.where(flip, dst, src)
+        d2 = np.where(flip, src, dst)
+        o = np.lexsort((d2, s2))
+        src, dst, w = s2[o], d2[o], w[o]
+    else:
+        # duplicate edges and extra self-loops, still sorted by source
+        k = len(src) // 10
+        pick = rng.integers(0, len(src), k)
+        loops = rng.integers(0, n, k // 4).astype(np.int32)
+        src = np.concatenate([src, src[pick], loops])
+        dst = np.concatenate([dst, dst[pick], loops])
+        w = np.concatenate([w, w[pick], np.ones(len(loops), np.int32)])
+        o = np.argsort(src, kind="stable")
+        src, dst, w = src[o], dst[o], w[o]
+    gpu_case("csr paths %s" % variant)
+    eng, r = run_both(n, src.astype(np.int32), dst.astype(np.int32), w.astype(np.int32))
+    try:
+        assert_run_matches(eng, r, n)
+    finally:
+        eng.close()
