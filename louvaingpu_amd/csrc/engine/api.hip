@@ -74,7 +74,8 @@ struct lv_ctx {
     Lab *lab = nullptr;
     int64_t *deg = nullptr, *in = nullptr;
     int32_t *size = nullptr, *dsize = nullptr;
-    ull *ddeg = nullptr, *din = nullptr;
+    ull *ddeg = nullptr;
+    int size_shift = 0;  // see SweepArgs
     int32_t *verts = nullptr;
     int64_t tier_cnt[kTiers + 1] = {0}, tier_nnz[kTiers + 1] = {0}, tier_off[kTiers + 1] = {0};
     // every row of the level lists each neighbour once (coarsened levels always;
@@ -238,7 +239,6 @@ void free_sweep_state(lv_ctx *c)
     arena_free(a, c->size);
     arena_free(a, c->dsize);
     arena_free(a, c->ddeg);
-    arena_free(a, c->din);
     arena_free(a, c->verts);
     HubArgs &h = c->hub;
     a.release((void *)h.hq);
@@ -591,7 +591,6 @@ int init_level(lv_ctx *c)
     LV_TRY(arena_alloc(A, c->size, n));
     LV_TRY(arena_alloc(A, c->dsize, n));
     LV_TRY(arena_alloc(A, c->ddeg, n));
-    LV_TRY(arena_alloc(A, c->din, n));
     LV_TRY(arena_alloc(A, c->C, n));
     LV_TRY(arena_alloc(A, c->next, n));
     LV_TRY(arena_alloc(A, c->lab, n));
@@ -601,15 +600,22 @@ int init_level(lv_ctx *c)
     LV_TRY(arena_alloc(A, tkey2, n));
     LV_TRY(arena_alloc(A, v1, n));
     LV_TRY(arena_alloc(A, v2, n));
-    LV_TRY(zero_small(c, 2 * (kTiers + 1)));
+    LV_TRY(zero_small(c, 2 * (kTiers + 1) + 1));
     launch_init_vertex(s, n, c->rowptr, c->kdeg, c->loopw, c->selfsum, c->deg, c->in, c->size,
-                       c->ddeg, c->din, c->dsize, c->C, c->next, tkey, v1, c->d_small,
-                       c->d_small + (kTiers + 1));
+                       c->ddeg, c->dsize, c->C, c->next, tkey, v1, c->d_small,
+                       c->d_small + (kTiers + 1), (ull *)(c->d_small + 2 * (kTiers + 1)));
     uint32_t *ko = nullptr;
     int32_t *vo = nullptr;
     LV_TRY(radix_sort_u32_i32(A, s, tkey, v1, tkey2, v2, n, 4, &ko, &vo));
-    LV_TRY(d2h(c, c->h_small, c->d_small, 2 * (kTiers + 1)));
+    LV_TRY(d2h(c, c->h_small, c->d_small, 2 * (kTiers + 1) + 1));
     LV_TRY(sync(c));
+    {
+        // size deltas share the degree word when |delta_size| <= n < 2^(S-1) and
+        // |delta_deg| <= total degree leave the packed sum below 2^63
+        const int S = bits_for((uint64_t)n) + 1;
+        const int kb = bits_for((uint64_t)c->h_small[2 * (kTiers + 1)]);
+        c->size_shift = (kb + S <= 62) ? S : 0;
+    }
     // keep the sorted vertex list, release the other buffers
     if (vo == v1) {
         c->verts = v1;
@@ -655,8 +661,8 @@ int one_level(lv_ctx *c, double lower, double *q_out, int32_t *passes_out)
     a.deg = c->deg;
     a.size = c->size;
     a.ddeg = c->ddeg;
-    a.din = c->din;
     a.dsize = c->dsize;
+    a.size_shift = c->size_shift;
     a.m2 = 2.0 * (double)c->m;
     a.gamma = c->cfg.gamma;
     double curQ = -1.0;  // LouvainCUDA/OneLevelKernel.cu:528
@@ -702,8 +708,8 @@ int one_level(lv_ctx *c, double lower, double *q_out, int32_t *passes_out)
             c->cur_trace.push_back(std::move(nx));
         }
         hipEvent_t t1 = t_begin(c);
-        launch_apply(s, c->n, c->deg, c->in, c->size, c->ddeg, c->din, c->dsize, c->d_qpart,
-                     c->d_qacc);
+        launch_apply(s, c->n, c->deg, c->in, c->size, c->ddeg, c->dsize, c->size_shift,
+                     c->d_qpart, c->d_qacc);
         t_end(c, t1, CAT_APPLY);
         LV_HIP(hipGetLastError());
         LV_TRY(d2h(c, c->h_qacc, c->d_qacc, 1));
@@ -1051,8 +1057,8 @@ int batch_level_setup(lv_ctx *c, int32_t level)
 int batch_qagg(lv_ctx *c, bool apply)
 {
     fill_i64(c->stream, (int64_t *)c->d_cq, 4 * (int64_t)c->ncomp, 0);
-    launch_b_qagg(c->stream, c->n, c->deg, c->in, c->size, c->ddeg, c->din, c->dsize, c->vcomp,
-                  apply, c->d_cq);
+    launch_b_qagg(c->stream, c->n, c->deg, c->in, c->size, c->ddeg, c->dsize, c->size_shift,
+                  c->vcomp, apply, c->d_cq);
     LV_HIP(hipGetLastError());
     LV_TRY(d2h(c, c->h_cq, c->d_cq, c->ncomp));
     return sync(c);
@@ -1094,8 +1100,8 @@ int batch_one_level(lv_ctx *c, const std::vector<double> &lower, const std::vect
     a.deg = c->deg;
     a.size = c->size;
     a.ddeg = c->ddeg;
-    a.din = c->din;
     a.dsize = c->dsize;
+    a.size_shift = c->size_shift;
     a.vcomp = c->vcomp;
     a.cmode = c->d_cmode;
     a.cm2 = c->d_cm2;

@@ -146,9 +146,10 @@ __device__ __forceinline__ void seg_qacc_add(bool valid, int32_t key, long long 
 
 // Per component Q aggregates of the community state (sum of internals, sum of
 // deg^2); with `apply` the pass deltas are added first (k_apply per component).
-__global__ __launch_bounds__(256) void k_b_qagg(int32_t n, int64_t *deg, int64_t *in, int32_t *size,
-                                               ull *ddeg, ull *din, int32_t *dsize,
-                                               const int32_t *vcomp, int apply, QAcc *cq)
+__global__ __launch_bounds__(256) void k_b_qagg(int32_t n, int64_t *deg, const int64_t *in,
+                                               int32_t *size, ull *ddeg, int32_t *dsize,
+                                               int size_shift, const int32_t *vcomp, int apply,
+                                               QAcc *cq)
 {
     for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n;
          base += (int64_t)gridDim.x * blockDim.x) {
@@ -158,23 +159,9 @@ __global__ __launch_bounds__(256) void k_b_qagg(int32_t n, int64_t *deg, int64_t
         long long sin = 0;
         ull lo = 0, hi = 0;
         if (valid) {
-            int64_t dv = deg[c], iv = in[c];
-            if (apply) {
-                const ull dd = ddeg[c], di = din[c];
-                const int32_t ds = dsize[c];
-                if (dd | di | (ull)(uint32_t)ds) {
-                    dv += (int64_t)dd;
-                    iv += (int64_t)di;
-                    deg[c] = dv;
-                    in[c] = iv;
-                    size[c] += ds;
-                    ddeg[c] = 0;
-                    din[c] = 0;
-                    dsize[c] = 0;
-                }
-            }
+            const int64_t dv = apply ? apply_deltas(c, deg, size, ddeg, dsize, size_shift) : deg[c];
             key = vcomp[c];
-            sin = iv;
+            sin = in[c];
             acc_sq(lo, hi, (ull)dv);
         }
         seg_qacc_add(valid, key, sin, lo, hi, cq);
@@ -275,12 +262,13 @@ void launch_b_edge_sum(hipStream_t s, const int64_t *w, const int64_t *ueoff, in
                            etot, cm);
 }
 
-void launch_b_qagg(hipStream_t s, int32_t n, int64_t *deg, int64_t *in, int32_t *size, ull *ddeg,
-                   ull *din, int32_t *dsize, const int32_t *vcomp, bool apply, QAcc *cq)
+void launch_b_qagg(hipStream_t s, int32_t n, int64_t *deg, const int64_t *in, int32_t *size,
+                   ull *ddeg, int32_t *dsize, int size_shift, const int32_t *vcomp, bool apply,
+                   QAcc *cq)
 {
     if (n > 0)
         hipLaunchKernelGGL(k_b_qagg, grid_for(n, 256, 1 << 20), 256, 0, s, n, deg, in, size, ddeg,
-                           din, dsize, vcomp, apply ? 1 : 0, cq);
+                           dsize, size_shift, vcomp, apply ? 1 : 0, cq);
 }
 
 void launch_b_groups(hipStream_t s, const int32_t *gstart, const int32_t *perm,

@@ -165,6 +165,34 @@ __device__ __forceinline__ void seg_atomic_add(bool valid, long long key, ull va
     if (valid && tail && v != 0) atomicAdd(&base[k], v);
 }
 
+// Adds the pass deltas of community c to its degree and size and clears them
+// (size_shift as in SweepArgs).
+__device__ __forceinline__ int64_t apply_deltas(int64_t c, int64_t *deg, int32_t *size, ull *ddeg,
+                                                int32_t *dsize, int size_shift)
+{
+    int64_t dv = deg[c];
+    const ull dd = ddeg[c];
+    if (size_shift) {
+        if (dd) {
+            const int64_t ds = (int64_t)(dd << (64 - size_shift)) >> (64 - size_shift);
+            dv += (int64_t)(dd - (ull)ds) >> size_shift;
+            deg[c] = dv;
+            size[c] += (int32_t)ds;
+            ddeg[c] = 0;
+        }
+    } else {
+        const int32_t ds = dsize[c];
+        if (dd | (ull)(uint32_t)ds) {
+            dv += (int64_t)dd;
+            deg[c] = dv;
+            size[c] += ds;
+            ddeg[c] = 0;
+            dsize[c] = 0;
+        }
+    }
+    return dv;
+}
+
 __device__ __forceinline__ int tier_of(int64_t d, int64_t q)
 {
     if (d == 0) return 0;  // no neighbours: the vertex can never move

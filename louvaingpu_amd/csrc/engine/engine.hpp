@@ -121,9 +121,15 @@ struct SweepArgs {
     int32_t *next;
     const int64_t *deg;
     const int32_t *size;
+    // Move deltas per community.  With size_shift > 0 the size delta rides in
+    // the low size_shift bits of the degree word (ddeg = delta_deg * 2^size_shift
+    // + delta_size, exact while both fit); with 0 it goes to dsize.  The
+    // internal weights need no deltas: a mover adds to the joined community
+    // what it takes from the one left, so their sum - all the modularity reads
+    // - stays the level's self-loop total.
     ull *ddeg;
-    ull *din;
     int32_t *dsize;
+    int size_shift;
     double m2;     // 2.0 * (double)m
     double gamma;
     int last;      // last pass: fuse merge-isolated, drop move deltas
@@ -279,17 +285,17 @@ void launch_loop_gather(hipStream_t s, const ull *lastloop, const int64_t *w, in
 
 void launch_init_vertex(hipStream_t s, int32_t n, const int64_t *rowptr, const int64_t *kdeg,
                         const int64_t *loopw, const int64_t *selfsum, int64_t *deg, int64_t *in,
-                        int32_t *size, ull *ddeg, ull *din, int32_t *dsize, int32_t *C,
+                        int32_t *size, ull *ddeg, int32_t *dsize, int32_t *C,
                         int32_t *next, uint32_t *tier_key, int32_t *verts,
-                        long long *tier_cnt, long long *tier_nnz);
+                        long long *tier_cnt, long long *tier_nnz, ull *ksum);
 // Q aggregates: per-block partials in part[kQParts], summed into *acc.
 constexpr int kQParts = 1024;
 void launch_qagg(hipStream_t s, int32_t n, const int64_t *deg, const int64_t *in, QAcc *part,
                  QAcc *acc);
 void launch_lab(hipStream_t s, int32_t n, const int32_t *C, const int32_t *size,
                 const int64_t *deg, Lab *lab);
-void launch_apply(hipStream_t s, int32_t n, int64_t *deg, int64_t *in, int32_t *size, ull *ddeg,
-                  ull *din, int32_t *dsize, QAcc *part, QAcc *acc);
+void launch_apply(hipStream_t s, int32_t n, int64_t *deg, const int64_t *in, int32_t *size,
+                  ull *ddeg, int32_t *dsize, int size_shift, QAcc *part, QAcc *acc);
 void launch_sweep_tier(hipStream_t s, int tier, const SweepArgs &a, const int32_t *verts,
                        int32_t nv, bool batch);
 // First pass of a level on rows that hold each neighbour once: every community
@@ -334,8 +340,9 @@ void launch_b_union_edges(hipStream_t s, const int32_t *bsrc, const int32_t *bds
                           int64_t *uw, DevFlags *f);
 void launch_b_edge_sum(hipStream_t s, const int64_t *w, const int64_t *ueoff, int32_t ncomp,
                        int64_t etot, ull *cm);
-void launch_b_qagg(hipStream_t s, int32_t n, int64_t *deg, int64_t *in, int32_t *size, ull *ddeg,
-                   ull *din, int32_t *dsize, const int32_t *vcomp, bool apply, QAcc *cq);
+void launch_b_qagg(hipStream_t s, int32_t n, int64_t *deg, const int64_t *in, int32_t *size,
+                   ull *ddeg, int32_t *dsize, int size_shift, const int32_t *vcomp, bool apply,
+                   QAcc *cq);
 void launch_b_groups(hipStream_t s, const int32_t *gstart, const int32_t *perm,
                      const int32_t *vcomp, int32_t K, ull *kcnt);
 void launch_b_next_ids(hipStream_t s, const int32_t *gstart, const int32_t *perm,

@@ -522,23 +522,26 @@ void launch_loop_gather(hipStream_t s, const ull *lastloop, const int64_t *w, in
 
 __global__ __launch_bounds__(256) void k_init_vertex(
     int32_t n, const int64_t *rowptr, const int64_t *kdeg, const int64_t *loopw,
-    const int64_t *selfsum, int64_t *deg, int64_t *in, int32_t *size, ull *ddeg, ull *din,
-    int32_t *dsize, int32_t *C, int32_t *next, uint32_t *tier_key, int32_t *verts,
-    long long *tier_cnt, long long *tier_nnz)
+    const int64_t *selfsum, int64_t *deg, int64_t *in, int32_t *size, ull *ddeg, int32_t *dsize,
+    int32_t *C, int32_t *next, uint32_t *tier_key, int32_t *verts, long long *tier_cnt,
+    long long *tier_nnz, ull *ksum)
 {
     __shared__ long long s_cnt[kTiers + 1], s_nnz[kTiers + 1];
+    __shared__ ull s_k[4];
     if (threadIdx.x <= kTiers) {
         s_cnt[threadIdx.x] = 0;
         s_nnz[threadIdx.x] = 0;
     }
     __syncthreads();
+    ull kt = 0;
     GRID_STRIDE(i, n)
     {
-        deg[i] = kdeg[i];
+        const int64_t k = kdeg[i];
+        kt = sat_add(kt, (ull)k);
+        deg[i] = k;
         in[i] = loopw[i];
         size[i] = 1;
         ddeg[i] = 0;
-        din[i] = 0;
         dsize[i] = 0;
         C[i] = (int32_t)i;
         next[i] = (int32_t)i;
@@ -549,23 +552,29 @@ __global__ __launch_bounds__(256) void k_init_vertex(
         atomicAdd((ull *)&s_cnt[t], 1ull);
         atomicAdd((ull *)&s_nnz[t], (ull)d);
     }
+    kt = wave_sum_sat(kt);
+    if ((threadIdx.x & 63) == 0) s_k[threadIdx.x >> 6] = kt;
     __syncthreads();
     if (threadIdx.x <= kTiers) {
         if (s_cnt[threadIdx.x]) atomicAdd((ull *)&tier_cnt[threadIdx.x], (ull)s_cnt[threadIdx.x]);
         if (s_nnz[threadIdx.x]) atomicAdd((ull *)&tier_nnz[threadIdx.x], (ull)s_nnz[threadIdx.x]);
     }
+    if (threadIdx.x == 0) {  // total degree (saturating at kUnitLimit), one atomic per block
+        const ull t = sat_add(sat_add(s_k[0], s_k[1]), sat_add(s_k[2], s_k[3]));
+        if (t) atomicAdd(ksum, t);
+    }
 }
 
 void launch_init_vertex(hipStream_t s, int32_t n, const int64_t *rowptr, const int64_t *kdeg,
                         const int64_t *loopw, const int64_t *selfsum, int64_t *deg, int64_t *in,
-                        int32_t *size, ull *ddeg, ull *din, int32_t *dsize, int32_t *C,
-                        int32_t *next, uint32_t *tier_key, int32_t *verts, long long *tier_cnt,
-                        long long *tier_nnz)
+                        int32_t *size, ull *ddeg, int32_t *dsize, int32_t *C, int32_t *next,
+                        uint32_t *tier_key, int32_t *verts, long long *tier_cnt,
+                        long long *tier_nnz, ull *ksum)
 {
     if (n > 0)
         hipLaunchKernelGGL(k_init_vertex, grid_for(n, 256, 8192), 256, 0, s, n, rowptr, kdeg,
-                           loopw, selfsum, deg, in, size, ddeg, din, dsize, C, next, tier_key,
-                           verts, tier_cnt, tier_nnz);
+                           loopw, selfsum, deg, in, size, ddeg, dsize, C, next, tier_key, verts,
+                           tier_cnt, tier_nnz, ksum);
 }
 
 // ---------------------------------------------------------------------------
@@ -641,27 +650,16 @@ __global__ __launch_bounds__(256) void k_qagg_ull(int32_t n, const ull *dc, cons
     block_qpartial(sin, lo, hi, part);
 }
 
-__global__ __launch_bounds__(256) void k_apply(int32_t n, int64_t *deg, int64_t *in, int32_t *size,
-                                              ull *ddeg, ull *din, int32_t *dsize, QAcc *part)
+__global__ __launch_bounds__(256) void k_apply(int32_t n, int64_t *deg, const int64_t *in,
+                                              int32_t *size, ull *ddeg, int32_t *dsize,
+                                              int size_shift, QAcc *part)
 {
     long long sin = 0;
     ull lo = 0, hi = 0;
     GRID_STRIDE(c, n)
     {
-        int64_t dv = deg[c], iv = in[c];
-        ull dd = ddeg[c], di = din[c];
-        int32_t ds = dsize[c];
-        if (dd | di | (ull)(uint32_t)ds) {
-            dv += (int64_t)dd;
-            iv += (int64_t)di;
-            deg[c] = dv;
-            in[c] = iv;
-            size[c] += ds;
-            ddeg[c] = 0;
-            din[c] = 0;
-            dsize[c] = 0;
-        }
-        sin += iv;
+        const int64_t dv = apply_deltas(c, deg, size, ddeg, dsize, size_shift);
+        sin += in[c];
         acc_sq(lo, hi, (ull)dv);
     }
     block_qpartial(sin, lo, hi, part);
@@ -703,11 +701,11 @@ void launch_qagg_ull(hipStream_t s, int32_t n, const ull *dc, const ull *lc, QAc
     hipLaunchKernelGGL(k_qfinal, 1, 256, 0, s, part, nb, acc);
 }
 
-void launch_apply(hipStream_t s, int32_t n, int64_t *deg, int64_t *in, int32_t *size, ull *ddeg,
-                  ull *din, int32_t *dsize, QAcc *part, QAcc *acc)
+void launch_apply(hipStream_t s, int32_t n, int64_t *deg, const int64_t *in, int32_t *size,
+                  ull *ddeg, int32_t *dsize, int size_shift, QAcc *part, QAcc *acc)
 {
     int nb = qblocks(n);
-    hipLaunchKernelGGL(k_apply, nb, 256, 0, s, n, deg, in, size, ddeg, din, dsize, part);
+    hipLaunchKernelGGL(k_apply, nb, 256, 0, s, n, deg, in, size, ddeg, dsize, size_shift, part);
     hipLaunchKernelGGL(k_qfinal, 1, 256, 0, s, part, nb, acc);
 }
 
@@ -783,34 +781,23 @@ __device__ __forceinline__ void finalize_vertex(const SweepArgs &a, int32_t v, i
         a.next[v] = own;
         return;
     }
-    const bool has = bc != INT_MAX;
-    const int32_t c0 = has ? bc : own;
-    const int64_t w0 = has ? bw : 0;
+    const int32_t c0 = bc != INT_MAX ? bc : own;
     int32_t choice = c0;
     if (so == 1 && a.size[c0] == 1 && c0 > own) choice = own;
-    const int64_t k = a.kdeg[v], lp = a.loopw[v];
-    if (mode == 1) {
-        a.next[v] = choice;
-        if (choice != own) {
-            atomicAdd(&a.ddeg[own], (ull)(-k));
-            atomicAdd(&a.ddeg[choice], (ull)k);
-            atomicAdd(&a.din[own], (ull)(-(w0 + lp)));
-            atomicAdd(&a.din[choice], (ull)(w0 + lp));
-            atomicAdd(&a.dsize[own], -1);
-            atomicAdd(&a.dsize[choice], 1);
-        }
+    int32_t to = choice;  // mode 2: only the merge-isolated move
+    if (mode == 2) to = (n == 1 && so == 1 && c0 != own && own < c0) ? c0 : own;
+    a.next[v] = mode == 2 && to == own ? choice : to;
+    if (to == own) return;
+    const int64_t k = a.kdeg[v];
+    if (a.size_shift) {  // one atomic per side: delta_deg * 2^shift + delta_size
+        const ull pk = ((ull)k << a.size_shift) + 1ull;
+        atomicAdd(&a.ddeg[own], 0ull - pk);
+        atomicAdd(&a.ddeg[to], pk);
     } else {
-        int32_t nv = choice;
-        if (n == 1 && so == 1 && c0 != own && own < c0) {
-            nv = c0;
-            atomicAdd(&a.ddeg[own], (ull)(-k));
-            atomicAdd(&a.ddeg[c0], (ull)k);
-            atomicAdd(&a.din[own], (ull)(-lp));
-            atomicAdd(&a.din[c0], (ull)lp);
-            atomicAdd(&a.dsize[own], -1);
-            atomicAdd(&a.dsize[c0], 1);
-        }
-        a.next[v] = nv;
+        atomicAdd(&a.ddeg[own], (ull)(-k));
+        atomicAdd(&a.ddeg[to], (ull)k);
+        atomicAdd(&a.dsize[own], -1);
+        atomicAdd(&a.dsize[to], 1);
     }
 }
 

@@ -190,6 +190,23 @@ def test_arbitrary_ids_float_weights_and_loops():
         assert np.array_equal(node, onode) and np.array_equal(comm, ocomm)
 
 
+def test_wide_weight_units():
+    """Weights up to 2^18 next to 2^-20 make the weight unit 2^-20, so the
+    total degree is about 2^57 units: with n = 2^15 the per-community size
+    delta no longer fits beside the degree delta in one 64-bit word and the
+    sweep keeps them in separate arrays."""
+    gpu_case("wide weight units")
+    n, src, dst, _ = generators.rmat(15, 16, 0.57, 0.19, 0.19, 11, "int")
+    rng = np.random.Generator(np.random.PCG64(5))
+    w = rng.integers(1, 1 << 18, len(src)).astype(np.float32)
+    w[rng.integers(0, len(src), 64)] = np.float32(2.0 ** -20)
+    eng, r = run_both(n, src, dst, w)
+    with eng:
+        k = np.asarray(r.vertex(0)[0], np.float64)
+        assert math.log2(k.sum()) + (n.bit_length() + 1) > 62  # the unpacked case
+        assert_run_matches(eng, r, n)
+
+
 def test_repeat_runs_are_identical():
     gpu_case("repeat runs")
     n, src, dst, w = generators.rmat(14, 16, 0.57, 0.19, 0.19, 7, "int")
