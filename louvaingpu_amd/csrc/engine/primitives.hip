@@ -350,12 +350,32 @@ __global__ __launch_bounds__(kRadixBlock) void k_radix_hist(const K *keys, int64
     hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = s_h[threadIdx.x];
 }
 
-// Scatter of one digit pass.  A block round covers 1024 consecutive items:
-// wave w owns [w*256, w*256+256) of the round as four sub-rounds of 64 lanes.
-// Each sub-round ranks its items among equal digits with eight ballots and the
-// wave's running per-digit counters in LDS, so item order (round, wave,
-// sub-round, lane) is preserved: the sort is stable.
-constexpr int kRadixSub = 4;  // sub-rounds of 64 items per wave per round
+// Scatter of one digit pass.  A block round covers kRadixTile consecutive
+// items: wave w owns the round's w-th quarter as kRadixSub sub-rounds of 64
+// lanes.  Each sub-round ranks its items among equal digits with eight
+// ballots and the wave's running per-digit counters in LDS, so item order
+// (round, wave, sub-round, lane) is preserved: the sort is stable.  The round
+// is then reordered by digit in LDS and written out by consecutive threads,
+// so each digit's items of the round leave as one contiguous run.
+constexpr int kRadixSub = 8;  // sub-rounds of 64 items per wave per round
+constexpr int kRadixTile = kRadixBlock * kRadixSub;
+
+// Exclusive scan of one int per thread over a 256-thread block.
+__device__ __forceinline__ int block_exclusive_scan_256(int x, int *s_wave)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int inc = x;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(inc, off, 64);
+        if (lane >= off) inc += y;
+    }
+    if (lane == 63) s_wave[wave] = inc;
+    __syncthreads();
+    int pre = 0;
+    for (int w = 0; w < wave; ++w) pre += s_wave[w];
+    return pre + inc - x;
+}
 
 template <typename K, typename V>
 __global__ __launch_bounds__(kRadixBlock) void k_radix_scatter(const K *kin, const V *vin, K *kout,
@@ -364,16 +384,18 @@ __global__ __launch_bounds__(kRadixBlock) void k_radix_scatter(const K *kin, con
                                                               int64_t nblocks)
 {
     constexpr int NW = kRadixBlock / 64;
-    constexpr int PER_ROUND = kRadixBlock * kRadixSub;
-    __shared__ int64_t s_base[256];
-    __shared__ int s_cnt[NW][256];
-    __shared__ int64_t s_off[NW][256];
+    __shared__ int64_t s_base[256];  // next output position of each digit for this block
+    __shared__ int s_cnt[NW][256];   // per-wave digit counts, then tile offsets
+    __shared__ int s_start[256];     // first tile position of each digit
+    __shared__ int s_wave[NW];
+    __shared__ K s_k[kRadixTile];
+    __shared__ V s_v[kRadixTile];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     s_base[tid] = hscan[(int64_t)tid * nblocks + blockIdx.x];
     const ull lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
     const int64_t base = (int64_t)blockIdx.x * rounds * kRadixBlock;
     const int64_t end = min(n, base + (int64_t)rounds * kRadixBlock);
-    for (int64_t r0 = base; r0 < end; r0 += PER_ROUND) {
+    for (int64_t r0 = base; r0 < end; r0 += kRadixTile) {
 #pragma unroll
         for (int w = 0; w < NW; ++w) s_cnt[w][tid] = 0;
         __syncthreads();
@@ -404,22 +426,36 @@ __global__ __launch_bounds__(kRadixBlock) void k_radix_scatter(const K *kin, con
             __builtin_amdgcn_wave_barrier();
         }
         __syncthreads();
-        int64_t run = s_base[tid];
+        int tot = 0, woff[NW];
 #pragma unroll
         for (int w = 0; w < NW; ++w) {
-            s_off[w][tid] = run;
-            run += s_cnt[w][tid];
+            woff[w] = tot;
+            tot += s_cnt[w][tid];
         }
-        s_base[tid] = run;
+        const int start = block_exclusive_scan_256(tot, s_wave);
+        s_start[tid] = start;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) s_cnt[w][tid] = start + woff[w];
         __syncthreads();
 #pragma unroll
         for (int u = 0; u < kRadixSub; ++u) {
             if (valid[u]) {
-                const int64_t dst = s_off[wave][d[u]] + rank[u];
-                kout[dst] = k[u];
-                vout[dst] = v[u];
+                const int lp = s_cnt[wave][d[u]] + rank[u];
+                s_k[lp] = k[u];
+                s_v[lp] = v[u];
             }
         }
+        __syncthreads();
+        const int items = (int)min((int64_t)kRadixTile, end - r0);
+        for (int t = tid; t < items; t += kRadixBlock) {
+            const K kk = s_k[t];
+            const int dd = (int)((kk >> shift) & 255);
+            const int64_t dst = s_base[dd] + (t - s_start[dd]);
+            kout[dst] = kk;
+            vout[dst] = s_v[t];
+        }
+        __syncthreads();
+        s_base[tid] += tot;
     }
 }
 
@@ -430,7 +466,7 @@ static int radix_sort(Arena &a, hipStream_t s, K *keys, V *vals, K *ktmp, V *vtm
     *kout = keys;
     *vout = vals;
     if (n <= 1 || bits <= 0) return LV_OK;
-    int rounds = 4;
+    int rounds = kRadixSub;  // at least one full round per block
     while ((n + (int64_t)rounds * kRadixBlock - 1) / ((int64_t)rounds * kRadixBlock) > 32768)
         rounds *= 2;
     int64_t nblocks = (n + (int64_t)rounds * kRadixBlock - 1) / ((int64_t)rounds * kRadixBlock);
