@@ -353,10 +353,13 @@ double current_q(lv_ctx *c, int &rc)
 // ---------------------------------------------------------------------------
 // phases
 
-// Level-0 CSR for edges sorted by source (self[i] = edge i is a self-loop):
-// forward entries stay in edge order, reverse entries are radix-sorted by
-// destination, row v = its reverse entries then its forward entries.
-int build_level0_csr_sorted(lv_ctx *c, int32_t *self, int wbits, const ull *lastloop)
+// Level-0 CSR for edges sorted by source (self[i] = edge i is a self-loop, fs[v]
+// = first edge with src >= v): forward entries stay in edge order, reverse
+// entries are radix-sorted by destination, row v = its reverse entries then
+// its forward entries.  A canonical, strictly sorted list (canonical) gives rows
+// that list each neighbour once in increasing order; otherwise the fills check.
+int build_level0_csr_sorted(lv_ctx *c, int32_t *self, const int64_t *fs, bool canonical,
+                            int wbits, const ull *lastloop)
 {
     Arena &A = c->arena;
     hipStream_t s = c->stream;
@@ -369,16 +372,16 @@ int build_level0_csr_sorted(lv_ctx *c, int32_t *self, int wbits, const ull *last
     LV_TRY(d2h(c, &nself, (const int32_t *)(slscan + e), 1));
     LV_TRY(sync(c));
     const int64_t nr = e - nself;
-    int64_t *fs = nullptr, *rs = nullptr;
+    int64_t *rs = nullptr, *ffwd = nullptr;
     uint32_t *k1 = nullptr, *k2 = nullptr;
     uint64_t *p1 = nullptr, *p2 = nullptr;
-    LV_TRY(arena_alloc(A, fs, (int64_t)n + 1));
     LV_TRY(arena_alloc(A, rs, (int64_t)n + 1));
+    LV_TRY(arena_alloc(A, ffwd, (int64_t)n + 1));
     LV_TRY(arena_alloc(A, k1, nr));
     LV_TRY(arena_alloc(A, k2, nr));
     LV_TRY(arena_alloc(A, p1, nr));
     LV_TRY(arena_alloc(A, p2, nr));
-    launch_csr_sorted(s, 0, c->d_src, c->d_dst, c->d_w, slscan, e, n, wbits, fs, k1, p1);
+    launch_rev_entries(s, c->d_src, c->d_dst, c->d_w, slscan, e, wbits, k1, p1);
     uint32_t *sk = nullptr;
     uint64_t *sp = nullptr;
     LV_TRY(radix_sort_u32_u64(A, s, k1, p1, k2, p2, nr, bits_for(n > 0 ? (uint64_t)(n - 1) : 0),
@@ -388,20 +391,24 @@ int build_level0_csr_sorted(lv_ctx *c, int32_t *self, int wbits, const ull *last
     LV_TRY(arena_alloc(A, c->col, c->nnz));
     LV_TRY(arena_alloc(A, c->cw, c->nnz));
     LV_HIP(hipMemsetAsync(&c->d_flags->err, 0, sizeof(int), s));
-    launch_csr_sorted_fill(s, c->d_src, c->d_dst, c->d_w, slscan, e, n, wbits, fs, sk, sp, nr, rs,
-                           c->rowptr, c->col, c->cw, c->kdeg, c->d_flags);
+    launch_csr_sorted_fill(s, !canonical, c->d_src, c->d_dst, c->d_w, slscan, e, n, wbits, fs, sk,
+                           sp, nr, rs, c->rowptr, ffwd, c->col, c->cw, c->kdeg, c->d_flags);
     launch_loop_gather(s, lastloop, c->d_w, n, c->loopw);
     LV_HIP(hipGetLastError());
-    LV_TRY(d2h(c, c->h_flags, c->d_flags, 1));
-    LV_TRY(sync(c));
-    c->rows_unique = (c->h_flags->err & 4) == 0;
+    if (canonical) {
+        c->rows_unique = true;
+    } else {
+        LV_TRY(d2h(c, c->h_flags, c->d_flags, 1));
+        LV_TRY(sync(c));
+        c->rows_unique = (c->h_flags->err & 4) == 0;
+    }
     arena_free(A, k1);
     arena_free(A, k2);
     arena_free(A, p1);
     arena_free(A, p2);
     arena_free(A, slscan);
-    arena_free(A, fs);
     arena_free(A, rs);
+    arena_free(A, ffwd);
     return LV_OK;
 }
 
@@ -434,19 +441,24 @@ int build_level0_csr(lv_ctx *c)
     fill_i64(s, c->kdeg, n, 0);
     fill_i64(s, c->selfsum, n, 0);
     fill_i64(s, (int64_t *)lastloop, n, 0);
-    launch_self_loops(s, c->d_src, c->d_dst, c->d_w, e, c->kdeg, c->selfsum, lastloop);
     if (packed && e < INT32_MAX) {
         // edges sorted by source: only the reverse entries need the row sort
+        // (the same pass over the edges accumulates the self-loops)
         int32_t *self = nullptr;
+        int64_t *fs = nullptr;
         LV_TRY(arena_alloc(A, self, e + 1));
+        LV_TRY(arena_alloc(A, fs, (int64_t)n + 1));
         LV_HIP(hipMemsetAsync(&c->d_flags->err, 0, sizeof(int), s));
-        launch_src_sorted(s, c->d_src, c->d_dst, e, self, c->d_flags);
+        launch_src_sorted(s, c->d_src, c->d_dst, c->d_w, e, n, self, fs, c->kdeg, c->selfsum,
+                          lastloop, c->d_flags);
         LV_HIP(hipGetLastError());
         LV_TRY(d2h(c, c->h_flags, c->d_flags, 1));
         LV_TRY(sync(c));
-        if ((c->h_flags->err & 8) == 0) {
-            const int rc = build_level0_csr_sorted(c, self, wbits, lastloop);
+        const int err = c->h_flags->err;
+        if ((err & 8) == 0) {
+            const int rc = build_level0_csr_sorted(c, self, fs, (err & 16) == 0, wbits, lastloop);
             arena_free(A, self);
+            arena_free(A, fs);
             arena_free(A, lastloop);
             if (rc != LV_OK) return rc;
             t_end(c, t0, CAT_CSR);
@@ -455,6 +467,9 @@ int build_level0_csr(lv_ctx *c)
             return LV_OK;
         }
         arena_free(A, self);
+        arena_free(A, fs);
+    } else {
+        launch_self_loops(s, c->d_src, c->d_dst, c->d_w, e, c->kdeg, c->selfsum, lastloop);
     }
     LV_TRY(arena_alloc(A, k1, total));
     LV_TRY(arena_alloc(A, k2, total));

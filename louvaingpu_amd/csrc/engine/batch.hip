@@ -36,11 +36,11 @@ __global__ void k_b_graph_exponent(const uint32_t *bw, int wtype, const int64_t 
             }
             g = lo;
             if (wtype == LV_WEIGHT_INT32) {
-                if ((int32_t)bw[i] < 0) bad = 1;
+                if ((int32_t)bw[i] < 0) bad |= 1;
             } else {
                 sc = f32_unit_exponent(bw[i]);
                 if (sc < 0) {
-                    bad = 1;
+                    bad |= 1;
                     sc = 0;
                 }
             }
@@ -54,7 +54,7 @@ __global__ void k_b_graph_exponent(const uint32_t *bw, int wtype, const int64_t 
         const int32_t ng = __shfl_down(g, 1, 64);
         if (valid && (lane == 63 || ng != g) && sc > 0) atomicMax(&gscale[g], sc);
     }
-    bad = wave_max_i(bad);
+    bad = wave_or_i(bad);
     if (lane == 0 && bad) atomicOr(&f->err, 1);
 }
 
@@ -80,7 +80,7 @@ __global__ __launch_bounds__(256) void k_b_union_edges(const int32_t *bsrc, cons
         const int32_t g = lo / nres;
         const int64_t i = beoff[g] + (j - ueoff[lo]);
         const int32_t a = bsrc[i], b = bdst[i], ng = bn[g];
-        if (a < 0 || a >= ng || b < 0 || b >= ng) bad = 1;
+        if (a < 0 || a >= ng || b < 0 || b >= ng) bad |= 1;
         usrc[j] = a + (int32_t)uvoff[lo];
         udst[j] = b + (int32_t)uvoff[lo];
         const int64_t u = weight_units(bw[i], wtype, gscale[g], bad);

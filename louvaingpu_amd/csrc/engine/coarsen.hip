@@ -248,15 +248,20 @@ __global__ __launch_bounds__(256) void k_cr_seg_tier(const int32_t *gstart, cons
     __shared__ long long s_cnt[kTiers + 1];
     if (threadIdx.x <= kTiers) s_cnt[threadIdx.x] = 0;
     __syncthreads();
-    GRID_STRIDE(g, K)
-    {
-        int64_t b = estart[gstart[g]], e = estart[gstart[g + 1]];
-        sb[g] = b;
-        slen[g] = e - b;
-        int t = newid_g[g] < 0 ? 0 : tier_of(e - b, e - b);
-        tkey[g] = (uint32_t)t;
-        sid[g] = (int32_t)g;
-        atomicAdd((ull *)&s_cnt[t], 1ull);
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < K;
+         base += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t g = base + threadIdx.x;
+        const bool valid = g < K;
+        int t = 0;
+        if (valid) {
+            int64_t b = estart[gstart[g]], e = estart[gstart[g + 1]];
+            sb[g] = b;
+            slen[g] = e - b;
+            t = newid_g[g] < 0 ? 0 : tier_of(e - b, e - b);
+            tkey[g] = (uint32_t)t;
+            sid[g] = (int32_t)g;
+        }
+        wave_bucket_add(valid, t, 0, s_cnt, nullptr);
     }
     __syncthreads();
     if (threadIdx.x <= kTiers && s_cnt[threadIdx.x])

@@ -35,7 +35,7 @@ __device__ __forceinline__ int f32_unit_exponent(uint32_t x)
 }
 
 // Weight in units of 2^-scale from its raw 32-bit pattern (int32 or float32);
-// sets bad = 2 when it reaches the exact range limit.
+// sets bit 2 of bad when it reaches the exact range limit.
 __device__ __forceinline__ int64_t weight_units(uint32_t x, int wtype, int scale, int &bad)
 {
     int64_t u;
@@ -48,13 +48,13 @@ __device__ __forceinline__ int64_t weight_units(uint32_t x, int wtype, int scale
         uint64_t M = ex ? (mant | 0x800000u) : mant;
         int sh = (ex ? (int)ex - 150 : -149) + scale;
         if (sh > 36) {
-            bad = 2;
+            bad |= 2;
             u = 0;
         } else {
             u = (int64_t)(sh >= 0 ? (M << sh) : (M >> (-sh)));
         }
     }
-    if (u >= kUnitLimit) bad = 2;
+    if (u >= kUnitLimit) bad |= 2;
     return u;
 }
 
@@ -67,6 +67,14 @@ __device__ __forceinline__ long long wave_sum(long long v)
 __device__ __forceinline__ int wave_max_i(int v)
 {
     for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// OR of an error-flag word over the wavefront (flags of different lanes may
+// hold different bits).
+__device__ __forceinline__ int wave_or_i(int v)
+{
+    for (int off = 32; off > 0; off >>= 1) v |= __shfl_xor(v, off, 64);
     return v;
 }
 
@@ -94,7 +102,7 @@ __device__ __forceinline__ void block_units_flush(ull tot, ull msum, ull umax, i
     tot = wave_sum_sat(tot);
     msum = wave_sum_sat(msum);
     for (int off = 32; off > 0; off >>= 1) umax = max(umax, __shfl_xor(umax, off, 64));
-    bad = wave_max_i(bad);
+    bad = wave_or_i(bad);
     const int wave = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
         s_t[wave] = tot;
@@ -191,6 +199,27 @@ __device__ __forceinline__ int64_t apply_deltas(int64_t c, int64_t *deg, int32_t
         }
     }
     return dv;
+}
+
+// Adds (1, val) of every valid lane to (cnt[key], sum[key]) in LDS with one
+// atomic pair per distinct key of the wave (all lanes of the wave call it).
+__device__ __forceinline__ void wave_bucket_add(bool valid, int key, long long val, long long *cnt,
+                                                long long *sum)
+{
+    const int lane = threadIdx.x & 63;
+    ull rem = __ballot(valid);
+    while (rem) {
+        const int leader = __ffsll((long long)rem) - 1;
+        const int k = __shfl(key, leader, 64);
+        const bool mine = valid && key == k;
+        const ull m = __ballot(mine);
+        const long long t = wave_sum(mine ? val : 0);
+        if (lane == 0) {
+            atomicAdd((ull *)&cnt[k], (ull)__popcll(m));
+            if (sum && t) atomicAdd((ull *)&sum[k], (ull)t);
+        }
+        rem &= ~m;
+    }
 }
 
 __device__ __forceinline__ int tier_of(int64_t d, int64_t q)

@@ -245,7 +245,9 @@ int bits_for(uint64_t maxval);  // bits needed to hold values 0..maxval
 // louvain_kernels.hip (host launchers)
 struct DevFlags {  // error flags and small scalars written by kernels
     int32_t err;              // bit 1: invalid input, bit 2: out of exact range,
-                              // bit 4: CSR rows not strictly increasing
+                              // bit 4: CSR rows not strictly increasing,
+                              // bit 8: src not sorted, bit 16: edge list not
+                              // canonical and strictly sorted
     int32_t scale;            // weight unit exponent s
     int32_t pad0, pad1;
     long long total_units;    // sum of (w == 0 ? 1 : w) in units
@@ -306,16 +308,17 @@ void launch_sweep1_tier(hipStream_t s, int tier, const SweepArgs &a, const int32
 int run_hub_sweep1(hipStream_t s, const SweepArgs &a, const HubArgs &h, int32_t nh,
                    int64_t nchunks, bool batch);
 // CSR build for edges sorted by source (see louvain_kernels.hip).
-void launch_src_sorted(hipStream_t s, const int32_t *src, const int32_t *dst, int64_t e,
-                       int32_t *self, DevFlags *f);
-void launch_csr_sorted(hipStream_t s, int stage, const int32_t *src, const int32_t *dst,
-                       const int64_t *w, const int32_t *slscan, int64_t e, int32_t n, int wbits,
-                       int64_t *fs, uint32_t *keys, uint64_t *payload);
-void launch_csr_sorted_fill(hipStream_t s, const int32_t *src, const int32_t *dst,
+void launch_src_sorted(hipStream_t s, const int32_t *src, const int32_t *dst, const int64_t *w,
+                       int64_t e, int32_t n, int32_t *self, int64_t *fs, int64_t *kdeg,
+                       int64_t *selfsum, ull *lastloop, DevFlags *f);
+void launch_rev_entries(hipStream_t s, const int32_t *src, const int32_t *dst, const int64_t *w,
+                        const int32_t *slscan, int64_t e, int wbits, uint32_t *keys,
+                        uint64_t *payload);
+void launch_csr_sorted_fill(hipStream_t s, bool check, const int32_t *src, const int32_t *dst,
                             const int64_t *w, const int32_t *slscan, int64_t e, int32_t n,
                             int wbits, const int64_t *fs, const uint32_t *skey,
                             const uint64_t *spay, int64_t nr, const int64_t *rs, int64_t *rowptr,
-                            int32_t *col, int64_t *cw, int64_t *kdeg, DevFlags *f);
+                            int64_t *ffwd, int32_t *col, int64_t *cw, int64_t *kdeg, DevFlags *f);
 // flags->err |= 4 when some row's columns are not strictly increasing.
 void launch_rows_check(hipStream_t s, const uint32_t *skeys, const int32_t *col, int64_t nnz,
                        DevFlags *f);

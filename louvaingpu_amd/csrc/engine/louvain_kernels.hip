@@ -20,17 +20,17 @@ __global__ void k_weight_exponent(const void *w, int wtype, int64_t e, DevFlags 
     GRID_STRIDE(i, e)
     {
         if (wtype == LV_WEIGHT_INT32) {
-            if (((const int32_t *)w)[i] < 0) bad = 1;
+            if (((const int32_t *)w)[i] < 0) bad |= 1;
         } else {
             int s = f32_unit_exponent(((const uint32_t *)w)[i]);
             if (s < 0)
-                bad = 1;
+                bad |= 1;
             else
                 smax = max(smax, s);
         }
     }
     smax = wave_max_i(smax);
-    bad = wave_max_i(bad);
+    bad = wave_or_i(bad);
     if ((threadIdx.x & 63) == 0) {
         if (smax) atomicMax(&f->scale, smax);
         if (bad) atomicOr(&f->err, 1);
@@ -60,9 +60,9 @@ __global__ void k_check_endpoints(const int32_t *src, const int32_t *dst, int64_
     GRID_STRIDE(i, e)
     {
         int32_t a = src[i], b = dst[i];
-        if (a < 0 || a >= n || b < 0 || b >= n) bad = 1;
+        if (a < 0 || a >= n || b < 0 || b >= n) bad |= 1;
     }
-    bad = wave_max_i(bad);
+    bad = wave_or_i(bad);
     if ((threadIdx.x & 63) == 0 && bad) atomicOr(&f->err, 1);
 }
 
@@ -76,9 +76,9 @@ __global__ void k_check_sorted_unique(const uint32_t *keys, int64_t n, DevFlags 
     int bad = 0;
     GRID_STRIDE(i, n)
     {
-        if (i > 0 && keys[i] == keys[i - 1]) bad = 1;
+        if (i > 0 && keys[i] == keys[i - 1]) bad |= 1;
     }
-    bad = wave_max_i(bad);
+    bad = wave_or_i(bad);
     if ((threadIdx.x & 63) == 0 && bad) atomicOr(&f->err, 1);
 }
 
@@ -103,12 +103,12 @@ __global__ void k_map_endpoints(const uint32_t *skeys, const int32_t *spos, int3
                 hi = mid - 1;
         }
         if (found < 0) {
-            bad = 1;
+            bad |= 1;
             found = 0;
         }
         ep[i] = found;
     }
-    bad = wave_max_i(bad);
+    bad = wave_or_i(bad);
     if ((threadIdx.x & 63) == 0 && bad) atomicOr(&f->err, 1);
 }
 
@@ -249,33 +249,80 @@ __global__ void k_self_loops(const int32_t *src, const int32_t *dst, const int64
 // the reverse entries (row = dst) go through the row sort, and row v is its
 // reverse entries (in edge order) followed by its forward ones.
 
-// bit 8 of err when src is not non-decreasing; self[i] = 1 for a self-loop edge.
-__global__ void k_src_sorted(const int32_t *src, const int32_t *dst, int64_t e, int32_t *self,
-                             DevFlags *f)
+// Every lane of the wavefront calls this: out[v] = val for v in the lane's
+// range (lo, hi]; a range longer than a few entries is written by the whole
+// wavefront.
+__device__ __forceinline__ void wave_range_fill(long long lo, long long hi, long long val,
+                                                int64_t *out)
 {
-    int bad = 0;
-    GRID_STRIDE(i, e)
-    {
-        const int32_t a = src[i];
-        if (i > 0 && a < src[i - 1]) bad = 8;
-        self[i] = (a == dst[i]) ? 1 : 0;
+    const int lane = threadIdx.x & 63;
+    const bool wide = hi - lo > 8;
+    if (!wide)
+        for (long long v = lo + 1; v <= hi; ++v) out[v] = val;
+    ull m = __ballot(wide);
+    while (m) {
+        const int l = __ffsll((unsigned long long)m) - 1;
+        const long long a = __shfl(lo, l, 64), b = __shfl(hi, l, 64), x = __shfl(val, l, 64);
+        for (long long v = a + 1 + lane; v <= b; v += 64) out[v] = x;
+        m &= m - 1;
     }
-    bad = wave_max_i(bad);
-    if ((threadIdx.x & 63) == 0 && bad) atomicOr(&f->err, bad);
 }
 
-// fs[v] = first edge with src >= v (v in [0, n]).
-__global__ void k_src_starts(const int32_t *src, int64_t e, int32_t n, int64_t *fs)
+// The two ends of "out[v] = first position r with min(keys[r], n) >= v" over
+// ascending keys: 0 up to the first key, total past the last one.  (The
+// ranges between keys are written where the keys change.)
+template <typename K>
+__global__ void k_range_ends(const K *keys, int64_t total, int32_t n, int64_t *out)
 {
+    const int64_t k0 = min((int64_t)keys[0], (int64_t)n);
+    const int64_t k1 = min((int64_t)keys[total - 1], (int64_t)n);
     GRID_STRIDE(v, (int64_t)n + 1)
     {
-        int64_t lo = 0, hi = e;
-        while (lo < hi) {
-            const int64_t mid = (lo + hi) >> 1;
-            if (src[mid] < (int32_t)v) lo = mid + 1; else hi = mid;
-        }
-        fs[v] = lo;
+        if (v <= k0)
+            out[v] = 0;
+        else if (v > k1)
+            out[v] = total;
     }
+}
+
+// One pass over the edge list: err bit 8 when src is not non-decreasing, bit
+// 16 when the list is not canonical and strictly sorted (src <= dst and
+// (src, dst) strictly increasing - then every row lists each neighbour once
+// and in increasing order); self[i] = 1 for a self-loop edge, whose weight
+// goes to selfsum/kdeg of its vertex and whose position to lastloop; and, for
+// a sorted list, fs[v] = the first edge with src >= v between the first and
+// the last source (k_range_ends writes the two ends).
+__global__ void k_src_sorted(const int32_t *src, const int32_t *dst, const int64_t *w, int64_t e,
+                             int32_t *self, int64_t *fs, int64_t *kdeg, int64_t *selfsum,
+                             ull *lastloop, DevFlags *f)
+{
+    int bad = 0;
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < e;
+         base += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = base + threadIdx.x;
+        long long lo = 0, hi = 0;
+        if (i < e) {
+            const int32_t a = src[i], b = dst[i];
+            const int32_t pa = i > 0 ? src[i - 1] : -1;
+            if (a < pa) bad |= 8;
+            if (a > b || (a == pa && b <= dst[i - 1])) bad |= 16;
+            self[i] = (a == b) ? 1 : 0;
+            if (a == b) {
+                const int64_t u = w[i];
+                const ull wp = (ull)(u == 0 ? 1 : u);
+                atomicAdd((ull *)&selfsum[a], wp);
+                atomicAdd((ull *)&kdeg[a], 4ull * wp);
+                atomicMax(&lastloop[a], (ull)(i + 1));
+            }
+            if (i > 0 && a > pa) {  // fs[v] = i for v in (pa, a]; in bounds for any order
+                lo = pa;
+                hi = a;
+            }
+        }
+        wave_range_fill(lo, hi, (long long)i, fs);
+    }
+    bad = wave_or_i(bad);
+    if ((threadIdx.x & 63) == 0 && bad) atomicOr(&f->err, bad);
 }
 
 // Reverse entries of non-self edges, compacted: key dst, payload src << wbits | w.
@@ -293,16 +340,26 @@ __global__ void k_rev_entries(const int32_t *src, const int32_t *dst, const int6
     }
 }
 
-// rowptr[v] = reverse entries of rows < v + forward (non-self) entries of rows < v.
+// rowptr[v] = reverse entries of rows < v + forward (non-self) entries of rows
+// < v; ffwd[v] = rowptr[v] + (reverse entries of v) - (non-self edges before
+// fs[v]), so forward edge i of row v lands at ffwd[v] + (i - slscan[i]).
 __global__ void k_rowptr_sum(const int64_t *rs, const int64_t *fs, const int32_t *slscan, int32_t n,
-                             int64_t *rowptr)
+                             int64_t *rowptr, int64_t *ffwd)
 {
-    GRID_STRIDE(v, (int64_t)n + 1) rowptr[v] = rs[v] + (fs[v] - slscan[fs[v]]);
+    GRID_STRIDE(v, (int64_t)n + 1)
+    {
+        const int64_t nf = fs[v] - slscan[fs[v]];
+        const int64_t rp = rs[v] + nf;
+        rowptr[v] = rp;
+        if (v < n) ffwd[v] = rp + (rs[v + 1] - rs[v]) - nf;
+    }
 }
 
-// The fills also check that every row's columns strictly increase (err bit
-// 4): within the reverse part, within the forward part (skipping self-loop
-// edges), and from the last reverse column to the first forward one.
+// The fills sum k per row and, with CHECK, test that every row's columns
+// strictly increase (err bit 4): within the reverse part, within the forward
+// part (skipping self-loop edges), and from the last reverse column to the
+// first forward one.  Without CHECK the list is canonical and strictly sorted.
+template <bool CHECK>
 __global__ __launch_bounds__(256) void k_fill_rev(const uint32_t *skey, const uint64_t *spay,
                                                  int64_t nr, int wbits, const int64_t *rs,
                                                  const int64_t *rowptr, int32_t *col, int64_t *cw,
@@ -325,19 +382,23 @@ __global__ __launch_bounds__(256) void k_fill_rev(const uint32_t *skey, const ui
             cw[pos] = u;
             row = v;
             wp = (ull)(u == 0 ? 1 : u);
-            if (r > 0 && skey[r - 1] == v && (int32_t)(spay[r - 1] >> wbits) >= cc) bad = 4;
+            if constexpr (CHECK)
+                if (r > 0 && skey[r - 1] == v && (int32_t)(spay[r - 1] >> wbits) >= cc) bad = 4;
         }
         seg_atomic_add(valid, row, wp, (ull *)kdeg);
     }
-    bad = wave_max_i(bad);
-    if ((threadIdx.x & 63) == 0 && bad) atomicOr(&f->err, bad);
+    if constexpr (CHECK) {
+        bad = wave_or_i(bad);
+        if ((threadIdx.x & 63) == 0 && bad) atomicOr(&f->err, bad);
+    }
 }
 
+template <bool CHECK>
 __global__ __launch_bounds__(256) void k_fill_fwd(const int32_t *src, const int32_t *dst,
                                                  const int64_t *w, const int32_t *slscan, int64_t e,
                                                  const int64_t *fs, const int64_t *rs,
                                                  const uint64_t *spay, int wbits,
-                                                 const int64_t *rowptr, int32_t *col, int64_t *cw,
+                                                 const int64_t *ffwd, int32_t *col, int64_t *cw,
                                                  int64_t *kdeg, DevFlags *f)
 {
     int bad = 0;
@@ -352,76 +413,96 @@ __global__ __launch_bounds__(256) void k_fill_fwd(const int32_t *src, const int3
             row = v;  // a self-loop edge keeps its row's key with weight 0, so the
                       // runs of equal keys stay contiguous for the segmented sum
             if (v != b) {
-                const int64_t f0 = fs[v];
-                const int64_t rank = (i - f0) - (slscan[i] - slscan[f0]);
-                const int64_t nrev = rs[v + 1] - rs[v];
-                const int64_t pos = rowptr[v] + nrev + rank;
+                const int64_t pos = ffwd[v] + (i - slscan[i]);
                 const int64_t u = w[i];
                 col[pos] = b;
                 cw[pos] = u;
                 wp = (ull)(u == 0 ? 1 : u);
-                if (rank > 0) {
-                    int64_t k = i - 1;  // the previous non-self edge of the row
-                    while (src[k] == dst[k]) --k;
-                    if (dst[k] >= b) bad = 4;
-                } else if (nrev > 0 && (int32_t)(spay[rs[v + 1] - 1] >> wbits) >= b) {
-                    bad = 4;
+                if constexpr (CHECK) {
+                    const int64_t f0 = fs[v];
+                    const int64_t rank = (i - f0) - (slscan[i] - slscan[f0]);
+                    if (rank > 0) {
+                        int64_t k = i - 1;  // the previous non-self edge of the row
+                        while (src[k] == dst[k]) --k;
+                        if (dst[k] >= b) bad = 4;
+                    } else if (rs[v + 1] > rs[v] &&
+                               (int32_t)(spay[rs[v + 1] - 1] >> wbits) >= b) {
+                        bad = 4;
+                    }
                 }
             }
         }
         seg_atomic_add(valid, row, wp, (ull *)kdeg);
     }
-    bad = wave_max_i(bad);
-    if ((threadIdx.x & 63) == 0 && bad) atomicOr(&f->err, bad);
-}
-
-void launch_src_sorted(hipStream_t s, const int32_t *src, const int32_t *dst, int64_t e,
-                       int32_t *self, DevFlags *f)
-{
-    if (e > 0)
-        hipLaunchKernelGGL(k_src_sorted, grid_for(e, 256, 1 << 20), 256, 0, s, src, dst, e, self, f);
-}
-
-void launch_csr_sorted(hipStream_t s, int stage, const int32_t *src, const int32_t *dst,
-                       const int64_t *w, const int32_t *slscan, int64_t e, int32_t n, int wbits,
-                       int64_t *fs, uint32_t *keys, uint64_t *payload)
-{
-    if (stage == 0) {
-        hipLaunchKernelGGL(k_src_starts, grid_for((int64_t)n + 1, 256, 1 << 20), 256, 0, s, src, e,
-                           n, fs);
-        if (e > 0)
-            hipLaunchKernelGGL(k_rev_entries, grid_for(e, 256, 1 << 20), 256, 0, s, src, dst, w,
-                               slscan, e, wbits, keys, payload);
+    if constexpr (CHECK) {
+        bad = wave_or_i(bad);
+        if ((threadIdx.x & 63) == 0 && bad) atomicOr(&f->err, bad);
     }
 }
 
-void launch_csr_sorted_fill(hipStream_t s, const int32_t *src, const int32_t *dst,
+void launch_src_sorted(hipStream_t s, const int32_t *src, const int32_t *dst, const int64_t *w,
+                       int64_t e, int32_t n, int32_t *self, int64_t *fs, int64_t *kdeg,
+                       int64_t *selfsum, ull *lastloop, DevFlags *f)
+{
+    if (e > 0) {
+        hipLaunchKernelGGL(k_src_sorted, grid_for(e, 256, 1 << 20), 256, 0, s, src, dst, w, e,
+                           self, fs, kdeg, selfsum, lastloop, f);
+        hipLaunchKernelGGL(k_range_ends<int32_t>, grid_for((int64_t)n + 1, 256, 1 << 20), 256, 0,
+                           s, src, e, n, fs);
+    } else {
+        fill_i64(s, fs, (int64_t)n + 1, 0);
+    }
+}
+
+void launch_rev_entries(hipStream_t s, const int32_t *src, const int32_t *dst, const int64_t *w,
+                        const int32_t *slscan, int64_t e, int wbits, uint32_t *keys,
+                        uint64_t *payload)
+{
+    if (e > 0)
+        hipLaunchKernelGGL(k_rev_entries, grid_for(e, 256, 1 << 20), 256, 0, s, src, dst, w,
+                           slscan, e, wbits, keys, payload);
+}
+
+void launch_csr_sorted_fill(hipStream_t s, bool check, const int32_t *src, const int32_t *dst,
                             const int64_t *w, const int32_t *slscan, int64_t e, int32_t n,
                             int wbits, const int64_t *fs, const uint32_t *skey,
                             const uint64_t *spay, int64_t nr, const int64_t *rs, int64_t *rowptr,
-                            int32_t *col, int64_t *cw, int64_t *kdeg, DevFlags *f)
+                            int64_t *ffwd, int32_t *col, int64_t *cw, int64_t *kdeg, DevFlags *f)
 {
     hipLaunchKernelGGL(k_rowptr_sum, grid_for((int64_t)n + 1, 256, 1 << 20), 256, 0, s, rs, fs,
-                       slscan, n, rowptr);
-    if (nr > 0)
-        hipLaunchKernelGGL(k_fill_rev, grid_for(nr, 256, 1 << 20), 256, 0, s, skey, spay, nr, wbits,
-                           rs, rowptr, col, cw, kdeg, f);
-    if (e > 0)
-        hipLaunchKernelGGL(k_fill_fwd, grid_for(e, 256, 1 << 20), 256, 0, s, src, dst, w, slscan,
-                           e, fs, rs, spay, wbits, rowptr, col, cw, kdeg, f);
+                       slscan, n, rowptr, ffwd);
+    if (nr > 0) {
+        if (check)
+            hipLaunchKernelGGL(k_fill_rev<true>, grid_for(nr, 256, 1 << 20), 256, 0, s, skey, spay,
+                               nr, wbits, rs, rowptr, col, cw, kdeg, f);
+        else
+            hipLaunchKernelGGL(k_fill_rev<false>, grid_for(nr, 256, 1 << 20), 256, 0, s, skey, spay,
+                               nr, wbits, rs, rowptr, col, cw, kdeg, f);
+    }
+    if (e > 0) {
+        if (check)
+            hipLaunchKernelGGL(k_fill_fwd<true>, grid_for(e, 256, 1 << 20), 256, 0, s, src, dst, w,
+                               slscan, e, fs, rs, spay, wbits, ffwd, col, cw, kdeg, f);
+        else
+            hipLaunchKernelGGL(k_fill_fwd<false>, grid_for(e, 256, 1 << 20), 256, 0, s, src, dst, w,
+                               slscan, e, fs, rs, spay, wbits, ffwd, col, cw, kdeg, f);
+    }
 }
 
-// rowptr[v] = first position of a key >= v in the sorted keys (v in [0, n]).
-__global__ void k_rowptr_search(const uint32_t *skeys, int64_t total, int32_t n, int64_t *rowptr)
+// out[v] = the first position r with min(keys[r], n) >= v (v in [0, n]) of
+// ascending keys, between the first and the last key: each change of key
+// writes the rows it passes (k_range_ends writes the two ends).
+__global__ void k_row_starts(const uint32_t *keys, int64_t total, int32_t n, int64_t *out)
 {
-    GRID_STRIDE(v, (int64_t)n + 1)
-    {
-        int64_t lo = 0, hi = total;
-        while (lo < hi) {
-            int64_t mid = (lo + hi) >> 1;
-            if (skeys[mid] < (uint32_t)v) lo = mid + 1; else hi = mid;
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < total;
+         base += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = base + threadIdx.x;
+        long long lo = 0, hi = 0;
+        if (r < total && r > 0) {
+            lo = min((int64_t)keys[r - 1], (int64_t)n);
+            hi = min((int64_t)keys[r], (int64_t)n);
         }
-        rowptr[v] = lo;
+        wave_range_fill(lo, hi, (long long)r, out);
     }
 }
 
@@ -497,8 +578,13 @@ void launch_self_loops(hipStream_t s, const int32_t *src, const int32_t *dst, co
 void launch_rowptr_search(hipStream_t s, const uint32_t *skeys, int64_t total, int32_t n,
                           int64_t *rowptr)
 {
-    hipLaunchKernelGGL(k_rowptr_search, grid_for((int64_t)n + 1, 256, 1 << 20), 256, 0, s, skeys,
-                       total, n, rowptr);
+    if (total > 0) {
+        hipLaunchKernelGGL(k_row_starts, grid_for(total, 256, 1 << 20), 256, 0, s, skeys, total, n,
+                           rowptr);
+        hipLaunchKernelGGL(k_range_ends<uint32_t>, grid_for((int64_t)n + 1, 256, 1 << 20), 256, 0,
+                           s, skeys, total, n, rowptr);
+    } else
+        fill_i64(s, rowptr, (int64_t)n + 1, 0);
 }
 
 void launch_csr_gather(hipStream_t s, const uint32_t *skeys, const uint32_t *scodes, int64_t nnz,
@@ -534,23 +620,28 @@ __global__ __launch_bounds__(256) void k_init_vertex(
     }
     __syncthreads();
     ull kt = 0;
-    GRID_STRIDE(i, n)
-    {
-        const int64_t k = kdeg[i];
-        kt = sat_add(kt, (ull)k);
-        deg[i] = k;
-        in[i] = loopw[i];
-        size[i] = 1;
-        ddeg[i] = 0;
-        dsize[i] = 0;
-        C[i] = (int32_t)i;
-        next[i] = (int32_t)i;
-        int64_t d = rowptr[i + 1] - rowptr[i];
-        int t = tier_of(d, d + (selfsum[i] > 0 ? 1 : 0));
-        tier_key[i] = (uint32_t)t;
-        verts[i] = (int32_t)i;
-        atomicAdd((ull *)&s_cnt[t], 1ull);
-        atomicAdd((ull *)&s_nnz[t], (ull)d);
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n;
+         base += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = base + threadIdx.x;
+        const bool valid = i < n;
+        int t = 0;
+        int64_t d = 0;
+        if (valid) {
+            const int64_t k = kdeg[i];
+            kt = sat_add(kt, (ull)k);
+            deg[i] = k;
+            in[i] = loopw[i];
+            size[i] = 1;
+            ddeg[i] = 0;
+            dsize[i] = 0;
+            C[i] = (int32_t)i;
+            next[i] = (int32_t)i;
+            d = rowptr[i + 1] - rowptr[i];
+            t = tier_of(d, d + (selfsum[i] > 0 ? 1 : 0));
+            tier_key[i] = (uint32_t)t;
+            verts[i] = (int32_t)i;
+        }
+        wave_bucket_add(valid, t, d, s_cnt, s_nnz);  // one LDS atomic pair per tier and wave
     }
     kt = wave_sum_sat(kt);
     if ((threadIdx.x & 63) == 0) s_k[threadIdx.x >> 6] = kt;
@@ -1276,7 +1367,7 @@ __global__ void k_rows_check(const uint32_t *skeys, const int32_t *col, int64_t 
     {
         if (p > 0 && skeys[p] == skeys[p - 1] && col[p] <= col[p - 1]) bad = 4;
     }
-    bad = wave_max_i(bad);
+    bad = wave_or_i(bad);
     if ((threadIdx.x & 63) == 0 && bad) atomicOr(&f->err, bad);
 }
 
@@ -1652,12 +1743,12 @@ __global__ void k_layout_first(const uint32_t *skey, int32_t n, int32_t ncomm, i
     {
         const uint32_t c = skey[j];
         if (c >= (uint32_t)ncomm) {
-            bad = 1;
+            bad |= 1;
             continue;
         }
         if (j == 0 || skey[j - 1] != c) first[c] = (int32_t)j;
     }
-    bad = wave_max_i(bad);
+    bad = wave_or_i(bad);
     if ((threadIdx.x & 63) == 0 && bad) atomicOr(&f->err, 1);
 }
 

@@ -190,6 +190,21 @@ def test_arbitrary_ids_float_weights_and_loops():
         assert np.array_equal(node, onode) and np.array_equal(comm, ocomm)
 
 
+@pytest.mark.parametrize("gamma", [0.5, 1.0, 2.0])
+def test_unsorted_list_with_repeated_head(gamma):
+    """One wavefront sees both order faults of the edge list: a repeated pair
+    at the head (same source, destination not increasing) and a later edge
+    whose source decreases.  Both must reach the flags so the list takes the
+    full row sort."""
+    gpu_case("unsorted list with repeated head, gamma %g" % gamma)
+    src = np.array([3, 3, 0, 0, 3, 3, 0, 1], np.int32)
+    dst = np.array([3, 1, 1, 3, 1, 1, 2, 2], np.int32)
+    w = np.array([1, 1, 2, 2, 2, 2, 2, 1], np.int32)
+    eng, r = run_both(4, src, dst, w, gamma=gamma)
+    with eng:
+        assert_run_matches(eng, r, 4)
+
+
 def test_wide_weight_units():
     """Weights up to 2^18 next to 2^-20 make the weight unit 2^-20, so the
     total degree is about 2^57 units: with n = 2^15 the per-community size
