@@ -11,7 +11,7 @@
 // community's row entries are gathered contiguously in member order as
 // (newc[target], w' = w==0?1:w); each community segment is then aggregated by
 // neighbour community - in one LDS table for segments of up to 2048 entries, or
-// by hash partitions of about 1024 entries for larger ones - which gives row x
+// by hash partitions of about 2048 entries for larger ones - which gives row x
 // of the next level: one entry (y, E'(x,y) + E'(y,x)) per neighbour community
 // y != x.  Entries with y == x and the members' self-loops form the next
 // level's self-loop E'(x,x) = internal/2 + sum(selfsum).
