@@ -89,6 +89,7 @@ struct lv_ctx {
     // expected entries per hub partition; the LDS table of a partition holds 4096
     // distinct communities (more fall back to a global table)
     int32_t hub_part_target = kHubPartTarget;
+    int chunk_mul = kChunkMul;  // chunk_span(P, chunk_mul); LV_CHUNK_MUL overrides
     // groups of the last renumbering (sorted order): head flags, their exclusive
     // scan, first sorted index of each group, new id of each group
     int32_t *g_head = nullptr, *g_hscan = nullptr, *g_start = nullptr, *g_newid = nullptr;
@@ -547,7 +548,9 @@ int plan_hubs(lv_ctx *c, int32_t nh)
     LV_TRY(arena_alloc(A, ebase, (int64_t)nh + 1));
     LV_TRY(arena_alloc(A, pbase, (int64_t)nh + 1));
     LV_TRY(arena_alloc(A, cbase, (int64_t)nh + 1));
-    launch_hub_setup(s, hv, nh, c->rowptr, c->selfsum, c->hub_part_target, hq, hP, hP64, hch);
+    launch_hub_setup(s, hv, nh, c->rowptr, c->selfsum, c->hub_part_target, c->chunk_mul, hq, hP,
+                     hP64, hch);
+    h.chunk_mul = c->chunk_mul;
     LV_TRY(scan_exclusive_i64(A, s, hq, ebase, nh, ebase + nh));
     LV_TRY(scan_exclusive_i64(A, s, hP64, pbase, nh, pbase + nh));
     LV_TRY(scan_exclusive_i64(A, s, hch, cbase, nh, cbase + nh));
@@ -904,7 +907,9 @@ int induce(lv_ctx *c, const int32_t *newc, const int32_t *newid_g, int32_t Kn)
         LV_TRY(arena_alloc(A, blscan, (int64_t)nbs + 1));
         LV_TRY(arena_alloc(A, pbase, (int64_t)nbs + 1));
         LV_TRY(arena_alloc(A, cbase, (int64_t)nbs + 1));
-        launch_cr_big_setup(s, bsegs, nbs, slen, c->hub_part_target, bP, bP64, bch, blen64);
+        launch_cr_big_setup(s, bsegs, nbs, slen, c->hub_part_target, c->chunk_mul, bP, bP64, bch,
+                            blen64);
+        B.chunk_mul = c->chunk_mul;
         LV_TRY(scan_exclusive_i64(A, s, bP64, pbase, nbs, pbase + nbs));
         LV_TRY(scan_exclusive_i64(A, s, bch, cbase, nbs, cbase + nbs));
         LV_TRY(scan_exclusive_i64(A, s, blen64, blscan, nbs, blscan + nbs));
@@ -1372,6 +1377,10 @@ int lv_create(lv_ctx **out, const lv_config *cfg)
     if (const char *e = getenv("LV_HUB_PART_TARGET")) {
         long t = strtol(e, nullptr, 10);
         if (t >= 1 && t <= (1 << 30)) c->hub_part_target = (int32_t)t;
+    }
+    if (const char *e = getenv("LV_CHUNK_MUL")) {
+        long t = strtol(e, nullptr, 10);
+        if (t >= 0 && t <= 1024) c->chunk_mul = (int)t;
     }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= c->device || c->device < 0) {

@@ -437,8 +437,8 @@ void launch_cr_agg_tier(hipStream_t s, int tier, const CoarsenArgs &a, const int
 // distinct keys); uniques land at poff[p] of buy/buw.
 
 __global__ void k_cr_big_setup(const int32_t *bsegs, int32_t nbs, const int64_t *slen,
-                               int32_t target, int32_t *bP, int64_t *bP64, int64_t *bch,
-                               int64_t *blen)
+                               int32_t target, int chunk_mul, int32_t *bP, int64_t *bP64,
+                               int64_t *bch, int64_t *blen)
 {
     GRID_STRIDE(j, nbs)
     {
@@ -446,7 +446,7 @@ __global__ void k_cr_big_setup(const int32_t *bsegs, int32_t nbs, const int64_t 
         int64_t P = (L + target - 1) / target;
         if (P < 1) P = 1;
         if (P > kHubMaxParts) P = kHubMaxParts;
-        const int64_t cs = chunk_span((int)P);
+        const int64_t cs = chunk_span((int)P, chunk_mul);
         bP[j] = (int32_t)P;
         bP64[j] = P;
         bch[j] = (L + cs - 1) / cs;
@@ -466,7 +466,7 @@ __global__ __launch_bounds__(256) void k_cr_big_count(CoarsenArgs a, BigSegArgs 
     const int64_t b = a.sb[g], L = a.slen[g];
     for (int t = threadIdx.x; t < P; t += 256) hist[t] = 0;
     __syncthreads();
-    const int64_t cs = chunk_span(P);
+    const int64_t cs = chunk_span(P, B.chunk_mul);
     const int64_t t0 = (int64_t)B.chunk_k[ci] * cs;
     const int64_t t1 = min(L, t0 + cs);
     ull inter = 0, tot = 0;
@@ -508,7 +508,7 @@ __global__ __launch_bounds__(256) void k_cr_big_scatter(CoarsenArgs a, BigSegArg
     const int64_t b = a.sb[g], L = a.slen[g];
     for (int t = threadIdx.x; t < P; t += 256) hist[t] = 0;
     __syncthreads();
-    const int64_t cs = chunk_span(P);
+    const int64_t cs = chunk_span(P, B.chunk_mul);
     const int64_t t0 = (int64_t)B.chunk_k[ci] * cs;
     const int64_t t1 = min(L, t0 + cs);
     for (int64_t t = t0 + threadIdx.x; t < t1; t += 256) {
@@ -606,11 +606,12 @@ __global__ void k_cr_big_prefix(CoarsenArgs a, BigSegArgs B, const int64_t *ucsc
 }
 
 void launch_cr_big_setup(hipStream_t s, const int32_t *bsegs, int32_t nbs, const int64_t *slen,
-                         int32_t target, int32_t *bP, int64_t *bP64, int64_t *bch, int64_t *blen)
+                         int32_t target, int chunk_mul, int32_t *bP, int64_t *bP64, int64_t *bch,
+                         int64_t *blen)
 {
     if (nbs > 0)
         hipLaunchKernelGGL(k_cr_big_setup, grid_for(nbs, 256), 256, 0, s, bsegs, nbs, slen, target,
-                           bP, bP64, bch, blen);
+                           chunk_mul, bP, bP64, bch, blen);
 }
 
 int run_cr_big(Arena &A, hipStream_t s, const CoarsenArgs &a, const BigSegArgs &B, int32_t nbs,

@@ -307,10 +307,14 @@ __device__ __forceinline__ int hub_part(int32_t c, int P)
     return (int)(((uint64_t)fmix32((uint32_t)c) * (uint64_t)P) >> 32);
 }
 
-// Entries per chunk of a row or segment cut into P hash partitions.  (Larger
-// chunks for large P make longer write runs per partition but fewer, longer
-// blocks; on the R-MAT workloads the short chunks measured faster.)
-__host__ __device__ inline int64_t chunk_span(int) { return kHubChunk; }
+// Entries per chunk of a row or segment cut into P hash partitions: at least
+// kHubChunk, and mul entries per partition when mul > 0 (longer write runs per
+// partition for large P, fewer and longer blocks).
+__host__ __device__ inline int64_t chunk_span(int P, int mul)
+{
+    const int64_t c = (int64_t)P * mul;
+    return c > kHubChunk ? c : kHubChunk;
+}
 
 // Insert with at most 2^log2cap probes; false when the table is full.
 __device__ __forceinline__ bool lds_insert_bounded(int32_t *key, ull *val, int log2cap, int32_t c,

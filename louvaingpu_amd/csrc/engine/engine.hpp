@@ -148,6 +148,7 @@ struct SweepArgs {
 // P_j = ceil(q_j / kHubPartTarget) partitions, aggregates each partition in an
 // LDS table, then reduces the partitions of a hub.
 constexpr int kHubChunk = 4096;
+constexpr int kChunkMul = 8;  // entries per partition of a chunk (0: kHubChunk fixed)
 constexpr int kHubPartTarget = 2048;  // default; LV_HUB_PART_TARGET overrides
 constexpr int kHubMaxParts = 2048;
 constexpr int kHubAggLog2 = 12;  // 4096 LDS slots per partition block
@@ -173,6 +174,7 @@ struct HubArgs {
     int64_t *pw;
     int32_t *pn;
     const int64_t *cbase;     // first chunk of hub j (exclusive scan of chunks)
+    int chunk_mul;            // chunk_span(P, chunk_mul) entries per chunk
     double *cg;               // per-chunk best of the first-pass sweep
     int32_t *cc;
     int64_t *cw;
@@ -211,6 +213,7 @@ struct BigSegArgs {
     ull *fbv;
     int32_t *ucnt;          // uniques per partition
     int64_t *pre;           // prefix of ucnt within the segment
+    int chunk_mul;          // chunk_span(P, chunk_mul) entries per chunk
 };
 
 // ---------------------------------------------------------------------------
@@ -323,8 +326,8 @@ void launch_csr_sorted_fill(hipStream_t s, bool check, const int32_t *src, const
 void launch_rows_check(hipStream_t s, const uint32_t *skeys, const int32_t *col, int64_t nnz,
                        DevFlags *f);
 void launch_hub_setup(hipStream_t s, const int32_t *hv, int32_t nh, const int64_t *rowptr,
-                      const int64_t *selfsum, int32_t target, int64_t *hq, int32_t *hP,
-                      int64_t *hP64, int64_t *hch);
+                      const int64_t *selfsum, int32_t target, int chunk_mul, int64_t *hq,
+                      int32_t *hP, int64_t *hP64, int64_t *hch);
 // owner[i] = segment j of i under exclusive offsets base[0..nseg]; rank[i] =
 // i - base[j] (rank may be null).
 void launch_expand_owner(hipStream_t s, const int64_t *base, int32_t nseg, int64_t total,
@@ -377,7 +380,8 @@ void launch_cr_seg_tier(hipStream_t s, const int32_t *gstart, const int64_t *est
                         uint32_t *tkey, int32_t *sid, long long *tier_cnt);
 void launch_cr_agg_tier(hipStream_t s, int tier, const CoarsenArgs &a, const int32_t *segs, int32_t ns);
 void launch_cr_big_setup(hipStream_t s, const int32_t *bsegs, int32_t nbs, const int64_t *slen,
-                         int32_t target, int32_t *bP, int64_t *bP64, int64_t *bch, int64_t *blen);
+                         int32_t target, int chunk_mul, int32_t *bP, int64_t *bP64, int64_t *bch,
+                         int64_t *blen);
 int run_cr_big(Arena &A, hipStream_t s, const CoarsenArgs &a, const BigSegArgs &B, int32_t nbs,
                int64_t nparts, int64_t nchunks);
 void launch_cr_rowcnt(hipStream_t s, const int32_t *newid_g, const int32_t *nuniq, int32_t K,

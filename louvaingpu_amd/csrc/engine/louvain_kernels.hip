@@ -1263,7 +1263,7 @@ __global__ __launch_bounds__(256) void k_hub1_chunk(SweepArgs a, HubArgs h)
     const int64_t rb = a.rowptr[v], d = a.rowptr[v + 1] - rb;
     const VParam vp = vparam<B>(a, v);
     const double f = (double)a.kdeg[v] / vp.m2;
-    const int64_t cs = chunk_span(h.hP[j]);
+    const int64_t cs = chunk_span(h.hP[j], h.chunk_mul);
     const int64_t t0 = (int64_t)h.chunk_k[ci] * cs;
     const int64_t t1 = vp.mode == 0 ? t0 : min(q, t0 + cs);
     int cnt = 0;
@@ -1383,8 +1383,8 @@ void launch_rows_check(hipStream_t s, const uint32_t *skeys, const int32_t *col,
 // table per hub, so the work is streaming reads/writes plus LDS atomics.
 
 __global__ void k_hub_setup(const int32_t *hv, int32_t nh, const int64_t *rowptr,
-                            const int64_t *selfsum, int32_t target, int64_t *hq, int32_t *hP,
-                            int64_t *hP64, int64_t *hch)
+                            const int64_t *selfsum, int32_t target, int chunk_mul, int64_t *hq,
+                            int32_t *hP, int64_t *hP64, int64_t *hch)
 {
     GRID_STRIDE(j, nh)
     {
@@ -1396,7 +1396,7 @@ __global__ void k_hub_setup(const int32_t *hv, int32_t nh, const int64_t *rowptr
         hq[j] = q;
         hP[j] = (int32_t)P;
         hP64[j] = P;
-        const int64_t cs = chunk_span((int)P);
+        const int64_t cs = chunk_span((int)P, chunk_mul);
         hch[j] = (q + cs - 1) / cs;
     }
 }
@@ -1436,12 +1436,12 @@ void launch_widen_i32(hipStream_t s, const int32_t *in, int64_t n, int64_t *out)
 }
 
 void launch_hub_setup(hipStream_t s, const int32_t *hv, int32_t nh, const int64_t *rowptr,
-                      const int64_t *selfsum, int32_t target, int64_t *hq, int32_t *hP,
-                      int64_t *hP64, int64_t *hch)
+                      const int64_t *selfsum, int32_t target, int chunk_mul, int64_t *hq,
+                      int32_t *hP, int64_t *hP64, int64_t *hch)
 {
     if (nh > 0)
         hipLaunchKernelGGL(k_hub_setup, grid_for(nh, 256), 256, 0, s, hv, nh, rowptr, selfsum,
-                           target, hq, hP, hP64, hch);
+                           target, chunk_mul, hq, hP, hP64, hch);
 }
 
 // Entry t of hub j: t < d is row entry rowptr[v]+t, t == d the virtual
@@ -1458,7 +1458,7 @@ __global__ __launch_bounds__(256) void k_hub_count(SweepArgs a, HubArgs h)
     const int32_t own = a.C[v];
     for (int t = threadIdx.x; t < P; t += 256) hist[t] = 0;
     __syncthreads();
-    const int64_t cs = chunk_span(P);
+    const int64_t cs = chunk_span(P, h.chunk_mul);
     const int64_t t0 = (int64_t)h.chunk_k[ci] * cs;
     const int64_t t1 = min(q, t0 + cs);
     int32_t *tc = h.tmp_c + h.ebase[j];
@@ -1484,7 +1484,7 @@ __global__ __launch_bounds__(256) void k_hub_scatter(SweepArgs a, HubArgs h)
     const int64_t rb = a.rowptr[v], d = a.rowptr[v + 1] - rb;
     for (int t = threadIdx.x; t < P; t += 256) hist[t] = 0;
     __syncthreads();
-    const int64_t cs = chunk_span(P);
+    const int64_t cs = chunk_span(P, h.chunk_mul);
     const int64_t t0 = (int64_t)h.chunk_k[ci] * cs;
     const int64_t t1 = min(q, t0 + cs);
     const int32_t *tc = h.tmp_c + h.ebase[j];
