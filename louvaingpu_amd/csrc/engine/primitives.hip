@@ -466,8 +466,10 @@ static int radix_sort(Arena &a, hipStream_t s, K *keys, V *vals, K *ktmp, V *vtm
     *kout = keys;
     *vout = vals;
     if (n <= 1 || bits <= 0) return LV_OK;
-    int rounds = kRadixSub;  // at least one full round per block
-    while ((n + (int64_t)rounds * kRadixBlock - 1) / ((int64_t)rounds * kRadixBlock) > 32768)
+    // at least one full round per block, at most 8192 blocks: the digit-major
+    // histogram (256 x blocks) and its scan stay small next to the data
+    int rounds = kRadixSub;
+    while ((n + (int64_t)rounds * kRadixBlock - 1) / ((int64_t)rounds * kRadixBlock) > 8192)
         rounds *= 2;
     int64_t nblocks = (n + (int64_t)rounds * kRadixBlock - 1) / ((int64_t)rounds * kRadixBlock);
     int64_t *hist = nullptr;
