@@ -310,6 +310,22 @@ def test_hub_partitions():
         assert_run_matches(eng, r, n)
 
 
+def test_star_with_long_chunks():
+    """A hub with 1.2 M neighbours (P = 586 hash partitions) and the largest
+    community of level 0 (2.25 M entries, P = 1098): both partition pipelines
+    cut chunks of 8 entries per partition, longer than the 4096-entry minimum."""
+    gpu_case("star with long chunks")
+    rng = np.random.Generator(np.random.PCG64(17))
+    leaves = 1_200_000
+    n = leaves + 1
+    src = np.concatenate([np.zeros(leaves, np.int64), rng.integers(1, n, 100_000)])
+    dst = np.concatenate([np.arange(1, n), rng.integers(1, n, 100_000)])
+    w = rng.integers(1, 4, len(src)).astype(np.int32)
+    eng, r = run_both(n, src.astype(np.int32), dst.astype(np.int32), w)
+    with eng:
+        assert_run_matches(eng, r, n, trace=False)
+
+
 def test_hub_global_fallback(monkeypatch):
     """One partition per hub holds ~9000 distinct communities, more than the
     4096-slot LDS table: the partition is redone in its global table."""
