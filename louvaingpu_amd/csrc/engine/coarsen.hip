@@ -273,7 +273,7 @@ void launch_cr_seg_tier(hipStream_t s, const int32_t *gstart, const int64_t *est
                         uint32_t *tkey, int32_t *sid, long long *tier_cnt)
 {
     if (K > 0)
-        hipLaunchKernelGGL(k_cr_seg_tier, grid_for(K, 256, 8192), 256, 0, s, gstart, estart,
+        hipLaunchKernelGGL(k_cr_seg_tier, grid_for(K, 256, 1024), 256, 0, s, gstart, estart,
                            newid_g, K, sb, slen, tkey, sid, tier_cnt);
 }
 

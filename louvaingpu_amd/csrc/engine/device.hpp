@@ -130,6 +130,21 @@ __device__ __forceinline__ void block_units_flush(ull tot, ull msum, ull umax, i
     }
 }
 
+// End of a kernel that raises error flags: the block's flags reach the device
+// word with one atomic (all threads of the block call it).  A flag such as
+// "rows not increasing" can be set in every wavefront of a big launch, where
+// per-wave atomics on the one word would serialise.
+__device__ __forceinline__ void block_flag_or(int bad, DevFlags *f)
+{
+    __shared__ int s_or;
+    if (threadIdx.x == 0) s_or = 0;
+    __syncthreads();
+    bad = wave_or_i(bad);
+    if ((threadIdx.x & 63) == 0 && bad) atomicOr(&s_or, bad);
+    __syncthreads();
+    if (threadIdx.x == 0 && s_or) atomicOr(&f->err, s_or);
+}
+
 // 128-bit accumulate of deg^2 as (lo, hi).
 __device__ __forceinline__ void acc_sq(ull &lo, ull &hi, ull d)
 {

@@ -321,8 +321,7 @@ __global__ void k_src_sorted(const int32_t *src, const int32_t *dst, const int64
         }
         wave_range_fill(lo, hi, (long long)i, fs);
     }
-    bad = wave_or_i(bad);
-    if ((threadIdx.x & 63) == 0 && bad) atomicOr(&f->err, bad);
+    block_flag_or(bad, f);
 }
 
 // Reverse entries of non-self edges, compacted: key dst, payload src << wbits | w.
@@ -387,10 +386,7 @@ __global__ __launch_bounds__(256) void k_fill_rev(const uint32_t *skey, const ui
         }
         seg_atomic_add(valid, row, wp, (ull *)kdeg);
     }
-    if constexpr (CHECK) {
-        bad = wave_or_i(bad);
-        if ((threadIdx.x & 63) == 0 && bad) atomicOr(&f->err, bad);
-    }
+    if constexpr (CHECK) block_flag_or(bad, f);
 }
 
 template <bool CHECK>
@@ -434,10 +430,7 @@ __global__ __launch_bounds__(256) void k_fill_fwd(const int32_t *src, const int3
         }
         seg_atomic_add(valid, row, wp, (ull *)kdeg);
     }
-    if constexpr (CHECK) {
-        bad = wave_or_i(bad);
-        if ((threadIdx.x & 63) == 0 && bad) atomicOr(&f->err, bad);
-    }
+    if constexpr (CHECK) block_flag_or(bad, f);
 }
 
 void launch_src_sorted(hipStream_t s, const int32_t *src, const int32_t *dst, const int64_t *w,
@@ -445,7 +438,7 @@ void launch_src_sorted(hipStream_t s, const int32_t *src, const int32_t *dst, co
                        int64_t *selfsum, ull *lastloop, DevFlags *f)
 {
     if (e > 0) {
-        hipLaunchKernelGGL(k_src_sorted, grid_for(e, 256, 1 << 20), 256, 0, s, src, dst, w, e,
+        hipLaunchKernelGGL(k_src_sorted, grid_for(e, 256, 8192), 256, 0, s, src, dst, w, e,
                            self, fs, kdeg, selfsum, lastloop, f);
         hipLaunchKernelGGL(k_range_ends<int32_t>, grid_for((int64_t)n + 1, 256, 1 << 20), 256, 0,
                            s, src, e, n, fs);
@@ -473,7 +466,7 @@ void launch_csr_sorted_fill(hipStream_t s, bool check, const int32_t *src, const
                        slscan, n, rowptr, ffwd);
     if (nr > 0) {
         if (check)
-            hipLaunchKernelGGL(k_fill_rev<true>, grid_for(nr, 256, 1 << 20), 256, 0, s, skey, spay,
+            hipLaunchKernelGGL(k_fill_rev<true>, grid_for(nr, 256, 8192), 256, 0, s, skey, spay,
                                nr, wbits, rs, rowptr, col, cw, kdeg, f);
         else
             hipLaunchKernelGGL(k_fill_rev<false>, grid_for(nr, 256, 1 << 20), 256, 0, s, skey, spay,
@@ -481,7 +474,7 @@ void launch_csr_sorted_fill(hipStream_t s, bool check, const int32_t *src, const
     }
     if (e > 0) {
         if (check)
-            hipLaunchKernelGGL(k_fill_fwd<true>, grid_for(e, 256, 1 << 20), 256, 0, s, src, dst, w,
+            hipLaunchKernelGGL(k_fill_fwd<true>, grid_for(e, 256, 8192), 256, 0, s, src, dst, w,
                                slscan, e, fs, rs, spay, wbits, ffwd, col, cw, kdeg, f);
         else
             hipLaunchKernelGGL(k_fill_fwd<false>, grid_for(e, 256, 1 << 20), 256, 0, s, src, dst, w,
@@ -663,7 +656,7 @@ void launch_init_vertex(hipStream_t s, int32_t n, const int64_t *rowptr, const i
                         long long *tier_nnz, ull *ksum)
 {
     if (n > 0)
-        hipLaunchKernelGGL(k_init_vertex, grid_for(n, 256, 8192), 256, 0, s, n, rowptr, kdeg,
+        hipLaunchKernelGGL(k_init_vertex, grid_for(n, 256, 1024), 256, 0, s, n, rowptr, kdeg,
                            loopw, selfsum, deg, in, size, ddeg, dsize, C, next, tier_key, verts,
                            tier_cnt, tier_nnz, ksum);
 }
@@ -1367,15 +1360,14 @@ __global__ void k_rows_check(const uint32_t *skeys, const int32_t *col, int64_t 
     {
         if (p > 0 && skeys[p] == skeys[p - 1] && col[p] <= col[p - 1]) bad = 4;
     }
-    bad = wave_or_i(bad);
-    if ((threadIdx.x & 63) == 0 && bad) atomicOr(&f->err, bad);
+    block_flag_or(bad, f);
 }
 
 void launch_rows_check(hipStream_t s, const uint32_t *skeys, const int32_t *col, int64_t nnz,
                        DevFlags *f)
 {
     if (nnz > 1)
-        hipLaunchKernelGGL(k_rows_check, grid_for(nnz, 256, 1 << 20), 256, 0, s, skeys, col, nnz, f);
+        hipLaunchKernelGGL(k_rows_check, grid_for(nnz, 256, 8192), 256, 0, s, skeys, col, nnz, f);
 }
 
 // ---------------------------------------------------------------------------
