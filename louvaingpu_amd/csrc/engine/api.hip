@@ -718,6 +718,11 @@ int one_level(lv_ctx *c, double lower, double *q_out, int32_t *passes_out)
             t_end(c, t0, CAT_SWEEP, t, 12.0 * (double)c->tier_nnz[t] + 24.0 * (double)nv,
                   c->tier_nnz[t]);
         }
+        if (!last) {
+            hipEvent_t td = t_begin(c);
+            launch_move_deltas(s, a, c->n, pass == 1, false);  // pass 1: C[v] = v
+            t_end(c, td, CAT_APPLY);
+        }
         LV_HIP(hipGetLastError());
         if (trace) {
             std::vector<int32_t> nx((size_t)c->n);
@@ -1166,6 +1171,11 @@ int batch_one_level(lv_ctx *c, const std::vector<double> &lower, const std::vect
                                      c->hub_chunks, true));
             t_end(c, t0, CAT_SWEEP, t, 12.0 * (double)c->tier_nnz[t] + 24.0 * (double)nv,
                   c->tier_nnz[t]);
+        }
+        {
+            hipEvent_t td = t_begin(c);
+            launch_move_deltas(s, a, c->n, pass == 1, true);  // pass 1: C[v] = v
+            t_end(c, td, CAT_APPLY);
         }
         LV_HIP(hipGetLastError());
         hipEvent_t t1 = t_begin(c);

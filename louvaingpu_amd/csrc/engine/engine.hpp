@@ -301,6 +301,9 @@ void launch_lab(hipStream_t s, int32_t n, const int32_t *C, const int32_t *size,
                 const int64_t *deg, Lab *lab);
 void launch_apply(hipStream_t s, int32_t n, int64_t *deg, const int64_t *in, int32_t *size,
                   ull *ddeg, int32_t *dsize, int size_shift, QAcc *part, QAcc *acc);
+// Move deltas of a pass that is not a last pass, from C and next (own_direct:
+// the first pass of a level, where every community left is a singleton).
+void launch_move_deltas(hipStream_t s, const SweepArgs &a, int32_t n, bool own_direct, bool batch);
 void launch_sweep_tier(hipStream_t s, int tier, const SweepArgs &a, const int32_t *verts,
                        int32_t nv, bool batch);
 // First pass of a level on rows that hold each neighbour once: every community

@@ -857,7 +857,9 @@ __device__ __forceinline__ void group_reduce(int &cnt, double &bg, int32_t &bc, 
 
 // Guard, move and deltas of one vertex (oneLevelKernel + remove/insert,
 // OneLevelKernel.cu:52-78,400-421); on the last pass the deltas are the
-// merge-isolated ones only (:109-171,921-927).
+// merge-isolated ones only (:109-171,921-927).  The deltas of an ordinary
+// pass are formed afterwards from C and next by k_move_deltas; the last pass
+// adds its (rare) merge-isolated deltas here.
 __device__ __forceinline__ void finalize_vertex(const SweepArgs &a, int32_t v, int32_t own,
                                                 int32_t so, int n, int32_t bc, int64_t bw, int mode)
 {
@@ -868,9 +870,12 @@ __device__ __forceinline__ void finalize_vertex(const SweepArgs &a, int32_t v, i
     const int32_t c0 = bc != INT_MAX ? bc : own;
     int32_t choice = c0;
     if (so == 1 && a.size[c0] == 1 && c0 > own) choice = own;
-    int32_t to = choice;  // mode 2: only the merge-isolated move
-    if (mode == 2) to = (n == 1 && so == 1 && c0 != own && own < c0) ? c0 : own;
-    a.next[v] = mode == 2 && to == own ? choice : to;
+    if (mode == 1) {
+        a.next[v] = choice;
+        return;
+    }
+    const int32_t to = (n == 1 && so == 1 && c0 != own && own < c0) ? c0 : own;
+    a.next[v] = to != own ? to : choice;
     if (to == own) return;
     const int64_t k = a.kdeg[v];
     if (a.size_shift) {  // one atomic per side: delta_deg * 2^shift + delta_size
@@ -883,6 +888,99 @@ __device__ __forceinline__ void finalize_vertex(const SweepArgs &a, int32_t v, i
         atomicAdd(&a.dsize[own], -1);
         atomicAdd(&a.dsize[to], 1);
     }
+}
+
+// Move deltas of an ordinary pass: every vertex whose next community differs
+// from its current one moves k and one member from C[v] to next[v].  A block
+// combines the deltas of its kMoveBlock vertices by community in LDS before
+// they reach the global words, so a community joined by 10^5 movers in one
+// pass takes one atomic per block instead of one per mover (same-address
+// device atomics serialise; a scattered stream of them runs about 40x faster
+// than the same number on one address).  A community that finds no slot
+// within kDeltaProbes probes adds directly.  The community left is a singleton
+// on the first pass of a level (own = v) and goes straight to global memory.
+constexpr int kMoveBlock = 2048;
+constexpr int kMoveLog2 = 12;
+constexpr int kDeltaProbes = 8;
+
+__device__ __forceinline__ void move_add(const SweepArgs &a, int32_t c, int64_t dk, int32_t ds)
+{
+    if (a.size_shift) {
+        atomicAdd(&a.ddeg[c], ((ull)dk << a.size_shift) + (ull)(int64_t)ds);
+    } else {
+        atomicAdd(&a.ddeg[c], (ull)dk);
+        atomicAdd(&a.dsize[c], ds);
+    }
+}
+
+__device__ __forceinline__ void move_combine(const SweepArgs &a, int32_t *key, ull *dk, int32_t *ds,
+                                             int32_t c, int64_t k, int32_t s)
+{
+    constexpr uint32_t mask = (1u << kMoveLog2) - 1u;
+    uint32_t sl = hslot(c, kMoveLog2);
+    for (int p = 0; p < kDeltaProbes; ++p) {
+        int32_t cur = ((volatile int32_t *)key)[sl];
+        if (cur == -1) {
+            cur = atomicCAS(&key[sl], -1, c);
+            if (cur == -1) cur = c;
+        }
+        if (cur == c) {
+            atomicAdd(&dk[sl], (ull)k);
+            atomicAdd(&ds[sl], s);
+            return;
+        }
+        sl = (sl + 1) & mask;
+    }
+    move_add(a, c, k, s);
+}
+
+template <bool B>
+__global__ __launch_bounds__(256) void k_move_deltas(SweepArgs a, int32_t n, int own_direct)
+{
+    constexpr int CAP = 1 << kMoveLog2;
+    __shared__ int32_t s_key[CAP];
+    __shared__ ull s_dk[CAP];
+    __shared__ int32_t s_ds[CAP];
+    for (int64_t base = (int64_t)blockIdx.x * kMoveBlock; base < n;
+         base += (int64_t)gridDim.x * kMoveBlock) {
+        for (int s = threadIdx.x; s < CAP; s += 256) {
+            s_key[s] = -1;
+            s_dk[s] = 0;
+            s_ds[s] = 0;
+        }
+        __syncthreads();
+        for (int u = 0; u < kMoveBlock / 256; ++u) {
+            const int64_t v = base + u * 256 + threadIdx.x;
+            if (v >= n) break;
+            const int32_t own = a.C[v], to = a.next[v];
+            if (to == own) continue;
+            if constexpr (B) {
+                if (a.cmode[a.vcomp[v]] != 1) continue;  // frozen, or last pass (done inline)
+            }
+            const int64_t k = a.kdeg[v];
+            if (own_direct)
+                move_add(a, own, -k, -1);
+            else
+                move_combine(a, s_key, s_dk, s_ds, own, -k, -1);
+            move_combine(a, s_key, s_dk, s_ds, to, k, 1);
+        }
+        __syncthreads();
+        for (int s = threadIdx.x; s < CAP; s += 256) {
+            const int32_t c = s_key[s];
+            if (c >= 0 && (s_dk[s] != 0 || s_ds[s] != 0)) move_add(a, c, (int64_t)s_dk[s], s_ds[s]);
+        }
+        __syncthreads();
+    }
+}
+
+void launch_move_deltas(hipStream_t s, const SweepArgs &a, int32_t n, bool own_direct, bool batch)
+{
+    if (n <= 0) return;
+    const int g = grid_for(n, kMoveBlock, 1 << 20);
+    if (batch)
+        hipLaunchKernelGGL(k_move_deltas<true>, g, 256, 0, s, a, n, own_direct ? 1 : 0);
+    else
+        hipLaunchKernelGGL(k_move_deltas<false>, g, 256, 0, s, a, n, own_direct ? 1 : 0);
 }
 
 // Inserts row entries [beg, end) with stride S starting at beg+lane: four
