@@ -237,6 +237,49 @@ __device__ __forceinline__ void wave_bucket_add(bool valid, int key, long long v
     }
 }
 
+// Running segmented sum of a wavefront over consecutive rounds of 64 keyed
+// values whose keys never decrease along the wavefront's span: each run of
+// the span reaches base[key] with one atomic (seg_atomic_add takes one per
+// run per round, so a row of 10^6 entries would take 1.6e4 atomics on one
+// address, where device atomics serialise).  All lanes call add() every round
+// and flush() after the span; invalid lanes only at the end of the data.
+constexpr int kSpanRounds = 8;
+
+struct SegCarry {
+    long long key = LLONG_MIN;
+    ull sum = 0;
+
+    __device__ __forceinline__ void add(bool valid, long long k_in, ull val, ull *base)
+    {
+        const int lane = threadIdx.x & 63;
+        long long k = valid ? k_in : LLONG_MIN;
+        ull v = valid ? val : 0ull;
+        if (lane == 0) {
+            if (k == key)
+                v += sum;
+            else if (key != LLONG_MIN && sum != 0)
+                atomicAdd(&base[key], sum);
+        }
+        for (int off = 1; off < 64; off <<= 1) {
+            const ull ov = __shfl_up(v, off, 64);
+            const long long ok = __shfl_up(k, off, 64);
+            if (lane >= off && ok == k) v += ov;
+        }
+        const long long nk = __shfl_down(k, 1, 64);
+        const bool tail = (lane == 63) || (nk != k);
+        if (valid && tail && lane != 63 && v != 0) atomicAdd(&base[k], v);
+        key = __shfl(k, 63, 64);
+        sum = __shfl(v, 63, 64);
+    }
+
+    __device__ __forceinline__ void flush(ull *base)
+    {
+        if ((threadIdx.x & 63) == 0 && key != LLONG_MIN && sum != 0) atomicAdd(&base[key], sum);
+        key = LLONG_MIN;
+        sum = 0;
+    }
+};
+
 __device__ __forceinline__ int tier_of(int64_t d, int64_t q)
 {
     if (d == 0) return 0;  // no neighbours: the vertex can never move

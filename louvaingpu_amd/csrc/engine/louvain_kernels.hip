@@ -193,21 +193,27 @@ __global__ __launch_bounds__(256) void k_csr_unpack(const uint32_t *skeys, const
                                                    int64_t nnz, int wbits, int32_t *col,
                                                    int64_t *cw, int64_t *kdeg)
 {
-    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < nnz;
-         base += (int64_t)gridDim.x * blockDim.x) {
-        int64_t p = base + threadIdx.x;
-        bool valid = p < nnz;
-        long long row = 0;
-        ull wp = 0;
-        if (valid) {
-            uint64_t x = spay[p];
-            int64_t u = (int64_t)(x & ((1ull << wbits) - 1ull));
-            col[p] = (int32_t)(x >> wbits);
-            cw[p] = u;
-            row = (long long)skeys[p];
-            wp = (ull)(u == 0 ? 1 : u);
+    constexpr int64_t SPAN = 256 * kSpanRounds;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    SegCarry sc;
+    for (int64_t base = (int64_t)blockIdx.x * SPAN; base < nnz; base += (int64_t)gridDim.x * SPAN) {
+        const int64_t w0 = base + (int64_t)wave * 64 * kSpanRounds;
+        for (int rr = 0; rr < kSpanRounds; ++rr) {
+            const int64_t p = w0 + rr * 64 + lane;
+            const bool valid = p < nnz;
+            long long row = 0;
+            ull wp = 0;
+            if (valid) {
+                uint64_t x = spay[p];
+                int64_t u = (int64_t)(x & ((1ull << wbits) - 1ull));
+                col[p] = (int32_t)(x >> wbits);
+                cw[p] = u;
+                row = (long long)skeys[p];
+                wp = (ull)(u == 0 ? 1 : u);
+            }
+            sc.add(valid, row, wp, (ull *)kdeg);
         }
-        seg_atomic_add(valid, row, wp, (ull *)kdeg);
+        sc.flush((ull *)kdeg);
     }
 }
 
@@ -224,7 +230,7 @@ void launch_csr_unpack(hipStream_t s, const uint32_t *skeys, const uint64_t *spa
                        int wbits, int32_t *col, int64_t *cw, int64_t *kdeg)
 {
     if (nnz > 0)
-        hipLaunchKernelGGL(k_csr_unpack, grid_for(nnz, 256, 1 << 20), 256, 0, s, skeys, spay, nnz,
+        hipLaunchKernelGGL(k_csr_unpack, grid_for(nnz, 256 * kSpanRounds, 1 << 20), 256, 0, s, skeys, spay, nnz,
                            wbits, col, cw, kdeg);
 }
 
@@ -365,26 +371,32 @@ __global__ __launch_bounds__(256) void k_fill_rev(const uint32_t *skey, const ui
                                                  int64_t *kdeg, DevFlags *f)
 {
     int bad = 0;
-    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < nr;
-         base += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t r = base + threadIdx.x;
-        const bool valid = r < nr;
-        long long row = 0;
-        ull wp = 0;
-        if (valid) {
-            const uint32_t v = skey[r];
-            const uint64_t x = spay[r];
-            const int64_t u = (int64_t)(x & ((1ull << wbits) - 1ull));
-            const int64_t pos = rowptr[v] + (r - rs[v]);
-            const int32_t cc = (int32_t)(x >> wbits);
-            col[pos] = cc;
-            cw[pos] = u;
-            row = v;
-            wp = (ull)(u == 0 ? 1 : u);
-            if constexpr (CHECK)
-                if (r > 0 && skey[r - 1] == v && (int32_t)(spay[r - 1] >> wbits) >= cc) bad = 4;
+    constexpr int64_t SPAN = 256 * kSpanRounds;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    SegCarry sc;
+    for (int64_t base = (int64_t)blockIdx.x * SPAN; base < nr; base += (int64_t)gridDim.x * SPAN) {
+        const int64_t w0 = base + (int64_t)wave * 64 * kSpanRounds;
+        for (int rr = 0; rr < kSpanRounds; ++rr) {
+            const int64_t r = w0 + rr * 64 + lane;
+            const bool valid = r < nr;
+            long long row = 0;
+            ull wp = 0;
+            if (valid) {
+                const uint32_t v = skey[r];
+                const uint64_t x = spay[r];
+                const int64_t u = (int64_t)(x & ((1ull << wbits) - 1ull));
+                const int64_t pos = rowptr[v] + (r - rs[v]);
+                const int32_t cc = (int32_t)(x >> wbits);
+                col[pos] = cc;
+                cw[pos] = u;
+                row = v;
+                wp = (ull)(u == 0 ? 1 : u);
+                if constexpr (CHECK)
+                    if (r > 0 && skey[r - 1] == v && (int32_t)(spay[r - 1] >> wbits) >= cc) bad = 4;
+            }
+            sc.add(valid, row, wp, (ull *)kdeg);
         }
-        seg_atomic_add(valid, row, wp, (ull *)kdeg);
+        sc.flush((ull *)kdeg);
     }
     if constexpr (CHECK) block_flag_or(bad, f);
 }
@@ -398,37 +410,43 @@ __global__ __launch_bounds__(256) void k_fill_fwd(const int32_t *src, const int3
                                                  int64_t *kdeg, DevFlags *f)
 {
     int bad = 0;
-    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < e;
-         base += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t i = base + threadIdx.x;
-        const bool valid = i < e;
-        long long row = 0;
-        ull wp = 0;
-        if (valid) {
-            const int32_t v = src[i], b = dst[i];
-            row = v;  // a self-loop edge keeps its row's key with weight 0, so the
-                      // runs of equal keys stay contiguous for the segmented sum
-            if (v != b) {
-                const int64_t pos = ffwd[v] + (i - slscan[i]);
-                const int64_t u = w[i];
-                col[pos] = b;
-                cw[pos] = u;
-                wp = (ull)(u == 0 ? 1 : u);
-                if constexpr (CHECK) {
-                    const int64_t f0 = fs[v];
-                    const int64_t rank = (i - f0) - (slscan[i] - slscan[f0]);
-                    if (rank > 0) {
-                        int64_t k = i - 1;  // the previous non-self edge of the row
-                        while (src[k] == dst[k]) --k;
-                        if (dst[k] >= b) bad = 4;
-                    } else if (rs[v + 1] > rs[v] &&
-                               (int32_t)(spay[rs[v + 1] - 1] >> wbits) >= b) {
-                        bad = 4;
+    constexpr int64_t SPAN = 256 * kSpanRounds;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    SegCarry sc;
+    for (int64_t base = (int64_t)blockIdx.x * SPAN; base < e; base += (int64_t)gridDim.x * SPAN) {
+        const int64_t w0 = base + (int64_t)wave * 64 * kSpanRounds;
+        for (int rr = 0; rr < kSpanRounds; ++rr) {
+            const int64_t i = w0 + rr * 64 + lane;
+            const bool valid = i < e;
+            long long row = 0;
+            ull wp = 0;
+            if (valid) {
+                const int32_t v = src[i], b = dst[i];
+                row = v;  // a self-loop edge keeps its row's key with weight 0, so the
+                          // runs of equal keys stay contiguous for the segmented sum
+                if (v != b) {
+                    const int64_t pos = ffwd[v] + (i - slscan[i]);
+                    const int64_t u = w[i];
+                    col[pos] = b;
+                    cw[pos] = u;
+                    wp = (ull)(u == 0 ? 1 : u);
+                    if constexpr (CHECK) {
+                        const int64_t f0 = fs[v];
+                        const int64_t rank = (i - f0) - (slscan[i] - slscan[f0]);
+                        if (rank > 0) {
+                            int64_t k = i - 1;  // the previous non-self edge of the row
+                            while (src[k] == dst[k]) --k;
+                            if (dst[k] >= b) bad = 4;
+                        } else if (rs[v + 1] > rs[v] &&
+                                   (int32_t)(spay[rs[v + 1] - 1] >> wbits) >= b) {
+                            bad = 4;
+                        }
                     }
                 }
             }
+            sc.add(valid, row, wp, (ull *)kdeg);
         }
-        seg_atomic_add(valid, row, wp, (ull *)kdeg);
+        sc.flush((ull *)kdeg);
     }
     if constexpr (CHECK) block_flag_or(bad, f);
 }
@@ -466,18 +484,18 @@ void launch_csr_sorted_fill(hipStream_t s, bool check, const int32_t *src, const
                        slscan, n, rowptr, ffwd);
     if (nr > 0) {
         if (check)
-            hipLaunchKernelGGL(k_fill_rev<true>, grid_for(nr, 256, 8192), 256, 0, s, skey, spay,
+            hipLaunchKernelGGL(k_fill_rev<true>, grid_for(nr, 256 * kSpanRounds, 8192), 256, 0, s, skey, spay,
                                nr, wbits, rs, rowptr, col, cw, kdeg, f);
         else
-            hipLaunchKernelGGL(k_fill_rev<false>, grid_for(nr, 256, 1 << 20), 256, 0, s, skey, spay,
+            hipLaunchKernelGGL(k_fill_rev<false>, grid_for(nr, 256 * kSpanRounds, 1 << 20), 256, 0, s, skey, spay,
                                nr, wbits, rs, rowptr, col, cw, kdeg, f);
     }
     if (e > 0) {
         if (check)
-            hipLaunchKernelGGL(k_fill_fwd<true>, grid_for(e, 256, 8192), 256, 0, s, src, dst, w,
+            hipLaunchKernelGGL(k_fill_fwd<true>, grid_for(e, 256 * kSpanRounds, 8192), 256, 0, s, src, dst, w,
                                slscan, e, fs, rs, spay, wbits, ffwd, col, cw, kdeg, f);
         else
-            hipLaunchKernelGGL(k_fill_fwd<false>, grid_for(e, 256, 1 << 20), 256, 0, s, src, dst, w,
+            hipLaunchKernelGGL(k_fill_fwd<false>, grid_for(e, 256 * kSpanRounds, 1 << 20), 256, 0, s, src, dst, w,
                                slscan, e, fs, rs, spay, wbits, ffwd, col, cw, kdeg, f);
     }
 }
