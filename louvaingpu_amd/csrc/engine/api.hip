@@ -72,6 +72,9 @@ struct lv_ctx {
     int64_t *kdeg = nullptr, *loopw = nullptr, *selfsum = nullptr;
     int32_t *C = nullptr, *next = nullptr;
     Lab *lab = nullptr;
+    Lab8 *lab8 = nullptr;        // narrow levels: 8-byte label records and 32-bit totals
+    uint32_t *deg32 = nullptr;
+    bool narrow = false;
     int64_t *deg = nullptr, *in = nullptr;
     int32_t *size = nullptr, *dsize = nullptr;
     ull *ddeg = nullptr;
@@ -235,6 +238,9 @@ void free_sweep_state(lv_ctx *c)
     arena_free(a, c->C);
     arena_free(a, c->next);
     arena_free(a, c->lab);
+    arena_free(a, c->lab8);
+    arena_free(a, c->deg32);
+    c->narrow = false;
     arena_free(a, c->deg);
     arena_free(a, c->in);
     arena_free(a, c->size);
@@ -611,7 +617,7 @@ int init_level(lv_ctx *c)
     LV_TRY(arena_alloc(A, c->ddeg, n));
     LV_TRY(arena_alloc(A, c->C, n));
     LV_TRY(arena_alloc(A, c->next, n));
-    LV_TRY(arena_alloc(A, c->lab, n));
+    LV_TRY(arena_alloc(A, c->deg32, n));
     uint32_t *tkey = nullptr, *tkey2 = nullptr;
     int32_t *v1 = nullptr, *v2 = nullptr;
     LV_TRY(arena_alloc(A, tkey, n));
@@ -621,7 +627,8 @@ int init_level(lv_ctx *c)
     LV_TRY(zero_small(c, 2 * (kTiers + 1) + 1));
     launch_init_vertex(s, n, c->rowptr, c->kdeg, c->loopw, c->selfsum, c->deg, c->in, c->size,
                        c->ddeg, c->dsize, c->C, c->next, tkey, v1, c->d_small,
-                       c->d_small + (kTiers + 1), (ull *)(c->d_small + 2 * (kTiers + 1)));
+                       c->d_small + (kTiers + 1), (ull *)(c->d_small + 2 * (kTiers + 1)),
+                       c->deg32);
     uint32_t *ko = nullptr;
     int32_t *vo = nullptr;
     LV_TRY(radix_sort_u32_i32(A, s, tkey, v1, tkey2, v2, n, 4, &ko, &vo));
@@ -633,7 +640,11 @@ int init_level(lv_ctx *c)
         const int S = bits_for((uint64_t)n) + 1;
         const int kb = bits_for((uint64_t)c->h_small[2 * (kTiers + 1)]);
         c->size_shift = (kb + S <= 62) ? S : 0;
+        // every community total is at most the total degree: 32 bits when it fits
+        c->narrow = kb <= 32;
     }
+    LV_TRY(arena_alloc(A, c->lab, n));
+    if (c->narrow) LV_TRY(arena_alloc(A, c->lab8, n));
     // keep the sorted vertex list, release the other buffers
     if (vo == v1) {
         c->verts = v1;
@@ -670,6 +681,8 @@ int one_level(lv_ctx *c, double lower, double *q_out, int32_t *passes_out)
     const bool trace = (c->cfg.flags & LV_FLAG_TRACE) != 0;
     SweepArgs a{};
     a.lab = c->lab;
+    a.lab8 = c->narrow ? c->lab8 : nullptr;
+    a.deg32 = c->narrow ? c->deg32 : nullptr;
     a.rowptr = c->rowptr;
     a.col = c->col;
     a.w = c->cw;
@@ -699,7 +712,7 @@ int one_level(lv_ctx *c, double lower, double *q_out, int32_t *passes_out)
         const bool singles = pass == 1 && c->rows_unique;
         if (!singles) {
             hipEvent_t tl = t_begin(c);
-            launch_lab(s, c->n, c->C, c->size, c->deg, c->lab);
+            launch_lab(s, c->n, c->C, c->size, c->deg, c->lab, c->narrow ? c->lab8 : nullptr);
             t_end(c, tl, CAT_APPLY);
         }
         for (int t = 1; t <= kTiers; ++t) {
@@ -732,7 +745,7 @@ int one_level(lv_ctx *c, double lower, double *q_out, int32_t *passes_out)
         }
         hipEvent_t t1 = t_begin(c);
         launch_apply(s, c->n, c->deg, c->in, c->size, c->ddeg, c->dsize, c->size_shift,
-                     c->d_qpart, c->d_qacc);
+                     c->narrow ? c->deg32 : nullptr, c->d_qpart, c->d_qacc);
         t_end(c, t1, CAT_APPLY);
         LV_HIP(hipGetLastError());
         LV_TRY(d2h(c, c->h_qacc, c->d_qacc, 1));
@@ -1083,7 +1096,7 @@ int batch_qagg(lv_ctx *c, bool apply)
 {
     fill_i64(c->stream, (int64_t *)c->d_cq, 4 * (int64_t)c->ncomp, 0);
     launch_b_qagg(c->stream, c->n, c->deg, c->in, c->size, c->ddeg, c->dsize, c->size_shift,
-                  c->vcomp, apply, c->d_cq);
+                  c->narrow ? c->deg32 : nullptr, c->vcomp, apply, c->d_cq);
     LV_HIP(hipGetLastError());
     LV_TRY(d2h(c, c->h_cq, c->d_cq, c->ncomp));
     return sync(c);
@@ -1116,6 +1129,8 @@ int batch_one_level(lv_ctx *c, const std::vector<double> &lower, const std::vect
     }
     SweepArgs a{};
     a.lab = c->lab;
+    a.lab8 = c->narrow ? c->lab8 : nullptr;
+    a.deg32 = c->narrow ? c->deg32 : nullptr;
     a.rowptr = c->rowptr;
     a.col = c->col;
     a.w = c->cw;
@@ -1153,7 +1168,7 @@ int batch_one_level(lv_ctx *c, const std::vector<double> &lower, const std::vect
         const bool singles = pass == 1 && c->rows_unique;  // see one_level
         if (!singles) {
             hipEvent_t tl = t_begin(c);
-            launch_lab(s, c->n, c->C, c->size, c->deg, c->lab);
+            launch_lab(s, c->n, c->C, c->size, c->deg, c->lab, c->narrow ? c->lab8 : nullptr);
             t_end(c, tl, CAT_APPLY);
         }
         for (int t = 1; t <= kTiers; ++t) {

@@ -148,8 +148,8 @@ __device__ __forceinline__ void seg_qacc_add(bool valid, int32_t key, long long 
 // deg^2); with `apply` the pass deltas are added first (k_apply per component).
 __global__ __launch_bounds__(256) void k_b_qagg(int32_t n, int64_t *deg, const int64_t *in,
                                                int32_t *size, ull *ddeg, int32_t *dsize,
-                                               int size_shift, const int32_t *vcomp, int apply,
-                                               QAcc *cq)
+                                               int size_shift, uint32_t *deg32,
+                                               const int32_t *vcomp, int apply, QAcc *cq)
 {
     for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n;
          base += (int64_t)gridDim.x * blockDim.x) {
@@ -159,7 +159,7 @@ __global__ __launch_bounds__(256) void k_b_qagg(int32_t n, int64_t *deg, const i
         long long sin = 0;
         ull lo = 0, hi = 0;
         if (valid) {
-            const int64_t dv = apply ? apply_deltas(c, deg, size, ddeg, dsize, size_shift) : deg[c];
+            const int64_t dv = apply ? apply_deltas(c, deg, size, ddeg, dsize, size_shift, deg32) : deg[c];
             key = vcomp[c];
             sin = in[c];
             acc_sq(lo, hi, (ull)dv);
@@ -263,12 +263,12 @@ void launch_b_edge_sum(hipStream_t s, const int64_t *w, const int64_t *ueoff, in
 }
 
 void launch_b_qagg(hipStream_t s, int32_t n, int64_t *deg, const int64_t *in, int32_t *size,
-                   ull *ddeg, int32_t *dsize, int size_shift, const int32_t *vcomp, bool apply,
-                   QAcc *cq)
+                   ull *ddeg, int32_t *dsize, int size_shift, uint32_t *deg32,
+                   const int32_t *vcomp, bool apply, QAcc *cq)
 {
     if (n > 0)
         hipLaunchKernelGGL(k_b_qagg, grid_for(n, 256, 1 << 20), 256, 0, s, n, deg, in, size, ddeg,
-                           dsize, size_shift, vcomp, apply ? 1 : 0, cq);
+                           dsize, size_shift, deg32, vcomp, apply ? 1 : 0, cq);
 }
 
 void launch_b_groups(hipStream_t s, const int32_t *gstart, const int32_t *perm,

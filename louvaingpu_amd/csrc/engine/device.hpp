@@ -189,9 +189,9 @@ __device__ __forceinline__ void seg_atomic_add(bool valid, long long key, ull va
 }
 
 // Adds the pass deltas of community c to its degree and size and clears them
-// (size_shift as in SweepArgs).
+// (size_shift as in SweepArgs); deg32, when given, follows deg.
 __device__ __forceinline__ int64_t apply_deltas(int64_t c, int64_t *deg, int32_t *size, ull *ddeg,
-                                                int32_t *dsize, int size_shift)
+                                                int32_t *dsize, int size_shift, uint32_t *deg32)
 {
     int64_t dv = deg[c];
     const ull dd = ddeg[c];
@@ -200,6 +200,7 @@ __device__ __forceinline__ int64_t apply_deltas(int64_t c, int64_t *deg, int32_t
             const int64_t ds = (int64_t)(dd << (64 - size_shift)) >> (64 - size_shift);
             dv += (int64_t)(dd - (ull)ds) >> size_shift;
             deg[c] = dv;
+            if (deg32) deg32[c] = (uint32_t)dv;
             size[c] += (int32_t)ds;
             ddeg[c] = 0;
         }
@@ -208,6 +209,7 @@ __device__ __forceinline__ int64_t apply_deltas(int64_t c, int64_t *deg, int32_t
         if (dd | (ull)(uint32_t)ds) {
             dv += (int64_t)dd;
             deg[c] = dv;
+            if (deg32) deg32[c] = (uint32_t)dv;
             size[c] += ds;
             ddeg[c] = 0;
             dsize[c] = 0;

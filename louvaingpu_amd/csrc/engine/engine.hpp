@@ -101,6 +101,14 @@ struct alignas(16) Lab {
     int64_t deg;
 };
 
+// The same record without the size when every community total fits 32 bits
+// (the level's total degree does): half the bytes per neighbour gather (the
+// size is needed only for the vertex's own community, from the wide record).
+struct alignas(8) Lab8 {
+    int32_t c;
+    uint32_t deg;
+};
+
 // One (community, weight) entry of a hash partition: a single 16-byte store
 // per entry when partitions are scattered.
 struct alignas(16) PEntry {
@@ -110,7 +118,9 @@ struct alignas(16) PEntry {
 };
 
 struct SweepArgs {
-    const Lab *lab;
+    const Lab *lab;         // label records; narrow levels also lab8 and deg32
+    const Lab8 *lab8;
+    const uint32_t *deg32;  // deg[] of narrow levels
     const int64_t *rowptr;
     const int32_t *col;
     const int64_t *w;
@@ -292,15 +302,16 @@ void launch_init_vertex(hipStream_t s, int32_t n, const int64_t *rowptr, const i
                         const int64_t *loopw, const int64_t *selfsum, int64_t *deg, int64_t *in,
                         int32_t *size, ull *ddeg, int32_t *dsize, int32_t *C,
                         int32_t *next, uint32_t *tier_key, int32_t *verts,
-                        long long *tier_cnt, long long *tier_nnz, ull *ksum);
+                        long long *tier_cnt, long long *tier_nnz, ull *ksum, uint32_t *deg32);
 // Q aggregates: per-block partials in part[kQParts], summed into *acc.
 constexpr int kQParts = 1024;
 void launch_qagg(hipStream_t s, int32_t n, const int64_t *deg, const int64_t *in, QAcc *part,
                  QAcc *acc);
 void launch_lab(hipStream_t s, int32_t n, const int32_t *C, const int32_t *size,
-                const int64_t *deg, Lab *lab);
+                const int64_t *deg, Lab *lab, Lab8 *lab8);
 void launch_apply(hipStream_t s, int32_t n, int64_t *deg, const int64_t *in, int32_t *size,
-                  ull *ddeg, int32_t *dsize, int size_shift, QAcc *part, QAcc *acc);
+                  ull *ddeg, int32_t *dsize, int size_shift, uint32_t *deg32, QAcc *part,
+                  QAcc *acc);
 // Move deltas of a pass that is not a last pass, from C and next (own_direct:
 // the first pass of a level, where every community left is a singleton).
 void launch_move_deltas(hipStream_t s, const SweepArgs &a, int32_t n, bool own_direct, bool batch);
@@ -350,8 +361,8 @@ void launch_b_union_edges(hipStream_t s, const int32_t *bsrc, const int32_t *bds
 void launch_b_edge_sum(hipStream_t s, const int64_t *w, const int64_t *ueoff, int32_t ncomp,
                        int64_t etot, ull *cm);
 void launch_b_qagg(hipStream_t s, int32_t n, int64_t *deg, const int64_t *in, int32_t *size,
-                   ull *ddeg, int32_t *dsize, int size_shift, const int32_t *vcomp, bool apply,
-                   QAcc *cq);
+                   ull *ddeg, int32_t *dsize, int size_shift, uint32_t *deg32,
+                   const int32_t *vcomp, bool apply, QAcc *cq);
 void launch_b_groups(hipStream_t s, const int32_t *gstart, const int32_t *perm,
                      const int32_t *vcomp, int32_t K, ull *kcnt);
 void launch_b_next_ids(hipStream_t s, const int32_t *gstart, const int32_t *perm,

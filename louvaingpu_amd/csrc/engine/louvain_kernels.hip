@@ -621,7 +621,7 @@ __global__ __launch_bounds__(256) void k_init_vertex(
     int32_t n, const int64_t *rowptr, const int64_t *kdeg, const int64_t *loopw,
     const int64_t *selfsum, int64_t *deg, int64_t *in, int32_t *size, ull *ddeg, int32_t *dsize,
     int32_t *C, int32_t *next, uint32_t *tier_key, int32_t *verts, long long *tier_cnt,
-    long long *tier_nnz, ull *ksum)
+    long long *tier_nnz, ull *ksum, uint32_t *deg32)
 {
     __shared__ long long s_cnt[kTiers + 1], s_nnz[kTiers + 1];
     __shared__ ull s_k[4];
@@ -641,6 +641,7 @@ __global__ __launch_bounds__(256) void k_init_vertex(
             const int64_t k = kdeg[i];
             kt = sat_add(kt, (ull)k);
             deg[i] = k;
+            deg32[i] = (uint32_t)k;  // used only when the level's total degree fits 32 bits
             in[i] = loopw[i];
             size[i] = 1;
             ddeg[i] = 0;
@@ -671,12 +672,12 @@ void launch_init_vertex(hipStream_t s, int32_t n, const int64_t *rowptr, const i
                         const int64_t *loopw, const int64_t *selfsum, int64_t *deg, int64_t *in,
                         int32_t *size, ull *ddeg, int32_t *dsize, int32_t *C, int32_t *next,
                         uint32_t *tier_key, int32_t *verts, long long *tier_cnt,
-                        long long *tier_nnz, ull *ksum)
+                        long long *tier_nnz, ull *ksum, uint32_t *deg32)
 {
     if (n > 0)
         hipLaunchKernelGGL(k_init_vertex, grid_for(n, 256, 1024), 256, 0, s, n, rowptr, kdeg,
                            loopw, selfsum, deg, in, size, ddeg, dsize, C, next, tier_key, verts,
-                           tier_cnt, tier_nnz, ksum);
+                           tier_cnt, tier_nnz, ksum, deg32);
 }
 
 // ---------------------------------------------------------------------------
@@ -754,36 +755,45 @@ __global__ __launch_bounds__(256) void k_qagg_ull(int32_t n, const ull *dc, cons
 
 __global__ __launch_bounds__(256) void k_apply(int32_t n, int64_t *deg, const int64_t *in,
                                               int32_t *size, ull *ddeg, int32_t *dsize,
-                                              int size_shift, QAcc *part)
+                                              int size_shift, uint32_t *deg32, QAcc *part)
 {
     long long sin = 0;
     ull lo = 0, hi = 0;
     GRID_STRIDE(c, n)
     {
-        const int64_t dv = apply_deltas(c, deg, size, ddeg, dsize, size_shift);
+        const int64_t dv = apply_deltas(c, deg, size, ddeg, dsize, size_shift, deg32);
         sin += in[c];
         acc_sq(lo, hi, (ull)dv);
     }
     block_qpartial(sin, lo, hi, part);
 }
 
-__global__ void k_lab(int32_t n, const int32_t *C, const int32_t *size, const int64_t *deg, Lab *lab)
+__global__ void k_lab(int32_t n, const int32_t *C, const int32_t *size, const int64_t *deg, Lab *lab,
+                      Lab8 *lab8)
 {
     GRID_STRIDE(v, n)
     {
         const int32_t c = C[v];
+        const int64_t d = deg[c];
         Lab L;
         L.c = c;
         L.sz = size[c];
-        L.deg = deg[c];
+        L.deg = d;
         lab[v] = L;
+        if (lab8) {
+            Lab8 N;
+            N.c = c;
+            N.deg = (uint32_t)d;
+            lab8[v] = N;
+        }
     }
 }
 
 void launch_lab(hipStream_t s, int32_t n, const int32_t *C, const int32_t *size,
-                const int64_t *deg, Lab *lab)
+                const int64_t *deg, Lab *lab, Lab8 *lab8)
 {
-    if (n > 0) hipLaunchKernelGGL(k_lab, grid_for(n, 256, 1 << 20), 256, 0, s, n, C, size, deg, lab);
+    if (n > 0)
+        hipLaunchKernelGGL(k_lab, grid_for(n, 256, 1 << 20), 256, 0, s, n, C, size, deg, lab, lab8);
 }
 
 static int qblocks(int32_t n) { return grid_for(n, 256, kQParts); }
@@ -804,10 +814,12 @@ void launch_qagg_ull(hipStream_t s, int32_t n, const ull *dc, const ull *lc, QAc
 }
 
 void launch_apply(hipStream_t s, int32_t n, int64_t *deg, const int64_t *in, int32_t *size,
-                  ull *ddeg, int32_t *dsize, int size_shift, QAcc *part, QAcc *acc)
+                  ull *ddeg, int32_t *dsize, int size_shift, uint32_t *deg32, QAcc *part,
+                  QAcc *acc)
 {
     int nb = qblocks(n);
-    hipLaunchKernelGGL(k_apply, nb, 256, 0, s, n, deg, in, size, ddeg, dsize, size_shift, part);
+    hipLaunchKernelGGL(k_apply, nb, 256, 0, s, n, deg, in, size, ddeg, dsize, size_shift, deg32,
+                       part);
     hipLaunchKernelGGL(k_qfinal, 1, 256, 0, s, part, nb, acc);
 }
 
@@ -841,6 +853,26 @@ __device__ __forceinline__ VParam vparam(const SweepArgs &a, int32_t v)
         return {a.m2, a.gamma, a.last ? 2 : 1};
     }
 }
+
+// Gathers of a community total and of a neighbour's label record, from the
+// narrow arrays when the level has them (the choice is uniform per launch).
+__device__ __forceinline__ int64_t deg_of(const SweepArgs &a, int32_t c)
+{
+    return a.deg32 ? (int64_t)a.deg32[c] : a.deg[c];
+}
+
+__device__ __forceinline__ Lab lab_of(const SweepArgs &a, int32_t j)
+{
+    if (a.lab8) {
+        const Lab8 L = a.lab8[j];
+        return Lab{L.c, 0, (int64_t)L.deg};
+    }
+    return a.lab[j];
+}
+
+// The vertex's own record, with the size of its community (the wide records
+// are always written; narrow levels gather neighbours from lab8).
+__device__ __forceinline__ Lab own_lab(const SweepArgs &a, int32_t v) { return a.lab[v]; }
 
 __device__ __forceinline__ void consider(int32_t c, int64_t W, int64_t degc, double gamma,
                                          double f, int &cnt, double &bg, int32_t &bc, int64_t &bw)
@@ -1023,7 +1055,7 @@ __device__ __forceinline__ void insert_row(const SweepArgs &a, int32_t *key, ull
             Lab L[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u)
-                if (j[u] >= 0) L[u] = a.lab[j[u]];
+                if (j[u] >= 0) L[u] = lab_of(a, j[u]);
 #pragma unroll
             for (int u = 0; u < 4; ++u)
                 if (j[u] >= 0) lds_insert_d(key, val, dg, l2, L[u].c, w[u], L[u].deg);
@@ -1066,7 +1098,7 @@ __global__ __launch_bounds__(256) void k_sweep_group(SweepArgs a, const int32_t 
         if (active) {
             v = verts[vi];
             vp = vparam<B>(a, v);
-            me = a.lab[v];
+            me = own_lab(a, v);
             beg = a.rowptr[v];
             end = a.rowptr[v + 1];
             loop = a.selfsum[v] > 0;
@@ -1098,7 +1130,7 @@ __global__ __launch_bounds__(256) void k_sweep_group(SweepArgs a, const int32_t 
             for (int s = lg; s < cap; s += G) {
                 int32_t c = key[s];
                 if (c >= 0)
-                    consider(c, (int64_t)val[s], DL ? dg[s] : a.deg[c], vp.gamma, f, cnt, bg, bc, bw);
+                    consider(c, (int64_t)val[s], DL ? dg[s] : deg_of(a, c), vp.gamma, f, cnt, bg, bc, bw);
             }
         }
         group_reduce<G>(cnt, bg, bc, bw);
@@ -1125,7 +1157,7 @@ __global__ __launch_bounds__(BLOCK) void k_sweep_block(SweepArgs a, const int32_
     for (int64_t vi = blockIdx.x; vi < nv; vi += gridDim.x) {
         const int32_t v = verts[vi];
         const VParam vp = vparam<B>(a, v);
-        const Lab me = a.lab[v];
+        const Lab me = own_lab(a, v);
         const int64_t beg = a.rowptr[v];
         const int64_t end = vp.mode == 0 ? beg : a.rowptr[v + 1];  // frozen: nothing to gather
         const bool loop = vp.mode != 0 && a.selfsum[v] > 0;
@@ -1146,7 +1178,7 @@ __global__ __launch_bounds__(BLOCK) void k_sweep_block(SweepArgs a, const int32_
         const double f = (double)a.kdeg[v] / vp.m2;
         for (int s = tid; s < cap; s += BLOCK) {
             int32_t c = s_key[s];
-            if (c >= 0) consider(c, (int64_t)s_val[s], a.deg[c], vp.gamma, f, cnt, bg, bc, bw);
+            if (c >= 0) consider(c, (int64_t)s_val[s], deg_of(a, c), vp.gamma, f, cnt, bg, bc, bw);
         }
         group_reduce<64>(cnt, bg, bc, bw);
         if (lane == 0) {
@@ -1250,7 +1282,7 @@ __global__ __launch_bounds__(256) void k_sweep1_group(SweepArgs a, const int32_t
                     w[u] = (p < end) ? a.w[p] : 0;
                 }
 #pragma unroll
-                for (int u = 0; u < 4; ++u) d[u] = (j[u] >= 0) ? a.deg[j[u]] : 0;
+                for (int u = 0; u < 4; ++u) d[u] = (j[u] >= 0) ? deg_of(a, j[u]) : 0;
 #pragma unroll
                 for (int u = 0; u < 4; ++u)
                     if (j[u] >= 0) consider(j[u], w[u], d[u], vp.gamma, f, cnt, bg, bc, bw);
@@ -1294,7 +1326,7 @@ __global__ __launch_bounds__(BLOCK) void k_sweep1_block(SweepArgs a, const int32
                 w[u] = (p < end) ? a.w[p] : 0;
             }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) d[u] = (j[u] >= 0) ? a.deg[j[u]] : 0;
+            for (int u = 0; u < 4; ++u) d[u] = (j[u] >= 0) ? deg_of(a, j[u]) : 0;
 #pragma unroll
             for (int u = 0; u < 4; ++u)
                 if (j[u] >= 0) consider(j[u], w[u], d[u], vp.gamma, f, cnt, bg, bc, bw);
@@ -1391,7 +1423,7 @@ __global__ __launch_bounds__(256) void k_hub1_chunk(SweepArgs a, HubArgs h)
             w[u] = (jj[u] >= 0) ? a.w[rb + t] : 0;
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) dg[u] = (jj[u] >= 0) ? a.deg[jj[u]] : 0;
+        for (int u = 0; u < 4; ++u) dg[u] = (jj[u] >= 0) ? deg_of(a, jj[u]) : 0;
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             if (jj[u] >= 0)
@@ -1656,7 +1688,7 @@ __global__ __launch_bounds__(256) void k_hub_agg(SweepArgs a, HubArgs h)
     if (!s_over) {
         for (int s = tid; s < cap; s += 256) {
             int32_t c = s_key[s];
-            if (c >= 0) consider(c, (int64_t)s_val[s], a.deg[c], vp.gamma, f, cnt, bg, bc, bw);
+            if (c >= 0) consider(c, (int64_t)s_val[s], deg_of(a, c), vp.gamma, f, cnt, bg, bc, bw);
         }
     } else {
         // more than CAP distinct keys: redo the partition in its global region
@@ -1676,7 +1708,7 @@ __global__ __launch_bounds__(256) void k_hub_agg(SweepArgs a, HubArgs h)
             int32_t c = __hip_atomic_load(&gk[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (c >= 0)
                 consider(c, (int64_t)__hip_atomic_load(&gv[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
-                         a.deg[c], vp.gamma, f, cnt, bg, bc, bw);
+                         deg_of(a, c), vp.gamma, f, cnt, bg, bc, bw);
         }
     }
     group_reduce<64>(cnt, bg, bc, bw);
@@ -1731,7 +1763,7 @@ __global__ __launch_bounds__(256) void k_hub_final(SweepArgs a, HubArgs h, int32
     group_reduce<64>(cnt, bg, bc, bw);
     if (j < nh && lane == 0) {
         const int32_t v = h.hv[j];
-        const Lab me = a.lab[v];
+        const Lab me = own_lab(a, v);
         finalize_vertex(a, v, me.c, me.sz, cnt, bc, bw, vparam<B>(a, v).mode);
     }
 }
