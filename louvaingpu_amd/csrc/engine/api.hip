@@ -239,8 +239,7 @@ void free_sweep_state(lv_ctx *c)
     arena_free(a, c->next);
     arena_free(a, c->lab);
     arena_free(a, c->lab8);
-    arena_free(a, c->deg32);
-    c->narrow = false;
+    arena_free(a, c->deg32);  // (narrow stays: the level's coarsening reads it)
     arena_free(a, c->deg);
     arena_free(a, c->in);
     arena_free(a, c->size);
@@ -853,7 +852,9 @@ int induce(lv_ctx *c, const int32_t *newc, const int32_t *newid_g, int32_t Kn)
     ull *selfl = nullptr, *ssum = nullptr, *internal = nullptr;
     int32_t *ey = nullptr, *bound = nullptr, *uy = nullptr, *nuniq = nullptr;
     int64_t *ew = nullptr, *uw = nullptr;
+    uint32_t *ew32 = nullptr;  // narrow level: every weight is below 2^32
     int64_t *mstart = nullptr;
+    const bool nar = c->narrow;
     LV_TRY(arena_alloc(A, mdeg, n));
     LV_TRY(arena_alloc(A, mstart, n));
     LV_TRY(arena_alloc(A, estart, (int64_t)n + 1));
@@ -862,7 +863,10 @@ int induce(lv_ctx *c, const int32_t *newc, const int32_t *newid_g, int32_t Kn)
     LV_TRY(arena_alloc(A, internal, K));
     LV_TRY(arena_alloc(A, nuniq, K));
     LV_TRY(arena_alloc(A, ey, nnz));
-    LV_TRY(arena_alloc(A, ew, nnz));
+    if (nar)
+        LV_TRY(arena_alloc(A, ew32, nnz));
+    else
+        LV_TRY(arena_alloc(A, ew, nnz));
     LV_TRY(arena_alloc(A, uy, nnz));
     LV_TRY(arena_alloc(A, uw, nnz));
     LV_TRY(arena_alloc(A, bound, nnz / 1024 + 2));
@@ -873,7 +877,7 @@ int induce(lv_ctx *c, const int32_t *newc, const int32_t *newid_g, int32_t Kn)
     fill_i64(s, (int64_t *)internal, K, 0);
     fill_i32(s, nuniq, K, 0);
     launch_cr_selfl(s, L.perm, c->selfsum, c->g_head, c->g_hscan, n, selfl);
-    launch_cr_gather(s, estart, bound, mstart, c->col, c->cw, newc, n, nnz, ey, ew);
+    launch_cr_gather(s, estart, bound, mstart, c->col, c->cw, newc, n, nnz, ey, ew, ew32);
     // segments by length
     int64_t *sb = nullptr, *slen = nullptr;
     uint32_t *tk1 = nullptr, *tk2 = nullptr;
@@ -897,9 +901,10 @@ int induce(lv_ctx *c, const int32_t *newc, const int32_t *newid_g, int32_t Kn)
         toff[t] = off;
         off += tcnt[t];
     }
-    CoarsenArgs ca;
+    CoarsenArgs ca{};
     ca.ey = ey;
     ca.ew = ew;
+    ca.ew32 = ew32;
     ca.sb = sb;
     ca.slen = slen;
     ca.newid_g = newid_g;
@@ -954,7 +959,10 @@ int induce(lv_ctx *c, const int32_t *newc, const int32_t *newid_g, int32_t Kn)
         LV_TRY(arena_alloc(A, B.pcount, nparts + 1));
         LV_TRY(arena_alloc(A, B.poff, nparts + 1));
         LV_TRY(arena_alloc(A, B.pcur, nparts));
-        LV_TRY(arena_alloc(A, B.pe, blen));
+        if (nar)
+            LV_TRY(arena_alloc(A, B.pe8, blen));
+        else
+            LV_TRY(arena_alloc(A, B.pe, blen));
         LV_TRY(arena_alloc(A, B.buy, blen));
         LV_TRY(arena_alloc(A, B.buw, blen));
         LV_TRY(arena_alloc(A, B.fbk, 2 * blen));
@@ -987,9 +995,9 @@ int induce(lv_ctx *c, const int32_t *newc, const int32_t *newid_g, int32_t Kn)
     LV_HIP(hipGetLastError());
     t_end(c, t0, CAT_COARSEN);
     // release the transient buffers (later launches on this stream may reuse them)
-    void *tmp[] = {mdeg, mstart, estart, selfl, ssum, internal, nuniq, ey, ew, uy, uw, bound, sb, slen,
+    void *tmp[] = {mdeg, mstart, estart, selfl, ssum, internal, nuniq, ey, ew, ew32, uy, uw, bound, sb, slen,
                    tk1, tk2, sid1, sid2, bP, bP64, bch, pbase, cbase, part_big, chunk_big, chunk_k,
-                   B.pcount, B.poff, B.pcur, B.pe, B.buy, B.buw, B.fbk, B.fbv, B.ucnt, B.pre,
+                   B.pcount, B.poff, B.pcur, B.pe, B.pe8, B.buy, B.buw, B.fbk, B.fbv, B.ucnt, B.pre,
                    rowcnt};
     for (void *p : tmp) A.release(p);
     free_level_arrays(c);

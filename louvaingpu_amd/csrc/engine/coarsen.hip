@@ -19,6 +19,22 @@
 
 namespace lv {
 
+// Weight of member-order entry e (4-byte array on narrow levels).
+__device__ __forceinline__ int64_t entry_w(const CoarsenArgs &a, int64_t e)
+{
+    return a.ew32 ? (int64_t)a.ew32[e] : a.ew[e];
+}
+
+// Partition entry e as (community, weight).
+__device__ __forceinline__ PEntry part_entry(const BigSegArgs &B, int64_t e)
+{
+    if (B.pe8) {
+        const PEntry8 x = B.pe8[e];
+        return PEntry{x.c, 0, (int64_t)x.w};
+    }
+    return B.pe[e];
+}
+
 // ---------------------------------------------------------------------------
 // renumbering
 
@@ -144,7 +160,8 @@ constexpr int kGatherLds = 2048; // member offsets a block can stage in LDS
 __global__ __launch_bounds__(256) void k_cr_gather(const int64_t *estart, const int32_t *bound,
                                                   const int64_t *mstart, const int32_t *col,
                                                   const int64_t *cw, const int32_t *newc,
-                                                  int64_t total, int32_t *ey, int64_t *ew)
+                                                  int64_t total, int32_t *ey, int64_t *ew,
+                                                  uint32_t *ew32)
 {
     __shared__ int64_t s_est[kGatherLds];
     const int64_t b = blockIdx.x;
@@ -203,7 +220,11 @@ __global__ __launch_bounds__(256) void k_cr_gather(const int64_t *estart, const 
         const int64_t o = b * kGatherT + r * 256 + threadIdx.x;
         if (ok[r]) {
             ey[o] = cc[r];
-            ew[o] = uu[r] == 0 ? 1 : uu[r];
+            const int64_t wp = uu[r] == 0 ? 1 : uu[r];
+            if (ew32)
+                ew32[o] = (uint32_t)wp;
+            else
+                ew[o] = wp;
         }
     }
 }
@@ -225,14 +246,14 @@ void launch_cr_selfl(hipStream_t s, const int32_t *perm, const int64_t *selfsum,
 
 void launch_cr_gather(hipStream_t s, const int64_t *estart, int32_t *bound, const int64_t *mstart,
                       const int32_t *col, const int64_t *cw, const int32_t *newc, int32_t n,
-                      int64_t total, int32_t *ey, int64_t *ew)
+                      int64_t total, int32_t *ey, int64_t *ew, uint32_t *ew32)
 {
     if (total <= 0) return;
     const int64_t nb = (total + kGatherT - 1) / kGatherT;
     hipLaunchKernelGGL(k_lbs_bounds, grid_for(nb + 1, 256), 256, 0, s, estart, n, total, kGatherT,
                        nb, bound);
     hipLaunchKernelGGL(k_cr_gather, (int)nb, 256, 0, s, estart, bound, mstart, col, cw, newc,
-                       total, ey, ew);
+                       total, ey, ew, ew32);
 }
 
 // ---------------------------------------------------------------------------
@@ -315,7 +336,7 @@ __global__ __launch_bounds__(256) void k_cr_agg_group(CoarsenArgs a, const int32
         if (active) {
             for (int64_t e = b + lg; e < b + L; e += G) {
                 int32_t y = a.ey[e];
-                int64_t w = a.ew[e];
+                const int64_t w = entry_w(a, e);
                 tot += (ull)w;
                 if (y == x)
                     inter += (ull)w;
@@ -374,7 +395,7 @@ __global__ __launch_bounds__(256) void k_cr_agg_block(CoarsenArgs a, const int32
         ull inter = 0, tot = 0;
         for (int64_t e = b + tid; e < b + L; e += 256) {
             int32_t y = a.ey[e];
-            int64_t w = a.ew[e];
+            const int64_t w = entry_w(a, e);
             tot += (ull)w;
             if (y == x)
                 inter += (ull)w;
@@ -472,7 +493,7 @@ __global__ __launch_bounds__(256) void k_cr_big_count(CoarsenArgs a, BigSegArgs 
     ull inter = 0, tot = 0;
     for (int64_t t = t0 + threadIdx.x; t < t1; t += 256) {
         int32_t y = a.ey[b + t];
-        const ull w = (ull)a.ew[b + t];
+        const ull w = (ull)entry_w(a, b + t);
         tot += w;
         if (y == x)
             inter += w;
@@ -526,11 +547,18 @@ __global__ __launch_bounds__(256) void k_cr_big_scatter(CoarsenArgs a, BigSegArg
         if (y == x) continue;
         int part = hub_part(y, P);
         int64_t pos = lbase[part] + atomicAdd(&hist[part], 1);
-        PEntry en;
-        en.c = y;
-        en.pad = 0;
-        en.w = a.ew[b + t];
-        B.pe[pos] = en;
+        if (B.pe8) {
+            PEntry8 en;
+            en.c = y;
+            en.w = a.ew32[b + t];
+            B.pe8[pos] = en;
+        } else {
+            PEntry en;
+            en.c = y;
+            en.pad = 0;
+            en.w = a.ew[b + t];
+            B.pe[pos] = en;
+        }
     }
 }
 
@@ -556,7 +584,7 @@ __global__ __launch_bounds__(256) void k_cr_big_agg(BigSegArgs B)
     }
     __syncthreads();
     for (int64_t e = beg + tid; e < end; e += 256) {
-        const PEntry en = B.pe[e];
+        const PEntry en = part_entry(B, e);
         if (!lds_insert_bounded(s_key, s_val, l2, en.c, en.w)) s_over = 1;
     }
     __syncthreads();
@@ -579,7 +607,10 @@ __global__ __launch_bounds__(256) void k_cr_big_agg(BigSegArgs B)
         }
         __threadfence();
         __syncthreads();
-        for (int64_t e = beg + tid; e < end; e += 256) glb_insert_mod(gk, gv, cap, B.pe[e].c, B.pe[e].w);
+        for (int64_t e = beg + tid; e < end; e += 256) {
+            const PEntry en = part_entry(B, e);
+            glb_insert_mod(gk, gv, cap, en.c, en.w);
+        }
         __threadfence();
         __syncthreads();
         for (int64_t s = tid; s < cap; s += 256) {

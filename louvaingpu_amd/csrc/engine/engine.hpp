@@ -109,6 +109,12 @@ struct alignas(8) Lab8 {
     uint32_t deg;
 };
 
+// A partition entry of a narrow level (every weight below 2^32).
+struct alignas(8) PEntry8 {
+    int32_t c;
+    uint32_t w;
+};
+
 // One (community, weight) entry of a hash partition: a single 16-byte store
 // per entry when partitions are scattered.
 struct alignas(16) PEntry {
@@ -194,7 +200,8 @@ struct HubArgs {
 // Coarsening: community segments of the member-order entry list.
 struct CoarsenArgs {
     const int32_t *ey;      // newc[target] of every entry, member order
-    const int64_t *ew;      // w' of every entry
+    const int64_t *ew;      // w' of every entry, or (narrow levels) ew32 instead
+    const uint32_t *ew32;
     const int64_t *sb;      // first entry of group g
     const int64_t *slen;    // entries of group g
     const int32_t *newid_g; // new id of group g
@@ -216,7 +223,8 @@ struct BigSegArgs {
     int64_t *pcount;
     int64_t *poff;
     int64_t *pcur;
-    PEntry *pe;             // partitioned entries
+    PEntry *pe;             // partitioned entries, or (narrow levels) pe8 instead
+    PEntry8 *pe8;
     int32_t *buy;           // uniques per partition, at poff[p]
     int64_t *buw;
     int32_t *fbk;           // fallback tables
@@ -388,7 +396,7 @@ void launch_cr_selfl(hipStream_t s, const int32_t *perm, const int64_t *selfsum,
                      const int32_t *head, const int32_t *hscan, int32_t n, ull *selfl);
 void launch_cr_gather(hipStream_t s, const int64_t *estart, int32_t *bound, const int64_t *mstart,
                       const int32_t *col, const int64_t *cw, const int32_t *newc, int32_t n,
-                      int64_t total, int32_t *ey, int64_t *ew);
+                      int64_t total, int32_t *ey, int64_t *ew, uint32_t *ew32);
 void launch_cr_seg_tier(hipStream_t s, const int32_t *gstart, const int64_t *estart,
                         const int32_t *newid_g, int32_t K, int64_t *sb, int64_t *slen,
                         uint32_t *tkey, int32_t *sid, long long *tier_cnt);
