@@ -583,9 +583,15 @@ __global__ __launch_bounds__(256) void k_cr_big_agg(BigSegArgs B)
         s_cnt = 0;
     }
     __syncthreads();
-    for (int64_t e = beg + tid; e < end; e += 256) {
-        const PEntry en = part_entry(B, e);
-        if (!lds_insert_bounded(s_key, s_val, l2, en.c, en.w)) s_over = 1;
+    for (int64_t e0 = beg + tid; e0 < end; e0 += 4 * 256) {
+        PEntry en[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (e0 + u * 256 < end) en[u] = part_entry(B, e0 + u * 256);
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (e0 + u * 256 < end && !lds_insert_bounded(s_key, s_val, l2, en[u].c, en[u].w))
+                s_over = 1;
     }
     __syncthreads();
     if (!s_over) {

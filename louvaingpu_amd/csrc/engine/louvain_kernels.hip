@@ -888,6 +888,27 @@ __device__ __forceinline__ void consider(int32_t c, int64_t W, int64_t degc, dou
     }
 }
 
+// The candidates of an LDS table, slots s0, s0 + S, ... below cap, with the
+// community totals gathered four at a time (the order of consider calls does
+// not change the result).
+template <int S>
+__device__ __forceinline__ void consider_slots(const SweepArgs &a, const int32_t *key, const ull *val,
+                                               int s0, int cap, double gamma, double f, int &cnt,
+                                               double &bg, int32_t &bc, int64_t &bw)
+{
+    for (int s = s0; s < cap; s += 4 * S) {
+        int32_t c[4];
+        int64_t d[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) c[u] = (s + u * S < cap) ? key[s + u * S] : -1;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) d[u] = (c[u] >= 0) ? deg_of(a, c[u]) : 0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (c[u] >= 0) consider(c[u], (int64_t)val[s + u * S], d[u], gamma, f, cnt, bg, bc, bw);
+    }
+}
+
 template <int W>
 __device__ __forceinline__ void group_reduce(int &cnt, double &bg, int32_t &bc, int64_t &bw)
 {
@@ -1127,10 +1148,13 @@ __global__ __launch_bounds__(256) void k_sweep_group(SweepArgs a, const int32_t 
         int64_t bw = 0;
         if (active) {
             const double f = (double)a.kdeg[v] / vp.m2;
-            for (int s = lg; s < cap; s += G) {
-                int32_t c = key[s];
-                if (c >= 0)
-                    consider(c, (int64_t)val[s], DL ? dg[s] : deg_of(a, c), vp.gamma, f, cnt, bg, bc, bw);
+            if constexpr (DL) {
+                for (int s = lg; s < cap; s += G) {
+                    const int32_t c = key[s];
+                    if (c >= 0) consider(c, (int64_t)val[s], dg[s], vp.gamma, f, cnt, bg, bc, bw);
+                }
+            } else {
+                consider_slots<G>(a, key, val, lg, cap, vp.gamma, f, cnt, bg, bc, bw);
             }
         }
         group_reduce<G>(cnt, bg, bc, bw);
@@ -1176,10 +1200,7 @@ __global__ __launch_bounds__(BLOCK) void k_sweep_block(SweepArgs a, const int32_
         int32_t bc = INT_MAX;
         int64_t bw = 0;
         const double f = (double)a.kdeg[v] / vp.m2;
-        for (int s = tid; s < cap; s += BLOCK) {
-            int32_t c = s_key[s];
-            if (c >= 0) consider(c, (int64_t)s_val[s], deg_of(a, c), vp.gamma, f, cnt, bg, bc, bw);
-        }
+        consider_slots<BLOCK>(a, s_key, s_val, tid, cap, vp.gamma, f, cnt, bg, bc, bw);
         group_reduce<64>(cnt, bg, bc, bw);
         if (lane == 0) {
             r_cnt[wave] = cnt;
@@ -1676,9 +1697,15 @@ __global__ __launch_bounds__(256) void k_hub_agg(SweepArgs a, HubArgs h)
     }
     if (tid == 0) s_over = 0;
     __syncthreads();
-    for (int64_t e = beg + tid; e < end; e += 256) {
-        const PEntry en = h.buf[e];
-        if (!lds_insert_bounded(s_key, s_val, l2, en.c, en.w)) s_over = 1;
+    for (int64_t e0 = beg + tid; e0 < end; e0 += 4 * 256) {
+        PEntry en[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (e0 + u * 256 < end) en[u] = h.buf[e0 + u * 256];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (e0 + u * 256 < end && !lds_insert_bounded(s_key, s_val, l2, en[u].c, en[u].w))
+                s_over = 1;
     }
     __syncthreads();
     int cnt = 0;
@@ -1686,10 +1713,7 @@ __global__ __launch_bounds__(256) void k_hub_agg(SweepArgs a, HubArgs h)
     int32_t bc = INT_MAX;
     int64_t bw = 0;
     if (!s_over) {
-        for (int s = tid; s < cap; s += 256) {
-            int32_t c = s_key[s];
-            if (c >= 0) consider(c, (int64_t)s_val[s], deg_of(a, c), vp.gamma, f, cnt, bg, bc, bw);
-        }
+        consider_slots<256>(a, s_key, s_val, tid, cap, vp.gamma, f, cnt, bg, bc, bw);
     } else {
         // more than CAP distinct keys: redo the partition in its global region
         const int64_t cap = 2 * (end - beg);
