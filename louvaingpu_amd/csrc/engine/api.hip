@@ -378,27 +378,43 @@ int build_level0_csr_sorted(lv_ctx *c, int32_t *self, const int64_t *fs, bool ca
     LV_TRY(d2h(c, &nself, (const int32_t *)(slscan + e), 1));
     LV_TRY(sync(c));
     const int64_t nr = e - nself;
+    const int kb = bits_for(n > 0 ? (uint64_t)(n - 1) : 0);
+    // one word per reverse entry (dst | src | w) when the weight fits the bits
+    // that two ids leave: the row sort then moves 8 bytes per entry, not 12
+    const int wb1 = 64 - 2 * kb;
+    const bool one_word = kb >= 1 && wb1 >= 1 && c->max_units0 < (1ull << wb1);
     int64_t *rs = nullptr, *ffwd = nullptr;
     uint32_t *k1 = nullptr, *k2 = nullptr;
     uint64_t *p1 = nullptr, *p2 = nullptr;
     LV_TRY(arena_alloc(A, rs, (int64_t)n + 1));
     LV_TRY(arena_alloc(A, ffwd, (int64_t)n + 1));
-    LV_TRY(arena_alloc(A, k1, nr));
-    LV_TRY(arena_alloc(A, k2, nr));
+    if (!one_word) {
+        LV_TRY(arena_alloc(A, k1, nr));
+        LV_TRY(arena_alloc(A, k2, nr));
+    }
     LV_TRY(arena_alloc(A, p1, nr));
     LV_TRY(arena_alloc(A, p2, nr));
-    launch_rev_entries(s, c->d_src, c->d_dst, c->d_w, slscan, e, wbits, k1, p1);
     uint32_t *sk = nullptr;
     uint64_t *sp = nullptr;
-    LV_TRY(radix_sort_u32_u64(A, s, k1, p1, k2, p2, nr, bits_for(n > 0 ? (uint64_t)(n - 1) : 0),
-                              &sk, &sp));
-    launch_rowptr_search(s, sk, nr, n, rs);
+    int fbits = wbits, vshift = 0;
+    if (one_word) {
+        launch_rev_entries1(s, c->d_src, c->d_dst, c->d_w, slscan, e, kb, p1);
+        LV_TRY(radix_sort_u64_keys(A, s, p1, p2, nr, 64 - kb, &sp));
+        launch_row_starts64(s, sp, nr, n, 64 - kb, rs);
+        fbits = wb1;
+        vshift = 64 - kb;
+    } else {
+        launch_rev_entries(s, c->d_src, c->d_dst, c->d_w, slscan, e, wbits, k1, p1);
+        LV_TRY(radix_sort_u32_u64(A, s, k1, p1, k2, p2, nr, kb, &sk, &sp));
+        launch_rowptr_search(s, sk, nr, n, rs);
+    }
     c->nnz = 2 * nr;
     LV_TRY(arena_alloc(A, c->col, c->nnz));
     LV_TRY(arena_alloc(A, c->cw, c->nnz));
     LV_HIP(hipMemsetAsync(&c->d_flags->err, 0, sizeof(int), s));
-    launch_csr_sorted_fill(s, !canonical, c->d_src, c->d_dst, c->d_w, slscan, e, n, wbits, fs, sk,
-                           sp, nr, rs, c->rowptr, ffwd, c->col, c->cw, c->kdeg, c->d_flags);
+    launch_csr_sorted_fill(s, !canonical, c->d_src, c->d_dst, c->d_w, slscan, e, n, fbits, vshift,
+                           kb, fs, sk, sp, nr, rs, c->rowptr, ffwd, c->col, c->cw, c->kdeg,
+                           c->d_flags);
     launch_loop_gather(s, lastloop, c->d_w, n, c->loopw);
     LV_HIP(hipGetLastError());
     if (canonical) {

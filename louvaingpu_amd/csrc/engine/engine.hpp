@@ -248,6 +248,9 @@ int radix_sort_u64_i64(Arena &a, hipStream_t s, uint64_t *keys, int64_t *vals, u
 int radix_sort_u32_i32(Arena &a, hipStream_t s, uint32_t *keys, int32_t *vals, uint32_t *ktmp,
                        int32_t *vtmp, int64_t n, int bits, uint32_t **keys_out,
                        int32_t **vals_out);
+// Sorts 64-bit keys by their bits [lo, 64), stably, without values.
+int radix_sort_u64_keys(Arena &a, hipStream_t s, uint64_t *keys, uint64_t *ktmp, int64_t n,
+                        int lo, uint64_t **keys_out);
 int radix_sort_u32_u32(Arena &a, hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *ktmp,
                        uint32_t *vtmp, int64_t n, int bits, uint32_t **keys_out,
                        uint32_t **vals_out);
@@ -341,9 +344,16 @@ void launch_rev_entries(hipStream_t s, const int32_t *src, const int32_t *dst, c
                         uint64_t *payload);
 void launch_csr_sorted_fill(hipStream_t s, bool check, const int32_t *src, const int32_t *dst,
                             const int64_t *w, const int32_t *slscan, int64_t e, int32_t n,
-                            int wbits, const int64_t *fs, const uint32_t *skey,
-                            const uint64_t *spay, int64_t nr, const int64_t *rs, int64_t *rowptr,
-                            int64_t *ffwd, int32_t *col, int64_t *cw, int64_t *kdeg, DevFlags *f);
+                            int wbits, int vshift, int kb, const int64_t *fs,
+                            const uint32_t *skey, const uint64_t *spay, int64_t nr,
+                            const int64_t *rs, int64_t *rowptr, int64_t *ffwd, int32_t *col,
+                            int64_t *cw, int64_t *kdeg, DevFlags *f);
+// One-word reverse entries dst << (64 - kb) | src << (64 - 2 kb) | w, and row
+// starts over such words sorted by their top bits (key = word >> kshift).
+void launch_rev_entries1(hipStream_t s, const int32_t *src, const int32_t *dst, const int64_t *w,
+                         const int32_t *slscan, int64_t e, int kb, uint64_t *words);
+void launch_row_starts64(hipStream_t s, const uint64_t *keys, int64_t total, int32_t n,
+                         int kshift, int64_t *rowptr);
 // flags->err |= 4 when some row's columns are not strictly increasing.
 void launch_rows_check(hipStream_t s, const uint32_t *skeys, const int32_t *col, int64_t nnz,
                        DevFlags *f);

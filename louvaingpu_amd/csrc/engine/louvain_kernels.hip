@@ -278,10 +278,10 @@ __device__ __forceinline__ void wave_range_fill(long long lo, long long hi, long
 // ascending keys: 0 up to the first key, total past the last one.  (The
 // ranges between keys are written where the keys change.)
 template <typename K>
-__global__ void k_range_ends(const K *keys, int64_t total, int32_t n, int64_t *out)
+__global__ void k_range_ends(const K *keys, int64_t total, int32_t n, int kshift, int64_t *out)
 {
-    const int64_t k0 = min((int64_t)keys[0], (int64_t)n);
-    const int64_t k1 = min((int64_t)keys[total - 1], (int64_t)n);
+    const int64_t k0 = min((int64_t)(keys[0] >> kshift), (int64_t)n);
+    const int64_t k1 = min((int64_t)(keys[total - 1] >> kshift), (int64_t)n);
     GRID_STRIDE(v, (int64_t)n + 1)
     {
         if (v <= k0)
@@ -345,6 +345,22 @@ __global__ void k_rev_entries(const int32_t *src, const int32_t *dst, const int6
     }
 }
 
+// The same as one word per entry, dst << (64 - kb) | src << wb | w with
+// kb = bits of the largest id and wb = 64 - 2 kb (when every weight fits wb
+// bits): the row sort moves 8 bytes per entry instead of 12.
+__global__ void k_rev_entries1(const int32_t *src, const int32_t *dst, const int64_t *w,
+                               const int32_t *slscan, int64_t e, int kb, uint64_t *words)
+{
+    const int wb = 64 - 2 * kb;
+    GRID_STRIDE(i, e)
+    {
+        const int32_t a = src[i], b = dst[i];
+        if (a == b) continue;
+        words[i - slscan[i]] = ((uint64_t)(uint32_t)b << (64 - kb)) |
+                               ((uint64_t)(uint32_t)a << wb) | (uint64_t)w[i];
+    }
+}
+
 // rowptr[v] = reverse entries of rows < v + forward (non-self) entries of rows
 // < v; ffwd[v] = rowptr[v] + (reverse entries of v) - (non-self edges before
 // fs[v]), so forward edge i of row v lands at ffwd[v] + (i - slscan[i]).
@@ -364,11 +380,15 @@ __global__ void k_rowptr_sum(const int64_t *rs, const int64_t *fs, const int32_t
 // strictly increase (err bit 4): within the reverse part, within the forward
 // part (skipping self-loop edges), and from the last reverse column to the
 // first forward one.  Without CHECK the list is canonical and strictly sorted.
+// Reverse entries come as (skey[r], spay[r] = src << wbits | w), or with skey
+// null as one word spay[r] = row << vshift | src << wbits | w; columns are
+// (spay >> wbits) & cmask either way.
 template <bool CHECK>
 __global__ __launch_bounds__(256) void k_fill_rev(const uint32_t *skey, const uint64_t *spay,
-                                                 int64_t nr, int wbits, const int64_t *rs,
-                                                 const int64_t *rowptr, int32_t *col, int64_t *cw,
-                                                 int64_t *kdeg, DevFlags *f)
+                                                 int64_t nr, int wbits, int vshift, uint64_t cmask,
+                                                 const int64_t *rs, const int64_t *rowptr,
+                                                 int32_t *col, int64_t *cw, int64_t *kdeg,
+                                                 DevFlags *f)
 {
     int bad = 0;
     constexpr int64_t SPAN = 256 * kSpanRounds;
@@ -382,17 +402,22 @@ __global__ __launch_bounds__(256) void k_fill_rev(const uint32_t *skey, const ui
             long long row = 0;
             ull wp = 0;
             if (valid) {
-                const uint32_t v = skey[r];
                 const uint64_t x = spay[r];
+                const uint32_t v = skey ? skey[r] : (uint32_t)(x >> vshift);
                 const int64_t u = (int64_t)(x & ((1ull << wbits) - 1ull));
                 const int64_t pos = rowptr[v] + (r - rs[v]);
-                const int32_t cc = (int32_t)(x >> wbits);
+                const int32_t cc = (int32_t)((x >> wbits) & cmask);
                 col[pos] = cc;
                 cw[pos] = u;
                 row = v;
                 wp = (ull)(u == 0 ? 1 : u);
-                if constexpr (CHECK)
-                    if (r > 0 && skey[r - 1] == v && (int32_t)(spay[r - 1] >> wbits) >= cc) bad = 4;
+                if constexpr (CHECK) {
+                    if (r > 0) {
+                        const uint64_t px = spay[r - 1];
+                        const uint32_t pv = skey ? skey[r - 1] : (uint32_t)(px >> vshift);
+                        if (pv == v && (int32_t)((px >> wbits) & cmask) >= cc) bad = 4;
+                    }
+                }
             }
             sc.add(valid, row, wp, (ull *)kdeg);
         }
@@ -405,7 +430,7 @@ template <bool CHECK>
 __global__ __launch_bounds__(256) void k_fill_fwd(const int32_t *src, const int32_t *dst,
                                                  const int64_t *w, const int32_t *slscan, int64_t e,
                                                  const int64_t *fs, const int64_t *rs,
-                                                 const uint64_t *spay, int wbits,
+                                                 const uint64_t *spay, int wbits, uint64_t cmask,
                                                  const int64_t *ffwd, int32_t *col, int64_t *cw,
                                                  int64_t *kdeg, DevFlags *f)
 {
@@ -438,7 +463,7 @@ __global__ __launch_bounds__(256) void k_fill_fwd(const int32_t *src, const int3
                             while (src[k] == dst[k]) --k;
                             if (dst[k] >= b) bad = 4;
                         } else if (rs[v + 1] > rs[v] &&
-                                   (int32_t)(spay[rs[v + 1] - 1] >> wbits) >= b) {
+                                   (int32_t)((spay[rs[v + 1] - 1] >> wbits) & cmask) >= b) {
                             bad = 4;
                         }
                     }
@@ -459,7 +484,7 @@ void launch_src_sorted(hipStream_t s, const int32_t *src, const int32_t *dst, co
         hipLaunchKernelGGL(k_src_sorted, grid_for(e, 256, 8192), 256, 0, s, src, dst, w, e,
                            self, fs, kdeg, selfsum, lastloop, f);
         hipLaunchKernelGGL(k_range_ends<int32_t>, grid_for((int64_t)n + 1, 256, 1 << 20), 256, 0,
-                           s, src, e, n, fs);
+                           s, src, e, n, 0, fs);
     } else {
         fill_i64(s, fs, (int64_t)n + 1, 0);
     }
@@ -474,44 +499,74 @@ void launch_rev_entries(hipStream_t s, const int32_t *src, const int32_t *dst, c
                            slscan, e, wbits, keys, payload);
 }
 
+// Reverse entries as (skey, spay = src << wbits | w), or with skey null as one
+// word spay = row << vshift | src << wbits | w; kb = bits of the largest id.
 void launch_csr_sorted_fill(hipStream_t s, bool check, const int32_t *src, const int32_t *dst,
                             const int64_t *w, const int32_t *slscan, int64_t e, int32_t n,
-                            int wbits, const int64_t *fs, const uint32_t *skey,
-                            const uint64_t *spay, int64_t nr, const int64_t *rs, int64_t *rowptr,
-                            int64_t *ffwd, int32_t *col, int64_t *cw, int64_t *kdeg, DevFlags *f)
+                            int wbits, int vshift, int kb, const int64_t *fs,
+                            const uint32_t *skey, const uint64_t *spay, int64_t nr,
+                            const int64_t *rs, int64_t *rowptr, int64_t *ffwd, int32_t *col,
+                            int64_t *cw, int64_t *kdeg, DevFlags *f)
 {
+    const uint64_t cmask = kb >= 64 ? ~0ull : ((1ull << kb) - 1ull);
     hipLaunchKernelGGL(k_rowptr_sum, grid_for((int64_t)n + 1, 256, 1 << 20), 256, 0, s, rs, fs,
                        slscan, n, rowptr, ffwd);
+    constexpr int SPAN = 256 * kSpanRounds;
     if (nr > 0) {
         if (check)
-            hipLaunchKernelGGL(k_fill_rev<true>, grid_for(nr, 256 * kSpanRounds, 8192), 256, 0, s, skey, spay,
-                               nr, wbits, rs, rowptr, col, cw, kdeg, f);
+            hipLaunchKernelGGL(k_fill_rev<true>, grid_for(nr, SPAN, 8192), 256, 0, s, skey, spay, nr,
+                               wbits, vshift, cmask, rs, rowptr, col, cw, kdeg, f);
         else
-            hipLaunchKernelGGL(k_fill_rev<false>, grid_for(nr, 256 * kSpanRounds, 1 << 20), 256, 0, s, skey, spay,
-                               nr, wbits, rs, rowptr, col, cw, kdeg, f);
+            hipLaunchKernelGGL(k_fill_rev<false>, grid_for(nr, SPAN, 1 << 20), 256, 0, s, skey, spay,
+                               nr, wbits, vshift, cmask, rs, rowptr, col, cw, kdeg, f);
     }
     if (e > 0) {
         if (check)
-            hipLaunchKernelGGL(k_fill_fwd<true>, grid_for(e, 256 * kSpanRounds, 8192), 256, 0, s, src, dst, w,
-                               slscan, e, fs, rs, spay, wbits, ffwd, col, cw, kdeg, f);
+            hipLaunchKernelGGL(k_fill_fwd<true>, grid_for(e, SPAN, 8192), 256, 0, s, src, dst, w,
+                               slscan, e, fs, rs, spay, wbits, cmask, ffwd, col, cw, kdeg, f);
         else
-            hipLaunchKernelGGL(k_fill_fwd<false>, grid_for(e, 256 * kSpanRounds, 1 << 20), 256, 0, s, src, dst, w,
-                               slscan, e, fs, rs, spay, wbits, ffwd, col, cw, kdeg, f);
+            hipLaunchKernelGGL(k_fill_fwd<false>, grid_for(e, SPAN, 1 << 20), 256, 0, s, src, dst, w,
+                               slscan, e, fs, rs, spay, wbits, cmask, ffwd, col, cw, kdeg, f);
     }
 }
 
-// out[v] = the first position r with min(keys[r], n) >= v (v in [0, n]) of
-// ascending keys, between the first and the last key: each change of key
-// writes the rows it passes (k_range_ends writes the two ends).
-__global__ void k_row_starts(const uint32_t *keys, int64_t total, int32_t n, int64_t *out)
+void launch_rev_entries1(hipStream_t s, const int32_t *src, const int32_t *dst, const int64_t *w,
+                         const int32_t *slscan, int64_t e, int kb, uint64_t *words)
+{
+    if (e > 0)
+        hipLaunchKernelGGL(k_rev_entries1, grid_for(e, 256, 1 << 20), 256, 0, s, src, dst, w,
+                           slscan, e, kb, words);
+}
+
+template <typename K>
+__global__ void k_row_starts(const K *keys, int64_t total, int32_t n, int kshift, int64_t *out);
+
+void launch_row_starts64(hipStream_t s, const uint64_t *keys, int64_t total, int32_t n,
+                         int kshift, int64_t *rowptr)
+{
+    if (total > 0) {
+        hipLaunchKernelGGL(k_row_starts<uint64_t>, grid_for(total, 256, 1 << 20), 256, 0, s, keys,
+                           total, n, kshift, rowptr);
+        hipLaunchKernelGGL(k_range_ends<uint64_t>, grid_for((int64_t)n + 1, 256, 1 << 20), 256, 0,
+                           s, keys, total, n, kshift, rowptr);
+    } else {
+        fill_i64(s, rowptr, (int64_t)n + 1, 0);
+    }
+}
+
+// out[v] = the first position r with min(keys[r] >> kshift, n) >= v (v in
+// [0, n]) of ascending keys, between the first and the last key: each change
+// of key writes the rows it passes (k_range_ends writes the two ends).
+template <typename K>
+__global__ void k_row_starts(const K *keys, int64_t total, int32_t n, int kshift, int64_t *out)
 {
     for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < total;
          base += (int64_t)gridDim.x * blockDim.x) {
         const int64_t r = base + threadIdx.x;
         long long lo = 0, hi = 0;
         if (r < total && r > 0) {
-            lo = min((int64_t)keys[r - 1], (int64_t)n);
-            hi = min((int64_t)keys[r], (int64_t)n);
+            lo = min((int64_t)(keys[r - 1] >> kshift), (int64_t)n);
+            hi = min((int64_t)(keys[r] >> kshift), (int64_t)n);
         }
         wave_range_fill(lo, hi, (long long)r, out);
     }
@@ -590,10 +645,10 @@ void launch_rowptr_search(hipStream_t s, const uint32_t *skeys, int64_t total, i
                           int64_t *rowptr)
 {
     if (total > 0) {
-        hipLaunchKernelGGL(k_row_starts, grid_for(total, 256, 1 << 20), 256, 0, s, skeys, total, n,
-                           rowptr);
+        hipLaunchKernelGGL(k_row_starts<uint32_t>, grid_for(total, 256, 1 << 20), 256, 0, s, skeys,
+                           total, n, 0, rowptr);
         hipLaunchKernelGGL(k_range_ends<uint32_t>, grid_for((int64_t)n + 1, 256, 1 << 20), 256, 0,
-                           s, skeys, total, n, rowptr);
+                           s, skeys, total, n, 0, rowptr);
     } else
         fill_i64(s, rowptr, (int64_t)n + 1, 0);
 }

@@ -7,6 +7,7 @@
 // All kernels assume 64-lane wavefronts (gfx950).
 #include <cstdio>
 #include <cstring>
+#include <type_traits>
 
 #include "engine.hpp"
 
@@ -377,19 +378,23 @@ __device__ __forceinline__ int block_exclusive_scan_256(int x, int *s_wave)
     return pre + inc - x;
 }
 
+// Keys without values (V = NoVal): only the keys move.
+struct NoVal {};
+
 template <typename K, typename V>
 __global__ __launch_bounds__(kRadixBlock) void k_radix_scatter(const K *kin, const V *vin, K *kout,
                                                               V *vout, int64_t n, int shift,
                                                               int rounds, const int64_t *hscan,
                                                               int64_t nblocks)
 {
+    constexpr bool HV = !std::is_same<V, NoVal>::value;
     constexpr int NW = kRadixBlock / 64;
     __shared__ int64_t s_base[256];  // next output position of each digit for this block
     __shared__ int s_cnt[NW][256];   // per-wave digit counts, then tile offsets
     __shared__ int s_start[256];     // first tile position of each digit
     __shared__ int s_wave[NW];
     __shared__ K s_k[kRadixTile];
-    __shared__ V s_v[kRadixTile];
+    __shared__ V s_v[HV ? kRadixTile : 1];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     s_base[tid] = hscan[(int64_t)tid * nblocks + blockIdx.x];
     const ull lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
@@ -408,7 +413,7 @@ __global__ __launch_bounds__(kRadixBlock) void k_radix_scatter(const K *kin, con
             const int64_t i = r0 + wave * 64 * kRadixSub + u * 64 + lane;
             valid[u] = i < end;
             k[u] = valid[u] ? kin[i] : (K)0;
-            v[u] = valid[u] ? vin[i] : (V)0;
+            if constexpr (HV) v[u] = valid[u] ? vin[i] : (V)0;
             d[u] = (int)((k[u] >> shift) & 255);
         }
 #pragma unroll
@@ -442,7 +447,7 @@ __global__ __launch_bounds__(kRadixBlock) void k_radix_scatter(const K *kin, con
             if (valid[u]) {
                 const int lp = s_cnt[wave][d[u]] + rank[u];
                 s_k[lp] = k[u];
-                s_v[lp] = v[u];
+                if constexpr (HV) s_v[lp] = v[u];
             }
         }
         __syncthreads();
@@ -452,7 +457,7 @@ __global__ __launch_bounds__(kRadixBlock) void k_radix_scatter(const K *kin, con
             const int dd = (int)((kk >> shift) & 255);
             const int64_t dst = s_base[dd] + (t - s_start[dd]);
             kout[dst] = kk;
-            vout[dst] = s_v[t];
+            if constexpr (HV) vout[dst] = s_v[t];
         }
         __syncthreads();
         s_base[tid] += tot;
@@ -461,11 +466,11 @@ __global__ __launch_bounds__(kRadixBlock) void k_radix_scatter(const K *kin, con
 
 template <typename K, typename V>
 static int radix_sort(Arena &a, hipStream_t s, K *keys, V *vals, K *ktmp, V *vtmp, int64_t n,
-                      int bits, K **kout, V **vout)
+                      int bits, K **kout, V **vout, int lo = 0)
 {
     *kout = keys;
     *vout = vals;
-    if (n <= 1 || bits <= 0) return LV_OK;
+    if (n <= 1 || bits <= lo) return LV_OK;
     // at least one full round per block, at most 8192 blocks: the digit-major
     // histogram (256 x blocks) and its scan stay small next to the data
     int rounds = kRadixSub;
@@ -476,7 +481,7 @@ static int radix_sort(Arena &a, hipStream_t s, K *keys, V *vals, K *ktmp, V *vtm
     LV_TRY(arena_alloc(a, hist, 256 * nblocks));
     K *ka = keys, *kb = ktmp;
     V *va = vals, *vb = vtmp;
-    for (int shift = 0; shift < bits; shift += 8) {
+    for (int shift = lo; shift < bits; shift += 8) {
         hipLaunchKernelGGL(k_radix_hist<K>, (int)nblocks, kRadixBlock, 0, s, ka, n, shift, rounds,
                            hist, nblocks);
         int rc = scan_exclusive_i64(a, s, hist, hist, 256 * nblocks, nullptr);
@@ -522,6 +527,13 @@ int radix_sort_u32_u64(Arena &a, hipStream_t s, uint32_t *keys, uint64_t *vals, 
 {
     return radix_sort<uint32_t, uint64_t>(a, s, keys, vals, ktmp, vtmp, n, bits, keys_out,
                                           vals_out);
+}
+
+int radix_sort_u64_keys(Arena &a, hipStream_t s, uint64_t *keys, uint64_t *ktmp, int64_t n,
+                        int lo, uint64_t **keys_out)
+{
+    NoVal *none = nullptr, *out = nullptr;
+    return radix_sort<uint64_t, NoVal>(a, s, keys, none, ktmp, none, n, 64, keys_out, &out, lo);
 }
 
 int radix_sort_u32_u32(Arena &a, hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *ktmp,
