@@ -493,6 +493,38 @@ int build_level0_csr(lv_ctx *c)
     } else {
         launch_self_loops(s, c->d_src, c->d_dst, c->d_w, e, c->kdeg, c->selfsum, lastloop);
     }
+    // one word per entry (row | col | w) when the weight fits the bits the row
+    // key (self-loops keyed n) and the column leave: 8 bytes per sorted entry
+    const int cb = bits_for(n > 0 ? (uint64_t)(n - 1) : 0);
+    const int wb1 = 64 - bits - cb;
+    const bool one_word = bits >= 1 && wb1 >= 1 && c->max_units0 < (1ull << wb1);
+    if (one_word) {
+        uint64_t *x1 = nullptr, *x2 = nullptr, *sx = nullptr;
+        LV_TRY(arena_alloc(A, x1, total));
+        LV_TRY(arena_alloc(A, x2, total));
+        launch_csr_entries1(s, c->d_src, c->d_dst, c->d_w, e, n, bits, cb, x1);
+        LV_TRY(radix_sort_u64_keys(A, s, x1, x2, total, 64 - bits, &sx));
+        launch_row_starts64(s, sx, total, n, 64 - bits, c->rowptr);
+        LV_TRY(d2h(c, &c->h_small[0], (const long long *)(c->rowptr + n), 1));
+        LV_TRY(sync(c));
+        c->nnz = c->h_small[0];
+        LV_TRY(arena_alloc(A, c->col, c->nnz));
+        LV_TRY(arena_alloc(A, c->cw, c->nnz));
+        LV_HIP(hipMemsetAsync(&c->d_flags->err, 0, sizeof(int), s));
+        launch_csr_unpack1(s, sx, c->nnz, bits, cb, c->col, c->cw, c->kdeg, c->d_flags);
+        launch_loop_gather(s, lastloop, c->d_w, n, c->loopw);
+        LV_HIP(hipGetLastError());
+        LV_TRY(d2h(c, c->h_flags, c->d_flags, 1));
+        LV_TRY(sync(c));
+        c->rows_unique = (c->h_flags->err & 4) == 0;
+        t_end(c, t0, CAT_CSR);
+        arena_free(A, lastloop);
+        arena_free(A, x1);
+        arena_free(A, x2);
+        c->n = n;
+        c->m = c->m0;
+        return LV_OK;
+    }
     LV_TRY(arena_alloc(A, k1, total));
     LV_TRY(arena_alloc(A, k2, total));
     if (packed) {

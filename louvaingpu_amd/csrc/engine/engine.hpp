@@ -350,6 +350,12 @@ void launch_csr_sorted_fill(hipStream_t s, bool check, const int32_t *src, const
                             int64_t *cw, int64_t *kdeg, DevFlags *f);
 // One-word reverse entries dst << (64 - kb) | src << (64 - 2 kb) | w, and row
 // starts over such words sorted by their top bits (key = word >> kshift).
+// General path as one word per entry (row | col | w, row = n for self-loops);
+// the unpack also checks that every row's columns strictly increase (err bit 4).
+void launch_csr_entries1(hipStream_t s, const int32_t *src, const int32_t *dst, const int64_t *w,
+                         int64_t e, int32_t n, int rb, int cb, uint64_t *words);
+void launch_csr_unpack1(hipStream_t s, const uint64_t *words, int64_t nnz, int rb, int cb,
+                        int32_t *col, int64_t *cw, int64_t *kdeg, DevFlags *f);
 void launch_rev_entries1(hipStream_t s, const int32_t *src, const int32_t *dst, const int64_t *w,
                          const int32_t *slscan, int64_t e, int kb, uint64_t *words);
 void launch_row_starts64(hipStream_t s, const uint64_t *keys, int64_t total, int32_t n,

@@ -173,6 +173,80 @@ __global__ void k_csr_entries(const int32_t *src, const int32_t *dst, int64_t e,
 }
 
 // payload = col << wbits | w (the host checks that w < 2^wbits and col < 2^(64-wbits)).
+// Both entries of every edge as one word, row << (64 - rb) | col << wb | w with
+// rb = bits of n (self-loops keyed n, past every row), cb = bits of n - 1 and
+// wb = 64 - rb - cb (when every weight fits wb bits).
+__global__ void k_csr_entries1(const int32_t *src, const int32_t *dst, const int64_t *w, int64_t e,
+                               int32_t n, int rb, int cb, uint64_t *words)
+{
+    const int wb = 64 - rb - cb;
+    GRID_STRIDE(i, e)
+    {
+        const int32_t a = src[i], b = dst[i];
+        const uint64_t u = (uint64_t)w[i];
+        const bool self = (a == b);
+        const uint64_t ka = self ? (uint64_t)n : (uint64_t)(uint32_t)a;
+        const uint64_t kb = self ? (uint64_t)n : (uint64_t)(uint32_t)b;
+        words[2 * i] = (ka << (64 - rb)) | ((uint64_t)(uint32_t)b << wb) | u;
+        words[2 * i + 1] = (kb << (64 - rb)) | ((uint64_t)(uint32_t)a << wb) | u;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_csr_unpack1(const uint64_t *words, int64_t nnz, int rb,
+                                                    int cb, int32_t *col, int64_t *cw,
+                                                    int64_t *kdeg, DevFlags *f)
+{
+    const int wb = 64 - rb - cb;
+    const uint64_t cmask = (1ull << cb) - 1ull, wmask = (1ull << wb) - 1ull;
+    constexpr int64_t SPAN = 256 * kSpanRounds;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int bad = 0;
+    SegCarry sc;
+    for (int64_t base = (int64_t)blockIdx.x * SPAN; base < nnz; base += (int64_t)gridDim.x * SPAN) {
+        const int64_t w0 = base + (int64_t)wave * 64 * kSpanRounds;
+        for (int rr = 0; rr < kSpanRounds; ++rr) {
+            const int64_t p = w0 + rr * 64 + lane;
+            const bool valid = p < nnz;
+            long long row = 0;
+            ull wp = 0;
+            if (valid) {
+                const uint64_t x = words[p];
+                const int64_t u = (int64_t)(x & wmask);
+                const int32_t cc = (int32_t)((x >> wb) & cmask);
+                col[p] = cc;
+                cw[p] = u;
+                row = (long long)(x >> (64 - rb));
+                wp = (ull)(u == 0 ? 1 : u);
+                // rows must list their columns strictly increasing (err bit 4)
+                if (p > 0) {
+                    const uint64_t px = words[p - 1];
+                    if ((px >> (64 - rb)) == (uint64_t)row && (int32_t)((px >> wb) & cmask) >= cc)
+                        bad = 4;
+                }
+            }
+            sc.add(valid, row, wp, (ull *)kdeg);
+        }
+        sc.flush((ull *)kdeg);
+    }
+    block_flag_or(bad, f);
+}
+
+void launch_csr_entries1(hipStream_t s, const int32_t *src, const int32_t *dst, const int64_t *w,
+                         int64_t e, int32_t n, int rb, int cb, uint64_t *words)
+{
+    if (e > 0)
+        hipLaunchKernelGGL(k_csr_entries1, grid_for(e, 256, 1 << 20), 256, 0, s, src, dst, w, e, n,
+                           rb, cb, words);
+}
+
+void launch_csr_unpack1(hipStream_t s, const uint64_t *words, int64_t nnz, int rb, int cb,
+                        int32_t *col, int64_t *cw, int64_t *kdeg, DevFlags *f)
+{
+    if (nnz > 0)
+        hipLaunchKernelGGL(k_csr_unpack1, grid_for(nnz, 256 * kSpanRounds, 8192), 256, 0, s, words,
+                           nnz, rb, cb, col, cw, kdeg, f);
+}
+
 __global__ void k_csr_entries_packed(const int32_t *src, const int32_t *dst, const int64_t *w,
                                      int64_t e, int32_t n, int wbits, uint32_t *keys,
                                      uint64_t *payload)
