@@ -209,8 +209,8 @@ __global__ __launch_bounds__(256) void k_cr_gather(const int64_t *estart, const 
     int64_t uu[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        cc[r] = ok[r] ? col[pp[r]] : 0;
-        uu[r] = ok[r] ? cw[pp[r]] : 0;
+        cc[r] = ok[r] ? __builtin_nontemporal_load(&col[pp[r]]) : 0;
+        uu[r] = ok[r] ? __builtin_nontemporal_load(&cw[pp[r]]) : 0;
     }
 #pragma unroll
     for (int r = 0; r < R; ++r)
@@ -219,12 +219,12 @@ __global__ __launch_bounds__(256) void k_cr_gather(const int64_t *estart, const 
     for (int r = 0; r < R; ++r) {
         const int64_t o = b * kGatherT + r * 256 + threadIdx.x;
         if (ok[r]) {
-            ey[o] = cc[r];
+            __builtin_nontemporal_store(cc[r], &ey[o]);
             const int64_t wp = uu[r] == 0 ? 1 : uu[r];
             if (ew32)
-                ew32[o] = (uint32_t)wp;
+                __builtin_nontemporal_store((uint32_t)wp, &ew32[o]);
             else
-                ew[o] = wp;
+                __builtin_nontemporal_store(wp, &ew[o]);
         }
     }
 }

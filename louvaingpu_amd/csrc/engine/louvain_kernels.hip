@@ -1198,8 +1198,8 @@ __device__ __forceinline__ void insert_row(const SweepArgs &a, int32_t *key, ull
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int64_t p = p0 + (int64_t)u * S;
-            j[u] = (p < end) ? a.col[p] : -1;
-            w[u] = (p < end) ? a.w[p] : 0;
+            j[u] = (p < end) ? __builtin_nontemporal_load(&a.col[p]) : -1;
+            w[u] = (p < end) ? __builtin_nontemporal_load(&a.w[p]) : 0;
         }
         if constexpr (DL) {
             Lab L[4];
@@ -1428,8 +1428,8 @@ __global__ __launch_bounds__(256) void k_sweep1_group(SweepArgs a, const int32_t
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const int64_t p = p0 + (int64_t)u * G;
-                    j[u] = (p < end) ? a.col[p] : -1;
-                    w[u] = (p < end) ? a.w[p] : 0;
+                    j[u] = (p < end) ? __builtin_nontemporal_load(&a.col[p]) : -1;
+                    w[u] = (p < end) ? __builtin_nontemporal_load(&a.w[p]) : 0;
                 }
 #pragma unroll
                 for (int u = 0; u < 4; ++u) d[u] = (j[u] >= 0) ? deg_of(a, j[u]) : 0;
@@ -1472,8 +1472,8 @@ __global__ __launch_bounds__(BLOCK) void k_sweep1_block(SweepArgs a, const int32
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const int64_t p = p0 + (int64_t)u * BLOCK;
-                j[u] = (p < end) ? a.col[p] : -1;
-                w[u] = (p < end) ? a.w[p] : 0;
+                j[u] = (p < end) ? __builtin_nontemporal_load(&a.col[p]) : -1;
+                w[u] = (p < end) ? __builtin_nontemporal_load(&a.w[p]) : 0;
             }
 #pragma unroll
             for (int u = 0; u < 4; ++u) d[u] = (j[u] >= 0) ? deg_of(a, j[u]) : 0;
@@ -1569,8 +1569,8 @@ __global__ __launch_bounds__(256) void k_hub1_chunk(SweepArgs a, HubArgs h)
         for (int u = 0; u < 4; ++u) {
             const int64_t t = p0 + (int64_t)u * 256;
             own[u] = (t == d);
-            jj[u] = (t < t1 && t < d) ? a.col[rb + t] : -1;
-            w[u] = (jj[u] >= 0) ? a.w[rb + t] : 0;
+            jj[u] = (t < t1 && t < d) ? __builtin_nontemporal_load(&a.col[rb + t]) : -1;
+            w[u] = (jj[u] >= 0) ? __builtin_nontemporal_load(&a.w[rb + t]) : 0;
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) dg[u] = (jj[u] >= 0) ? deg_of(a, jj[u]) : 0;
