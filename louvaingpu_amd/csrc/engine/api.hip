@@ -359,23 +359,26 @@ double current_q(lv_ctx *c, int &rc)
 // ---------------------------------------------------------------------------
 // phases
 
-// Level-0 CSR for edges sorted by source (self[i] = edge i is a self-loop, fs[v]
-// = first edge with src >= v): forward entries stay in edge order, reverse
-// entries are radix-sorted by destination, row v = its reverse entries then
-// its forward entries.  A canonical, strictly sorted list (canonical) gives rows
-// that list each neighbour once in increasing order; otherwise the fills check.
-int build_level0_csr_sorted(lv_ctx *c, int32_t *self, const int64_t *fs, bool canonical,
-                            int wbits, const ull *lastloop)
+// Level-0 CSR for edges sorted by source (tself/vself = self-loop edges per edge
+// tile and per source vertex, fs[v] = first edge with src >= v): forward
+// entries stay in edge order, reverse entries are radix-sorted by destination,
+// row v = its reverse entries then its forward entries.  A canonical, strictly
+// sorted list (canonical) gives rows that list each neighbour once in
+// increasing order; otherwise the fills check.
+int build_level0_csr_sorted(lv_ctx *c, int32_t *tself, int32_t *vself, const int64_t *fs,
+                            bool canonical, int wbits, const ull *lastloop)
 {
     Arena &A = c->arena;
     hipStream_t s = c->stream;
     const int32_t n = c->n0;
     const int64_t e = c->e0;
-    int32_t *slscan = nullptr;
-    LV_TRY(arena_alloc(A, slscan, e + 1));
-    LV_TRY(scan_exclusive_i32(A, s, self, slscan, e, slscan + e));
+    const int64_t ntiles = (e + kEdgeTile - 1) / kEdgeTile;
+    // in place: tscan[t] = self-loop edges before tile t, vscan[v] = those with src < v
+    int32_t *tscan = tself, *vscan = vself;
+    LV_TRY(scan_exclusive_i32(A, s, tself, tscan, ntiles, tscan + ntiles));
+    LV_TRY(scan_exclusive_i32(A, s, vself, vscan, n, vscan + n));
     int32_t nself = 0;
-    LV_TRY(d2h(c, &nself, (const int32_t *)(slscan + e), 1));
+    LV_TRY(d2h(c, &nself, (const int32_t *)(tscan + ntiles), 1));
     LV_TRY(sync(c));
     const int64_t nr = e - nself;
     const int kb = bits_for(n > 0 ? (uint64_t)(n - 1) : 0);
@@ -398,13 +401,13 @@ int build_level0_csr_sorted(lv_ctx *c, int32_t *self, const int64_t *fs, bool ca
     uint64_t *sp = nullptr;
     int fbits = wbits, vshift = 0;
     if (one_word) {
-        launch_rev_entries1(s, c->d_src, c->d_dst, c->d_w, slscan, e, kb, p1);
+        launch_rev_entries1(s, c->d_src, c->d_dst, c->d_w, tscan, e, kb, p1);
         LV_TRY(radix_sort_u64_keys(A, s, p1, p2, nr, 64 - kb, &sp));
         launch_row_starts64(s, sp, nr, n, 64 - kb, rs);
         fbits = wb1;
         vshift = 64 - kb;
     } else {
-        launch_rev_entries(s, c->d_src, c->d_dst, c->d_w, slscan, e, wbits, k1, p1);
+        launch_rev_entries(s, c->d_src, c->d_dst, c->d_w, tscan, e, wbits, k1, p1);
         LV_TRY(radix_sort_u32_u64(A, s, k1, p1, k2, p2, nr, kb, &sk, &sp));
         launch_rowptr_search(s, sk, nr, n, rs);
     }
@@ -412,8 +415,8 @@ int build_level0_csr_sorted(lv_ctx *c, int32_t *self, const int64_t *fs, bool ca
     LV_TRY(arena_alloc(A, c->col, c->nnz));
     LV_TRY(arena_alloc(A, c->cw, c->nnz));
     LV_HIP(hipMemsetAsync(&c->d_flags->err, 0, sizeof(int), s));
-    launch_csr_sorted_fill(s, !canonical, c->d_src, c->d_dst, c->d_w, slscan, e, n, fbits, vshift,
-                           kb, fs, sk, sp, nr, rs, c->rowptr, ffwd, c->col, c->cw, c->kdeg,
+    launch_csr_sorted_fill(s, !canonical, c->d_src, c->d_dst, c->d_w, tscan, vscan, e, n, fbits,
+                           vshift, kb, fs, sk, sp, nr, rs, c->rowptr, ffwd, c->col, c->cw, c->kdeg,
                            c->d_flags);
     launch_loop_gather(s, lastloop, c->d_w, n, c->loopw);
     LV_HIP(hipGetLastError());
@@ -428,7 +431,6 @@ int build_level0_csr_sorted(lv_ctx *c, int32_t *self, const int64_t *fs, bool ca
     arena_free(A, k2);
     arena_free(A, p1);
     arena_free(A, p2);
-    arena_free(A, slscan);
     arena_free(A, rs);
     arena_free(A, ffwd);
     return LV_OK;
@@ -466,20 +468,23 @@ int build_level0_csr(lv_ctx *c)
     if (packed && e < INT32_MAX) {
         // edges sorted by source: only the reverse entries need the row sort
         // (the same pass over the edges accumulates the self-loops)
-        int32_t *self = nullptr;
+        int32_t *tself = nullptr, *vself = nullptr;
         int64_t *fs = nullptr;
-        LV_TRY(arena_alloc(A, self, e + 1));
+        LV_TRY(arena_alloc(A, tself, (e + kEdgeTile - 1) / kEdgeTile + 1));
+        LV_TRY(arena_alloc(A, vself, (int64_t)n + 1));
         LV_TRY(arena_alloc(A, fs, (int64_t)n + 1));
         LV_HIP(hipMemsetAsync(&c->d_flags->err, 0, sizeof(int), s));
-        launch_src_sorted(s, c->d_src, c->d_dst, c->d_w, e, n, self, fs, c->kdeg, c->selfsum,
-                          lastloop, c->d_flags);
+        launch_src_sorted(s, c->d_src, c->d_dst, c->d_w, e, n, tself, vself, fs, c->kdeg,
+                          c->selfsum, lastloop, c->d_flags);
         LV_HIP(hipGetLastError());
         LV_TRY(d2h(c, c->h_flags, c->d_flags, 1));
         LV_TRY(sync(c));
         const int err = c->h_flags->err;
         if ((err & 8) == 0) {
-            const int rc = build_level0_csr_sorted(c, self, fs, (err & 16) == 0, wbits, lastloop);
-            arena_free(A, self);
+            const int rc =
+                build_level0_csr_sorted(c, tself, vself, fs, (err & 16) == 0, wbits, lastloop);
+            arena_free(A, tself);
+            arena_free(A, vself);
             arena_free(A, fs);
             arena_free(A, lastloop);
             if (rc != LV_OK) return rc;
@@ -488,7 +493,8 @@ int build_level0_csr(lv_ctx *c)
             c->m = c->m0;
             return LV_OK;
         }
-        arena_free(A, self);
+        arena_free(A, tself);
+        arena_free(A, vself);
         arena_free(A, fs);
     } else {
         launch_self_loops(s, c->d_src, c->d_dst, c->d_w, e, c->kdeg, c->selfsum, lastloop);

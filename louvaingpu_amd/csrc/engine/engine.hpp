@@ -335,16 +335,20 @@ void launch_sweep1_tier(hipStream_t s, int tier, const SweepArgs &a, const int32
                         int32_t nv, bool batch);
 int run_hub_sweep1(hipStream_t s, const SweepArgs &a, const HubArgs &h, int32_t nh,
                    int64_t nchunks, bool batch);
-// CSR build for edges sorted by source (see louvain_kernels.hip).
+// CSR build for edges sorted by source (see louvain_kernels.hip).  Edges are
+// processed in tiles of kEdgeTile; tself[t] (ntiles + 1 ints) and vself[v]
+// (n + 1 ints) receive the self-loop edges of tile t and of source vertex v,
+// and after in-place exclusive scans tscan/vscan are what the later kernels read.
+constexpr int kEdgeTile = 2048;
 void launch_src_sorted(hipStream_t s, const int32_t *src, const int32_t *dst, const int64_t *w,
-                       int64_t e, int32_t n, int32_t *self, int64_t *fs, int64_t *kdeg,
-                       int64_t *selfsum, ull *lastloop, DevFlags *f);
+                       int64_t e, int32_t n, int32_t *tself, int32_t *vself, int64_t *fs,
+                       int64_t *kdeg, int64_t *selfsum, ull *lastloop, DevFlags *f);
 void launch_rev_entries(hipStream_t s, const int32_t *src, const int32_t *dst, const int64_t *w,
-                        const int32_t *slscan, int64_t e, int wbits, uint32_t *keys,
+                        const int32_t *tscan, int64_t e, int wbits, uint32_t *keys,
                         uint64_t *payload);
 void launch_csr_sorted_fill(hipStream_t s, bool check, const int32_t *src, const int32_t *dst,
-                            const int64_t *w, const int32_t *slscan, int64_t e, int32_t n,
-                            int wbits, int vshift, int kb, const int64_t *fs,
+                            const int64_t *w, const int32_t *tscan, const int32_t *vscan,
+                            int64_t e, int32_t n, int wbits, int vshift, int kb, const int64_t *fs,
                             const uint32_t *skey, const uint64_t *spay, int64_t nr,
                             const int64_t *rs, int64_t *rowptr, int64_t *ffwd, int32_t *col,
                             int64_t *cw, int64_t *kdeg, DevFlags *f);
@@ -357,7 +361,7 @@ void launch_csr_entries1(hipStream_t s, const int32_t *src, const int32_t *dst, 
 void launch_csr_unpack1(hipStream_t s, const uint64_t *words, int64_t nnz, int rb, int cb,
                         int32_t *col, int64_t *cw, int64_t *kdeg, DevFlags *f);
 void launch_rev_entries1(hipStream_t s, const int32_t *src, const int32_t *dst, const int64_t *w,
-                         const int32_t *slscan, int64_t e, int kb, uint64_t *words);
+                         const int32_t *tscan, int64_t e, int kb, uint64_t *words);
 void launch_row_starts64(hipStream_t s, const uint64_t *keys, int64_t total, int32_t n,
                          int kshift, int64_t *rowptr);
 // flags->err |= 4 when some row's columns are not strictly increasing.

@@ -365,85 +365,154 @@ __global__ void k_range_ends(const K *keys, int64_t total, int32_t n, int kshift
     }
 }
 
+// Edges go through these kernels in tiles of kEdgeTile consecutive edges, one
+// tile per 256-thread block iteration.  Instead of a per-edge scan of
+// self-loop flags, the pass over the list counts self-loop edges per tile
+// (tself) and per source vertex (vself); after exclusive scans of both,
+// tscan[t] = self-loop edges before tile t and vscan[v] = self-loop edges
+// with src < v (= before fs[v] in a list sorted by source).
+static_assert(kEdgeTile == 256 * kSpanRounds, "an edge tile is one block iteration of the fills");
+
+// Wave w of the block owns edges [tile + w*64*kSpanRounds, +64*kSpanRounds) of
+// tile `t`, round rr holding lanes' edges tile + w*64*kSpanRounds + rr*64 +
+// lane.  Given which of them are self-loops, returns the self-loop edges
+// before the wave's span and sets m[rr] to the ballot of each round, so the
+// self-loop edges before edge (rr, lane) are the returned count plus the
+// bits of m[0..rr) and of m[rr] below the lane.  Every thread of the block
+// calls it once per tile; a tile without self-loops costs two loads.
+__device__ __forceinline__ int32_t tile_self_base(const int32_t *tscan, int64_t t,
+                                                  const bool (&sl)[kSpanRounds],
+                                                  ull (&m)[kSpanRounds], int32_t *s_wc)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int32_t off = tscan[t];
+    const bool any = tscan[t + 1] != off;  // the same for the whole block
+    if (any) {
+        int wc = 0;
+#pragma unroll
+        for (int rr = 0; rr < kSpanRounds; ++rr) {
+            m[rr] = __ballot(sl[rr]);
+            wc += __popcll(m[rr]);
+        }
+        if (lane == 0) s_wc[wave] = wc;
+        __syncthreads();
+        for (int w = 0; w < wave; ++w) off += s_wc[w];
+        __syncthreads();
+    } else {
+#pragma unroll
+        for (int rr = 0; rr < kSpanRounds; ++rr) m[rr] = 0;
+    }
+    return off;
+}
+
 // One pass over the edge list: err bit 8 when src is not non-decreasing, bit
 // 16 when the list is not canonical and strictly sorted (src <= dst and
 // (src, dst) strictly increasing - then every row lists each neighbour once
-// and in increasing order); self[i] = 1 for a self-loop edge, whose weight
-// goes to selfsum/kdeg of its vertex and whose position to lastloop; and, for
+// and in increasing order); a self-loop edge's weight goes to selfsum/kdeg of
+// its vertex, its position to lastloop and its count to tself/vself; and, for
 // a sorted list, fs[v] = the first edge with src >= v between the first and
 // the last source (k_range_ends writes the two ends).
-__global__ void k_src_sorted(const int32_t *src, const int32_t *dst, const int64_t *w, int64_t e,
-                             int32_t *self, int64_t *fs, int64_t *kdeg, int64_t *selfsum,
-                             ull *lastloop, DevFlags *f)
+__global__ __launch_bounds__(256) void k_src_sorted(const int32_t *src, const int32_t *dst,
+                                                   const int64_t *w, int64_t e, int32_t *tself,
+                                                   int32_t *vself, int64_t *fs, int64_t *kdeg,
+                                                   int64_t *selfsum, ull *lastloop, DevFlags *f)
 {
+    __shared__ int32_t s_wc[4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int bad = 0;
-    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < e;
-         base += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t i = base + threadIdx.x;
-        long long lo = 0, hi = 0;
-        if (i < e) {
-            const int32_t a = src[i], b = dst[i];
-            const int32_t pa = i > 0 ? src[i - 1] : -1;
-            if (a < pa) bad |= 8;
-            if (a > b || (a == pa && b <= dst[i - 1])) bad |= 16;
-            self[i] = (a == b) ? 1 : 0;
-            if (a == b) {
-                const int64_t u = w[i];
-                const ull wp = (ull)(u == 0 ? 1 : u);
-                atomicAdd((ull *)&selfsum[a], wp);
-                atomicAdd((ull *)&kdeg[a], 4ull * wp);
-                atomicMax(&lastloop[a], (ull)(i + 1));
+    for (int64_t base = (int64_t)blockIdx.x * kEdgeTile; base < e;
+         base += (int64_t)gridDim.x * kEdgeTile) {
+        int wc = 0;
+        for (int rr = 0; rr < kSpanRounds; ++rr) {
+            const int64_t i = base + rr * 256 + threadIdx.x;
+            long long lo = 0, hi = 0;
+            bool sl = false;
+            if (i < e) {
+                const int32_t a = src[i], b = dst[i];
+                const int32_t pa = i > 0 ? src[i - 1] : -1;
+                if (a < pa) bad |= 8;
+                if (a > b || (a == pa && b <= dst[i - 1])) bad |= 16;
+                sl = a == b;
+                if (sl) {
+                    const int64_t u = w[i];
+                    const ull wp = (ull)(u == 0 ? 1 : u);
+                    atomicAdd((ull *)&selfsum[a], wp);
+                    atomicAdd((ull *)&kdeg[a], 4ull * wp);
+                    atomicMax(&lastloop[a], (ull)(i + 1));
+                    atomicAdd(&vself[a], 1);
+                }
+                if (i > 0 && a > pa) {  // fs[v] = i for v in (pa, a]; in bounds for any order
+                    lo = pa;
+                    hi = a;
+                }
             }
-            if (i > 0 && a > pa) {  // fs[v] = i for v in (pa, a]; in bounds for any order
-                lo = pa;
-                hi = a;
-            }
+            wc += __popcll(__ballot(sl));
+            wave_range_fill(lo, hi, (long long)i, fs);
         }
-        wave_range_fill(lo, hi, (long long)i, fs);
+        if (lane == 0) s_wc[wave] = wc;
+        __syncthreads();
+        if (threadIdx.x == 0) tself[base / kEdgeTile] = s_wc[0] + s_wc[1] + s_wc[2] + s_wc[3];
+        __syncthreads();
     }
     block_flag_or(bad, f);
 }
 
-// Reverse entries of non-self edges, compacted: key dst, payload src << wbits | w.
-__global__ void k_rev_entries(const int32_t *src, const int32_t *dst, const int64_t *w,
-                              const int32_t *slscan, int64_t e, int wbits, uint32_t *keys,
-                              uint64_t *payload)
+// Reverse entries of non-self edges, compacted (edge i goes to i minus the
+// self-loop edges before it): with ONE, one word per entry dst << (64 - kb) |
+// src << wb | w, kb = bits of the largest id and wb = 64 - 2 kb (when every
+// weight fits wb bits), so the row sort moves 8 bytes per entry instead of 12;
+// otherwise key dst and payload src << wbits | w.
+template <bool ONE>
+__global__ __launch_bounds__(256) void k_rev_entries(const int32_t *src, const int32_t *dst,
+                                                    const int64_t *w, const int32_t *tscan,
+                                                    int64_t e, int bits, uint32_t *keys,
+                                                    uint64_t *out)
 {
-    GRID_STRIDE(i, e)
-    {
-        const int32_t a = src[i], b = dst[i];
-        if (a == b) continue;
-        const int64_t r = i - slscan[i];
-        keys[r] = (uint32_t)b;
-        payload[r] = ((uint64_t)(uint32_t)a << wbits) | (uint64_t)w[i];
-    }
-}
-
-// The same as one word per entry, dst << (64 - kb) | src << wb | w with
-// kb = bits of the largest id and wb = 64 - 2 kb (when every weight fits wb
-// bits): the row sort moves 8 bytes per entry instead of 12.
-__global__ void k_rev_entries1(const int32_t *src, const int32_t *dst, const int64_t *w,
-                               const int32_t *slscan, int64_t e, int kb, uint64_t *words)
-{
-    const int wb = 64 - 2 * kb;
-    GRID_STRIDE(i, e)
-    {
-        const int32_t a = src[i], b = dst[i];
-        if (a == b) continue;
-        words[i - slscan[i]] = ((uint64_t)(uint32_t)b << (64 - kb)) |
-                               ((uint64_t)(uint32_t)a << wb) | (uint64_t)w[i];
+    __shared__ int32_t s_wc[4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const ull below = (1ull << lane) - 1ull;
+    for (int64_t base = (int64_t)blockIdx.x * kEdgeTile; base < e;
+         base += (int64_t)gridDim.x * kEdgeTile) {
+        const int64_t w0 = base + (int64_t)wave * 64 * kSpanRounds;
+        int32_t a[kSpanRounds], b[kSpanRounds];
+        bool sl[kSpanRounds];
+#pragma unroll
+        for (int rr = 0; rr < kSpanRounds; ++rr) {
+            const int64_t i = w0 + rr * 64 + lane;
+            a[rr] = i < e ? src[i] : 0;
+            b[rr] = i < e ? dst[i] : 0;
+            sl[rr] = i < e && a[rr] == b[rr];
+        }
+        ull m[kSpanRounds];
+        int32_t off = tile_self_base(tscan, base / kEdgeTile, sl, m, s_wc);
+#pragma unroll
+        for (int rr = 0; rr < kSpanRounds; ++rr) {
+            const int64_t i = w0 + rr * 64 + lane;
+            if (i < e && !sl[rr]) {
+                const int64_t r = i - (off + __popcll(m[rr] & below));
+                if constexpr (ONE) {
+                    out[r] = ((uint64_t)(uint32_t)b[rr] << (64 - bits)) |
+                             ((uint64_t)(uint32_t)a[rr] << (64 - 2 * bits)) | (uint64_t)w[i];
+                } else {
+                    keys[r] = (uint32_t)b[rr];
+                    out[r] = ((uint64_t)(uint32_t)a[rr] << bits) | (uint64_t)w[i];
+                }
+            }
+            off += __popcll(m[rr]);
+        }
     }
 }
 
 // rowptr[v] = reverse entries of rows < v + forward (non-self) entries of rows
 // < v; ffwd[v] = rowptr[v] + (reverse entries of v) - (non-self edges before
-// fs[v]), so forward edge i of row v lands at ffwd[v] + (i - slscan[i]).
-__global__ void k_rowptr_sum(const int64_t *rs, const int64_t *fs, const int32_t *slscan, int32_t n,
+// fs[v]), so forward edge i of row v lands at ffwd[v] + i - (self-loop edges
+// before i).
+__global__ void k_rowptr_sum(const int64_t *rs, const int64_t *fs, const int32_t *vscan, int32_t n,
                              int64_t *rowptr, int64_t *ffwd)
 {
     GRID_STRIDE(v, (int64_t)n + 1)
     {
-        const int64_t nf = fs[v] - slscan[fs[v]];
+        const int64_t nf = fs[v] - vscan[v];
         const int64_t rp = rs[v] + nf;
         rowptr[v] = rp;
         if (v < n) ffwd[v] = rp + (rs[v + 1] - rs[v]) - nf;
@@ -502,36 +571,52 @@ __global__ __launch_bounds__(256) void k_fill_rev(const uint32_t *skey, const ui
 
 template <bool CHECK>
 __global__ __launch_bounds__(256) void k_fill_fwd(const int32_t *src, const int32_t *dst,
-                                                 const int64_t *w, const int32_t *slscan, int64_t e,
+                                                 const int64_t *w, const int32_t *tscan,
+                                                 const int32_t *vscan, int64_t e,
                                                  const int64_t *fs, const int64_t *rs,
                                                  const uint64_t *spay, int wbits, uint64_t cmask,
                                                  const int64_t *ffwd, int32_t *col, int64_t *cw,
                                                  int64_t *kdeg, DevFlags *f)
 {
+    __shared__ int32_t s_wc[4];
     int bad = 0;
-    constexpr int64_t SPAN = 256 * kSpanRounds;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const ull below = (1ull << lane) - 1ull;
     SegCarry sc;
-    for (int64_t base = (int64_t)blockIdx.x * SPAN; base < e; base += (int64_t)gridDim.x * SPAN) {
+    for (int64_t base = (int64_t)blockIdx.x * kEdgeTile; base < e;
+         base += (int64_t)gridDim.x * kEdgeTile) {
         const int64_t w0 = base + (int64_t)wave * 64 * kSpanRounds;
+        int32_t sv[kSpanRounds], sb[kSpanRounds];
+        bool sl[kSpanRounds];
+#pragma unroll
+        for (int rr = 0; rr < kSpanRounds; ++rr) {
+            const int64_t i = w0 + rr * 64 + lane;
+            sv[rr] = i < e ? src[i] : 0;
+            sb[rr] = i < e ? dst[i] : 0;
+            sl[rr] = i < e && sv[rr] == sb[rr];
+        }
+        ull m[kSpanRounds];
+        int32_t off = tile_self_base(tscan, base / kEdgeTile, sl, m, s_wc);
+#pragma unroll
         for (int rr = 0; rr < kSpanRounds; ++rr) {
             const int64_t i = w0 + rr * 64 + lane;
             const bool valid = i < e;
             long long row = 0;
             ull wp = 0;
             if (valid) {
-                const int32_t v = src[i], b = dst[i];
+                const int32_t v = sv[rr], b = sb[rr];
                 row = v;  // a self-loop edge keeps its row's key with weight 0, so the
                           // runs of equal keys stay contiguous for the segmented sum
                 if (v != b) {
-                    const int64_t pos = ffwd[v] + (i - slscan[i]);
+                    const int64_t sli = off + __popcll(m[rr] & below);  // self-loops before i
+                    const int64_t pos = ffwd[v] + (i - sli);
                     const int64_t u = w[i];
                     col[pos] = b;
                     cw[pos] = u;
                     wp = (ull)(u == 0 ? 1 : u);
                     if constexpr (CHECK) {
                         const int64_t f0 = fs[v];
-                        const int64_t rank = (i - f0) - (slscan[i] - slscan[f0]);
+                        const int64_t rank = (i - f0) - (sli - vscan[v]);
                         if (rank > 0) {
                             int64_t k = i - 1;  // the previous non-self edge of the row
                             while (src[k] == dst[k]) --k;
@@ -543,6 +628,7 @@ __global__ __launch_bounds__(256) void k_fill_fwd(const int32_t *src, const int3
                     }
                 }
             }
+            off += __popcll(m[rr]);
             sc.add(valid, row, wp, (ull *)kdeg);
         }
         sc.flush((ull *)kdeg);
@@ -551,12 +637,13 @@ __global__ __launch_bounds__(256) void k_fill_fwd(const int32_t *src, const int3
 }
 
 void launch_src_sorted(hipStream_t s, const int32_t *src, const int32_t *dst, const int64_t *w,
-                       int64_t e, int32_t n, int32_t *self, int64_t *fs, int64_t *kdeg,
-                       int64_t *selfsum, ull *lastloop, DevFlags *f)
+                       int64_t e, int32_t n, int32_t *tself, int32_t *vself, int64_t *fs,
+                       int64_t *kdeg, int64_t *selfsum, ull *lastloop, DevFlags *f)
 {
+    fill_i32(s, vself, n, 0);
     if (e > 0) {
-        hipLaunchKernelGGL(k_src_sorted, grid_for(e, 256, 8192), 256, 0, s, src, dst, w, e,
-                           self, fs, kdeg, selfsum, lastloop, f);
+        hipLaunchKernelGGL(k_src_sorted, grid_for(e, kEdgeTile, 8192), 256, 0, s, src, dst, w, e,
+                           tself, vself, fs, kdeg, selfsum, lastloop, f);
         hipLaunchKernelGGL(k_range_ends<int32_t>, grid_for((int64_t)n + 1, 256, 1 << 20), 256, 0,
                            s, src, e, n, 0, fs);
     } else {
@@ -565,26 +652,26 @@ void launch_src_sorted(hipStream_t s, const int32_t *src, const int32_t *dst, co
 }
 
 void launch_rev_entries(hipStream_t s, const int32_t *src, const int32_t *dst, const int64_t *w,
-                        const int32_t *slscan, int64_t e, int wbits, uint32_t *keys,
+                        const int32_t *tscan, int64_t e, int wbits, uint32_t *keys,
                         uint64_t *payload)
 {
     if (e > 0)
-        hipLaunchKernelGGL(k_rev_entries, grid_for(e, 256, 1 << 20), 256, 0, s, src, dst, w,
-                           slscan, e, wbits, keys, payload);
+        hipLaunchKernelGGL(k_rev_entries<false>, grid_for(e, kEdgeTile, 1 << 20), 256, 0, s, src,
+                           dst, w, tscan, e, wbits, keys, payload);
 }
 
 // Reverse entries as (skey, spay = src << wbits | w), or with skey null as one
 // word spay = row << vshift | src << wbits | w; kb = bits of the largest id.
 void launch_csr_sorted_fill(hipStream_t s, bool check, const int32_t *src, const int32_t *dst,
-                            const int64_t *w, const int32_t *slscan, int64_t e, int32_t n,
-                            int wbits, int vshift, int kb, const int64_t *fs,
+                            const int64_t *w, const int32_t *tscan, const int32_t *vscan,
+                            int64_t e, int32_t n, int wbits, int vshift, int kb, const int64_t *fs,
                             const uint32_t *skey, const uint64_t *spay, int64_t nr,
                             const int64_t *rs, int64_t *rowptr, int64_t *ffwd, int32_t *col,
                             int64_t *cw, int64_t *kdeg, DevFlags *f)
 {
     const uint64_t cmask = kb >= 64 ? ~0ull : ((1ull << kb) - 1ull);
     hipLaunchKernelGGL(k_rowptr_sum, grid_for((int64_t)n + 1, 256, 1 << 20), 256, 0, s, rs, fs,
-                       slscan, n, rowptr, ffwd);
+                       vscan, n, rowptr, ffwd);
     constexpr int SPAN = 256 * kSpanRounds;
     if (nr > 0) {
         if (check)
@@ -596,20 +683,22 @@ void launch_csr_sorted_fill(hipStream_t s, bool check, const int32_t *src, const
     }
     if (e > 0) {
         if (check)
-            hipLaunchKernelGGL(k_fill_fwd<true>, grid_for(e, SPAN, 8192), 256, 0, s, src, dst, w,
-                               slscan, e, fs, rs, spay, wbits, cmask, ffwd, col, cw, kdeg, f);
+            hipLaunchKernelGGL(k_fill_fwd<true>, grid_for(e, kEdgeTile, 8192), 256, 0, s, src, dst,
+                               w, tscan, vscan, e, fs, rs, spay, wbits, cmask, ffwd, col, cw, kdeg,
+                               f);
         else
-            hipLaunchKernelGGL(k_fill_fwd<false>, grid_for(e, SPAN, 1 << 20), 256, 0, s, src, dst, w,
-                               slscan, e, fs, rs, spay, wbits, cmask, ffwd, col, cw, kdeg, f);
+            hipLaunchKernelGGL(k_fill_fwd<false>, grid_for(e, kEdgeTile, 1 << 20), 256, 0, s, src,
+                               dst, w, tscan, vscan, e, fs, rs, spay, wbits, cmask, ffwd, col, cw,
+                               kdeg, f);
     }
 }
 
 void launch_rev_entries1(hipStream_t s, const int32_t *src, const int32_t *dst, const int64_t *w,
-                         const int32_t *slscan, int64_t e, int kb, uint64_t *words)
+                         const int32_t *tscan, int64_t e, int kb, uint64_t *words)
 {
     if (e > 0)
-        hipLaunchKernelGGL(k_rev_entries1, grid_for(e, 256, 1 << 20), 256, 0, s, src, dst, w,
-                           slscan, e, kb, words);
+        hipLaunchKernelGGL(k_rev_entries<true>, grid_for(e, kEdgeTile, 1 << 20), 256, 0, s, src,
+                           dst, w, tscan, e, kb, nullptr, words);
 }
 
 template <typename K>
