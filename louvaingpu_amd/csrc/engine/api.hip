@@ -69,6 +69,7 @@ struct lv_ctx {
     int64_t *rowptr = nullptr;
     int32_t *col = nullptr;
     int64_t *cw = nullptr;
+    uint32_t *cw32 = nullptr;  // instead of cw when every weight of the level is below 2^32
     int64_t *kdeg = nullptr, *loopw = nullptr, *selfsum = nullptr;
     int32_t *C = nullptr, *next = nullptr;
     Lab *lab = nullptr;
@@ -282,6 +283,7 @@ void free_level_arrays(lv_ctx *c)
     arena_free(a, c->rowptr);
     arena_free(a, c->col);
     arena_free(a, c->cw);
+    arena_free(a, c->cw32);
     arena_free(a, c->kdeg);
     arena_free(a, c->loopw);
     arena_free(a, c->selfsum);
@@ -359,6 +361,13 @@ double current_q(lv_ctx *c, int &rc)
 // ---------------------------------------------------------------------------
 // phases
 
+// Row-entry weights of a level: 4 bytes each when every weight fits 32 bits.
+int alloc_csr_w(lv_ctx *c, int64_t nnz, bool w32)
+{
+    if (w32) return arena_alloc(c->arena, c->cw32, nnz);
+    return arena_alloc(c->arena, c->cw, nnz);
+}
+
 // Level-0 CSR for edges sorted by source (tself/vself = self-loop edges per edge
 // tile and per source vertex, fs[v] = first edge with src >= v): forward
 // entries stay in edge order, reverse entries are radix-sorted by destination,
@@ -413,11 +422,11 @@ int build_level0_csr_sorted(lv_ctx *c, int32_t *tself, int32_t *vself, const int
     }
     c->nnz = 2 * nr;
     LV_TRY(arena_alloc(A, c->col, c->nnz));
-    LV_TRY(arena_alloc(A, c->cw, c->nnz));
+    LV_TRY(alloc_csr_w(c, c->nnz, c->max_units0 < (1ull << 32)));
     LV_HIP(hipMemsetAsync(&c->d_flags->err, 0, sizeof(int), s));
     launch_csr_sorted_fill(s, !canonical, c->d_src, c->d_dst, c->d_w, tscan, vscan, e, n, fbits,
-                           vshift, kb, fs, sk, sp, nr, rs, c->rowptr, ffwd, c->col, c->cw, c->kdeg,
-                           c->d_flags);
+                           vshift, kb, fs, sk, sp, nr, rs, c->rowptr, ffwd, c->col, c->cw, c->cw32,
+                           c->kdeg, c->d_flags);
     launch_loop_gather(s, lastloop, c->d_w, n, c->loopw);
     LV_HIP(hipGetLastError());
     if (canonical) {
@@ -515,9 +524,9 @@ int build_level0_csr(lv_ctx *c)
         LV_TRY(sync(c));
         c->nnz = c->h_small[0];
         LV_TRY(arena_alloc(A, c->col, c->nnz));
-        LV_TRY(arena_alloc(A, c->cw, c->nnz));
+        LV_TRY(alloc_csr_w(c, c->nnz, c->max_units0 < (1ull << 32)));
         LV_HIP(hipMemsetAsync(&c->d_flags->err, 0, sizeof(int), s));
-        launch_csr_unpack1(s, sx, c->nnz, bits, cb, c->col, c->cw, c->kdeg, c->d_flags);
+        launch_csr_unpack1(s, sx, c->nnz, bits, cb, c->col, c->cw, c->cw32, c->kdeg, c->d_flags);
         launch_loop_gather(s, lastloop, c->d_w, n, c->loopw);
         LV_HIP(hipGetLastError());
         LV_TRY(d2h(c, c->h_flags, c->d_flags, 1));
@@ -554,11 +563,12 @@ int build_level0_csr(lv_ctx *c)
     LV_TRY(sync(c));
     c->nnz = c->h_small[0];
     LV_TRY(arena_alloc(A, c->col, c->nnz));
-    LV_TRY(arena_alloc(A, c->cw, c->nnz));
+    LV_TRY(alloc_csr_w(c, c->nnz, c->max_units0 < (1ull << 32)));
     if (packed)
-        launch_csr_unpack(s, sk, sp, c->nnz, wbits, c->col, c->cw, c->kdeg);
+        launch_csr_unpack(s, sk, sp, c->nnz, wbits, c->col, c->cw, c->cw32, c->kdeg);
     else
-        launch_csr_gather(s, sk, sv, c->nnz, c->d_src, c->d_dst, c->d_w, c->col, c->cw, c->kdeg);
+        launch_csr_gather(s, sk, sv, c->nnz, c->d_src, c->d_dst, c->d_w, c->col, c->cw, c->cw32,
+                          c->kdeg);
     launch_loop_gather(s, lastloop, c->d_w, n, c->loopw);
     LV_HIP(hipMemsetAsync(&c->d_flags->err, 0, sizeof(int), s));
     launch_rows_check(s, sk, c->col, c->nnz, c->d_flags);
@@ -739,6 +749,7 @@ int one_level(lv_ctx *c, double lower, double *q_out, int32_t *passes_out)
     a.rowptr = c->rowptr;
     a.col = c->col;
     a.w = c->cw;
+    a.w32 = c->cw32;
     a.kdeg = c->kdeg;
     a.loopw = c->loopw;
     a.selfsum = c->selfsum;
@@ -931,7 +942,7 @@ int induce(lv_ctx *c, const int32_t *newc, const int32_t *newid_g, int32_t Kn)
     fill_i64(s, (int64_t *)internal, K, 0);
     fill_i32(s, nuniq, K, 0);
     launch_cr_selfl(s, L.perm, c->selfsum, c->g_head, c->g_hscan, n, selfl);
-    launch_cr_gather(s, estart, bound, mstart, c->col, c->cw, newc, n, nnz, ey, ew, ew32);
+    launch_cr_gather(s, estart, bound, mstart, c->col, c->cw, c->cw32, newc, n, nnz, ey, ew, ew32);
     // segments by length
     int64_t *sb = nullptr, *slen = nullptr;
     uint32_t *tk1 = nullptr, *tk2 = nullptr;
@@ -1028,6 +1039,7 @@ int induce(lv_ctx *c, const int32_t *newc, const int32_t *newid_g, int32_t Kn)
     // next level rows
     int64_t *rowcnt = nullptr, *rowptr2 = nullptr, *kdeg2 = nullptr, *loop2 = nullptr,
             *self2 = nullptr, *cw2 = nullptr;
+    uint32_t *cw2_32 = nullptr;
     int32_t *col2 = nullptr;
     LV_TRY(arena_alloc(A, rowcnt, (int64_t)Kn + 1));
     LV_TRY(arena_alloc(A, rowptr2, (int64_t)Kn + 1));
@@ -1044,8 +1056,13 @@ int induce(lv_ctx *c, const int32_t *newc, const int32_t *newid_g, int32_t Kn)
     const int64_t nnz2 = c->h_small[0];
     const int64_t m2 = c->h_small[1] / 2;
     LV_TRY(arena_alloc(A, col2, nnz2));
-    LV_TRY(arena_alloc(A, cw2, nnz2));
-    launch_cr_copy(s, ca, B, K, nparts, rowptr2, col2, cw2);
+    // the total degree bounds every weight and is the same at every level, so
+    // a narrow level's coarse rows keep 4-byte weights
+    if (nar)
+        LV_TRY(arena_alloc(A, cw2_32, nnz2));
+    else
+        LV_TRY(arena_alloc(A, cw2, nnz2));
+    launch_cr_copy(s, ca, B, K, nparts, rowptr2, col2, cw2, cw2_32);
     LV_HIP(hipGetLastError());
     t_end(c, t0, CAT_COARSEN);
     // release the transient buffers (later launches on this stream may reuse them)
@@ -1058,6 +1075,7 @@ int induce(lv_ctx *c, const int32_t *newc, const int32_t *newid_g, int32_t Kn)
     c->rowptr = rowptr2;
     c->col = col2;
     c->cw = cw2;
+    c->cw32 = cw2_32;
     c->kdeg = kdeg2;
     c->loopw = loop2;
     c->selfsum = self2;
@@ -1196,6 +1214,7 @@ int batch_one_level(lv_ctx *c, const std::vector<double> &lower, const std::vect
     a.rowptr = c->rowptr;
     a.col = c->col;
     a.w = c->cw;
+    a.w32 = c->cw32;
     a.kdeg = c->kdeg;
     a.loopw = c->loopw;
     a.selfsum = c->selfsum;
@@ -1947,6 +1966,13 @@ int lv_level_csr(lv_ctx *c, int64_t *rowptr, int32_t *col, int64_t *w)
     LV_TRY(set_device(c));
     LV_TRY(d2h(c, rowptr, c->rowptr, (int64_t)c->n + 1));
     LV_TRY(d2h(c, col, c->col, c->nnz));
+    if (c->cw32) {  // widened on the host
+        std::vector<uint32_t> w32((size_t)c->nnz);
+        LV_TRY(d2h(c, w32.data(), c->cw32, c->nnz));
+        LV_TRY(sync(c));
+        for (int64_t p = 0; p < c->nnz; ++p) w[p] = (int64_t)w32[(size_t)p];
+        return LV_OK;
+    }
     LV_TRY(d2h(c, w, c->cw, c->nnz));
     return sync(c);
 }

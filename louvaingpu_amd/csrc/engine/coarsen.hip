@@ -159,7 +159,8 @@ constexpr int kGatherLds = 2048; // member offsets a block can stage in LDS
 // of dependent global loads per entry is: member record, row entry, newc.
 __global__ __launch_bounds__(256) void k_cr_gather(const int64_t *estart, const int32_t *bound,
                                                   const int64_t *mstart, const int32_t *col,
-                                                  const int64_t *cw, const int32_t *newc,
+                                                  const int64_t *cw, const uint32_t *cw32,
+                                                  const int32_t *newc,
                                                   int64_t total, int32_t *ey, int64_t *ew,
                                                   uint32_t *ew32)
 {
@@ -210,7 +211,9 @@ __global__ __launch_bounds__(256) void k_cr_gather(const int64_t *estart, const 
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         cc[r] = ok[r] ? __builtin_nontemporal_load(&col[pp[r]]) : 0;
-        uu[r] = ok[r] ? __builtin_nontemporal_load(&cw[pp[r]]) : 0;
+        uu[r] = !ok[r] ? 0
+                : cw32 ? (int64_t)__builtin_nontemporal_load(&cw32[pp[r]])
+                       : __builtin_nontemporal_load(&cw[pp[r]]);
     }
 #pragma unroll
     for (int r = 0; r < R; ++r)
@@ -245,14 +248,15 @@ void launch_cr_selfl(hipStream_t s, const int32_t *perm, const int64_t *selfsum,
 }
 
 void launch_cr_gather(hipStream_t s, const int64_t *estart, int32_t *bound, const int64_t *mstart,
-                      const int32_t *col, const int64_t *cw, const int32_t *newc, int32_t n,
+                      const int32_t *col, const int64_t *cw, const uint32_t *cw32,
+                      const int32_t *newc, int32_t n,
                       int64_t total, int32_t *ey, int64_t *ew, uint32_t *ew32)
 {
     if (total <= 0) return;
     const int64_t nb = (total + kGatherT - 1) / kGatherT;
     hipLaunchKernelGGL(k_lbs_bounds, grid_for(nb + 1, 256), 256, 0, s, estart, n, total, kGatherT,
                        nb, bound);
-    hipLaunchKernelGGL(k_cr_gather, (int)nb, 256, 0, s, estart, bound, mstart, col, cw, newc,
+    hipLaunchKernelGGL(k_cr_gather, (int)nb, 256, 0, s, estart, bound, mstart, col, cw, cw32, newc,
                        total, ey, ew, ew32);
 }
 
@@ -688,7 +692,7 @@ __global__ void k_cr_rowcnt(const int32_t *newid_g, const int32_t *nuniq, int32_
 
 // 16 lanes per run.
 __global__ void k_cr_copy_small(CoarsenArgs a, const int64_t *rowptr2, int32_t K, int32_t *col2,
-                                int64_t *cw2)
+                                int64_t *cw2, uint32_t *cw2_32)
 {
     const int lg = threadIdx.x & 15;
     for (int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4; g < K;
@@ -699,13 +703,13 @@ __global__ void k_cr_copy_small(CoarsenArgs a, const int64_t *rowptr2, int32_t K
         const int32_t len = a.nuniq[g];
         for (int32_t t = lg; t < len; t += 16) {
             col2[dst + t] = a.uy[src + t];
-            cw2[dst + t] = a.uw[src + t];
+            put_w(cw2, cw2_32, dst + t, a.uw[src + t]);
         }
     }
 }
 
 __global__ void k_cr_copy_big(CoarsenArgs a, BigSegArgs B, const int64_t *rowptr2, int64_t nparts,
-                              int32_t *col2, int64_t *cw2)
+                              int32_t *col2, int64_t *cw2, uint32_t *cw2_32)
 {
     const int lg = threadIdx.x & 15;
     for (int64_t p = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4; p < nparts;
@@ -715,7 +719,7 @@ __global__ void k_cr_copy_big(CoarsenArgs a, BigSegArgs B, const int64_t *rowptr
         const int32_t len = B.ucnt[p];
         for (int32_t t = lg; t < len; t += 16) {
             col2[dst + t] = B.buy[src + t];
-            cw2[dst + t] = B.buw[src + t];
+            put_w(cw2, cw2_32, dst + t, B.buw[src + t]);
         }
     }
 }
@@ -753,14 +757,15 @@ void launch_cr_rowcnt(hipStream_t s, const int32_t *newid_g, const int32_t *nuni
 }
 
 void launch_cr_copy(hipStream_t s, const CoarsenArgs &a, const BigSegArgs &B, int32_t K,
-                    int64_t nparts, const int64_t *rowptr2, int32_t *col2, int64_t *cw2)
+                    int64_t nparts, const int64_t *rowptr2, int32_t *col2, int64_t *cw2,
+                    uint32_t *cw2_32)
 {
     if (K > 0)
         hipLaunchKernelGGL(k_cr_copy_small, grid_for((int64_t)K * 16, 256, 1 << 18), 256, 0, s, a,
-                           rowptr2, K, col2, cw2);
+                           rowptr2, K, col2, cw2, cw2_32);
     if (nparts > 0)
         hipLaunchKernelGGL(k_cr_copy_big, grid_for(nparts * 16, 256, 1 << 18), 256, 0, s, a, B,
-                           rowptr2, nparts, col2, cw2);
+                           rowptr2, nparts, col2, cw2, cw2_32);
 }
 
 void launch_cr_vertex(hipStream_t s, const int32_t *newid_g, const ull *ssum, const ull *internal,

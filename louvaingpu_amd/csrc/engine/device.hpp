@@ -247,6 +247,16 @@ __device__ __forceinline__ void wave_bucket_add(bool valid, int key, long long v
 // and flush() after the span; invalid lanes only at the end of the data.
 constexpr int kSpanRounds = 8;
 
+// Stores weight u of CSR entry p: in the 4-byte array when the level keeps
+// one (every weight of the level below 2^32), else in the 8-byte one.
+__device__ __forceinline__ void put_w(int64_t *w, uint32_t *w32, int64_t p, int64_t u)
+{
+    if (w32)
+        w32[p] = (uint32_t)u;
+    else
+        w[p] = u;
+}
+
 struct SegCarry {
     long long key = LLONG_MIN;
     ull sum = 0;

@@ -129,7 +129,8 @@ struct SweepArgs {
     const uint32_t *deg32;  // deg[] of narrow levels
     const int64_t *rowptr;
     const int32_t *col;
-    const int64_t *w;
+    const int64_t *w;       // row entry weights (units), or w32 when every weight fits 32 bits
+    const uint32_t *w32;
     const int64_t *kdeg;
     const int64_t *loopw;
     const int64_t *selfsum;
@@ -298,14 +299,14 @@ void launch_csr_entries_packed(hipStream_t s, const int32_t *src, const int32_t 
                                const int64_t *w, int64_t e, int32_t n, int wbits, uint32_t *keys,
                                uint64_t *payload);
 void launch_csr_unpack(hipStream_t s, const uint32_t *skeys, const uint64_t *spay, int64_t nnz,
-                       int wbits, int32_t *col, int64_t *cw, int64_t *kdeg);
+                       int wbits, int32_t *col, int64_t *cw, uint32_t *cw32, int64_t *kdeg);
 void launch_self_loops(hipStream_t s, const int32_t *src, const int32_t *dst, const int64_t *w,
                        int64_t e, int64_t *kdeg, int64_t *selfsum, ull *lastloop);
 void launch_rowptr_search(hipStream_t s, const uint32_t *skeys, int64_t total, int32_t n,
                           int64_t *rowptr);
 void launch_csr_gather(hipStream_t s, const uint32_t *skeys, const uint32_t *scodes, int64_t nnz,
                        const int32_t *src, const int32_t *dst, const int64_t *w, int32_t *col,
-                       int64_t *cw, int64_t *kdeg);
+                       int64_t *cw, uint32_t *cw32, int64_t *kdeg);
 void launch_loop_gather(hipStream_t s, const ull *lastloop, const int64_t *w, int32_t n,
                         int64_t *loopw);
 
@@ -351,7 +352,7 @@ void launch_csr_sorted_fill(hipStream_t s, bool check, const int32_t *src, const
                             int64_t e, int32_t n, int wbits, int vshift, int kb, const int64_t *fs,
                             const uint32_t *skey, const uint64_t *spay, int64_t nr,
                             const int64_t *rs, int64_t *rowptr, int64_t *ffwd, int32_t *col,
-                            int64_t *cw, int64_t *kdeg, DevFlags *f);
+                            int64_t *cw, uint32_t *cw32, int64_t *kdeg, DevFlags *f);
 // One-word reverse entries dst << (64 - kb) | src << (64 - 2 kb) | w, and row
 // starts over such words sorted by their top bits (key = word >> kshift).
 // General path as one word per entry (row | col | w, row = n for self-loops);
@@ -359,7 +360,7 @@ void launch_csr_sorted_fill(hipStream_t s, bool check, const int32_t *src, const
 void launch_csr_entries1(hipStream_t s, const int32_t *src, const int32_t *dst, const int64_t *w,
                          int64_t e, int32_t n, int rb, int cb, uint64_t *words);
 void launch_csr_unpack1(hipStream_t s, const uint64_t *words, int64_t nnz, int rb, int cb,
-                        int32_t *col, int64_t *cw, int64_t *kdeg, DevFlags *f);
+                        int32_t *col, int64_t *cw, uint32_t *cw32, int64_t *kdeg, DevFlags *f);
 void launch_rev_entries1(hipStream_t s, const int32_t *src, const int32_t *dst, const int64_t *w,
                          const int32_t *tscan, int64_t e, int kb, uint64_t *words);
 void launch_row_starts64(hipStream_t s, const uint64_t *keys, int64_t total, int32_t n,
@@ -415,8 +416,9 @@ void launch_cr_memdeg(hipStream_t s, const int32_t *perm, const int64_t *rowptr,
 void launch_cr_selfl(hipStream_t s, const int32_t *perm, const int64_t *selfsum,
                      const int32_t *head, const int32_t *hscan, int32_t n, ull *selfl);
 void launch_cr_gather(hipStream_t s, const int64_t *estart, int32_t *bound, const int64_t *mstart,
-                      const int32_t *col, const int64_t *cw, const int32_t *newc, int32_t n,
-                      int64_t total, int32_t *ey, int64_t *ew, uint32_t *ew32);
+                      const int32_t *col, const int64_t *cw, const uint32_t *cw32,
+                      const int32_t *newc, int32_t n, int64_t total, int32_t *ey, int64_t *ew,
+                      uint32_t *ew32);
 void launch_cr_seg_tier(hipStream_t s, const int32_t *gstart, const int64_t *estart,
                         const int32_t *newid_g, int32_t K, int64_t *sb, int64_t *slen,
                         uint32_t *tkey, int32_t *sid, long long *tier_cnt);
@@ -429,7 +431,8 @@ int run_cr_big(Arena &A, hipStream_t s, const CoarsenArgs &a, const BigSegArgs &
 void launch_cr_rowcnt(hipStream_t s, const int32_t *newid_g, const int32_t *nuniq, int32_t K,
                       int64_t *rowcnt);
 void launch_cr_copy(hipStream_t s, const CoarsenArgs &a, const BigSegArgs &B, int32_t K,
-                    int64_t nparts, const int64_t *rowptr2, int32_t *col2, int64_t *cw2);
+                    int64_t nparts, const int64_t *rowptr2, int32_t *col2, int64_t *cw2,
+                    uint32_t *cw2_32);
 void launch_cr_vertex(hipStream_t s, const int32_t *newid_g, const ull *ssum, const ull *internal,
                       const ull *selfl, int32_t K, int64_t *loop2, int64_t *self2, int64_t *kdeg2,
                       long long *msum);

@@ -193,7 +193,7 @@ __global__ void k_csr_entries1(const int32_t *src, const int32_t *dst, const int
 }
 
 __global__ __launch_bounds__(256) void k_csr_unpack1(const uint64_t *words, int64_t nnz, int rb,
-                                                    int cb, int32_t *col, int64_t *cw,
+                                                    int cb, int32_t *col, int64_t *cw, uint32_t *cw32,
                                                     int64_t *kdeg, DevFlags *f)
 {
     const int wb = 64 - rb - cb;
@@ -214,7 +214,7 @@ __global__ __launch_bounds__(256) void k_csr_unpack1(const uint64_t *words, int6
                 const int64_t u = (int64_t)(x & wmask);
                 const int32_t cc = (int32_t)((x >> wb) & cmask);
                 col[p] = cc;
-                cw[p] = u;
+                put_w(cw, cw32, p, u);
                 row = (long long)(x >> (64 - rb));
                 wp = (ull)(u == 0 ? 1 : u);
                 // rows must list their columns strictly increasing (err bit 4)
@@ -240,11 +240,11 @@ void launch_csr_entries1(hipStream_t s, const int32_t *src, const int32_t *dst, 
 }
 
 void launch_csr_unpack1(hipStream_t s, const uint64_t *words, int64_t nnz, int rb, int cb,
-                        int32_t *col, int64_t *cw, int64_t *kdeg, DevFlags *f)
+                        int32_t *col, int64_t *cw, uint32_t *cw32, int64_t *kdeg, DevFlags *f)
 {
     if (nnz > 0)
         hipLaunchKernelGGL(k_csr_unpack1, grid_for(nnz, 256 * kSpanRounds, 8192), 256, 0, s, words,
-                           nnz, rb, cb, col, cw, kdeg, f);
+                           nnz, rb, cb, col, cw, cw32, kdeg, f);
 }
 
 __global__ void k_csr_entries_packed(const int32_t *src, const int32_t *dst, const int64_t *w,
@@ -265,7 +265,7 @@ __global__ void k_csr_entries_packed(const int32_t *src, const int32_t *dst, con
 
 __global__ __launch_bounds__(256) void k_csr_unpack(const uint32_t *skeys, const uint64_t *spay,
                                                    int64_t nnz, int wbits, int32_t *col,
-                                                   int64_t *cw, int64_t *kdeg)
+                                                   int64_t *cw, uint32_t *cw32, int64_t *kdeg)
 {
     constexpr int64_t SPAN = 256 * kSpanRounds;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -281,7 +281,7 @@ __global__ __launch_bounds__(256) void k_csr_unpack(const uint32_t *skeys, const
                 uint64_t x = spay[p];
                 int64_t u = (int64_t)(x & ((1ull << wbits) - 1ull));
                 col[p] = (int32_t)(x >> wbits);
-                cw[p] = u;
+                put_w(cw, cw32, p, u);
                 row = (long long)skeys[p];
                 wp = (ull)(u == 0 ? 1 : u);
             }
@@ -301,11 +301,11 @@ void launch_csr_entries_packed(hipStream_t s, const int32_t *src, const int32_t 
 }
 
 void launch_csr_unpack(hipStream_t s, const uint32_t *skeys, const uint64_t *spay, int64_t nnz,
-                       int wbits, int32_t *col, int64_t *cw, int64_t *kdeg)
+                       int wbits, int32_t *col, int64_t *cw, uint32_t *cw32, int64_t *kdeg)
 {
     if (nnz > 0)
         hipLaunchKernelGGL(k_csr_unpack, grid_for(nnz, 256 * kSpanRounds, 1 << 20), 256, 0, s, skeys, spay, nnz,
-                           wbits, col, cw, kdeg);
+                           wbits, col, cw, cw32, kdeg);
 }
 
 __global__ void k_self_loops(const int32_t *src, const int32_t *dst, const int64_t *w, int64_t e,
@@ -530,7 +530,7 @@ template <bool CHECK>
 __global__ __launch_bounds__(256) void k_fill_rev(const uint32_t *skey, const uint64_t *spay,
                                                  int64_t nr, int wbits, int vshift, uint64_t cmask,
                                                  const int64_t *rs, const int64_t *rowptr,
-                                                 int32_t *col, int64_t *cw, int64_t *kdeg,
+                                                 int32_t *col, int64_t *cw, uint32_t *cw32, int64_t *kdeg,
                                                  DevFlags *f)
 {
     int bad = 0;
@@ -551,7 +551,7 @@ __global__ __launch_bounds__(256) void k_fill_rev(const uint32_t *skey, const ui
                 const int64_t pos = rowptr[v] + (r - rs[v]);
                 const int32_t cc = (int32_t)((x >> wbits) & cmask);
                 col[pos] = cc;
-                cw[pos] = u;
+                put_w(cw, cw32, pos, u);
                 row = v;
                 wp = (ull)(u == 0 ? 1 : u);
                 if constexpr (CHECK) {
@@ -575,7 +575,7 @@ __global__ __launch_bounds__(256) void k_fill_fwd(const int32_t *src, const int3
                                                  const int32_t *vscan, int64_t e,
                                                  const int64_t *fs, const int64_t *rs,
                                                  const uint64_t *spay, int wbits, uint64_t cmask,
-                                                 const int64_t *ffwd, int32_t *col, int64_t *cw,
+                                                 const int64_t *ffwd, int32_t *col, int64_t *cw, uint32_t *cw32,
                                                  int64_t *kdeg, DevFlags *f)
 {
     __shared__ int32_t s_wc[4];
@@ -612,7 +612,7 @@ __global__ __launch_bounds__(256) void k_fill_fwd(const int32_t *src, const int3
                     const int64_t pos = ffwd[v] + (i - sli);
                     const int64_t u = w[i];
                     col[pos] = b;
-                    cw[pos] = u;
+                    put_w(cw, cw32, pos, u);
                     wp = (ull)(u == 0 ? 1 : u);
                     if constexpr (CHECK) {
                         const int64_t f0 = fs[v];
@@ -667,7 +667,7 @@ void launch_csr_sorted_fill(hipStream_t s, bool check, const int32_t *src, const
                             int64_t e, int32_t n, int wbits, int vshift, int kb, const int64_t *fs,
                             const uint32_t *skey, const uint64_t *spay, int64_t nr,
                             const int64_t *rs, int64_t *rowptr, int64_t *ffwd, int32_t *col,
-                            int64_t *cw, int64_t *kdeg, DevFlags *f)
+                            int64_t *cw, uint32_t *cw32, int64_t *kdeg, DevFlags *f)
 {
     const uint64_t cmask = kb >= 64 ? ~0ull : ((1ull << kb) - 1ull);
     hipLaunchKernelGGL(k_rowptr_sum, grid_for((int64_t)n + 1, 256, 1 << 20), 256, 0, s, rs, fs,
@@ -676,19 +676,19 @@ void launch_csr_sorted_fill(hipStream_t s, bool check, const int32_t *src, const
     if (nr > 0) {
         if (check)
             hipLaunchKernelGGL(k_fill_rev<true>, grid_for(nr, SPAN, 8192), 256, 0, s, skey, spay, nr,
-                               wbits, vshift, cmask, rs, rowptr, col, cw, kdeg, f);
+                               wbits, vshift, cmask, rs, rowptr, col, cw, cw32, kdeg, f);
         else
             hipLaunchKernelGGL(k_fill_rev<false>, grid_for(nr, SPAN, 1 << 20), 256, 0, s, skey, spay,
-                               nr, wbits, vshift, cmask, rs, rowptr, col, cw, kdeg, f);
+                               nr, wbits, vshift, cmask, rs, rowptr, col, cw, cw32, kdeg, f);
     }
     if (e > 0) {
         if (check)
             hipLaunchKernelGGL(k_fill_fwd<true>, grid_for(e, kEdgeTile, 8192), 256, 0, s, src, dst,
-                               w, tscan, vscan, e, fs, rs, spay, wbits, cmask, ffwd, col, cw, kdeg,
+                               w, tscan, vscan, e, fs, rs, spay, wbits, cmask, ffwd, col, cw, cw32, kdeg,
                                f);
         else
             hipLaunchKernelGGL(k_fill_fwd<false>, grid_for(e, kEdgeTile, 1 << 20), 256, 0, s, src,
-                               dst, w, tscan, vscan, e, fs, rs, spay, wbits, cmask, ffwd, col, cw,
+                               dst, w, tscan, vscan, e, fs, rs, spay, wbits, cmask, ffwd, col, cw, cw32,
                                kdeg, f);
     }
 }
@@ -738,7 +738,7 @@ __global__ void k_row_starts(const K *keys, int64_t total, int32_t n, int kshift
 __global__ __launch_bounds__(256) void k_csr_gather(const uint32_t *skeys, const uint32_t *scodes,
                                                    int64_t nnz, const int32_t *src,
                                                    const int32_t *dst, const int64_t *w,
-                                                   int32_t *col, int64_t *cw, int64_t *kdeg)
+                                                   int32_t *col, int64_t *cw, uint32_t *cw32, int64_t *kdeg)
 {
     // four entries per thread, 256 apart, so each wave covers 64 consecutive
     // entries (rows sorted within the wave for the segmented atomics)
@@ -771,7 +771,7 @@ __global__ __launch_bounds__(256) void k_csr_gather(const uint32_t *skeys, const
             long long row = 0;
             if (valid) {
                 col[p] = cv[r];
-                cw[p] = u[r];
+                put_w(cw, cw32, p, u[r]);
                 row = (long long)skeys[p];
             }
             seg_atomic_add(valid, row, (ull)(u[r] == 0 ? 1 : u[r]), (ull *)kdeg);
@@ -818,11 +818,11 @@ void launch_rowptr_search(hipStream_t s, const uint32_t *skeys, int64_t total, i
 
 void launch_csr_gather(hipStream_t s, const uint32_t *skeys, const uint32_t *scodes, int64_t nnz,
                        const int32_t *src, const int32_t *dst, const int64_t *w, int32_t *col,
-                       int64_t *cw, int64_t *kdeg)
+                       int64_t *cw, uint32_t *cw32, int64_t *kdeg)
 {
     if (nnz > 0)
         hipLaunchKernelGGL(k_csr_gather, grid_for(nnz, 1024, 1 << 20), 256, 0, s, skeys, scodes,
-                           nnz, src, dst, w, col, cw, kdeg);
+                           nnz, src, dst, w, col, cw, cw32, kdeg);
 }
 
 void launch_loop_gather(hipStream_t s, const ull *lastloop, const int64_t *w, int32_t n,
@@ -1088,6 +1088,18 @@ __device__ __forceinline__ Lab lab_of(const SweepArgs &a, int32_t j)
     return a.lab[j];
 }
 
+// Weight of row entry p, streamed past the caches (read once per pass); W4:
+// the level keeps 4-byte weights (w32).  The width is a template parameter of
+// the row-walking kernels so their unrolled loads carry no branch.
+template <bool W4>
+__device__ __forceinline__ int64_t w_of(const SweepArgs &a, int64_t p)
+{
+    if constexpr (W4)
+        return (int64_t)__builtin_nontemporal_load(&a.w32[p]);
+    else
+        return __builtin_nontemporal_load(&a.w[p]);
+}
+
 // The vertex's own record, with the size of its community (the wide records
 // are always written; narrow levels gather neighbours from lab8).
 __device__ __forceinline__ Lab own_lab(const SweepArgs &a, int32_t v) { return a.lab[v]; }
@@ -1277,7 +1289,7 @@ void launch_move_deltas(hipStream_t s, const SweepArgs &a, int32_t n, bool own_d
 // With DL the gather is the neighbour's 16-byte label record and the
 // community total goes into the table (dg); otherwise only C[target] is
 // gathered and the totals are read per candidate later.
-template <int S, bool DL>
+template <int S, bool DL, bool W4>
 __device__ __forceinline__ void insert_row(const SweepArgs &a, int32_t *key, ull *val, int64_t *dg,
                                            int l2, int64_t beg, int64_t end, int lane)
 {
@@ -1288,7 +1300,7 @@ __device__ __forceinline__ void insert_row(const SweepArgs &a, int32_t *key, ull
         for (int u = 0; u < 4; ++u) {
             const int64_t p = p0 + (int64_t)u * S;
             j[u] = (p < end) ? __builtin_nontemporal_load(&a.col[p]) : -1;
-            w[u] = (p < end) ? __builtin_nontemporal_load(&a.w[p]) : 0;
+            w[u] = (p < end) ? w_of<W4>(a, p) : 0;
         }
         if constexpr (DL) {
             Lab L[4];
@@ -1312,7 +1324,7 @@ __device__ __forceinline__ void insert_row(const SweepArgs &a, int32_t *key, ull
 
 // G lanes per vertex, 256/G vertices per block, an LDS table of up to
 // 2^LOG2CAP slots per vertex (2^MINLOG2 .. 2^LOG2CAP sized to the row).
-template <int G, int LOG2CAP, int MINLOG2, bool DL, bool B>
+template <int G, int LOG2CAP, int MINLOG2, bool DL, bool B, bool W4>
 __global__ __launch_bounds__(256) void k_sweep_group(SweepArgs a, const int32_t *__restrict__ verts,
                                                     int32_t nv)
 {
@@ -1351,7 +1363,7 @@ __global__ __launch_bounds__(256) void k_sweep_group(SweepArgs a, const int32_t 
         }
         __syncthreads();
         if (active) {
-            insert_row<G, DL>(a, key, val, dg, l2, beg, end, lg);
+            insert_row<G, DL, W4>(a, key, val, dg, l2, beg, end, lg);
             if (lg == 0 && loop) {
                 if constexpr (DL)
                     lds_insert_d(key, val, dg, l2, me.c, 0, me.deg);
@@ -1383,7 +1395,7 @@ __global__ __launch_bounds__(256) void k_sweep_group(SweepArgs a, const int32_t 
 
 // One vertex per block of BLOCK threads with an LDS table of up to
 // 2^LOG2CAP slots sized to the row.
-template <int BLOCK, int LOG2CAP, int MINLOG2, bool B>
+template <int BLOCK, int LOG2CAP, int MINLOG2, bool B, bool W4>
 __global__ __launch_bounds__(BLOCK) void k_sweep_block(SweepArgs a, const int32_t *__restrict__ verts,
                                                       int32_t nv)
 {
@@ -1410,7 +1422,7 @@ __global__ __launch_bounds__(BLOCK) void k_sweep_block(SweepArgs a, const int32_
             s_val[s] = 0;
         }
         __syncthreads();
-        insert_row<BLOCK, false>(a, s_key, s_val, nullptr, l2, beg, end, tid);
+        insert_row<BLOCK, false, W4>(a, s_key, s_val, nullptr, l2, beg, end, tid);
         if (tid == 0 && loop) lds_insert(s_key, s_val, l2, me.c, 0);
         __syncthreads();
         int cnt = 0;
@@ -1454,20 +1466,20 @@ static void launch_grid(KernelT k, int64_t blocks, int block, hipStream_t s, con
     hipLaunchKernelGGL(k, g, block, 0, s, a, verts, nv);
 }
 
-template <bool B>
+template <bool B, bool W4>
 static void launch_sweep_tier_t(hipStream_t s, int tier, const SweepArgs &a, const int32_t *verts,
                                 int32_t nv)
 {
     switch (tier) {
-    case 1: launch_grid(k_sweep_group<4, 3, 3, true, B>, (nv + 63) / 64, 256, s, a, verts, nv); break;
-    case 2: launch_grid(k_sweep_group<8, 4, 4, true, B>, (nv + 31) / 32, 256, s, a, verts, nv); break;
-    case 3: launch_grid(k_sweep_group<16, 5, 5, true, B>, (nv + 15) / 16, 256, s, a, verts, nv); break;
-    case 4: launch_grid(k_sweep_group<32, 6, 6, true, B>, (nv + 7) / 8, 256, s, a, verts, nv); break;
-    case 5: launch_grid(k_sweep_group<64, 7, 7, true, B>, (nv + 3) / 4, 256, s, a, verts, nv); break;
-    case 6: launch_grid(k_sweep_group<64, 8, 8, true, B>, (nv + 3) / 4, 256, s, a, verts, nv); break;
-    case 7: launch_grid(k_sweep_group<64, 9, 9, true, B>, (nv + 3) / 4, 256, s, a, verts, nv); break;
-    case 8: launch_grid(k_sweep_group<64, 10, 10, false, B>, (nv + 3) / 4, 256, s, a, verts, nv); break;
-    case 9: launch_grid(k_sweep_block<256, 12, 11, B>, nv, 256, s, a, verts, nv); break;
+    case 1: launch_grid(k_sweep_group<4, 3, 3, true, B, W4>, (nv + 63) / 64, 256, s, a, verts, nv); break;
+    case 2: launch_grid(k_sweep_group<8, 4, 4, true, B, W4>, (nv + 31) / 32, 256, s, a, verts, nv); break;
+    case 3: launch_grid(k_sweep_group<16, 5, 5, true, B, W4>, (nv + 15) / 16, 256, s, a, verts, nv); break;
+    case 4: launch_grid(k_sweep_group<32, 6, 6, true, B, W4>, (nv + 7) / 8, 256, s, a, verts, nv); break;
+    case 5: launch_grid(k_sweep_group<64, 7, 7, true, B, W4>, (nv + 3) / 4, 256, s, a, verts, nv); break;
+    case 6: launch_grid(k_sweep_group<64, 8, 8, true, B, W4>, (nv + 3) / 4, 256, s, a, verts, nv); break;
+    case 7: launch_grid(k_sweep_group<64, 9, 9, true, B, W4>, (nv + 3) / 4, 256, s, a, verts, nv); break;
+    case 8: launch_grid(k_sweep_group<64, 10, 10, false, B, W4>, (nv + 3) / 4, 256, s, a, verts, nv); break;
+    case 9: launch_grid(k_sweep_block<256, 12, 11, B, W4>, nv, 256, s, a, verts, nv); break;
     default: break;
     }
 }
@@ -1477,10 +1489,13 @@ void launch_sweep_tier(hipStream_t s, int tier, const SweepArgs &a, const int32_
                        bool batch)
 {
     if (nv <= 0) return;
+    const bool w4 = a.w32 != nullptr;
     if (batch)
-        launch_sweep_tier_t<true>(s, tier, a, verts, nv);
+        w4 ? launch_sweep_tier_t<true, true>(s, tier, a, verts, nv)
+           : launch_sweep_tier_t<true, false>(s, tier, a, verts, nv);
     else
-        launch_sweep_tier_t<false>(s, tier, a, verts, nv);
+        w4 ? launch_sweep_tier_t<false, true>(s, tier, a, verts, nv)
+           : launch_sweep_tier_t<false, false>(s, tier, a, verts, nv);
 }
 
 // ---------------------------------------------------------------------------
@@ -1490,7 +1505,7 @@ void launch_sweep_tier(hipStream_t s, int tier, const SweepArgs &a, const int32_
 // and the only gather per entry is deg[j].  n_v = row length + has_loop (the
 // own community enters through a self-loop with W = 0).
 
-template <int G, bool B>
+template <int G, bool B, bool W4>
 __global__ __launch_bounds__(256) void k_sweep1_group(SweepArgs a, const int32_t *__restrict__ verts,
                                                      int32_t nv)
 {
@@ -1518,7 +1533,7 @@ __global__ __launch_bounds__(256) void k_sweep1_group(SweepArgs a, const int32_t
                 for (int u = 0; u < 4; ++u) {
                     const int64_t p = p0 + (int64_t)u * G;
                     j[u] = (p < end) ? __builtin_nontemporal_load(&a.col[p]) : -1;
-                    w[u] = (p < end) ? __builtin_nontemporal_load(&a.w[p]) : 0;
+                    w[u] = (p < end) ? w_of<W4>(a, p) : 0;
                 }
 #pragma unroll
                 for (int u = 0; u < 4; ++u) d[u] = (j[u] >= 0) ? deg_of(a, j[u]) : 0;
@@ -1535,7 +1550,7 @@ __global__ __launch_bounds__(256) void k_sweep1_group(SweepArgs a, const int32_t
     }
 }
 
-template <int BLOCK, bool B>
+template <int BLOCK, bool B, bool W4>
 __global__ __launch_bounds__(BLOCK) void k_sweep1_block(SweepArgs a, const int32_t *__restrict__ verts,
                                                        int32_t nv)
 {
@@ -1562,7 +1577,7 @@ __global__ __launch_bounds__(BLOCK) void k_sweep1_block(SweepArgs a, const int32
             for (int u = 0; u < 4; ++u) {
                 const int64_t p = p0 + (int64_t)u * BLOCK;
                 j[u] = (p < end) ? __builtin_nontemporal_load(&a.col[p]) : -1;
-                w[u] = (p < end) ? __builtin_nontemporal_load(&a.w[p]) : 0;
+                w[u] = (p < end) ? w_of<W4>(a, p) : 0;
             }
 #pragma unroll
             for (int u = 0; u < 4; ++u) d[u] = (j[u] >= 0) ? deg_of(a, j[u]) : 0;
@@ -1598,20 +1613,20 @@ __global__ __launch_bounds__(BLOCK) void k_sweep1_block(SweepArgs a, const int32
     }
 }
 
-template <bool B>
+template <bool B, bool W4>
 static void launch_sweep1_tier_t(hipStream_t s, int tier, const SweepArgs &a, const int32_t *verts,
                                  int32_t nv)
 {
     switch (tier) {
-    case 1: launch_grid(k_sweep1_group<4, B>, (nv + 63) / 64, 256, s, a, verts, nv); break;
-    case 2: launch_grid(k_sweep1_group<8, B>, (nv + 31) / 32, 256, s, a, verts, nv); break;
-    case 3: launch_grid(k_sweep1_group<16, B>, (nv + 15) / 16, 256, s, a, verts, nv); break;
-    case 4: launch_grid(k_sweep1_group<32, B>, (nv + 7) / 8, 256, s, a, verts, nv); break;
+    case 1: launch_grid(k_sweep1_group<4, B, W4>, (nv + 63) / 64, 256, s, a, verts, nv); break;
+    case 2: launch_grid(k_sweep1_group<8, B, W4>, (nv + 31) / 32, 256, s, a, verts, nv); break;
+    case 3: launch_grid(k_sweep1_group<16, B, W4>, (nv + 15) / 16, 256, s, a, verts, nv); break;
+    case 4: launch_grid(k_sweep1_group<32, B, W4>, (nv + 7) / 8, 256, s, a, verts, nv); break;
     case 5:
     case 6:
     case 7:
-    case 8: launch_grid(k_sweep1_group<64, B>, (nv + 3) / 4, 256, s, a, verts, nv); break;
-    case 9: launch_grid(k_sweep1_block<256, B>, nv, 256, s, a, verts, nv); break;
+    case 8: launch_grid(k_sweep1_group<64, B, W4>, (nv + 3) / 4, 256, s, a, verts, nv); break;
+    case 9: launch_grid(k_sweep1_block<256, B, W4>, nv, 256, s, a, verts, nv); break;
     default: break;
     }
 }
@@ -1620,15 +1635,18 @@ void launch_sweep1_tier(hipStream_t s, int tier, const SweepArgs &a, const int32
                         int32_t nv, bool batch)
 {
     if (nv <= 0) return;
+    const bool w4 = a.w32 != nullptr;
     if (batch)
-        launch_sweep1_tier_t<true>(s, tier, a, verts, nv);
+        w4 ? launch_sweep1_tier_t<true, true>(s, tier, a, verts, nv)
+           : launch_sweep1_tier_t<true, false>(s, tier, a, verts, nv);
     else
-        launch_sweep1_tier_t<false>(s, tier, a, verts, nv);
+        w4 ? launch_sweep1_tier_t<false, true>(s, tier, a, verts, nv)
+           : launch_sweep1_tier_t<false, false>(s, tier, a, verts, nv);
 }
 
 // Hubs: one block per chunk of a hub's entries (entry t == row length is the
 // self-loop's own community, W = 0), then one wavefront per hub over its chunks.
-template <bool B>
+template <bool B, bool W4>
 __global__ __launch_bounds__(256) void k_hub1_chunk(SweepArgs a, HubArgs h)
 {
     __shared__ int r_cnt[4];
@@ -1659,7 +1677,7 @@ __global__ __launch_bounds__(256) void k_hub1_chunk(SweepArgs a, HubArgs h)
             const int64_t t = p0 + (int64_t)u * 256;
             own[u] = (t == d);
             jj[u] = (t < t1 && t < d) ? __builtin_nontemporal_load(&a.col[rb + t]) : -1;
-            w[u] = (jj[u] >= 0) ? __builtin_nontemporal_load(&a.w[rb + t]) : 0;
+            w[u] = (jj[u] >= 0) ? w_of<W4>(a, rb + t) : 0;
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) dg[u] = (jj[u] >= 0) ? deg_of(a, jj[u]) : 0;
@@ -1729,11 +1747,18 @@ int run_hub_sweep1(hipStream_t s, const SweepArgs &a, const HubArgs &h, int32_t 
                    bool batch)
 {
     if (nh <= 0) return LV_OK;
+    const bool w4 = a.w32 != nullptr;
     if (batch) {
-        hipLaunchKernelGGL(k_hub1_chunk<true>, (int)nchunks, 256, 0, s, a, h);
+        if (w4)
+            hipLaunchKernelGGL((k_hub1_chunk<true, true>), (int)nchunks, 256, 0, s, a, h);
+        else
+            hipLaunchKernelGGL((k_hub1_chunk<true, false>), (int)nchunks, 256, 0, s, a, h);
         hipLaunchKernelGGL(k_hub1_final<true>, (int)((nh + 3) / 4), 256, 0, s, a, h, nh);
     } else {
-        hipLaunchKernelGGL(k_hub1_chunk<false>, (int)nchunks, 256, 0, s, a, h);
+        if (w4)
+            hipLaunchKernelGGL((k_hub1_chunk<false, true>), (int)nchunks, 256, 0, s, a, h);
+        else
+            hipLaunchKernelGGL((k_hub1_chunk<false, false>), (int)nchunks, 256, 0, s, a, h);
         hipLaunchKernelGGL(k_hub1_final<false>, (int)((nh + 3) / 4), 256, 0, s, a, h, nh);
     }
     LV_HIP(hipGetLastError());
@@ -1851,6 +1876,7 @@ __global__ __launch_bounds__(256) void k_hub_count(SweepArgs a, HubArgs h)
         if (hist[t]) atomicAdd((ull *)&h.pcount[h.pbase[j] + t], (ull)hist[t]);
 }
 
+template <bool W4>
 __global__ __launch_bounds__(256) void k_hub_scatter(SweepArgs a, HubArgs h)
 {
     __shared__ int hist[kHubMaxParts];
@@ -1881,7 +1907,7 @@ __global__ __launch_bounds__(256) void k_hub_scatter(SweepArgs a, HubArgs h)
         PEntry en;
         en.c = c;
         en.pad = 0;
-        en.w = (t < d) ? a.w[rb + t] : 0;
+        en.w = (t < d) ? w_of<W4>(a, rb + t) : 0;
         h.buf[pos] = en;
     }
 }
@@ -2019,7 +2045,10 @@ int run_hub_sweep(Arena &A, hipStream_t s, const SweepArgs &a, const HubArgs &h,
     LV_TRY(scan_exclusive_i64(A, s, h.pcount, (int64_t *)h.poff, nparts, (int64_t *)h.poff + nparts));
     LV_HIP(hipMemcpyAsync(h.pcur, h.poff, sizeof(int64_t) * (size_t)nparts,
                           hipMemcpyDeviceToDevice, s));
-    hipLaunchKernelGGL(k_hub_scatter, (int)nchunks, 256, 0, s, a, h);
+    if (a.w32)
+        hipLaunchKernelGGL(k_hub_scatter<true>, (int)nchunks, 256, 0, s, a, h);
+    else
+        hipLaunchKernelGGL(k_hub_scatter<false>, (int)nchunks, 256, 0, s, a, h);
     if (batch) {
         hipLaunchKernelGGL(k_hub_agg<true>, (int)nparts, 256, 0, s, a, h);
         hipLaunchKernelGGL(k_hub_final<true>, (int)((nh + 3) / 4), 256, 0, s, a, h, nh);

@@ -206,17 +206,22 @@ def test_unsorted_list_with_repeated_head(gamma):
 
 
 @pytest.mark.parametrize("order", ["sorted", "shuffled"])
-def test_wide_weight_units(order):
+@pytest.mark.parametrize("wmax_log2", [18, 11])
+def test_wide_weight_units(order, wmax_log2):
     """Weights up to 2^18 next to 2^-20 make the weight unit 2^-20, so the
     total degree is about 2^57 units: with n = 2^15 the per-community size
     delta no longer fits beside the degree delta in one 64-bit word and the
     sweep keeps them in separate arrays; the level is not narrow, and the
     CSR entries (weights up to 2^38 units) take the two-array row sort of
-    either edge order instead of one word per entry."""
-    gpu_case("wide weight units, %s" % order)
+    either edge order instead of one word per entry and keep 8-byte weights.
+    With weights up to 2^11 every level-0 weight fits 32 bits (2^31 units):
+    level 0 keeps 4-byte row weights while the coarser levels, whose
+    weights the total degree (about 2^50) no longer bounds by 2^32, keep
+    8-byte ones."""
+    gpu_case("wide weight units, %s, weights below 2^%d" % (order, wmax_log2))
     n, src, dst, _ = generators.rmat(15, 16, 0.57, 0.19, 0.19, 11, "int")
     rng = np.random.Generator(np.random.PCG64(5))
-    w = rng.integers(1, 1 << 18, len(src)).astype(np.float32)
+    w = rng.integers(1, 1 << wmax_log2, len(src)).astype(np.float32)
     w[rng.integers(0, len(src), 64)] = np.float32(2.0 ** -20)
     if order == "shuffled":
         p = rng.permutation(len(src))
