@@ -257,6 +257,9 @@ void free_sweep_state(lv_ctx *c)
     a.release((void *)h.chunk_k);
     a.release((void *)h.tmp_c);
     a.release((void *)h.buf);
+    a.release((void *)h.buf8);
+    a.release((void *)h.chist);
+    a.release((void *)h.hbase);
     a.release((void *)h.pcount);
     a.release((void *)h.poff);
     a.release((void *)h.pcur);
@@ -607,29 +610,34 @@ int plan_hubs(lv_ctx *c, int32_t nh)
     hipStream_t s = c->stream;
     HubArgs &h = c->hub;
     const int32_t *hv = c->verts + c->tier_off[kTiers];
-    int64_t *hq = nullptr, *hP64 = nullptr, *hch = nullptr, *ebase = nullptr, *pbase = nullptr,
-            *cbase = nullptr;
+    int64_t *hq = nullptr, *hP64 = nullptr, *hch = nullptr, *hpc = nullptr, *ebase = nullptr,
+            *pbase = nullptr, *cbase = nullptr, *hbase = nullptr;
     int32_t *hP = nullptr;
     LV_TRY(arena_alloc(A, hq, nh));
     LV_TRY(arena_alloc(A, hP, nh));
     LV_TRY(arena_alloc(A, hP64, nh));
     LV_TRY(arena_alloc(A, hch, nh));
+    LV_TRY(arena_alloc(A, hpc, nh));
     LV_TRY(arena_alloc(A, ebase, (int64_t)nh + 1));
     LV_TRY(arena_alloc(A, pbase, (int64_t)nh + 1));
     LV_TRY(arena_alloc(A, cbase, (int64_t)nh + 1));
+    LV_TRY(arena_alloc(A, hbase, (int64_t)nh + 1));
     launch_hub_setup(s, hv, nh, c->rowptr, c->selfsum, c->hub_part_target, c->chunk_mul, hq, hP,
-                     hP64, hch);
+                     hP64, hch, hpc);
     h.chunk_mul = c->chunk_mul;
     LV_TRY(scan_exclusive_i64(A, s, hq, ebase, nh, ebase + nh));
     LV_TRY(scan_exclusive_i64(A, s, hP64, pbase, nh, pbase + nh));
     LV_TRY(scan_exclusive_i64(A, s, hch, cbase, nh, cbase + nh));
+    LV_TRY(scan_exclusive_i64(A, s, hpc, hbase, nh, hbase + nh));
     LV_TRY(d2h(c, c->h_small, (const long long *)(ebase + nh), 1));
     LV_TRY(d2h(c, c->h_small + 1, (const long long *)(pbase + nh), 1));
     LV_TRY(d2h(c, c->h_small + 2, (const long long *)(cbase + nh), 1));
+    LV_TRY(d2h(c, c->h_small + 3, (const long long *)(hbase + nh), 1));
     LV_TRY(sync(c));
     c->hub_entries = c->h_small[0];
     c->hub_parts = c->h_small[1];
     c->hub_chunks = c->h_small[2];
+    const int64_t nhist = c->h_small[3];
     c->n_hubs = nh;
     h.hv = hv;
     h.hq = hq;
@@ -648,9 +656,15 @@ int plan_hubs(lv_ctx *c, int32_t nh)
     h.cbase = cbase;
     arena_free(A, hP64);
     arena_free(A, hch);
+    arena_free(A, hpc);
     const int64_t E = c->hub_entries, P = c->hub_parts;
+    h.hbase = hbase;
+    LV_TRY(arena_alloc(A, h.chist, nhist));
     LV_TRY(arena_alloc(A, h.tmp_c, E));
-    LV_TRY(arena_alloc(A, h.buf, E));
+    if (c->narrow)  // a partition's weights sum below the level's total degree, < 2^32
+        LV_TRY(arena_alloc(A, h.buf8, E));
+    else
+        LV_TRY(arena_alloc(A, h.buf, E));
     LV_TRY(arena_alloc(A, h.fbk, 2 * E));
     LV_TRY(arena_alloc(A, h.fbv, 2 * E));
     LV_TRY(arena_alloc(A, h.pcount, P + 1));
@@ -984,32 +998,40 @@ int induce(lv_ctx *c, const int32_t *newc, const int32_t *newid_g, int32_t Kn)
     const int32_t nbs = (int32_t)tcnt[kTiers];
     int64_t nparts = 0, nchunks = 0, blen = 0;
     int32_t *bP = nullptr, *part_big = nullptr, *chunk_big = nullptr, *chunk_k = nullptr;
-    int64_t *bP64 = nullptr, *bch = nullptr, *pbase = nullptr, *cbase = nullptr;
+    int64_t *bP64 = nullptr, *bch = nullptr, *pbase = nullptr, *cbase = nullptr, *bhbase = nullptr;
     if (nbs > 0) {
         const int32_t *bsegs = segs + toff[kTiers];
-        int64_t *blen64 = nullptr, *blscan = nullptr;
+        int64_t *blen64 = nullptr, *blscan = nullptr, *bpc = nullptr;
         LV_TRY(arena_alloc(A, bP, nbs));
         LV_TRY(arena_alloc(A, bP64, nbs));
         LV_TRY(arena_alloc(A, bch, nbs));
         LV_TRY(arena_alloc(A, blen64, nbs));
+        LV_TRY(arena_alloc(A, bpc, nbs));
         LV_TRY(arena_alloc(A, blscan, (int64_t)nbs + 1));
         LV_TRY(arena_alloc(A, pbase, (int64_t)nbs + 1));
         LV_TRY(arena_alloc(A, cbase, (int64_t)nbs + 1));
+        LV_TRY(arena_alloc(A, bhbase, (int64_t)nbs + 1));
         launch_cr_big_setup(s, bsegs, nbs, slen, c->hub_part_target, c->chunk_mul, bP, bP64, bch,
-                            blen64);
+                            blen64, bpc);
         B.chunk_mul = c->chunk_mul;
         LV_TRY(scan_exclusive_i64(A, s, bP64, pbase, nbs, pbase + nbs));
         LV_TRY(scan_exclusive_i64(A, s, bch, cbase, nbs, cbase + nbs));
         LV_TRY(scan_exclusive_i64(A, s, blen64, blscan, nbs, blscan + nbs));
+        LV_TRY(scan_exclusive_i64(A, s, bpc, bhbase, nbs, bhbase + nbs));
         LV_TRY(d2h(c, c->h_small, (const long long *)(pbase + nbs), 1));
         LV_TRY(d2h(c, c->h_small + 1, (const long long *)(cbase + nbs), 1));
         LV_TRY(d2h(c, c->h_small + 2, (const long long *)(blscan + nbs), 1));
+        LV_TRY(d2h(c, c->h_small + 3, (const long long *)(bhbase + nbs), 1));
         LV_TRY(sync(c));
         nparts = c->h_small[0];
         nchunks = c->h_small[1];
         blen = c->h_small[2];  // entries of the big segments
+        const int64_t nhist = c->h_small[3];
         arena_free(A, blen64);
         arena_free(A, blscan);
+        arena_free(A, bpc);
+        B.hbase = bhbase;
+        LV_TRY(arena_alloc(A, B.chist, nhist));
         LV_TRY(arena_alloc(A, part_big, nparts));
         LV_TRY(arena_alloc(A, chunk_big, nchunks));
         LV_TRY(arena_alloc(A, chunk_k, nchunks));
@@ -1069,7 +1091,7 @@ int induce(lv_ctx *c, const int32_t *newc, const int32_t *newid_g, int32_t Kn)
     void *tmp[] = {mdeg, mstart, estart, selfl, ssum, internal, nuniq, ey, ew, ew32, uy, uw, bound, sb, slen,
                    tk1, tk2, sid1, sid2, bP, bP64, bch, pbase, cbase, part_big, chunk_big, chunk_k,
                    B.pcount, B.poff, B.pcur, B.pe, B.pe8, B.buy, B.buw, B.fbk, B.fbv, B.ucnt, B.pre,
-                   rowcnt};
+                   B.chist, bhbase, rowcnt};
     for (void *p : tmp) A.release(p);
     free_level_arrays(c);
     c->rowptr = rowptr2;

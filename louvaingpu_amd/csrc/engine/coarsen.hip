@@ -463,7 +463,7 @@ void launch_cr_agg_tier(hipStream_t s, int tier, const CoarsenArgs &a, const int
 
 __global__ void k_cr_big_setup(const int32_t *bsegs, int32_t nbs, const int64_t *slen,
                                int32_t target, int chunk_mul, int32_t *bP, int64_t *bP64,
-                               int64_t *bch, int64_t *blen)
+                               int64_t *bch, int64_t *blen, int64_t *bpc)
 {
     GRID_STRIDE(j, nbs)
     {
@@ -475,6 +475,7 @@ __global__ void k_cr_big_setup(const int32_t *bsegs, int32_t nbs, const int64_t 
         bP[j] = (int32_t)P;
         bP64[j] = P;
         bch[j] = (L + cs - 1) / cs;
+        bpc[j] = bch[j] * P;
         blen[j] = L;
     }
 }
@@ -511,8 +512,11 @@ __global__ __launch_bounds__(256) void k_cr_big_count(CoarsenArgs a, BigSegArgs 
         s_tot[threadIdx.x >> 6] = tot;
     }
     __syncthreads();
-    for (int t = threadIdx.x; t < P; t += 256)
+    int32_t *ch = B.chist + B.hbase[j] + (int64_t)B.chunk_k[ci] * P;
+    for (int t = threadIdx.x; t < P; t += 256) {
+        ch[t] = hist[t];  // the scatter reserves the chunk's ranges from these
         if (hist[t]) atomicAdd((ull *)&B.pcount[B.pbase[j] + t], (ull)hist[t]);
+    }
     if (threadIdx.x == 0) {
         ull it = s_inter[0] + s_inter[1] + s_inter[2] + s_inter[3];
         if (it) atomicAdd(&a.internal[g], it);
@@ -536,14 +540,10 @@ __global__ __launch_bounds__(256) void k_cr_big_scatter(CoarsenArgs a, BigSegArg
     const int64_t cs = chunk_span(P, B.chunk_mul);
     const int64_t t0 = (int64_t)B.chunk_k[ci] * cs;
     const int64_t t1 = min(L, t0 + cs);
-    for (int64_t t = t0 + threadIdx.x; t < t1; t += 256) {
-        int32_t y = a.ey[b + t];
-        if (y != x) atomicAdd(&hist[hub_part(y, P)], 1);
-    }
-    __syncthreads();
+    const int32_t *ch = B.chist + B.hbase[j] + (int64_t)B.chunk_k[ci] * P;
     for (int t = threadIdx.x; t < P; t += 256) {
-        if (hist[t]) lbase[t] = (long long)atomicAdd((ull *)&B.pcur[B.pbase[j] + t], (ull)hist[t]);
-        hist[t] = 0;
+        const int32_t k = ch[t];
+        if (k) lbase[t] = (long long)atomicAdd((ull *)&B.pcur[B.pbase[j] + t], (ull)k);
     }
     __syncthreads();
     for (int64_t t = t0 + threadIdx.x; t < t1; t += 256) {
@@ -648,11 +648,11 @@ __global__ void k_cr_big_prefix(CoarsenArgs a, BigSegArgs B, const int64_t *ucsc
 
 void launch_cr_big_setup(hipStream_t s, const int32_t *bsegs, int32_t nbs, const int64_t *slen,
                          int32_t target, int chunk_mul, int32_t *bP, int64_t *bP64, int64_t *bch,
-                         int64_t *blen)
+                         int64_t *blen, int64_t *bpc)
 {
     if (nbs > 0)
         hipLaunchKernelGGL(k_cr_big_setup, grid_for(nbs, 256), 256, 0, s, bsegs, nbs, slen, target,
-                           chunk_mul, bP, bP64, bch, blen);
+                           chunk_mul, bP, bP64, bch, blen, bpc);
 }
 
 int run_cr_big(Arena &A, hipStream_t s, const CoarsenArgs &a, const BigSegArgs &B, int32_t nbs,

@@ -180,7 +180,10 @@ struct HubArgs {
     const int32_t *chunk_hub; // hub of each chunk
     const int32_t *chunk_k;   // chunk number within its hub
     int32_t *tmp_c;           // community of each hub entry (entry order)
-    PEntry *buf;              // partitioned (community, weight) entries
+    PEntry *buf;              // partitioned (community, weight) entries, or (narrow
+    PEntry8 *buf8;            // levels) buf8 instead
+    int32_t *chist;           // per chunk its P partition counts, at hbase[j] + k * P_j
+    const int64_t *hbase;     // exclusive scan over hubs of chunks * P
     int64_t *pcount;          // entries per partition
     const int64_t *poff;      // partition offsets (exclusive scan of pcount)
     int64_t *pcur;            // scatter cursors
@@ -226,6 +229,8 @@ struct BigSegArgs {
     int64_t *pcur;
     PEntry *pe;             // partitioned entries, or (narrow levels) pe8 instead
     PEntry8 *pe8;
+    int32_t *chist;         // per chunk its P partition counts, at hbase[j] + k * P_j
+    const int64_t *hbase;   // exclusive scan over big segments of chunks * P
     int32_t *buy;           // uniques per partition, at poff[p]
     int64_t *buw;
     int32_t *fbk;           // fallback tables
@@ -370,7 +375,7 @@ void launch_rows_check(hipStream_t s, const uint32_t *skeys, const int32_t *col,
                        DevFlags *f);
 void launch_hub_setup(hipStream_t s, const int32_t *hv, int32_t nh, const int64_t *rowptr,
                       const int64_t *selfsum, int32_t target, int chunk_mul, int64_t *hq,
-                      int32_t *hP, int64_t *hP64, int64_t *hch);
+                      int32_t *hP, int64_t *hP64, int64_t *hch, int64_t *hpc);
 // owner[i] = segment j of i under exclusive offsets base[0..nseg]; rank[i] =
 // i - base[j] (rank may be null).
 void launch_expand_owner(hipStream_t s, const int64_t *base, int32_t nseg, int64_t total,
@@ -425,7 +430,7 @@ void launch_cr_seg_tier(hipStream_t s, const int32_t *gstart, const int64_t *est
 void launch_cr_agg_tier(hipStream_t s, int tier, const CoarsenArgs &a, const int32_t *segs, int32_t ns);
 void launch_cr_big_setup(hipStream_t s, const int32_t *bsegs, int32_t nbs, const int64_t *slen,
                          int32_t target, int chunk_mul, int32_t *bP, int64_t *bP64, int64_t *bch,
-                         int64_t *blen);
+                         int64_t *blen, int64_t *bpc);
 int run_cr_big(Arena &A, hipStream_t s, const CoarsenArgs &a, const BigSegArgs &B, int32_t nbs,
                int64_t nparts, int64_t nchunks);
 void launch_cr_rowcnt(hipStream_t s, const int32_t *newid_g, const int32_t *nuniq, int32_t K,

@@ -1788,7 +1788,7 @@ void launch_rows_check(hipStream_t s, const uint32_t *skeys, const int32_t *col,
 
 __global__ void k_hub_setup(const int32_t *hv, int32_t nh, const int64_t *rowptr,
                             const int64_t *selfsum, int32_t target, int chunk_mul, int64_t *hq,
-                            int32_t *hP, int64_t *hP64, int64_t *hch)
+                            int32_t *hP, int64_t *hP64, int64_t *hch, int64_t *hpc)
 {
     GRID_STRIDE(j, nh)
     {
@@ -1802,6 +1802,7 @@ __global__ void k_hub_setup(const int32_t *hv, int32_t nh, const int64_t *rowptr
         hP64[j] = P;
         const int64_t cs = chunk_span((int)P, chunk_mul);
         hch[j] = (q + cs - 1) / cs;
+        hpc[j] = hch[j] * P;
     }
 }
 
@@ -1841,11 +1842,11 @@ void launch_widen_i32(hipStream_t s, const int32_t *in, int64_t n, int64_t *out)
 
 void launch_hub_setup(hipStream_t s, const int32_t *hv, int32_t nh, const int64_t *rowptr,
                       const int64_t *selfsum, int32_t target, int chunk_mul, int64_t *hq,
-                      int32_t *hP, int64_t *hP64, int64_t *hch)
+                      int32_t *hP, int64_t *hP64, int64_t *hch, int64_t *hpc)
 {
     if (nh > 0)
         hipLaunchKernelGGL(k_hub_setup, grid_for(nh, 256), 256, 0, s, hv, nh, rowptr, selfsum,
-                           target, chunk_mul, hq, hP, hP64, hch);
+                           target, chunk_mul, hq, hP, hP64, hch, hpc);
 }
 
 // Entry t of hub j: t < d is row entry rowptr[v]+t, t == d the virtual
@@ -1872,8 +1873,21 @@ __global__ __launch_bounds__(256) void k_hub_count(SweepArgs a, HubArgs h)
         atomicAdd(&hist[hub_part(c, P)], 1);
     }
     __syncthreads();
-    for (int t = threadIdx.x; t < P; t += 256)
+    int32_t *ch = h.chist + h.hbase[j] + (int64_t)h.chunk_k[ci] * P;
+    for (int t = threadIdx.x; t < P; t += 256) {
+        ch[t] = hist[t];  // the scatter reserves the chunk's ranges from these
         if (hist[t]) atomicAdd((ull *)&h.pcount[h.pbase[j] + t], (ull)hist[t]);
+    }
+}
+
+// Partitioned hub entry e as (community, weight).
+__device__ __forceinline__ PEntry hub_entry(const HubArgs &h, int64_t e)
+{
+    if (h.buf8) {
+        const PEntry8 x = h.buf8[e];
+        return PEntry{x.c, 0, (int64_t)x.w};
+    }
+    return h.buf[e];
 }
 
 template <bool W4>
@@ -1893,22 +1907,26 @@ __global__ __launch_bounds__(256) void k_hub_scatter(SweepArgs a, HubArgs h)
     const int64_t t0 = (int64_t)h.chunk_k[ci] * cs;
     const int64_t t1 = min(q, t0 + cs);
     const int32_t *tc = h.tmp_c + h.ebase[j];
-    for (int64_t t = t0 + threadIdx.x; t < t1; t += 256) atomicAdd(&hist[hub_part(tc[t], P)], 1);
-    __syncthreads();
+    const int32_t *ch = h.chist + h.hbase[j] + (int64_t)h.chunk_k[ci] * P;
     for (int t = threadIdx.x; t < P; t += 256) {
-        if (hist[t]) lbase[t] = (long long)atomicAdd((ull *)&h.pcur[h.pbase[j] + t], (ull)hist[t]);
-        hist[t] = 0;
+        const int32_t k = ch[t];
+        if (k) lbase[t] = (long long)atomicAdd((ull *)&h.pcur[h.pbase[j] + t], (ull)k);
     }
     __syncthreads();
     for (int64_t t = t0 + threadIdx.x; t < t1; t += 256) {
         int32_t c = tc[t];
         int part = hub_part(c, P);
         int64_t pos = lbase[part] + atomicAdd(&hist[part], 1);
-        PEntry en;
-        en.c = c;
-        en.pad = 0;
-        en.w = (t < d) ? w_of<W4>(a, rb + t) : 0;
-        h.buf[pos] = en;
+        const int64_t w = (t < d) ? w_of<W4>(a, rb + t) : 0;
+        if (h.buf8) {
+            h.buf8[pos] = PEntry8{c, (uint32_t)w};
+        } else {
+            PEntry en;
+            en.c = c;
+            en.pad = 0;
+            en.w = w;
+            h.buf[pos] = en;
+        }
     }
 }
 
@@ -1945,7 +1963,7 @@ __global__ __launch_bounds__(256) void k_hub_agg(SweepArgs a, HubArgs h)
         PEntry en[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u)
-            if (e0 + u * 256 < end) en[u] = h.buf[e0 + u * 256];
+            if (e0 + u * 256 < end) en[u] = hub_entry(h, e0 + u * 256);
 #pragma unroll
         for (int u = 0; u < 4; ++u)
             if (e0 + u * 256 < end && !lds_insert_bounded(s_key, s_val, l2, en[u].c, en[u].w))
@@ -1969,7 +1987,10 @@ __global__ __launch_bounds__(256) void k_hub_agg(SweepArgs a, HubArgs h)
         }
         __threadfence();
         __syncthreads();
-        for (int64_t e = beg + tid; e < end; e += 256) glb_insert_mod(gk, gv, cap, h.buf[e].c, h.buf[e].w);
+        for (int64_t e = beg + tid; e < end; e += 256) {
+            const PEntry en = hub_entry(h, e);
+            glb_insert_mod(gk, gv, cap, en.c, en.w);
+        }
         __threadfence();
         __syncthreads();
         for (int64_t s = tid; s < cap; s += 256) {
