@@ -304,18 +304,20 @@ void launch_cr_seg_tier(hipStream_t s, const int32_t *gstart, const int64_t *est
 
 // Segments of up to 2048 entries: G lanes (or a block) per segment, LDS table
 // sized to the segment; uniques are written at sb[g] in ey's index space.
-template <int G, int LOG2CAP, int MINLOG2>
+// N32: a narrow level, whose sums all stay below 2^32, keeps 4-byte table sums.
+template <int G, int LOG2CAP, int MINLOG2, bool N32>
 __global__ __launch_bounds__(256) void k_cr_agg_group(CoarsenArgs a, const int32_t *__restrict__ segs,
                                                      int32_t ns)
 {
+    using V = TabV<N32>;
     constexpr int CAP = 1 << LOG2CAP;
     constexpr int GPB = 256 / G;
     __shared__ int32_t s_key[GPB * CAP];
-    __shared__ ull s_val[GPB * CAP];
+    __shared__ V s_val[GPB * CAP];
     __shared__ int s_cnt[GPB];
     const int lg = threadIdx.x % G, grp = threadIdx.x / G;
     int32_t *key = s_key + grp * CAP;
-    ull *val = s_val + grp * CAP;
+    V *val = s_val + grp * CAP;
     for (int64_t base = (int64_t)blockIdx.x * GPB; base < ns; base += (int64_t)gridDim.x * GPB) {
         const int64_t vi = base + grp;
         const bool active = vi < ns;
@@ -374,13 +376,13 @@ __global__ __launch_bounds__(256) void k_cr_agg_group(CoarsenArgs a, const int32
     }
 }
 
-template <int LOG2CAP, int MINLOG2>
+template <int LOG2CAP, int MINLOG2, bool N32>
 __global__ __launch_bounds__(256) void k_cr_agg_block(CoarsenArgs a, const int32_t *__restrict__ segs,
                                                      int32_t ns)
 {
     constexpr int CAP = 1 << LOG2CAP;
     __shared__ int32_t s_key[CAP];
-    __shared__ ull s_val[CAP];
+    __shared__ TabV<N32> s_val[CAP];
     __shared__ int s_cnt;
     __shared__ ull s_inter[4], s_tot[4];
     const int tid = threadIdx.x;
@@ -439,21 +441,32 @@ static void launch_seg(KernelT k, int64_t blocks, hipStream_t s, const CoarsenAr
     hipLaunchKernelGGL(k, g, 256, 0, s, a, segs, ns);
 }
 
+template <bool N32>
+static void launch_cr_agg_tier_t(hipStream_t s, int tier, const CoarsenArgs &a,
+                                 const int32_t *segs, int32_t ns)
+{
+    switch (tier) {
+    case 1: launch_seg(k_cr_agg_group<4, 3, 3, N32>, (ns + 63) / 64, s, a, segs, ns); break;
+    case 2: launch_seg(k_cr_agg_group<8, 4, 4, N32>, (ns + 31) / 32, s, a, segs, ns); break;
+    case 3: launch_seg(k_cr_agg_group<16, 5, 5, N32>, (ns + 15) / 16, s, a, segs, ns); break;
+    case 4: launch_seg(k_cr_agg_group<32, 6, 6, N32>, (ns + 7) / 8, s, a, segs, ns); break;
+    case 5: launch_seg(k_cr_agg_group<64, 7, 7, N32>, (ns + 3) / 4, s, a, segs, ns); break;
+    case 6: launch_seg(k_cr_agg_group<64, 8, 8, N32>, (ns + 3) / 4, s, a, segs, ns); break;
+    case 7: launch_seg(k_cr_agg_group<64, 9, 9, N32>, (ns + 3) / 4, s, a, segs, ns); break;
+    case 8: launch_seg(k_cr_agg_group<64, 10, 10, N32>, (ns + 3) / 4, s, a, segs, ns); break;
+    case 9: launch_seg(k_cr_agg_block<12, 11, N32>, ns, s, a, segs, ns); break;
+    default: break;
+    }
+}
+
+// Narrow levels (the 4-byte entry weights ew32) aggregate in 4-byte sums.
 void launch_cr_agg_tier(hipStream_t s, int tier, const CoarsenArgs &a, const int32_t *segs, int32_t ns)
 {
     if (ns <= 0) return;
-    switch (tier) {
-    case 1: launch_seg(k_cr_agg_group<4, 3, 3>, (ns + 63) / 64, s, a, segs, ns); break;
-    case 2: launch_seg(k_cr_agg_group<8, 4, 4>, (ns + 31) / 32, s, a, segs, ns); break;
-    case 3: launch_seg(k_cr_agg_group<16, 5, 5>, (ns + 15) / 16, s, a, segs, ns); break;
-    case 4: launch_seg(k_cr_agg_group<32, 6, 6>, (ns + 7) / 8, s, a, segs, ns); break;
-    case 5: launch_seg(k_cr_agg_group<64, 7, 7>, (ns + 3) / 4, s, a, segs, ns); break;
-    case 6: launch_seg(k_cr_agg_group<64, 8, 8>, (ns + 3) / 4, s, a, segs, ns); break;
-    case 7: launch_seg(k_cr_agg_group<64, 9, 9>, (ns + 3) / 4, s, a, segs, ns); break;
-    case 8: launch_seg(k_cr_agg_group<64, 10, 10>, (ns + 3) / 4, s, a, segs, ns); break;
-    case 9: launch_seg(k_cr_agg_block<12, 11>, ns, s, a, segs, ns); break;
-    default: break;
-    }
+    if (a.ew32)
+        launch_cr_agg_tier_t<true>(s, tier, a, segs, ns);
+    else
+        launch_cr_agg_tier_t<false>(s, tier, a, segs, ns);
 }
 
 // ---------------------------------------------------------------------------
@@ -566,11 +579,12 @@ __global__ __launch_bounds__(256) void k_cr_big_scatter(CoarsenArgs a, BigSegArg
     }
 }
 
+template <bool N32>
 __global__ __launch_bounds__(256) void k_cr_big_agg(BigSegArgs B)
 {
     constexpr int CAP = 1 << kHubAggLog2;
     __shared__ int32_t s_key[CAP];
-    __shared__ ull s_val[CAP];
+    __shared__ TabV<N32> s_val[CAP];
     __shared__ int s_over;
     __shared__ int s_cnt;
     const int tid = threadIdx.x;
@@ -665,7 +679,10 @@ int run_cr_big(Arena &A, hipStream_t s, const CoarsenArgs &a, const BigSegArgs &
     LV_HIP(hipMemcpyAsync(B.pcur, B.poff, sizeof(int64_t) * (size_t)nparts,
                           hipMemcpyDeviceToDevice, s));
     hipLaunchKernelGGL(k_cr_big_scatter, (int)nchunks, 256, 0, s, a, B);
-    hipLaunchKernelGGL(k_cr_big_agg, (int)nparts, 256, 0, s, B);
+    if (B.pe8)  // narrow level
+        hipLaunchKernelGGL(k_cr_big_agg<true>, (int)nparts, 256, 0, s, B);
+    else
+        hipLaunchKernelGGL(k_cr_big_agg<false>, (int)nparts, 256, 0, s, B);
     int64_t *ucscan = nullptr, *uc64 = nullptr;
     LV_TRY(arena_alloc(A, uc64, nparts));
     LV_TRY(arena_alloc(A, ucscan, nparts + 1));

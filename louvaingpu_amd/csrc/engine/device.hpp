@@ -3,6 +3,7 @@
 #pragma once
 
 #include <climits>
+#include <type_traits>
 
 #include "engine.hpp"
 
@@ -306,8 +307,15 @@ __device__ __forceinline__ uint32_t hslot(int32_t c, int log2cap)
     return ((uint32_t)c * 0x9E3779B1u) >> (32 - log2cap);
 }
 
-__device__ __forceinline__ void lds_insert(int32_t *key, ull *val, int log2cap, int32_t c,
-                                           int64_t w)
+// LDS tables keep their sums (V) and community totals (D) in 8 bytes, or in 4
+// on narrow levels (total degree below 2^32 bounds every sum and total).
+template <bool N32>
+using TabV = typename std::conditional<N32, uint32_t, ull>::type;
+template <bool N32>
+using TabD = typename std::conditional<N32, uint32_t, int64_t>::type;
+
+template <typename V>
+__device__ __forceinline__ void lds_insert(int32_t *key, V *val, int log2cap, int32_t c, int64_t w)
 {
     const uint32_t mask = (1u << log2cap) - 1u;
     uint32_t s = hslot(c, log2cap);
@@ -320,13 +328,14 @@ __device__ __forceinline__ void lds_insert(int32_t *key, ull *val, int log2cap, 
         }
         s = (s + 1) & mask;
     }
-    if (w != 0) atomicAdd(&val[s], (ull)w);
+    if (w != 0) atomicAdd(&val[s], (V)w);
 }
 
 // As lds_insert; the lane that creates a slot also stores the key's
 // community total d in dg[slot].
-__device__ __forceinline__ void lds_insert_d(int32_t *key, ull *val, int64_t *dg, int log2cap,
-                                             int32_t c, int64_t w, int64_t d)
+template <typename V, typename D>
+__device__ __forceinline__ void lds_insert_d(int32_t *key, V *val, D *dg, int log2cap, int32_t c,
+                                             int64_t w, int64_t d)
 {
     const uint32_t mask = (1u << log2cap) - 1u;
     uint32_t s = hslot(c, log2cap);
@@ -336,14 +345,14 @@ __device__ __forceinline__ void lds_insert_d(int32_t *key, ull *val, int64_t *dg
         if (k == -1) {
             int32_t prev = atomicCAS(&key[s], -1, c);
             if (prev == -1) {
-                dg[s] = d;
+                dg[s] = (D)d;
                 break;
             }
             if (prev == c) break;
         }
         s = (s + 1) & mask;
     }
-    if (w != 0) atomicAdd(&val[s], (ull)w);
+    if (w != 0) atomicAdd(&val[s], (V)w);
 }
 
 __device__ __forceinline__ bool better(double g1, int32_t c1, double g2, int32_t c2)
@@ -387,7 +396,8 @@ __host__ __device__ inline int64_t chunk_span(int P, int mul)
 }
 
 // Insert with at most 2^log2cap probes; false when the table is full.
-__device__ __forceinline__ bool lds_insert_bounded(int32_t *key, ull *val, int log2cap, int32_t c,
+template <typename V>
+__device__ __forceinline__ bool lds_insert_bounded(int32_t *key, V *val, int log2cap, int32_t c,
                                                    int64_t w)
 {
     const uint32_t mask = (1u << log2cap) - 1u;
@@ -397,11 +407,11 @@ __device__ __forceinline__ bool lds_insert_bounded(int32_t *key, ull *val, int l
         if (k == -1) {
             int32_t prev = atomicCAS(&key[s], -1, c);
             if (prev == -1 || prev == c) {
-                if (w != 0) atomicAdd(&val[s], (ull)w);
+                if (w != 0) atomicAdd(&val[s], (V)w);
                 return true;
             }
         } else if (k == c) {
-            if (w != 0) atomicAdd(&val[s], (ull)w);
+            if (w != 0) atomicAdd(&val[s], (V)w);
             return true;
         }
         s = (s + 1) & mask;

@@ -1121,8 +1121,8 @@ __device__ __forceinline__ void consider(int32_t c, int64_t W, int64_t degc, dou
 // The candidates of an LDS table, slots s0, s0 + S, ... below cap, with the
 // community totals gathered four at a time (the order of consider calls does
 // not change the result).
-template <int S>
-__device__ __forceinline__ void consider_slots(const SweepArgs &a, const int32_t *key, const ull *val,
+template <int S, typename V>
+__device__ __forceinline__ void consider_slots(const SweepArgs &a, const int32_t *key, const V *val,
                                                int s0, int cap, double gamma, double f, int &cnt,
                                                double &bg, int32_t &bc, int64_t &bw)
 {
@@ -1289,8 +1289,8 @@ void launch_move_deltas(hipStream_t s, const SweepArgs &a, int32_t n, bool own_d
 // With DL the gather is the neighbour's 16-byte label record and the
 // community total goes into the table (dg); otherwise only C[target] is
 // gathered and the totals are read per candidate later.
-template <int S, bool DL, bool W4>
-__device__ __forceinline__ void insert_row(const SweepArgs &a, int32_t *key, ull *val, int64_t *dg,
+template <int S, bool DL, bool W4, typename V, typename D>
+__device__ __forceinline__ void insert_row(const SweepArgs &a, int32_t *key, V *val, D *dg,
                                            int l2, int64_t beg, int64_t end, int lane)
 {
     for (int64_t p0 = beg + lane; p0 < end; p0 += 4 * S) {
@@ -1324,19 +1324,21 @@ __device__ __forceinline__ void insert_row(const SweepArgs &a, int32_t *key, ull
 
 // G lanes per vertex, 256/G vertices per block, an LDS table of up to
 // 2^LOG2CAP slots per vertex (2^MINLOG2 .. 2^LOG2CAP sized to the row).
-template <int G, int LOG2CAP, int MINLOG2, bool DL, bool B, bool W4>
+template <int G, int LOG2CAP, int MINLOG2, bool DL, bool B, bool W4, bool N32>
 __global__ __launch_bounds__(256) void k_sweep_group(SweepArgs a, const int32_t *__restrict__ verts,
                                                     int32_t nv)
 {
+    using V = TabV<N32>;
+    using D = TabD<N32>;
     constexpr int CAP = 1 << LOG2CAP;
     constexpr int GPB = 256 / G;
     __shared__ int32_t s_key[GPB * CAP];
-    __shared__ ull s_val[GPB * CAP];
-    __shared__ int64_t s_dg[DL ? GPB * CAP : 1];
+    __shared__ V s_val[GPB * CAP];
+    __shared__ D s_dg[DL ? GPB * CAP : 1];
     const int lg = threadIdx.x % G, grp = threadIdx.x / G;
     int32_t *key = s_key + grp * CAP;
-    ull *val = s_val + grp * CAP;
-    int64_t *dg = s_dg + (DL ? grp * CAP : 0);
+    V *val = s_val + grp * CAP;
+    D *dg = s_dg + (DL ? grp * CAP : 0);
     for (int64_t base = (int64_t)blockIdx.x * GPB; base < nv; base += (int64_t)gridDim.x * GPB) {
         const int64_t vi = base + grp;
         const bool active = vi < nv;
@@ -1381,7 +1383,8 @@ __global__ __launch_bounds__(256) void k_sweep_group(SweepArgs a, const int32_t 
             if constexpr (DL) {
                 for (int s = lg; s < cap; s += G) {
                     const int32_t c = key[s];
-                    if (c >= 0) consider(c, (int64_t)val[s], dg[s], vp.gamma, f, cnt, bg, bc, bw);
+                    if (c >= 0)
+                        consider(c, (int64_t)val[s], (int64_t)dg[s], vp.gamma, f, cnt, bg, bc, bw);
                 }
             } else {
                 consider_slots<G>(a, key, val, lg, cap, vp.gamma, f, cnt, bg, bc, bw);
@@ -1395,14 +1398,15 @@ __global__ __launch_bounds__(256) void k_sweep_group(SweepArgs a, const int32_t 
 
 // One vertex per block of BLOCK threads with an LDS table of up to
 // 2^LOG2CAP slots sized to the row.
-template <int BLOCK, int LOG2CAP, int MINLOG2, bool B, bool W4>
+template <int BLOCK, int LOG2CAP, int MINLOG2, bool B, bool W4, bool N32>
 __global__ __launch_bounds__(BLOCK) void k_sweep_block(SweepArgs a, const int32_t *__restrict__ verts,
                                                       int32_t nv)
 {
+    using V = TabV<N32>;
     constexpr int CAP = 1 << LOG2CAP;
     constexpr int NW = BLOCK / 64;
     __shared__ int32_t s_key[CAP];
-    __shared__ ull s_val[CAP];
+    __shared__ V s_val[CAP];
     __shared__ int r_cnt[NW];
     __shared__ double r_g[NW];
     __shared__ int32_t r_c[NW];
@@ -1422,7 +1426,7 @@ __global__ __launch_bounds__(BLOCK) void k_sweep_block(SweepArgs a, const int32_
             s_val[s] = 0;
         }
         __syncthreads();
-        insert_row<BLOCK, false, W4>(a, s_key, s_val, nullptr, l2, beg, end, tid);
+        insert_row<BLOCK, false, W4>(a, s_key, s_val, (TabD<N32> *)nullptr, l2, beg, end, tid);
         if (tid == 0 && loop) lds_insert(s_key, s_val, l2, me.c, 0);
         __syncthreads();
         int cnt = 0;
@@ -1466,20 +1470,20 @@ static void launch_grid(KernelT k, int64_t blocks, int block, hipStream_t s, con
     hipLaunchKernelGGL(k, g, block, 0, s, a, verts, nv);
 }
 
-template <bool B, bool W4>
+template <bool B, bool W4, bool N32>
 static void launch_sweep_tier_t(hipStream_t s, int tier, const SweepArgs &a, const int32_t *verts,
                                 int32_t nv)
 {
     switch (tier) {
-    case 1: launch_grid(k_sweep_group<4, 3, 3, true, B, W4>, (nv + 63) / 64, 256, s, a, verts, nv); break;
-    case 2: launch_grid(k_sweep_group<8, 4, 4, true, B, W4>, (nv + 31) / 32, 256, s, a, verts, nv); break;
-    case 3: launch_grid(k_sweep_group<16, 5, 5, true, B, W4>, (nv + 15) / 16, 256, s, a, verts, nv); break;
-    case 4: launch_grid(k_sweep_group<32, 6, 6, true, B, W4>, (nv + 7) / 8, 256, s, a, verts, nv); break;
-    case 5: launch_grid(k_sweep_group<64, 7, 7, true, B, W4>, (nv + 3) / 4, 256, s, a, verts, nv); break;
-    case 6: launch_grid(k_sweep_group<64, 8, 8, true, B, W4>, (nv + 3) / 4, 256, s, a, verts, nv); break;
-    case 7: launch_grid(k_sweep_group<64, 9, 9, true, B, W4>, (nv + 3) / 4, 256, s, a, verts, nv); break;
-    case 8: launch_grid(k_sweep_group<64, 10, 10, false, B, W4>, (nv + 3) / 4, 256, s, a, verts, nv); break;
-    case 9: launch_grid(k_sweep_block<256, 12, 11, B, W4>, nv, 256, s, a, verts, nv); break;
+    case 1: launch_grid(k_sweep_group<4, 3, 3, true, B, W4, N32>, (nv + 63) / 64, 256, s, a, verts, nv); break;
+    case 2: launch_grid(k_sweep_group<8, 4, 4, true, B, W4, N32>, (nv + 31) / 32, 256, s, a, verts, nv); break;
+    case 3: launch_grid(k_sweep_group<16, 5, 5, true, B, W4, N32>, (nv + 15) / 16, 256, s, a, verts, nv); break;
+    case 4: launch_grid(k_sweep_group<32, 6, 6, true, B, W4, N32>, (nv + 7) / 8, 256, s, a, verts, nv); break;
+    case 5: launch_grid(k_sweep_group<64, 7, 7, true, B, W4, N32>, (nv + 3) / 4, 256, s, a, verts, nv); break;
+    case 6: launch_grid(k_sweep_group<64, 8, 8, true, B, W4, N32>, (nv + 3) / 4, 256, s, a, verts, nv); break;
+    case 7: launch_grid(k_sweep_group<64, 9, 9, true, B, W4, N32>, (nv + 3) / 4, 256, s, a, verts, nv); break;
+    case 8: launch_grid(k_sweep_group<64, 10, 10, false, B, W4, N32>, (nv + 3) / 4, 256, s, a, verts, nv); break;
+    case 9: launch_grid(k_sweep_block<256, 12, 11, B, W4, N32>, nv, 256, s, a, verts, nv); break;
     default: break;
     }
 }
@@ -1489,13 +1493,23 @@ void launch_sweep_tier(hipStream_t s, int tier, const SweepArgs &a, const int32_
                        bool batch)
 {
     if (nv <= 0) return;
-    const bool w4 = a.w32 != nullptr;
-    if (batch)
-        w4 ? launch_sweep_tier_t<true, true>(s, tier, a, verts, nv)
-           : launch_sweep_tier_t<true, false>(s, tier, a, verts, nv);
-    else
-        w4 ? launch_sweep_tier_t<false, true>(s, tier, a, verts, nv)
-           : launch_sweep_tier_t<false, false>(s, tier, a, verts, nv);
+    // 4-byte tables where the level is narrow and keeps 4-byte weights
+    const bool w4 = a.w32 != nullptr, n32 = w4 && a.lab8 != nullptr;
+    if (batch) {
+        if (n32)
+            launch_sweep_tier_t<true, true, true>(s, tier, a, verts, nv);
+        else if (w4)
+            launch_sweep_tier_t<true, true, false>(s, tier, a, verts, nv);
+        else
+            launch_sweep_tier_t<true, false, false>(s, tier, a, verts, nv);
+    } else {
+        if (n32)
+            launch_sweep_tier_t<false, true, true>(s, tier, a, verts, nv);
+        else if (w4)
+            launch_sweep_tier_t<false, true, false>(s, tier, a, verts, nv);
+        else
+            launch_sweep_tier_t<false, false, false>(s, tier, a, verts, nv);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1931,12 +1945,12 @@ __global__ __launch_bounds__(256) void k_hub_scatter(SweepArgs a, HubArgs h)
 }
 
 
-template <bool B>
+template <bool B, bool N32>
 __global__ __launch_bounds__(256) void k_hub_agg(SweepArgs a, HubArgs h)
 {
     constexpr int CAP = 1 << kHubAggLog2;
     __shared__ int32_t s_key[CAP];
-    __shared__ ull s_val[CAP];
+    __shared__ TabV<N32> s_val[CAP];
     __shared__ int s_over;
     __shared__ int r_cnt[4];
     __shared__ double r_g[4];
@@ -2070,11 +2084,19 @@ int run_hub_sweep(Arena &A, hipStream_t s, const SweepArgs &a, const HubArgs &h,
         hipLaunchKernelGGL(k_hub_scatter<true>, (int)nchunks, 256, 0, s, a, h);
     else
         hipLaunchKernelGGL(k_hub_scatter<false>, (int)nchunks, 256, 0, s, a, h);
+    // a partition's sums stay below a narrow level's total degree: 4-byte table
+    const bool n32 = a.lab8 != nullptr;
     if (batch) {
-        hipLaunchKernelGGL(k_hub_agg<true>, (int)nparts, 256, 0, s, a, h);
+        if (n32)
+            hipLaunchKernelGGL((k_hub_agg<true, true>), (int)nparts, 256, 0, s, a, h);
+        else
+            hipLaunchKernelGGL((k_hub_agg<true, false>), (int)nparts, 256, 0, s, a, h);
         hipLaunchKernelGGL(k_hub_final<true>, (int)((nh + 3) / 4), 256, 0, s, a, h, nh);
     } else {
-        hipLaunchKernelGGL(k_hub_agg<false>, (int)nparts, 256, 0, s, a, h);
+        if (n32)
+            hipLaunchKernelGGL((k_hub_agg<false, true>), (int)nparts, 256, 0, s, a, h);
+        else
+            hipLaunchKernelGGL((k_hub_agg<false, false>), (int)nparts, 256, 0, s, a, h);
         hipLaunchKernelGGL(k_hub_final<false>, (int)((nh + 3) / 4), 256, 0, s, a, h, nh);
     }
     LV_HIP(hipGetLastError());
