@@ -302,6 +302,52 @@ void launch_cr_seg_tier(hipStream_t s, const int32_t *gstart, const int64_t *est
                            newid_g, K, sb, slen, tkey, sid, tier_cnt);
 }
 
+// Writes the occupied slots (key >= 0) of a 256-thread block's LDS table of
+// `cap` <= 256 * SPT slots compactly to oy/ow, in slot order, and returns how
+// many on every thread.  Each thread stages its slots in registers, so the
+// cross-wave prefix goes through the table's own first words and the kernel
+// keeps no counter in LDS (a 32 KiB table leaves room for five blocks per CU).
+// Ends with a barrier: the caller may reuse the table at once.
+template <int SPT, typename V>
+__device__ __forceinline__ int table_compact(int32_t *key, const V *val, int cap, int32_t *oy,
+                                             int64_t *ow)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    int32_t ky[SPT];
+    V vv[SPT];
+    int mine = 0;
+#pragma unroll
+    for (int k = 0; k < SPT; ++k) {
+        const int s = tid + k * 256;
+        ky[k] = s < cap ? key[s] : -1;
+        vv[k] = ky[k] >= 0 ? val[s] : (V)0;
+        mine += ky[k] >= 0 ? 1 : 0;
+    }
+    int inc = mine;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int o = __shfl_up(inc, off, 64);
+        if (lane >= off) inc += o;
+    }
+    __syncthreads();  // the table is read: its first words take the wave totals
+    if (lane == 63) key[wave] = inc;
+    __syncthreads();
+    int pos = inc - mine, total = 0;
+    for (int w = 0; w < 4; ++w) {
+        const int t = key[w];
+        if (w < wave) pos += t;
+        total += t;
+    }
+#pragma unroll
+    for (int k = 0; k < SPT; ++k)
+        if (ky[k] >= 0) {
+            oy[pos] = ky[k];
+            ow[pos] = (int64_t)vv[k];
+            ++pos;
+        }
+    __syncthreads();
+    return total;
+}
+
 // Segments of up to 2048 entries: G lanes (or a block) per segment, LDS table
 // sized to the segment; uniques are written at sb[g] in ey's index space.
 // N32: a narrow level, whose sums all stay below 2^32, keeps 4-byte table sums.
@@ -314,7 +360,11 @@ __global__ __launch_bounds__(256) void k_cr_agg_group(CoarsenArgs a, const int32
     constexpr int GPB = 256 / G;
     __shared__ int32_t s_key[GPB * CAP];
     __shared__ V s_val[GPB * CAP];
-    __shared__ int s_cnt[GPB];
+    int *s_cnt = nullptr;  // groups narrower than a wave count their uniques in LDS
+    if constexpr (G < 64) {
+        __shared__ int cnt_lds[GPB];
+        s_cnt = cnt_lds;
+    }
     const int lg = threadIdx.x % G, grp = threadIdx.x / G;
     int32_t *key = s_key + grp * CAP;
     V *val = s_val + grp * CAP;
@@ -336,7 +386,8 @@ __global__ __launch_bounds__(256) void k_cr_agg_group(CoarsenArgs a, const int32
             key[s] = -1;
             val[s] = 0;
         }
-        if (lg == 0) s_cnt[grp] = 0;
+        if constexpr (G < 64)
+            if (lg == 0) s_cnt[grp] = 0;
         __syncthreads();
         ull inter = 0, tot = 0;
         if (active) {
@@ -351,13 +402,28 @@ __global__ __launch_bounds__(256) void k_cr_agg_group(CoarsenArgs a, const int32
             }
         }
         __syncthreads();
+        int nun = 0;  // (G == 64) uniques of the wave's group
         if (active) {
-            for (int s = lg; s < cap; s += G) {
-                int32_t y = key[s];
-                if (y >= 0) {
-                    int pos = atomicAdd(&s_cnt[grp], 1);
-                    a.uy[b + pos] = y;
-                    a.uw[b + pos] = (int64_t)val[s];
+            if constexpr (G == 64) {
+                const ull below = (1ull << lg) - 1ull;
+                for (int s = lg; s < cap; s += 64) {
+                    const int32_t y = key[s];
+                    const ull m = __ballot(y >= 0);
+                    if (y >= 0) {
+                        const int pos = nun + __popcll(m & below);
+                        a.uy[b + pos] = y;
+                        a.uw[b + pos] = (int64_t)val[s];
+                    }
+                    nun += __popcll(m);
+                }
+            } else {
+                for (int s = lg; s < cap; s += G) {
+                    int32_t y = key[s];
+                    if (y >= 0) {
+                        int pos = atomicAdd(&s_cnt[grp], 1);
+                        a.uy[b + pos] = y;
+                        a.uw[b + pos] = (int64_t)val[s];
+                    }
                 }
             }
         }
@@ -368,7 +434,10 @@ __global__ __launch_bounds__(256) void k_cr_agg_group(CoarsenArgs a, const int32
         }
         __syncthreads();
         if (active && lg == 0) {
-            a.nuniq[g] = s_cnt[grp];
+            if constexpr (G == 64)
+                a.nuniq[g] = nun;
+            else
+                a.nuniq[g] = s_cnt[grp];
             a.internal[g] = inter;
             a.ssum[g] = tot;
         }
@@ -381,10 +450,8 @@ __global__ __launch_bounds__(256) void k_cr_agg_block(CoarsenArgs a, const int32
                                                      int32_t ns)
 {
     constexpr int CAP = 1 << LOG2CAP;
-    __shared__ int32_t s_key[CAP];
+    __shared__ __align__(16) int32_t s_key[CAP];
     __shared__ TabV<N32> s_val[CAP];
-    __shared__ int s_cnt;
-    __shared__ ull s_inter[4], s_tot[4];
     const int tid = threadIdx.x;
     for (int64_t vi = blockIdx.x; vi < ns; vi += gridDim.x) {
         const int32_t g = segs[vi];
@@ -396,7 +463,6 @@ __global__ __launch_bounds__(256) void k_cr_agg_block(CoarsenArgs a, const int32
             s_key[s] = -1;
             s_val[s] = 0;
         }
-        if (tid == 0) s_cnt = 0;
         __syncthreads();
         ull inter = 0, tot = 0;
         for (int64_t e = b + tid; e < b + L; e += 256) {
@@ -409,25 +475,19 @@ __global__ __launch_bounds__(256) void k_cr_agg_block(CoarsenArgs a, const int32
                 lds_insert(s_key, s_val, l2, y, w);
         }
         __syncthreads();
-        for (int s = tid; s < cap; s += 256) {
-            int32_t y = s_key[s];
-            if (y >= 0) {
-                int pos = atomicAdd(&s_cnt, 1);
-                a.uy[b + pos] = y;
-                a.uw[b + pos] = (int64_t)s_val[s];
-            }
-        }
+        const int nun = table_compact<CAP / 256>(s_key, s_val, cap, a.uy + b, a.uw + b);
         inter = (ull)wave_sum((long long)inter);
         tot = (ull)wave_sum((long long)tot);
+        ull *red = reinterpret_cast<ull *>(s_key);  // the table is free again
         if ((tid & 63) == 0) {
-            s_inter[tid >> 6] = inter;
-            s_tot[tid >> 6] = tot;
+            red[tid >> 6] = inter;
+            red[4 + (tid >> 6)] = tot;
         }
         __syncthreads();
         if (tid == 0) {
-            a.nuniq[g] = s_cnt;
-            a.internal[g] = s_inter[0] + s_inter[1] + s_inter[2] + s_inter[3];
-            a.ssum[g] = s_tot[0] + s_tot[1] + s_tot[2] + s_tot[3];
+            a.nuniq[g] = nun;
+            a.internal[g] = red[0] + red[1] + red[2] + red[3];
+            a.ssum[g] = red[4] + red[5] + red[6] + red[7];
         }
         __syncthreads();
     }
@@ -583,10 +643,8 @@ template <bool N32>
 __global__ __launch_bounds__(256) void k_cr_big_agg(BigSegArgs B)
 {
     constexpr int CAP = 1 << kHubAggLog2;
-    __shared__ int32_t s_key[CAP];
+    __shared__ __align__(16) int32_t s_key[CAP];
     __shared__ TabV<N32> s_val[CAP];
-    __shared__ int s_over;
-    __shared__ int s_cnt;
     const int tid = threadIdx.x;
     const int64_t pi = blockIdx.x;
     const int64_t beg = B.poff[pi], end = B.poff[pi + 1];
@@ -596,10 +654,6 @@ __global__ __launch_bounds__(256) void k_cr_big_agg(BigSegArgs B)
         s_key[s] = -1;
         s_val[s] = 0;
     }
-    if (tid == 0) {
-        s_over = 0;
-        s_cnt = 0;
-    }
     __syncthreads();
     for (int64_t e0 = beg + tid; e0 < end; e0 += 4 * 256) {
         PEntry en[4];
@@ -608,24 +662,21 @@ __global__ __launch_bounds__(256) void k_cr_big_agg(BigSegArgs B)
             if (e0 + u * 256 < end) en[u] = part_entry(B, e0 + u * 256);
 #pragma unroll
         for (int u = 0; u < 4; ++u)
-            if (e0 + u * 256 < end && !lds_insert_bounded(s_key, s_val, l2, en[u].c, en[u].w))
-                s_over = 1;
+            if (e0 + u * 256 < end) (void)lds_insert_bounded(s_key, s_val, l2, en[u].c, en[u].w);
     }
     __syncthreads();
-    if (!s_over) {
-        for (int s = tid; s < cap; s += 256) {
-            int32_t y = s_key[s];
-            if (y >= 0) {
-                int pos = atomicAdd(&s_cnt, 1);
-                B.buy[beg + pos] = y;
-                B.buw[beg + pos] = (int64_t)s_val[s];
-            }
-        }
-    } else {
-        const int64_t cap = 2 * (end - beg);
+    int total = table_compact<CAP / 256>(s_key, s_val, cap, B.buy + beg, B.buw + beg);
+    // An insert that found no slot left the table full, which more than CAP
+    // entries can do: then (or when exactly CAP keys filled it) the partition
+    // is redone in its global table, whose uniques overwrite these.
+    if (total == cap && end - beg > cap) {
+        // more than CAP distinct keys: the partition's global table; the
+        // dropped LDS table's first word counts the uniques
+        const int64_t gcap = 2 * (end - beg);
         int32_t *gk = B.fbk + 2 * beg;
         ull *gv = B.fbv + 2 * beg;
-        for (int64_t s = tid; s < cap; s += 256) {
+        if (tid == 0) s_key[0] = 0;
+        for (int64_t s = tid; s < gcap; s += 256) {
             gk[s] = -1;
             gv[s] = 0;
         }
@@ -633,22 +684,23 @@ __global__ __launch_bounds__(256) void k_cr_big_agg(BigSegArgs B)
         __syncthreads();
         for (int64_t e = beg + tid; e < end; e += 256) {
             const PEntry en = part_entry(B, e);
-            glb_insert_mod(gk, gv, cap, en.c, en.w);
+            glb_insert_mod(gk, gv, gcap, en.c, en.w);
         }
         __threadfence();
         __syncthreads();
-        for (int64_t s = tid; s < cap; s += 256) {
+        for (int64_t s = tid; s < gcap; s += 256) {
             int32_t y = __hip_atomic_load(&gk[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (y >= 0) {
-                int pos = atomicAdd(&s_cnt, 1);
+                int pos = atomicAdd(&s_key[0], 1);
                 B.buy[beg + pos] = y;
                 B.buw[beg + pos] =
                     (int64_t)__hip_atomic_load(&gv[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
         }
+        __syncthreads();
+        total = s_key[0];
     }
-    __syncthreads();
-    if (tid == 0) B.ucnt[pi] = s_cnt;
+    if (tid == 0) B.ucnt[pi] = total;
 }
 
 // ucscan = exclusive scan of ucnt over all partitions (segment-major):

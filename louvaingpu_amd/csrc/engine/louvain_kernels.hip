@@ -1156,6 +1156,43 @@ __device__ __forceinline__ void group_reduce(int &cnt, double &bg, int32_t &bc, 
     }
 }
 
+// The block's candidate count and best candidate from its waves' results
+// (each after group_reduce<64>), passed through `scratch`: LDS of a table that
+// no wave reads any more, so the kernel needs no LDS of its own for them (a
+// 32 KiB table then leaves room for five blocks per CU).  True on thread 0,
+// which holds the result; the caller syncs before the table is reused.
+struct WaveBest {
+    double g;
+    long long w;
+    int cnt;
+    int32_t c;
+};
+
+template <int NW>
+__device__ __forceinline__ bool block_best(void *scratch, int &cnt, double &bg, int32_t &bc,
+                                           int64_t &bw)
+{
+    WaveBest *r = reinterpret_cast<WaveBest *>(scratch);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __syncthreads();  // every wave is done with the table under the scratch
+    if (lane == 0) r[wave] = WaveBest{bg, (long long)bw, cnt, bc};
+    __syncthreads();
+    if (threadIdx.x != 0) return false;
+    cnt = r[0].cnt;
+    bg = r[0].g;
+    bc = r[0].c;
+    bw = r[0].w;
+    for (int w = 1; w < NW; ++w) {
+        cnt += r[w].cnt;
+        if (better(r[w].g, r[w].c, bg, bc)) {
+            bg = r[w].g;
+            bc = r[w].c;
+            bw = r[w].w;
+        }
+    }
+    return true;
+}
+
 // Guard, move and deltas of one vertex (oneLevelKernel + remove/insert,
 // OneLevelKernel.cu:52-78,400-421); on the last pass the deltas are the
 // merge-isolated ones only (:109-171,921-927).  The deltas of an ordinary
@@ -1405,13 +1442,9 @@ __global__ __launch_bounds__(BLOCK) void k_sweep_block(SweepArgs a, const int32_
     using V = TabV<N32>;
     constexpr int CAP = 1 << LOG2CAP;
     constexpr int NW = BLOCK / 64;
-    __shared__ int32_t s_key[CAP];
+    __shared__ __align__(16) int32_t s_key[CAP];
     __shared__ V s_val[CAP];
-    __shared__ int r_cnt[NW];
-    __shared__ double r_g[NW];
-    __shared__ int32_t r_c[NW];
-    __shared__ long long r_w[NW];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x;
     for (int64_t vi = blockIdx.x; vi < nv; vi += gridDim.x) {
         const int32_t v = verts[vi];
         const VParam vp = vparam<B>(a, v);
@@ -1436,28 +1469,8 @@ __global__ __launch_bounds__(BLOCK) void k_sweep_block(SweepArgs a, const int32_
         const double f = (double)a.kdeg[v] / vp.m2;
         consider_slots<BLOCK>(a, s_key, s_val, tid, cap, vp.gamma, f, cnt, bg, bc, bw);
         group_reduce<64>(cnt, bg, bc, bw);
-        if (lane == 0) {
-            r_cnt[wave] = cnt;
-            r_g[wave] = bg;
-            r_c[wave] = bc;
-            r_w[wave] = bw;
-        }
-        __syncthreads();
-        if (tid == 0) {
-            int tc = r_cnt[0];
-            double g0 = r_g[0];
-            int32_t c0 = r_c[0];
-            int64_t w0 = r_w[0];
-            for (int w = 1; w < NW; ++w) {
-                tc += r_cnt[w];
-                if (better(r_g[w], r_c[w], g0, c0)) {
-                    g0 = r_g[w];
-                    c0 = r_c[w];
-                    w0 = r_w[w];
-                }
-            }
-            finalize_vertex(a, v, me.c, me.sz, tc, c0, w0, vp.mode);
-        }
+        if (block_best<NW>(s_key, cnt, bg, bc, bw))
+            finalize_vertex(a, v, me.c, me.sz, cnt, bc, bw, vp.mode);
         __syncthreads();
     }
 }
@@ -1949,14 +1962,9 @@ template <bool B, bool N32>
 __global__ __launch_bounds__(256) void k_hub_agg(SweepArgs a, HubArgs h)
 {
     constexpr int CAP = 1 << kHubAggLog2;
-    __shared__ int32_t s_key[CAP];
+    __shared__ __align__(16) int32_t s_key[CAP];
     __shared__ TabV<N32> s_val[CAP];
-    __shared__ int s_over;
-    __shared__ int r_cnt[4];
-    __shared__ double r_g[4];
-    __shared__ int32_t r_c[4];
-    __shared__ long long r_w[4];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x;
     const int64_t pi = blockIdx.x;
     const int64_t beg = h.poff[pi], end = h.poff[pi + 1];
     const int32_t j = h.part_hub[pi];
@@ -1971,7 +1979,6 @@ __global__ __launch_bounds__(256) void k_hub_agg(SweepArgs a, HubArgs h)
         s_key[s] = -1;
         s_val[s] = 0;
     }
-    if (tid == 0) s_over = 0;
     __syncthreads();
     for (int64_t e0 = beg + tid; e0 < end; e0 += 4 * 256) {
         PEntry en[4];
@@ -1980,22 +1987,31 @@ __global__ __launch_bounds__(256) void k_hub_agg(SweepArgs a, HubArgs h)
             if (e0 + u * 256 < end) en[u] = hub_entry(h, e0 + u * 256);
 #pragma unroll
         for (int u = 0; u < 4; ++u)
-            if (e0 + u * 256 < end && !lds_insert_bounded(s_key, s_val, l2, en[u].c, en[u].w))
-                s_over = 1;
+            if (e0 + u * 256 < end) (void)lds_insert_bounded(s_key, s_val, l2, en[u].c, en[u].w);
     }
     __syncthreads();
     int cnt = 0;
     double bg = 0.0;
     int32_t bc = INT_MAX;
     int64_t bw = 0;
-    if (!s_over) {
-        consider_slots<256>(a, s_key, s_val, tid, cap, vp.gamma, f, cnt, bg, bc, bw);
-    } else {
-        // more than CAP distinct keys: redo the partition in its global region
-        const int64_t cap = 2 * (end - beg);
+    consider_slots<256>(a, s_key, s_val, tid, cap, vp.gamma, f, cnt, bg, bc, bw);
+    group_reduce<64>(cnt, bg, bc, bw);
+    const bool lead = block_best<4>(s_key, cnt, bg, bc, bw);
+    // An insert that found no slot left the table full, which more than CAP
+    // entries can do: then (or when exactly CAP keys filled it) the partition
+    // is redone in its global region.  The flag sits past block_best's scratch.
+    int *redo = s_key + 32;
+    if (lead) *redo = (cnt == cap && end - beg > cap) ? 1 : 0;
+    __syncthreads();
+    if (*redo) {
+        cnt = 0;
+        bg = 0.0;
+        bc = INT_MAX;
+        bw = 0;
+        const int64_t gcap = 2 * (end - beg);
         int32_t *gk = h.fbk + 2 * beg;
         ull *gv = h.fbv + 2 * beg;
-        for (int64_t s = tid; s < cap; s += 256) {
+        for (int64_t s = tid; s < gcap; s += 256) {
             gk[s] = -1;
             gv[s] = 0;
         }
@@ -2003,43 +2019,25 @@ __global__ __launch_bounds__(256) void k_hub_agg(SweepArgs a, HubArgs h)
         __syncthreads();
         for (int64_t e = beg + tid; e < end; e += 256) {
             const PEntry en = hub_entry(h, e);
-            glb_insert_mod(gk, gv, cap, en.c, en.w);
+            glb_insert_mod(gk, gv, gcap, en.c, en.w);
         }
         __threadfence();
         __syncthreads();
-        for (int64_t s = tid; s < cap; s += 256) {
+        for (int64_t s = tid; s < gcap; s += 256) {
             int32_t c = __hip_atomic_load(&gk[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (c >= 0)
                 consider(c, (int64_t)__hip_atomic_load(&gv[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
                          deg_of(a, c), vp.gamma, f, cnt, bg, bc, bw);
         }
+        group_reduce<64>(cnt, bg, bc, bw);
+        if (!block_best<4>(s_key, cnt, bg, bc, bw)) return;
+    } else if (!lead) {
+        return;
     }
-    group_reduce<64>(cnt, bg, bc, bw);
-    if (lane == 0) {
-        r_cnt[wave] = cnt;
-        r_g[wave] = bg;
-        r_c[wave] = bc;
-        r_w[wave] = bw;
-    }
-    __syncthreads();
-    if (tid == 0) {
-        int tc = r_cnt[0];
-        double g0 = r_g[0];
-        int32_t c0 = r_c[0];
-        int64_t w0 = r_w[0];
-        for (int w = 1; w < 4; ++w) {
-            tc += r_cnt[w];
-            if (better(r_g[w], r_c[w], g0, c0)) {
-                g0 = r_g[w];
-                c0 = r_c[w];
-                w0 = r_w[w];
-            }
-        }
-        h.pn[pi] = tc;
-        h.pg[pi] = g0;
-        h.pc[pi] = c0;
-        h.pw[pi] = w0;
-    }
+    h.pn[pi] = cnt;
+    h.pg[pi] = bg;
+    h.pc[pi] = bc;
+    h.pw[pi] = bw;
 }
 
 // One wavefront per hub: combine its partitions, then guard/move as usual.
