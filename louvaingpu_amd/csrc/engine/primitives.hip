@@ -334,6 +334,16 @@ int scan_exclusive_i32(Arena &a, hipStream_t s, const int32_t *in, int32_t *out,
 
 constexpr int kRadixBlock = 256;
 
+// Workgroups are dispatched to the 8 XCDs round-robin.  The radix kernels work
+// on logical block xcd_block(blockIdx): consecutive logical blocks - the ones
+// whose digit-major histogram entries share a cache line - then run on one
+// XCD, so each line is assembled in one L2 instead of from eight.
+__device__ __forceinline__ int64_t xcd_block(int64_t bid, int64_t nblocks)
+{
+    const int64_t q = nblocks >> 3, r = nblocks & 7, x = bid & 7, i = bid >> 3;
+    return x * q + (x < r ? x : r) + i;
+}
+
 template <typename K>
 __global__ __launch_bounds__(kRadixBlock) void k_radix_hist(const K *keys, int64_t n, int shift,
                                                            int rounds, int64_t *hist,
@@ -342,13 +352,23 @@ __global__ __launch_bounds__(kRadixBlock) void k_radix_hist(const K *keys, int64
     __shared__ int s_h[256];
     s_h[threadIdx.x] = 0;
     __syncthreads();
-    int64_t base = (int64_t)blockIdx.x * rounds * kRadixBlock;
-    for (int r = 0; r < rounds; ++r) {
-        int64_t i = base + (int64_t)r * kRadixBlock + threadIdx.x;
-        if (i < n) atomicAdd(&s_h[(int)((keys[i] >> shift) & 255)], 1);
+    const int64_t lb = xcd_block(blockIdx.x, nblocks);
+    int64_t base = lb * rounds * kRadixBlock;
+    // eight independent loads in flight before their eight LDS increments
+    constexpr int U = 8;
+    for (int r = 0; r < rounds; r += U) {
+        int d[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t i = base + (int64_t)(r + u) * kRadixBlock + threadIdx.x;
+            d[u] = (r + u < rounds && i < n) ? (int)((keys[i] >> shift) & 255) : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (d[u] >= 0) atomicAdd(&s_h[d[u]], 1);
     }
     __syncthreads();
-    hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = s_h[threadIdx.x];
+    hist[(int64_t)threadIdx.x * nblocks + lb] = s_h[threadIdx.x];
 }
 
 // Scatter of one digit pass.  A block round covers kRadixTile consecutive
@@ -396,9 +416,10 @@ __global__ __launch_bounds__(kRadixBlock) void k_radix_scatter(const K *kin, con
     __shared__ K s_k[kRadixTile];
     __shared__ V s_v[HV ? kRadixTile : 1];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    s_base[tid] = hscan[(int64_t)tid * nblocks + blockIdx.x];
+    const int64_t lb = xcd_block(blockIdx.x, nblocks);
+    s_base[tid] = hscan[(int64_t)tid * nblocks + lb];
     const ull lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
-    const int64_t base = (int64_t)blockIdx.x * rounds * kRadixBlock;
+    const int64_t base = lb * rounds * kRadixBlock;
     const int64_t end = min(n, base + (int64_t)rounds * kRadixBlock);
     for (int64_t r0 = base; r0 < end; r0 += kRadixTile) {
 #pragma unroll
