@@ -759,34 +759,37 @@ __global__ void k_cr_rowcnt(const int32_t *newid_g, const int32_t *nuniq, int32_
     }
 }
 
-// 16 lanes per run.
+// kCopySmallLanes lanes per row (small segments average about ten uniques).
+constexpr int kCopySmallLanes = 4;
 __global__ void k_cr_copy_small(CoarsenArgs a, const int64_t *rowptr2, int32_t K, int32_t *col2,
                                 int64_t *cw2, uint32_t *cw2_32)
 {
-    const int lg = threadIdx.x & 15;
-    for (int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4; g < K;
-         g += ((int64_t)gridDim.x * blockDim.x) >> 4) {
+    constexpr int L = kCopySmallLanes;
+    const int lg = threadIdx.x % L;
+    for (int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / L; g < K;
+         g += ((int64_t)gridDim.x * blockDim.x) / L) {
         if (a.slen[g] > kTierMaxQ[kTiers - 2]) continue;  // big segments copy per partition
         if (a.newid_g[g] < 0) continue;                     // stopped component
         const int64_t src = a.sb[g], dst = rowptr2[a.newid_g[g]];
         const int32_t len = a.nuniq[g];
-        for (int32_t t = lg; t < len; t += 16) {
+        for (int32_t t = lg; t < len; t += L) {
             col2[dst + t] = a.uy[src + t];
             put_w(cw2, cw2_32, dst + t, a.uw[src + t]);
         }
     }
 }
 
+// One wavefront per partition (about 500 uniques each).
 __global__ void k_cr_copy_big(CoarsenArgs a, BigSegArgs B, const int64_t *rowptr2, int64_t nparts,
                               int32_t *col2, int64_t *cw2, uint32_t *cw2_32)
 {
-    const int lg = threadIdx.x & 15;
-    for (int64_t p = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4; p < nparts;
-         p += ((int64_t)gridDim.x * blockDim.x) >> 4) {
+    const int lg = threadIdx.x & 63;
+    for (int64_t p = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; p < nparts;
+         p += ((int64_t)gridDim.x * blockDim.x) >> 6) {
         const int32_t g = B.bsegs[B.part_big[p]];  // (big segments are never stopped groups)
         const int64_t src = B.poff[p], dst = rowptr2[a.newid_g[g]] + B.pre[p];
         const int32_t len = B.ucnt[p];
-        for (int32_t t = lg; t < len; t += 16) {
+        for (int32_t t = lg; t < len; t += 64) {
             col2[dst + t] = B.buy[src + t];
             put_w(cw2, cw2_32, dst + t, B.buw[src + t]);
         }
@@ -830,10 +833,10 @@ void launch_cr_copy(hipStream_t s, const CoarsenArgs &a, const BigSegArgs &B, in
                     uint32_t *cw2_32)
 {
     if (K > 0)
-        hipLaunchKernelGGL(k_cr_copy_small, grid_for((int64_t)K * 16, 256, 1 << 18), 256, 0, s, a,
+        hipLaunchKernelGGL(k_cr_copy_small, grid_for((int64_t)K * kCopySmallLanes, 256, 1 << 18), 256, 0, s, a,
                            rowptr2, K, col2, cw2, cw2_32);
     if (nparts > 0)
-        hipLaunchKernelGGL(k_cr_copy_big, grid_for(nparts * 16, 256, 1 << 18), 256, 0, s, a, B,
+        hipLaunchKernelGGL(k_cr_copy_big, grid_for(nparts * 64, 256, 1 << 18), 256, 0, s, a, B,
                            rowptr2, nparts, col2, cw2, cw2_32);
 }
 
