@@ -165,7 +165,7 @@ struct SweepArgs {
 // P_j = ceil(q_j / kHubPartTarget) partitions, aggregates each partition in an
 // LDS table, then reduces the partitions of a hub.
 constexpr int kHubChunk = 4096;
-constexpr int kChunkMul = 8;  // entries per partition of a chunk (0: kHubChunk fixed)
+constexpr int kChunkMul = 32;  // entries per partition of a chunk (0: kHubChunk fixed)
 constexpr int kHubPartTarget = 2048;  // default; LV_HUB_PART_TARGET overrides
 constexpr int kHubMaxParts = 2048;
 constexpr int kHubAggLog2 = 12;  // 4096 LDS slots per partition block
