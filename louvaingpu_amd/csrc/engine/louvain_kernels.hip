@@ -539,16 +539,33 @@ __global__ __launch_bounds__(256) void k_fill_rev(const uint32_t *skey, const ui
     SegCarry sc;
     for (int64_t base = (int64_t)blockIdx.x * SPAN; base < nr; base += (int64_t)gridDim.x * SPAN) {
         const int64_t w0 = base + (int64_t)wave * 64 * kSpanRounds;
+        // all rounds' entries, then all their row offsets, then the stores: the
+        // loads of a span are in flight together
+        uint64_t xs[kSpanRounds];
+        uint32_t vs[kSpanRounds];
+        int64_t ro[kSpanRounds];
+#pragma unroll
+        for (int rr = 0; rr < kSpanRounds; ++rr) {
+            const int64_t r = w0 + rr * 64 + lane;
+            xs[rr] = r < nr ? spay[r] : 0;
+            vs[rr] = r < nr ? (skey ? skey[r] : (uint32_t)(xs[rr] >> vshift)) : 0;
+        }
+#pragma unroll
+        for (int rr = 0; rr < kSpanRounds; ++rr) {
+            const int64_t r = w0 + rr * 64 + lane;
+            ro[rr] = r < nr ? rowptr[vs[rr]] - rs[vs[rr]] : 0;
+        }
+#pragma unroll
         for (int rr = 0; rr < kSpanRounds; ++rr) {
             const int64_t r = w0 + rr * 64 + lane;
             const bool valid = r < nr;
             long long row = 0;
             ull wp = 0;
             if (valid) {
-                const uint64_t x = spay[r];
-                const uint32_t v = skey ? skey[r] : (uint32_t)(x >> vshift);
+                const uint64_t x = xs[rr];
+                const uint32_t v = vs[rr];
                 const int64_t u = (int64_t)(x & ((1ull << wbits) - 1ull));
-                const int64_t pos = rowptr[v] + (r - rs[v]);
+                const int64_t pos = ro[rr] + r;
                 const int32_t cc = (int32_t)((x >> wbits) & cmask);
                 col[pos] = cc;
                 put_w(cw, cw32, pos, u);
@@ -597,6 +614,15 @@ __global__ __launch_bounds__(256) void k_fill_fwd(const int32_t *src, const int3
         }
         ull m[kSpanRounds];
         int32_t off = tile_self_base(tscan, base / kEdgeTile, sl, m, s_wc);
+        // every round's row offset and weight in flight before the stores
+        int64_t fo[kSpanRounds], uw[kSpanRounds];
+#pragma unroll
+        for (int rr = 0; rr < kSpanRounds; ++rr) {
+            const int64_t i = w0 + rr * 64 + lane;
+            const bool fwd = i < e && !sl[rr];
+            fo[rr] = fwd ? ffwd[sv[rr]] : 0;
+            uw[rr] = fwd ? w[i] : 0;
+        }
 #pragma unroll
         for (int rr = 0; rr < kSpanRounds; ++rr) {
             const int64_t i = w0 + rr * 64 + lane;
@@ -609,8 +635,8 @@ __global__ __launch_bounds__(256) void k_fill_fwd(const int32_t *src, const int3
                           // runs of equal keys stay contiguous for the segmented sum
                 if (v != b) {
                     const int64_t sli = off + __popcll(m[rr] & below);  // self-loops before i
-                    const int64_t pos = ffwd[v] + (i - sli);
-                    const int64_t u = w[i];
+                    const int64_t pos = fo[rr] + (i - sli);
+                    const int64_t u = uw[rr];
                     col[pos] = b;
                     put_w(cw, cw32, pos, u);
                     wp = (ull)(u == 0 ? 1 : u);
