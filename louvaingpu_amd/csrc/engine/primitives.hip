@@ -291,11 +291,47 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_apply(const T *in, T *out, 
     }
 }
 
+// One block for a short array (hub plans, segment plans): tiles in sequence
+// with a carry, one launch instead of three.  in may equal out.
+constexpr int64_t kScanSingleMax = 8 * kScanTile;
+
+template <typename T>
+__global__ __launch_bounds__(kScanBlock) void k_scan_single(const T *in, T *out, int64_t n,
+                                                           T *total)
+{
+    T carry = 0;
+    for (int64_t base = 0; base < n; base += kScanTile) {
+        const int64_t i0 = base + (int64_t)threadIdx.x * kScanItems;
+        T loc[kScanItems];
+        T acc = 0;
+#pragma unroll
+        for (int j = 0; j < kScanItems; ++j) {
+            loc[j] = (i0 + j < n) ? in[i0 + j] : 0;
+            acc += loc[j];
+        }
+        T tot;
+        T pre = block_excl_scan(acc, &tot) + carry;
+#pragma unroll
+        for (int j = 0; j < kScanItems; ++j) {
+            if (i0 + j < n) out[i0 + j] = pre;
+            pre += loc[j];
+        }
+        carry += tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && total) *total = carry;
+}
+
 template <typename T>
 static int scan_exclusive(Arena &a, hipStream_t s, const T *in, T *out, int64_t n, T *total_dev)
 {
     if (n <= 0) {
         if (total_dev) LV_HIP(hipMemsetAsync(total_dev, 0, sizeof(T), s));
+        return LV_OK;
+    }
+    if (n <= kScanSingleMax) {
+        hipLaunchKernelGGL(k_scan_single<T>, 1, kScanBlock, 0, s, in, out, n, total_dev);
+        LV_HIP(hipGetLastError());
         return LV_OK;
     }
     int64_t nb = (n + kScanTile - 1) / kScanTile;
