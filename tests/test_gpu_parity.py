@@ -286,15 +286,16 @@ def test_phase_api_matches_run():
 
 @pytest.mark.timeout(600)
 def test_rmat22_full_size():
-    """The R-MAT scale-22 benchmark graph end to end: final partition identical,
-    every level's pass Qs bit-equal, modularity within 1e-9."""
+    """The R-MAT scale-22 benchmark graph end to end: every sweep's next[] of
+    every level, every pass Q bit-equal, Partition arrays and final partition
+    identical, modularity within 1e-9."""
     gpu_case("rmat22 full")
     n, src, dst, w = generators.rmat22()
-    r = oracle.run(n, src, dst, w, trace=False, threads=16)
-    with Engine() as eng:
+    r = oracle.run(n, src, dst, w, trace=True, threads=len(os.sched_getaffinity(0)))
+    with Engine(trace=True) as eng:
         eng.load_edges(n, src, dst, w)
         eng.run()
-        assert_run_matches(eng, r, n, trace=False)
+        assert_run_matches(eng, r, n, trace=True)
 
 
 def _hub_graph(seed, hubs=3, hub_deg=9000, n=30000):
