@@ -29,7 +29,7 @@ extern "C" {
 typedef struct lvs_server lvs_server;
 
 typedef struct {
-    int32_t status;       /* LV_OK or the engine status of the request's set */
+    int32_t status;       /* LV_OK or the engine status of the request's own run */
     int32_t levels;       /* dendrogram levels */
     int32_t communities;  /* communities of the final partition */
     int32_t batch_size;   /* requests in the set it ran in */
@@ -48,11 +48,17 @@ typedef struct {
 int lvs_create(lvs_server **out, const lv_config *cfg, int32_t workers, uint64_t budget_bytes);
 /* Queues the graph (ids 0..n-1); *id identifies the request.  The arrays are
  * not copied: they must stay valid and unchanged until lvs_wait returns for
- * this request (the worker uploads them straight to the device). */
+ * this request (the worker uploads them straight to the device).  A graph whose
+ * footprint estimate (200 bytes per edge + 320 per vertex) exceeds the budget
+ * is refused here with LV_ERR_NOMEM.  Invalid input (an id outside 0..n-1, a
+ * negative or non-finite weight) is reported by lvs_wait as that request's
+ * status; it does not affect the other requests of its set. */
 int lvs_submit(lvs_server *s, int32_t n, int64_t e, const int32_t *src, const int32_t *dst,
                const void *w, int wtype, double gamma, int64_t *id);
 /* Blocks until the request is done; copies its final partition (n entries,
- * comm may be NULL) and summary, then forgets the request. */
+ * comm may be NULL) and summary, then forgets the request.  Returns the
+ * request's status; LV_ERR_ARG for an unknown id or one that another lvs_wait
+ * is already waiting for. */
 int lvs_wait(lvs_server *s, int64_t id, int32_t *comm, lvs_result *res);
 int lvs_get_stats(lvs_server *s, lvs_stats *out);
 /* Milliseconds since the server was created (the clock of lvs_result). */

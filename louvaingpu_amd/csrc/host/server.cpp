@@ -28,6 +28,7 @@ struct Request {
     double gamma = 1.0;
     uint64_t footprint = 0;
     bool done = false;
+    bool claimed = false;  // an lvs_wait is waiting for it (a second one is refused)
     lvs_result res{};
     std::vector<int32_t> comm;
 };
@@ -59,7 +60,8 @@ struct lvs_server {
 
     // Takes the waiting requests that fit the budget, in arrival order, that
     // share the first one's weight type and resolution (a graph set has one of
-    // each).  Called with mu held; returns an empty set only when stopping.
+    // each).  Called with mu held and a non-empty queue; lvs_submit refuses a
+    // request whose footprint exceeds the budget, so the head always fits.
     std::vector<Request *> take_set()
     {
         std::vector<Request *> set;
@@ -68,7 +70,7 @@ struct lvs_server {
         for (auto it = queue.begin(); it != queue.end();) {
             Request *r = *it;
             const bool same = r->wtype == head->wtype && r->gamma == head->gamma;
-            if (same && (set.empty() || used + r->footprint <= budget)) {
+            if (same && used + r->footprint <= budget) {
                 set.push_back(r);
                 used += r->footprint;
                 it = queue.erase(it);
@@ -80,10 +82,37 @@ struct lvs_server {
         return set;
     }
 
+    // Runs the requests as one graph set and fills their results.  A failed set
+    // is re-run one request at a time, so an invalid or oversized request gets
+    // its own status and never fails the others (components are independent:
+    // a request's result does not depend on the set it runs in).
     void run_set(lv_ctx *ctx, std::vector<Request *> &set)
     {
         const double t_start = now();
+        const int rc = run_members(ctx, set.data(), (int32_t)set.size());
+        if (rc != LV_OK && set.size() > 1)
+            for (Request *r : set) (void)run_members(ctx, &r, 1);
         const int32_t ng = (int32_t)set.size();
+        for (Request *r : set) {
+            r->res.batch_size = ng;
+            r->res.t_start_ms = t_start;
+        }
+        std::lock_guard<std::mutex> lk(mu);
+        const double t_done = now();
+        for (Request *r : set) {
+            r->res.t_done_ms = t_done;
+            r->done = true;
+        }
+        stats.completed += ng;
+        stats.sets += 1;
+        stats.max_set = std::max(stats.max_set, ng);
+        cv_done.notify_all();
+    }
+
+    // One graph-set run of set[0..ng); every member's status is the set's, or
+    // its own result query's.  Returns the set's status.
+    int run_members(lv_ctx *ctx, Request *const *set, int32_t ng)
+    {
         std::vector<int32_t> ns((size_t)ng);
         std::vector<int64_t> es((size_t)ng);
         std::vector<const int32_t *> ps((size_t)ng), pd((size_t)ng);
@@ -102,8 +131,6 @@ struct lvs_server {
         for (int32_t g = 0; g < ng; ++g) {
             Request *r = set[g];
             r->res.status = rc;
-            r->res.batch_size = ng;
-            r->res.t_start_ms = t_start;
             if (rc != LV_OK) continue;
             r->comm.resize((size_t)r->n);
             int rr = r->n ? lv_batch_final_assignment(ctx, g, r->comm.data()) : LV_OK;
@@ -118,16 +145,7 @@ struct lvs_server {
             for (int32_t v : r->comm) mx = std::max(mx, v);
             r->res.communities = mx + 1;
         }
-        std::lock_guard<std::mutex> lk(mu);
-        const double t_done = now();
-        for (Request *r : set) {
-            r->res.t_done_ms = t_done;
-            r->done = true;
-        }
-        stats.completed += ng;
-        stats.sets += 1;
-        stats.max_set = std::max(stats.max_set, ng);
-        cv_done.notify_all();
+        return rc;
     }
 
     void worker(lv_ctx *ctx)
@@ -190,6 +208,7 @@ int lvs_submit(lvs_server *s, int32_t n, int64_t e, const int32_t *src, const in
     r->wtype = wtype;
     r->gamma = gamma;
     r->footprint = footprint(n, e);
+    if (r->footprint > s->budget) return LV_ERR_NOMEM;  // could never be admitted
     r->src = src;
     r->dst = dst;
     r->w = w;
@@ -212,6 +231,8 @@ int lvs_wait(lvs_server *s, int64_t id, int32_t *comm, lvs_result *res)
     auto it = s->reqs.find(id);
     if (it == s->reqs.end()) return LV_ERR_ARG;
     Request *r = it->second.get();
+    if (r->claimed) return LV_ERR_ARG;  // another lvs_wait owns this request
+    r->claimed = true;
     s->cv_done.wait(lk, [&] { return r->done; });
     if (comm && r->res.status == LV_OK && r->n)
         std::memcpy(comm, r->comm.data(), sizeof(int32_t) * (size_t)r->n);

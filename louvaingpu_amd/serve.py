@@ -1,10 +1,13 @@
 """Serving: continuous batching of Louvain requests (``include/louvain_serve.h``).
 
 ``Server`` wraps the C++ request loop of ``liblouvain_host.so``: ``submit``
-copies a graph and queues it, worker threads (one engine context and HIP
-stream each) admit waiting requests by HBM footprint into graph sets and run
-them while later requests queue, and ``wait`` returns a request's final
-partition with its timings.
+queues a graph without copying it (the server keeps references to the arrays
+until ``wait`` returns, since the worker uploads them straight to the device),
+worker threads (one engine context and HIP stream each) admit waiting requests
+by HBM footprint into graph sets and run them while later requests queue, and
+``wait`` returns a request's final partition with its timings, or raises
+``LouvainError`` with that request's own status.  A graph too large for the
+budget is refused by ``submit`` (``LV_ERR_NOMEM``).
 """
 from __future__ import annotations
 

@@ -9,6 +9,7 @@ import pytest
 
 from conftest import gpu_case
 from louvaingpu_amd import Engine, Server, generators
+from louvaingpu_amd._lib import LV_ERR_INVALID_INPUT, LV_ERR_NOMEM, LouvainError
 
 pytestmark = pytest.mark.gpu
 
@@ -68,3 +69,47 @@ def test_concurrent_submitters():
             t.join()
     for i, (g, gamma) in enumerate(graphs):
         assert np.array_equal(out[i], _alone(g, gamma)[0]), i
+
+
+def test_bad_requests_get_their_own_status():
+    """An invalid request (an endpoint outside 0..n-1, a negative weight) in a
+    set with valid ones fails alone; a request larger than the budget is
+    refused at submit; every valid request equals its run alone."""
+    params = generators.batch_params(12, seed=21)
+    good = [generators.sbm(min(n, 3000), 100, p, p / 50, sd) for n, p, sd in params]
+    n0, s0, d0, w0 = good[0]
+    bad_id = (n0, s0, np.where(np.arange(len(d0)) == 5, n0, d0).astype(np.int32), w0)
+    bad_w = (n0, s0, d0, np.where(np.arange(len(w0)) == 3, -1, w0).astype(np.int32))
+    blocker = generators.rmat(16, 16, 0.57, 0.19, 0.19, 9, "int")
+    budget = 512 << 20
+    huge_n = budget // 320 + 1
+    gpu_case("serve bad requests in a mix, 1 worker, budget %d MiB" % (budget >> 20))
+    with Server(workers=1, budget_bytes=budget) as srv:
+        first = srv.submit(*blocker)  # keeps the worker busy while the mix queues
+        ids, kinds = [], []
+        for i, g in enumerate(good):
+            ids.append(srv.submit(*g))
+            kinds.append(("good", i))
+            if i == 3:
+                ids.append(srv.submit(*bad_id))
+                kinds.append(("bad", None))
+            if i == 7:
+                ids.append(srv.submit(*bad_w))
+                kinds.append(("bad", None))
+        with pytest.raises(LouvainError) as ex:
+            srv.submit(huge_n, np.zeros(0, np.int32), np.zeros(0, np.int32),
+                       np.zeros(0, np.int32))
+        assert ex.value.status == LV_ERR_NOMEM
+        srv.wait(first)
+        mixed = 0
+        for rid, (kind, i) in zip(ids, kinds):
+            if kind == "bad":
+                with pytest.raises(LouvainError) as ex:
+                    srv.wait(rid)
+                assert ex.value.status == LV_ERR_INVALID_INPUT
+                continue
+            comm, res = srv.wait(rid)
+            assert res.status == 0
+            mixed += res.batch_size > 1
+            assert np.array_equal(comm, _alone(good[i], 1.0)[0]), i
+        assert mixed > 0  # some valid requests ran in a set with a bad one
