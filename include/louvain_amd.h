@@ -40,7 +40,8 @@ enum {
     LV_ERR_RANGE = -3,         /* weight sums exceed the exact int64 range */
     LV_ERR_ARG = -4,           /* bad argument or call out of order */
     LV_ERR_DEVICE = -5,        /* HIP runtime error */
-    LV_ERR_NO_TRACE = -6       /* trace requested without LV_FLAG_TRACE */
+    LV_ERR_NO_TRACE = -6,      /* trace requested without LV_FLAG_TRACE */
+    LV_ERR_TOO_LARGE = -7      /* input beyond the static scratch of lv_run_static */
 };
 
 /* weight element types for lv_load_edges */
@@ -213,6 +214,28 @@ int lv_batch_level_assignment(lv_ctx *ctx, int32_t comp, int32_t level, int32_t 
 /* Final community of every level-0 vertex of the component (comp = -1: all
  * components, concatenated in component order). */
 int lv_batch_final_assignment(lv_ctx *ctx, int32_t comp, int32_t *comm);
+
+/* No-allocation run.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (serving loops and harnesses that forbid
+ * hipMalloc).  lv_load_edges + lv_run + lv_final_assignment (+
+ * lv_final_modularity when `modularity` is not NULL) of one graph, with every
+ * device buffer - the arena and the context's scalars - carved out of one
+ * file-scope device array of LV_STATIC_SCRATCH_BYTES that each GPU holds until
+ * the process ends.  No hipMalloc, hipFree, pinned allocation, stream or event
+ * is created; kernels and copies go to `stream` (a hipStream_t, NULL = the
+ * null stream), which is synchronised before the call returns.  Inputs and
+ * outputs are the caller's host arrays (final_comm: n entries).  cfg: NULL =
+ * lv_config_default; flags must be 0 and arena_bytes is ignored.  A graph whose
+ * footprint bound (200 bytes per edge + 320 per vertex) exceeds the array
+ * gives LV_ERR_TOO_LARGE before any device work; LV_ERR_NOMEM if a run still
+ * outgrows it.  Every call on one GPU uses that GPU's one array, whose contents
+ * persist from the last call (each run initialises what it reads): calls on
+ * one device are serialised by a process-wide lock, so calls that overlap on
+ * one GPU from this process wait for each other rather than race. */
+#define LV_STATIC_SCRATCH_BYTES ((size_t)256 << 20)
+int lv_run_static(const lv_config *cfg, void *stream, int32_t n, const int32_t *ids, int64_t e,
+                  const int32_t *src, const int32_t *dst, const void *w, int wtype,
+                  int32_t *final_comm, int32_t *levels, double *modularity);
 
 int lv_get_stats(lv_ctx *ctx, lv_stats *out);
 int lv_reset_stats(lv_ctx *ctx);

@@ -21,6 +21,9 @@ LV_ERR_RANGE = -3
 LV_ERR_ARG = -4
 LV_ERR_DEVICE = -5
 LV_ERR_NO_TRACE = -6
+LV_ERR_TOO_LARGE = -7
+
+LV_STATIC_SCRATCH_BYTES = 256 << 20
 
 LV_WEIGHT_INT32 = 0
 LV_WEIGHT_FLOAT32 = 1
@@ -40,6 +43,7 @@ ENGINE_SYMBOLS = (
     "lv_get_stats", "lv_reset_stats", "lv_last_run_ms",
     "lv_batch_load", "lv_batch_load_graphs", "lv_batch_run", "lv_batch_num_components", "lv_batch_num_levels",
     "lv_batch_level_info", "lv_batch_level_assignment", "lv_batch_final_assignment",
+    "lv_run_static",
 )
 HOST_SYMBOLS = (
     "lvh_generate_dendrogram", "lvh_num_levels", "lvh_level_count", "lvh_level_partition",
@@ -191,6 +195,8 @@ def engine():
         "lv_batch_level_info": (ct.c_int, [ctx, ct.c_int32, ct.c_int32, ct.POINTER(LvLevelInfo)]),
         "lv_batch_level_assignment": (ct.c_int, [ctx, ct.c_int32, ct.c_int32, _I32P]),
         "lv_batch_final_assignment": (ct.c_int, [ctx, ct.c_int32, _I32P]),
+        "lv_run_static": (ct.c_int, [ct.POINTER(LvConfig), ct.c_void_p, ct.c_int32, _I32P,
+                                     ct.c_int64, _I32P, _I32P, _VP, ct.c_int, _I32P, _I32P, _DP]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

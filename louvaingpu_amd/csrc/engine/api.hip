@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -134,6 +135,13 @@ struct lv_ctx {
     std::vector<hipEvent_t> ev_pool;
     std::vector<PendingTime> pending;
     double last_run_ms = 0;
+    // lv_run_static: the device scalars and the arena live in the static
+    // scratch array, the stream is the caller's, and the host copies of the
+    // scalars are these ordinary (pageable) members
+    bool borrowed = false;
+    DevFlags hs_flags{};
+    QAcc hs_qacc{};
+    long long hs_small[64] = {0};
 };
 
 // ---------------------------------------------------------------------------
@@ -1481,6 +1489,7 @@ const char *lv_strerror(int status)
     case LV_ERR_ARG: return "invalid argument or call out of order";
     case LV_ERR_DEVICE: return "HIP runtime error";
     case LV_ERR_NO_TRACE: return "trace not recorded (create the context with LV_FLAG_TRACE)";
+    case LV_ERR_TOO_LARGE: return "graph too large for the static scratch array of lv_run_static";
     default: return "unknown status";
     }
 }
@@ -2372,6 +2381,112 @@ int lv_batch_final_assignment(lv_ctx *c, int32_t comp, int32_t *comm)
     for (int32_t k = 0; k < comp; ++k) off += c->comp_n0[k];
     LV_TRY(d2h(c, comm, c->d_final + off, c->comp_n0[comp]));
     return sync(c);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// lv_run_static: the single-graph run with every device buffer carved out of
+// one file-scope array, for callers that cannot allocate device memory.
+
+// Each device that loads this library holds its own copy of the array until
+// the process ends.  In host code `scratch` names a host-side stand-in: its
+// device address comes from hipGetSymbolAddress, once per device.
+static __device__ __align__(256) unsigned char scratch[LV_STATIC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kStaticMaxDevices = 64;
+// Head of the array: the context's device scalars; the arena takes the rest.
+constexpr size_t kStaticFlagsOff = 0;
+constexpr size_t kStaticQaccOff = 256;
+constexpr size_t kStaticQpartOff = 512;
+constexpr size_t kStaticSmallOff = kStaticQpartOff + sizeof(QAcc) * kQParts;
+constexpr size_t kStaticHead = 64 << 10;
+static_assert(kStaticSmallOff + 64 * sizeof(long long) <= kStaticHead, "static head");
+static_assert(sizeof(DevFlags) <= kStaticQaccOff - kStaticFlagsOff, "static flags");
+
+std::mutex g_static_mu[kStaticMaxDevices];  // one run at a time per device
+void *g_static_ptr[kStaticMaxDevices];      // device address of `scratch` per device
+
+// Device address of this device's copy of the array (the device is current).
+int static_scratch(int32_t device, char **out)
+{
+    if (!g_static_ptr[device]) {
+        void *p = nullptr;
+        LV_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return LV_ERR_DEVICE;
+        g_static_ptr[device] = p;
+    }
+    *out = (char *)g_static_ptr[device];
+    return LV_OK;
+}
+
+int static_run(lv_ctx *c, int32_t n, const int32_t *ids, int64_t e, const int32_t *src,
+               const int32_t *dst, const void *w, int wtype, int32_t *final_comm,
+               int32_t *levels, double *modularity)
+{
+    int32_t nl = 0;
+    LV_TRY(lv_load_edges(c, n, ids, e, src, dst, w, wtype));
+    LV_TRY(lv_run(c, &nl));
+    LV_TRY(lv_final_assignment(c, final_comm));
+    if (modularity) LV_TRY(lv_final_modularity(c, modularity));
+    if (levels) *levels = nl;
+    return LV_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lv_run_static(const lv_config *cfg, void *stream, int32_t n, const int32_t *ids, int64_t e,
+                  const int32_t *src, const int32_t *dst, const void *w, int wtype,
+                  int32_t *final_comm, int32_t *levels, double *modularity)
+{
+    lv_config cf;
+    if (cfg)
+        cf = *cfg;
+    else
+        lv_config_default(&cf);
+    if (n < 0 || e < 0 || (e > 0 && (!src || !dst || !w)) || (n > 0 && !final_comm) ||
+        (wtype != LV_WEIGHT_INT32 && wtype != LV_WEIGHT_FLOAT32) || cf.flags != 0 ||
+        !(cf.gamma >= 0.0) || !(cf.min_gain == cf.min_gain) || cf.max_passes < 0 ||
+        cf.max_levels < 0 || cf.device < 0 || cf.device >= kStaticMaxDevices)
+        return LV_ERR_ARG;
+    // the footprint bound of the arena sizing (200 B per edge, 320 B per vertex)
+    const size_t room = (size_t)LV_STATIC_SCRATCH_BYTES - kStaticHead;
+    if ((double)e * 200.0 + (double)n * 320.0 > (double)room) return LV_ERR_TOO_LARGE;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || cf.device >= ndev) {
+        (void)hipGetLastError();
+        return LV_ERR_DEVICE;
+    }
+    std::lock_guard<std::mutex> lock(g_static_mu[cf.device]);
+    LV_HIP(hipSetDevice(cf.device));
+    char *base = nullptr;
+    LV_TRY(static_scratch(cf.device, &base));
+    lv_ctx *c = new (std::nothrow) lv_ctx();
+    if (!c) return LV_ERR_NOMEM;
+    c->cfg = cf;
+    c->cfg.arena_bytes = room;  // fixed: lv_load_edges does not size the arena
+    c->device = cf.device;
+    c->stream = (hipStream_t)stream;
+    c->borrowed = true;
+    memset(&c->stats, 0, sizeof(c->stats));
+    c->d_flags = (DevFlags *)(base + kStaticFlagsOff);
+    c->d_qacc = (QAcc *)(base + kStaticQaccOff);
+    c->d_qpart = (QAcc *)(base + kStaticQpartOff);
+    c->d_small = (long long *)(base + kStaticSmallOff);
+    c->h_flags = &c->hs_flags;
+    c->h_qacc = &c->hs_qacc;
+    c->h_small = c->hs_small;
+    c->arena.adopt(base + kStaticHead, room);
+    const int rc = static_run(c, n, ids, e, src, dst, w, wtype, final_comm, levels, modularity);
+    // leave nothing in flight on the caller's stream that reads the array
+    const hipError_t se = hipStreamSynchronize(c->stream);
+    delete c;  // host bookkeeping only: the arena does not own its block
+    if (rc != LV_OK) return rc;
+    return se == hipSuccess ? LV_OK : hip_fail(se, "hipStreamSynchronize", __FILE__, __LINE__);
 }
 
 }  // extern "C"

@@ -50,6 +50,8 @@ class Arena {
 public:
     ~Arena();
     int reserve(size_t bytes);  // (re)allocates only when larger than the current block
+    // Manages a block the arena does not own (never freed or grown here).
+    void adopt(void *base, size_t bytes);
     void *alloc(size_t bytes);
     void release(void *p);
     void release_all();
@@ -60,6 +62,7 @@ public:
 private:
     char *base_ = nullptr;
     size_t cap_ = 0;
+    bool owned_ = true;
     size_t used_bytes_ = 0, high_ = 0;
     std::map<size_t, size_t> free_;  // offset -> size
     std::map<size_t, size_t> used_;  // offset -> size

@@ -25,13 +25,25 @@ int hip_fail(hipError_t e, const char *what, const char *file, int line)
 
 Arena::~Arena()
 {
-    if (base_) (void)hipFree(base_);
+    if (base_ && owned_) (void)hipFree(base_);
+}
+
+void Arena::adopt(void *base, size_t bytes)
+{
+    base_ = (char *)base;
+    cap_ = bytes & ~(size_t)255;
+    owned_ = false;
+    used_.clear();
+    free_.clear();
+    if (cap_) free_[0] = cap_;
+    used_bytes_ = high_ = 0;
 }
 
 int Arena::reserve(size_t bytes)
 {
     bytes = (bytes + 255) & ~(size_t)255;
     if (base_ && cap_ >= bytes) return LV_OK;
+    if (!owned_) return LV_ERR_NOMEM;  // a borrowed block cannot grow
     if (!used_.empty()) return LV_ERR_ARG;  // cannot grow under live allocations
     if (base_) {
         (void)hipFree(base_);

@@ -4,6 +4,7 @@ import ctypes as ct
 import os
 import re
 
+import numpy as np
 import pytest
 
 from louvaingpu_amd import _lib
@@ -74,6 +75,24 @@ def test_bad_config_is_rejected_before_device_use():
     cfg.gamma = -1.0
     h = ct.c_void_p()
     assert lib.lv_create(ct.byref(h), ct.byref(cfg)) == _lib.LV_ERR_ARG
+
+
+def test_static_run_rejects_before_device_use():
+    lib = _lib.engine()
+    a = np.zeros(4, np.int32)
+    p = a.ctypes.data_as(ct.POINTER(ct.c_int32))
+    big_e = _lib.LV_STATIC_SCRATCH_BYTES // 200 + 1
+    # beyond the static array (the arrays are not read), then argument errors
+    assert lib.lv_run_static(None, None, 4, None, big_e, p, p, p, 0, p, None, None) == \
+        _lib.LV_ERR_TOO_LARGE
+    assert lib.lv_run_static(None, None, 4, None, 2, None, p, p, 0, p, None, None) == _lib.LV_ERR_ARG
+    assert lib.lv_run_static(None, None, 4, None, 2, p, p, p, 7, p, None, None) == _lib.LV_ERR_ARG
+    cfg = _lib.LvConfig()
+    lib.lv_config_default(ct.byref(cfg))
+    cfg.flags = _lib.LV_FLAG_TIMING
+    assert lib.lv_run_static(ct.byref(cfg), None, 4, None, 2, p, p, p, 0, p, None, None) == \
+        _lib.LV_ERR_ARG
+    assert "static scratch" in _lib.strerror(_lib.LV_ERR_TOO_LARGE)
 
 
 def test_missing_library_fails_loudly(tmp_path, monkeypatch):
