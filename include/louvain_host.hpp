@@ -157,7 +157,11 @@ int *calculateClusterSize(Graph &graph, Partition partitions, int numberOfCluste
 void layoutProcess(int *clusterSize, Graph &graph, int nrPartitions);
 
 /* Text loaders of the reference CLI (LouvainCPP.cpp:660-737): a count, then the
- * ids (nodes.txt) or `source target weight` triples (edges.txt). */
+ * ids (nodes.txt) or `source target weight` triples (edges.txt), separated by
+ * any whitespace, parsed with atoi/atof.  As in the reference, a node id that
+ * ends the file (no trailing whitespace) is not read.  Unlike it, more entries
+ * than the count (an overrun there) and an incomplete last triple (filled from
+ * a stale token there) throw std::runtime_error, as does a missing file. */
 std::vector<int> loadNodesFile(const std::string &path);
 std::vector<Edge> loadEdgesFile(const std::string &path);
 

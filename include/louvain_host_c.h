@@ -8,6 +8,8 @@
  *        language boundary.
  * lvg_*  deterministic synthetic graphs (the reference ships none): SBM and
  *        R-MAT edge lists, canonical (u <= v) and lexicographically sorted.
+ * lvh_load_text / lvh_binary_*  graph files: the reference CLI's nodes.txt and
+ *        edges.txt (LouvainCUDA/LouvainCPP.cpp:660-737) and a binary edge list.
  */
 #ifndef LOUVAIN_HOST_C_H
 #define LOUVAIN_HOST_C_H
@@ -54,6 +56,30 @@ int64_t lvg_num_edges(const lvg_graph *g);
 int lvg_wtype(const lvg_graph *g);
 int lvg_copy(const lvg_graph *g, int32_t *src, int32_t *dst, void *w);
 void lvg_free(lvg_graph *g);
+
+/* loadNodes + loadEdges (LouvainCUDA/LouvainCPP.cpp:660-737), the rules of
+ * loadNodesFile/loadEdgesFile in louvain_host.hpp: node ids and float
+ * weights; edges name node ids.  LV_ERR_ARG when a file cannot be opened,
+ * LV_ERR_INVALID_INPUT when its content breaks the rules. */
+typedef struct lvh_file_graph lvh_file_graph;
+int lvh_load_text(const char *nodes_path, const char *edges_path, lvh_file_graph **out);
+int32_t lvh_file_num_nodes(const lvh_file_graph *g);
+int64_t lvh_file_num_edges(const lvh_file_graph *g);
+/* ids[num_nodes], src/dst/w[num_edges] (ids may be NULL). */
+int lvh_file_copy(const lvh_file_graph *g, int32_t *ids, int32_t *src, int32_t *dst, float *w);
+void lvh_file_free(lvh_file_graph *g);
+
+/* Binary edge list, little-endian: a 32-byte header - "LVEB", u32 version 1,
+ * i32 n, u32 flags (bit 0: node ids present, bit 1: float32 weights, else
+ * int32), i64 e, 8 zero bytes - then ids int32[n] (if flagged), src int32[e],
+ * dst int32[e], w int32/float32[e].  No text parse: for graphs of billions of
+ * edges.  lvh_binary_header checks the header against the file size;
+ * lvh_binary_read fills the caller's arrays (ids may be NULL); lvh_binary_write
+ * writes one (ids NULL: none stored). */
+int lvh_binary_header(const char *path, int32_t *n, int64_t *e, int *wtype, int *has_ids);
+int lvh_binary_read(const char *path, int32_t *ids, int32_t *src, int32_t *dst, void *w);
+int lvh_binary_write(const char *path, int32_t n, const int32_t *ids, int64_t e,
+                     const int32_t *src, const int32_t *dst, const void *w, int wtype);
 
 #ifdef __cplusplus
 }

@@ -50,6 +50,8 @@ HOST_SYMBOLS = (
     "lvh_partition_at_level", "lvh_get_partitions", "lvh_count_partitions", "lvh_free",
     "lvh_cluster_layout",
     "lvg_sbm", "lvg_rmat", "lvg_num_nodes", "lvg_num_edges", "lvg_wtype", "lvg_copy", "lvg_free",
+    "lvh_load_text", "lvh_file_num_nodes", "lvh_file_num_edges", "lvh_file_copy", "lvh_file_free",
+    "lvh_binary_header", "lvh_binary_read", "lvh_binary_write",
 )
 SERVE_SYMBOLS = (
     "lvs_create", "lvs_submit", "lvs_wait", "lvs_get_stats", "lvs_now_ms", "lvs_destroy",
@@ -236,6 +238,16 @@ def host():
         "lvg_wtype": (ct.c_int, [d]),
         "lvg_copy": (ct.c_int, [d, _I32P, _I32P, ct.c_void_p]),
         "lvg_free": (None, [d]),
+        "lvh_load_text": (ct.c_int, [ct.c_char_p, ct.c_char_p, ct.POINTER(d)]),
+        "lvh_file_num_nodes": (ct.c_int32, [d]),
+        "lvh_file_num_edges": (ct.c_int64, [d]),
+        "lvh_file_copy": (ct.c_int, [d, _I32P, _I32P, _I32P, ct.POINTER(ct.c_float)]),
+        "lvh_file_free": (None, [d]),
+        "lvh_binary_header": (ct.c_int, [ct.c_char_p, _I32P, _I64P, ct.POINTER(ct.c_int),
+                                         ct.POINTER(ct.c_int)]),
+        "lvh_binary_read": (ct.c_int, [ct.c_char_p, _I32P, _I32P, _I32P, ct.c_void_p]),
+        "lvh_binary_write": (ct.c_int, [ct.c_char_p, ct.c_int32, _I32P, ct.c_int64, _I32P, _I32P,
+                                        ct.c_void_p, ct.c_int]),
         "lvs_create": (ct.c_int, [ct.POINTER(d), ct.POINTER(LvConfig), ct.c_int32, ct.c_uint64]),
         "lvs_submit": (ct.c_int, [d, ct.c_int32, ct.c_int64, _I32P, _I32P, ct.c_void_p, ct.c_int,
                                   ct.c_double, _I64P]),

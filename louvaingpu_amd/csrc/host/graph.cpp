@@ -1,6 +1,5 @@
 // graph.cpp - host Graph/EdgeContainer storage (LouvainCUDA/Graph.cpp:28-160,
-// 218-347, LouvainCUDA/Edge.h:26-63) and the CLI text loaders
-// (LouvainCUDA/LouvainCPP.cpp:660-737).
+// 218-347, LouvainCUDA/Edge.h:26-63).  The CLI text loaders are in graphio.cpp.
 #include <cstdlib>
 #include <fstream>
 #include <stdexcept>
@@ -163,40 +162,4 @@ float *Graph::getYPtr(int idx)
 {
     ensureNodeAttrs();
     return &y_.at((size_t)idx);
-}
-
-// The reference reads a count and then tokens until end of file; every token
-// after the count is taken here (the reference drops the last node id when the
-// file lacks trailing whitespace, LouvainCPP.cpp:676-692).
-std::vector<int> loadNodesFile(const std::string &path)
-{
-    std::ifstream in(path);
-    if (!in) throw std::runtime_error("cannot open " + path);
-    long long count = 0;
-    if (!(in >> count) || count < 0) throw std::runtime_error("bad node count in " + path);
-    std::vector<int> ids;
-    ids.reserve((size_t)count);
-    std::string tok;
-    while (in >> tok) ids.push_back(atoi(tok.c_str()));
-    return ids;
-}
-
-std::vector<Edge> loadEdgesFile(const std::string &path)
-{
-    std::ifstream in(path);
-    if (!in) throw std::runtime_error("cannot open " + path);
-    long long count = 0;
-    if (!(in >> count) || count < 0) throw std::runtime_error("bad edge count in " + path);
-    std::vector<Edge> edges;
-    edges.reserve((size_t)count);
-    std::string a, b, w;
-    while (in >> a) {
-        if (!(in >> b >> w)) throw std::runtime_error("truncated edge triple in " + path);
-        Edge e;
-        e.source = atoi(a.c_str());
-        e.target = atoi(b.c_str());
-        e.weight = (float)atof(w.c_str());
-        edges.push_back(e);
-    }
-    return edges;
 }
