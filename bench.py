@@ -1,12 +1,16 @@
 #!/usr/bin/env python3
 """Louvain benchmark: TEPS of the full multi-level Louvain on one MI355X.
 
-One step = one complete run of the reference's pipeline (CSR build, init,
-local-moving sweeps, renumbering, coarsening at every level, flattening) on
-the R-MAT scale-22 graph (edge factor 16, a/b/c/d = .57/.19/.19/.05,
-duplicates merged into int32 weights, seed 1), with the edge list already
-resident in HBM.  value = TEPS = directed CSR entries of the input (2E) /
-seconds per step.
+One step = one end-to-end run as the metric defines it: the caller's edge list
+in page-locked host memory is uploaded, the reference's whole pipeline runs
+(CSR build, init, local-moving sweeps, renumbering, coarsening at every level,
+flattening), and every level's Partition (node, community) plus the final
+flattened partition are copied back into page-locked host buffers; graph
+generation is outside the step.  The graph is R-MAT scale 22 (edge factor 16,
+a/b/c/d = .57/.19/.19/.05, duplicates merged into int32 weights, seed 1).
+value = TEPS = directed CSR entries of the input (2E) / seconds per step.
+The same run with the input already resident in HBM is reported beside it
+(device_resident_ms).
 
 Besides the JSON line's required fields it reports:
   roofline     the dominant local-moving sweep tier (one kernel, or the four
@@ -15,8 +19,14 @@ Besides the JSON line's required fields it reports:
                duration, against 8 TB/s; traffic = measured HBM bytes per launch
                from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
   cpu_baseline the oracle (C restatement of the reference semantics, OpenMP)
-               on the same graph, timed on this host; its partition is also
-               the validation of the GPU result
+               on the same graph with every thread this process may use, timed
+               on this host; its partition is also the validation of the GPU
+               result.  cpu_baseline_1thread: the oracle on one thread over a
+               bounded sample (R-MAT scale 20, same parameters); host_cpu: the
+               host's model, nproc, affinity and cgroup CPU quota
+  rmat26f      the R-MAT scale-26 float32 graph (seed 2, 2.1 G entries): the same
+               end-to-end and device-resident times, phases and roofline, its
+               final partition checked against the committed oracle digest
   batched      the batching workload: 4096 SBM graphs (n log-uniform in
                [100, 50000], blocks of 100, p_in ~ U[0.02, 0.1], p_out =
                p_in/50, seed 3) x 16 resolutions 0.25..4.0 as graph sets, in
@@ -69,12 +79,44 @@ TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "louvain_%s_hbm_traffic.txt")
 
 
 def host_threads():
+    """Threads this process can run at once: its CPU affinity, bounded by the
+    cgroup CPU quota when one is set."""
     n = os.cpu_count() or 1
     try:
         n = len(os.sched_getaffinity(0))
     except Exception:
         pass
-    return max(1, min(n, 16))
+    quota = cgroup_cpus()
+    if quota:
+        n = min(n, max(1, int(quota + 0.999)))
+    return max(1, n)
+
+
+def cgroup_cpus():
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, per = f.read().split()[:2]
+        return None if q == "max" else int(q) / int(per)
+    except Exception:
+        return None
+
+
+def host_cpu():
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except Exception:
+        pass
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except Exception:
+        aff = None
+    return {"model": model, "nproc": os.cpu_count(), "affinity": aff,
+            "cgroup_cpus": cgroup_cpus(), "threads_used": host_threads()}
 
 
 def copy_probe(nbytes):
@@ -336,6 +378,170 @@ def workload(name):
     raise SystemExit("unknown workload " + name)
 
 
+def e2e_runs(eng, n, hsrc, hdst, hw, steps, warmup, label):
+    """End-to-end steps (pinned edge list -> device -> run -> every level's
+    Partition and the final partition in pinned host buffers).  Returns
+    (seconds per step, per-part ms, levels)."""
+    from louvaingpu_amd.engine import pinned_empty
+    outs, fin = None, None
+    parts = np.zeros(3)
+    levels = 0
+    total = 0.0
+    for i in range(warmup + steps):
+        t0 = time.perf_counter()
+        eng.load_edges(n, hsrc, hdst, hw)
+        t1 = time.perf_counter()
+        levels = eng.run()
+        t2 = time.perf_counter()
+        for l in range(levels):
+            eng.level_partition(l, out=None if outs is None else outs[l])
+        eng.final_assignment(out=fin)
+        t3 = time.perf_counter()
+        if i >= warmup:
+            parts += (t1 - t0, t2 - t1, t3 - t2)
+            total += t3 - t0
+        else:
+            print("[bench] %s e2e warmup %d: %.2f ms (load %.2f, run %.2f, results %.2f)"
+                  % (label, i, (t3 - t0) * 1e3, (t1 - t0) * 1e3, (t2 - t1) * 1e3,
+                     (t3 - t2) * 1e3), file=sys.stderr, flush=True)
+        if outs is None:  # page-locked result buffers, sized by the first run
+            outs = [(pinned_empty(eng.level_info(l).n, np.int32),
+                     pinned_empty(eng.level_info(l).n, np.int32)) for l in range(levels)]
+            fin = pinned_empty(n, np.int32)
+    k = max(steps, 1)
+    return total / k, {"load_ms": parts[0] * 1e3 / k, "run_ms": parts[1] * 1e3 / k,
+                       "results_ms": parts[2] * 1e3 / k}, levels
+
+
+def measure_graph(wname, graph, steps, warmup, traffic_key):
+    """End-to-end and device-resident timing of one graph, with the dominant
+    sweep tier's roofline.  Returns the measurement dict and the engine's final
+    partition and modularity."""
+    from louvaingpu_amd import Engine
+    from louvaingpu_amd._lib import LV_SWEEP_TIERS
+    from louvaingpu_amd.engine import pinned_empty
+    n, src, dst, w = graph
+    e = len(src)
+    nnz0 = 2 * e
+    # the caller's edge list in page-locked host memory (filled outside the steps)
+    hsrc, hdst, hw = pinned_empty(e, np.int32), pinned_empty(e, np.int32), pinned_empty(e, w.dtype)
+    hsrc[:], hdst[:], hw[:] = src, dst, w
+    with Engine() as eng:
+        e2e_s, e2e_parts, levels = e2e_runs(eng, n, hsrc, hdst, hw, steps, warmup, wname)
+    del hsrc, hdst, hw
+    # the same runs with the input resident in HBM, HIP events per phase and tier
+    eng = Engine(timing=True)
+    try:
+        eng.load_edges(n, src, dst, w)
+        for i in range(warmup):
+            eng.run()
+        eng.reset_stats()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            eng.run()
+        dev_ms = (time.perf_counter() - t0) * 1000.0 / max(steps, 1)
+        st = eng.stats()
+        infos = [eng.level_info(l) for l in range(eng.num_levels())]
+        final = eng.final_assignment()
+        q_gpu = eng.final_modularity()
+        arena = (st.arena_capacity / 1e9, st.arena_high_water / 1e9)
+    finally:
+        eng.close()
+    tier_ms = [st.tier_ms[t] for t in range(LV_SWEEP_TIERS)]
+    dom = int(np.argmax(tier_ms))
+    launches = max(1, st.tier_launches[dom])
+    avg_ms = st.tier_ms[dom] / launches
+    bytes_per_launch = st.tier_bytes[dom] / launches
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    k = max(steps, 1)
+    phases = {"csr": st.csr_ms / k, "init": st.init_ms / k, "sweep": st.sweep_ms / k,
+              "apply": st.apply_ms / k, "renumber": st.renumber_ms / k,
+              "coarsen": st.coarsen_ms / k, "flatten": st.flatten_ms / k}
+    m = {
+        "config": {"workload": wname, "n": n, "e": e, "nnz": nnz0, "levels": levels,
+                   "passes": [i.passes for i in infos], "level_n": [i.n for i in infos],
+                   "level_nnz": [i.nnz for i in infos], "parallelism": "single-gpu"},
+        "value": nnz0 / e2e_s,
+        "ms_per_step": e2e_s * 1e3,
+        "e2e_parts_ms": e2e_parts,
+        "device_resident_ms": dev_ms,
+        "device_resident_teps": nnz0 / (dev_ms * 1e-3),
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "sweep tier %d (%s)" % (dom + 1, TIER_NAMES[dom + 1]),
+            "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": measured_traffic(traffic_key, dom + 1),
+            "bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_ms,
+        },
+        "phase_ms_per_step": phases,
+        "coarsen_fraction": phases["coarsen"] / dev_ms if dev_ms > 0 else None,
+        "sweep_gbs": st.sweep_bytes / (st.sweep_ms * 1e-3) / 1e9 if st.sweep_ms > 0 else 0.0,
+        "sweep_frac": (st.sweep_bytes / (st.sweep_ms * 1e-3) / 1e9 / HBM_PEAK_GBS
+                       if st.sweep_ms > 0 else 0.0),
+        "sweep_teps": st.sweep_entries / (st.sweep_ms * 1e-3) if st.sweep_ms > 0 else 0.0,
+        "tier_ms_per_step": [t / k for t in tier_ms],
+        "arena_gb": arena[0], "arena_high_water_gb": arena[1],
+        "modularity": q_gpu, "communities": int(final.max()) + 1,
+    }
+    return m, final, q_gpu
+
+
+def cpu_baselines(n, src, dst, w, final, q_gpu, wname, nnz0, gpu_s):
+    """The oracle on every usable thread (full graph) and on one thread (R-MAT
+    scale-20 sample)."""
+    import oracle
+    from louvaingpu_amd import generators
+    threads = host_threads()
+    c0 = time.perf_counter()
+    r = oracle.run(n, src, dst, w, threads=threads)
+    cpu_s = time.perf_counter() - c0
+    same = bool(np.array_equal(r.final_comm(), final))
+    q_or = r.final_modularity()
+    r.close()
+    sn, ss, sd, sw = generators.rmat(20, 16, 0.57, 0.19, 0.19, 1, "int")
+    c0 = time.perf_counter()
+    r1 = oracle.run(sn, ss, sd, sw, threads=1)
+    one_s = time.perf_counter() - c0
+    r1.close()
+    return {
+        "cpu_baseline": {
+            "value": nnz0 / cpu_s, "unit": "TEPS", "cores": threads, "kind": "port",
+            "sample": "oracle (C restatement, OpenMP, %d threads) full multi-level run on the "
+                      "same %s graph, %.1f s" % (threads, wname, cpu_s)},
+        "cpu_baseline_1thread": {
+            "value": 2 * len(ss) / one_s, "unit": "TEPS", "cores": 1, "kind": "port",
+            "sample": "oracle on one thread, full multi-level run on R-MAT scale 20 (edge factor "
+                      "16, .57/.19/.19/.05, int weights, seed 1; %d entries), %.1f s"
+                      % (2 * len(ss), one_s)},
+        "host_cpu": host_cpu(),
+        "validated": same and abs(q_or - q_gpu) <= 1e-9 * max(1.0, abs(q_or)),
+        "oracle_modularity": q_or,
+        "speedup_vs_cpu": cpu_s / gpu_s,
+    }
+
+
+def rmat26f_object(args):
+    """The R-MAT scale-26 float graph: GPU side only, validated against the
+    committed oracle digest of its final partition."""
+    import hashlib
+    from louvaingpu_amd import generators
+    t0 = time.perf_counter()
+    graph = generators.rmat26_float()
+    gen_s = time.perf_counter() - t0
+    print("[bench] rmat26f generated in %.1f s" % gen_s, file=sys.stderr, flush=True)
+    m, final, q = measure_graph("rmat26_ef16_float_seed2", graph, args.rmat26_steps, 1, "rmat26f")
+    with open(os.path.join(ROOT, "tests", "golden", "rmat26f_oracle.json")) as f:
+        fx = json.load(f)
+    m["validated"] = (hashlib.sha256(np.ascontiguousarray(final).tobytes()).hexdigest()
+                      == fx["final_sha256"] and
+                      abs(q - float.fromhex(fx["final_modularity"])) <= 1e-9)
+    m["oracle_s_16_threads"] = fx["oracle_s"]
+    m["speedup_vs_cpu"] = fx["oracle_s"] / (m["ms_per_step"] * 1e-3)
+    m["steps"] = args.rmat26_steps
+    return m
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -343,7 +549,6 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="rmat22")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline / validation")
-    ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (calibrates "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE against a known byte count)")
@@ -352,6 +557,9 @@ def main():
     ap.add_argument("--batch-graphs", type=int, default=4096)
     ap.add_argument("--no-serve", action="store_true",
                     help="skip the serving measurement of the default run")
+    ap.add_argument("--no-rmat26", action="store_true",
+                    help="skip the R-MAT scale-26 object of the default run")
+    ap.add_argument("--rmat26-steps", type=int, default=3)
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-workers", type=int, default=2)
@@ -373,128 +581,41 @@ def main():
     if args.workload == "serve":
         print(json.dumps(run_serving(args)), flush=True)
         return 0
-    from louvaingpu_amd import Engine
-    from louvaingpu_amd._lib import LV_SWEEP_TIERS
 
-    wname, (n, src, dst, w) = workload(args.workload)
-    e = len(src)
-    nnz0 = 2 * e
-    print("[bench] workload %s n=%d e=%d nnz=%d" % (wname, n, e, nnz0), file=sys.stderr, flush=True)
-
-    eng = Engine(timing=True)
-    t_load0 = time.perf_counter()
-    eng.load_edges(n, src, dst, w)
-    t_load = time.perf_counter() - t_load0
-    for i in range(args.warmup):
-        eng.run()
-        print("[bench] warmup %d: %.2f ms" % (i, eng.last_run_ms()), file=sys.stderr, flush=True)
-    eng.reset_stats()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        eng.run()
-    t1 = time.perf_counter()
-    ms = (t1 - t0) * 1000.0 / max(args.steps, 1)
-    st = eng.stats()
-    levels = eng.num_levels()
-    infos = [eng.level_info(l) for l in range(levels)]
-    final = eng.final_assignment()
-    q_gpu = eng.final_modularity()
-
-    # PCIe-inclusive: host edge list -> device -> dendrogram and final partition on the host
-    tp0 = time.perf_counter()
-    eng.load_edges(n, src, dst, w)
-    eng.run()
-    for l in range(levels):
-        eng.level_partition(l)
-    eng.final_assignment()
-    pcie_s = time.perf_counter() - tp0
-
-    # dominant sweep kernel (tier with the largest device time)
-    tier_ms = [st.tier_ms[t] for t in range(LV_SWEEP_TIERS)]
-    dom = int(np.argmax(tier_ms))
-    launches = max(1, st.tier_launches[dom])
-    avg_ms = st.tier_ms[dom] / launches
-    bytes_per_launch = st.tier_bytes[dom] / launches
-    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
-    sweep_gbs = st.sweep_bytes / (st.sweep_ms * 1e-3) / 1e9 if st.sweep_ms > 0 else 0.0
-    sweep_teps = st.sweep_entries / (st.sweep_ms * 1e-3) if st.sweep_ms > 0 else 0.0
-    steps = max(args.steps, 1)
-
+    wname, graph = workload(args.workload)
+    n, src, dst, w = graph
+    nnz0 = 2 * len(src)
+    print("[bench] workload %s n=%d e=%d nnz=%d" % (wname, n, len(src), nnz0), file=sys.stderr,
+          flush=True)
+    m, final, q_gpu = measure_graph(wname, graph, args.steps, args.warmup, args.workload)
     out = {
         "metric": "Louvain TEPS & end-to-end s at oracle modularity; batched vs per-graph",
-        "value": nnz0 / (ms * 1e-3),
+        "value": m.pop("value"),
         "unit": "TEPS",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms,
+        "ms_per_step": m.pop("ms_per_step"),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "int64",
         "data": "synthetic",
-        "config": {
-            "workload": wname,
-            "n": n, "e": e, "nnz": nnz0,
-            "levels": levels,
-            "passes": [i.passes for i in infos],
-            "level_n": [i.n for i in infos],
-            "level_nnz": [i.nnz for i in infos],
-            "parallelism": "single-gpu",
-        },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": "sweep tier %d (%s)" % (dom + 1, TIER_NAMES[dom + 1]),
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(args.workload, dom + 1),
-            "bytes_per_launch": bytes_per_launch,
-            "avg_launch_ms": avg_ms,
-        },
-        "e2e_s": ms * 1e-3,
-        "pcie_inclusive_s": pcie_s,
-        "load_s": t_load,
-        "phase_ms_per_step": {
-            "csr": st.csr_ms / steps, "init": st.init_ms / steps, "sweep": st.sweep_ms / steps,
-            "apply": st.apply_ms / steps, "renumber": st.renumber_ms / steps,
-            "coarsen": st.coarsen_ms / steps, "flatten": st.flatten_ms / steps,
-        },
-        "sweep_gbs": sweep_gbs,
-        "sweep_teps": sweep_teps,
-        "tier_ms_per_step": [t / steps for t in tier_ms],
-        "arena_gb": st.arena_capacity / 1e9,
-        "arena_high_water_gb": st.arena_high_water / 1e9,
-        "modularity": q_gpu,
-        "communities": int(final.max()) + 1,
+        "config": m.pop("config"),
+        "roofline": m.pop("roofline"),
     }
-
+    out["e2e_s"] = out["ms_per_step"] * 1e-3
+    out.update(m)
     if not args.no_cpu:
-        import oracle
-        threads = args.cpu_threads or host_threads()
-        c0 = time.perf_counter()
-        r = oracle.run(n, src, dst, w, threads=threads)
-        cpu_s = time.perf_counter() - c0
-        same = bool(np.array_equal(r.final_comm(), final))
-        q_or = r.final_modularity()
-        out["cpu_baseline"] = {
-            "value": nnz0 / cpu_s,
-            "unit": "TEPS",
-            "cores": threads,
-            "kind": "port",
-            "sample": "oracle (C restatement, OpenMP) full multi-level run on the same %s graph, "
-                      "%.1f s" % (wname, cpu_s),
-        }
-        out["validated"] = same and abs(q_or - q_gpu) <= 1e-9 * max(1.0, abs(q_or))
-        out["oracle_modularity"] = q_or
-        out["speedup_vs_cpu"] = cpu_s / (ms * 1e-3)
+        out.update(cpu_baselines(n, src, dst, w, final, q_gpu, wname, nnz0, out["e2e_s"]))
     else:
         out["cpu_baseline"] = None
         out["validated"] = None
+    del graph, src, dst, w
     if args.copy_probe_mb:
         out["copy_probe_bytes"] = copy_probe(args.copy_probe_mb << 20)
-    eng.close()
+    if args.workload == "rmat22" and not args.no_rmat26:
+        out["rmat26f"] = rmat26f_object(args)
     if not args.no_batch:
         b = run_batched(args)
         out["batched"] = {k: b[k] for k in (

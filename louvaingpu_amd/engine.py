@@ -9,6 +9,7 @@ Louvain on the host.
 from __future__ import annotations
 
 import ctypes as ct
+import weakref
 
 import numpy as np
 
@@ -38,6 +39,37 @@ def make_config(gamma=1.0, min_gain=MIN, max_passes=0, max_levels=0, device=0, t
     cfg.flags = (_lib.LV_FLAG_TRACE if trace else 0) | (_lib.LV_FLAG_TIMING if timing else 0)
     cfg.arena_bytes = int(arena_bytes)
     return cfg
+
+
+_hip = None
+
+
+def pinned_empty(count, dtype) -> np.ndarray:
+    """An uninitialised numpy array in page-locked host memory (hipHostMalloc),
+    freed with hipHostFree when the array is collected.  Copies between it and
+    the device run at the link's full rate without a staging copy."""
+    global _hip
+    if _hip is None:
+        _hip = ct.CDLL("libamdhip64.so")
+        _hip.hipHostMalloc.argtypes = [ct.POINTER(ct.c_void_p), ct.c_size_t, ct.c_uint]
+        _hip.hipHostFree.argtypes = [ct.c_void_p]
+    dtype = np.dtype(dtype)
+    nbytes = max(1, int(count) * dtype.itemsize)
+    p = ct.c_void_p()
+    rc = _hip.hipHostMalloc(ct.byref(p), nbytes, 0)
+    if rc != 0 or not p.value:
+        raise MemoryError("hipHostMalloc of %d bytes failed (%d)" % (nbytes, rc))
+    buf = (ct.c_char * nbytes).from_address(p.value)
+    weakref.finalize(buf, _hip.hipHostFree, ct.c_void_p(p.value))
+    return np.frombuffer(buf, dtype, count=int(count))
+
+
+def _out(out, n, dtype=np.int32):
+    if out is None:
+        return np.empty(n, dtype)
+    if out.dtype != dtype or len(out) < n or not out.flags.c_contiguous:
+        raise ValueError("out must be a contiguous %s array of at least %d entries" % (dtype, n))
+    return out[:n]
 
 
 def weights_array(w):
@@ -160,10 +192,11 @@ class Engine:
         check(self._lib.lv_level_info_get(self._h, int(level), ct.byref(info)), "lv_level_info_get")
         return info
 
-    def level_partition(self, level):
+    def level_partition(self, level, out=None):
+        """(node, comm) of the level's Partition; out=(node, comm) buffers to fill."""
         n = self.level_info(level).n
-        node = np.empty(n, np.int32)
-        comm = np.empty(n, np.int32)
+        node = _out(None if out is None else out[0], n)
+        comm = _out(None if out is None else out[1], n)
         check(self._lib.lv_level_partition(self._h, int(level), _p32(node), _p32(comm)),
               "lv_level_partition")
         return node, comm
@@ -180,8 +213,8 @@ class Engine:
         check(self._lib.lv_flatten(self._h, int(level), _p32(node), _p32(comm)), "lv_flatten")
         return node, comm
 
-    def final_assignment(self):
-        out = np.empty(self.n, np.int32)
+    def final_assignment(self, out=None):
+        out = _out(out, self.n)
         check(self._lib.lv_final_assignment(self._h, _p32(out)), "lv_final_assignment")
         return out
 
