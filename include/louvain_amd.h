@@ -63,6 +63,17 @@ typedef struct {
                             LouvainCUDA/LouvainCPP.cpp:774) */
     int32_t flags;       /* LV_FLAG_* */
     uint64_t arena_bytes; /* 0 = sized from the graph at lv_load_edges */
+    /* Launch-shape tunables; 0 = the tuning table (tuning.txt beside the library,
+     * or the file LV_TUNING_FILE names, written by louvaingpu_amd/tune.py) for
+     * the graph's size, else the built-in default.  A row or coarsening segment
+     * with more than hub_q candidates (64..2048; default 2048) takes the hub
+     * hash partitions of about hub_part_target entries (64..2^30; default
+     * 2048), cut into chunks of chunk_mul entries per partition (1..1024;
+     * default 32).  Results do not depend on them. */
+    int32_t hub_q;
+    int32_t hub_part_target;
+    int32_t chunk_mul;
+    int32_t reserved;    /* 0 */
 } lv_config;
 
 /* Per-level summary (the reference keeps only Partition{count,...}). */

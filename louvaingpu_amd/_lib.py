@@ -67,6 +67,10 @@ class LvConfig(ct.Structure):
         ("device", ct.c_int32),
         ("flags", ct.c_int32),
         ("arena_bytes", ct.c_uint64),
+        ("hub_q", ct.c_int32),
+        ("hub_part_target", ct.c_int32),
+        ("chunk_mul", ct.c_int32),
+        ("reserved", ct.c_int32),
     ]
 
 

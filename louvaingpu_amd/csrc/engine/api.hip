@@ -6,7 +6,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <dlfcn.h>
 #include <mutex>
+#include <string>
 #include <new>
 #include <vector>
 
@@ -94,7 +96,9 @@ struct lv_ctx {
     // expected entries per hub partition; the LDS table of a partition holds 4096
     // distinct communities (more fall back to a global table)
     int32_t hub_part_target = kHubPartTarget;
-    int chunk_mul = kChunkMul;  // chunk_span(P, chunk_mul); LV_CHUNK_MUL overrides
+    int chunk_mul = kChunkMul;  // chunk_span(P, chunk_mul)
+    // rows and coarsening segments with more candidates take the hub partitions
+    int32_t hub_q = kTierMaxQ[kTiers - 2];
     // groups of the last renumbering (sorted order): head flags, their exclusive
     // scan, first sorted index of each group, new id of each group
     int32_t *g_head = nullptr, *g_hscan = nullptr, *g_start = nullptr, *g_newid = nullptr;
@@ -713,7 +717,7 @@ int init_level(lv_ctx *c)
     launch_init_vertex(s, n, c->rowptr, c->kdeg, c->loopw, c->selfsum, c->deg, c->in, c->size,
                        c->ddeg, c->dsize, c->C, c->next, tkey, v1, c->d_small,
                        c->d_small + (kTiers + 1), (ull *)(c->d_small + 2 * (kTiers + 1)),
-                       c->deg32);
+                       c->deg32, c->hub_q);
     uint32_t *ko = nullptr;
     int32_t *vo = nullptr;
     LV_TRY(radix_sort_u32_i32(A, s, tkey, v1, tkey2, v2, n, 4, &ko, &vo));
@@ -976,7 +980,8 @@ int induce(lv_ctx *c, const int32_t *newc, const int32_t *newid_g, int32_t Kn)
     LV_TRY(arena_alloc(A, sid1, K));
     LV_TRY(arena_alloc(A, sid2, K));
     LV_TRY(zero_small(c, kTiers + 1));
-    launch_cr_seg_tier(s, c->g_start, estart, newid_g, K, sb, slen, tk1, sid1, c->d_small);
+    launch_cr_seg_tier(s, c->g_start, estart, newid_g, K, sb, slen, tk1, sid1, c->d_small,
+                       c->hub_q);
     uint32_t *tko = nullptr;
     int32_t *segs = nullptr;
     LV_TRY(radix_sort_u32_i32(A, s, tk1, sid1, tk2, sid2, K, 4, &tko, &segs));
@@ -1000,6 +1005,7 @@ int induce(lv_ctx *c, const int32_t *newc, const int32_t *newid_g, int32_t Kn)
     ca.nuniq = nuniq;
     ca.internal = internal;
     ca.ssum = ssum;
+    ca.big_q = c->hub_q;
     for (int t = 1; t < kTiers; ++t) launch_cr_agg_tier(s, t, ca, segs + toff[t], (int32_t)tcnt[t]);
     // segments above 2048 entries: hash partitions
     BigSegArgs B{};
@@ -1451,6 +1457,81 @@ int batch_run(lv_ctx *c)
     return LV_OK;
 }
 
+// ---------------------------------------------------------------------------
+// launch-shape tunables: lv_config fields, else the tuning table, else defaults
+
+bool tunables_valid(const lv_config &c)
+{
+    return (c.hub_q == 0 || (c.hub_q >= 64 && c.hub_q <= kTierMaxQ[kTiers - 2])) &&
+           (c.hub_part_target == 0 || (c.hub_part_target >= 64 && c.hub_part_target <= (1 << 30))) &&
+           (c.chunk_mul == 0 || (c.chunk_mul >= 1 && c.chunk_mul <= 1024)) && c.reserved == 0;
+}
+
+struct TuneRow {
+    int64_t max_nnz;
+    int32_t hub_q, part_target, chunk_mul;
+};
+
+// The table: lines "max_nnz hub_q hub_part_target chunk_mul" ('#' comments),
+// read once per process from LV_TUNING_FILE or tuning.txt in the directory of
+// this library.  Rows with a value out of range are ignored.
+const std::vector<TuneRow> &tuning_table()
+{
+    static std::once_flag once;
+    static std::vector<TuneRow> rows;
+    std::call_once(once, [] {
+        std::string path;
+        if (const char *e = getenv("LV_TUNING_FILE")) {
+            path = e;
+        } else {
+            Dl_info info;
+            if (dladdr((void *)&tuning_table, &info) && info.dli_fname) {
+                path = info.dli_fname;
+                const size_t cut = path.find_last_of('/');
+                path = (cut == std::string::npos ? std::string(".") : path.substr(0, cut)) +
+                       "/tuning.txt";
+            }
+        }
+        FILE *f = path.empty() ? nullptr : fopen(path.c_str(), "r");
+        if (!f) return;
+        char line[256];
+        while (fgets(line, sizeof(line), f)) {
+            if (line[0] == '#') continue;
+            long long m = 0;
+            int q = 0, t = 0, k = 0;
+            if (sscanf(line, "%lld %d %d %d", &m, &q, &t, &k) != 4) continue;
+            lv_config probe{};
+            probe.hub_q = q;
+            probe.hub_part_target = t;
+            probe.chunk_mul = k;
+            if (m <= 0 || q <= 0 || t <= 0 || k <= 0 || !tunables_valid(probe)) continue;
+            rows.push_back({m, q, t, k});
+        }
+        fclose(f);
+        std::sort(rows.begin(), rows.end(),
+                  [](const TuneRow &a, const TuneRow &b) { return a.max_nnz < b.max_nnz; });
+    });
+    return rows;
+}
+
+// Launch shapes for a graph of nnz directed entries: each configured field, else
+// the first table row whose max_nnz covers nnz (the last row beyond it), else
+// the built-in default.
+void apply_tuning(lv_ctx *c, int64_t nnz)
+{
+    const std::vector<TuneRow> &t = tuning_table();
+    const TuneRow *row = nullptr;
+    for (const TuneRow &r : t) {
+        row = &r;
+        if (nnz <= r.max_nnz) break;
+    }
+    c->hub_q = c->cfg.hub_q ? c->cfg.hub_q : row ? row->hub_q : kTierMaxQ[kTiers - 2];
+    c->hub_part_target = c->cfg.hub_part_target ? c->cfg.hub_part_target
+                         : row                  ? row->part_target
+                                                : kHubPartTarget;
+    c->chunk_mul = c->cfg.chunk_mul ? c->cfg.chunk_mul : row ? row->chunk_mul : kChunkMul;
+}
+
 int check_ctx(lv_ctx *c) { return c ? LV_OK : LV_ERR_ARG; }
 
 int set_device(lv_ctx *c)
@@ -1506,19 +1587,11 @@ int lv_create(lv_ctx **out, const lv_config *cfg)
         lv_config_default(&c->cfg);
     memset(&c->stats, 0, sizeof(c->stats));
     if (!(c->cfg.gamma >= 0.0) || !(c->cfg.min_gain == c->cfg.min_gain) || c->cfg.max_passes < 0 ||
-        c->cfg.max_levels < 0) {
+        c->cfg.max_levels < 0 || !tunables_valid(c->cfg)) {
         delete c;
         return LV_ERR_ARG;
     }
     c->device = c->cfg.device;
-    if (const char *e = getenv("LV_HUB_PART_TARGET")) {
-        long t = strtol(e, nullptr, 10);
-        if (t >= 1 && t <= (1 << 30)) c->hub_part_target = (int32_t)t;
-    }
-    if (const char *e = getenv("LV_CHUNK_MUL")) {
-        long t = strtol(e, nullptr, 10);
-        if (t >= 0 && t <= 1024) c->chunk_mul = (int)t;
-    }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= c->device || c->device < 0) {
         (void)hipGetLastError();
@@ -1584,6 +1657,7 @@ int lv_load_edges(lv_ctx *c, int32_t n, const int32_t *ids, int64_t e, const int
     reset_run(c);
     free_input(c);
     c->state = ST_EMPTY;
+    apply_tuning(c, 2 * e);
     Arena &A = c->arena;
     hipStream_t s = c->stream;
     // arena: sized for the largest level-0 phase unless the caller fixed it
@@ -2156,6 +2230,7 @@ int batch_load(lv_ctx *c, int32_t ngraphs, const int32_t *n, const int64_t *e, c
     Arena &A = c->arena;
     hipStream_t s = c->stream;
     const int32_t nc = ngraphs * nres;
+    apply_tuning(c, 2 * eu);
     if (!c->cfg.arena_bytes) {
         size_t want = (size_t)200 * (size_t)eu + (size_t)320 * (size_t)nu + (size_t)16 * (size_t)eb +
                       (size_t)96 * (size_t)nc + ((size_t)256 << 20);
@@ -2451,7 +2526,8 @@ int lv_run_static(const lv_config *cfg, void *stream, int32_t n, const int32_t *
     if (n < 0 || e < 0 || (e > 0 && (!src || !dst || !w)) || (n > 0 && !final_comm) ||
         (wtype != LV_WEIGHT_INT32 && wtype != LV_WEIGHT_FLOAT32) || cf.flags != 0 ||
         !(cf.gamma >= 0.0) || !(cf.min_gain == cf.min_gain) || cf.max_passes < 0 ||
-        cf.max_levels < 0 || cf.device < 0 || cf.device >= kStaticMaxDevices)
+        cf.max_levels < 0 || cf.device < 0 || cf.device >= kStaticMaxDevices ||
+        !tunables_valid(cf))
         return LV_ERR_ARG;
     // the footprint bound of the arena sizing (200 B per edge, 320 B per vertex)
     const size_t room = (size_t)LV_STATIC_SCRATCH_BYTES - kStaticHead;

@@ -268,7 +268,7 @@ void launch_cr_gather(hipStream_t s, const int64_t *estart, int32_t *bound, cons
 __global__ __launch_bounds__(256) void k_cr_seg_tier(const int32_t *gstart, const int64_t *estart,
                                                     const int32_t *newid_g, int32_t K, int64_t *sb,
                                                     int64_t *slen, uint32_t *tkey, int32_t *sid,
-                                                    long long *tier_cnt)
+                                                    long long *tier_cnt, int32_t hub_q)
 {
     __shared__ long long s_cnt[kTiers + 1];
     if (threadIdx.x <= kTiers) s_cnt[threadIdx.x] = 0;
@@ -282,7 +282,7 @@ __global__ __launch_bounds__(256) void k_cr_seg_tier(const int32_t *gstart, cons
             int64_t b = estart[gstart[g]], e = estart[gstart[g + 1]];
             sb[g] = b;
             slen[g] = e - b;
-            t = newid_g[g] < 0 ? 0 : tier_of(e - b, e - b);
+            t = newid_g[g] < 0 ? 0 : tier_of(e - b, e - b, hub_q);
             tkey[g] = (uint32_t)t;
             sid[g] = (int32_t)g;
         }
@@ -295,11 +295,11 @@ __global__ __launch_bounds__(256) void k_cr_seg_tier(const int32_t *gstart, cons
 
 void launch_cr_seg_tier(hipStream_t s, const int32_t *gstart, const int64_t *estart,
                         const int32_t *newid_g, int32_t K, int64_t *sb, int64_t *slen,
-                        uint32_t *tkey, int32_t *sid, long long *tier_cnt)
+                        uint32_t *tkey, int32_t *sid, long long *tier_cnt, int32_t hub_q)
 {
     if (K > 0)
         hipLaunchKernelGGL(k_cr_seg_tier, grid_for(K, 256, 1024), 256, 0, s, gstart, estart,
-                           newid_g, K, sb, slen, tkey, sid, tier_cnt);
+                           newid_g, K, sb, slen, tkey, sid, tier_cnt, hub_q);
 }
 
 // Writes the occupied slots (key >= 0) of a 256-thread block's LDS table of
@@ -768,7 +768,7 @@ __global__ void k_cr_copy_small(CoarsenArgs a, const int64_t *rowptr2, int32_t K
     const int lg = threadIdx.x % L;
     for (int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / L; g < K;
          g += ((int64_t)gridDim.x * blockDim.x) / L) {
-        if (a.slen[g] > kTierMaxQ[kTiers - 2]) continue;  // big segments copy per partition
+        if (a.slen[g] > a.big_q) continue;  // big segments copy per partition
         if (a.newid_g[g] < 0) continue;                     // stopped component
         const int64_t src = a.sb[g], dst = rowptr2[a.newid_g[g]];
         const int32_t len = a.nuniq[g];

@@ -293,9 +293,12 @@ struct SegCarry {
     }
 };
 
-__device__ __forceinline__ int tier_of(int64_t d, int64_t q)
+// Tier of a row of d entries and q candidates; rows with q > hub_q (at most
+// kTierMaxQ[kTiers - 2]) take the hub partitions.
+__device__ __forceinline__ int tier_of(int64_t d, int64_t q, int64_t hub_q)
 {
     if (d == 0) return 0;  // no neighbours: the vertex can never move
+    if (q > hub_q) return kTiers;
 #pragma unroll
     for (int t = 0; t < kTiers - 1; ++t)
         if (q <= kTierMaxQ[t]) return t + 1;

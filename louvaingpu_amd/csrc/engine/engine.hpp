@@ -169,7 +169,7 @@ struct SweepArgs {
 // LDS table, then reduces the partitions of a hub.
 constexpr int kHubChunk = 4096;
 constexpr int kChunkMul = 32;  // entries per partition of a chunk (0: kHubChunk fixed)
-constexpr int kHubPartTarget = 2048;  // default; LV_HUB_PART_TARGET overrides
+constexpr int kHubPartTarget = 2048;  // default (lv_config / tuning table override)
 constexpr int kHubMaxParts = 2048;
 constexpr int kHubAggLog2 = 12;  // 4096 LDS slots per partition block
 
@@ -217,6 +217,7 @@ struct CoarsenArgs {
     int32_t *nuniq;         // uniques per group
     ull *internal;          // sum of w' over entries inside the group
     ull *ssum;              // sum of w' over all entries of the group
+    int64_t big_q;          // segments longer than this take the hash partitions
 };
 
 // Segments of more than 2048 entries, partitioned like the hub rows.
@@ -322,7 +323,8 @@ void launch_init_vertex(hipStream_t s, int32_t n, const int64_t *rowptr, const i
                         const int64_t *loopw, const int64_t *selfsum, int64_t *deg, int64_t *in,
                         int32_t *size, ull *ddeg, int32_t *dsize, int32_t *C,
                         int32_t *next, uint32_t *tier_key, int32_t *verts,
-                        long long *tier_cnt, long long *tier_nnz, ull *ksum, uint32_t *deg32);
+                        long long *tier_cnt, long long *tier_nnz, ull *ksum, uint32_t *deg32,
+                        int32_t hub_q);
 // Q aggregates: per-block partials in part[kQParts], summed into *acc.
 constexpr int kQParts = 1024;
 void launch_qagg(hipStream_t s, int32_t n, const int64_t *deg, const int64_t *in, QAcc *part,
@@ -429,7 +431,7 @@ void launch_cr_gather(hipStream_t s, const int64_t *estart, int32_t *bound, cons
                       uint32_t *ew32);
 void launch_cr_seg_tier(hipStream_t s, const int32_t *gstart, const int64_t *estart,
                         const int32_t *newid_g, int32_t K, int64_t *sb, int64_t *slen,
-                        uint32_t *tkey, int32_t *sid, long long *tier_cnt);
+                        uint32_t *tkey, int32_t *sid, long long *tier_cnt, int32_t hub_q);
 void launch_cr_agg_tier(hipStream_t s, int tier, const CoarsenArgs &a, const int32_t *segs, int32_t ns);
 void launch_cr_big_setup(hipStream_t s, const int32_t *bsegs, int32_t nbs, const int64_t *slen,
                          int32_t target, int chunk_mul, int32_t *bP, int64_t *bP64, int64_t *bch,

@@ -865,7 +865,7 @@ __global__ __launch_bounds__(256) void k_init_vertex(
     int32_t n, const int64_t *rowptr, const int64_t *kdeg, const int64_t *loopw,
     const int64_t *selfsum, int64_t *deg, int64_t *in, int32_t *size, ull *ddeg, int32_t *dsize,
     int32_t *C, int32_t *next, uint32_t *tier_key, int32_t *verts, long long *tier_cnt,
-    long long *tier_nnz, ull *ksum, uint32_t *deg32)
+    long long *tier_nnz, ull *ksum, uint32_t *deg32, int32_t hub_q)
 {
     __shared__ long long s_cnt[kTiers + 1], s_nnz[kTiers + 1];
     __shared__ ull s_k[4];
@@ -893,7 +893,7 @@ __global__ __launch_bounds__(256) void k_init_vertex(
             C[i] = (int32_t)i;
             next[i] = (int32_t)i;
             d = rowptr[i + 1] - rowptr[i];
-            t = tier_of(d, d + (selfsum[i] > 0 ? 1 : 0));
+            t = tier_of(d, d + (selfsum[i] > 0 ? 1 : 0), hub_q);
             tier_key[i] = (uint32_t)t;
             verts[i] = (int32_t)i;
         }
@@ -916,12 +916,12 @@ void launch_init_vertex(hipStream_t s, int32_t n, const int64_t *rowptr, const i
                         const int64_t *loopw, const int64_t *selfsum, int64_t *deg, int64_t *in,
                         int32_t *size, ull *ddeg, int32_t *dsize, int32_t *C, int32_t *next,
                         uint32_t *tier_key, int32_t *verts, long long *tier_cnt,
-                        long long *tier_nnz, ull *ksum, uint32_t *deg32)
+                        long long *tier_nnz, ull *ksum, uint32_t *deg32, int32_t hub_q)
 {
     if (n > 0)
         hipLaunchKernelGGL(k_init_vertex, grid_for(n, 256, 1024), 256, 0, s, n, rowptr, kdeg,
                            loopw, selfsum, deg, in, size, ddeg, dsize, C, next, tier_key, verts,
-                           tier_cnt, tier_nnz, ksum, deg32);
+                           tier_cnt, tier_nnz, ksum, deg32, hub_q);
 }
 
 // ---------------------------------------------------------------------------

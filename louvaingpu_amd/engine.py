@@ -28,7 +28,8 @@ def _p64(a):
 
 
 def make_config(gamma=1.0, min_gain=MIN, max_passes=0, max_levels=0, device=0, trace=False,
-                timing=False, arena_bytes=0) -> LvConfig:
+                timing=False, arena_bytes=0, hub_q=0, hub_part_target=0, chunk_mul=0) -> LvConfig:
+    """lv_config; hub_q / hub_part_target / chunk_mul = 0 take the tuning table."""
     cfg = LvConfig()
     _lib.engine().lv_config_default(ct.byref(cfg))
     cfg.gamma = float(gamma)
@@ -38,6 +39,9 @@ def make_config(gamma=1.0, min_gain=MIN, max_passes=0, max_levels=0, device=0, t
     cfg.device = int(device)
     cfg.flags = (_lib.LV_FLAG_TRACE if trace else 0) | (_lib.LV_FLAG_TIMING if timing else 0)
     cfg.arena_bytes = int(arena_bytes)
+    cfg.hub_q = int(hub_q)
+    cfg.hub_part_target = int(hub_part_target)
+    cfg.chunk_mul = int(chunk_mul)
     return cfg
 
 

@@ -74,9 +74,9 @@ def assert_run_matches(eng, r, n, trace=True):
     assert math.isclose(qe, qo, rel_tol=Q_REL, abs_tol=Q_REL), (qe, qo)
 
 
-def run_both(n, src, dst, w, ids=None, gamma=1.0, threads=8):
+def run_both(n, src, dst, w, ids=None, gamma=1.0, threads=8, **tunables):
     r = oracle.run(n, src, dst, w, ids=ids, gamma=gamma, trace=True, threads=threads)
-    eng = Engine(gamma=gamma, trace=True)
+    eng = Engine(gamma=gamma, trace=True, **tunables)
     eng.load_edges(n, src, dst, w, ids=ids)
     eng.run()
     return eng, r
@@ -338,13 +338,24 @@ def test_star_with_long_chunks():
         assert_run_matches(eng, r, n, trace=False)
 
 
-def test_hub_global_fallback(monkeypatch):
+@pytest.mark.parametrize("hub_q,part,chunk", [(512, 1024, 8), (64, 256, 1), (1024, 4096, 64)])
+def test_launch_shapes_do_not_change_results(hub_q, part, chunk):
+    """Any hub threshold, partition target and chunk length (the autotuner's
+    knobs) gives the oracle's results: more rows and coarsening segments take
+    the partition pipelines at low thresholds."""
+    gpu_case("launch shapes hub_q=%d part=%d chunk=%d" % (hub_q, part, chunk))
+    n, src, dst, w = generators.rmat(15, 16, 0.57, 0.19, 0.19, 13, "int")
+    eng, r = run_both(n, src, dst, w, hub_q=hub_q, hub_part_target=part, chunk_mul=chunk)
+    with eng:
+        assert_run_matches(eng, r, n)
+
+
+def test_hub_global_fallback():
     """One partition per hub holds ~9000 distinct communities, more than the
     4096-slot LDS table: the partition is redone in its global table."""
     gpu_case("hub global fallback")
-    monkeypatch.setenv("LV_HUB_PART_TARGET", "1000000")
     n, src, dst, w = _hub_graph(6)
-    eng, r = run_both(n, src, dst, w)
+    eng, r = run_both(n, src, dst, w, hub_part_target=1000000)
     with eng:
         assert_run_matches(eng, r, n)
 
