@@ -139,6 +139,8 @@ struct lv_ctx {
     std::vector<hipEvent_t> ev_pool;
     std::vector<PendingTime> pending;
     double last_run_ms = 0;
+    // arena allocations of the loaded input (a run starts from exactly these)
+    std::map<size_t, size_t> load_mark;
     // lv_run_static: the device scalars and the arena live in the static
     // scratch array, the stream is the caller's, and the host copies of the
     // scalars are these ordinary (pageable) members
@@ -356,6 +358,7 @@ void reset_run(lv_ctx *c)
     free_level_arrays(c);
     free_dendrogram(c);
     free_batch_run(c);
+    if (c->loaded) c->arena.restore(c->load_mark);
     c->cur_pass_q.clear();
     c->cur_trace.clear();
     c->level = 0;
@@ -1656,6 +1659,7 @@ int lv_load_edges(lv_ctx *c, int32_t n, const int32_t *ids, int64_t e, const int
     LV_HIP(hipStreamSynchronize(c->stream));
     reset_run(c);
     free_input(c);
+    c->arena.release_all();  // also what an earlier failed call left behind
     c->state = ST_EMPTY;
     apply_tuning(c, 2 * e);
     Arena &A = c->arena;
@@ -1736,6 +1740,7 @@ int lv_load_edges(lv_ctx *c, int32_t n, const int32_t *ids, int64_t e, const int
     c->n0 = n;
     c->e0 = e;
     c->loaded = true;
+    c->load_mark = c->arena.mark();
     c->level = 0;
     c->state = ST_LOADED;
     return LV_OK;
@@ -2226,6 +2231,7 @@ int batch_load(lv_ctx *c, int32_t ngraphs, const int32_t *n, const int64_t *e, c
     LV_HIP(hipStreamSynchronize(c->stream));
     reset_run(c);
     free_input(c);
+    c->arena.release_all();  // also what an earlier failed call left behind
     c->state = ST_EMPTY;
     Arena &A = c->arena;
     hipStream_t s = c->stream;
@@ -2338,6 +2344,7 @@ int batch_load(lv_ctx *c, int32_t ngraphs, const int32_t *n, const int64_t *e, c
     c->nres = nres;
     c->batch = true;
     c->loaded = true;
+    c->load_mark = c->arena.mark();
     c->level = 0;
     c->state = ST_LOADED;
     return LV_OK;

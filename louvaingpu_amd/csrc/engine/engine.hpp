@@ -55,6 +55,10 @@ public:
     void *alloc(size_t bytes);
     void release(void *p);
     void release_all();
+    // The live allocations now; restore() releases every allocation made since
+    // (what a call that failed part-way left behind).
+    std::map<size_t, size_t> mark() const { return used_; }
+    void restore(const std::map<size_t, size_t> &m);
     size_t capacity() const { return cap_; }
     size_t in_use() const { return used_bytes_; }
     size_t high_water() const { return high_; }

@@ -105,6 +105,14 @@ void Arena::release(void *p)
     free_[off] = sz;
 }
 
+void Arena::restore(const std::map<size_t, size_t> &m)
+{
+    std::vector<size_t> extra;
+    for (const auto &u : used_)
+        if (m.find(u.first) == m.end()) extra.push_back(u.first);
+    for (size_t off : extra) release(base_ + off);
+}
+
 void Arena::release_all()
 {
     used_.clear();

@@ -4,9 +4,10 @@
 (``LouvainCUDA/LouvainCPP.cpp:808-941``): node dicts with ``"id"``, edge dicts
 with ``"source"``, ``"target"``, ``"weight"``; it returns
 ``{"size": communities, node_id: community, ...}`` from ``getPartitions`` and
-writes ``"cluster"``/``"my_community"`` into each node dict.  ``part_init`` is
-accepted and ignored, as in the reference.  The circle layout of the
-reference's visualisation post-processing is not part of this package.
+writes ``"cluster"``/``"my_community"`` and the circle-layout position
+``"x"``/``"y"`` of the reference's visualisation post-processing
+(``calculateClusterSize`` + ``layoutProcess``, ``LouvainCUDA/LouvainCPP.cpp:555-651``)
+into each node dict.  ``part_init`` is accepted and ignored, as in the reference.
 
 ``generate_dendogram`` runs the reference's level loop through the C++ host
 layer (``generate_dendogram`` in ``csrc/host/louvain.cpp``), which calls the

@@ -6,7 +6,7 @@ import pytest
 
 import oracle
 from conftest import gpu_case
-from louvaingpu_amd import count_partitions, generate_dendogram, generators, start
+from louvaingpu_amd import Engine, count_partitions, generate_dendogram, generators, start
 
 pytestmark = pytest.mark.gpu
 
@@ -59,3 +59,22 @@ def test_start_dict_api():
                                      count_partitions(ocomm))
     assert [nd["x"] for nd in nodes] == [float(v) for v in x]
     assert [nd["y"] for nd in nodes] == [float(v) for v in y]
+
+
+def test_undersized_arena_reports_nomem_and_recovers():
+    """An arena too small for the graph gives LV_ERR_NOMEM (no abort, no retry
+    loop as in LouvainCUDA/LouvainCPP.cpp:921-936); a graph that fits still
+    runs on the same context afterwards."""
+    from louvaingpu_amd._lib import LV_ERR_NOMEM, LouvainError
+    gpu_case("undersized arena")
+    big = generators.rmat(14, 16, 0.57, 0.19, 0.19, 3, "int")
+    small = generators.sbm(500, 100, 0.1, 0.002, 3)
+    with Engine(arena_bytes=4 << 20) as eng:
+        with pytest.raises(LouvainError) as ex:
+            eng.load_edges(*big)
+            eng.run()
+        assert ex.value.status == LV_ERR_NOMEM
+        eng.load_edges(*small)
+        eng.run()
+        r = oracle.run(*small)
+        assert np.array_equal(eng.final_assignment(), r.final_comm())
