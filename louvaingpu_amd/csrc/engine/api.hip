@@ -115,6 +115,12 @@ struct lv_ctx {
     // union graph; component c is graph c / nres at resolution c % nres
     bool batch = false;
     int32_t ncomp = 0, nres = 1;
+    // the input holds the graphs once (ngraphs graphs, base_n0 vertices and
+    // base_e0 edges, graph g's vertices from d_bvoff[g]); level 0 is built over
+    // them and then written out for every resolution
+    int32_t ngraphs = 0, base_n0 = 0;
+    int64_t base_e0 = 0;
+    int64_t *d_bvoff = nullptr;
     std::vector<int32_t> comp_n0;           // level-0 vertices per component
     std::vector<int64_t> comp_e0;           // input edges per component
     std::vector<double> comp_gamma;
@@ -326,6 +332,7 @@ void free_dendrogram(lv_ctx *c)
 
 void free_input(lv_ctx *c)
 {
+    arena_free(c->arena, c->d_bvoff);
     arena_free(c->arena, c->d_ids);
     arena_free(c->arena, c->d_src);
     arena_free(c->arena, c->d_dst);
@@ -1126,6 +1133,63 @@ int induce(lv_ctx *c, const int32_t *newc, const int32_t *newid_g, int32_t Kn)
     return LV_OK;
 }
 
+// Level 0 of a graph set: the CSR of the graphs (built once) written out for
+// every resolution, so each component's rows are its graph's.
+int build_level0_csr_sets(lv_ctx *c)
+{
+    Arena &A = c->arena;
+    hipStream_t s = c->stream;
+    const int32_t nu = c->n0;
+    const int64_t eu = c->e0;
+    c->n0 = c->base_n0;
+    c->e0 = c->base_e0;
+    const int rc = build_level0_csr(c);
+    c->n0 = nu;
+    c->e0 = eu;
+    if (rc != LV_OK) return rc;
+    const int32_t nres = c->nres, nb = c->base_n0;
+    if (nres == 1) return LV_OK;
+    hipEvent_t t0 = t_begin(c);
+    const int64_t unnz = (int64_t)nres * c->nnz;
+    int64_t *urow = nullptr, *uk = nullptr, *ul = nullptr, *us = nullptr, *uw = nullptr,
+            *geoff = nullptr;
+    int32_t *ucol = nullptr;
+    uint32_t *uw32 = nullptr;
+    LV_TRY(arena_alloc(A, urow, (int64_t)nu + 1));
+    LV_TRY(arena_alloc(A, uk, nu));
+    LV_TRY(arena_alloc(A, ul, nu));
+    LV_TRY(arena_alloc(A, us, nu));
+    LV_TRY(arena_alloc(A, ucol, unnz));
+    if (c->cw32)
+        LV_TRY(arena_alloc(A, uw32, unnz));
+    else
+        LV_TRY(arena_alloc(A, uw, unnz));
+    LV_TRY(arena_alloc(A, geoff, (int64_t)c->ngraphs + 1));
+    launch_b_replicate_csr(s, c->d_bvoff, geoff, c->ngraphs, nres, nb, c->nnz, c->rowptr, c->col,
+                           c->cw, c->cw32, c->kdeg, c->loopw, c->selfsum, urow, ucol, uw, uw32, uk,
+                           ul, us);
+    LV_HIP(hipGetLastError());
+    t_end(c, t0, CAT_CSR);
+    arena_free(A, geoff);
+    arena_free(A, c->rowptr);
+    arena_free(A, c->col);
+    arena_free(A, c->cw);
+    arena_free(A, c->cw32);
+    arena_free(A, c->kdeg);
+    arena_free(A, c->loopw);
+    arena_free(A, c->selfsum);
+    c->rowptr = urow;
+    c->col = ucol;
+    c->cw = uw;
+    c->cw32 = uw32;
+    c->kdeg = uk;
+    c->loopw = ul;
+    c->selfsum = us;
+    c->nnz = unnz;
+    c->n = nu;
+    return LV_OK;
+}
+
 int flatten_final(lv_ctx *c)
 {
     hipEvent_t t0 = t_begin(c);
@@ -1357,7 +1421,7 @@ int batch_run(lv_ctx *c)
     std::vector<double> mod((size_t)nc, 0.0), lower((size_t)nc, 0.0), q;
     std::vector<int32_t> passes;
     std::vector<int64_t> K((size_t)nc, 0);
-    LV_TRY(build_level0_csr(c));
+    LV_TRY(build_level0_csr_sets(c));
     for (int32_t level = 0;; ++level) {
         LV_TRY(batch_level_setup(c, level));
         if (level == 0) {
@@ -2251,10 +2315,12 @@ int batch_load(lv_ctx *c, int32_t ngraphs, const int32_t *n, const int64_t *e, c
             if (rc != LV_OK) return rc;
         }
     }
-    // offsets of the graphs and of the union's components
-    std::vector<int64_t> beoff((size_t)ngraphs + 1, 0), ueoff((size_t)nc + 1, 0),
-        uvoff((size_t)nc + 1, 0);
-    for (int32_t g = 0; g < ngraphs; ++g) beoff[g + 1] = beoff[g] + e[g];
+    // offsets of the graphs (edges, vertices); component k is graph k / nres
+    std::vector<int64_t> beoff((size_t)ngraphs + 1, 0), bvoff((size_t)ngraphs + 1, 0);
+    for (int32_t g = 0; g < ngraphs; ++g) {
+        beoff[g + 1] = beoff[g] + e[g];
+        bvoff[g + 1] = bvoff[g] + n[g];
+    }
     c->comp_n0.assign((size_t)nc, 0);
     c->comp_e0.assign((size_t)nc, 0);
     c->comp_gamma.assign((size_t)nc, 1.0);
@@ -2263,24 +2329,21 @@ int batch_load(lv_ctx *c, int32_t ngraphs, const int32_t *n, const int64_t *e, c
         c->comp_n0[k] = n[g];
         c->comp_e0[k] = e[g];
         c->comp_gamma[k] = gamma[k % nres];
-        ueoff[k + 1] = ueoff[k] + e[g];
-        uvoff[k + 1] = uvoff[k] + n[g];
     }
     int32_t *bsrc = nullptr, *bdst = nullptr, *bn = nullptr;
     uint32_t *bw = nullptr;
     int *gscale = nullptr;
-    int64_t *dbeoff = nullptr, *dueoff = nullptr, *duvoff = nullptr;
+    int64_t *dbeoff = nullptr;
     LV_TRY(arena_alloc(A, bsrc, eb));
     LV_TRY(arena_alloc(A, bdst, eb));
     LV_TRY(arena_alloc(A, bw, eb));
     LV_TRY(arena_alloc(A, bn, ngraphs));
     LV_TRY(arena_alloc(A, gscale, ngraphs));
     LV_TRY(arena_alloc(A, dbeoff, (int64_t)ngraphs + 1));
-    LV_TRY(arena_alloc(A, dueoff, (int64_t)nc + 1));
-    LV_TRY(arena_alloc(A, duvoff, (int64_t)nc + 1));
-    LV_TRY(arena_alloc(A, c->d_src, eu));
-    LV_TRY(arena_alloc(A, c->d_dst, eu));
-    LV_TRY(arena_alloc(A, c->d_w, eu));
+    LV_TRY(arena_alloc(A, c->d_bvoff, (int64_t)ngraphs + 1));
+    LV_TRY(arena_alloc(A, c->d_src, eb));
+    LV_TRY(arena_alloc(A, c->d_dst, eb));
+    LV_TRY(arena_alloc(A, c->d_w, eb));
     if (srcs) {
         LV_TRY(upload_graphs(c, ngraphs, e, beoff, srcs, dsts, ws, bsrc, bdst, bw));
     } else {
@@ -2290,13 +2353,13 @@ int batch_load(lv_ctx *c, int32_t ngraphs, const int32_t *n, const int64_t *e, c
     }
     LV_TRY(h2d(c, bn, n, ngraphs));
     LV_TRY(h2d(c, dbeoff, (const int64_t *)beoff.data(), (int64_t)ngraphs + 1));
-    LV_TRY(h2d(c, dueoff, (const int64_t *)ueoff.data(), (int64_t)nc + 1));
-    LV_TRY(h2d(c, duvoff, (const int64_t *)uvoff.data(), (int64_t)nc + 1));
+    LV_TRY(h2d(c, c->d_bvoff, (const int64_t *)bvoff.data(), (int64_t)ngraphs + 1));
     LV_HIP(hipMemsetAsync(c->d_flags, 0, sizeof(DevFlags), s));
     fill_i32(s, gscale, ngraphs, 0);
     launch_b_graph_exponent(s, bw, wtype, dbeoff, ngraphs, eb, gscale, c->d_flags);
-    launch_b_union_edges(s, bsrc, bdst, bw, wtype, gscale, dbeoff, bn, dueoff, duvoff, nc, nres, eu,
-                         c->d_src, c->d_dst, c->d_w, c->d_flags);
+    // the graphs once, in their own units, with vertex ids offset by graph
+    launch_b_union_edges(s, bsrc, bdst, bw, wtype, gscale, dbeoff, bn, dbeoff, c->d_bvoff,
+                         ngraphs, 1, eb, c->d_src, c->d_dst, c->d_w, c->d_flags);
     LV_HIP(hipGetLastError());
     std::vector<int> gs((size_t)ngraphs, 0);
     LV_TRY(d2h(c, gs.data(), gscale, ngraphs));
@@ -2307,10 +2370,8 @@ int batch_load(lv_ctx *c, int32_t ngraphs, const int32_t *n, const int64_t *e, c
     arena_free(A, bw);
     arena_free(A, bn);
     arena_free(A, gscale);
-    arena_free(A, dbeoff);
     if (c->h_flags->err) {
-        arena_free(A, dueoff);
-        arena_free(A, duvoff);
+        arena_free(A, dbeoff);
         free_input(c);
         return (c->h_flags->err & 1) ? LV_ERR_INVALID_INPUT : LV_ERR_RANGE;
     }
@@ -2320,20 +2381,22 @@ int batch_load(lv_ctx *c, int32_t ngraphs, const int32_t *n, const int64_t *e, c
         c->comp_scale[k] = gs[k / nres];
         scale = std::max(scale, c->comp_scale[k]);
     }
-    // m of every component (exact units)
+    // m of every graph (exact units), that of each of its components
     LV_TRY(pinned_cap(c, nc));
     ull *dcm = nullptr;
-    LV_TRY(arena_alloc(A, dcm, nc));
-    fill_i64(s, (int64_t *)dcm, nc, 0);
-    launch_b_edge_sum(s, c->d_w, dueoff, nc, eu, dcm);
+    LV_TRY(arena_alloc(A, dcm, ngraphs));
+    fill_i64(s, (int64_t *)dcm, ngraphs, 0);
+    launch_b_edge_sum(s, c->d_w, dbeoff, ngraphs, eb, dcm);
     LV_HIP(hipGetLastError());
-    LV_TRY(d2h(c, c->h_ccnt, dcm, nc));
+    LV_TRY(d2h(c, c->h_ccnt, dcm, ngraphs));
     LV_TRY(sync(c));
     c->comp_m0.assign((size_t)nc, 0);
-    for (int32_t k = 0; k < nc; ++k) c->comp_m0[k] = (int64_t)c->h_ccnt[k];
+    for (int32_t k = 0; k < nc; ++k) c->comp_m0[k] = (int64_t)c->h_ccnt[k / nres];
     arena_free(A, dcm);
-    arena_free(A, dueoff);
-    arena_free(A, duvoff);
+    arena_free(A, dbeoff);
+    c->ngraphs = ngraphs;
+    c->base_n0 = (int32_t)nb;
+    c->base_e0 = eb;
     c->scale = scale;
     c->m0 = 0;
     for (int32_t k = 0; k < nc; ++k) c->m0 += c->comp_m0[k];

@@ -91,6 +91,80 @@ __global__ __launch_bounds__(256) void k_b_union_edges(const int32_t *bsrc, cons
     block_units_flush(tot, 0, umax, bad, f);
 }
 
+// The level-0 CSR of a graph set is built once over the graphs (vertex ranges
+// bvoff, each graph's entries contiguous from geoff[g]) and then written out for
+// every resolution: component c = g * nres + r holds graph g's rows, its
+// vertices at nres * bvoff[g] + r * n_g and its entries at nres * geoff[g] +
+// r * nnz_g, columns moved by the same vertex offset.
+__device__ __forceinline__ int32_t graph_of(const int64_t *bvoff, int32_t ngraphs, int64_t v)
+{
+    int32_t lo = 0, hi = ngraphs;  // bvoff[lo] <= v < bvoff[hi]
+    while (hi - lo > 1) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (bvoff[mid] <= v) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void k_b_graph_entry_off(const int64_t *bvoff, const int64_t *rowptr, int32_t ngraphs,
+                                    int64_t *geoff)
+{
+    GRID_STRIDE(g, (int64_t)ngraphs + 1) geoff[g] = rowptr[bvoff[g]];
+}
+
+__global__ __launch_bounds__(256) void k_b_replicate_entries(
+    const int64_t *bvoff, const int64_t *geoff, int32_t ngraphs, int32_t nres, int64_t nnz,
+    const int32_t *__restrict__ col, const int64_t *__restrict__ w, const uint32_t *__restrict__ w32,
+    int32_t *__restrict__ ucol, int64_t *__restrict__ uw, uint32_t *__restrict__ uw32)
+{
+    GRID_STRIDE(q, nnz)
+    {
+        const int32_t y = col[q];
+        const int32_t g = graph_of(bvoff, ngraphs, y);  // rows and columns share their graph
+        const int64_t vb = bvoff[g], ng = bvoff[g + 1] - vb;
+        const int64_t eg0 = geoff[g], eg = geoff[g + 1] - eg0;
+        const int64_t p0 = (int64_t)nres * eg0 + (q - eg0);
+        const int32_t y0 = (int32_t)((int64_t)nres * vb + (y - vb));
+        if (w32) {
+            const uint32_t x = w32[q];
+            for (int32_t r = 0; r < nres; ++r) {
+                ucol[p0 + r * eg] = y0 + (int32_t)(r * ng);
+                uw32[p0 + r * eg] = x;
+            }
+        } else {
+            const int64_t x = w[q];
+            for (int32_t r = 0; r < nres; ++r) {
+                ucol[p0 + r * eg] = y0 + (int32_t)(r * ng);
+                uw[p0 + r * eg] = x;
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_b_replicate_vertices(
+    const int64_t *bvoff, const int64_t *geoff, int32_t ngraphs, int32_t nres, int32_t nb,
+    const int64_t *rowptr, const int64_t *kdeg, const int64_t *loopw, const int64_t *selfsum,
+    int64_t *urowptr, int64_t *ukdeg, int64_t *uloopw, int64_t *uselfsum, int64_t unnz)
+{
+    GRID_STRIDE(i, nb)
+    {
+        const int32_t g = graph_of(bvoff, ngraphs, i);
+        const int64_t vb = bvoff[g], ng = bvoff[g + 1] - vb;
+        const int64_t eg0 = geoff[g], eg = geoff[g + 1] - eg0;
+        const int64_t v0 = (int64_t)nres * vb + (i - vb);
+        const int64_t p0 = (int64_t)nres * eg0 + (rowptr[i] - eg0);
+        const int64_t k = kdeg[i], lp = loopw[i], ss = selfsum[i];
+        for (int32_t r = 0; r < nres; ++r) {
+            const int64_t v = v0 + r * ng;
+            urowptr[v] = p0 + r * eg;
+            ukdeg[v] = k;
+            uloopw[v] = lp;
+            uselfsum[v] = ss;
+        }
+        if (i == 0) urowptr[(int64_t)nres * nb] = unnz;
+    }
+}
+
 // cm[c] += sum of w over the union edges of component c (edge ranges ueoff).
 __global__ void k_b_edge_sum(const int64_t *w, const int64_t *ueoff, int32_t ncomp, int64_t etot,
                              ull *cm)
@@ -252,6 +326,24 @@ void launch_b_union_edges(hipStream_t s, const int32_t *bsrc, const int32_t *bds
         hipLaunchKernelGGL(k_b_union_edges, grid_for(etot, 256, 8192), 256, 0, s, bsrc, bdst, bw,
                            wtype, gscale, beoff, bn, ueoff, uvoff, ncomp, nres, etot, usrc, udst,
                            uw, f);
+}
+
+void launch_b_replicate_csr(hipStream_t s, const int64_t *bvoff, int64_t *geoff, int32_t ngraphs,
+                            int32_t nres, int32_t nb, int64_t nnz, const int64_t *rowptr,
+                            const int32_t *col, const int64_t *w, const uint32_t *w32,
+                            const int64_t *kdeg, const int64_t *loopw, const int64_t *selfsum,
+                            int64_t *urowptr, int32_t *ucol, int64_t *uw, uint32_t *uw32,
+                            int64_t *ukdeg, int64_t *uloopw, int64_t *uselfsum)
+{
+    hipLaunchKernelGGL(k_b_graph_entry_off, grid_for((int64_t)ngraphs + 1, 256), 256, 0, s, bvoff,
+                       rowptr, ngraphs, geoff);
+    if (nnz > 0)
+        hipLaunchKernelGGL(k_b_replicate_entries, grid_for(nnz, 256, 1 << 16), 256, 0, s, bvoff,
+                           geoff, ngraphs, nres, nnz, col, w, w32, ucol, uw, uw32);
+    if (nb > 0)
+        hipLaunchKernelGGL(k_b_replicate_vertices, grid_for(nb, 256, 1 << 16), 256, 0, s, bvoff,
+                           geoff, ngraphs, nres, nb, rowptr, kdeg, loopw, selfsum, urowptr, ukdeg,
+                           uloopw, uselfsum, (int64_t)nres * nnz);
 }
 
 void launch_b_edge_sum(hipStream_t s, const int64_t *w, const int64_t *ueoff, int32_t ncomp,

@@ -403,6 +403,14 @@ void launch_b_union_edges(hipStream_t s, const int32_t *bsrc, const int32_t *bds
                           int64_t *uw, DevFlags *f);
 void launch_b_edge_sum(hipStream_t s, const int64_t *w, const int64_t *ueoff, int32_t ncomp,
                        int64_t etot, ull *cm);
+// The union CSR of nres copies of the graphs' level-0 CSR (geoff: ngraphs + 1
+// scratch entries); see batch.hip.
+void launch_b_replicate_csr(hipStream_t s, const int64_t *bvoff, int64_t *geoff, int32_t ngraphs,
+                            int32_t nres, int32_t nb, int64_t nnz, const int64_t *rowptr,
+                            const int32_t *col, const int64_t *w, const uint32_t *w32,
+                            const int64_t *kdeg, const int64_t *loopw, const int64_t *selfsum,
+                            int64_t *urowptr, int32_t *ucol, int64_t *uw, uint32_t *uw32,
+                            int64_t *ukdeg, int64_t *uloopw, int64_t *uselfsum);
 void launch_b_qagg(hipStream_t s, int32_t n, int64_t *deg, const int64_t *in, int32_t *size,
                    ull *ddeg, int32_t *dsize, int size_shift, uint32_t *deg32,
                    const int32_t *vcomp, bool apply, QAcc *cq);
