@@ -221,6 +221,8 @@ def run_batched(args):
                     finals[gi] = [gs.final(j * nr + r) for r in range(nr)]
             print("[bench] batch chunk %d/%d: %d graphs, %.1f ms" % (ci + 1, len(chunks), len(ch), ms),
                   file=sys.stderr, flush=True)
+        st = gs.engine.stats()
+        arena_gb = (st.arena_capacity / 1e9, st.arena_high_water / 1e9)
     runs = count * nr
     # plain per-graph loop over the sample: one context per resolution, input resident
     engines = [Engine(gamma=g) for g in gammas]
@@ -265,6 +267,7 @@ def run_batched(args):
                    "chunk_union_edges": budget, "parallelism": "single-gpu"},
         "batched_s": batched_s,
         "batched_load_s": load_s,
+        "arena_gb": arena_gb[0], "arena_high_water_gb": arena_gb[1],
         "per_graph_loop_s_est": loop_est_runs_s,
         "per_graph_loop_s_est_by_edges": loop_est_s,
         "batch_vs_loop": loop_est_runs_s / batched_s,
@@ -619,8 +622,8 @@ def main():
     if not args.no_batch:
         b = run_batched(args)
         out["batched"] = {k: b[k] for k in (
-            "value", "unit", "config", "batched_s", "batched_load_s", "per_graph_loop_s_est",
-            "batch_vs_loop", "loop_sample", "validated")}
+            "value", "unit", "config", "batched_s", "batched_load_s", "arena_high_water_gb",
+            "per_graph_loop_s_est", "batch_vs_loop", "loop_sample", "validated")}
     if not args.no_serve:
         v = run_serving(args)
         out["serving"] = {k: v[k] for k in (

@@ -1607,6 +1607,21 @@ int set_device(lv_ctx *c)
     return LV_OK;
 }
 
+// The input is in place: record it as the state every run starts from.
+int loaded(lv_ctx *c, int32_t n, int64_t e, int scale)
+{
+    c->scale = scale;
+    c->m0 = c->h_flags->m_units;
+    c->max_units0 = c->h_flags->max_units;
+    c->n0 = n;
+    c->e0 = e;
+    c->loaded = true;
+    c->load_mark = c->arena.mark();
+    c->level = 0;
+    c->state = ST_LOADED;
+    return LV_OK;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1750,11 +1765,25 @@ int lv_load_edges(lv_ctx *c, int32_t n, const int32_t *ids, int64_t e, const int
     LV_TRY(arena_alloc(A, c->d_w, e));
     uint32_t *wraw = nullptr;
     LV_TRY(arena_alloc(A, wraw, e));
+    LV_HIP(hipMemsetAsync(c->d_flags, 0, sizeof(DevFlags), s));
     LV_TRY(h2d(c, c->d_src, src, e));
     LV_TRY(h2d(c, c->d_dst, dst, e));
     LV_TRY(h2d(c, wraw, (const uint32_t *)w, e));
-    LV_HIP(hipMemsetAsync(c->d_flags, 0, sizeof(DevFlags), s));
-    if (!iota_ids) {
+    if (iota_ids) {
+        // one pass: endpoints, weights, and (int32 weights) the units
+        launch_load_checks(s, c->d_src, c->d_dst, wraw, wtype, e, n,
+                          wtype == LV_WEIGHT_INT32 ? c->d_w : nullptr, c->d_flags);
+        if (wtype == LV_WEIGHT_INT32) {
+            arena_free(A, wraw);
+            LV_TRY(d2h(c, c->h_flags, c->d_flags, 1));
+            LV_TRY(sync(c));
+            if (c->h_flags->err) {
+                free_input(c);
+                return (c->h_flags->err & 1) ? LV_ERR_INVALID_INPUT : LV_ERR_RANGE;
+            }
+            return loaded(c, n, e, 0);
+        }
+    } else {
         LV_TRY(arena_alloc(A, c->d_ids, n));
         LV_TRY(h2d(c, c->d_ids, ids, n));
         uint32_t *k1 = nullptr, *k2 = nullptr;
@@ -1776,10 +1805,8 @@ int lv_load_edges(lv_ctx *c, int32_t n, const int32_t *ids, int64_t e, const int
         arena_free(A, k2);
         arena_free(A, v1);
         arena_free(A, v2);
-    } else {
-        launch_check_endpoints(s, c->d_src, c->d_dst, e, n, c->d_flags);
+        launch_weight_exponent(s, wraw, wtype, e, c->d_flags);
     }
-    launch_weight_exponent(s, wraw, wtype, e, c->d_flags);
     LV_HIP(hipGetLastError());
     LV_TRY(d2h(c, c->h_flags, c->d_flags, 1));
     LV_TRY(sync(c));
@@ -1798,16 +1825,7 @@ int lv_load_edges(lv_ctx *c, int32_t n, const int32_t *ids, int64_t e, const int
         free_input(c);
         return (c->h_flags->err & 1) ? LV_ERR_INVALID_INPUT : LV_ERR_RANGE;
     }
-    c->scale = scale;
-    c->m0 = c->h_flags->m_units;
-    c->max_units0 = c->h_flags->max_units;
-    c->n0 = n;
-    c->e0 = e;
-    c->loaded = true;
-    c->load_mark = c->arena.mark();
-    c->level = 0;
-    c->state = ST_LOADED;
-    return LV_OK;
+    return loaded(c, n, e, scale);
 }
 
 int lv_neighbour_compute(lv_ctx *c)
@@ -2029,7 +2047,7 @@ int lv_final_assignment(lv_ctx *c, int32_t *comm)
 int lv_final_modularity(lv_ctx *c, double *q)
 {
     LV_TRY(check_ctx(c));
-    if (!q || !c->final_valid) return LV_ERR_ARG;
+    if (!q || !c->final_valid || c->batch) return LV_ERR_ARG;  // graph sets: per component only
     LV_TRY(set_device(c));
     if (c->e0 == 0 || c->m0 == 0) {
         *q = 0.0;

@@ -299,6 +299,9 @@ void launch_weight_units(hipStream_t s, const void *w, int wtype, int64_t e, int
                          int64_t *wu, DevFlags *f);
 void launch_check_endpoints(hipStream_t s, const int32_t *src, const int32_t *dst, int64_t e,
                             int32_t n, DevFlags *f);
+// Checks of an uploaded edge list (ids = positions); int32 weights also to units.
+void launch_load_checks(hipStream_t s, const int32_t *src, const int32_t *dst, const uint32_t *w,
+                       int wtype, int64_t count, int32_t n, int64_t *wu, DevFlags *f);
 void launch_check_sorted_unique(hipStream_t s, const uint32_t *keys, int64_t n, DevFlags *f);
 void launch_map_endpoints(hipStream_t s, const uint32_t *sorted_ids, const int32_t *sorted_pos,
                           int32_t n, int32_t *ep, int64_t e, DevFlags *f);
