@@ -1108,7 +1108,7 @@ int induce(lv_ctx *c, const int32_t *newc, const int32_t *newid_g, int32_t Kn)
         LV_TRY(arena_alloc(A, cw2_32, nnz2));
     else
         LV_TRY(arena_alloc(A, cw2, nnz2));
-    launch_cr_copy(s, ca, B, K, nparts, rowptr2, col2, cw2, cw2_32);
+    launch_cr_copy(s, ca, B, K, nparts, rowptr2, nnz2, col2, cw2, cw2_32);
     LV_HIP(hipGetLastError());
     t_end(c, t0, CAT_COARSEN);
     // release the transient buffers (later launches on this stream may reuse them)

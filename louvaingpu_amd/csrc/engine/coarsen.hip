@@ -759,12 +759,12 @@ __global__ void k_cr_rowcnt(const int32_t *newid_g, const int32_t *nuniq, int32_
     }
 }
 
-// kCopySmallLanes lanes per row (small segments average about ten uniques).
-constexpr int kCopySmallLanes = 4;
+// L lanes per row, chosen by the level's average row length (R-MAT's small
+// segments average about ten uniques, the SBM sets' hundreds).
+template <int L>
 __global__ void k_cr_copy_small(CoarsenArgs a, const int64_t *rowptr2, int32_t K, int32_t *col2,
                                 int64_t *cw2, uint32_t *cw2_32)
 {
-    constexpr int L = kCopySmallLanes;
     const int lg = threadIdx.x % L;
     for (int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / L; g < K;
          g += ((int64_t)gridDim.x * blockDim.x) / L) {
@@ -829,12 +829,19 @@ void launch_cr_rowcnt(hipStream_t s, const int32_t *newid_g, const int32_t *nuni
 }
 
 void launch_cr_copy(hipStream_t s, const CoarsenArgs &a, const BigSegArgs &B, int32_t K,
-                    int64_t nparts, const int64_t *rowptr2, int32_t *col2, int64_t *cw2,
-                    uint32_t *cw2_32)
+                    int64_t nparts, const int64_t *rowptr2, int64_t nnz2, int32_t *col2,
+                    int64_t *cw2, uint32_t *cw2_32)
 {
-    if (K > 0)
-        hipLaunchKernelGGL(k_cr_copy_small, grid_for((int64_t)K * kCopySmallLanes, 256, 1 << 18), 256, 0, s, a,
-                           rowptr2, K, col2, cw2, cw2_32);
+    const int64_t avg = K > 0 ? nnz2 / K : 0;
+    if (K > 0 && avg >= 48)
+        hipLaunchKernelGGL(k_cr_copy_small<64>, grid_for((int64_t)K * 64, 256, 1 << 18), 256, 0, s,
+                           a, rowptr2, K, col2, cw2, cw2_32);
+    else if (K > 0 && avg >= 12)
+        hipLaunchKernelGGL(k_cr_copy_small<16>, grid_for((int64_t)K * 16, 256, 1 << 18), 256, 0, s,
+                           a, rowptr2, K, col2, cw2, cw2_32);
+    else if (K > 0)
+        hipLaunchKernelGGL(k_cr_copy_small<4>, grid_for((int64_t)K * 4, 256, 1 << 18), 256, 0, s,
+                           a, rowptr2, K, col2, cw2, cw2_32);
     if (nparts > 0)
         hipLaunchKernelGGL(k_cr_copy_big, grid_for(nparts * 64, 256, 1 << 18), 256, 0, s, a, B,
                            rowptr2, nparts, col2, cw2, cw2_32);

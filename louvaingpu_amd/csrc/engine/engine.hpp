@@ -455,9 +455,10 @@ int run_cr_big(Arena &A, hipStream_t s, const CoarsenArgs &a, const BigSegArgs &
                int64_t nparts, int64_t nchunks);
 void launch_cr_rowcnt(hipStream_t s, const int32_t *newid_g, const int32_t *nuniq, int32_t K,
                       int64_t *rowcnt);
+// Row copies into the next level's CSR (nnz2 entries over K groups).
 void launch_cr_copy(hipStream_t s, const CoarsenArgs &a, const BigSegArgs &B, int32_t K,
-                    int64_t nparts, const int64_t *rowptr2, int32_t *col2, int64_t *cw2,
-                    uint32_t *cw2_32);
+                    int64_t nparts, const int64_t *rowptr2, int64_t nnz2, int32_t *col2,
+                    int64_t *cw2, uint32_t *cw2_32);
 void launch_cr_vertex(hipStream_t s, const int32_t *newid_g, const ull *ssum, const ull *internal,
                       const ull *selfl, int32_t K, int64_t *loop2, int64_t *self2, int64_t *kdeg2,
                       long long *msum);
