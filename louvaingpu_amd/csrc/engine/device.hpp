@@ -59,6 +59,38 @@ __device__ __forceinline__ int64_t weight_units(uint32_t x, int wtype, int scale
     return u;
 }
 
+// Butterfly exchange: lane l receives v from lane l ^ OFF (OFF < 64; within an
+// aligned group of G > OFF lanes this stays inside the group).  Offsets 1 and 2
+// are DPP quad permutations (no LDS hardware), 4-16 ds_swizzle in bitmask mode
+// (and 0x1f, xor OFF within each 32-lane half), 32 a ds_bpermute across the
+// halves.  All lanes of the wave must be active.
+template <int OFF>
+__device__ __forceinline__ int xor_lanes32(int v)
+{
+    if constexpr (OFF == 1)
+        return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, true);  // quad_perm [1,0,3,2]
+    else if constexpr (OFF == 2)
+        return __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, true);  // quad_perm [2,3,0,1]
+    else if constexpr (OFF < 32)
+        return __builtin_amdgcn_ds_swizzle(v, 0x1F | (OFF << 10));
+    else
+        return __shfl_xor(v, OFF, 64);
+}
+
+template <int OFF, typename T>
+__device__ __forceinline__ T xor_lanes(T v)
+{
+    static_assert(sizeof(T) == 4 || sizeof(T) == 8, "32- or 64-bit values");
+    if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(T, xor_lanes32<OFF>(__builtin_bit_cast(int, v)));
+    } else {
+        const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+        const int lo = xor_lanes32<OFF>((int)(unsigned)u);
+        const int hi = xor_lanes32<OFF>((int)(unsigned)(u >> 32));
+        return __builtin_bit_cast(T, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+    }
+}
+
 __device__ __forceinline__ long long wave_sum(long long v)
 {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);

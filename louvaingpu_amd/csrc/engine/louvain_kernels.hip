@@ -1207,21 +1207,33 @@ __device__ __forceinline__ void consider_slots(const SweepArgs &a, const int32_t
     }
 }
 
+// One butterfly step of the argmax (largest gain, lowest community id) and
+// the candidate count.
+template <int OFF>
+__device__ __forceinline__ void reduce_step(int &cnt, double &bg, int32_t &bc, int64_t &bw)
+{
+    cnt += xor_lanes<OFF>(cnt);
+    const double og = xor_lanes<OFF>(bg);
+    const int32_t oc = xor_lanes<OFF>(bc);
+    const int64_t ow = xor_lanes<OFF>(bw);
+    if (better(og, oc, bg, bc)) {
+        bg = og;
+        bc = oc;
+        bw = ow;
+    }
+}
+
+// Every lane of each aligned group of W lanes ends with the group's count and
+// best candidate (DPP / ds_swizzle / ds_bpermute butterflies, see xor_lanes).
 template <int W>
 __device__ __forceinline__ void group_reduce(int &cnt, double &bg, int32_t &bc, int64_t &bw)
 {
-#pragma unroll
-    for (int off = W / 2; off > 0; off >>= 1) {
-        cnt += __shfl_xor(cnt, off, W);
-        double og = __shfl_xor(bg, off, W);
-        int32_t oc = __shfl_xor(bc, off, W);
-        long long ow = __shfl_xor((long long)bw, off, W);
-        if (better(og, oc, bg, bc)) {
-            bg = og;
-            bc = oc;
-            bw = ow;
-        }
-    }
+    if constexpr (W > 32) reduce_step<32>(cnt, bg, bc, bw);
+    if constexpr (W > 16) reduce_step<16>(cnt, bg, bc, bw);
+    if constexpr (W > 8) reduce_step<8>(cnt, bg, bc, bw);
+    if constexpr (W > 4) reduce_step<4>(cnt, bg, bc, bw);
+    if constexpr (W > 2) reduce_step<2>(cnt, bg, bc, bw);
+    if constexpr (W > 1) reduce_step<1>(cnt, bg, bc, bw);
 }
 
 // The block's candidate count and best candidate from its waves' results
