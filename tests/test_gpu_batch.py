@@ -159,3 +159,30 @@ def test_reloading_one_context_with_growing_sets():
                         assert np.array_equal(gs.final(gi * 2 + ri), eng.final_assignment()), (k, gi)
                     finally:
                         eng.close()
+
+
+def test_batch_workload_sample_full_size():
+    """Members of the batching workload itself (4,096 SBMs with n log-uniform in
+    [100, 50000], seed 3) at full size and all 16 resolutions: every 256th
+    graph, each component equal to its single-graph run, the largest two also
+    to the oracle at three resolutions."""
+    params = generators.batch_params(4096, seed=3)[::256]
+    graphs = [generators.sbm(n, 100, p, p / 50, sd) for n, p, sd in params]
+    gammas = generators.resolutions(16)
+    gpu_case("batch workload sample: %d graphs (n %d..%d) x %d gammas"
+             % (len(graphs), min(g[0] for g in graphs), max(g[0] for g in graphs), len(gammas)))
+    with GraphSet() as gs:
+        gs.load(graphs, gammas)
+        gs.run()
+        for gi, g in enumerate(graphs):
+            for ri, gamma in enumerate(gammas):
+                eng = _single(g, gamma)
+                try:
+                    assert_component_matches(gs, gi * len(gammas) + ri, eng)
+                finally:
+                    eng.close()
+        big = sorted(range(len(graphs)), key=lambda i: -graphs[i][0])[:2]
+        for gi in big:
+            for ri in (0, 7, 15):
+                r = oracle.run(*graphs[gi], gamma=gammas[ri], threads=8)
+                assert np.array_equal(gs.final(gi * len(gammas) + ri), r.final_comm()), (gi, ri)

@@ -113,3 +113,30 @@ def test_bad_requests_get_their_own_status():
             mixed += res.batch_size > 1
             assert np.array_equal(comm, _alone(good[i], 1.0)[0]), i
         assert mixed > 0  # some valid requests ran in a set with a bad one
+
+
+def test_poisson_stream_of_the_serving_workload():
+    """Requests drawn as in the serving workload (SBMs with n log-uniform in
+    [100, 50000], seed 4, plus R-MAT scale-20 float graphs) arriving as a
+    Poisson stream at 200 requests/s on two workers: every result equals the
+    request run alone."""
+    rng = np.random.Generator(np.random.PCG64(4))
+    nreq = 80
+    gaps = rng.exponential(1.0 / 200.0, nreq)
+    params = generators.batch_params(nreq, seed=4)
+    rmat = [generators.rmat(20, 16, 0.57, 0.19, 0.19, sd, "float") for sd in (4, 5)]
+    reqs = [generators.sbm(n, 100, p, p / 50, sd) for n, p, sd in params]
+    reqs[11], reqs[47] = rmat  # two R-MAT 20 requests inside the stream
+    gpu_case("serve poisson stream: %d requests, 2 R-MAT 20" % nreq)
+    with Server(workers=2, budget_bytes=8 << 30) as srv:
+        ids = []
+        for i, g in enumerate(reqs):
+            time.sleep(gaps[i])
+            ids.append(srv.submit(*g))
+        results = [srv.wait(r) for r in ids]
+    sizes = []
+    for g, (comm, res) in zip(reqs, results):
+        assert res.status == 0
+        assert np.array_equal(comm, _alone(g, 1.0)[0])
+        sizes.append(res.batch_size)
+    assert max(sizes) > 1  # requests did share sets
