@@ -14,10 +14,12 @@ The same run with the input already resident in HBM is reported beside it
 
 Besides the JSON line's required fields it reports:
   roofline     the dominant local-moving sweep tier (one kernel, or the four
-               kernels of the hub pipeline): algorithmic bytes 12*nnz + 24*N per
-               launch (community-total reads P not counted) / its mean HIP-event
-               duration, against 8 TB/s; traffic = measured HBM bytes per launch
-               from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
+               kernels of the hub pipeline): algorithmic bytes 12*nnz + 24*N + 8*P
+               per launch (P = the distinct (vertex, community) candidates, whose
+               community totals the sweep reads; counted by the kernels) / its
+               mean HIP-event duration, against 8 TB/s (frac_p0: the same without
+               the 8*P term); traffic = measured HBM bytes per launch from the
+               committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
   cpu_baseline the oracle (C restatement of the reference semantics, OpenMP)
                on the same graph with every thread this process may use, timed
                on this host; its partition is also the validation of the GPU
@@ -431,18 +433,21 @@ def measure_graph(wname, graph, steps, warmup, traffic_key):
     hsrc[:], hdst[:], hw[:] = src, dst, w
     with Engine() as eng:
         e2e_s, e2e_parts, levels = e2e_runs(eng, n, hsrc, hdst, hw, steps, warmup, wname)
+        # the same runs with the input already resident in HBM (the last load's)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            eng.run()
+        dev_ms = (time.perf_counter() - t0) * 1000.0 / max(steps, 1)
     del hsrc, hdst, hw
-    # the same runs with the input resident in HBM, HIP events per phase and tier
+    # phases and the roofline: HIP events around every phase and tier launch
     eng = Engine(timing=True)
     try:
         eng.load_edges(n, src, dst, w)
         for i in range(warmup):
             eng.run()
         eng.reset_stats()
-        t0 = time.perf_counter()
         for _ in range(steps):
             eng.run()
-        dev_ms = (time.perf_counter() - t0) * 1000.0 / max(steps, 1)
         st = eng.stats()
         infos = [eng.level_info(l) for l in range(eng.num_levels())]
         final = eng.final_assignment()
@@ -455,7 +460,10 @@ def measure_graph(wname, graph, steps, warmup, traffic_key):
     launches = max(1, st.tier_launches[dom])
     avg_ms = st.tier_ms[dom] / launches
     bytes_per_launch = st.tier_bytes[dom] / launches
+    cand_per_launch = st.tier_cand[dom] / launches
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    achieved_p0 = ((bytes_per_launch - 8.0 * cand_per_launch) / (avg_ms * 1e-3) / 1e9
+                   if avg_ms > 0 else 0.0)
     k = max(steps, 1)
     phases = {"csr": st.csr_ms / k, "init": st.init_ms / k, "sweep": st.sweep_ms / k,
               "apply": st.apply_ms / k, "renumber": st.renumber_ms / k,
@@ -476,6 +484,8 @@ def measure_graph(wname, graph, steps, warmup, traffic_key):
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": measured_traffic(traffic_key, dom + 1),
             "bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_ms,
+            "candidates_per_launch": cand_per_launch,
+            "frac_p0": achieved_p0 / HBM_PEAK_GBS,
         },
         "phase_ms_per_step": phases,
         "coarsen_fraction": phases["coarsen"] / dev_ms if dev_ms > 0 else None,

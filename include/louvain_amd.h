@@ -95,7 +95,10 @@ typedef struct {
     double csr_ms, init_ms, sweep_ms, apply_ms, renumber_ms, coarsen_ms, flatten_ms;
     double tier_ms[LV_SWEEP_TIERS];
     int64_t tier_launches[LV_SWEEP_TIERS];
-    double tier_bytes[LV_SWEEP_TIERS];     /* algorithmic bytes 12*nnz+24*N+8*P */
+    double tier_bytes[LV_SWEEP_TIERS];     /* algorithmic bytes 12*nnz+24*N+8*P, with P the
+                                              distinct (vertex, community) candidates
+                                              the sweep read a community total for */
+    double tier_cand[LV_SWEEP_TIERS];      /* P per tier (summed over launches) */
     double sweep_bytes;                    /* sum over tiers */
     int64_t sweep_entries;                 /* sum of nnz over passes */
     int32_t levels, passes;

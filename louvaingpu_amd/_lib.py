@@ -122,6 +122,7 @@ class LvStats(ct.Structure):
         ("tier_ms", ct.c_double * LV_SWEEP_TIERS),
         ("tier_launches", ct.c_int64 * LV_SWEEP_TIERS),
         ("tier_bytes", ct.c_double * LV_SWEEP_TIERS),
+        ("tier_cand", ct.c_double * LV_SWEEP_TIERS),
         ("sweep_bytes", ct.c_double),
         ("sweep_entries", ct.c_int64),
         ("levels", ct.c_int32),

@@ -84,6 +84,7 @@ struct lv_ctx {
     ull *ddeg = nullptr;
     int size_shift = 0;  // see SweepArgs
     int32_t *verts = nullptr;
+    int32_t *pcnt = nullptr;  // timing: distinct candidates of each vertex in the last pass
     int64_t tier_cnt[kTiers + 1] = {0}, tier_nnz[kTiers + 1] = {0}, tier_off[kTiers + 1] = {0};
     // every row of the level lists each neighbour once (coarsened levels always;
     // level 0 when the CSR rows are strictly increasing): the first pass of the
@@ -230,6 +231,37 @@ int sync(lv_ctx *c)
     return LV_OK;
 }
 
+// Timing: the candidate counts P of a pass's tiers (the 8*P term of the
+// local-moving bytes) are summed per tier into d_small[kPSlot + t] and fetched
+// with the pass's other results.
+constexpr int kPSlot = 40;
+
+void p_begin(lv_ctx *c)
+{
+    if (timing(c)) fill_i64(c->stream, (int64_t *)c->d_small + kPSlot, kTiers + 1, 0);
+}
+
+void p_tier(lv_ctx *c, int t, int64_t nv)
+{
+    if (timing(c) && c->pcnt)
+        sum_gather_i32(c->stream, c->verts + c->tier_off[t], nv, c->pcnt,
+                       (int64_t *)c->d_small + kPSlot + t);
+}
+
+int p_fetch(lv_ctx *c);  // below d2h
+
+// After the pass's sync: P into the tier byte counts.
+void p_resolve(lv_ctx *c)
+{
+    if (!timing(c)) return;
+    for (int t = 1; t <= kTiers; ++t) {
+        const double p = (double)c->h_small[kPSlot + t];
+        c->stats.tier_cand[t - 1] += p;
+        c->stats.tier_bytes[t - 1] += 8.0 * p;
+        c->stats.sweep_bytes += 8.0 * p;
+    }
+}
+
 template <typename T>
 int d2h(lv_ctx *c, T *host, const T *dev, int64_t count)
 {
@@ -244,6 +276,12 @@ int h2d(lv_ctx *c, T *dev, const T *host, int64_t count)
     if (count <= 0) return LV_OK;
     LV_HIP(hipMemcpyAsync(dev, host, sizeof(T) * (size_t)count, hipMemcpyHostToDevice, c->stream));
     return LV_OK;
+}
+
+int p_fetch(lv_ctx *c)
+{
+    if (!timing(c)) return LV_OK;
+    return d2h(c, c->h_small + kPSlot, c->d_small + kPSlot, kTiers + 1);
 }
 
 int zero_small(lv_ctx *c, int slots)
@@ -267,6 +305,7 @@ void free_sweep_state(lv_ctx *c)
     arena_free(a, c->dsize);
     arena_free(a, c->ddeg);
     arena_free(a, c->verts);
+    arena_free(a, c->pcnt);
     HubArgs &h = c->hub;
     a.release((void *)h.hq);
     a.release((void *)h.hP);
@@ -717,6 +756,7 @@ int init_level(lv_ctx *c)
     LV_TRY(arena_alloc(A, c->C, n));
     LV_TRY(arena_alloc(A, c->next, n));
     LV_TRY(arena_alloc(A, c->deg32, n));
+    if (timing(c)) LV_TRY(arena_alloc(A, c->pcnt, n));
     uint32_t *tkey = nullptr, *tkey2 = nullptr;
     int32_t *v1 = nullptr, *v2 = nullptr;
     LV_TRY(arena_alloc(A, tkey, n));
@@ -796,6 +836,7 @@ int one_level(lv_ctx *c, double lower, double *q_out, int32_t *passes_out)
     a.size_shift = c->size_shift;
     a.m2 = 2.0 * (double)c->m;
     a.gamma = c->cfg.gamma;
+    a.pcnt = c->pcnt;
     double curQ = -1.0;  // LouvainCUDA/OneLevelKernel.cu:528
     int32_t pass = 0;
     double q_level = 0.0;
@@ -815,6 +856,7 @@ int one_level(lv_ctx *c, double lower, double *q_out, int32_t *passes_out)
             launch_lab(s, c->n, c->C, c->size, c->deg, c->lab, c->narrow ? c->lab8 : nullptr);
             t_end(c, tl, CAT_APPLY);
         }
+        p_begin(c);
         for (int t = 1; t <= kTiers; ++t) {
             const int64_t nv = c->tier_cnt[t];
             if (nv == 0) continue;
@@ -830,7 +872,9 @@ int one_level(lv_ctx *c, double lower, double *q_out, int32_t *passes_out)
                                      c->hub_chunks, false));
             t_end(c, t0, CAT_SWEEP, t, 12.0 * (double)c->tier_nnz[t] + 24.0 * (double)nv,
                   c->tier_nnz[t]);
+            p_tier(c, t, nv);
         }
+        LV_TRY(p_fetch(c));
         if (!last) {
             hipEvent_t td = t_begin(c);
             launch_move_deltas(s, a, c->n, pass == 1, false);  // pass 1: C[v] = v
@@ -850,6 +894,7 @@ int one_level(lv_ctx *c, double lower, double *q_out, int32_t *passes_out)
         LV_HIP(hipGetLastError());
         LV_TRY(d2h(c, c->h_qacc, c->d_qacc, 1));
         LV_TRY(sync(c));
+        p_resolve(c);
         const double q_new = q_from_aggregates(c->h_qacc->sum_in, c->h_qacc->d2_lo,
                                                c->h_qacc->d2_hi, c->m, c->cfg.gamma);
         c->cur_pass_q.push_back(q_prev);
@@ -1330,6 +1375,7 @@ int batch_one_level(lv_ctx *c, const std::vector<double> &lower, const std::vect
     a.cmode = c->d_cmode;
     a.cm2 = c->d_cm2;
     a.cgam = c->d_cgam;
+    a.pcnt = c->pcnt;
     int32_t pass = 0;
     for (;;) {
         bool any = false;
@@ -1355,6 +1401,7 @@ int batch_one_level(lv_ctx *c, const std::vector<double> &lower, const std::vect
             launch_lab(s, c->n, c->C, c->size, c->deg, c->lab, c->narrow ? c->lab8 : nullptr);
             t_end(c, tl, CAT_APPLY);
         }
+        p_begin(c);
         for (int t = 1; t <= kTiers; ++t) {
             const int64_t nv = c->tier_cnt[t];
             if (nv == 0) continue;
@@ -1370,7 +1417,9 @@ int batch_one_level(lv_ctx *c, const std::vector<double> &lower, const std::vect
                                      c->hub_chunks, true));
             t_end(c, t0, CAT_SWEEP, t, 12.0 * (double)c->tier_nnz[t] + 24.0 * (double)nv,
                   c->tier_nnz[t]);
+            p_tier(c, t, nv);
         }
+        LV_TRY(p_fetch(c));
         {
             hipEvent_t td = t_begin(c);
             launch_move_deltas(s, a, c->n, pass == 1, true);  // pass 1: C[v] = v
@@ -1380,6 +1429,7 @@ int batch_one_level(lv_ctx *c, const std::vector<double> &lower, const std::vect
         hipEvent_t t1 = t_begin(c);
         LV_TRY(batch_qagg(c, true));  // synchronises: mode[] may be rewritten below
         t_end(c, t1, CAT_APPLY);
+        p_resolve(c);
         std::swap(c->C, c->next);
         for (int32_t k = 0; k < nc; ++k) {
             if (mode[k] == 0) continue;

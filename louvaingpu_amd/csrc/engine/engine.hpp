@@ -157,6 +157,7 @@ struct SweepArgs {
     double m2;     // 2.0 * (double)m
     double gamma;
     int last;      // last pass: fuse merge-isolated, drop move deltas
+    int32_t *pcnt; // timing only: each vertex's count of distinct candidate communities
     // Graph sets (batched kernels only): component of each vertex, and per
     // component its mode (0 frozen: next = C; 1 sweep; 2 last pass), 2m and
     // resolution.
@@ -276,6 +277,8 @@ void fill_i64(hipStream_t s, int64_t *p, int64_t n, int64_t v);
 void iota_i32(hipStream_t s, int32_t *p, int64_t n);
 void copy_i32(hipStream_t s, int32_t *dst, const int32_t *src, int64_t n);
 void sum_i64(hipStream_t s, const int64_t *p, int64_t n, int64_t *out_dev);  // *out += sum
+// *out += sum of p[idx[i]], i < n
+void sum_gather_i32(hipStream_t s, const int32_t *idx, int64_t n, const int32_t *p, int64_t *out_dev);
 
 int bits_for(uint64_t maxval);  // bits needed to hold values 0..maxval
 

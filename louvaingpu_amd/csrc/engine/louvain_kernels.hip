@@ -1281,6 +1281,7 @@ __device__ __forceinline__ bool block_best(void *scratch, int &cnt, double &bg, 
 __device__ __forceinline__ void finalize_vertex(const SweepArgs &a, int32_t v, int32_t own,
                                                 int32_t so, int n, int32_t bc, int64_t bw, int mode)
 {
+    if (a.pcnt) a.pcnt[v] = n;
     if (mode == 0) {  // frozen component: keeps its communities
         a.next[v] = own;
         return;

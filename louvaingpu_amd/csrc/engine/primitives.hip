@@ -187,6 +187,24 @@ __global__ void k_sum_i64(const int64_t *p, int64_t n, long long *out)
     if ((threadIdx.x & 63) == 0 && acc != 0) atomicAdd((ull *)out, (ull)acc);
 }
 
+// *out += sum of p[idx[i]] over i < n (one atomic per wave).
+__global__ void k_sum_gather_i32(const int32_t *idx, int64_t n, const int32_t *p, long long *out)
+{
+    long long acc = 0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        acc += p[idx[i]];
+    acc = wave_sum_ll(acc);
+    if ((threadIdx.x & 63) == 0 && acc != 0) atomicAdd((ull *)out, (ull)acc);
+}
+
+void sum_gather_i32(hipStream_t s, const int32_t *idx, int64_t n, const int32_t *p, int64_t *out_dev)
+{
+    if (n > 0)
+        hipLaunchKernelGGL(k_sum_gather_i32, grid_for(n, 256, 1024), 256, 0, s, idx, n, p,
+                           (long long *)out_dev);
+}
+
 void fill_i32(hipStream_t s, int32_t *p, int64_t n, int32_t v)
 {
     if (n > 0) hipLaunchKernelGGL(k_fill<int32_t>, grid_for(n, 256), 256, 0, s, p, n, v);

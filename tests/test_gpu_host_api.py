@@ -78,3 +78,34 @@ def test_undersized_arena_reports_nomem_and_recovers():
         eng.run()
         r = oracle.run(*small)
         assert np.array_equal(eng.final_assignment(), r.final_comm())
+
+
+def test_timing_stats_count_candidates():
+    """With LV_FLAG_TIMING the per-tier stats carry the candidates P of the
+    local-moving byte count: on the first pass of a level whose rows list each
+    neighbour once, every row entry is a candidate (plus the own community of
+    a vertex with a self-loop), and no pass can have more candidates than row
+    entries plus vertices."""
+    gpu_case("timing stats with candidate counts")
+    n, src, dst, w = generators.rmat(14, 16, 0.57, 0.19, 0.19, 3, "int")
+    with Engine(timing=True, max_levels=1, max_passes=1) as eng:
+        eng.load_edges(n, src, dst, w)
+        eng.run()
+        st = eng.stats()
+        info = eng.level_info(0)
+        cand = sum(st.tier_cand[t] for t in range(10))
+        nnz = info.nnz
+        ns = src != dst
+        deg = np.bincount(src[ns], minlength=n) + np.bincount(dst[ns], minlength=n)
+        has_loop = np.bincount(src[~ns], minlength=n) > 0
+        loops = int(np.count_nonzero(has_loop & (deg > 0)))  # rows without neighbours never sweep
+        assert cand == nnz + loops  # one pass, rows unique: every entry, plus own for loops
+        for t in range(10):
+            if st.tier_launches[t]:
+                assert st.tier_bytes[t] >= 8 * st.tier_cand[t]
+    with Engine(timing=True) as eng:
+        eng.load_edges(n, src, dst, w)
+        eng.run()
+        st = eng.stats()
+        bound = sum(eng.level_info(l).nnz + eng.level_info(l).n for l in range(eng.num_levels()))
+        assert 0 < sum(st.tier_cand[t] for t in range(10)) <= 2 * bound
