@@ -385,8 +385,10 @@ def workload(name):
 
 def e2e_runs(eng, n, hsrc, hdst, hw, steps, warmup, label):
     """End-to-end steps (pinned edge list -> device -> run -> every level's
-    Partition and the final partition in pinned host buffers).  Returns
-    (seconds per step, per-part ms, levels)."""
+    Partition and the final partition in pinned host buffers; the Partitions
+    are copied while later levels run, lv_run_partitions).  Returns (seconds
+    per step, per-part ms, levels); 'run' includes the overlapped copies,
+    'results' what is left after the run."""
     from louvaingpu_amd.engine import pinned_empty
     outs, fin = None, None
     parts = np.zeros(3)
@@ -396,11 +398,15 @@ def e2e_runs(eng, n, hsrc, hdst, hw, steps, warmup, label):
         t0 = time.perf_counter()
         eng.load_edges(n, hsrc, hdst, hw)
         t1 = time.perf_counter()
-        levels = eng.run()
-        t2 = time.perf_counter()
-        for l in range(levels):
-            eng.level_partition(l, out=None if outs is None else outs[l])
-        eng.final_assignment(out=fin)
+        if outs is None:  # the first run finds the level count
+            levels = eng.run()
+            t2 = time.perf_counter()
+            for l in range(levels):
+                eng.level_partition(l)
+            eng.final_assignment()
+        else:
+            levels = eng.run_partitions([o[0] for o in outs], [o[1] for o in outs], fin)
+            t2 = time.perf_counter()
         t3 = time.perf_counter()
         if i >= warmup:
             parts += (t1 - t0, t2 - t1, t3 - t2)
@@ -409,9 +415,8 @@ def e2e_runs(eng, n, hsrc, hdst, hw, steps, warmup, label):
             print("[bench] %s e2e warmup %d: %.2f ms (load %.2f, run %.2f, results %.2f)"
                   % (label, i, (t3 - t0) * 1e3, (t1 - t0) * 1e3, (t2 - t1) * 1e3,
                      (t3 - t2) * 1e3), file=sys.stderr, flush=True)
-        if outs is None:  # page-locked result buffers, sized by the first run
-            outs = [(pinned_empty(eng.level_info(l).n, np.int32),
-                     pinned_empty(eng.level_info(l).n, np.int32)) for l in range(levels)]
+        if outs is None:  # page-locked result buffers for the level count of the first run
+            outs = [(pinned_empty(n, np.int32), pinned_empty(n, np.int32)) for l in range(levels)]
             fin = pinned_empty(n, np.int32)
     k = max(steps, 1)
     return total / k, {"load_ms": parts[0] * 1e3 / k, "run_ms": parts[1] * 1e3 / k,

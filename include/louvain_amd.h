@@ -157,6 +157,16 @@ int lv_induce(lv_ctx *ctx);
  * entirely on the device, with the same termination rule; the dendrogram
  * stays in HBM.  *levels receives its length. */
 int lv_run(lv_ctx *ctx, int32_t *levels);
+/* lv_run that also delivers the dendrogram to the host, as generate_dendogram
+ * collects one Partition per level (renumberCUDA's copies,
+ * LouvainCUDA/RenumberKernel.cu:202-203): each level's Partition (the layout of
+ * lv_level_partition) is copied into node[l]/comm[l] (l < max_levels, each of
+ * at least n entries of the input) on a second stream while the next level
+ * runs, and the final partition into final_comm (n entries).  Page-locked
+ * buffers let the copies overlap the run.  Levels beyond max_levels stay on the
+ * device for lv_level_partition. */
+int lv_run_partitions(lv_ctx *ctx, int32_t max_levels, int32_t *const *node, int32_t *const *comm,
+                      int32_t *final_comm, int32_t *levels);
 
 /* Results. */
 int lv_num_levels(lv_ctx *ctx, int32_t *levels);

@@ -43,7 +43,7 @@ ENGINE_SYMBOLS = (
     "lv_get_stats", "lv_reset_stats", "lv_last_run_ms",
     "lv_batch_load", "lv_batch_load_graphs", "lv_batch_run", "lv_batch_num_components", "lv_batch_num_levels",
     "lv_batch_level_info", "lv_batch_level_assignment", "lv_batch_final_assignment",
-    "lv_run_static",
+    "lv_run_static", "lv_run_partitions",
 )
 HOST_SYMBOLS = (
     "lvh_generate_dendrogram", "lvh_num_levels", "lvh_level_count", "lvh_level_partition",
@@ -174,6 +174,8 @@ def engine():
         "lv_renumber": (ct.c_int, [ctx, _I32P]),
         "lv_induce": (ct.c_int, [ctx]),
         "lv_run": (ct.c_int, [ctx, _I32P]),
+        "lv_run_partitions": (ct.c_int, [ctx, ct.c_int32, ct.POINTER(ct.c_void_p),
+                                         ct.POINTER(ct.c_void_p), _I32P, _I32P]),
         "lv_num_levels": (ct.c_int, [ctx, _I32P]),
         "lv_level_info_get": (ct.c_int, [ctx, ct.c_int32, ct.POINTER(LvLevelInfo)]),
         "lv_level_partition": (ct.c_int, [ctx, ct.c_int32, _I32P, _I32P]),

@@ -35,6 +35,16 @@ struct LevelRec {
     std::vector<std::vector<int32_t>> trace;
 };
 
+// lv_run_partitions: caller host buffers that receive each level's Partition
+// while the run goes on (copies on a second stream), and the device buffers the
+// copies read from (kept until the run ends).
+struct PartSink {
+    int32_t max = 0;
+    int32_t *const *node = nullptr;
+    int32_t *const *comm = nullptr;
+    std::vector<void *> temps;
+};
+
 struct PendingTime {
     int cat, tier;
     hipEvent_t a, b;
@@ -148,6 +158,10 @@ struct lv_ctx {
     double last_run_ms = 0;
     // arena allocations of the loaded input (a run starts from exactly these)
     std::map<size_t, size_t> load_mark;
+    // lv_run_partitions
+    PartSink *sink = nullptr;
+    hipStream_t copy_stream = nullptr;
+    std::vector<hipEvent_t> sink_ev;
     // lv_run_static: the device scalars and the arena live in the static
     // scratch array, the stream is the caller's, and the host copies of the
     // scalars are these ordinary (pageable) members
@@ -1249,6 +1263,35 @@ int flatten_final(lv_ctx *c)
     return LV_OK;
 }
 
+// lv_run_partitions: the newest level's Partition (reference layout) goes to the
+// caller's host buffers on the copy stream while the main stream runs on.
+int sink_level(lv_ctx *c)
+{
+    PartSink *k = c->sink;
+    const int32_t l = (int32_t)c->levels.size() - 1;
+    if (!k || l < 0 || l >= k->max) return LV_OK;
+    const LevelRec &L = c->levels[l];
+    if (L.n == 0) return LV_OK;
+    int32_t *dn = nullptr, *dc = nullptr;
+    LV_TRY(arena_alloc(c->arena, dn, L.n));
+    k->temps.push_back(dn);
+    LV_TRY(arena_alloc(c->arena, dc, L.n));
+    k->temps.push_back(dc);
+    launch_gather_partition(c->stream, L.perm, l == 0 ? c->d_ids : nullptr, L.newc, L.n, dn, dc);
+    LV_HIP(hipGetLastError());
+    while ((int32_t)c->sink_ev.size() <= l) {
+        hipEvent_t e = nullptr;
+        LV_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        c->sink_ev.push_back(e);
+    }
+    LV_HIP(hipEventRecord(c->sink_ev[l], c->stream));
+    LV_HIP(hipStreamWaitEvent(c->copy_stream, c->sink_ev[l], 0));
+    const size_t b = sizeof(int32_t) * (size_t)L.n;
+    LV_HIP(hipMemcpyAsync(k->node[l], dn, b, hipMemcpyDeviceToHost, c->copy_stream));
+    LV_HIP(hipMemcpyAsync(k->comm[l], dc, b, hipMemcpyDeviceToHost, c->copy_stream));
+    return LV_OK;
+}
+
 // ---------------------------------------------------------------------------
 // graph sets: the level loop of every component, run on the union graph
 
@@ -1762,6 +1805,8 @@ int lv_destroy(lv_ctx *c)
         (void)hipEventDestroy(p.b);
     }
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
+    for (auto e : c->sink_ev) (void)hipEventDestroy(e);
+    if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
     if (c->d_flags) (void)hipFree(c->d_flags);
     if (c->d_qacc) (void)hipFree(c->d_qacc);
     if (c->d_qpart) (void)hipFree(c->d_qpart);
@@ -1963,6 +2008,7 @@ int lv_run(lv_ctx *c, int32_t *levels_out)
         iota_i32(c->stream, L.perm, c->n0);
         iota_i32(c->stream, L.newc, c->n0);
         c->levels.push_back(std::move(L));
+        LV_TRY(sink_level(c));
         LV_TRY(flatten_final(c));
         LV_TRY(sync(c));
         c->state = ST_RENUMBERED;
@@ -1982,6 +2028,7 @@ int lv_run(lv_ctx *c, int32_t *levels_out)
     double q = 0;
     LV_TRY(one_level(c, mod, &q, nullptr));
     LV_TRY(renumber(c, nullptr));
+    LV_TRY(sink_level(c));
     c->state = ST_RENUMBERED;
     mod = q;
     // later levels, :420-507
@@ -1993,6 +2040,7 @@ int lv_run(lv_ctx *c, int32_t *levels_out)
         c->state = ST_INIT;
         LV_TRY(one_level(c, mod, &q, nullptr));
         LV_TRY(renumber(c, nullptr));
+        LV_TRY(sink_level(c));
         c->state = ST_RENUMBERED;
         if (q - mod < c->cfg.min_gain) break;
         mod = q;
@@ -2005,6 +2053,39 @@ int lv_run(lv_ctx *c, int32_t *levels_out)
     c->stats.levels += (int32_t)c->levels.size();
     if (levels_out) *levels_out = (int32_t)c->levels.size();
     c->last_run_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    return LV_OK;
+}
+
+int lv_run_partitions(lv_ctx *c, int32_t max_levels, int32_t *const *node, int32_t *const *comm,
+                      int32_t *final_comm, int32_t *levels_out)
+{
+    LV_TRY(check_ctx(c));
+    if (max_levels < 0 || (max_levels > 0 && (!node || !comm)) || !final_comm) return LV_ERR_ARG;
+    for (int32_t l = 0; l < max_levels; ++l)
+        if (!node[l] || !comm[l]) return LV_ERR_ARG;
+    if (!c->loaded || c->batch) return LV_ERR_ARG;
+    LV_TRY(set_device(c));
+    if (!c->copy_stream)
+        LV_HIP(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
+    PartSink sink;
+    sink.max = max_levels;
+    sink.node = node;
+    sink.comm = comm;
+    c->sink = &sink;
+    int32_t nl = 0;
+    int rc = lv_run(c, &nl);
+    c->sink = nullptr;
+    if (rc == LV_OK && c->n0 > 0) {
+        // the run ended with a sync: the final partition is complete on the device
+        const hipError_t e = hipMemcpyAsync(final_comm, c->d_final, sizeof(int32_t) * (size_t)c->n0,
+                                            hipMemcpyDeviceToHost, c->copy_stream);
+        if (e != hipSuccess) rc = hip_fail(e, "hipMemcpyAsync", __FILE__, __LINE__);
+    }
+    const hipError_t se = hipStreamSynchronize(c->copy_stream);
+    for (void *p : sink.temps) c->arena.release(p);
+    if (rc != LV_OK) return rc;
+    if (se != hipSuccess) return hip_fail(se, "hipStreamSynchronize", __FILE__, __LINE__);
+    if (levels_out) *levels_out = nl;
     return LV_OK;
 }
 

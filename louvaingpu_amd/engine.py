@@ -185,6 +185,24 @@ class Engine:
         check(self._lib.lv_run(self._h, ct.byref(lv)), "lv_run")
         return lv.value
 
+    def run_partitions(self, node_bufs, comm_bufs, final):
+        """run() with the dendrogram delivered to the host during the run:
+        Partition of level l into node_bufs[l] / comm_bufs[l] (int32 arrays of at
+        least n entries each; page-locked ones let the copies overlap the run),
+        the final partition into `final`.  Returns the number of levels."""
+        k = len(node_bufs)
+        if len(comm_bufs) != k:
+            raise ValueError("node_bufs and comm_bufs must pair up")
+        for a in list(node_bufs) + list(comm_bufs) + [final]:
+            if a.dtype != np.int32 or len(a) < self.n or not a.flags.c_contiguous:
+                raise ValueError("buffers must be contiguous int32 arrays of at least n entries")
+        np_ = (ct.c_void_p * max(k, 1))(*[a.ctypes.data for a in node_bufs])
+        cp_ = (ct.c_void_p * max(k, 1))(*[a.ctypes.data for a in comm_bufs])
+        lv = ct.c_int32()
+        check(self._lib.lv_run_partitions(self._h, k, np_, cp_, _p32(final), ct.byref(lv)),
+              "lv_run_partitions")
+        return lv.value
+
     # results --------------------------------------------------------------
     def num_levels(self) -> int:
         lv = ct.c_int32()

@@ -394,3 +394,37 @@ def test_csr_paths_agree(variant):
         assert_run_matches(eng, r, n)
     finally:
         eng.close()
+
+
+def test_run_partitions_delivers_the_dendrogram():
+    """lv_run_partitions copies every level's Partition and the final partition
+    to the host during the run: equal to lv_level_partition / lv_final_assignment
+    after lv_run, and to the oracle; fewer buffers than levels leave the rest
+    on the device; an edgeless graph gives its one identity level."""
+    from louvaingpu_amd.engine import pinned_empty
+    n, src, dst, w = generators.rmat(14, 16, 0.57, 0.19, 0.19, 9, "int")
+    gpu_case("run_partitions rmat14 n=%d e=%d" % (n, len(src)))
+    r = oracle.run(n, src, dst, w)
+    with Engine() as eng:
+        eng.load_edges(n, src, dst, w)
+        L = r.num_levels
+        nodes = [pinned_empty(n, np.int32) for _ in range(L)]
+        comms = [np.full(n, -7, np.int32) for _ in range(L)]  # pageable works too
+        fin = pinned_empty(n, np.int32)
+        assert eng.run_partitions(nodes, comms, fin) == L
+        for l in range(L):
+            k = r.level_n(l)
+            onode, ocomm = r.partition(l)
+            assert np.array_equal(nodes[l][:k], onode) and np.array_equal(comms[l][:k], ocomm), l
+        assert np.array_equal(fin, r.final_comm())
+        # one buffer pair: level 0 delivered, the others fetched afterwards
+        fin2 = np.zeros(n, np.int32)
+        n0, c0 = np.zeros(n, np.int32), np.zeros(n, np.int32)
+        assert eng.run_partitions([n0], [c0], fin2) == L
+        assert np.array_equal(n0, r.partition(0)[0]) and np.array_equal(c0, r.partition(0)[1])
+        assert np.array_equal(eng.level_partition(L - 1)[1], r.partition(L - 1)[1])
+        assert np.array_equal(fin2, fin)
+        eng.load_edges(6, np.zeros(0, np.int32), np.zeros(0, np.int32), np.zeros(0, np.int32))
+        a, b, f = np.zeros(6, np.int32), np.zeros(6, np.int32), np.zeros(6, np.int32)
+        assert eng.run_partitions([a], [b], f) == 1
+        assert f.tolist() == list(range(6)) and b.tolist() == list(range(6))
