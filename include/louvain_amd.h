@@ -73,7 +73,13 @@ typedef struct {
     int32_t hub_q;
     int32_t hub_part_target;
     int32_t chunk_mul;
-    int32_t reserved;    /* 0 */
+    /* Edge-list upload of lv_load_edges (positions as ids): 0 = encoded when
+     * the list has at least 2^18 edges and the process may use 4 or more CPUs
+     * (host threads turn a source-sorted src into row offsets and int weights
+     * below 255 into bytes while dst is copied; what does not encode is copied
+     * as it is), 1 = plain copies, 2 = encoded at any size.  Results do not
+     * depend on it. */
+    int32_t upload;
 } lv_config;
 
 /* Per-level summary (the reference keeps only Partition{count,...}). */
@@ -104,6 +110,8 @@ typedef struct {
     int32_t levels, passes;
     uint64_t arena_capacity;               /* bytes of the HBM arena */
     uint64_t arena_high_water;             /* most bytes allocated at once */
+    uint64_t upload_bytes;                 /* host-to-device bytes of the last lv_load_edges
+                                              (12 per edge as plain copies) */
 } lv_stats;
 
 void lv_config_default(lv_config *cfg);

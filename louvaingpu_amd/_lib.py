@@ -70,7 +70,7 @@ class LvConfig(ct.Structure):
         ("hub_q", ct.c_int32),
         ("hub_part_target", ct.c_int32),
         ("chunk_mul", ct.c_int32),
-        ("reserved", ct.c_int32),
+        ("upload", ct.c_int32),
     ]
 
 
@@ -129,6 +129,7 @@ class LvStats(ct.Structure):
         ("passes", ct.c_int32),
         ("arena_capacity", ct.c_uint64),
         ("arena_high_water", ct.c_uint64),
+        ("upload_bytes", ct.c_uint64),
     ]
 
 

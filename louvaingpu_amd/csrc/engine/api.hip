@@ -166,6 +166,9 @@ struct lv_ctx {
     // scratch array, the stream is the caller's, and the host copies of the
     // scalars are these ordinary (pageable) members
     bool borrowed = false;
+    // lv_load_edges: host threads and page-locked staging of the encoded upload
+    EdgeUploader uploader;
+    int64_t upload_bytes = 0;  // copied to the device by the last load
     DevFlags hs_flags{};
     QAcc hs_qacc{};
     long long hs_small[64] = {0};
@@ -1624,7 +1627,8 @@ bool tunables_valid(const lv_config &c)
 {
     return (c.hub_q == 0 || (c.hub_q >= 64 && c.hub_q <= kTierMaxQ[kTiers - 2])) &&
            (c.hub_part_target == 0 || (c.hub_part_target >= 64 && c.hub_part_target <= (1 << 30))) &&
-           (c.chunk_mul == 0 || (c.chunk_mul >= 1 && c.chunk_mul <= 1024)) && c.reserved == 0;
+           (c.chunk_mul == 0 || (c.chunk_mul >= 1 && c.chunk_mul <= 1024)) && c.upload >= 0 &&
+           c.upload <= 2;
 }
 
 struct TuneRow {
@@ -1861,23 +1865,72 @@ int lv_load_edges(lv_ctx *c, int32_t n, const int32_t *ids, int64_t e, const int
     uint32_t *wraw = nullptr;
     LV_TRY(arena_alloc(A, wraw, e));
     LV_HIP(hipMemsetAsync(c->d_flags, 0, sizeof(DevFlags), s));
-    LV_TRY(h2d(c, c->d_src, src, e));
-    LV_TRY(h2d(c, c->d_dst, dst, e));
-    LV_TRY(h2d(c, wraw, (const uint32_t *)w, e));
+    // encoded upload (offsets and weight bytes, decoded by the checks) for
+    // position ids, unless the context may not allocate (lv_run_static)
+    const int mode = c->cfg.upload;
+    const bool encode = iota_ids && e > 0 && !c->borrowed && mode != 1 &&
+                        (mode == 2 || (e >= kEncodeMinEdges && c->uploader.threads() >= 4));
+    UploadBufs ub;
+    ub.src = c->d_src;
+    ub.dst = c->d_dst;
+    ub.w = wraw;
+    int64_t *d_off = nullptr, *esc_i = nullptr;
+    uint8_t *w8 = nullptr;
+    uint32_t *esc_w = nullptr;
+    int32_t *tile_v = nullptr;
+    if (encode) {
+        ub.esc_cap = EdgeUploader::esc_cap(e);
+        LV_TRY(arena_alloc(A, d_off, (int64_t)n + 1));
+        LV_TRY(arena_alloc(A, w8, e));
+        LV_TRY(arena_alloc(A, esc_i, ub.esc_cap));
+        LV_TRY(arena_alloc(A, esc_w, ub.esc_cap));
+        LV_TRY(arena_alloc(A, tile_v, (e + kDecTile - 1) / kDecTile + 1));
+        ub.off = d_off;
+        ub.w8 = w8;
+        ub.esc_i = esc_i;
+        ub.esc_w = esc_w;
+    }
+    UploadPlan plan;
+    LV_TRY(c->uploader.upload(s, n, e, src, dst, w, wtype, encode, ub, &plan));
+    c->upload_bytes = plan.bytes;
+    if (!iota_ids) c->upload_bytes += 4 * (int64_t)n;
     if (iota_ids) {
-        // one pass: endpoints, weights, and (int32 weights) the units
-        launch_load_checks(s, c->d_src, c->d_dst, wraw, wtype, e, n,
-                          wtype == LV_WEIGHT_INT32 ? c->d_w : nullptr, c->d_flags);
+        // one pass: sources decoded, endpoints, weights, and (int32 weights) the units
+        LoadedEdges L;
+        L.dst = c->d_dst;
+        if (plan.src_off) {
+            if (plan.off64)
+                L.off64 = d_off;
+            else
+                L.off32 = (const int32_t *)d_off;
+            L.tile_v = tile_v;
+            L.src_out = c->d_src;
+        } else {
+            L.src = c->d_src;
+        }
+        if (plan.w8) {
+            L.w8 = w8;
+            L.esc_i = esc_i;
+            L.esc_w = esc_w;
+            L.nesc = plan.nesc;
+        } else {
+            L.w = wraw;
+        }
+        launch_load_checks(s, L, wtype, e, n, wtype == LV_WEIGHT_INT32 ? c->d_w : nullptr,
+                           c->d_flags);
         if (wtype == LV_WEIGHT_INT32) {
-            arena_free(A, wraw);
             LV_TRY(d2h(c, c->h_flags, c->d_flags, 1));
             LV_TRY(sync(c));
+            void *tmp[] = {wraw, d_off, w8, esc_i, esc_w, tile_v};
+            for (void *p : tmp) A.release(p);
             if (c->h_flags->err) {
                 free_input(c);
                 return (c->h_flags->err & 1) ? LV_ERR_INVALID_INPUT : LV_ERR_RANGE;
             }
             return loaded(c, n, e, 0);
         }
+        void *tmp[] = {d_off, w8, esc_i, esc_w, tile_v};  // later launches may reuse them
+        for (void *p : tmp) A.release(p);
     } else {
         LV_TRY(arena_alloc(A, c->d_ids, n));
         LV_TRY(h2d(c, c->d_ids, ids, n));
@@ -2330,6 +2383,7 @@ int lv_get_stats(lv_ctx *c, lv_stats *out)
     *out = c->stats;
     out->arena_capacity = (uint64_t)c->arena.capacity();
     out->arena_high_water = (uint64_t)c->arena.high_water();
+    out->upload_bytes = (uint64_t)c->upload_bytes;
     return LV_OK;
 }
 

@@ -302,9 +302,72 @@ void launch_weight_units(hipStream_t s, const void *w, int wtype, int64_t e, int
                          int64_t *wu, DevFlags *f);
 void launch_check_endpoints(hipStream_t s, const int32_t *src, const int32_t *dst, int64_t e,
                             int32_t n, DevFlags *f);
-// Checks of an uploaded edge list (ids = positions); int32 weights also to units.
-void launch_load_checks(hipStream_t s, const int32_t *src, const int32_t *dst, const uint32_t *w,
-                       int wtype, int64_t count, int32_t n, int64_t *wu, DevFlags *f);
+// An uploaded edge list as lv_load_edges sent it: sources raw, or as the row
+// offsets (n + 1) of a source-sorted list, decoded into src_out; weights as raw
+// 32-bit patterns, or int weights below 255 as bytes with the others (byte 255)
+// in an escape list sorted by edge.
+struct LoadedEdges {
+    const int32_t *src = nullptr;    // raw sources, or null
+    const int32_t *off32 = nullptr;  // row offsets, 32-bit when e <= INT32_MAX
+    const int64_t *off64 = nullptr;
+    int32_t *tile_v = nullptr;       // first source of every decode tile (ntiles + 1)
+    int32_t *src_out = nullptr;      // decoded sources
+    const int32_t *dst = nullptr;
+    const uint32_t *w = nullptr;     // raw weights, or null
+    const uint8_t *w8 = nullptr;
+    const int64_t *esc_i = nullptr;
+    const uint32_t *esc_w = nullptr;
+    int64_t nesc = 0;
+};
+constexpr int kDecTile = 4096;             // edges per decode tile
+constexpr int64_t kEncodeMinEdges = 1 << 18;  // smaller lists are copied as they are
+// Checks of an uploaded edge list (ids = positions): endpoints in [0, n) and the
+// weight checks of k_weight_exponent; int32 weights also to units (wu), as
+// k_weight_units does; offsets decoded to sources.
+void launch_load_checks(hipStream_t s, const LoadedEdges &L, int wtype, int64_t e, int32_t n,
+                        int64_t *wu, DevFlags *f);
+
+// Host side of lv_load_edges' upload.  The copy of dst goes out first; host
+// threads meanwhile pack int weights into bytes (escape list for 255 and up)
+// and turn a source-sorted src into row offsets, whose copies follow: 12 bytes
+// per edge become about 5 plus 4-8 per vertex.  What does not encode (unsorted
+// or out-of-range sources, more escapes than esc_cap) goes out raw, so the
+// device checks see the caller's values either way.
+struct UploadBufs {  // device destinations
+    int32_t *src = nullptr, *dst = nullptr;
+    uint32_t *w = nullptr;
+    void *off = nullptr;  // 8 * (n + 1) bytes
+    uint8_t *w8 = nullptr;
+    int64_t *esc_i = nullptr;
+    uint32_t *esc_w = nullptr;
+    int64_t esc_cap = 0;
+};
+struct UploadPlan {
+    bool src_off = false, off64 = false, w8 = false;
+    int64_t nesc = 0;
+    int64_t bytes = 0;  // copied host to device
+};
+class HostPool;
+class EdgeUploader {
+  public:
+    EdgeUploader() = default;
+    EdgeUploader(const EdgeUploader &) = delete;
+    EdgeUploader &operator=(const EdgeUploader &) = delete;
+    ~EdgeUploader();
+    static int64_t esc_cap(int64_t e) { return e / 32 + 1024; }
+    int threads();  // host threads of the pool (created on first use)
+    // Enqueues the copies on s; encode = false sends every array raw.  The host
+    // buffers must stay unchanged until the stream has finished the copies.
+    int upload(hipStream_t s, int32_t n, int64_t e, const int32_t *src, const int32_t *dst,
+               const void *w, int wtype, bool encode, const UploadBufs &d, UploadPlan *plan);
+
+  private:
+    int stage(void **p, size_t *cap, size_t bytes);
+    HostPool *pool_ = nullptr;
+    void *h_w8_ = nullptr, *h_off_ = nullptr, *h_esc_ = nullptr;
+    size_t w8_cap_ = 0, off_cap_ = 0, esc_bytes_ = 0;
+    std::vector<std::vector<int64_t>> esc_parts_;
+};
 void launch_check_sorted_unique(hipStream_t s, const uint32_t *keys, int64_t n, DevFlags *f);
 void launch_map_endpoints(hipStream_t s, const uint32_t *sorted_ids, const int32_t *sorted_pos,
                           int32_t n, int32_t *ep, int64_t e, DevFlags *f);

@@ -53,48 +53,6 @@ __global__ __launch_bounds__(256) void k_weight_units(const void *w, int wtype, 
     block_units_flush(tot, msum, umax, bad, f);
 }
 
-// An uploaded edge list whose ids are positions: endpoints in [0, n), and the
-// weight checks of k_weight_exponent; int32 weights (unit exponent 0) are also
-// converted to units here, as k_weight_units does.
-__global__ __launch_bounds__(256) void k_load_checks(const int32_t *src, const int32_t *dst,
-                                                   const uint32_t *w, int wtype, int64_t count,
-                                                   int32_t n, int64_t *wu, DevFlags *f)
-{
-    int bad = 0, smax = 0;
-    ull tot = 0, msum = 0, umax = 0;
-    GRID_STRIDE(i, count)
-    {
-        const int32_t a = src[i], b = dst[i];
-        if (a < 0 || a >= n || b < 0 || b >= n) bad |= 1;
-        const uint32_t x = w[i];
-        if (wtype == LV_WEIGHT_INT32) {
-            if ((int32_t)x < 0) bad |= 1;
-            const int64_t u = weight_units(x, wtype, 0, bad);
-            wu[i] = u;
-            umax = max(umax, (ull)u);
-            tot = sat_add(tot, (ull)(u == 0 ? 1 : u));
-            msum = sat_add(msum, (ull)u);
-        } else {
-            const int s = f32_unit_exponent(x);
-            if (s < 0)
-                bad |= 1;
-            else
-                smax = max(smax, s);
-        }
-    }
-    smax = wave_max_i(smax);
-    if ((threadIdx.x & 63) == 0 && smax) atomicMax(&f->scale, smax);
-    block_units_flush(tot, msum, umax, bad, f);
-}
-
-void launch_load_checks(hipStream_t s, const int32_t *src, const int32_t *dst, const uint32_t *w,
-                       int wtype, int64_t count, int32_t n, int64_t *wu, DevFlags *f)
-{
-    if (count > 0)
-        hipLaunchKernelGGL(k_load_checks, grid_for(count, 256, 4096), 256, 0, s, src, dst, w, wtype,
-                           count, n, wu, f);
-}
-
 __global__ void k_check_endpoints(const int32_t *src, const int32_t *dst, int64_t e, int32_t n,
                                   DevFlags *f)
 {
@@ -1975,10 +1933,24 @@ __global__ __launch_bounds__(256) void k_hub_count(SweepArgs a, HubArgs h)
     const int64_t t0 = (int64_t)h.chunk_k[ci] * cs;
     const int64_t t1 = min(q, t0 + cs);
     int32_t *tc = h.tmp_c + h.ebase[j];
-    for (int64_t t = t0 + threadIdx.x; t < t1; t += 256) {
-        int32_t c = (t < d) ? a.C[a.col[rb + t]] : own;
-        tc[t] = c;
-        atomicAdd(&hist[hub_part(c, P)], 1);
+    // four community gathers in flight per thread before their stores and counts
+    for (int64_t p0 = t0 + threadIdx.x; p0 < t1; p0 += 4 * 256) {
+        int32_t y[4], c[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int64_t t = p0 + (int64_t)u * 256;
+            y[u] = (t < t1 && t < d) ? __builtin_nontemporal_load(&a.col[rb + t]) : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) c[u] = y[u] >= 0 ? a.C[y[u]] : own;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int64_t t = p0 + (int64_t)u * 256;
+            if (t < t1) {
+                tc[t] = c[u];
+                atomicAdd(&hist[hub_part(c[u], P)], 1);
+            }
+        }
     }
     __syncthreads();
     int32_t *ch = h.chist + h.hbase[j] + (int64_t)h.chunk_k[ci] * P;

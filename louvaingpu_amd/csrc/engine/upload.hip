@@ -1,0 +1,517 @@
+// upload.hip - the edge-list upload of lv_load_edges and its device checks.
+//
+// The reference copies src, dst and w to the device as they are
+// (LouvainCUDA/LouvainCPP.cpp:283-287): 12 bytes per edge over PCIe, which at
+// R-MAT scale 22 is half of the end-to-end time.  Here the copy of dst starts
+// at once while host threads encode the rest: int weights below 255 become
+// bytes (the others go to an escape list), and a source-sorted src becomes its
+// row offsets (n + 1 entries).  The device decodes both inside the pass that
+// checks the endpoints and weights, so everything after the load sees the
+// caller's arrays.
+#include <immintrin.h>
+#include <sched.h>
+
+#include <algorithm>
+#include <atomic>
+#include <climits>
+#include <cstring>
+#include <condition_variable>
+#include <functional>
+#include <mutex>
+#include <thread>
+
+#include "device.hpp"
+
+namespace lv {
+
+// ---------------------------------------------------------------------------
+// host threads
+
+namespace {
+
+int usable_cpus()
+{
+    cpu_set_t set;
+    CPU_ZERO(&set);
+    int n = 0;
+    if (sched_getaffinity(0, sizeof(set), &set) == 0) n = CPU_COUNT(&set);
+    if (n <= 0) n = (int)std::thread::hardware_concurrency();
+    return std::max(1, std::min(n, 64));
+}
+
+}  // namespace
+
+// Fixed workers that run one job at a time: run(fn) calls fn(t) for every
+// t in [0, size()), t = 0 on the calling thread, and returns when all are done.
+class HostPool {
+  public:
+    explicit HostPool(int n) : n_(n)
+    {
+        for (int t = 1; t < n_; ++t) th_.emplace_back([this, t] { loop(t); });
+    }
+    ~HostPool()
+    {
+        {
+            std::lock_guard<std::mutex> g(m_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto &t : th_) t.join();
+    }
+    int size() const { return n_; }
+    void run(const std::function<void(int)> &fn)
+    {
+        if (n_ > 1) {
+            std::lock_guard<std::mutex> g(m_);
+            job_ = &fn;
+            left_ = n_ - 1;
+            ++gen_;
+        }
+        cv_.notify_all();
+        fn(0);
+        if (n_ > 1) {
+            std::unique_lock<std::mutex> g(m_);
+            done_.wait(g, [this] { return left_ == 0; });
+            job_ = nullptr;
+        }
+    }
+
+  private:
+    void loop(int t)
+    {
+        uint64_t seen = 0;
+        for (;;) {
+            const std::function<void(int)> *job;
+            {
+                std::unique_lock<std::mutex> g(m_);
+                cv_.wait(g, [&] { return stop_ || gen_ != seen; });
+                if (stop_) return;
+                seen = gen_;
+                job = job_;
+            }
+            (*job)(t);
+            std::lock_guard<std::mutex> g(m_);
+            if (--left_ == 0) done_.notify_one();
+        }
+    }
+    int n_;
+    std::vector<std::thread> th_;
+    std::mutex m_;
+    std::condition_variable cv_, done_;
+    const std::function<void(int)> *job_ = nullptr;
+    uint64_t gen_ = 0;
+    int left_ = 0;
+    bool stop_ = false;
+};
+
+EdgeUploader::~EdgeUploader()
+{
+    delete pool_;
+    if (h_w8_) (void)hipHostFree(h_w8_);
+    if (h_off_) (void)hipHostFree(h_off_);
+    if (h_esc_) (void)hipHostFree(h_esc_);
+}
+
+int EdgeUploader::threads()
+{
+    if (!pool_) pool_ = new HostPool(usable_cpus());
+    return pool_->size();
+}
+
+// Page-locked staging, kept for the next upload (the copies from it have
+// finished before lv_load_edges returns).
+int EdgeUploader::stage(void **p, size_t *cap, size_t bytes)
+{
+    if (*cap >= bytes) return LV_OK;
+    if (*p) (void)hipHostFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    if (hipHostMalloc(p, bytes, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        *p = nullptr;
+        return LV_ERR_NOMEM;
+    }
+    *cap = bytes;
+    return LV_OK;
+}
+
+namespace {
+
+// Row offsets from a part [a, b) of a source-sorted list: the rows
+// (src[a-1], src[b-1]] start in this part, and the last part also ends the
+// empty rows after the last source at e; the parts write disjoint entries when
+// src is sorted.  Every write is checked first (each row change must follow
+// the last row written, and stay below n), so an unsorted or out-of-range list
+// only returns false.
+template <typename O>
+bool offsets_scalar(const int32_t *src, int64_t a, int64_t b, int64_t e, int32_t n, O *off)
+{
+    int32_t prev = a == 0 ? -1 : src[a - 1];
+    if (a == 0 && src[0] < 0) return false;
+    for (int64_t i = a; i < b; ++i) {
+        const int32_t v = src[i];
+        if (v == prev) continue;
+        if (v < prev || v >= n) return false;
+        for (int32_t u = prev + 1; u <= v; ++u) off[u] = (O)i;
+        prev = v;
+    }
+    if (b == e)
+        for (int64_t u = (int64_t)prev + 1; u <= n; ++u) off[u] = (O)e;
+    return true;
+}
+
+// The same with eight sources per compare: only the row changes (set bits of
+// src[i] > src[i-1]) take the scalar path, so the irregular run lengths of a
+// skewed graph cost no branch per edge.
+template <typename O>
+__attribute__((target("avx2"))) bool offsets_avx2(const int32_t *src, int64_t a, int64_t b,
+                                                  int64_t e, int32_t n, O *off)
+{
+    if (a == 0 && src[0] < 0) return false;
+    int32_t last = a == 0 ? -1 : src[a - 1];  // rows up to last are written
+    int64_t i = a;
+    auto change = [&](int64_t j) {  // src[j] != src[j-1] (or j == 0)
+        const int32_t v = src[j];
+        const int32_t p = j == 0 ? -1 : src[j - 1];
+        if (p != last || v <= p || v >= n) return false;
+        for (int32_t u = p + 1; u <= v; ++u) off[u] = (O)j;
+        last = v;
+        return true;
+    };
+    if (i < b && (i == 0 || src[i] != src[i - 1])) {
+        if (!change(i)) return false;
+    }
+    ++i;
+    int down = 0;
+    for (; i + 8 <= b; i += 8) {
+        const __m256i cur = _mm256_loadu_si256((const __m256i *)(src + i));
+        const __m256i prv = _mm256_loadu_si256((const __m256i *)(src + i - 1));
+        down |= _mm256_movemask_epi8(_mm256_cmpgt_epi32(prv, cur));
+        unsigned m = (unsigned)_mm256_movemask_ps(_mm256_castsi256_ps(_mm256_cmpgt_epi32(cur, prv)));
+        while (m) {
+            const int k = __builtin_ctz(m);
+            m &= m - 1;
+            if (!change(i + k)) return false;
+        }
+    }
+    for (; i < b; ++i) {
+        if (src[i] < src[i - 1]) return false;
+        if (src[i] != src[i - 1] && !change(i)) return false;
+    }
+    if (down) return false;
+    if (b == e)
+        for (int64_t u = (int64_t)last + 1; u <= n; ++u) off[u] = (O)e;
+    return true;
+}
+
+// Int weights below 255 as bytes; the positions of the others (byte 255) go
+// to esc, at most cap of them (false beyond).
+bool pack_scalar(const uint32_t *x, int64_t a, int64_t b, uint8_t *o, std::vector<int64_t> &esc,
+                 int64_t cap)
+{
+    for (int64_t i = a; i < b; ++i) {
+        const uint32_t y = x[i];
+        o[i] = (uint8_t)(y < 255u ? y : 255u);
+        if (y >= 255u) {
+            if ((int64_t)esc.size() >= cap) return false;
+            esc.push_back(i);
+        }
+    }
+    return true;
+}
+
+__attribute__((target("avx2"))) bool pack_avx2(const uint32_t *x, int64_t a, int64_t b,
+                                               uint8_t *o, std::vector<int64_t> &esc, int64_t cap)
+{
+    const __m256i k254 = _mm256_set1_epi32(254), k255 = _mm256_set1_epi32(255);
+    int64_t i = a;
+    for (; i + 8 <= b; i += 8) {
+        const __m256i v = _mm256_loadu_si256((const __m256i *)(x + i));
+        // unsigned v > 254: max(v, 254) != 254
+        const __m256i small = _mm256_cmpeq_epi32(_mm256_max_epu32(v, k254), k254);
+        unsigned m = ~(unsigned)_mm256_movemask_ps(_mm256_castsi256_ps(small)) & 0xffu;
+        const __m256i c = _mm256_min_epu32(v, k255);
+        const __m256i p8 = _mm256_packus_epi16(_mm256_packus_epi32(c, c), k255);
+        const uint32_t lo = (uint32_t)_mm256_extract_epi32(p8, 0);
+        const uint32_t hi = (uint32_t)_mm256_extract_epi32(p8, 4);
+        memcpy(o + i, &lo, 4);
+        memcpy(o + i + 4, &hi, 4);
+        while (m) {
+            const int k = __builtin_ctz(m);
+            m &= m - 1;
+            if ((int64_t)esc.size() >= cap) return false;
+            esc.push_back(i + k);
+        }
+    }
+    return pack_scalar(x, i, b, o, esc, cap);
+}
+
+template <typename O>
+bool offsets_part(const int32_t *src, int64_t a, int64_t b, int64_t e, int32_t n, O *off)
+{
+    static const bool avx2 = __builtin_cpu_supports("avx2");
+    return avx2 ? offsets_avx2(src, a, b, e, n, off) : offsets_scalar(src, a, b, e, n, off);
+}
+
+bool pack_part(const uint32_t *x, int64_t a, int64_t b, uint8_t *o, std::vector<int64_t> &esc,
+               int64_t cap)
+{
+    static const bool avx2 = __builtin_cpu_supports("avx2");
+    return avx2 ? pack_avx2(x, a, b, o, esc, cap) : pack_scalar(x, a, b, o, esc, cap);
+}
+
+}  // namespace
+
+int EdgeUploader::upload(hipStream_t s, int32_t n, int64_t e, const int32_t *src,
+                         const int32_t *dst, const void *w, int wtype, bool encode,
+                         const UploadBufs &d, UploadPlan *plan)
+{
+    *plan = UploadPlan{};
+    if (e <= 0) return LV_OK;
+    const size_t eb = 4 * (size_t)e;
+    LV_HIP(hipMemcpyAsync(d.dst, dst, eb, hipMemcpyHostToDevice, s));
+    plan->bytes = (int64_t)eb;
+    if (!encode) {
+        LV_HIP(hipMemcpyAsync(d.w, w, eb, hipMemcpyHostToDevice, s));
+        LV_HIP(hipMemcpyAsync(d.src, src, eb, hipMemcpyHostToDevice, s));
+        plan->bytes += 2 * (int64_t)eb;
+        return LV_OK;
+    }
+    const int T = threads();
+    auto part = [&](int t, int64_t &a, int64_t &b) {
+        a = e * t / T;
+        b = e * (t + 1) / T;
+    };
+    // weights first: their copy then queues behind dst while src is scanned
+    bool packed = false;
+    if (wtype == LV_WEIGHT_INT32) {
+        LV_TRY(stage(&h_w8_, &w8_cap_, (size_t)e));
+        esc_parts_.resize(T);
+        std::atomic<bool> over{false};
+        const uint32_t *x = (const uint32_t *)w;
+        uint8_t *o = (uint8_t *)h_w8_;
+        const int64_t cap = d.esc_cap;
+        pool_->run([&](int t) {
+            std::vector<int64_t> &esc = esc_parts_[t];
+            esc.clear();
+            int64_t a, b;
+            part(t, a, b);
+            if (!pack_part(x, a, b, o, esc, cap)) over = true;
+        });
+        int64_t ne = 0;
+        for (const auto &v : esc_parts_) ne += (int64_t)v.size();
+        if (!over && ne <= cap) {
+            LV_TRY(stage(&h_esc_, &esc_bytes_, 12 * (size_t)std::max<int64_t>(ne, 1)));
+            int64_t *hi = (int64_t *)h_esc_;
+            uint32_t *hw = (uint32_t *)(hi + ne);
+            int64_t j = 0;
+            for (const auto &v : esc_parts_)
+                for (int64_t i : v) {
+                    hi[j] = i;
+                    hw[j] = x[i];
+                    ++j;
+                }
+            LV_HIP(hipMemcpyAsync(d.w8, h_w8_, (size_t)e, hipMemcpyHostToDevice, s));
+            if (ne > 0) {
+                LV_HIP(hipMemcpyAsync(d.esc_i, hi, 8 * (size_t)ne, hipMemcpyHostToDevice, s));
+                LV_HIP(hipMemcpyAsync(d.esc_w, hw, 4 * (size_t)ne, hipMemcpyHostToDevice, s));
+            }
+            plan->w8 = true;
+            plan->nesc = ne;
+            plan->bytes += e + 12 * ne;
+            packed = true;
+        }
+    }
+    if (!packed) {
+        LV_HIP(hipMemcpyAsync(d.w, w, eb, hipMemcpyHostToDevice, s));
+        plan->bytes += (int64_t)eb;
+    }
+    // sources: row offsets when sorted and in range
+    const bool o64 = e > INT32_MAX;
+    const size_t ob = (size_t)(n + 1) * (o64 ? 8 : 4);
+    LV_TRY(stage(&h_off_, &off_cap_, ob));
+    std::atomic<bool> bad{false};
+    pool_->run([&](int t) {
+        int64_t a, b;
+        part(t, a, b);
+        if (a >= b) return;
+        const bool ok = o64 ? offsets_part(src, a, b, e, n, (int64_t *)h_off_)
+                            : offsets_part(src, a, b, e, n, (int32_t *)h_off_);
+        if (!ok) bad = true;
+    });
+    if (!bad) {
+        LV_HIP(hipMemcpyAsync(d.off, h_off_, ob, hipMemcpyHostToDevice, s));
+        plan->src_off = true;
+        plan->off64 = o64;
+        plan->bytes += (int64_t)ob;
+    } else {
+        LV_HIP(hipMemcpyAsync(d.src, src, eb, hipMemcpyHostToDevice, s));
+        plan->bytes += (int64_t)eb;
+    }
+    return LV_OK;
+}
+
+// ---------------------------------------------------------------------------
+// device: decode and check
+
+constexpr int kDecSpan = 4096;  // offsets of a decode tile held in LDS
+
+// Source of the first edge of every decode tile, and (entry ntiles) of edge
+// e - 1: the largest v with off[v] <= t (off[0] = 0 <= t < e = off[n]).
+template <typename O>
+__global__ void k_dec_tiles(const O *off, int32_t n, int64_t e, int64_t ntiles, int32_t *tile_v)
+{
+    GRID_STRIDE(k, ntiles + 1)
+    {
+        const int64_t t = k < ntiles ? k * (int64_t)kDecTile : e - 1;
+        int32_t lo = 0, hi = n - 1;
+        while (lo < hi) {
+            const int32_t mid = lo + ((hi - lo + 1) >> 1);
+            if ((int64_t)off[mid] <= t)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        tile_v[k] = lo;
+    }
+}
+
+// Largest j in [0, hi] with o[j] <= i (o[0] <= i).
+template <typename A>
+__device__ __forceinline__ int32_t last_le(const A &o, int32_t hi, int64_t i)
+{
+    int32_t lo = 0;
+    while (lo < hi) {
+        const int32_t mid = lo + ((hi - lo + 1) >> 1);
+        if ((int64_t)o[mid] <= i)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// One pass over the edges in tiles of kDecTile: decodes the sources (SOFF),
+// checks the endpoints, and checks the weights as k_weight_exponent does (int32
+// weights also to units, as k_weight_units does with exponent 0).  Escaped
+// bytes (W8, value 255) are left to k_load_escapes.
+template <bool SOFF, bool W8, typename O>
+__global__ __launch_bounds__(256) void k_load_checks(LoadedEdges L, const O *off, int wtype,
+                                                    int64_t e, int32_t n, int64_t *wu,
+                                                    DevFlags *f)
+{
+    __shared__ O s_off[SOFF ? kDecSpan : 1];
+    int bad = 0, smax = 0;
+    ull tot = 0, msum = 0, umax = 0;
+    const int64_t ntiles = (e + kDecTile - 1) / kDecTile;
+    for (int64_t k = blockIdx.x; k < ntiles; k += gridDim.x) {
+        const int64_t t0 = k * (int64_t)kDecTile, t1 = min(e, t0 + kDecTile);
+        int32_t lo = 0, span = 0;
+        if (SOFF) {
+            // the tile's edges have sources in [lo, tile_v[k + 1]]
+            lo = L.tile_v[k];
+            span = L.tile_v[k + 1] - lo + 2;  // off[lo .. tile_v[k + 1] + 1]
+            if (span <= kDecSpan)
+                for (int j = threadIdx.x; j < span; j += 256) s_off[j] = off[lo + j];
+            __syncthreads();
+        }
+        for (int64_t i = t0 + threadIdx.x; i < t1; i += 256) {
+            int32_t a;
+            if (SOFF) {
+                a = lo + (span <= kDecSpan ? last_le(s_off, span - 2, i)
+                                           : last_le(off + lo, span - 2, i));
+                L.src_out[i] = a;
+            } else {
+                a = L.src[i];
+            }
+            const int32_t b = L.dst[i];
+            if (a < 0 || a >= n || b < 0 || b >= n) bad |= 1;
+            if (W8) {
+                const uint32_t x = L.w8[i];
+                if (x != 255u) {
+                    wu[i] = (int64_t)x;
+                    umax = max(umax, (ull)x);
+                    tot = sat_add(tot, (ull)(x == 0 ? 1 : x));
+                    msum = sat_add(msum, (ull)x);
+                }
+            } else {
+                const uint32_t x = L.w[i];
+                if (wtype == LV_WEIGHT_INT32) {
+                    if ((int32_t)x < 0) bad |= 1;
+                    const int64_t u = weight_units(x, wtype, 0, bad);
+                    wu[i] = u;
+                    umax = max(umax, (ull)u);
+                    tot = sat_add(tot, (ull)(u == 0 ? 1 : u));
+                    msum = sat_add(msum, (ull)u);
+                } else {
+                    const int sx = f32_unit_exponent(x);
+                    if (sx < 0)
+                        bad |= 1;
+                    else
+                        smax = max(smax, sx);
+                }
+            }
+        }
+        if (SOFF) __syncthreads();  // s_off is refilled for the next tile
+    }
+    smax = wave_max_i(smax);
+    if ((threadIdx.x & 63) == 0 && smax) atomicMax(&f->scale, smax);
+    block_units_flush(tot, msum, umax, bad, f);
+}
+
+// The escaped int weights (255 and up, negative ones included) of a byte upload.
+__global__ __launch_bounds__(256) void k_load_escapes(const int64_t *ei, const uint32_t *ew,
+                                                     int64_t nesc, int64_t *wu, DevFlags *f)
+{
+    int bad = 0;
+    ull tot = 0, msum = 0, umax = 0;
+    GRID_STRIDE(j, nesc)
+    {
+        const uint32_t x = ew[j];
+        if ((int32_t)x < 0) bad |= 1;
+        const int64_t u = weight_units(x, LV_WEIGHT_INT32, 0, bad);
+        wu[ei[j]] = u;
+        umax = max(umax, (ull)u);
+        tot = sat_add(tot, (ull)(u == 0 ? 1 : u));
+        msum = sat_add(msum, (ull)u);
+    }
+    block_units_flush(tot, msum, umax, bad, f);
+}
+
+template <bool SOFF, typename O>
+static void launch_checks_w(hipStream_t s, const LoadedEdges &L, const O *off, int wtype, int64_t e,
+                            int32_t n, int64_t *wu, DevFlags *f, int grid)
+{
+    if (L.w8)
+        hipLaunchKernelGGL((k_load_checks<SOFF, true, O>), grid, 256, 0, s, L, off, wtype, e, n,
+                           wu, f);
+    else
+        hipLaunchKernelGGL((k_load_checks<SOFF, false, O>), grid, 256, 0, s, L, off, wtype, e, n,
+                           wu, f);
+}
+
+void launch_load_checks(hipStream_t s, const LoadedEdges &L, int wtype, int64_t e, int32_t n,
+                        int64_t *wu, DevFlags *f)
+{
+    if (e <= 0) return;
+    const int64_t ntiles = (e + kDecTile - 1) / kDecTile;
+    const int grid = (int)std::min<int64_t>(ntiles, 8192);
+    if (L.off32 || L.off64) {
+        if (L.off32) {
+            hipLaunchKernelGGL(k_dec_tiles<int32_t>, grid_for(ntiles + 1, 256), 256, 0, s, L.off32,
+                               n, e, ntiles, L.tile_v);
+            launch_checks_w<true>(s, L, L.off32, wtype, e, n, wu, f, grid);
+        } else {
+            hipLaunchKernelGGL(k_dec_tiles<int64_t>, grid_for(ntiles + 1, 256), 256, 0, s, L.off64,
+                               n, e, ntiles, L.tile_v);
+            launch_checks_w<true>(s, L, L.off64, wtype, e, n, wu, f, grid);
+        }
+    } else {
+        launch_checks_w<false>(s, L, (const int32_t *)nullptr, wtype, e, n, wu, f, grid);
+    }
+    if (L.w8 && L.nesc > 0)
+        hipLaunchKernelGGL(k_load_escapes, grid_for(L.nesc, 256, 1024), 256, 0, s, L.esc_i,
+                           L.esc_w, L.nesc, wu, f);
+}
+
+}  // namespace lv
