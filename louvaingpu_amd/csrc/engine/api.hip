@@ -74,6 +74,7 @@ struct lv_ctx {
     int32_t *d_ids = nullptr;  // null when ids are positions
     int32_t *d_src = nullptr, *d_dst = nullptr;
     int64_t *d_w = nullptr;
+    uint32_t *d_w32 = nullptr;  // instead of d_w when every weight's units fit 32 bits
     // current level
     int state = ST_EMPTY;
     int level = 0;
@@ -393,6 +394,7 @@ void free_input(lv_ctx *c)
     arena_free(c->arena, c->d_src);
     arena_free(c->arena, c->d_dst);
     arena_free(c->arena, c->d_w);
+    arena_free(c->arena, c->d_w32);
     c->loaded = false;
     c->batch = false;
     c->ncomp = 0;
@@ -441,6 +443,21 @@ double current_q(lv_ctx *c, int &rc)
 
 // ---------------------------------------------------------------------------
 // phases
+
+// Units of a non-negative float32 bit pattern at exponent scale (as
+// weight_units on the device; saturates past 2^62).
+uint64_t f32_units_host(uint32_t x, int scale)
+{
+    if ((x & 0x7fffffffu) == 0) return 0;
+    const uint32_t ex = (x >> 23) & 0xffu, mant = x & 0x7fffffu;
+    const uint64_t M = ex ? (mant | 0x800000u) : mant;
+    const int sh = (ex ? (int)ex - 150 : -149) + scale;
+    if (sh > 38) return 1ull << 62;
+    return sh >= 0 ? (M << sh) : (M >> (-sh));
+}
+
+// The input weights in units (4 or 8 bytes each).
+WUnits wunits(const lv_ctx *c) { return WUnits(c->d_w, c->d_w32); }
 
 // Row-entry weights of a level: 4 bytes each when every weight fits 32 bits.
 int alloc_csr_w(lv_ctx *c, int64_t nnz, bool w32)
@@ -491,13 +508,13 @@ int build_level0_csr_sorted(lv_ctx *c, int32_t *tself, int32_t *vself, const int
     uint64_t *sp = nullptr;
     int fbits = wbits, vshift = 0;
     if (one_word) {
-        launch_rev_entries1(s, c->d_src, c->d_dst, c->d_w, tscan, e, kb, p1);
+        launch_rev_entries1(s, c->d_src, c->d_dst, wunits(c), tscan, e, kb, p1);
         LV_TRY(radix_sort_u64_keys(A, s, p1, p2, nr, 64 - kb, &sp));
         launch_row_starts64(s, sp, nr, n, 64 - kb, rs);
         fbits = wb1;
         vshift = 64 - kb;
     } else {
-        launch_rev_entries(s, c->d_src, c->d_dst, c->d_w, tscan, e, wbits, k1, p1);
+        launch_rev_entries(s, c->d_src, c->d_dst, wunits(c), tscan, e, wbits, k1, p1);
         LV_TRY(radix_sort_u32_u64(A, s, k1, p1, k2, p2, nr, kb, &sk, &sp));
         launch_rowptr_search(s, sk, nr, n, rs);
     }
@@ -505,10 +522,10 @@ int build_level0_csr_sorted(lv_ctx *c, int32_t *tself, int32_t *vself, const int
     LV_TRY(arena_alloc(A, c->col, c->nnz));
     LV_TRY(alloc_csr_w(c, c->nnz, c->max_units0 < (1ull << 32)));
     LV_HIP(hipMemsetAsync(&c->d_flags->err, 0, sizeof(int), s));
-    launch_csr_sorted_fill(s, !canonical, c->d_src, c->d_dst, c->d_w, tscan, vscan, e, n, fbits,
+    launch_csr_sorted_fill(s, !canonical, c->d_src, c->d_dst, wunits(c), tscan, vscan, e, n, fbits,
                            vshift, kb, fs, sk, sp, nr, rs, c->rowptr, ffwd, c->col, c->cw, c->cw32,
                            c->kdeg, c->d_flags);
-    launch_loop_gather(s, lastloop, c->d_w, n, c->loopw);
+    launch_loop_gather(s, lastloop, wunits(c), n, c->loopw);
     LV_HIP(hipGetLastError());
     if (canonical) {
         c->rows_unique = true;
@@ -564,7 +581,7 @@ int build_level0_csr(lv_ctx *c)
         LV_TRY(arena_alloc(A, vself, (int64_t)n + 1));
         LV_TRY(arena_alloc(A, fs, (int64_t)n + 1));
         LV_HIP(hipMemsetAsync(&c->d_flags->err, 0, sizeof(int), s));
-        launch_src_sorted(s, c->d_src, c->d_dst, c->d_w, e, n, tself, vself, fs, c->kdeg,
+        launch_src_sorted(s, c->d_src, c->d_dst, wunits(c), e, n, tself, vself, fs, c->kdeg,
                           c->selfsum, lastloop, c->d_flags);
         LV_HIP(hipGetLastError());
         LV_TRY(d2h(c, c->h_flags, c->d_flags, 1));
@@ -587,7 +604,7 @@ int build_level0_csr(lv_ctx *c)
         arena_free(A, vself);
         arena_free(A, fs);
     } else {
-        launch_self_loops(s, c->d_src, c->d_dst, c->d_w, e, c->kdeg, c->selfsum, lastloop);
+        launch_self_loops(s, c->d_src, c->d_dst, wunits(c), e, c->kdeg, c->selfsum, lastloop);
     }
     // one word per entry (row | col | w) when the weight fits the bits the row
     // key (self-loops keyed n) and the column leave: 8 bytes per sorted entry
@@ -598,7 +615,7 @@ int build_level0_csr(lv_ctx *c)
         uint64_t *x1 = nullptr, *x2 = nullptr, *sx = nullptr;
         LV_TRY(arena_alloc(A, x1, total));
         LV_TRY(arena_alloc(A, x2, total));
-        launch_csr_entries1(s, c->d_src, c->d_dst, c->d_w, e, n, bits, cb, x1);
+        launch_csr_entries1(s, c->d_src, c->d_dst, wunits(c), e, n, bits, cb, x1);
         LV_TRY(radix_sort_u64_keys(A, s, x1, x2, total, 64 - bits, &sx));
         launch_row_starts64(s, sx, total, n, 64 - bits, c->rowptr);
         LV_TRY(d2h(c, &c->h_small[0], (const long long *)(c->rowptr + n), 1));
@@ -608,7 +625,7 @@ int build_level0_csr(lv_ctx *c)
         LV_TRY(alloc_csr_w(c, c->nnz, c->max_units0 < (1ull << 32)));
         LV_HIP(hipMemsetAsync(&c->d_flags->err, 0, sizeof(int), s));
         launch_csr_unpack1(s, sx, c->nnz, bits, cb, c->col, c->cw, c->cw32, c->kdeg, c->d_flags);
-        launch_loop_gather(s, lastloop, c->d_w, n, c->loopw);
+        launch_loop_gather(s, lastloop, wunits(c), n, c->loopw);
         LV_HIP(hipGetLastError());
         LV_TRY(d2h(c, c->h_flags, c->d_flags, 1));
         LV_TRY(sync(c));
@@ -633,7 +650,7 @@ int build_level0_csr(lv_ctx *c)
     uint32_t *sk = nullptr, *sv = nullptr;
     uint64_t *sp = nullptr;
     if (packed) {
-        launch_csr_entries_packed(s, c->d_src, c->d_dst, c->d_w, e, n, wbits, k1, p1);
+        launch_csr_entries_packed(s, c->d_src, c->d_dst, wunits(c), e, n, wbits, k1, p1);
         LV_TRY(radix_sort_u32_u64(A, s, k1, p1, k2, p2, total, bits, &sk, &sp));
     } else {
         launch_csr_entries(s, c->d_src, c->d_dst, e, n, k1, v1);
@@ -648,9 +665,9 @@ int build_level0_csr(lv_ctx *c)
     if (packed)
         launch_csr_unpack(s, sk, sp, c->nnz, wbits, c->col, c->cw, c->cw32, c->kdeg);
     else
-        launch_csr_gather(s, sk, sv, c->nnz, c->d_src, c->d_dst, c->d_w, c->col, c->cw, c->cw32,
+        launch_csr_gather(s, sk, sv, c->nnz, c->d_src, c->d_dst, wunits(c), c->col, c->cw, c->cw32,
                           c->kdeg);
-    launch_loop_gather(s, lastloop, c->d_w, n, c->loopw);
+    launch_loop_gather(s, lastloop, wunits(c), n, c->loopw);
     LV_HIP(hipMemsetAsync(&c->d_flags->err, 0, sizeof(int), s));
     launch_rows_check(s, sk, c->col, c->nnz, c->d_flags);
     LV_HIP(hipGetLastError());
@@ -1861,7 +1878,7 @@ int lv_load_edges(lv_ctx *c, int32_t n, const int32_t *ids, int64_t e, const int
         for (int32_t i = 0; i < n && iota_ids; ++i) iota_ids = (ids[i] == i);
     LV_TRY(arena_alloc(A, c->d_src, e));
     LV_TRY(arena_alloc(A, c->d_dst, e));
-    LV_TRY(arena_alloc(A, c->d_w, e));
+    // int32 weights are their own units: the uploaded array becomes d_w32
     uint32_t *wraw = nullptr;
     LV_TRY(arena_alloc(A, wraw, e));
     LV_HIP(hipMemsetAsync(c->d_flags, 0, sizeof(DevFlags), s));
@@ -1916,12 +1933,13 @@ int lv_load_edges(lv_ctx *c, int32_t n, const int32_t *ids, int64_t e, const int
         } else {
             L.w = wraw;
         }
-        launch_load_checks(s, L, wtype, e, n, wtype == LV_WEIGHT_INT32 ? c->d_w : nullptr,
-                           c->d_flags);
+        // weight bytes are widened into the upload's own weight array
+        launch_load_checks(s, L, wtype, e, n, plan.w8 ? wraw : nullptr, c->d_flags);
         if (wtype == LV_WEIGHT_INT32) {
+            c->d_w32 = wraw;
             LV_TRY(d2h(c, c->h_flags, c->d_flags, 1));
             LV_TRY(sync(c));
-            void *tmp[] = {wraw, d_off, w8, esc_i, esc_w, tile_v};
+            void *tmp[] = {d_off, w8, esc_i, esc_w, tile_v};
             for (void *p : tmp) A.release(p);
             if (c->h_flags->err) {
                 free_input(c);
@@ -1964,11 +1982,21 @@ int lv_load_edges(lv_ctx *c, int32_t n, const int32_t *ids, int64_t e, const int
         return (c->h_flags->err & 1) ? LV_ERR_INVALID_INPUT : LV_ERR_RANGE;
     }
     const int scale = c->h_flags->scale;
-    launch_weight_units(s, wraw, wtype, e, scale, c->d_w, c->d_flags);
+    if (wtype == LV_WEIGHT_INT32) {
+        // their own units: checked and summed in place
+        c->d_w32 = wraw;
+        launch_weight_units(s, wraw, wtype, e, scale, nullptr, nullptr, c->d_flags);
+    } else if (f32_units_host(c->h_flags->max_bits, scale) < (1ull << 32)) {
+        LV_TRY(arena_alloc(A, c->d_w32, e));
+        launch_weight_units(s, wraw, wtype, e, scale, nullptr, c->d_w32, c->d_flags);
+    } else {
+        LV_TRY(arena_alloc(A, c->d_w, e));
+        launch_weight_units(s, wraw, wtype, e, scale, c->d_w, nullptr, c->d_flags);
+    }
     LV_HIP(hipGetLastError());
     LV_TRY(d2h(c, c->h_flags, c->d_flags, 1));
     LV_TRY(sync(c));
-    arena_free(A, wraw);
+    if (wtype != LV_WEIGHT_INT32) arena_free(A, wraw);
     if (c->h_flags->err) {
         free_input(c);
         return (c->h_flags->err & 1) ? LV_ERR_INVALID_INPUT : LV_ERR_RANGE;
@@ -2242,7 +2270,7 @@ int lv_final_modularity(lv_ctx *c, double *q)
     LV_TRY(arena_alloc(c->arena, dc, c->n0));
     fill_i64(c->stream, (int64_t *)lc, c->n0, 0);
     fill_i64(c->stream, (int64_t *)dc, c->n0, 0);
-    launch_final_q(c->stream, c->d_src, c->d_dst, c->d_w, c->e0, c->d_final, lc, dc);
+    launch_final_q(c->stream, c->d_src, c->d_dst, wunits(c), c->e0, c->d_final, lc, dc);
     launch_qagg_ull(c->stream, c->n0, dc, lc, c->d_qpart, c->d_qacc);
     LV_HIP(hipGetLastError());
     LV_TRY(d2h(c, c->h_qacc, c->d_qacc, 1));

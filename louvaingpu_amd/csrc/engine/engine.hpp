@@ -284,13 +284,27 @@ int bits_for(uint64_t maxval);  // bits needed to hold values 0..maxval
 
 // ---------------------------------------------------------------------------
 // louvain_kernels.hip (host launchers)
+// Input edge weights in units 2^-s: 4 bytes each when every weight fits (int32
+// weights always do - they are their own units), else 8.
+struct WUnits {
+    const int64_t *w64 = nullptr;
+    const uint32_t *w32 = nullptr;
+    WUnits() = default;
+    WUnits(const int64_t *a, const uint32_t *b) : w64(a), w32(b) {}
+    __device__ __forceinline__ int64_t operator[](int64_t i) const
+    {
+        return w32 ? (int64_t)w32[i] : w64[i];
+    }
+};
+
 struct DevFlags {  // error flags and small scalars written by kernels
     int32_t err;              // bit 1: invalid input, bit 2: out of exact range,
                               // bit 4: CSR rows not strictly increasing,
                               // bit 8: src not sorted, bit 16: edge list not
                               // canonical and strictly sorted
     int32_t scale;            // weight unit exponent s
-    int32_t pad0, pad1;
+    uint32_t max_bits;        // float32 weights: bit pattern of the largest (non-negative) one
+    int32_t pad1;
     long long total_units;    // sum of (w == 0 ? 1 : w) in units
     long long m_units;        // sum of w in units
     unsigned long long max_units;  // largest weight in units
@@ -298,8 +312,9 @@ struct DevFlags {  // error flags and small scalars written by kernels
 };
 
 void launch_weight_exponent(hipStream_t s, const void *w, int wtype, int64_t e, DevFlags *f);
+// Units into wu (8 bytes), wu32 (4 bytes) or neither (int32 weights kept as they are).
 void launch_weight_units(hipStream_t s, const void *w, int wtype, int64_t e, int scale,
-                         int64_t *wu, DevFlags *f);
+                         int64_t *wu, uint32_t *wu32, DevFlags *f);
 void launch_check_endpoints(hipStream_t s, const int32_t *src, const int32_t *dst, int64_t e,
                             int32_t n, DevFlags *f);
 // An uploaded edge list as lv_load_edges sent it: sources raw, or as the row
@@ -322,10 +337,11 @@ struct LoadedEdges {
 constexpr int kDecTile = 4096;             // edges per decode tile
 constexpr int64_t kEncodeMinEdges = 1 << 18;  // smaller lists are copied as they are
 // Checks of an uploaded edge list (ids = positions): endpoints in [0, n) and the
-// weight checks of k_weight_exponent; int32 weights also to units (wu), as
-// k_weight_units does; offsets decoded to sources.
+// weight checks of k_weight_exponent, with int32 weights also the sums of
+// k_weight_units (they are their own units); offsets decoded to sources, and
+// weight bytes with their escapes to 32-bit units in wu32.
 void launch_load_checks(hipStream_t s, const LoadedEdges &L, int wtype, int64_t e, int32_t n,
-                        int64_t *wu, DevFlags *f);
+                        uint32_t *wu32, DevFlags *f);
 
 // Host side of lv_load_edges' upload.  The copy of dst goes out first; host
 // threads meanwhile pack int weights into bytes (escape list for 255 and up)
@@ -378,18 +394,18 @@ void launch_csr_entries(hipStream_t s, const int32_t *src, const int32_t *dst, i
 // Entries with the payload (col << wbits | w) when every weight fits in wbits
 // bits and every column in 64 - wbits.
 void launch_csr_entries_packed(hipStream_t s, const int32_t *src, const int32_t *dst,
-                               const int64_t *w, int64_t e, int32_t n, int wbits, uint32_t *keys,
+                               WUnits w, int64_t e, int32_t n, int wbits, uint32_t *keys,
                                uint64_t *payload);
 void launch_csr_unpack(hipStream_t s, const uint32_t *skeys, const uint64_t *spay, int64_t nnz,
                        int wbits, int32_t *col, int64_t *cw, uint32_t *cw32, int64_t *kdeg);
-void launch_self_loops(hipStream_t s, const int32_t *src, const int32_t *dst, const int64_t *w,
+void launch_self_loops(hipStream_t s, const int32_t *src, const int32_t *dst, WUnits w,
                        int64_t e, int64_t *kdeg, int64_t *selfsum, ull *lastloop);
 void launch_rowptr_search(hipStream_t s, const uint32_t *skeys, int64_t total, int32_t n,
                           int64_t *rowptr);
 void launch_csr_gather(hipStream_t s, const uint32_t *skeys, const uint32_t *scodes, int64_t nnz,
-                       const int32_t *src, const int32_t *dst, const int64_t *w, int32_t *col,
+                       const int32_t *src, const int32_t *dst, WUnits w, int32_t *col,
                        int64_t *cw, uint32_t *cw32, int64_t *kdeg);
-void launch_loop_gather(hipStream_t s, const ull *lastloop, const int64_t *w, int32_t n,
+void launch_loop_gather(hipStream_t s, const ull *lastloop, WUnits w, int32_t n,
                         int64_t *loopw);
 
 void launch_init_vertex(hipStream_t s, int32_t n, const int64_t *rowptr, const int64_t *kdeg,
@@ -424,14 +440,14 @@ int run_hub_sweep1(hipStream_t s, const SweepArgs &a, const HubArgs &h, int32_t 
 // (n + 1 ints) receive the self-loop edges of tile t and of source vertex v,
 // and after in-place exclusive scans tscan/vscan are what the later kernels read.
 constexpr int kEdgeTile = 2048;
-void launch_src_sorted(hipStream_t s, const int32_t *src, const int32_t *dst, const int64_t *w,
+void launch_src_sorted(hipStream_t s, const int32_t *src, const int32_t *dst, WUnits w,
                        int64_t e, int32_t n, int32_t *tself, int32_t *vself, int64_t *fs,
                        int64_t *kdeg, int64_t *selfsum, ull *lastloop, DevFlags *f);
-void launch_rev_entries(hipStream_t s, const int32_t *src, const int32_t *dst, const int64_t *w,
+void launch_rev_entries(hipStream_t s, const int32_t *src, const int32_t *dst, WUnits w,
                         const int32_t *tscan, int64_t e, int wbits, uint32_t *keys,
                         uint64_t *payload);
 void launch_csr_sorted_fill(hipStream_t s, bool check, const int32_t *src, const int32_t *dst,
-                            const int64_t *w, const int32_t *tscan, const int32_t *vscan,
+                            WUnits w, const int32_t *tscan, const int32_t *vscan,
                             int64_t e, int32_t n, int wbits, int vshift, int kb, const int64_t *fs,
                             const uint32_t *skey, const uint64_t *spay, int64_t nr,
                             const int64_t *rs, int64_t *rowptr, int64_t *ffwd, int32_t *col,
@@ -440,11 +456,11 @@ void launch_csr_sorted_fill(hipStream_t s, bool check, const int32_t *src, const
 // starts over such words sorted by their top bits (key = word >> kshift).
 // General path as one word per entry (row | col | w, row = n for self-loops);
 // the unpack also checks that every row's columns strictly increase (err bit 4).
-void launch_csr_entries1(hipStream_t s, const int32_t *src, const int32_t *dst, const int64_t *w,
+void launch_csr_entries1(hipStream_t s, const int32_t *src, const int32_t *dst, WUnits w,
                          int64_t e, int32_t n, int rb, int cb, uint64_t *words);
 void launch_csr_unpack1(hipStream_t s, const uint64_t *words, int64_t nnz, int rb, int cb,
                         int32_t *col, int64_t *cw, uint32_t *cw32, int64_t *kdeg, DevFlags *f);
-void launch_rev_entries1(hipStream_t s, const int32_t *src, const int32_t *dst, const int64_t *w,
+void launch_rev_entries1(hipStream_t s, const int32_t *src, const int32_t *dst, WUnits w,
                          const int32_t *tscan, int64_t e, int kb, uint64_t *words);
 void launch_row_starts64(hipStream_t s, const uint64_t *keys, int64_t total, int32_t n,
                          int kshift, int64_t *rowptr);
@@ -537,7 +553,7 @@ void launch_layout(hipStream_t s, const uint32_t *skey, const int32_t *perm, int
 void launch_gather_partition(hipStream_t s, const int32_t *perm, const int32_t *ids,
                              const int32_t *comm_by_pos, int32_t n, int32_t *node,
                              int32_t *comm);
-void launch_final_q(hipStream_t s, const int32_t *src, const int32_t *dst, const int64_t *w,
+void launch_final_q(hipStream_t s, const int32_t *src, const int32_t *dst, WUnits w,
                     int64_t e, const int32_t *comm, ull *lc, ull *dc);
 void launch_qagg_ull(hipStream_t s, int32_t n, const ull *dc, const ull *lc, QAcc *part, QAcc *acc);
 

@@ -13,39 +13,53 @@ namespace lv {
 // Input validation and weight units (the reference reads float weights and
 // truncates sums to int, LouvainCUDA/InitKernel.cu:119, StatusCUDA.h:31-41).
 
+// Float weights also leave the bit pattern of the largest one (valid weights
+// are non-negative, where bit patterns order as values), which bounds the
+// units and so decides their width.
 __global__ void k_weight_exponent(const void *w, int wtype, int64_t e, DevFlags *f)
 {
     int smax = 0;
     int bad = 0;
+    uint32_t mb = 0;
     GRID_STRIDE(i, e)
     {
         if (wtype == LV_WEIGHT_INT32) {
             if (((const int32_t *)w)[i] < 0) bad |= 1;
         } else {
-            int s = f32_unit_exponent(((const uint32_t *)w)[i]);
-            if (s < 0)
+            const uint32_t x = ((const uint32_t *)w)[i];
+            int s = f32_unit_exponent(x);
+            if (s < 0) {
                 bad |= 1;
-            else
+            } else {
                 smax = max(smax, s);
+                mb = max(mb, x & 0x7fffffffu);
+            }
         }
     }
     smax = wave_max_i(smax);
     bad = wave_or_i(bad);
+    mb = (uint32_t)wave_max_i((int)mb);
     if ((threadIdx.x & 63) == 0) {
         if (smax) atomicMax(&f->scale, smax);
+        if (mb) atomicMax(&f->max_bits, mb);
         if (bad) atomicOr(&f->err, 1);
     }
 }
 
+// Units of every weight into wu (8 bytes) or wu32 (4 bytes, when the largest
+// fits), or neither (int32 weights in place: only the sums and checks).
 __global__ __launch_bounds__(256) void k_weight_units(const void *w, int wtype, int64_t e, int scale,
-                                                     int64_t *wu, DevFlags *f)
+                                                     int64_t *wu, uint32_t *wu32, DevFlags *f)
 {
     ull tot = 0, msum = 0, umax = 0;
     int bad = 0;
     GRID_STRIDE(i, e)
     {
         const int64_t u = weight_units(((const uint32_t *)w)[i], wtype, scale, bad);
-        wu[i] = u;
+        if (wu32)
+            wu32[i] = (uint32_t)u;
+        else if (wu)
+            wu[i] = u;
         umax = max(umax, (ull)u);
         tot = sat_add(tot, (ull)(u == 0 ? 1 : u));
         msum = sat_add(msum, (ull)u);
@@ -118,11 +132,11 @@ void launch_weight_exponent(hipStream_t s, const void *w, int wtype, int64_t e, 
 }
 
 void launch_weight_units(hipStream_t s, const void *w, int wtype, int64_t e, int scale,
-                         int64_t *wu, DevFlags *f)
+                         int64_t *wu, uint32_t *wu32, DevFlags *f)
 {
     if (e > 0)
         hipLaunchKernelGGL(k_weight_units, grid_for(e, 256, 8192), 256, 0, s, w, wtype, e, scale,
-                           wu, f);
+                           wu, wu32, f);
 }
 
 void launch_check_endpoints(hipStream_t s, const int32_t *src, const int32_t *dst, int64_t e,
@@ -176,7 +190,7 @@ __global__ void k_csr_entries(const int32_t *src, const int32_t *dst, int64_t e,
 // Both entries of every edge as one word, row << (64 - rb) | col << wb | w with
 // rb = bits of n (self-loops keyed n, past every row), cb = bits of n - 1 and
 // wb = 64 - rb - cb (when every weight fits wb bits).
-__global__ void k_csr_entries1(const int32_t *src, const int32_t *dst, const int64_t *w, int64_t e,
+__global__ void k_csr_entries1(const int32_t *src, const int32_t *dst, WUnits w, int64_t e,
                                int32_t n, int rb, int cb, uint64_t *words)
 {
     const int wb = 64 - rb - cb;
@@ -231,7 +245,7 @@ __global__ __launch_bounds__(256) void k_csr_unpack1(const uint64_t *words, int6
     block_flag_or(bad, f);
 }
 
-void launch_csr_entries1(hipStream_t s, const int32_t *src, const int32_t *dst, const int64_t *w,
+void launch_csr_entries1(hipStream_t s, const int32_t *src, const int32_t *dst, WUnits w,
                          int64_t e, int32_t n, int rb, int cb, uint64_t *words)
 {
     if (e > 0)
@@ -247,7 +261,7 @@ void launch_csr_unpack1(hipStream_t s, const uint64_t *words, int64_t nnz, int r
                            nnz, rb, cb, col, cw, cw32, kdeg, f);
 }
 
-__global__ void k_csr_entries_packed(const int32_t *src, const int32_t *dst, const int64_t *w,
+__global__ void k_csr_entries_packed(const int32_t *src, const int32_t *dst, WUnits w,
                                      int64_t e, int32_t n, int wbits, uint32_t *keys,
                                      uint64_t *payload)
 {
@@ -292,7 +306,7 @@ __global__ __launch_bounds__(256) void k_csr_unpack(const uint32_t *skeys, const
 }
 
 void launch_csr_entries_packed(hipStream_t s, const int32_t *src, const int32_t *dst,
-                               const int64_t *w, int64_t e, int32_t n, int wbits, uint32_t *keys,
+                               WUnits w, int64_t e, int32_t n, int wbits, uint32_t *keys,
                                uint64_t *payload)
 {
     if (e > 0)
@@ -308,7 +322,7 @@ void launch_csr_unpack(hipStream_t s, const uint32_t *skeys, const uint64_t *spa
                            wbits, col, cw, cw32, kdeg);
 }
 
-__global__ void k_self_loops(const int32_t *src, const int32_t *dst, const int64_t *w, int64_t e,
+__global__ void k_self_loops(const int32_t *src, const int32_t *dst, WUnits w, int64_t e,
                              int64_t *kdeg, int64_t *selfsum, ull *lastloop)
 {
     GRID_STRIDE(i, e)
@@ -413,7 +427,7 @@ __device__ __forceinline__ int32_t tile_self_base(const int32_t *tscan, int64_t 
 // a sorted list, fs[v] = the first edge with src >= v between the first and
 // the last source (k_range_ends writes the two ends).
 __global__ __launch_bounds__(256) void k_src_sorted(const int32_t *src, const int32_t *dst,
-                                                   const int64_t *w, int64_t e, int32_t *tself,
+                                                   WUnits w, int64_t e, int32_t *tself,
                                                    int32_t *vself, int64_t *fs, int64_t *kdeg,
                                                    int64_t *selfsum, ull *lastloop, DevFlags *f)
 {
@@ -464,7 +478,7 @@ __global__ __launch_bounds__(256) void k_src_sorted(const int32_t *src, const in
 // otherwise key dst and payload src << wbits | w.
 template <bool ONE>
 __global__ __launch_bounds__(256) void k_rev_entries(const int32_t *src, const int32_t *dst,
-                                                    const int64_t *w, const int32_t *tscan,
+                                                    WUnits w, const int32_t *tscan,
                                                     int64_t e, int bits, uint32_t *keys,
                                                     uint64_t *out)
 {
@@ -588,7 +602,7 @@ __global__ __launch_bounds__(256) void k_fill_rev(const uint32_t *skey, const ui
 
 template <bool CHECK>
 __global__ __launch_bounds__(256) void k_fill_fwd(const int32_t *src, const int32_t *dst,
-                                                 const int64_t *w, const int32_t *tscan,
+                                                 WUnits w, const int32_t *tscan,
                                                  const int32_t *vscan, int64_t e,
                                                  const int64_t *fs, const int64_t *rs,
                                                  const uint64_t *spay, int wbits, uint64_t cmask,
@@ -662,7 +676,7 @@ __global__ __launch_bounds__(256) void k_fill_fwd(const int32_t *src, const int3
     if constexpr (CHECK) block_flag_or(bad, f);
 }
 
-void launch_src_sorted(hipStream_t s, const int32_t *src, const int32_t *dst, const int64_t *w,
+void launch_src_sorted(hipStream_t s, const int32_t *src, const int32_t *dst, WUnits w,
                        int64_t e, int32_t n, int32_t *tself, int32_t *vself, int64_t *fs,
                        int64_t *kdeg, int64_t *selfsum, ull *lastloop, DevFlags *f)
 {
@@ -677,7 +691,7 @@ void launch_src_sorted(hipStream_t s, const int32_t *src, const int32_t *dst, co
     }
 }
 
-void launch_rev_entries(hipStream_t s, const int32_t *src, const int32_t *dst, const int64_t *w,
+void launch_rev_entries(hipStream_t s, const int32_t *src, const int32_t *dst, WUnits w,
                         const int32_t *tscan, int64_t e, int wbits, uint32_t *keys,
                         uint64_t *payload)
 {
@@ -689,7 +703,7 @@ void launch_rev_entries(hipStream_t s, const int32_t *src, const int32_t *dst, c
 // Reverse entries as (skey, spay = src << wbits | w), or with skey null as one
 // word spay = row << vshift | src << wbits | w; kb = bits of the largest id.
 void launch_csr_sorted_fill(hipStream_t s, bool check, const int32_t *src, const int32_t *dst,
-                            const int64_t *w, const int32_t *tscan, const int32_t *vscan,
+                            WUnits w, const int32_t *tscan, const int32_t *vscan,
                             int64_t e, int32_t n, int wbits, int vshift, int kb, const int64_t *fs,
                             const uint32_t *skey, const uint64_t *spay, int64_t nr,
                             const int64_t *rs, int64_t *rowptr, int64_t *ffwd, int32_t *col,
@@ -719,7 +733,7 @@ void launch_csr_sorted_fill(hipStream_t s, bool check, const int32_t *src, const
     }
 }
 
-void launch_rev_entries1(hipStream_t s, const int32_t *src, const int32_t *dst, const int64_t *w,
+void launch_rev_entries1(hipStream_t s, const int32_t *src, const int32_t *dst, WUnits w,
                          const int32_t *tscan, int64_t e, int kb, uint64_t *words)
 {
     if (e > 0)
@@ -763,7 +777,7 @@ __global__ void k_row_starts(const K *keys, int64_t total, int32_t n, int kshift
 
 __global__ __launch_bounds__(256) void k_csr_gather(const uint32_t *skeys, const uint32_t *scodes,
                                                    int64_t nnz, const int32_t *src,
-                                                   const int32_t *dst, const int64_t *w,
+                                                   const int32_t *dst, WUnits w,
                                                    int32_t *col, int64_t *cw, uint32_t *cw32, int64_t *kdeg)
 {
     // four entries per thread, 256 apart, so each wave covers 64 consecutive
@@ -805,7 +819,7 @@ __global__ __launch_bounds__(256) void k_csr_gather(const uint32_t *skeys, const
     }
 }
 
-__global__ void k_loop_gather(const ull *lastloop, const int64_t *w, int32_t n, int64_t *loopw)
+__global__ void k_loop_gather(const ull *lastloop, WUnits w, int32_t n, int64_t *loopw)
 {
     GRID_STRIDE(i, n)
     {
@@ -822,7 +836,7 @@ void launch_csr_entries(hipStream_t s, const int32_t *src, const int32_t *dst, i
                            keys, codes);
 }
 
-void launch_self_loops(hipStream_t s, const int32_t *src, const int32_t *dst, const int64_t *w,
+void launch_self_loops(hipStream_t s, const int32_t *src, const int32_t *dst, WUnits w,
                        int64_t e, int64_t *kdeg, int64_t *selfsum, ull *lastloop)
 {
     if (e > 0)
@@ -843,7 +857,7 @@ void launch_rowptr_search(hipStream_t s, const uint32_t *skeys, int64_t total, i
 }
 
 void launch_csr_gather(hipStream_t s, const uint32_t *skeys, const uint32_t *scodes, int64_t nnz,
-                       const int32_t *src, const int32_t *dst, const int64_t *w, int32_t *col,
+                       const int32_t *src, const int32_t *dst, WUnits w, int32_t *col,
                        int64_t *cw, uint32_t *cw32, int64_t *kdeg)
 {
     if (nnz > 0)
@@ -851,7 +865,7 @@ void launch_csr_gather(hipStream_t s, const uint32_t *skeys, const uint32_t *sco
                            nnz, src, dst, w, col, cw, cw32, kdeg);
 }
 
-void launch_loop_gather(hipStream_t s, const ull *lastloop, const int64_t *w, int32_t n,
+void launch_loop_gather(hipStream_t s, const ull *lastloop, WUnits w, int32_t n,
                         int64_t *loopw)
 {
     if (n > 0) hipLaunchKernelGGL(k_loop_gather, grid_for(n, 256), 256, 0, s, lastloop, w, n, loopw);
@@ -2180,7 +2194,7 @@ constexpr int kFqLog2 = 12;
 constexpr int kFqChunk = 2048;
 
 __global__ __launch_bounds__(256) void k_final_q(const int32_t *src, const int32_t *dst,
-                                                const int64_t *w, int64_t e, const int32_t *comm,
+                                                WUnits w, int64_t e, const int32_t *comm,
                                                 ull *lc, ull *dc)
 {
     constexpr int CAP = 1 << kFqLog2;
@@ -2284,7 +2298,7 @@ void launch_layout(hipStream_t s, const uint32_t *skey, const int32_t *perm, int
                        count, index_in_cluster, x, y);
 }
 
-void launch_final_q(hipStream_t s, const int32_t *src, const int32_t *dst, const int64_t *w,
+void launch_final_q(hipStream_t s, const int32_t *src, const int32_t *dst, WUnits w,
                     int64_t e, const int32_t *comm, ull *lc, ull *dc)
 {
     if (e > 0)

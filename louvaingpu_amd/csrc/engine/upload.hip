@@ -393,15 +393,17 @@ __device__ __forceinline__ int32_t last_le(const A &o, int32_t hi, int64_t i)
 
 // One pass over the edges in tiles of kDecTile: decodes the sources (SOFF),
 // checks the endpoints, and checks the weights as k_weight_exponent does (int32
-// weights also to units, as k_weight_units does with exponent 0).  Escaped
-// bytes (W8, value 255) are left to k_load_escapes.
+// weights also summed as k_weight_units does with exponent 0: they are their
+// own units, and weight bytes (W8) are widened into wu32).  Escaped bytes
+// (W8, value 255) are left to k_load_escapes.
 template <bool SOFF, bool W8, typename O>
 __global__ __launch_bounds__(256) void k_load_checks(LoadedEdges L, const O *off, int wtype,
-                                                    int64_t e, int32_t n, int64_t *wu,
+                                                    int64_t e, int32_t n, uint32_t *wu32,
                                                     DevFlags *f)
 {
     __shared__ O s_off[SOFF ? kDecSpan : 1];
     int bad = 0, smax = 0;
+    uint32_t mb = 0;
     ull tot = 0, msum = 0, umax = 0;
     const int64_t ntiles = (e + kDecTile - 1) / kDecTile;
     for (int64_t k = blockIdx.x; k < ntiles; k += gridDim.x) {
@@ -429,7 +431,7 @@ __global__ __launch_bounds__(256) void k_load_checks(LoadedEdges L, const O *off
             if (W8) {
                 const uint32_t x = L.w8[i];
                 if (x != 255u) {
-                    wu[i] = (int64_t)x;
+                    wu32[i] = x;
                     umax = max(umax, (ull)x);
                     tot = sat_add(tot, (ull)(x == 0 ? 1 : x));
                     msum = sat_add(msum, (ull)x);
@@ -439,29 +441,34 @@ __global__ __launch_bounds__(256) void k_load_checks(LoadedEdges L, const O *off
                 if (wtype == LV_WEIGHT_INT32) {
                     if ((int32_t)x < 0) bad |= 1;
                     const int64_t u = weight_units(x, wtype, 0, bad);
-                    wu[i] = u;
                     umax = max(umax, (ull)u);
                     tot = sat_add(tot, (ull)(u == 0 ? 1 : u));
                     msum = sat_add(msum, (ull)u);
                 } else {
                     const int sx = f32_unit_exponent(x);
-                    if (sx < 0)
+                    if (sx < 0) {
                         bad |= 1;
-                    else
+                    } else {
                         smax = max(smax, sx);
+                        mb = max(mb, x & 0x7fffffffu);
+                    }
                 }
             }
         }
         if (SOFF) __syncthreads();  // s_off is refilled for the next tile
     }
     smax = wave_max_i(smax);
-    if ((threadIdx.x & 63) == 0 && smax) atomicMax(&f->scale, smax);
+    mb = (uint32_t)wave_max_i((int)mb);
+    if ((threadIdx.x & 63) == 0) {
+        if (smax) atomicMax(&f->scale, smax);
+        if (mb) atomicMax(&f->max_bits, mb);
+    }
     block_units_flush(tot, msum, umax, bad, f);
 }
 
 // The escaped int weights (255 and up, negative ones included) of a byte upload.
 __global__ __launch_bounds__(256) void k_load_escapes(const int64_t *ei, const uint32_t *ew,
-                                                     int64_t nesc, int64_t *wu, DevFlags *f)
+                                                     int64_t nesc, uint32_t *wu32, DevFlags *f)
 {
     int bad = 0;
     ull tot = 0, msum = 0, umax = 0;
@@ -470,7 +477,7 @@ __global__ __launch_bounds__(256) void k_load_escapes(const int64_t *ei, const u
         const uint32_t x = ew[j];
         if ((int32_t)x < 0) bad |= 1;
         const int64_t u = weight_units(x, LV_WEIGHT_INT32, 0, bad);
-        wu[ei[j]] = u;
+        wu32[ei[j]] = x;
         umax = max(umax, (ull)u);
         tot = sat_add(tot, (ull)(u == 0 ? 1 : u));
         msum = sat_add(msum, (ull)u);
@@ -480,7 +487,7 @@ __global__ __launch_bounds__(256) void k_load_escapes(const int64_t *ei, const u
 
 template <bool SOFF, typename O>
 static void launch_checks_w(hipStream_t s, const LoadedEdges &L, const O *off, int wtype, int64_t e,
-                            int32_t n, int64_t *wu, DevFlags *f, int grid)
+                            int32_t n, uint32_t *wu, DevFlags *f, int grid)
 {
     if (L.w8)
         hipLaunchKernelGGL((k_load_checks<SOFF, true, O>), grid, 256, 0, s, L, off, wtype, e, n,
@@ -491,7 +498,7 @@ static void launch_checks_w(hipStream_t s, const LoadedEdges &L, const O *off, i
 }
 
 void launch_load_checks(hipStream_t s, const LoadedEdges &L, int wtype, int64_t e, int32_t n,
-                        int64_t *wu, DevFlags *f)
+                        uint32_t *wu, DevFlags *f)
 {
     if (e <= 0) return;
     const int64_t ntiles = (e + kDecTile - 1) / kDecTile;
