@@ -81,6 +81,8 @@ struct lv_ctx {
     int32_t n = 0;
     int64_t nnz = 0, m = 0;
     int64_t *rowptr = nullptr;
+    int64_t *rowend_buf = nullptr;  // row ends of a level whose rows leave gaps (else rowptr + 1)
+    int64_t span = 0;               // entries of col/w (nnz when the rows are packed)
     int32_t *col = nullptr;
     int64_t *cw = nullptr;
     uint32_t *cw32 = nullptr;  // instead of cw when every weight of the level is below 2^32
@@ -361,6 +363,7 @@ void free_level_arrays(lv_ctx *c)
     Arena &a = c->arena;
     free_sweep_state(c);
     arena_free(a, c->rowptr);
+    arena_free(a, c->rowend_buf);
     arena_free(a, c->col);
     arena_free(a, c->cw);
     arena_free(a, c->cw32);
@@ -455,6 +458,9 @@ uint64_t f32_units_host(uint32_t x, int scale)
     if (sh > 38) return 1ull << 62;
     return sh >= 0 ? (M << sh) : (M >> (-sh));
 }
+
+// End of every row of the current level.
+const int64_t *row_end(const lv_ctx *c) { return c->rowend_buf ? c->rowend_buf : c->rowptr + 1; }
 
 // The input weights in units (4 or 8 bytes each).
 WUnits wunits(const lv_ctx *c) { return WUnits(c->d_w, c->d_w32); }
@@ -717,7 +723,7 @@ int plan_hubs(lv_ctx *c, int32_t nh)
     LV_TRY(arena_alloc(A, pbase, (int64_t)nh + 1));
     LV_TRY(arena_alloc(A, cbase, (int64_t)nh + 1));
     LV_TRY(arena_alloc(A, hbase, (int64_t)nh + 1));
-    launch_hub_setup(s, hv, nh, c->rowptr, c->selfsum, c->hub_part_target, c->chunk_mul, hq, hP,
+    launch_hub_setup(s, hv, nh, c->rowptr, row_end(c), c->selfsum, c->hub_part_target, c->chunk_mul, hq, hP,
                      hP64, hch, hpc);
     h.chunk_mul = c->chunk_mul;
     LV_TRY(scan_exclusive_i64(A, s, hq, ebase, nh, ebase + nh));
@@ -798,7 +804,7 @@ int init_level(lv_ctx *c)
     LV_TRY(arena_alloc(A, v1, n));
     LV_TRY(arena_alloc(A, v2, n));
     LV_TRY(zero_small(c, 2 * (kTiers + 1) + 1));
-    launch_init_vertex(s, n, c->rowptr, c->kdeg, c->loopw, c->selfsum, c->deg, c->in, c->size,
+    launch_init_vertex(s, n, c->rowptr, row_end(c), c->kdeg, c->loopw, c->selfsum, c->deg, c->in, c->size,
                        c->ddeg, c->dsize, c->C, c->next, tkey, v1, c->d_small,
                        c->d_small + (kTiers + 1), (ull *)(c->d_small + 2 * (kTiers + 1)),
                        c->deg32, c->hub_q);
@@ -857,6 +863,7 @@ int one_level(lv_ctx *c, double lower, double *q_out, int32_t *passes_out)
     a.lab8 = c->narrow ? c->lab8 : nullptr;
     a.deg32 = c->narrow ? c->deg32 : nullptr;
     a.rowptr = c->rowptr;
+    a.rowend = row_end(c);
     a.col = c->col;
     a.w = c->cw;
     a.w32 = c->cw32;
@@ -1032,7 +1039,7 @@ int induce(lv_ctx *c, const int32_t *newc, const int32_t *newid_g, int32_t Kn)
     ull *selfl = nullptr, *ssum = nullptr, *internal = nullptr;
     int32_t *ey = nullptr, *bound = nullptr, *uy = nullptr, *nuniq = nullptr;
     int64_t *ew = nullptr, *uw = nullptr;
-    uint32_t *ew32 = nullptr;  // narrow level: every weight is below 2^32
+    uint32_t *ew32 = nullptr, *uw32 = nullptr;  // narrow level: every weight is below 2^32
     int64_t *mstart = nullptr;
     const bool nar = c->narrow;
     LV_TRY(arena_alloc(A, mdeg, n));
@@ -1047,10 +1054,16 @@ int induce(lv_ctx *c, const int32_t *newc, const int32_t *newid_g, int32_t Kn)
         LV_TRY(arena_alloc(A, ew32, nnz));
     else
         LV_TRY(arena_alloc(A, ew, nnz));
+    // uy/uw become the next level's col/w: segment g's uniques at sb[g] are row
+    // newid_g[g] (the total degree bounds every weight and is the same at every
+    // level, so a narrow level's coarse rows keep 4-byte weights)
     LV_TRY(arena_alloc(A, uy, nnz));
-    LV_TRY(arena_alloc(A, uw, nnz));
+    if (nar)
+        LV_TRY(arena_alloc(A, uw32, nnz));
+    else
+        LV_TRY(arena_alloc(A, uw, nnz));
     LV_TRY(arena_alloc(A, bound, nnz / 1024 + 2));
-    launch_cr_memdeg(s, L.perm, c->rowptr, n, mdeg, mstart);
+    launch_cr_memdeg(s, L.perm, c->rowptr, row_end(c), n, mdeg, mstart);
     LV_TRY(scan_exclusive_i64(A, s, mdeg, estart, n, estart + n));
     fill_i64(s, (int64_t *)selfl, K, 0);
     fill_i64(s, (int64_t *)ssum, K, 0);
@@ -1091,6 +1104,7 @@ int induce(lv_ctx *c, const int32_t *newc, const int32_t *newid_g, int32_t Kn)
     ca.newid_g = newid_g;
     ca.uy = uy;
     ca.uw = uw;
+    ca.uw32 = uw32;
     ca.nuniq = nuniq;
     ca.internal = internal;
     ca.ssum = ssum;
@@ -1161,46 +1175,40 @@ int induce(lv_ctx *c, const int32_t *newc, const int32_t *newid_g, int32_t Kn)
         LV_TRY(arena_alloc(A, B.pre, nparts));
         LV_TRY(run_cr_big(A, s, ca, B, nbs, nparts, nchunks));
     }
-    // next level rows
-    int64_t *rowcnt = nullptr, *rowptr2 = nullptr, *kdeg2 = nullptr, *loop2 = nullptr,
-            *self2 = nullptr, *cw2 = nullptr;
-    uint32_t *cw2_32 = nullptr;
-    int32_t *col2 = nullptr;
-    LV_TRY(arena_alloc(A, rowcnt, (int64_t)Kn + 1));
+    // next level: row x = newid_g[g] is [sb[g], sb[g] + nuniq[g]) of uy/uw; the
+    // rows keep their segments' places (a segment holds at least its uniques),
+    // so no copy packs them
+    int64_t *rowptr2 = nullptr, *rowend2 = nullptr, *kdeg2 = nullptr, *loop2 = nullptr,
+            *self2 = nullptr;
     LV_TRY(arena_alloc(A, rowptr2, (int64_t)Kn + 1));
+    LV_TRY(arena_alloc(A, rowend2, Kn));
     LV_TRY(arena_alloc(A, kdeg2, Kn));
     LV_TRY(arena_alloc(A, loop2, Kn));
     LV_TRY(arena_alloc(A, self2, Kn));
-    launch_cr_rowcnt(s, newid_g, nuniq, K, rowcnt);
-    LV_TRY(scan_exclusive_i64(A, s, rowcnt, rowptr2, Kn, rowptr2 + Kn));
-    fill_i64(s, (int64_t *)c->d_small + 1, 1, 0);
-    launch_cr_vertex(s, newid_g, ssum, internal, selfl, K, loop2, self2, kdeg2, c->d_small + 1);
-    LV_TRY(d2h(c, c->h_small, (const long long *)(rowptr2 + Kn), 1));
-    LV_TRY(d2h(c, c->h_small + 1, c->d_small + 1, 1));
+    fill_i64(s, rowptr2 + Kn, 1, nnz);
+    fill_i64(s, (int64_t *)c->d_small + 1, 2, 0);
+    launch_cr_vertex(s, newid_g, ssum, internal, selfl, sb, nuniq, K, loop2, self2, kdeg2,
+                     rowptr2, rowend2, c->d_small + 1, c->d_small + 2);
+    launch_cr_copy_big(s, ca, B, nparts);
+    LV_TRY(d2h(c, c->h_small + 1, c->d_small + 1, 2));
     LV_TRY(sync(c));
-    const int64_t nnz2 = c->h_small[0];
     const int64_t m2 = c->h_small[1] / 2;
-    LV_TRY(arena_alloc(A, col2, nnz2));
-    // the total degree bounds every weight and is the same at every level, so
-    // a narrow level's coarse rows keep 4-byte weights
-    if (nar)
-        LV_TRY(arena_alloc(A, cw2_32, nnz2));
-    else
-        LV_TRY(arena_alloc(A, cw2, nnz2));
-    launch_cr_copy(s, ca, B, K, nparts, rowptr2, nnz2, col2, cw2, cw2_32);
+    const int64_t nnz2 = c->h_small[2];
     LV_HIP(hipGetLastError());
     t_end(c, t0, CAT_COARSEN);
     // release the transient buffers (later launches on this stream may reuse them)
-    void *tmp[] = {mdeg, mstart, estart, selfl, ssum, internal, nuniq, ey, ew, ew32, uy, uw, bound, sb, slen,
+    void *tmp[] = {mdeg, mstart, estart, selfl, ssum, internal, nuniq, ey, ew, ew32, bound, sb, slen,
                    tk1, tk2, sid1, sid2, bP, bP64, bch, pbase, cbase, part_big, chunk_big, chunk_k,
                    B.pcount, B.poff, B.pcur, B.pe, B.pe8, B.buy, B.buw, B.fbk, B.fbv, B.ucnt, B.pre,
-                   B.chist, bhbase, rowcnt};
+                   B.chist, bhbase};
     for (void *p : tmp) A.release(p);
     free_level_arrays(c);
     c->rowptr = rowptr2;
-    c->col = col2;
-    c->cw = cw2;
-    c->cw32 = cw2_32;
+    c->rowend_buf = rowend2;
+    c->span = nnz;
+    c->col = uy;
+    c->cw = uw;
+    c->cw32 = uw32;
     c->kdeg = kdeg2;
     c->loopw = loop2;
     c->selfsum = self2;
@@ -1423,6 +1431,7 @@ int batch_one_level(lv_ctx *c, const std::vector<double> &lower, const std::vect
     a.lab8 = c->narrow ? c->lab8 : nullptr;
     a.deg32 = c->narrow ? c->deg32 : nullptr;
     a.rowptr = c->rowptr;
+    a.rowend = row_end(c);
     a.col = c->col;
     a.w = c->cw;
     a.w32 = c->cw32;
@@ -2368,6 +2377,31 @@ int lv_level_csr(lv_ctx *c, int64_t *rowptr, int32_t *col, int64_t *w)
     if (c->state < ST_CSR || c->state > ST_RENUMBERED || !c->rowptr) return LV_ERR_ARG;
     if (!rowptr || (c->nnz > 0 && (!col || !w))) return LV_ERR_ARG;
     LV_TRY(set_device(c));
+    if (c->rowend_buf) {
+        // a coarsened level: its rows keep their segments' places; packed here
+        const int64_t n = c->n, sp = c->span;
+        std::vector<int64_t> rb((size_t)n), re((size_t)n), ww((size_t)sp);
+        std::vector<int32_t> cc((size_t)sp);
+        std::vector<uint32_t> w32(c->cw32 ? (size_t)sp : 0);
+        LV_TRY(d2h(c, rb.data(), c->rowptr, n));
+        LV_TRY(d2h(c, re.data(), c->rowend_buf, n));
+        LV_TRY(d2h(c, cc.data(), c->col, sp));
+        if (c->cw32)
+            LV_TRY(d2h(c, w32.data(), c->cw32, sp));
+        else
+            LV_TRY(d2h(c, ww.data(), c->cw, sp));
+        LV_TRY(sync(c));
+        int64_t o = 0;
+        for (int64_t v = 0; v < n; ++v) {
+            rowptr[v] = o;
+            for (int64_t p = rb[(size_t)v]; p < re[(size_t)v]; ++p, ++o) {
+                col[o] = cc[(size_t)p];
+                w[o] = c->cw32 ? (int64_t)w32[(size_t)p] : ww[(size_t)p];
+            }
+        }
+        rowptr[n] = o;
+        return o == c->nnz ? LV_OK : LV_ERR_DEVICE;
+    }
     LV_TRY(d2h(c, rowptr, c->rowptr, (int64_t)c->n + 1));
     LV_TRY(d2h(c, col, c->col, c->nnz));
     if (c->cw32) {  // widened on the host

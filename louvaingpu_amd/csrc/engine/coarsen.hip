@@ -102,14 +102,14 @@ void launch_cr_newc(hipStream_t s, const int32_t *head, const int32_t *hscan, co
 
 // Per member j (member order): its row length and row start, so the gather
 // below reaches a member's entries without dependent lookups.
-__global__ void k_cr_memdeg(const int32_t *perm, const int64_t *rowptr, int32_t n, int64_t *mdeg,
-                            int64_t *mstart)
+__global__ void k_cr_memdeg(const int32_t *perm, const int64_t *rowptr, const int64_t *rowend,
+                            int32_t n, int64_t *mdeg, int64_t *mstart)
 {
     GRID_STRIDE(j, n)
     {
         int32_t a = perm[j];
         const int64_t b = rowptr[a];
-        mdeg[j] = rowptr[a + 1] - b;
+        mdeg[j] = rowend[a] - b;
         mstart[j] = b;
     }
 }
@@ -232,11 +232,12 @@ __global__ __launch_bounds__(256) void k_cr_gather(const int64_t *estart, const 
     }
 }
 
-void launch_cr_memdeg(hipStream_t s, const int32_t *perm, const int64_t *rowptr, int32_t n,
-                      int64_t *mdeg, int64_t *mstart)
+void launch_cr_memdeg(hipStream_t s, const int32_t *perm, const int64_t *rowptr,
+                      const int64_t *rowend, int32_t n, int64_t *mdeg, int64_t *mstart)
 {
     if (n > 0)
-        hipLaunchKernelGGL(k_cr_memdeg, grid_for(n, 256), 256, 0, s, perm, rowptr, n, mdeg, mstart);
+        hipLaunchKernelGGL(k_cr_memdeg, grid_for(n, 256), 256, 0, s, perm, rowptr, rowend, n, mdeg,
+                           mstart);
 }
 
 void launch_cr_selfl(hipStream_t s, const int32_t *perm, const int64_t *selfsum,
@@ -310,7 +311,7 @@ void launch_cr_seg_tier(hipStream_t s, const int32_t *gstart, const int64_t *est
 // Ends with a barrier: the caller may reuse the table at once.
 template <int SPT, typename V>
 __device__ __forceinline__ int table_compact(int32_t *key, const V *val, int cap, int32_t *oy,
-                                             int64_t *ow)
+                                             int64_t *ow, uint32_t *ow32)
 {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     int32_t ky[SPT];
@@ -341,7 +342,7 @@ __device__ __forceinline__ int table_compact(int32_t *key, const V *val, int cap
     for (int k = 0; k < SPT; ++k)
         if (ky[k] >= 0) {
             oy[pos] = ky[k];
-            ow[pos] = (int64_t)vv[k];
+            put_w(ow, ow32, pos, (int64_t)vv[k]);
             ++pos;
         }
     __syncthreads();
@@ -412,7 +413,7 @@ __global__ __launch_bounds__(256) void k_cr_agg_group(CoarsenArgs a, const int32
                     if (y >= 0) {
                         const int pos = nun + __popcll(m & below);
                         a.uy[b + pos] = y;
-                        a.uw[b + pos] = (int64_t)val[s];
+                        put_w(a.uw, a.uw32, b + pos, (int64_t)val[s]);
                     }
                     nun += __popcll(m);
                 }
@@ -422,7 +423,7 @@ __global__ __launch_bounds__(256) void k_cr_agg_group(CoarsenArgs a, const int32
                     if (y >= 0) {
                         int pos = atomicAdd(&s_cnt[grp], 1);
                         a.uy[b + pos] = y;
-                        a.uw[b + pos] = (int64_t)val[s];
+                        put_w(a.uw, a.uw32, b + pos, (int64_t)val[s]);
                     }
                 }
             }
@@ -475,7 +476,9 @@ __global__ __launch_bounds__(256) void k_cr_agg_block(CoarsenArgs a, const int32
                 lds_insert(s_key, s_val, l2, y, w);
         }
         __syncthreads();
-        const int nun = table_compact<CAP / 256>(s_key, s_val, cap, a.uy + b, a.uw + b);
+        const int nun = table_compact<CAP / 256>(s_key, s_val, cap, a.uy + b,
+                                                 a.uw32 ? nullptr : a.uw + b,
+                                                 a.uw32 ? a.uw32 + b : nullptr);
         inter = (ull)wave_sum((long long)inter);
         tot = (ull)wave_sum((long long)tot);
         ull *red = reinterpret_cast<ull *>(s_key);  // the table is free again
@@ -665,7 +668,7 @@ __global__ __launch_bounds__(256) void k_cr_big_agg(BigSegArgs B)
             if (e0 + u * 256 < end) (void)lds_insert_bounded(s_key, s_val, l2, en[u].c, en[u].w);
     }
     __syncthreads();
-    int total = table_compact<CAP / 256>(s_key, s_val, cap, B.buy + beg, B.buw + beg);
+    int total = table_compact<CAP / 256>(s_key, s_val, cap, B.buy + beg, B.buw + beg, nullptr);
     // An insert that found no slot left the table full, which more than CAP
     // entries can do: then (or when exactly CAP keys filled it) the partition
     // is redone in its global table, whose uniques overwrite these.
@@ -749,60 +752,37 @@ int run_cr_big(Arena &A, hipStream_t s, const CoarsenArgs &a, const BigSegArgs &
 }
 
 // ---------------------------------------------------------------------------
-// next level: row counts, row copies, per-vertex k / loop / selfsum
+// next level: big segments' rows, per-vertex k / loop / selfsum and row bounds
 
-__global__ void k_cr_rowcnt(const int32_t *newid_g, const int32_t *nuniq, int32_t K, int64_t *rowcnt)
-{
-    GRID_STRIDE(g, K)
-    {
-        if (newid_g[g] >= 0) rowcnt[newid_g[g]] = nuniq[g];
-    }
-}
-
-// L lanes per row, chosen by the level's average row length (R-MAT's small
-// segments average about ten uniques, the SBM sets' hundreds).
-template <int L>
-__global__ void k_cr_copy_small(CoarsenArgs a, const int64_t *rowptr2, int32_t K, int32_t *col2,
-                                int64_t *cw2, uint32_t *cw2_32)
-{
-    const int lg = threadIdx.x % L;
-    for (int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / L; g < K;
-         g += ((int64_t)gridDim.x * blockDim.x) / L) {
-        if (a.slen[g] > a.big_q) continue;  // big segments copy per partition
-        if (a.newid_g[g] < 0) continue;                     // stopped component
-        const int64_t src = a.sb[g], dst = rowptr2[a.newid_g[g]];
-        const int32_t len = a.nuniq[g];
-        for (int32_t t = lg; t < len; t += L) {
-            col2[dst + t] = a.uy[src + t];
-            put_w(cw2, cw2_32, dst + t, a.uw[src + t]);
-        }
-    }
-}
-
-// One wavefront per partition (about 500 uniques each).
-__global__ void k_cr_copy_big(CoarsenArgs a, BigSegArgs B, const int64_t *rowptr2, int64_t nparts,
-                              int32_t *col2, int64_t *cw2, uint32_t *cw2_32)
+// A big segment's uniques (per hash partition in buy/buw) into its own place
+// sb[g] of uy/uw, which is row newid_g[g] of the next level.  One wavefront
+// per partition (about 500 uniques each).
+__global__ void k_cr_copy_big(CoarsenArgs a, BigSegArgs B, int64_t nparts)
 {
     const int lg = threadIdx.x & 63;
     for (int64_t p = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; p < nparts;
          p += ((int64_t)gridDim.x * blockDim.x) >> 6) {
         const int32_t g = B.bsegs[B.part_big[p]];  // (big segments are never stopped groups)
-        const int64_t src = B.poff[p], dst = rowptr2[a.newid_g[g]] + B.pre[p];
+        const int64_t src = B.poff[p], dst = a.sb[g] + B.pre[p];
         const int32_t len = B.ucnt[p];
         for (int32_t t = lg; t < len; t += 64) {
-            col2[dst + t] = B.buy[src + t];
-            put_w(cw2, cw2_32, dst + t, B.buw[src + t]);
+            a.uy[dst + t] = B.buy[src + t];
+            put_w(a.uw, a.uw32, dst + t, B.buw[src + t]);
         }
     }
 }
 
+// Next-level vertex x = newid_g[g]: its loop, k and row [sb[g], sb[g] + nuniq[g])
+// of uy/uw; sums 2m and the level's entries.
 __global__ __launch_bounds__(256) void k_cr_vertex(const int32_t *newid_g, const ull *ssum,
-                                                  const ull *internal, const ull *selfl, int32_t K,
+                                                  const ull *internal, const ull *selfl,
+                                                  const int64_t *sb, const int32_t *nuniq, int32_t K,
                                                   int64_t *loop2, int64_t *self2, int64_t *kdeg2,
-                                                  long long *msum)
+                                                  int64_t *rowptr2, int64_t *rowend2,
+                                                  long long *msum, long long *nsum)
 {
-    __shared__ long long s_acc[4];
-    long long acc = 0;
+    __shared__ long long s_acc[4], s_nn[4];
+    long long acc = 0, nn = 0;
     GRID_STRIDE(g, K)
     {
         const int32_t x = newid_g[g];
@@ -811,49 +791,41 @@ __global__ __launch_bounds__(256) void k_cr_vertex(const int32_t *newid_g, const
         loop2[x] = lp;
         self2[x] = lp;
         kdeg2[x] = (int64_t)(ssum[g] - internal[g]) + 4 * lp;
+        rowptr2[x] = sb[g];
+        rowend2[x] = sb[g] + nuniq[g];
         acc += (long long)(ssum[g] + 2 * selfl[g]);
+        nn += nuniq[g];
     }
     acc = wave_sum(acc);  // the level's total is below 2^61: no wrap
-    if ((threadIdx.x & 63) == 0) s_acc[threadIdx.x >> 6] = acc;
+    nn = wave_sum(nn);
+    if ((threadIdx.x & 63) == 0) {
+        s_acc[threadIdx.x >> 6] = acc;
+        s_nn[threadIdx.x >> 6] = nn;
+    }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0) {  // one atomic per block on each shared total
         const long long t = s_acc[0] + s_acc[1] + s_acc[2] + s_acc[3];
-        if (t) atomicAdd((ull *)msum, (ull)t);  // one atomic per block on the shared total
+        if (t) atomicAdd((ull *)msum, (ull)t);
+        const long long u = s_nn[0] + s_nn[1] + s_nn[2] + s_nn[3];
+        if (u) atomicAdd((ull *)nsum, (ull)u);
     }
 }
 
-void launch_cr_rowcnt(hipStream_t s, const int32_t *newid_g, const int32_t *nuniq, int32_t K,
-                      int64_t *rowcnt)
+void launch_cr_copy_big(hipStream_t s, const CoarsenArgs &a, const BigSegArgs &B, int64_t nparts)
 {
-    if (K > 0) hipLaunchKernelGGL(k_cr_rowcnt, grid_for(K, 256), 256, 0, s, newid_g, nuniq, K, rowcnt);
-}
-
-void launch_cr_copy(hipStream_t s, const CoarsenArgs &a, const BigSegArgs &B, int32_t K,
-                    int64_t nparts, const int64_t *rowptr2, int64_t nnz2, int32_t *col2,
-                    int64_t *cw2, uint32_t *cw2_32)
-{
-    const int64_t avg = K > 0 ? nnz2 / K : 0;
-    if (K > 0 && avg >= 48)
-        hipLaunchKernelGGL(k_cr_copy_small<64>, grid_for((int64_t)K * 64, 256, 1 << 18), 256, 0, s,
-                           a, rowptr2, K, col2, cw2, cw2_32);
-    else if (K > 0 && avg >= 12)
-        hipLaunchKernelGGL(k_cr_copy_small<16>, grid_for((int64_t)K * 16, 256, 1 << 18), 256, 0, s,
-                           a, rowptr2, K, col2, cw2, cw2_32);
-    else if (K > 0)
-        hipLaunchKernelGGL(k_cr_copy_small<4>, grid_for((int64_t)K * 4, 256, 1 << 18), 256, 0, s,
-                           a, rowptr2, K, col2, cw2, cw2_32);
     if (nparts > 0)
         hipLaunchKernelGGL(k_cr_copy_big, grid_for(nparts * 64, 256, 1 << 18), 256, 0, s, a, B,
-                           rowptr2, nparts, col2, cw2, cw2_32);
+                           nparts);
 }
 
 void launch_cr_vertex(hipStream_t s, const int32_t *newid_g, const ull *ssum, const ull *internal,
-                      const ull *selfl, int32_t K, int64_t *loop2, int64_t *self2, int64_t *kdeg2,
-                      long long *msum)
+                      const ull *selfl, const int64_t *sb, const int32_t *nuniq, int32_t K,
+                      int64_t *loop2, int64_t *self2, int64_t *kdeg2, int64_t *rowptr2,
+                      int64_t *rowend2, long long *msum, long long *nsum)
 {
     if (K > 0)
         hipLaunchKernelGGL(k_cr_vertex, grid_for(K, 256, 4096), 256, 0, s, newid_g, ssum, internal,
-                           selfl, K, loop2, self2, kdeg2, msum);
+                           selfl, sb, nuniq, K, loop2, self2, kdeg2, rowptr2, rowend2, msum, nsum);
 }
 
 }  // namespace lv

@@ -134,7 +134,8 @@ struct SweepArgs {
     const Lab *lab;         // label records; narrow levels also lab8 and deg32
     const Lab8 *lab8;
     const uint32_t *deg32;  // deg[] of narrow levels
-    const int64_t *rowptr;
+    const int64_t *rowptr;  // row v is [rowptr[v], rowend[v]) of col/w (rowend = rowptr + 1
+    const int64_t *rowend;  //   when the rows are packed; coarsened levels leave gaps)
     const int32_t *col;
     const int64_t *w;       // row entry weights (units), or w32 when every weight fits 32 bits
     const uint32_t *w32;
@@ -217,8 +218,9 @@ struct CoarsenArgs {
     const int64_t *sb;      // first entry of group g
     const int64_t *slen;    // entries of group g
     const int32_t *newid_g; // new id of group g
-    int32_t *uy;            // uniques of segments up to 2048 entries, at sb[g]
-    int64_t *uw;
+    int32_t *uy;            // uniques of segment g at sb[g]: row newid_g[g] of the next
+    int64_t *uw;            //   level, whose rows keep the segments' places (weights in
+    uint32_t *uw32;         //   uw, or uw32 on narrow levels)
     int32_t *nuniq;         // uniques per group
     ull *internal;          // sum of w' over entries inside the group
     ull *ssum;              // sum of w' over all entries of the group
@@ -408,10 +410,10 @@ void launch_csr_gather(hipStream_t s, const uint32_t *skeys, const uint32_t *sco
 void launch_loop_gather(hipStream_t s, const ull *lastloop, WUnits w, int32_t n,
                         int64_t *loopw);
 
-void launch_init_vertex(hipStream_t s, int32_t n, const int64_t *rowptr, const int64_t *kdeg,
-                        const int64_t *loopw, const int64_t *selfsum, int64_t *deg, int64_t *in,
-                        int32_t *size, ull *ddeg, int32_t *dsize, int32_t *C,
-                        int32_t *next, uint32_t *tier_key, int32_t *verts,
+void launch_init_vertex(hipStream_t s, int32_t n, const int64_t *rowptr, const int64_t *rowend,
+                        const int64_t *kdeg, const int64_t *loopw, const int64_t *selfsum,
+                        int64_t *deg, int64_t *in, int32_t *size, ull *ddeg, int32_t *dsize,
+                        int32_t *C, int32_t *next, uint32_t *tier_key, int32_t *verts,
                         long long *tier_cnt, long long *tier_nnz, ull *ksum, uint32_t *deg32,
                         int32_t hub_q);
 // Q aggregates: per-block partials in part[kQParts], summed into *acc.
@@ -468,8 +470,9 @@ void launch_row_starts64(hipStream_t s, const uint64_t *keys, int64_t total, int
 void launch_rows_check(hipStream_t s, const uint32_t *skeys, const int32_t *col, int64_t nnz,
                        DevFlags *f);
 void launch_hub_setup(hipStream_t s, const int32_t *hv, int32_t nh, const int64_t *rowptr,
-                      const int64_t *selfsum, int32_t target, int chunk_mul, int64_t *hq,
-                      int32_t *hP, int64_t *hP64, int64_t *hch, int64_t *hpc);
+                      const int64_t *rowend, const int64_t *selfsum, int32_t target,
+                      int chunk_mul, int64_t *hq, int32_t *hP, int64_t *hP64, int64_t *hch,
+                      int64_t *hpc);
 // owner[i] = segment j of i under exclusive offsets base[0..nseg]; rank[i] =
 // i - base[j] (rank may be null).
 void launch_expand_owner(hipStream_t s, const int64_t *base, int32_t nseg, int64_t total,
@@ -518,8 +521,8 @@ void launch_cr_newid(hipStream_t s, const int32_t *gstart, const int32_t *perm, 
                      int32_t K, int32_t *newid_g);
 void launch_cr_newc(hipStream_t s, const int32_t *head, const int32_t *hscan, const int32_t *perm,
                     const int32_t *newid_g, int32_t n, int32_t *newc);
-void launch_cr_memdeg(hipStream_t s, const int32_t *perm, const int64_t *rowptr, int32_t n,
-                      int64_t *mdeg, int64_t *mstart);
+void launch_cr_memdeg(hipStream_t s, const int32_t *perm, const int64_t *rowptr,
+                      const int64_t *rowend, int32_t n, int64_t *mdeg, int64_t *mstart);
 void launch_cr_selfl(hipStream_t s, const int32_t *perm, const int64_t *selfsum,
                      const int32_t *head, const int32_t *hscan, int32_t n, ull *selfl);
 void launch_cr_gather(hipStream_t s, const int64_t *estart, int32_t *bound, const int64_t *mstart,
@@ -535,15 +538,13 @@ void launch_cr_big_setup(hipStream_t s, const int32_t *bsegs, int32_t nbs, const
                          int64_t *blen, int64_t *bpc);
 int run_cr_big(Arena &A, hipStream_t s, const CoarsenArgs &a, const BigSegArgs &B, int32_t nbs,
                int64_t nparts, int64_t nchunks);
-void launch_cr_rowcnt(hipStream_t s, const int32_t *newid_g, const int32_t *nuniq, int32_t K,
-                      int64_t *rowcnt);
-// Row copies into the next level's CSR (nnz2 entries over K groups).
-void launch_cr_copy(hipStream_t s, const CoarsenArgs &a, const BigSegArgs &B, int32_t K,
-                    int64_t nparts, const int64_t *rowptr2, int64_t nnz2, int32_t *col2,
-                    int64_t *cw2, uint32_t *cw2_32);
+// The big segments' uniques from their hash partitions into their rows.
+void launch_cr_copy_big(hipStream_t s, const CoarsenArgs &a, const BigSegArgs &B, int64_t nparts);
+// Per next-level vertex: loop, selfsum, k and row bounds; msum += 2m, nsum += entries.
 void launch_cr_vertex(hipStream_t s, const int32_t *newid_g, const ull *ssum, const ull *internal,
-                      const ull *selfl, int32_t K, int64_t *loop2, int64_t *self2, int64_t *kdeg2,
-                      long long *msum);
+                      const ull *selfl, const int64_t *sb, const int32_t *nuniq, int32_t K,
+                      int64_t *loop2, int64_t *self2, int64_t *kdeg2, int64_t *rowptr2,
+                      int64_t *rowend2, long long *msum, long long *nsum);
 
 void launch_compose(hipStream_t s, int32_t *f, int32_t n, const int32_t *map);
 // skey/perm: cluster keys sorted stably and the positions they came from.

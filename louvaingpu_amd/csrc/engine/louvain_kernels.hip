@@ -876,8 +876,9 @@ void launch_loop_gather(hipStream_t s, const ull *lastloop, WUnits w, int32_t n,
 // OneLevelKernel.cu:80-91) and the degree buckets of the sweep.
 
 __global__ __launch_bounds__(256) void k_init_vertex(
-    int32_t n, const int64_t *rowptr, const int64_t *kdeg, const int64_t *loopw,
-    const int64_t *selfsum, int64_t *deg, int64_t *in, int32_t *size, ull *ddeg, int32_t *dsize,
+    int32_t n, const int64_t *rowptr, const int64_t *rowend, const int64_t *kdeg,
+    const int64_t *loopw, const int64_t *selfsum, int64_t *deg, int64_t *in, int32_t *size,
+    ull *ddeg, int32_t *dsize,
     int32_t *C, int32_t *next, uint32_t *tier_key, int32_t *verts, long long *tier_cnt,
     long long *tier_nnz, ull *ksum, uint32_t *deg32, int32_t hub_q)
 {
@@ -906,7 +907,7 @@ __global__ __launch_bounds__(256) void k_init_vertex(
             dsize[i] = 0;
             C[i] = (int32_t)i;
             next[i] = (int32_t)i;
-            d = rowptr[i + 1] - rowptr[i];
+            d = rowend[i] - rowptr[i];
             t = tier_of(d, d + (selfsum[i] > 0 ? 1 : 0), hub_q);
             tier_key[i] = (uint32_t)t;
             verts[i] = (int32_t)i;
@@ -926,14 +927,15 @@ __global__ __launch_bounds__(256) void k_init_vertex(
     }
 }
 
-void launch_init_vertex(hipStream_t s, int32_t n, const int64_t *rowptr, const int64_t *kdeg,
-                        const int64_t *loopw, const int64_t *selfsum, int64_t *deg, int64_t *in,
-                        int32_t *size, ull *ddeg, int32_t *dsize, int32_t *C, int32_t *next,
-                        uint32_t *tier_key, int32_t *verts, long long *tier_cnt,
-                        long long *tier_nnz, ull *ksum, uint32_t *deg32, int32_t hub_q)
+void launch_init_vertex(hipStream_t s, int32_t n, const int64_t *rowptr, const int64_t *rowend,
+                        const int64_t *kdeg, const int64_t *loopw, const int64_t *selfsum,
+                        int64_t *deg, int64_t *in, int32_t *size, ull *ddeg, int32_t *dsize,
+                        int32_t *C, int32_t *next, uint32_t *tier_key, int32_t *verts,
+                        long long *tier_cnt, long long *tier_nnz, ull *ksum, uint32_t *deg32,
+                        int32_t hub_q)
 {
     if (n > 0)
-        hipLaunchKernelGGL(k_init_vertex, grid_for(n, 256, 1024), 256, 0, s, n, rowptr, kdeg,
+        hipLaunchKernelGGL(k_init_vertex, grid_for(n, 256, 1024), 256, 0, s, n, rowptr, rowend, kdeg,
                            loopw, selfsum, deg, in, size, ddeg, dsize, C, next, tier_key, verts,
                            tier_cnt, tier_nnz, ksum, deg32, hub_q);
 }
@@ -1443,7 +1445,7 @@ __global__ __launch_bounds__(256) void k_sweep_group(SweepArgs a, const int32_t 
             vp = vparam<B>(a, v);
             me = own_lab(a, v);
             beg = a.rowptr[v];
-            end = a.rowptr[v + 1];
+            end = a.rowend[v];
             loop = a.selfsum[v] > 0;
             if (vp.mode == 0) end = beg, loop = false;  // frozen: nothing to gather
             if (MINLOG2 < LOG2CAP) l2 = table_log2(end - beg + (loop ? 1 : 0), MINLOG2);
@@ -1503,7 +1505,7 @@ __global__ __launch_bounds__(BLOCK) void k_sweep_block(SweepArgs a, const int32_
         const VParam vp = vparam<B>(a, v);
         const Lab me = own_lab(a, v);
         const int64_t beg = a.rowptr[v];
-        const int64_t end = vp.mode == 0 ? beg : a.rowptr[v + 1];  // frozen: nothing to gather
+        const int64_t end = vp.mode == 0 ? beg : a.rowend[v];  // frozen: nothing to gather
         const bool loop = vp.mode != 0 && a.selfsum[v] > 0;
         const int l2 = table_log2(end - beg + (loop ? 1 : 0), MINLOG2);
         const int cap = 1 << l2;
@@ -1604,7 +1606,7 @@ __global__ __launch_bounds__(256) void k_sweep1_group(SweepArgs a, const int32_t
             v = verts[vi];
             vp = vparam<B>(a, v);
             const int64_t beg = a.rowptr[v];
-            const int64_t end = vp.mode == 0 ? beg : a.rowptr[v + 1];
+            const int64_t end = vp.mode == 0 ? beg : a.rowend[v];
             const double f = (double)a.kdeg[v] / vp.m2;
             for (int64_t p0 = beg + lg; p0 < end; p0 += 4 * G) {
                 int32_t j[4];
@@ -1644,7 +1646,7 @@ __global__ __launch_bounds__(BLOCK) void k_sweep1_block(SweepArgs a, const int32
         const int32_t v = verts[vi];
         const VParam vp = vparam<B>(a, v);
         const int64_t beg = a.rowptr[v];
-        const int64_t end = vp.mode == 0 ? beg : a.rowptr[v + 1];
+        const int64_t end = vp.mode == 0 ? beg : a.rowend[v];
         const double f = (double)a.kdeg[v] / vp.m2;
         int cnt = 0;
         double bg = 0.0;
@@ -1738,7 +1740,7 @@ __global__ __launch_bounds__(256) void k_hub1_chunk(SweepArgs a, HubArgs h)
     const int32_t j = h.chunk_hub[ci];
     const int32_t v = h.hv[j];
     const int64_t q = h.hq[j];
-    const int64_t rb = a.rowptr[v], d = a.rowptr[v + 1] - rb;
+    const int64_t rb = a.rowptr[v], d = a.rowend[v] - rb;
     const VParam vp = vparam<B>(a, v);
     const double f = (double)a.kdeg[v] / vp.m2;
     const int64_t cs = chunk_span(h.hP[j], h.chunk_mul);
@@ -1867,13 +1869,14 @@ void launch_rows_check(hipStream_t s, const uint32_t *skeys, const int32_t *col,
 // table per hub, so the work is streaming reads/writes plus LDS atomics.
 
 __global__ void k_hub_setup(const int32_t *hv, int32_t nh, const int64_t *rowptr,
-                            const int64_t *selfsum, int32_t target, int chunk_mul, int64_t *hq,
+                            const int64_t *rowend, const int64_t *selfsum, int32_t target,
+                            int chunk_mul, int64_t *hq,
                             int32_t *hP, int64_t *hP64, int64_t *hch, int64_t *hpc)
 {
     GRID_STRIDE(j, nh)
     {
         int32_t v = hv[j];
-        int64_t q = rowptr[v + 1] - rowptr[v] + (selfsum[v] > 0 ? 1 : 0);
+        int64_t q = rowend[v] - rowptr[v] + (selfsum[v] > 0 ? 1 : 0);
         int64_t P = (q + target - 1) / target;
         if (P < 1) P = 1;
         if (P > kHubMaxParts) P = kHubMaxParts;
@@ -1921,11 +1924,12 @@ void launch_widen_i32(hipStream_t s, const int32_t *in, int64_t n, int64_t *out)
 }
 
 void launch_hub_setup(hipStream_t s, const int32_t *hv, int32_t nh, const int64_t *rowptr,
-                      const int64_t *selfsum, int32_t target, int chunk_mul, int64_t *hq,
-                      int32_t *hP, int64_t *hP64, int64_t *hch, int64_t *hpc)
+                      const int64_t *rowend, const int64_t *selfsum, int32_t target,
+                      int chunk_mul, int64_t *hq, int32_t *hP, int64_t *hP64, int64_t *hch,
+                      int64_t *hpc)
 {
     if (nh > 0)
-        hipLaunchKernelGGL(k_hub_setup, grid_for(nh, 256), 256, 0, s, hv, nh, rowptr, selfsum,
+        hipLaunchKernelGGL(k_hub_setup, grid_for(nh, 256), 256, 0, s, hv, nh, rowptr, rowend, selfsum,
                            target, chunk_mul, hq, hP, hP64, hch, hpc);
 }
 
@@ -1939,7 +1943,7 @@ __global__ __launch_bounds__(256) void k_hub_count(SweepArgs a, HubArgs h)
     const int32_t v = h.hv[j];
     const int64_t q = h.hq[j];
     const int P = h.hP[j];
-    const int64_t rb = a.rowptr[v], d = a.rowptr[v + 1] - rb;
+    const int64_t rb = a.rowptr[v], d = a.rowend[v] - rb;
     const int32_t own = a.C[v];
     for (int t = threadIdx.x; t < P; t += 256) hist[t] = 0;
     __syncthreads();
@@ -1994,7 +1998,7 @@ __global__ __launch_bounds__(256) void k_hub_scatter(SweepArgs a, HubArgs h)
     const int32_t v = h.hv[j];
     const int64_t q = h.hq[j];
     const int P = h.hP[j];
-    const int64_t rb = a.rowptr[v], d = a.rowptr[v + 1] - rb;
+    const int64_t rb = a.rowptr[v], d = a.rowend[v] - rb;
     for (int t = threadIdx.x; t < P; t += 256) hist[t] = 0;
     __syncthreads();
     const int64_t cs = chunk_span(P, h.chunk_mul);
