@@ -2,8 +2,9 @@
 //
 // Layout in HBM (one arena per context, first-fit with coalescing):
 //   input   : ids[n0] (only when ids are not 0..n0-1), el_src/el_dst[e0] positions,
-//             el_w[e0] int64 units                      (persistent until the next load)
-//   level l : CSR rowptr[n+1] int64, col[nnz] int32, w[nnz] int64 (non-self entries);
+//             el_w[e0] units, 4 or 8 bytes              (persistent until the next load)
+//   level l : CSR rows [rowptr[v], rowend[v]) of col int32 / w units (non-self entries;
+//             level 0 packed, coarsened levels keep each row where it was aggregated);
 //             per vertex k, loop, selfsum (int64), C/next (int32 ping-pong),
 //             per community deg/in (int64), size (int32) and their deltas;
 //             vertex lists bucketed by row length for the sweep tiers
