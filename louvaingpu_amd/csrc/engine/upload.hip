@@ -354,7 +354,12 @@ int EdgeUploader::upload(hipStream_t s, int32_t n, int64_t e, const int32_t *src
 // ---------------------------------------------------------------------------
 // device: decode and check
 
-constexpr int kDecSpan = 4096;  // offsets of a decode tile held in LDS
+constexpr int kDecSpan = 4096;  // vertices of a decode tile scattered in LDS
+constexpr int kDecPer = kDecTile / 256;  // tile positions per thread in the scan
+
+// Tile position p in LDS, one pad word per kDecPer so that the threads' runs of
+// kDecPer consecutive positions fall in different banks.
+__device__ __forceinline__ int dec_pad(int p) { return p + p / kDecPer; }
 
 // Source of the first edge of every decode tile, and (entry ntiles) of edge
 // e - 1: the largest v with off[v] <= t (off[0] = 0 <= t < e = off[n]).
@@ -391,8 +396,41 @@ __device__ __forceinline__ int32_t last_le(const A &o, int32_t hi, int64_t i)
     return lo;
 }
 
-// One pass over the edges in tiles of kDecTile: decodes the sources (SOFF),
-// checks the endpoints, and checks the weights as k_weight_exponent does (int32
+// The weight checks and sums of one edge (x: its byte with W8, else its raw
+// 32-bit pattern); an escaped byte (255) is left to k_load_escapes.
+template <bool W8>
+__device__ __forceinline__ void check_weight(uint32_t x, int wtype, int &bad, int &smax,
+                                             uint32_t &mb, ull &tot, ull &msum, ull &umax)
+{
+    if (W8) {
+        if (x != 255u) {
+            umax = max(umax, (ull)x);
+            tot = sat_add(tot, (ull)(x == 0 ? 1 : x));
+            msum = sat_add(msum, (ull)x);
+        }
+    } else if (wtype == LV_WEIGHT_INT32) {
+        if ((int32_t)x < 0) bad |= 1;
+        const int64_t u = weight_units(x, wtype, 0, bad);
+        umax = max(umax, (ull)u);
+        tot = sat_add(tot, (ull)(u == 0 ? 1 : u));
+        msum = sat_add(msum, (ull)u);
+    } else {
+        const int sx = f32_unit_exponent(x);
+        if (sx < 0) {
+            bad |= 1;
+        } else {
+            smax = max(smax, sx);
+            mb = max(mb, x & 0x7fffffffu);
+        }
+    }
+}
+
+// One pass over the edges in tiles of kDecTile: decodes the sources (SOFF: each
+// row start in the tile marks its position with the row - the largest row
+// when empty rows share a start - and a running maximum over the tile's
+// positions, seeded with the row of the tile's first edge, gives every edge
+// its row; a tile spanning more than kDecSpan rows binary-searches the
+// offsets instead), checks the endpoints, and checks the weights as k_weight_exponent does (int32
 // weights also summed as k_weight_units does with exponent 0: they are their
 // own units, and weight bytes (W8) are widened into wu32).  Escaped bytes
 // (W8, value 255) are left to k_load_escapes.
@@ -401,61 +439,120 @@ __global__ __launch_bounds__(256) void k_load_checks(LoadedEdges L, const O *off
                                                     int64_t e, int32_t n, uint32_t *wu32,
                                                     DevFlags *f)
 {
-    __shared__ O s_off[SOFF ? kDecSpan : 1];
+    __shared__ int32_t s_src[SOFF ? kDecTile + kDecTile / kDecPer : 1];
+    __shared__ int32_t s_wt[4];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     int bad = 0, smax = 0;
     uint32_t mb = 0;
     ull tot = 0, msum = 0, umax = 0;
     const int64_t ntiles = (e + kDecTile - 1) / kDecTile;
     for (int64_t k = blockIdx.x; k < ntiles; k += gridDim.x) {
         const int64_t t0 = k * (int64_t)kDecTile, t1 = min(e, t0 + kDecTile);
+        const int T = (int)(t1 - t0);
         int32_t lo = 0, span = 0;
         if (SOFF) {
             // the tile's edges have sources in [lo, tile_v[k + 1]]
             lo = L.tile_v[k];
             span = L.tile_v[k + 1] - lo + 2;  // off[lo .. tile_v[k + 1] + 1]
-            if (span <= kDecSpan)
-                for (int j = threadIdx.x; j < span; j += 256) s_off[j] = off[lo + j];
+            if (span <= kDecSpan) {
+                for (int p = tid; p < T; p += 256) s_src[dec_pad(p)] = -1;
+                __syncthreads();
+                for (int j = tid; j < span; j += 256) {
+                    const int64_t p = (int64_t)off[lo + j] - t0;
+                    if (p >= 0 && p < T) atomicMax(&s_src[dec_pad((int)p)], lo + j);
+                }
+                __syncthreads();
+                // running maximum: kDecPer consecutive positions per thread, then
+                // the threads' maxima scanned over the block
+                int32_t run = tid == 0 ? lo : -1;
+                const int p0 = tid * kDecPer;
+#pragma unroll
+                for (int q = 0; q < kDecPer; ++q)
+                    if (p0 + q < T) run = max(run, s_src[dec_pad(p0 + q)]);
+                int32_t inc = run;
+#pragma unroll
+                for (int off2 = 1; off2 < 64; off2 <<= 1) {
+                    const int32_t y = __shfl_up(inc, off2, 64);
+                    if (lane >= off2) inc = max(inc, y);
+                }
+                if (lane == 63) s_wt[wave] = inc;
+                __syncthreads();
+                int32_t carry = __shfl_up(inc, 1, 64);
+                if (lane == 0) carry = -1;
+                for (int w2 = 0; w2 < wave; ++w2) carry = max(carry, s_wt[w2]);
+                if (tid == 0) carry = lo;
+#pragma unroll
+                for (int q = 0; q < kDecPer; ++q)
+                    if (p0 + q < T) {
+                        carry = max(carry, s_src[dec_pad(p0 + q)]);
+                        s_src[dec_pad(p0 + q)] = carry;
+                    }
+            }
             __syncthreads();
         }
-        for (int64_t i = t0 + threadIdx.x; i < t1; i += 256) {
-            int32_t a;
-            if (SOFF) {
-                a = lo + (span <= kDecSpan ? last_le(s_off, span - 2, i)
-                                           : last_le(off + lo, span - 2, i));
-                L.src_out[i] = a;
-            } else {
-                a = L.src[i];
-            }
-            const int32_t b = L.dst[i];
-            if (a < 0 || a >= n || b < 0 || b >= n) bad |= 1;
-            if (W8) {
-                const uint32_t x = L.w8[i];
-                if (x != 255u) {
-                    wu32[i] = x;
-                    umax = max(umax, (ull)x);
-                    tot = sat_add(tot, (ull)(x == 0 ? 1 : x));
-                    msum = sat_add(msum, (ull)x);
-                }
-            } else {
-                const uint32_t x = L.w[i];
-                if (wtype == LV_WEIGHT_INT32) {
-                    if ((int32_t)x < 0) bad |= 1;
-                    const int64_t u = weight_units(x, wtype, 0, bad);
-                    umax = max(umax, (ull)u);
-                    tot = sat_add(tot, (ull)(u == 0 ? 1 : u));
-                    msum = sat_add(msum, (ull)u);
+        if (T == kDecTile) {
+            // a full tile: four consecutive edges per thread and round, every load
+            // of the tile issued before the first store
+            constexpr int R = kDecTile / 1024;
+            int4 b4[R], a4[R];
+            uint4 x4[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int64_t i = t0 + r * 1024 + tid * 4;
+                b4[r] = *(const int4 *)(L.dst + i);
+                if (!SOFF) a4[r] = *(const int4 *)(L.src + i);
+                if (W8) {
+                    const uint32_t q = *(const uint32_t *)(L.w8 + i);
+                    x4[r] = make_uint4(q & 255u, (q >> 8) & 255u, (q >> 16) & 255u, q >> 24);
                 } else {
-                    const int sx = f32_unit_exponent(x);
-                    if (sx < 0) {
-                        bad |= 1;
-                    } else {
-                        smax = max(smax, sx);
-                        mb = max(mb, x & 0x7fffffffu);
-                    }
+                    x4[r] = *(const uint4 *)(L.w + i);
                 }
+            }
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int p = r * 1024 + tid * 4;
+                const int64_t i = t0 + p;
+                if (SOFF) {
+                    if (span <= kDecSpan) {
+                        a4[r] = make_int4(s_src[dec_pad(p)], s_src[dec_pad(p + 1)],
+                                          s_src[dec_pad(p + 2)], s_src[dec_pad(p + 3)]);
+                    } else {
+                        a4[r].x = lo + last_le(off + lo, span - 2, i);
+                        a4[r].y = lo + last_le(off + lo, span - 2, i + 1);
+                        a4[r].z = lo + last_le(off + lo, span - 2, i + 2);
+                        a4[r].w = lo + last_le(off + lo, span - 2, i + 3);
+                    }
+                    *(int4 *)(L.src_out + i) = a4[r];
+                }
+                const int32_t av[4] = {a4[r].x, a4[r].y, a4[r].z, a4[r].w};
+                const int32_t bv[4] = {b4[r].x, b4[r].y, b4[r].z, b4[r].w};
+                const uint32_t xv[4] = {x4[r].x, x4[r].y, x4[r].z, x4[r].w};
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    if (av[q] < 0 || av[q] >= n || bv[q] < 0 || bv[q] >= n) bad |= 1;
+                    check_weight<W8>(xv[q], wtype, bad, smax, mb, tot, msum, umax);
+                }
+                // byte 255 (an escape) is written too; k_load_escapes overwrites it
+                if (W8) *(uint4 *)(wu32 + i) = x4[r];
+            }
+        } else {
+            for (int64_t i = t0 + tid; i < t1; i += 256) {
+                int32_t a;
+                if (SOFF) {
+                    a = span <= kDecSpan ? s_src[dec_pad((int)(i - t0))]
+                                         : lo + last_le(off + lo, span - 2, i);
+                    L.src_out[i] = a;
+                } else {
+                    a = L.src[i];
+                }
+                const int32_t b = L.dst[i];
+                if (a < 0 || a >= n || b < 0 || b >= n) bad |= 1;
+                const uint32_t x = W8 ? (uint32_t)L.w8[i] : L.w[i];
+                check_weight<W8>(x, wtype, bad, smax, mb, tot, msum, umax);
+                if (W8) wu32[i] = x;
             }
         }
-        if (SOFF) __syncthreads();  // s_off is refilled for the next tile
+        if (SOFF) __syncthreads();  // s_src is refilled for the next tile
     }
     smax = wave_max_i(smax);
     mb = (uint32_t)wave_max_i((int)mb);
