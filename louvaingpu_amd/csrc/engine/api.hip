@@ -1901,18 +1901,18 @@ int lv_load_edges(lv_ctx *c, int32_t n, const int32_t *ids, int64_t e, const int
     ub.dst = c->d_dst;
     ub.w = wraw;
     int64_t *d_off = nullptr, *esc_i = nullptr;
-    uint8_t *w8 = nullptr;
+    uint8_t *wpk = nullptr;
     uint32_t *esc_w = nullptr;
     int32_t *tile_v = nullptr;
     if (encode) {
         ub.esc_cap = EdgeUploader::esc_cap(e);
         LV_TRY(arena_alloc(A, d_off, (int64_t)n + 1));
-        LV_TRY(arena_alloc(A, w8, e));
+        LV_TRY(arena_alloc(A, wpk, e));
         LV_TRY(arena_alloc(A, esc_i, ub.esc_cap));
         LV_TRY(arena_alloc(A, esc_w, ub.esc_cap));
         LV_TRY(arena_alloc(A, tile_v, (e + kDecTile - 1) / kDecTile + 1));
         ub.off = d_off;
-        ub.w8 = w8;
+        ub.wpk = wpk;
         ub.esc_i = esc_i;
         ub.esc_w = esc_w;
     }
@@ -1934,8 +1934,9 @@ int lv_load_edges(lv_ctx *c, int32_t n, const int32_t *ids, int64_t e, const int
         } else {
             L.src = c->d_src;
         }
-        if (plan.w8) {
-            L.w8 = w8;
+        if (plan.wbits) {
+            L.wpk = wpk;
+            L.wbits = plan.wbits;
             L.esc_i = esc_i;
             L.esc_w = esc_w;
             L.nesc = plan.nesc;
@@ -1943,12 +1944,12 @@ int lv_load_edges(lv_ctx *c, int32_t n, const int32_t *ids, int64_t e, const int
             L.w = wraw;
         }
         // weight bytes are widened into the upload's own weight array
-        launch_load_checks(s, L, wtype, e, n, plan.w8 ? wraw : nullptr, c->d_flags);
+        launch_load_checks(s, L, wtype, e, n, plan.wbits ? wraw : nullptr, c->d_flags);
         if (wtype == LV_WEIGHT_INT32) {
             c->d_w32 = wraw;
             LV_TRY(d2h(c, c->h_flags, c->d_flags, 1));
             LV_TRY(sync(c));
-            void *tmp[] = {d_off, w8, esc_i, esc_w, tile_v};
+            void *tmp[] = {d_off, wpk, esc_i, esc_w, tile_v};
             for (void *p : tmp) A.release(p);
             if (c->h_flags->err) {
                 free_input(c);
@@ -1956,7 +1957,7 @@ int lv_load_edges(lv_ctx *c, int32_t n, const int32_t *ids, int64_t e, const int
             }
             return loaded(c, n, e, 0);
         }
-        void *tmp[] = {d_off, w8, esc_i, esc_w, tile_v};  // later launches may reuse them
+        void *tmp[] = {d_off, wpk, esc_i, esc_w, tile_v};  // later launches may reuse them
         for (void *p : tmp) A.release(p);
     } else {
         LV_TRY(arena_alloc(A, c->d_ids, n));

@@ -322,8 +322,9 @@ void launch_check_endpoints(hipStream_t s, const int32_t *src, const int32_t *ds
                             int32_t n, DevFlags *f);
 // An uploaded edge list as lv_load_edges sent it: sources raw, or as the row
 // offsets (n + 1) of a source-sorted list, decoded into src_out; weights as raw
-// 32-bit patterns, or int weights below 255 as bytes with the others (byte 255)
-// in an escape list sorted by edge.
+// 32-bit patterns, or int weights packed wbits (2, 4 or 8) each - edge i at bit
+// wbits * i - with the all-ones value marking an escape, whose weight is in an
+// escape list sorted by edge.
 struct LoadedEdges {
     const int32_t *src = nullptr;    // raw sources, or null
     const int32_t *off32 = nullptr;  // row offsets, 32-bit when e <= INT32_MAX
@@ -332,7 +333,8 @@ struct LoadedEdges {
     int32_t *src_out = nullptr;      // decoded sources
     const int32_t *dst = nullptr;
     const uint32_t *w = nullptr;     // raw weights, or null
-    const uint8_t *w8 = nullptr;
+    const uint8_t *wpk = nullptr;    // packed weights
+    int wbits = 0;
     const int64_t *esc_i = nullptr;
     const uint32_t *esc_w = nullptr;
     int64_t nesc = 0;
@@ -347,22 +349,24 @@ void launch_load_checks(hipStream_t s, const LoadedEdges &L, int wtype, int64_t 
                         uint32_t *wu32, DevFlags *f);
 
 // Host side of lv_load_edges' upload.  The copy of dst goes out first; host
-// threads meanwhile pack int weights into bytes (escape list for 255 and up)
-// and turn a source-sorted src into row offsets, whose copies follow: 12 bytes
-// per edge become about 5 plus 4-8 per vertex.  What does not encode (unsorted
+// threads meanwhile pack int weights into 2, 4 or 8 bits (the width that a
+// sample of the weights says sends the fewest bytes; an escape list holds the
+// ones that do not fit) and turn a source-sorted src into row offsets, whose
+// copies follow: 12 bytes per edge become 4.25 to 5 plus 4-8 per vertex.  What does not encode (unsorted
 // or out-of-range sources, more escapes than esc_cap) goes out raw, so the
 // device checks see the caller's values either way.
 struct UploadBufs {  // device destinations
     int32_t *src = nullptr, *dst = nullptr;
     uint32_t *w = nullptr;
     void *off = nullptr;  // 8 * (n + 1) bytes
-    uint8_t *w8 = nullptr;
+    uint8_t *wpk = nullptr;  // e bytes
     int64_t *esc_i = nullptr;
     uint32_t *esc_w = nullptr;
     int64_t esc_cap = 0;
 };
 struct UploadPlan {
-    bool src_off = false, off64 = false, w8 = false;
+    bool src_off = false, off64 = false;
+    int wbits = 0;  // packed weight width, 0 when the weights went raw
     int64_t nesc = 0;
     int64_t bytes = 0;  // copied host to device
 };
@@ -383,8 +387,8 @@ class EdgeUploader {
   private:
     int stage(void **p, size_t *cap, size_t bytes);
     HostPool *pool_ = nullptr;
-    void *h_w8_ = nullptr, *h_off_ = nullptr, *h_esc_ = nullptr;
-    size_t w8_cap_ = 0, off_cap_ = 0, esc_bytes_ = 0;
+    void *h_wpk_ = nullptr, *h_off_ = nullptr, *h_esc_ = nullptr;
+    size_t wpk_cap_ = 0, off_cap_ = 0, esc_bytes_ = 0;
     std::vector<std::vector<int64_t>> esc_parts_;
 };
 void launch_check_sorted_unique(hipStream_t s, const uint32_t *keys, int64_t n, DevFlags *f);

@@ -107,7 +107,7 @@ class HostPool {
 EdgeUploader::~EdgeUploader()
 {
     delete pool_;
-    if (h_w8_) (void)hipHostFree(h_w8_);
+    if (h_wpk_) (void)hipHostFree(h_wpk_);
     if (h_off_) (void)hipHostFree(h_off_);
     if (h_esc_) (void)hipHostFree(h_esc_);
 }
@@ -204,46 +204,119 @@ __attribute__((target("avx2"))) bool offsets_avx2(const int32_t *src, int64_t a,
     return true;
 }
 
-// Int weights below 255 as bytes; the positions of the others (byte 255) go
-// to esc, at most cap of them (false beyond).
+// Int weights below m = 2^wb - 1 packed wb bits each (edge i at bit wb * i,
+// little-endian within a byte); the others get m and their positions go to
+// esc, at most cap of them (false beyond).  a is a multiple of 8 / wb.
 bool pack_scalar(const uint32_t *x, int64_t a, int64_t b, uint8_t *o, std::vector<int64_t> &esc,
-                 int64_t cap)
+                 int64_t cap, int wb)
 {
-    for (int64_t i = a; i < b; ++i) {
-        const uint32_t y = x[i];
-        o[i] = (uint8_t)(y < 255u ? y : 255u);
-        if (y >= 255u) {
-            if ((int64_t)esc.size() >= cap) return false;
-            esc.push_back(i);
+    const uint32_t m = (1u << wb) - 1u;
+    const int per = 8 / wb;
+    for (int64_t i = a; i < b; i += per) {
+        uint32_t byte = 0;
+        for (int q = 0; q < per && i + q < b; ++q) {
+            const uint32_t y = x[i + q];
+            if (y >= m) {
+                if ((int64_t)esc.size() >= cap) return false;
+                esc.push_back(i + q);
+            }
+            byte |= (y < m ? y : m) << (q * wb);
         }
+        o[(i * wb) >> 3] = (uint8_t)byte;
     }
     return true;
 }
 
+// The same, 32 weights at a time: min(x, m) as 32 ordered bytes, then two
+// (wb 4) or four (wb 2) of them folded into each output byte.
 __attribute__((target("avx2"))) bool pack_avx2(const uint32_t *x, int64_t a, int64_t b,
-                                               uint8_t *o, std::vector<int64_t> &esc, int64_t cap)
+                                               uint8_t *o, std::vector<int64_t> &esc, int64_t cap,
+                                               int wb)
 {
-    const __m256i k254 = _mm256_set1_epi32(254), k255 = _mm256_set1_epi32(255);
+    const uint32_t m = (1u << wb) - 1u;
+    const __m256i vm = _mm256_set1_epi32((int)m), vm1 = _mm256_set1_epi32((int)(m - 1));
+    const __m256i order = _mm256_setr_epi32(0, 4, 1, 5, 2, 6, 3, 7);
     int64_t i = a;
-    for (; i + 8 <= b; i += 8) {
-        const __m256i v = _mm256_loadu_si256((const __m256i *)(x + i));
-        // unsigned v > 254: max(v, 254) != 254
-        const __m256i small = _mm256_cmpeq_epi32(_mm256_max_epu32(v, k254), k254);
-        unsigned m = ~(unsigned)_mm256_movemask_ps(_mm256_castsi256_ps(small)) & 0xffu;
-        const __m256i c = _mm256_min_epu32(v, k255);
-        const __m256i p8 = _mm256_packus_epi16(_mm256_packus_epi32(c, c), k255);
-        const uint32_t lo = (uint32_t)_mm256_extract_epi32(p8, 0);
-        const uint32_t hi = (uint32_t)_mm256_extract_epi32(p8, 4);
-        memcpy(o + i, &lo, 4);
-        memcpy(o + i + 4, &hi, 4);
-        while (m) {
-            const int k = __builtin_ctz(m);
-            m &= m - 1;
+    for (; i + 32 <= b; i += 32) {
+        __m256i c[4];
+        uint32_t em = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const __m256i v = _mm256_loadu_si256((const __m256i *)(x + i + 8 * k));
+            // unsigned v < m: max(v, m - 1) == m - 1
+            const __m256i fits = _mm256_cmpeq_epi32(_mm256_max_epu32(v, vm1), vm1);
+            em |= (~(uint32_t)_mm256_movemask_ps(_mm256_castsi256_ps(fits)) & 0xffu) << (8 * k);
+            c[k] = _mm256_min_epu32(v, vm);
+        }
+        const __m256i p = _mm256_permutevar8x32_epi32(
+            _mm256_packus_epi16(_mm256_packus_epi32(c[0], c[1]), _mm256_packus_epi32(c[2], c[3])),
+            order);  // the 32 values as bytes, in order
+        uint8_t *out = o + ((i * wb) >> 3);
+        if (wb == 8) {
+            _mm256_storeu_si256((__m256i *)out, p);
+        } else if (wb == 4) {
+            // 16-bit lane (v0, v1) -> v0 | v1 << 4
+            const __m256i t =
+                _mm256_or_si256(_mm256_and_si256(p, _mm256_set1_epi16(0x000f)),
+                                _mm256_and_si256(_mm256_srli_epi16(p, 4), _mm256_set1_epi16(0x00f0)));
+            const __m256i q = _mm256_packus_epi16(t, t);
+            const uint64_t u0 = (uint64_t)_mm256_extract_epi64(q, 0);
+            const uint64_t u1 = (uint64_t)_mm256_extract_epi64(q, 2);
+            memcpy(out, &u0, 8);
+            memcpy(out + 8, &u1, 8);
+        } else {
+            // 32-bit lane (v0, v1, v2, v3) -> v0 | v1 << 2 | v2 << 4 | v3 << 6
+            const __m256i t = _mm256_or_si256(
+                _mm256_or_si256(_mm256_and_si256(p, _mm256_set1_epi32(0x03)),
+                                _mm256_and_si256(_mm256_srli_epi32(p, 6), _mm256_set1_epi32(0x0c))),
+                _mm256_or_si256(_mm256_and_si256(_mm256_srli_epi32(p, 12), _mm256_set1_epi32(0x30)),
+                                _mm256_and_si256(_mm256_srli_epi32(p, 18), _mm256_set1_epi32(0xc0))));
+            const __m256i q16 = _mm256_packus_epi32(t, t);
+            const __m256i q = _mm256_packus_epi16(q16, q16);
+            const uint32_t u0 = (uint32_t)_mm256_extract_epi32(q, 0);
+            const uint32_t u1 = (uint32_t)_mm256_extract_epi32(q, 4);
+            memcpy(out, &u0, 4);
+            memcpy(out + 4, &u1, 4);
+        }
+        while (em) {
+            const int k = __builtin_ctz(em);
+            em &= em - 1;
             if ((int64_t)esc.size() >= cap) return false;
             esc.push_back(i + k);
         }
     }
-    return pack_scalar(x, i, b, o, esc, cap);
+    return pack_scalar(x, i, b, o, esc, cap, wb);
+}
+
+bool pack_part(const uint32_t *x, int64_t a, int64_t b, uint8_t *o, std::vector<int64_t> &esc,
+               int64_t cap, int wb)
+{
+    static const bool avx2 = __builtin_cpu_supports("avx2");
+    return avx2 ? pack_avx2(x, a, b, o, esc, cap, wb) : pack_scalar(x, a, b, o, esc, cap, wb);
+}
+
+// The packing width (2, 4 or 8 bits) whose cost a sample estimates lowest:
+// the packed bytes plus 64 per escape (12 bytes to copy, and a host-side
+// record and a re-read of its weight, about a cache line).  The sample is 256
+// runs of 64 consecutive weights spread over the list (whole cache lines, so
+// it costs a few hundred line reads).
+int pack_width(const uint32_t *x, int64_t e)
+{
+    constexpr int64_t kRuns = 256, kRun = 64;
+    int64_t ns = 0, c3 = 0, c15 = 0, c255 = 0;
+    for (int64_t r = 0; r < kRuns; ++r) {
+        const int64_t a = (e - std::min<int64_t>(e, kRun)) * r / (kRuns - 1) & ~(int64_t)15;
+        for (int64_t i = a; i < std::min(e, a + kRun); ++i, ++ns) {
+            const uint32_t y = x[i];
+            c3 += y >= 3u;
+            c15 += y >= 15u;
+            c255 += y >= 255u;
+        }
+    }
+    const double f = (double)e / (double)std::max<int64_t>(ns, 1);
+    const double b2 = e / 4.0 + 64.0 * f * c3, b4 = e / 2.0 + 64.0 * f * c15,
+                 b8 = (double)e + 64.0 * f * c255;
+    return (b2 <= b4 && b2 <= b8) ? 2 : (b4 <= b8 ? 4 : 8);
 }
 
 template <typename O>
@@ -253,12 +326,6 @@ bool offsets_part(const int32_t *src, int64_t a, int64_t b, int64_t e, int32_t n
     return avx2 ? offsets_avx2(src, a, b, e, n, off) : offsets_scalar(src, a, b, e, n, off);
 }
 
-bool pack_part(const uint32_t *x, int64_t a, int64_t b, uint8_t *o, std::vector<int64_t> &esc,
-               int64_t cap)
-{
-    static const bool avx2 = __builtin_cpu_supports("avx2");
-    return avx2 ? pack_avx2(x, a, b, o, esc, cap) : pack_scalar(x, a, b, o, esc, cap);
-}
 
 }  // namespace
 
@@ -285,40 +352,45 @@ int EdgeUploader::upload(hipStream_t s, int32_t n, int64_t e, const int32_t *src
     // weights first: their copy then queues behind dst while src is scanned
     bool packed = false;
     if (wtype == LV_WEIGHT_INT32) {
-        LV_TRY(stage(&h_w8_, &w8_cap_, (size_t)e));
+        LV_TRY(stage(&h_wpk_, &wpk_cap_, (size_t)e));
         esc_parts_.resize(T);
         std::atomic<bool> over{false};
         const uint32_t *x = (const uint32_t *)w;
-        uint8_t *o = (uint8_t *)h_w8_;
+        uint8_t *o = (uint8_t *)h_wpk_;
         const int64_t cap = d.esc_cap;
+        const int wb = pack_width(x, e);
         pool_->run([&](int t) {
             std::vector<int64_t> &esc = esc_parts_[t];
             esc.clear();
-            int64_t a, b;
-            part(t, a, b);
-            if (!pack_part(x, a, b, o, esc, cap)) over = true;
+            // parts start on 64-edge boundaries, so no byte is shared
+            const int64_t a = (e * t / T) & ~(int64_t)63, b = t + 1 == T ? e : (e * (t + 1) / T) & ~(int64_t)63;
+            if (a < b && !pack_part(x, a, b, o, esc, cap, wb)) over = true;
         });
-        int64_t ne = 0;
-        for (const auto &v : esc_parts_) ne += (int64_t)v.size();
+        std::vector<int64_t> pre((size_t)T + 1, 0);
+        for (int t = 0; t < T; ++t) pre[t + 1] = pre[t] + (int64_t)esc_parts_[t].size();
+        const int64_t ne = pre[T];
         if (!over && ne <= cap) {
             LV_TRY(stage(&h_esc_, &esc_bytes_, 12 * (size_t)std::max<int64_t>(ne, 1)));
             int64_t *hi = (int64_t *)h_esc_;
             uint32_t *hw = (uint32_t *)(hi + ne);
-            int64_t j = 0;
-            for (const auto &v : esc_parts_)
-                for (int64_t i : v) {
+            // the escape list in edge order, each thread its own part's
+            pool_->run([&](int t) {
+                int64_t j = pre[t];
+                for (int64_t i : esc_parts_[t]) {
                     hi[j] = i;
                     hw[j] = x[i];
                     ++j;
                 }
-            LV_HIP(hipMemcpyAsync(d.w8, h_w8_, (size_t)e, hipMemcpyHostToDevice, s));
+            });
+            const size_t pb = (size_t)((e * wb + 7) / 8);
+            LV_HIP(hipMemcpyAsync(d.wpk, h_wpk_, pb, hipMemcpyHostToDevice, s));
             if (ne > 0) {
                 LV_HIP(hipMemcpyAsync(d.esc_i, hi, 8 * (size_t)ne, hipMemcpyHostToDevice, s));
                 LV_HIP(hipMemcpyAsync(d.esc_w, hw, 4 * (size_t)ne, hipMemcpyHostToDevice, s));
             }
-            plan->w8 = true;
+            plan->wbits = wb;
             plan->nesc = ne;
-            plan->bytes += e + 12 * ne;
+            plan->bytes += (int64_t)pb + 12 * ne;
             packed = true;
         }
     }
@@ -396,14 +468,15 @@ __device__ __forceinline__ int32_t last_le(const A &o, int32_t hi, int64_t i)
     return lo;
 }
 
-// The weight checks and sums of one edge (x: its byte with W8, else its raw
-// 32-bit pattern); an escaped byte (255) is left to k_load_escapes.
-template <bool W8>
+// The weight checks and sums of one edge (x: its WB-bit value when packed,
+// else its raw 32-bit pattern); an escape (all WB bits set) is left to
+// k_load_escapes.
+template <int WB>
 __device__ __forceinline__ void check_weight(uint32_t x, int wtype, int &bad, int &smax,
                                              uint32_t &mb, ull &tot, ull &msum, ull &umax)
 {
-    if (W8) {
-        if (x != 255u) {
+    if (WB) {
+        if (x != (1u << WB) - 1u) {
             umax = max(umax, (ull)x);
             tot = sat_add(tot, (ull)(x == 0 ? 1 : x));
             msum = sat_add(msum, (ull)x);
@@ -432,9 +505,9 @@ __device__ __forceinline__ void check_weight(uint32_t x, int wtype, int &bad, in
 // its row; a tile spanning more than kDecSpan rows binary-searches the
 // offsets instead), checks the endpoints, and checks the weights as k_weight_exponent does (int32
 // weights also summed as k_weight_units does with exponent 0: they are their
-// own units, and weight bytes (W8) are widened into wu32).  Escaped bytes
-// (W8, value 255) are left to k_load_escapes.
-template <bool SOFF, bool W8, typename O>
+// own units, and packed weights (WB bits) are widened into wu32).  Escaped
+// weights (all WB bits set) are left to k_load_escapes.
+template <bool SOFF, int WB, typename O>
 __global__ __launch_bounds__(256) void k_load_checks(LoadedEdges L, const O *off, int wtype,
                                                     int64_t e, int32_t n, uint32_t *wu32,
                                                     DevFlags *f)
@@ -501,9 +574,14 @@ __global__ __launch_bounds__(256) void k_load_checks(LoadedEdges L, const O *off
                 const int64_t i = t0 + r * 1024 + tid * 4;
                 b4[r] = *(const int4 *)(L.dst + i);
                 if (!SOFF) a4[r] = *(const int4 *)(L.src + i);
-                if (W8) {
-                    const uint32_t q = *(const uint32_t *)(L.w8 + i);
-                    x4[r] = make_uint4(q & 255u, (q >> 8) & 255u, (q >> 16) & 255u, q >> 24);
+                if (WB) {
+                    // the four edges' 4 * WB bits, whole bytes since i is a multiple of 4
+                    constexpr uint32_t M = (1u << WB) - 1u;
+                    const uint32_t q = WB == 8   ? *(const uint32_t *)(L.wpk + i)
+                                       : WB == 4 ? (uint32_t)*(const uint16_t *)(L.wpk + i / 2)
+                                                 : (uint32_t)L.wpk[i / 4];
+                    x4[r] = make_uint4(q & M, (q >> WB) & M, (q >> (2 * WB)) & M,
+                                       (q >> (3 * WB)) & M);
                 } else {
                     x4[r] = *(const uint4 *)(L.w + i);
                 }
@@ -530,10 +608,10 @@ __global__ __launch_bounds__(256) void k_load_checks(LoadedEdges L, const O *off
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     if (av[q] < 0 || av[q] >= n || bv[q] < 0 || bv[q] >= n) bad |= 1;
-                    check_weight<W8>(xv[q], wtype, bad, smax, mb, tot, msum, umax);
+                    check_weight<WB>(xv[q], wtype, bad, smax, mb, tot, msum, umax);
                 }
-                // byte 255 (an escape) is written too; k_load_escapes overwrites it
-                if (W8) *(uint4 *)(wu32 + i) = x4[r];
+                // an escape's all-ones value is written too; k_load_escapes overwrites it
+                if (WB) *(uint4 *)(wu32 + i) = x4[r];
             }
         } else {
             for (int64_t i = t0 + tid; i < t1; i += 256) {
@@ -547,9 +625,11 @@ __global__ __launch_bounds__(256) void k_load_checks(LoadedEdges L, const O *off
                 }
                 const int32_t b = L.dst[i];
                 if (a < 0 || a >= n || b < 0 || b >= n) bad |= 1;
-                const uint32_t x = W8 ? (uint32_t)L.w8[i] : L.w[i];
-                check_weight<W8>(x, wtype, bad, smax, mb, tot, msum, umax);
-                if (W8) wu32[i] = x;
+                const uint32_t x =
+                    WB ? ((uint32_t)L.wpk[(i * WB) >> 3] >> ((i * WB) & 7)) & ((1u << WB) - 1u)
+                       : L.w[i];
+                check_weight<WB>(x, wtype, bad, smax, mb, tot, msum, umax);
+                if (WB) wu32[i] = x;
             }
         }
         if (SOFF) __syncthreads();  // s_src is refilled for the next tile
@@ -586,12 +666,19 @@ template <bool SOFF, typename O>
 static void launch_checks_w(hipStream_t s, const LoadedEdges &L, const O *off, int wtype, int64_t e,
                             int32_t n, uint32_t *wu, DevFlags *f, int grid)
 {
-    if (L.w8)
-        hipLaunchKernelGGL((k_load_checks<SOFF, true, O>), grid, 256, 0, s, L, off, wtype, e, n,
-                           wu, f);
-    else
-        hipLaunchKernelGGL((k_load_checks<SOFF, false, O>), grid, 256, 0, s, L, off, wtype, e, n,
-                           wu, f);
+    switch (L.wpk ? L.wbits : 0) {
+    case 2:
+        hipLaunchKernelGGL((k_load_checks<SOFF, 2, O>), grid, 256, 0, s, L, off, wtype, e, n, wu, f);
+        break;
+    case 4:
+        hipLaunchKernelGGL((k_load_checks<SOFF, 4, O>), grid, 256, 0, s, L, off, wtype, e, n, wu, f);
+        break;
+    case 8:
+        hipLaunchKernelGGL((k_load_checks<SOFF, 8, O>), grid, 256, 0, s, L, off, wtype, e, n, wu, f);
+        break;
+    default:
+        hipLaunchKernelGGL((k_load_checks<SOFF, 0, O>), grid, 256, 0, s, L, off, wtype, e, n, wu, f);
+    }
 }
 
 void launch_load_checks(hipStream_t s, const LoadedEdges &L, int wtype, int64_t e, int32_t n,
@@ -613,7 +700,7 @@ void launch_load_checks(hipStream_t s, const LoadedEdges &L, int wtype, int64_t 
     } else {
         launch_checks_w<false>(s, L, (const int32_t *)nullptr, wtype, e, n, wu, f, grid);
     }
-    if (L.w8 && L.nesc > 0)
+    if (L.wpk && L.nesc > 0)
         hipLaunchKernelGGL(k_load_escapes, grid_for(L.nesc, 256, 1024), 256, 0, s, L.esc_i,
                            L.esc_w, L.nesc, wu, f);
 }
