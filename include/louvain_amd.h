@@ -75,10 +75,10 @@ typedef struct {
     int32_t chunk_mul;
     /* Edge-list upload of lv_load_edges (positions as ids): 0 = encoded when
      * the list has at least 2^18 edges and the process may use 4 or more CPUs
-     * (host threads turn a source-sorted src into row offsets and int weights
-     * below 255 into bytes while dst is copied; what does not encode is copied
-     * as it is), 1 = plain copies, 2 = encoded at any size.  Results do not
-     * depend on it. */
+     * (host threads turn a source-sorted src into row offsets and pack int
+     * weights 2, 4 or 8 bits each, with an escape list, while dst is copied;
+     * what does not encode is copied as it is), 1 = plain copies, 2 = encoded
+     * at any size.  Results do not depend on it. */
     int32_t upload;
 } lv_config;
 
