@@ -387,6 +387,7 @@ class EdgeUploader {
   private:
     int stage(void **p, size_t *cap, size_t bytes);
     HostPool *pool_ = nullptr;
+    int pool_pid_ = 0;  // the process that started the pool
     void *h_wpk_ = nullptr, *h_off_ = nullptr, *h_esc_ = nullptr;
     size_t wpk_cap_ = 0, off_cap_ = 0, esc_bytes_ = 0;
     std::vector<std::vector<int64_t>> esc_parts_;

@@ -9,6 +9,7 @@
 // decodes both inside the pass that checks the endpoints and weights, so
 // everything after the load sees the caller's arrays.
 #include <sched.h>
+#include <unistd.h>
 
 #include <algorithm>
 #include <atomic>
@@ -106,7 +107,7 @@ class HostPool {
 
 EdgeUploader::~EdgeUploader()
 {
-    delete pool_;
+    if (pool_pid_ == getpid()) delete pool_;  // (a forked child's copy has no threads to join)
     if (h_wpk_) (void)hipHostFree(h_wpk_);
     if (h_off_) (void)hipHostFree(h_off_);
     if (h_esc_) (void)hipHostFree(h_esc_);
@@ -114,7 +115,13 @@ EdgeUploader::~EdgeUploader()
 
 int EdgeUploader::threads()
 {
-    if (!pool_) pool_ = new HostPool(usable_cpus());
+    // a child forked after the pool started has the pool's state but not its
+    // threads: it starts its own (the parent's copy is left alone)
+    if (pool_ && pool_pid_ != getpid()) pool_ = nullptr;
+    if (!pool_) {
+        pool_ = new HostPool(usable_cpus());
+        pool_pid_ = getpid();
+    }
     return pool_->size();
 }
 
