@@ -67,7 +67,8 @@ TIER_KERNELS = {
     4: (["k_sweep1_group<32,", "k_sweep_group<32, 6, 6,"], None),
     5: None, 6: None, 7: None, 8: None,
     9: (["k_sweep1_block<256,", "k_sweep_block<256, 12, 11,"], None),
-    10: (["k_hub1_chunk", "k_hub1_final", "k_hub_count", "k_hub_scatter", "k_hub_agg",
+    10: (["k_sweep1_block<512,", "k_sweep_block<512, 13, 12,"], None),
+    11: (["k_hub1_chunk", "k_hub1_final", "k_hub_count", "k_hub_scatter", "k_hub_agg",
           "k_hub_final"], ["k_hub1_final", "k_hub_final"]),
 }
 TIER_NAMES = {1: "k_sweep1_group<4> / k_sweep_group<4,3,3>", 2: "k_sweep1_group<8> / k_sweep_group<8,4,4>",
@@ -75,7 +76,8 @@ TIER_NAMES = {1: "k_sweep1_group<4> / k_sweep_group<4,3,3>", 2: "k_sweep1_group<
               5: "k_sweep1_group<64> / k_sweep_group<64,7,7>", 6: "k_sweep1_group<64> / k_sweep_group<64,8,8>",
               7: "k_sweep1_group<64> / k_sweep_group<64,9,9>", 8: "k_sweep1_group<64> / k_sweep_group<64,10,10>",
               9: "k_sweep1_block<256> / k_sweep_block<256,12,11>",
-              10: "first pass: k_hub1_chunk + k_hub1_final; later passes: k_hub_count + k_hub_scatter "
+              10: "k_sweep1_block<512> / k_sweep_block<512,13,12>",
+              11: "first pass: k_hub1_chunk + k_hub1_final; later passes: k_hub_count + k_hub_scatter "
                   "+ k_hub_agg + k_hub_final"}
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "louvain_%s_hbm_traffic.txt")
 

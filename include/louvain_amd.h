@@ -66,7 +66,7 @@ typedef struct {
     /* Launch-shape tunables; 0 = the tuning table (tuning.txt beside the library,
      * or the file LV_TUNING_FILE names, written by louvaingpu_amd/tune.py) for
      * the graph's size, else the built-in default.  A row or coarsening segment
-     * with more than hub_q candidates (64..2048; default 2048) takes the hub
+     * with more than hub_q candidates (64..4096; default 4096) takes the hub
      * hash partitions of about hub_part_target entries (64..2^30; default
      * 2048), cut into chunks of chunk_mul entries per partition (1..1024;
      * default 32).  Results do not depend on them. */
@@ -96,7 +96,7 @@ typedef struct {
 
 /* Timing counters, filled when LV_FLAG_TIMING is set (milliseconds of device
  * time between HIP events on the context stream). */
-#define LV_SWEEP_TIERS 10
+#define LV_SWEEP_TIERS 11
 typedef struct {
     double csr_ms, init_ms, sweep_ms, apply_ms, renumber_ms, coarsen_ms, flatten_ms;
     double tier_ms[LV_SWEEP_TIERS];

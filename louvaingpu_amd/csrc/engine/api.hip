@@ -103,7 +103,7 @@ struct lv_ctx {
     // level 0 when the CSR rows are strictly increasing): the first pass of the
     // level can then skip aggregation
     bool rows_unique = false;
-    // hub tier plan (rows with q > 2048), see HubArgs
+    // hub tier plan (rows with q > hub_q), see HubArgs
     HubArgs hub{};
     int32_t n_hubs = 0;
     int64_t hub_parts = 0, hub_chunks = 0, hub_entries = 0;
@@ -112,7 +112,7 @@ struct lv_ctx {
     int32_t hub_part_target = kHubPartTarget;
     int chunk_mul = kChunkMul;  // chunk_span(P, chunk_mul)
     // rows and coarsening segments with more candidates take the hub partitions
-    int32_t hub_q = kTierMaxQ[kTiers - 2];
+    int32_t hub_q = kHubQDefault;
     // groups of the last renumbering (sorted order): head flags, their exclusive
     // scan, first sorted index of each group, new id of each group
     int32_t *g_head = nullptr, *g_hscan = nullptr, *g_start = nullptr, *g_newid = nullptr;
@@ -1082,8 +1082,10 @@ int induce(lv_ctx *c, const int32_t *newc, const int32_t *newid_g, int32_t Kn)
     LV_TRY(arena_alloc(A, sid1, K));
     LV_TRY(arena_alloc(A, sid2, K));
     LV_TRY(zero_small(c, kTiers + 1));
-    launch_cr_seg_tier(s, c->g_start, estart, newid_g, K, sb, slen, tk1, sid1, c->d_small,
-                       c->hub_q);
+    // segments of more than 2048 entries take the hash partitions whatever hub_q
+    // is (one block with an 8192-slot table measured slower on them)
+    const int32_t seg_q = c->hub_q < kTierMaxQ[kTiers - 3] ? c->hub_q : kTierMaxQ[kTiers - 3];
+    launch_cr_seg_tier(s, c->g_start, estart, newid_g, K, sb, slen, tk1, sid1, c->d_small, seg_q);
     uint32_t *tko = nullptr;
     int32_t *segs = nullptr;
     LV_TRY(radix_sort_u32_i32(A, s, tk1, sid1, tk2, sid2, K, 4, &tko, &segs));
@@ -1108,9 +1110,9 @@ int induce(lv_ctx *c, const int32_t *newc, const int32_t *newid_g, int32_t Kn)
     ca.nuniq = nuniq;
     ca.internal = internal;
     ca.ssum = ssum;
-    ca.big_q = c->hub_q;
+    ca.big_q = seg_q;
     for (int t = 1; t < kTiers; ++t) launch_cr_agg_tier(s, t, ca, segs + toff[t], (int32_t)tcnt[t]);
-    // segments above 2048 entries: hash partitions
+    // segments above seg_q entries: hash partitions
     BigSegArgs B{};
     const int32_t nbs = (int32_t)tcnt[kTiers];
     int64_t nparts = 0, nchunks = 0, blen = 0;
@@ -1715,7 +1717,7 @@ void apply_tuning(lv_ctx *c, int64_t nnz)
         row = &r;
         if (nnz <= r.max_nnz) break;
     }
-    c->hub_q = c->cfg.hub_q ? c->cfg.hub_q : row ? row->hub_q : kTierMaxQ[kTiers - 2];
+    c->hub_q = c->cfg.hub_q ? c->cfg.hub_q : row ? row->hub_q : kHubQDefault;
     c->hub_part_target = c->cfg.hub_part_target ? c->cfg.hub_part_target
                          : row                  ? row->part_target
                                                 : kHubPartTarget;

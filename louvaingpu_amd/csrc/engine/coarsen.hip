@@ -349,8 +349,8 @@ __device__ __forceinline__ int table_compact(int32_t *key, const V *val, int cap
     return total;
 }
 
-// Segments of up to 2048 entries: G lanes (or a block) per segment, LDS table
-// sized to the segment; uniques are written at sb[g] in ey's index space.
+// Segments of up to 2048 entries (or hub_q if lower): G lanes (or a block) per
+// segment, LDS table sized to the segment; uniques are written at sb[g] in ey's index space.
 // N32: a narrow level, whose sums all stay below 2^32, keeps 4-byte table sums.
 template <int G, int LOG2CAP, int MINLOG2, bool N32>
 __global__ __launch_bounds__(256) void k_cr_agg_group(CoarsenArgs a, const int32_t *__restrict__ segs,
@@ -533,7 +533,7 @@ void launch_cr_agg_tier(hipStream_t s, int tier, const CoarsenArgs &a, const int
 }
 
 // ---------------------------------------------------------------------------
-// segments of more than 2048 entries: hash partitions of about kHubPartTarget
+// segments of more than 2048 entries (or hub_q): hash partitions of about kHubPartTarget
 // entries, each aggregated in LDS (global table when it has more than 4096
 // distinct keys); uniques land at poff[p] of buy/buw.
 

@@ -326,7 +326,8 @@ struct SegCarry {
 };
 
 // Tier of a row of d entries and q candidates; rows with q > hub_q (at most
-// kTierMaxQ[kTiers - 2]) take the hub partitions.
+// kTierMaxQ[kTiers - 2]) take the hub partitions; below that the first tier whose
+// bound holds q.
 __device__ __forceinline__ int tier_of(int64_t d, int64_t q, int64_t hub_q)
 {
     if (d == 0) return 0;  // no neighbours: the vertex can never move

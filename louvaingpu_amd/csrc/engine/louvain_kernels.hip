@@ -1552,11 +1552,12 @@ static void launch_sweep_tier_t(hipStream_t s, int tier, const SweepArgs &a, con
     case 7: launch_grid(k_sweep_group<64, 9, 9, true, B, W4, N32>, (nv + 3) / 4, 256, s, a, verts, nv); break;
     case 8: launch_grid(k_sweep_group<64, 10, 10, false, B, W4, N32>, (nv + 3) / 4, 256, s, a, verts, nv); break;
     case 9: launch_grid(k_sweep_block<256, 12, 11, B, W4, N32>, nv, 256, s, a, verts, nv); break;
+    case 10: launch_grid(k_sweep_block<512, 13, 12, B, W4, N32>, nv, 512, s, a, verts, nv); break;
     default: break;
     }
 }
 
-// Tiers 1..9 (tier 10, the hubs, runs through run_hub_sweep).
+// Tiers 1..10 (tier 11, the hubs, runs through run_hub_sweep).
 void launch_sweep_tier(hipStream_t s, int tier, const SweepArgs &a, const int32_t *verts, int32_t nv,
                        bool batch)
 {
@@ -1709,6 +1710,7 @@ static void launch_sweep1_tier_t(hipStream_t s, int tier, const SweepArgs &a, co
     case 7:
     case 8: launch_grid(k_sweep1_group<64, B, W4>, (nv + 3) / 4, 256, s, a, verts, nv); break;
     case 9: launch_grid(k_sweep1_block<256, B, W4>, nv, 256, s, a, verts, nv); break;
+    case 10: launch_grid(k_sweep1_block<512, B, W4>, nv, 512, s, a, verts, nv); break;
     default: break;
     }
 }

@@ -26,10 +26,10 @@ import numpy as np
 
 from . import Engine, generators
 
-HUB_Q = (512, 1024, 2048)
+HUB_Q = (512, 1024, 2048, 4096)
 PART_TARGET = (1024, 2048, 4096)
 CHUNK_MUL = (8, 16, 32, 64)
-DEFAULT = {"hub_q": 2048, "hub_part_target": 2048, "chunk_mul": 32}  # the built-in shapes
+DEFAULT = {"hub_q": 4096, "hub_part_target": 2048, "chunk_mul": 32}  # the built-in shapes
 MARGIN = 0.02  # a candidate replaces the default only when this much faster
 
 # (size class upper bound in directed entries, sample graph)

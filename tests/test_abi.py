@@ -75,7 +75,7 @@ def test_bad_config_is_rejected_before_device_use():
     cfg.gamma = -1.0
     h = ct.c_void_p()
     assert lib.lv_create(ct.byref(h), ct.byref(cfg)) == _lib.LV_ERR_ARG
-    for field, bad in (("hub_q", 63), ("hub_q", 4096), ("hub_part_target", 10),
+    for field, bad in (("hub_q", 63), ("hub_q", 4097), ("hub_part_target", 10),
                        ("chunk_mul", 2000), ("upload", 3), ("upload", -1)):
         lib.lv_config_default(ct.byref(cfg))
         assert cfg.hub_q == cfg.hub_part_target == cfg.chunk_mul == 0

@@ -7,6 +7,7 @@ import pytest
 import oracle
 from conftest import gpu_case
 from louvaingpu_amd import Engine, count_partitions, generate_dendogram, generators, start
+from louvaingpu_amd._lib import LV_SWEEP_TIERS
 
 pytestmark = pytest.mark.gpu
 
@@ -93,14 +94,14 @@ def test_timing_stats_count_candidates():
         eng.run()
         st = eng.stats()
         info = eng.level_info(0)
-        cand = sum(st.tier_cand[t] for t in range(10))
+        cand = sum(st.tier_cand[t] for t in range(LV_SWEEP_TIERS))
         nnz = info.nnz
         ns = src != dst
         deg = np.bincount(src[ns], minlength=n) + np.bincount(dst[ns], minlength=n)
         has_loop = np.bincount(src[~ns], minlength=n) > 0
         loops = int(np.count_nonzero(has_loop & (deg > 0)))  # rows without neighbours never sweep
         assert cand == nnz + loops  # one pass, rows unique: every entry, plus own for loops
-        for t in range(10):
+        for t in range(LV_SWEEP_TIERS):
             if st.tier_launches[t]:
                 assert st.tier_bytes[t] >= 8 * st.tier_cand[t]
     with Engine(timing=True) as eng:
@@ -108,4 +109,4 @@ def test_timing_stats_count_candidates():
         eng.run()
         st = eng.stats()
         bound = sum(eng.level_info(l).nnz + eng.level_info(l).n for l in range(eng.num_levels()))
-        assert 0 < sum(st.tier_cand[t] for t in range(10)) <= 2 * bound
+        assert 0 < sum(st.tier_cand[t] for t in range(LV_SWEEP_TIERS)) <= 2 * bound

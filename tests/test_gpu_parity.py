@@ -350,6 +350,24 @@ def test_launch_shapes_do_not_change_results(hub_q, part, chunk):
         assert_run_matches(eng, r, n)
 
 
+@pytest.mark.parametrize("wtype", ["int", "float"])
+def test_rows_of_2049_to_4096_in_block_tables(wtype):
+    """hub_q = 4096: rows of 2049..4096 candidates take one 512-lane block
+    with an 8192-slot LDS table instead of the hub partitions (4-byte table
+    sums on the int graph, 8-byte ones on the float graph, whose total degree
+    in units exceeds 2^32); coarsening segments above 2048 entries keep the
+    partitions."""
+    gpu_case("block tables for rows of 2049..4096, %s weights" % wtype)
+    n, src, dst, w = generators.rmat(17, 16, 0.57, 0.19, 0.19, 21, wtype)
+    ns = src != dst
+    q = np.bincount(src[ns], minlength=n) + np.bincount(dst[ns], minlength=n)
+    assert ((q > 2048) & (q <= 4096)).sum() > 10
+    eng, r = run_both(n, src, dst, w, hub_q=4096, timing=True)
+    with eng:
+        assert eng.stats().tier_launches[9] > 0
+        assert_run_matches(eng, r, n)
+
+
 def test_hub_global_fallback():
     """One partition per hub holds ~9000 distinct communities, more than the
     4096-slot LDS table: the partition is redone in its global table."""
