@@ -341,7 +341,6 @@ void free_sweep_state(lv_ctx *c)
     a.release((void *)h.hbase);
     a.release((void *)h.pcount);
     a.release((void *)h.poff);
-    a.release((void *)h.pcur);
     a.release((void *)h.fbk);
     a.release((void *)h.fbv);
     a.release((void *)h.pg);
@@ -769,8 +768,8 @@ int plan_hubs(lv_ctx *c, int32_t nh)
     LV_TRY(arena_alloc(A, h.fbk, 2 * E));
     LV_TRY(arena_alloc(A, h.fbv, 2 * E));
     LV_TRY(arena_alloc(A, h.pcount, P + 1));
+    fill_i64(c->stream, h.pcount, P + 1, 0);  // every later pass leaves it at zero
     LV_TRY(alloc_into(A, h.poff, P + 1));
-    LV_TRY(arena_alloc(A, h.pcur, P));
     LV_TRY(arena_alloc(A, h.pg, P));
     LV_TRY(arena_alloc(A, h.pc, P));
     LV_TRY(arena_alloc(A, h.pw, P));
@@ -1164,7 +1163,6 @@ int induce(lv_ctx *c, const int32_t *newc, const int32_t *newid_g, int32_t Kn)
         B.chunk_k = chunk_k;
         LV_TRY(arena_alloc(A, B.pcount, nparts + 1));
         LV_TRY(arena_alloc(A, B.poff, nparts + 1));
-        LV_TRY(arena_alloc(A, B.pcur, nparts));
         if (nar)
             LV_TRY(arena_alloc(A, B.pe8, blen));
         else
@@ -1201,7 +1199,7 @@ int induce(lv_ctx *c, const int32_t *newc, const int32_t *newid_g, int32_t Kn)
     // release the transient buffers (later launches on this stream may reuse them)
     void *tmp[] = {mdeg, mstart, estart, selfl, ssum, internal, nuniq, ey, ew, ew32, bound, sb, slen,
                    tk1, tk2, sid1, sid2, bP, bP64, bch, pbase, cbase, part_big, chunk_big, chunk_k,
-                   B.pcount, B.poff, B.pcur, B.pe, B.pe8, B.buy, B.buw, B.fbk, B.fbv, B.ucnt, B.pre,
+                   B.pcount, B.poff, B.pe, B.pe8, B.buy, B.buw, B.fbk, B.fbv, B.ucnt, B.pre,
                    B.chist, bhbase};
     for (void *p : tmp) A.release(p);
     free_level_arrays(c);

@@ -619,7 +619,10 @@ __global__ __launch_bounds__(256) void k_cr_big_scatter(CoarsenArgs a, BigSegArg
     const int32_t *ch = B.chist + B.hbase[j] + (int64_t)B.chunk_k[ci] * P;
     for (int t = threadIdx.x; t < P; t += 256) {
         const int32_t k = ch[t];
-        if (k) lbase[t] = (long long)atomicAdd((ull *)&B.pcur[B.pbase[j] + t], (ull)k);
+        if (k) {  // the run [old - k, old) of the count left in pcount
+            const int64_t p = B.pbase[j] + t;
+            lbase[t] = B.poff[p] + (long long)atomicAdd((ull *)&B.pcount[p], (ull)(-(int64_t)k)) - k;
+        }
     }
     __syncthreads();
     for (int64_t t = t0 + threadIdx.x; t < t1; t += 256) {
@@ -731,8 +734,6 @@ int run_cr_big(Arena &A, hipStream_t s, const CoarsenArgs &a, const BigSegArgs &
     fill_i64(s, B.pcount, nparts + 1, 0);
     hipLaunchKernelGGL(k_cr_big_count, (int)nchunks, 256, 0, s, a, B);
     LV_TRY(scan_exclusive_i64(A, s, B.pcount, B.poff, nparts, B.poff + nparts));
-    LV_HIP(hipMemcpyAsync(B.pcur, B.poff, sizeof(int64_t) * (size_t)nparts,
-                          hipMemcpyDeviceToDevice, s));
     hipLaunchKernelGGL(k_cr_big_scatter, (int)nchunks, 256, 0, s, a, B);
     if (B.pe8)  // narrow level
         hipLaunchKernelGGL(k_cr_big_agg<true>, (int)nparts, 256, 0, s, B);

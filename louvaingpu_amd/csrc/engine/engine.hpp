@@ -199,9 +199,9 @@ struct HubArgs {
     PEntry8 *buf8;            // levels) buf8 instead
     int32_t *chist;           // per chunk its P partition counts, at hbase[j] + k * P_j
     const int64_t *hbase;     // exclusive scan over hubs of chunks * P
-    int64_t *pcount;          // entries per partition
+    int64_t *pcount;          // entries per partition: counted up by the count pass,
+                              //   down to zero by the scatter's reservations
     const int64_t *poff;      // partition offsets (exclusive scan of pcount)
-    int64_t *pcur;            // scatter cursors
     int32_t *fbk;             // fallback global tables, 2 slots per entry
     ull *fbv;
     double *pg;               // per-partition best gain, community, weight, groups
@@ -241,9 +241,8 @@ struct BigSegArgs {
     const int32_t *part_big;
     const int32_t *chunk_big;
     const int32_t *chunk_k;
-    int64_t *pcount;
+    int64_t *pcount;        // entries per partition (up in the count, down in the scatter)
     int64_t *poff;
-    int64_t *pcur;
     PEntry *pe;             // partitioned entries, or (narrow levels) pe8 instead
     PEntry8 *pe8;
     int32_t *chist;         // per chunk its P partition counts, at hbase[j] + k * P_j

@@ -2010,7 +2010,10 @@ __global__ __launch_bounds__(256) void k_hub_scatter(SweepArgs a, HubArgs h)
     const int32_t *ch = h.chist + h.hbase[j] + (int64_t)h.chunk_k[ci] * P;
     for (int t = threadIdx.x; t < P; t += 256) {
         const int32_t k = ch[t];
-        if (k) lbase[t] = (long long)atomicAdd((ull *)&h.pcur[h.pbase[j] + t], (ull)k);
+        if (k) {  // the run [old - k, old) of the count left in pcount
+            const int64_t p = h.pbase[j] + t;
+            lbase[t] = h.poff[p] + (long long)atomicAdd((ull *)&h.pcount[p], (ull)(-(int64_t)k)) - k;
+        }
     }
     __syncthreads();
     for (int64_t t = t0 + threadIdx.x; t < t1; t += 256) {
@@ -2146,11 +2149,9 @@ int run_hub_sweep(Arena &A, hipStream_t s, const SweepArgs &a, const HubArgs &h,
                   int64_t nparts, int64_t nchunks, bool batch)
 {
     if (nh <= 0) return LV_OK;
-    fill_i64(s, h.pcount, nparts + 1, 0);
+    // pcount is zero here (the plan clears it, every scatter counts it back down)
     hipLaunchKernelGGL(k_hub_count, (int)nchunks, 256, 0, s, a, h);
     LV_TRY(scan_exclusive_i64(A, s, h.pcount, (int64_t *)h.poff, nparts, (int64_t *)h.poff + nparts));
-    LV_HIP(hipMemcpyAsync(h.pcur, h.poff, sizeof(int64_t) * (size_t)nparts,
-                          hipMemcpyDeviceToDevice, s));
     if (a.w32)
         hipLaunchKernelGGL(k_hub_scatter<true>, (int)nchunks, 256, 0, s, a, h);
     else
