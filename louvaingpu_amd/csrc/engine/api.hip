@@ -1130,7 +1130,7 @@ int induce(lv_ctx *c, const int32_t *newc, const int32_t *newid_g, int32_t Kn)
         LV_TRY(arena_alloc(A, cbase, (int64_t)nbs + 1));
         LV_TRY(arena_alloc(A, bhbase, (int64_t)nbs + 1));
         launch_cr_big_setup(s, bsegs, nbs, slen, c->hub_part_target, c->chunk_mul, bP, bP64, bch,
-                            blen64, bpc);
+                            blen64, bpc, nuniq);
         B.chunk_mul = c->chunk_mul;
         LV_TRY(scan_exclusive_i64(A, s, bP64, pbase, nbs, pbase + nbs));
         LV_TRY(scan_exclusive_i64(A, s, bch, cbase, nbs, cbase + nbs));
@@ -1167,12 +1167,8 @@ int induce(lv_ctx *c, const int32_t *newc, const int32_t *newid_g, int32_t Kn)
             LV_TRY(arena_alloc(A, B.pe8, blen));
         else
             LV_TRY(arena_alloc(A, B.pe, blen));
-        LV_TRY(arena_alloc(A, B.buy, blen));
-        LV_TRY(arena_alloc(A, B.buw, blen));
         LV_TRY(arena_alloc(A, B.fbk, 2 * blen));
         LV_TRY(arena_alloc(A, B.fbv, 2 * blen));
-        LV_TRY(arena_alloc(A, B.ucnt, nparts));
-        LV_TRY(arena_alloc(A, B.pre, nparts));
         LV_TRY(run_cr_big(A, s, ca, B, nbs, nparts, nchunks));
     }
     // next level: row x = newid_g[g] is [sb[g], sb[g] + nuniq[g]) of uy/uw; the
@@ -1189,7 +1185,6 @@ int induce(lv_ctx *c, const int32_t *newc, const int32_t *newid_g, int32_t Kn)
     fill_i64(s, (int64_t *)c->d_small + 1, 2, 0);
     launch_cr_vertex(s, newid_g, ssum, internal, selfl, sb, nuniq, K, loop2, self2, kdeg2,
                      rowptr2, rowend2, c->d_small + 1, c->d_small + 2);
-    launch_cr_copy_big(s, ca, B, nparts);
     LV_TRY(d2h(c, c->h_small + 1, c->d_small + 1, 2));
     LV_TRY(sync(c));
     const int64_t m2 = c->h_small[1] / 2;
@@ -1199,7 +1194,7 @@ int induce(lv_ctx *c, const int32_t *newc, const int32_t *newid_g, int32_t Kn)
     // release the transient buffers (later launches on this stream may reuse them)
     void *tmp[] = {mdeg, mstart, estart, selfl, ssum, internal, nuniq, ey, ew, ew32, bound, sb, slen,
                    tk1, tk2, sid1, sid2, bP, bP64, bch, pbase, cbase, part_big, chunk_big, chunk_k,
-                   B.pcount, B.poff, B.pe, B.pe8, B.buy, B.buw, B.fbk, B.fbv, B.ucnt, B.pre,
+                   B.pcount, B.poff, B.pe, B.pe8, B.fbk, B.fbv,
                    B.chist, bhbase};
     for (void *p : tmp) A.release(p);
     free_level_arrays(c);

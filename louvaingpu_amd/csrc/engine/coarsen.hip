@@ -539,10 +539,11 @@ void launch_cr_agg_tier(hipStream_t s, int tier, const CoarsenArgs &a, const int
 
 __global__ void k_cr_big_setup(const int32_t *bsegs, int32_t nbs, const int64_t *slen,
                                int32_t target, int chunk_mul, int32_t *bP, int64_t *bP64,
-                               int64_t *bch, int64_t *blen, int64_t *bpc)
+                               int64_t *bch, int64_t *blen, int64_t *bpc, int32_t *nuniq)
 {
     GRID_STRIDE(j, nbs)
     {
+        nuniq[bsegs[j]] = 0;  // the partitions' aggregation adds into it
         int64_t L = slen[bsegs[j]];
         int64_t P = (L + target - 1) / target;
         if (P < 1) P = 1;
@@ -645,8 +646,62 @@ __global__ __launch_bounds__(256) void k_cr_big_scatter(CoarsenArgs a, BigSegArg
     }
 }
 
+// Writes the occupied slots of a partition's LDS table (as table_compact) into
+// the row of its segment g: the block reserves its place with one atomic on
+// nuniq[g], so the partitions of a segment fill [sb[g], sb[g] + nuniq[g]) in
+// the order they finish.  Returns -1 without reserving or writing when the
+// table is full and the partition has more entries than slots (some inserts
+// may have found no slot).  Ends with a barrier.
+template <int SPT, typename V>
+__device__ __forceinline__ int table_to_row(int32_t *key, const V *val, int cap, int64_t entries,
+                                            const CoarsenArgs &a, int32_t g)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    int32_t ky[SPT];
+    V vv[SPT];
+    int mine = 0;
+#pragma unroll
+    for (int k = 0; k < SPT; ++k) {
+        const int s = tid + k * 256;
+        ky[k] = s < cap ? key[s] : -1;
+        vv[k] = ky[k] >= 0 ? val[s] : (V)0;
+        mine += ky[k] >= 0 ? 1 : 0;
+    }
+    int inc = mine;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int o = __shfl_up(inc, off, 64);
+        if (lane >= off) inc += o;
+    }
+    __syncthreads();  // the table is read: its first words take the wave totals
+    if (lane == 63) key[wave] = inc;
+    __syncthreads();
+    int pos = inc - mine, total = 0;
+    for (int w = 0; w < 4; ++w) {
+        const int t = key[w];
+        if (w < wave) pos += t;
+        total += t;
+    }
+    if (total == cap && entries > cap) {  // the same on every thread
+        __syncthreads();
+        return -1;
+    }
+    if (tid == 0) key[4] = atomicAdd(&a.nuniq[g], total);
+    __syncthreads();
+    const int64_t base = a.sb[g] + key[4] + pos;
+    int k2 = 0;
+#pragma unroll
+    for (int k = 0; k < SPT; ++k)
+        if (ky[k] >= 0) {
+            a.uy[base + k2] = ky[k];
+            put_w(a.uw, a.uw32, base + k2, (int64_t)vv[k]);
+            ++k2;
+        }
+    __syncthreads();
+    return total;
+}
+
 template <bool N32>
-__global__ __launch_bounds__(256) void k_cr_big_agg(BigSegArgs B)
+__global__ __launch_bounds__(256) void k_cr_big_agg(CoarsenArgs a, BigSegArgs B)
 {
     constexpr int CAP = 1 << kHubAggLog2;
     __shared__ __align__(16) int32_t s_key[CAP];
@@ -654,6 +709,7 @@ __global__ __launch_bounds__(256) void k_cr_big_agg(BigSegArgs B)
     const int tid = threadIdx.x;
     const int64_t pi = blockIdx.x;
     const int64_t beg = B.poff[pi], end = B.poff[pi + 1];
+    const int32_t g = B.bsegs[B.part_big[pi]];  // (big segments are never stopped groups)
     const int l2 = (end - beg) * 2 >= CAP ? kHubAggLog2 : table_log2(end - beg, 6);
     const int cap = 1 << l2;
     for (int s = tid; s < cap; s += 256) {
@@ -671,60 +727,56 @@ __global__ __launch_bounds__(256) void k_cr_big_agg(BigSegArgs B)
             if (e0 + u * 256 < end) (void)lds_insert_bounded(s_key, s_val, l2, en[u].c, en[u].w);
     }
     __syncthreads();
-    int total = table_compact<CAP / 256>(s_key, s_val, cap, B.buy + beg, B.buw + beg, nullptr);
+    if (table_to_row<CAP / 256>(s_key, s_val, cap, end - beg, a, g) >= 0) return;
     // An insert that found no slot left the table full, which more than CAP
     // entries can do: then (or when exactly CAP keys filled it) the partition
-    // is redone in its global table, whose uniques overwrite these.
-    if (total == cap && end - beg > cap) {
-        // more than CAP distinct keys: the partition's global table; the
-        // dropped LDS table's first word counts the uniques
-        const int64_t gcap = 2 * (end - beg);
-        int32_t *gk = B.fbk + 2 * beg;
-        ull *gv = B.fbv + 2 * beg;
-        if (tid == 0) s_key[0] = 0;
-        for (int64_t s = tid; s < gcap; s += 256) {
-            gk[s] = -1;
-            gv[s] = 0;
-        }
-        __threadfence();
-        __syncthreads();
-        for (int64_t e = beg + tid; e < end; e += 256) {
-            const PEntry en = part_entry(B, e);
-            glb_insert_mod(gk, gv, gcap, en.c, en.w);
-        }
-        __threadfence();
-        __syncthreads();
-        for (int64_t s = tid; s < gcap; s += 256) {
-            int32_t y = __hip_atomic_load(&gk[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (y >= 0) {
-                int pos = atomicAdd(&s_key[0], 1);
-                B.buy[beg + pos] = y;
-                B.buw[beg + pos] =
-                    (int64_t)__hip_atomic_load(&gv[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            }
-        }
-        __syncthreads();
-        total = s_key[0];
+    // is aggregated in its global table instead; its uniques are counted, their
+    // place reserved, then written.
+    const int64_t gcap = 2 * (end - beg);
+    int32_t *gk = B.fbk + 2 * beg;
+    ull *gv = B.fbv + 2 * beg;
+    for (int64_t s = tid; s < gcap; s += 256) {
+        gk[s] = -1;
+        gv[s] = 0;
     }
-    if (tid == 0) B.ucnt[pi] = total;
-}
-
-// ucscan = exclusive scan of ucnt over all partitions (segment-major):
-// pre[p] = its offset within its segment, nuniq = the segment's total.
-__global__ void k_cr_big_prefix(CoarsenArgs a, BigSegArgs B, const int64_t *ucscan, int64_t nparts,
-                                int32_t nbs)
-{
-    GRID_STRIDE(p, nparts) B.pre[p] = ucscan[p] - ucscan[B.pbase[B.part_big[p]]];
-    GRID_STRIDE(j, nbs) a.nuniq[B.bsegs[j]] = (int32_t)(ucscan[B.pbase[j + 1]] - ucscan[B.pbase[j]]);
+    __threadfence();
+    __syncthreads();
+    for (int64_t e = beg + tid; e < end; e += 256) {
+        const PEntry en = part_entry(B, e);
+        glb_insert_mod(gk, gv, gcap, en.c, en.w);
+    }
+    __threadfence();
+    __syncthreads();
+    int mine = 0;
+    for (int64_t s = tid; s < gcap; s += 256)
+        mine += __hip_atomic_load(&gk[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= 0 ? 1 : 0;
+    mine = (int)wave_sum((long long)mine);
+    if ((tid & 63) == 0) s_key[tid >> 6] = mine;
+    __syncthreads();
+    if (tid == 0) {
+        s_key[4] = atomicAdd(&a.nuniq[g], s_key[0] + s_key[1] + s_key[2] + s_key[3]);
+        s_key[5] = 0;
+    }
+    __syncthreads();
+    const int64_t base = a.sb[g] + s_key[4];
+    for (int64_t s = tid; s < gcap; s += 256) {
+        int32_t y = __hip_atomic_load(&gk[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (y >= 0) {
+            const int pos = atomicAdd(&s_key[5], 1);
+            a.uy[base + pos] = y;
+            put_w(a.uw, a.uw32, base + pos,
+                  (int64_t)__hip_atomic_load(&gv[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        }
+    }
 }
 
 void launch_cr_big_setup(hipStream_t s, const int32_t *bsegs, int32_t nbs, const int64_t *slen,
                          int32_t target, int chunk_mul, int32_t *bP, int64_t *bP64, int64_t *bch,
-                         int64_t *blen, int64_t *bpc)
+                         int64_t *blen, int64_t *bpc, int32_t *nuniq)
 {
     if (nbs > 0)
         hipLaunchKernelGGL(k_cr_big_setup, grid_for(nbs, 256), 256, 0, s, bsegs, nbs, slen, target,
-                           chunk_mul, bP, bP64, bch, blen, bpc);
+                           chunk_mul, bP, bP64, bch, blen, bpc, nuniq);
 }
 
 int run_cr_big(Arena &A, hipStream_t s, const CoarsenArgs &a, const BigSegArgs &B, int32_t nbs,
@@ -735,43 +787,18 @@ int run_cr_big(Arena &A, hipStream_t s, const CoarsenArgs &a, const BigSegArgs &
     hipLaunchKernelGGL(k_cr_big_count, (int)nchunks, 256, 0, s, a, B);
     LV_TRY(scan_exclusive_i64(A, s, B.pcount, B.poff, nparts, B.poff + nparts));
     hipLaunchKernelGGL(k_cr_big_scatter, (int)nchunks, 256, 0, s, a, B);
+    // the partitions add their uniques into nuniq of their segment (zeroed by
+    // the setup) and write them into the segment's row
     if (B.pe8)  // narrow level
-        hipLaunchKernelGGL(k_cr_big_agg<true>, (int)nparts, 256, 0, s, B);
+        hipLaunchKernelGGL(k_cr_big_agg<true>, (int)nparts, 256, 0, s, a, B);
     else
-        hipLaunchKernelGGL(k_cr_big_agg<false>, (int)nparts, 256, 0, s, B);
-    int64_t *ucscan = nullptr, *uc64 = nullptr;
-    LV_TRY(arena_alloc(A, uc64, nparts));
-    LV_TRY(arena_alloc(A, ucscan, nparts + 1));
-    launch_widen_i32(s, B.ucnt, nparts, uc64);
-    LV_TRY(scan_exclusive_i64(A, s, uc64, ucscan, nparts, ucscan + nparts));
-    hipLaunchKernelGGL(k_cr_big_prefix, grid_for(nparts > nbs ? nparts : nbs, 256), 256, 0, s, a,
-                       B, ucscan, nparts, nbs);
+        hipLaunchKernelGGL(k_cr_big_agg<false>, (int)nparts, 256, 0, s, a, B);
     LV_HIP(hipGetLastError());
-    arena_free(A, uc64);
-    arena_free(A, ucscan);
     return LV_OK;
 }
 
 // ---------------------------------------------------------------------------
-// next level: big segments' rows, per-vertex k / loop / selfsum and row bounds
-
-// A big segment's uniques (per hash partition in buy/buw) into its own place
-// sb[g] of uy/uw, which is row newid_g[g] of the next level.  One wavefront
-// per partition (about 500 uniques each).
-__global__ void k_cr_copy_big(CoarsenArgs a, BigSegArgs B, int64_t nparts)
-{
-    const int lg = threadIdx.x & 63;
-    for (int64_t p = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; p < nparts;
-         p += ((int64_t)gridDim.x * blockDim.x) >> 6) {
-        const int32_t g = B.bsegs[B.part_big[p]];  // (big segments are never stopped groups)
-        const int64_t src = B.poff[p], dst = a.sb[g] + B.pre[p];
-        const int32_t len = B.ucnt[p];
-        for (int32_t t = lg; t < len; t += 64) {
-            a.uy[dst + t] = B.buy[src + t];
-            put_w(a.uw, a.uw32, dst + t, B.buw[src + t]);
-        }
-    }
-}
+// next level: per-vertex k / loop / selfsum and row bounds
 
 // Next-level vertex x = newid_g[g]: its loop, k and row [sb[g], sb[g] + nuniq[g])
 // of uy/uw; sums 2m and the level's entries.
@@ -810,13 +837,6 @@ __global__ __launch_bounds__(256) void k_cr_vertex(const int32_t *newid_g, const
         const long long u = s_nn[0] + s_nn[1] + s_nn[2] + s_nn[3];
         if (u) atomicAdd((ull *)nsum, (ull)u);
     }
-}
-
-void launch_cr_copy_big(hipStream_t s, const CoarsenArgs &a, const BigSegArgs &B, int64_t nparts)
-{
-    if (nparts > 0)
-        hipLaunchKernelGGL(k_cr_copy_big, grid_for(nparts * 64, 256, 1 << 18), 256, 0, s, a, B,
-                           nparts);
 }
 
 void launch_cr_vertex(hipStream_t s, const int32_t *newid_g, const ull *ssum, const ull *internal,

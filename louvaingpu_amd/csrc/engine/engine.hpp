@@ -247,12 +247,8 @@ struct BigSegArgs {
     PEntry8 *pe8;
     int32_t *chist;         // per chunk its P partition counts, at hbase[j] + k * P_j
     const int64_t *hbase;   // exclusive scan over big segments of chunks * P
-    int32_t *buy;           // uniques per partition, at poff[p]
-    int64_t *buw;
     int32_t *fbk;           // fallback tables
     ull *fbv;
-    int32_t *ucnt;          // uniques per partition
-    int64_t *pre;           // prefix of ucnt within the segment
     int chunk_mul;          // chunk_span(P, chunk_mul) entries per chunk
 };
 
@@ -545,11 +541,10 @@ void launch_cr_seg_tier(hipStream_t s, const int32_t *gstart, const int64_t *est
 void launch_cr_agg_tier(hipStream_t s, int tier, const CoarsenArgs &a, const int32_t *segs, int32_t ns);
 void launch_cr_big_setup(hipStream_t s, const int32_t *bsegs, int32_t nbs, const int64_t *slen,
                          int32_t target, int chunk_mul, int32_t *bP, int64_t *bP64, int64_t *bch,
-                         int64_t *blen, int64_t *bpc);
+                         int64_t *blen, int64_t *bpc, int32_t *nuniq);
 int run_cr_big(Arena &A, hipStream_t s, const CoarsenArgs &a, const BigSegArgs &B, int32_t nbs,
                int64_t nparts, int64_t nchunks);
 // The big segments' uniques from their hash partitions into their rows.
-void launch_cr_copy_big(hipStream_t s, const CoarsenArgs &a, const BigSegArgs &B, int64_t nparts);
 // Per next-level vertex: loop, selfsum, k and row bounds; msum += 2m, nsum += entries.
 void launch_cr_vertex(hipStream_t s, const int32_t *newid_g, const ull *ssum, const ull *internal,
                       const ull *selfl, const int64_t *sb, const int32_t *nuniq, int32_t K,

@@ -378,6 +378,23 @@ def test_hub_global_fallback():
         assert_run_matches(eng, r, n)
 
 
+def test_coarsening_global_fallback():
+    """A hub tied to 6000 heavy pairs: each pair becomes a community, so the
+    hub's community coarsens into one partition (partition target 10^6) with
+    6000 distinct neighbour communities, more than its 4096-slot LDS table:
+    the partition is aggregated in its global table and written into the row."""
+    gpu_case("coarsening global fallback")
+    k = 6000
+    u = 1 + 2 * np.arange(k, dtype=np.int32)
+    src = np.concatenate([u, np.zeros(k, np.int32)])
+    dst = np.concatenate([u + 1, u])
+    w = np.concatenate([np.full(k, 100, np.int32), np.ones(k, np.int32)])
+    eng, r = run_both(1 + 2 * k, src, dst, w, hub_part_target=1000000)
+    with eng:
+        assert eng.level_info(1).n > 4096
+        assert_run_matches(eng, r, 1 + 2 * k)
+
+
 @pytest.mark.parametrize("variant", ["shuffled", "reversed_pairs", "dups_loops_sorted"])
 def test_csr_paths_agree(variant):
     """The CSR of an edge list sorted by source (forward entries kept in place,
