@@ -331,9 +331,20 @@ def run_serving(args):
         out.update({"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
                     "mean_ms": float(lat.mean()), "graphs_per_s": nreq / span,
                     "sets": int(st.sets) - 1, "max_set": int(st.max_set)})
-        lat_r = lat[is_rmat]
-        if len(lat_r):
-            out["p50_ms_rmat20"] = float(np.percentile(lat_r, 50))
+        # by class (the R-MAT 20 graphs are 30-60x the median SBM's edges), and
+        # each latency split into the wait for a worker and the set's run
+        wait = np.array([r[1].t_start_ms - r[1].t_submit_ms for r in res])
+        for key, sel in (("sbm", ~is_rmat), ("rmat20", is_rmat)):
+            if sel.any():
+                out["by_class"] = out.get("by_class", {})
+                out["by_class"][key] = {
+                    "requests": int(sel.sum()),
+                    "p50_ms": float(np.percentile(lat[sel], 50)),
+                    "p99_ms": float(np.percentile(lat[sel], 99)),
+                    "wait_p50_ms": float(np.percentile(wait[sel], 50)),
+                    "wait_p99_ms": float(np.percentile(wait[sel], 99)),
+                    "run_p50_ms": float(np.percentile((lat - wait)[sel], 50)),
+                    "run_p99_ms": float(np.percentile((lat - wait)[sel], 99))}
         print("[bench] serve: p50 %.1f ms p99 %.1f ms" % (out["p50_ms"], out["p99_ms"]),
               file=sys.stderr, flush=True)
         # saturated: every request queued at once
@@ -644,7 +655,7 @@ def main():
     if not args.no_serve:
         v = run_serving(args)
         out["serving"] = {k: v[k] for k in (
-            "config", "p50_ms", "p99_ms", "graphs_per_s", "saturated_graphs_per_s",
+            "config", "p50_ms", "p99_ms", "by_class", "graphs_per_s", "saturated_graphs_per_s",
             "per_graph_loop_graphs_per_s", "saturated_vs_loop", "alone_equals_mix", "max_set")}
     print(json.dumps(out), flush=True)
     return 0

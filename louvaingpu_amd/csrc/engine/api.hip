@@ -149,8 +149,6 @@ struct lv_ctx {
     double *d_cm2 = nullptr, *d_cgam = nullptr;
     QAcc *d_cq = nullptr;
     ull *d_ccnt = nullptr;
-    void *h_stage = nullptr;                // pinned staging of small graphs' edges
-    size_t stage_cap = 0;
     QAcc *h_cq = nullptr;                   // pinned, ncomp
     ull *h_ccnt = nullptr;                  // pinned, ncomp
     int32_t h_cap = 0;                      // capacity of the pinned arrays
@@ -296,6 +294,19 @@ int h2d(lv_ctx *c, T *dev, const T *host, int64_t count)
     if (count <= 0) return LV_OK;
     LV_HIP(hipMemcpyAsync(dev, host, sizeof(T) * (size_t)count, hipMemcpyHostToDevice, c->stream));
     return LV_OK;
+}
+
+// A result array to the caller: after the work queued so far, through the
+// staging ring when the caller's buffer is pageable (a context that may not
+// allocate copies it directly).  Returns when the values are there.
+int fetch(lv_ctx *c, int32_t *host, const int32_t *dev, int64_t count)
+{
+    if (count <= 0) return sync(c);
+    if (c->borrowed) {
+        LV_TRY(d2h(c, host, dev, count));
+        return sync(c);
+    }
+    return c->uploader.download(c->stream, host, dev, sizeof(int32_t) * (size_t)count);
 }
 
 int p_fetch(lv_ctx *c)
@@ -1841,7 +1852,6 @@ int lv_destroy(lv_ctx *c)
     if (c->h_small) (void)hipHostFree(c->h_small);
     if (c->h_cq) (void)hipHostFree(c->h_cq);
     if (c->h_ccnt) (void)hipHostFree(c->h_ccnt);
-    if (c->h_stage) (void)hipHostFree(c->h_stage);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;  // the arena destructor frees the HBM block
     return LV_OK;
@@ -1912,7 +1922,7 @@ int lv_load_edges(lv_ctx *c, int32_t n, const int32_t *ids, int64_t e, const int
         ub.esc_w = esc_w;
     }
     UploadPlan plan;
-    LV_TRY(c->uploader.upload(s, n, e, src, dst, w, wtype, encode, ub, &plan));
+    LV_TRY(c->uploader.upload(s, n, e, src, dst, w, wtype, encode, !c->borrowed, ub, &plan));
     c->upload_bytes = plan.bytes;
     if (!iota_ids) c->upload_bytes += 4 * (int64_t)n;
     if (iota_ids) {
@@ -2211,9 +2221,8 @@ int lv_level_partition(lv_ctx *c, int32_t level, int32_t *node, int32_t *comm)
     LV_TRY(arena_alloc(c->arena, dc, L.n));
     launch_gather_partition(c->stream, L.perm, level == 0 ? c->d_ids : nullptr, L.newc, L.n, dn, dc);
     LV_HIP(hipGetLastError());
-    LV_TRY(d2h(c, node, dn, L.n));
-    LV_TRY(d2h(c, comm, dc, L.n));
-    LV_TRY(sync(c));
+    LV_TRY(fetch(c, node, dn, L.n));
+    LV_TRY(fetch(c, comm, dc, L.n));
     arena_free(c->arena, dn);
     arena_free(c->arena, dc);
     return LV_OK;
@@ -2224,8 +2233,7 @@ int lv_level_assignment(lv_ctx *c, int32_t level, int32_t *comm)
     LV_TRY(check_ctx(c));
     if (!comm || level < 0 || level >= (int32_t)c->levels.size()) return LV_ERR_ARG;
     LV_TRY(set_device(c));
-    LV_TRY(d2h(c, comm, c->levels[level].newc, c->levels[level].n));
-    return sync(c);
+    return fetch(c, comm, c->levels[level].newc, c->levels[level].n);
 }
 
 int lv_flatten(lv_ctx *c, int32_t level, int32_t *node, int32_t *comm)
@@ -2243,9 +2251,8 @@ int lv_flatten(lv_ctx *c, int32_t level, int32_t *node, int32_t *comm)
     LV_TRY(arena_alloc(c->arena, dc, L0.n));
     launch_gather_partition(c->stream, L0.perm, c->d_ids, comp, L0.n, dn, dc);
     LV_HIP(hipGetLastError());
-    LV_TRY(d2h(c, node, dn, L0.n));
-    LV_TRY(d2h(c, comm, dc, L0.n));
-    LV_TRY(sync(c));
+    LV_TRY(fetch(c, node, dn, L0.n));
+    LV_TRY(fetch(c, comm, dc, L0.n));
     arena_free(c->arena, comp);
     arena_free(c->arena, dn);
     arena_free(c->arena, dc);
@@ -2257,8 +2264,7 @@ int lv_final_assignment(lv_ctx *c, int32_t *comm)
     LV_TRY(check_ctx(c));
     if (!comm || !c->final_valid) return LV_ERR_ARG;
     LV_TRY(set_device(c));
-    LV_TRY(d2h(c, comm, c->d_final, c->n0));
-    return sync(c);
+    return fetch(c, comm, c->d_final, c->n0);
 }
 
 int lv_final_modularity(lv_ctx *c, double *q)
@@ -2476,56 +2482,16 @@ int upload_graphs(lv_ctx *c, int32_t ng, const int64_t *e, const std::vector<int
                   const int32_t *const *srcs, const int32_t *const *dsts, const void *const *ws,
                   int32_t *bsrc, int32_t *bdst, uint32_t *bw)
 {
-    constexpr int64_t kDirect = 1 << 16;
-    int64_t small = 0;
+    // each union array is its graphs' arrays end to end: one staged stream per
+    // array, whatever the mix of small and large graphs
     for (int32_t g = 0; g < ng; ++g)
-        if (e[g] < kDirect) small += e[g];
-    const size_t need = (size_t)small * 12;
-    if (need > c->stage_cap) {
-        const size_t cap = std::max(need, std::max((size_t)64 << 20, 2 * c->stage_cap));
-        pinned_free(c->h_stage);
-        c->stage_cap = 0;
-        if (hipHostMalloc(&c->h_stage, cap, hipHostMallocDefault) != hipSuccess) {
-            (void)hipGetLastError();
-            return LV_ERR_NOMEM;
-        }
-        c->stage_cap = cap;
-    }
-    int32_t *ss = (int32_t *)c->h_stage;
-    int32_t *sd = ss + small;
-    uint32_t *sw = (uint32_t *)(sd + small);
-    int64_t run_dev = 0, run_stage = 0, run_len = 0, soff = 0;
-    auto flush = [&]() -> int {
-        if (run_len) {
-            LV_TRY(h2d(c, bsrc + run_dev, (const int32_t *)(ss + run_stage), run_len));
-            LV_TRY(h2d(c, bdst + run_dev, (const int32_t *)(sd + run_stage), run_len));
-            LV_TRY(h2d(c, bw + run_dev, (const uint32_t *)(sw + run_stage), run_len));
-        }
-        run_len = 0;
-        return LV_OK;
-    };
-    for (int32_t g = 0; g < ng; ++g) {
-        const int64_t eg = e[g];
-        if (eg == 0) continue;
-        if (!srcs[g] || !dsts[g] || !ws[g]) return LV_ERR_ARG;
-        if (eg >= kDirect) {
-            LV_TRY(flush());
-            LV_TRY(h2d(c, bsrc + beoff[g], srcs[g], eg));
-            LV_TRY(h2d(c, bdst + beoff[g], dsts[g], eg));
-            LV_TRY(h2d(c, bw + beoff[g], (const uint32_t *)ws[g], eg));
-            continue;
-        }
-        if (run_len == 0) {
-            run_dev = beoff[g];
-            run_stage = soff;
-        }
-        memcpy(ss + soff, srcs[g], sizeof(int32_t) * (size_t)eg);
-        memcpy(sd + soff, dsts[g], sizeof(int32_t) * (size_t)eg);
-        memcpy(sw + soff, ws[g], sizeof(uint32_t) * (size_t)eg);
-        soff += eg;
-        run_len += eg;
-    }
-    return flush();
+        if (e[g] > 0 && (!srcs[g] || !dsts[g] || !ws[g])) return LV_ERR_ARG;
+    std::vector<int64_t> boff((size_t)ng + 1);
+    for (int32_t g = 0; g <= ng; ++g) boff[g] = 4 * beoff[g];
+    LV_TRY(c->uploader.upload_concat(c->stream, bsrc, ng, (const void *const *)srcs, boff.data()));
+    LV_TRY(c->uploader.upload_concat(c->stream, bdst, ng, (const void *const *)dsts, boff.data()));
+    LV_TRY(c->uploader.upload_concat(c->stream, bw, ng, ws, boff.data()));
+    return LV_OK;
 }
 
 // lv_batch_load / lv_batch_load_graphs: either one contiguous edge list
@@ -2608,9 +2574,11 @@ int batch_load(lv_ctx *c, int32_t ngraphs, const int32_t *n, const int64_t *e, c
     if (srcs) {
         LV_TRY(upload_graphs(c, ngraphs, e, beoff, srcs, dsts, ws, bsrc, bdst, bw));
     } else {
-        LV_TRY(h2d(c, bsrc, src, eb));
-        LV_TRY(h2d(c, bdst, dst, eb));
-        LV_TRY(h2d(c, bw, (const uint32_t *)w, eb));
+        const int64_t boff[2] = {0, 4 * eb};
+        const void *hs[1] = {src}, *hd[1] = {dst}, *hw[1] = {w};
+        LV_TRY(c->uploader.upload_concat(s, bsrc, 1, hs, boff));
+        LV_TRY(c->uploader.upload_concat(s, bdst, 1, hd, boff));
+        LV_TRY(c->uploader.upload_concat(s, bw, 1, hw, boff));
     }
     LV_TRY(h2d(c, bn, n, ngraphs));
     LV_TRY(h2d(c, dbeoff, (const int64_t *)beoff.data(), (int64_t)ngraphs + 1));
@@ -2767,8 +2735,7 @@ int lv_batch_level_assignment(lv_ctx *c, int32_t comp, int32_t level, int32_t *c
         return LV_OK;
     }
     const LevelRec &L = c->levels[level];
-    LV_TRY(d2h(c, comm, L.newc + L.voff_h[comp], nv));
-    LV_TRY(sync(c));
+    LV_TRY(fetch(c, comm, L.newc + L.voff_h[comp], nv));
     const int32_t ko = (int32_t)L.koff_h[comp];
     for (int32_t i = 0; i < nv; ++i) comm[i] -= ko;
     return LV_OK;
@@ -2779,14 +2746,10 @@ int lv_batch_final_assignment(lv_ctx *c, int32_t comp, int32_t *comm)
     LV_TRY(check_ctx(c));
     if (!comm || !c->batch || !c->final_valid || comp < -1 || comp >= c->ncomp) return LV_ERR_ARG;
     LV_TRY(set_device(c));
-    if (comp < 0) {
-        LV_TRY(d2h(c, comm, c->d_final, c->n0));
-        return sync(c);
-    }
+    if (comp < 0) return fetch(c, comm, c->d_final, c->n0);
     int64_t off = 0;
     for (int32_t k = 0; k < comp; ++k) off += c->comp_n0[k];
-    LV_TRY(d2h(c, comm, c->d_final + off, c->comp_n0[comp]));
-    return sync(c);
+    return fetch(c, comm, c->d_final + off, c->comp_n0[comp]);
 }
 
 }  // extern "C"

@@ -13,6 +13,81 @@
 
 namespace lv {
 
+// Per-component sums over data grouped by component (vertices, edges,
+// communities) reach their component's word once per block, not once per 64
+// items: device atomics on one word serialise (~11 ns each), and a component
+// of 10^6 vertices would otherwise take 1.6e4 of them per pass.  Each wave
+// takes a contiguous range of 64-item rounds (the waves of a block take
+// consecutive ranges), carries its current run across rounds, and the block
+// merges its waves' last runs in LDS before adding them.
+static_assert(sizeof(QAcc) == 4 * sizeof(ull), "block_flush indexes QAcc as four words");
+
+__device__ __forceinline__ void wave_rounds(int64_t items, int64_t &r0, int64_t &r1)
+{
+    const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    const int64_t wv = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int64_t rounds = (items + 63) >> 6;
+    const int64_t per = (rounds + waves - 1) / waves;
+    r0 = min(rounds, wv * per);
+    r1 = min(rounds, r0 + per);
+}
+
+// The last runs of a block's waves (keys ascending by wave, LLONG_MIN for a
+// wave with none) merged and added: one atomic per distinct key.  Needs
+// blockDim.x / 64 <= 16 and every thread of the block.
+template <int NV, bool U128>
+__device__ __forceinline__ void block_flush(long long key, const ull (&v)[NV], ull *const (&dst)[NV])
+{
+    __shared__ long long s_key[16];
+    __shared__ ull s_v[16][NV];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    if (lane == 0) {
+        s_key[wave] = key;
+        for (int i = 0; i < NV; ++i) s_v[wave][i] = v[i];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long k = LLONG_MIN;
+        ull acc[NV] = {};
+        for (int w = 0; w <= nw; ++w) {
+            const long long kw = w < nw ? s_key[w] : LLONG_MIN;
+            if (w < nw && kw == LLONG_MIN) continue;
+            if (w < nw && kw == k) {
+                if constexpr (U128) {  // (sum, 128-bit lo/hi)
+                    acc[0] += s_v[w][0];
+                    const ull lo = acc[1] + s_v[w][1];
+                    acc[2] += s_v[w][2] + (lo < acc[1] ? 1ull : 0ull);
+                    acc[1] = lo;
+                } else {
+                    for (int i = 0; i < NV; ++i) acc[i] += s_v[w][i];
+                }
+                continue;
+            }
+            if (k != LLONG_MIN) {
+                if constexpr (U128) {
+                    if (acc[0]) atomicAdd(dst[0] + 4 * k, acc[0]);
+                    if (acc[1] | acc[2]) atomic_add_u128(dst[1] + 4 * k, dst[2] + 4 * k, acc[1], acc[2]);
+                } else {
+                    for (int i = 0; i < NV; ++i)
+                        if (acc[i]) atomicAdd(dst[i] + k, acc[i]);
+                }
+            }
+            if (w < nw) {
+                k = kw;
+                for (int i = 0; i < NV; ++i) acc[i] = s_v[w][i];
+            }
+        }
+    }
+}
+
+// SegCarry's run of a wave added through block_flush.
+__device__ __forceinline__ void seg_block_flush(SegCarry &sc, ull *base)
+{
+    const ull v[1] = {sc.key != LLONG_MIN ? sc.sum : 0ull};
+    ull *const d[1] = {base};
+    block_flush<1, false>(sc.key, v, d);
+}
+
 // Unit exponent of every graph of the set: gscale[g] = the smallest s making
 // all of graph g's weights integral in units of 2^-s (so every component
 // computes in the units a single-graph run of its graph would use); negative,
@@ -52,7 +127,11 @@ __global__ void k_b_graph_exponent(const uint32_t *bw, int wtype, const int64_t 
             if (lane >= off && og == g) sc = max(sc, os);
         }
         const int32_t ng = __shfl_down(g, 1, 64);
-        if (valid && (lane == 63 || ng != g) && sc > 0) atomicMax(&gscale[g], sc);
+        // only a run that raises its graph's maximum takes the atomic: every
+        // wave of a large graph would otherwise serialise on one word
+        if (valid && (lane == 63 || ng != g) && sc > 0 &&
+            sc > __hip_atomic_load(&gscale[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMax(&gscale[g], sc);
     }
     bad = wave_or_i(bad);
     if (lane == 0 && bad) atomicOr(&f->err, 1);
@@ -166,12 +245,15 @@ __global__ __launch_bounds__(256) void k_b_replicate_vertices(
 }
 
 // cm[c] += sum of w over the union edges of component c (edge ranges ueoff).
+// cm[c] = sum of w over component c's edges.
 __global__ void k_b_edge_sum(const int64_t *w, const int64_t *ueoff, int32_t ncomp, int64_t etot,
                              ull *cm)
 {
-    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < etot;
-         base += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t j = base + threadIdx.x;
+    int64_t r0, r1;
+    wave_rounds(etot, r0, r1);
+    SegCarry sc;
+    for (int64_t r = r0; r < r1; ++r) {
+        const int64_t j = (r << 6) + (threadIdx.x & 63);
         const bool valid = j < etot;
         long long key = 0;
         ull v = 0;
@@ -184,38 +266,9 @@ __global__ void k_b_edge_sum(const int64_t *w, const int64_t *ueoff, int32_t nco
             key = lo;
             v = (ull)w[j];
         }
-        seg_atomic_add(valid, key, v, cm);
+        sc.add(valid, key, v, cm);
     }
-}
-
-// Wave-segmented sum of (sin, 128-bit d2) over lanes with equal, sorted keys;
-// the last lane of each run adds the run into cq[key].
-__device__ __forceinline__ void seg_qacc_add(bool valid, int32_t key, long long sin, ull lo, ull hi,
-                                             QAcc *cq)
-{
-    const int lane = threadIdx.x & 63;
-    const int32_t k = valid ? key : INT_MIN;
-    if (!valid) {
-        sin = 0;
-        lo = 0;
-        hi = 0;
-    }
-    for (int off = 1; off < 64; off <<= 1) {
-        const long long os = __shfl_up(sin, off, 64);
-        const ull ol = __shfl_up(lo, off, 64), oh = __shfl_up(hi, off, 64);
-        const int32_t ok = __shfl_up(k, off, 64);
-        if (lane >= off && ok == k) {
-            sin += os;
-            lo += ol;
-            hi += oh + (lo < ol ? 1ull : 0ull);
-        }
-    }
-    const int32_t nk = __shfl_down(k, 1, 64);
-    const bool tail = (lane == 63) || (nk != k);
-    if (valid && tail) {
-        if (sin) atomicAdd((ull *)&cq[k].sum_in, (ull)sin);
-        if (lo | hi) atomic_add_u128(&cq[k].d2_lo, &cq[k].d2_hi, lo, hi);
-    }
+    seg_block_flush(sc, cm);
 }
 
 // Per component Q aggregates of the community state (sum of internals, sum of
@@ -225,21 +278,57 @@ __global__ __launch_bounds__(256) void k_b_qagg(int32_t n, int64_t *deg, const i
                                                int size_shift, uint32_t *deg32,
                                                const int32_t *vcomp, int apply, QAcc *cq)
 {
-    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n;
-         base += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t c = base + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    int64_t r0, r1;
+    wave_rounds(n, r0, r1);
+    long long ckey = LLONG_MIN;  // the wave's carried run (as lane 63 holds it)
+    long long csin = 0;
+    ull clo = 0, chi = 0;
+    for (int64_t r = r0; r < r1; ++r) {
+        const int64_t c = (r << 6) + lane;
         const bool valid = c < n;
-        int32_t key = 0;
-        long long sin = 0;
+        long long k = LLONG_MIN, sin = 0;
         ull lo = 0, hi = 0;
         if (valid) {
             const int64_t dv = apply ? apply_deltas(c, deg, size, ddeg, dsize, size_shift, deg32) : deg[c];
-            key = vcomp[c];
+            k = vcomp[c];
             sin = in[c];
             acc_sq(lo, hi, (ull)dv);
         }
-        seg_qacc_add(valid, key, sin, lo, hi, cq);
+        if (lane == 0 && ckey != LLONG_MIN) {  // the carried run joins lane 0's or is added
+            if (k == ckey) {
+                sin += csin;
+                const ull l2 = lo + clo;
+                hi += chi + (l2 < lo ? 1ull : 0ull);
+                lo = l2;
+            } else {
+                if (csin) atomicAdd((ull *)&cq[ckey].sum_in, (ull)csin);
+                if (clo | chi) atomic_add_u128(&cq[ckey].d2_lo, &cq[ckey].d2_hi, clo, chi);
+            }
+        }
+        for (int off = 1; off < 64; off <<= 1) {
+            const long long os = __shfl_up(sin, off, 64);
+            const ull ol = __shfl_up(lo, off, 64), oh = __shfl_up(hi, off, 64);
+            const long long ok = __shfl_up(k, off, 64);
+            if (lane >= off && ok == k) {
+                sin += os;
+                lo += ol;
+                hi += oh + (lo < ol ? 1ull : 0ull);
+            }
+        }
+        const long long nk = __shfl_down(k, 1, 64);
+        if (valid && lane != 63 && nk != k) {  // a run that ends inside the round
+            if (sin) atomicAdd((ull *)&cq[k].sum_in, (ull)sin);
+            if (lo | hi) atomic_add_u128(&cq[k].d2_lo, &cq[k].d2_hi, lo, hi);
+        }
+        ckey = __shfl(k, 63, 64);
+        csin = __shfl(sin, 63, 64);
+        clo = __shfl(lo, 63, 64);
+        chi = __shfl(hi, 63, 64);
     }
+    const ull v[3] = {(ull)csin, clo, chi};
+    ull *const d[3] = {(ull *)&cq[0].sum_in, &cq[0].d2_lo, &cq[0].d2_hi};
+    block_flush<3, true>(ckey, v, d);
 }
 
 // Communities (renumbering groups) per component: group g's component is its
@@ -247,13 +336,16 @@ __global__ __launch_bounds__(256) void k_b_qagg(int32_t n, int64_t *deg, const i
 __global__ void k_b_groups(const int32_t *gstart, const int32_t *perm, const int32_t *vcomp,
                            int32_t K, ull *kcnt)
 {
-    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < K;
-         base += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t g = base + threadIdx.x;
+    int64_t r0, r1;
+    wave_rounds(K, r0, r1);
+    SegCarry sc;
+    for (int64_t r = r0; r < r1; ++r) {
+        const int64_t g = (r << 6) + (threadIdx.x & 63);
         const bool valid = g < K;
         const long long key = valid ? vcomp[perm[gstart[g]]] : 0;
-        seg_atomic_add(valid, key, 1ull, kcnt);
+        sc.add(valid, key, 1ull, kcnt);
     }
+    seg_block_flush(sc, kcnt);
 }
 
 // Next-level vertex id of every group: its community's number within its
@@ -277,9 +369,11 @@ __global__ void k_b_next_ids(const int32_t *gstart, const int32_t *perm, const i
 __global__ void k_b_vertex_sum(const int64_t *kdeg, const int64_t *loopw, const int32_t *vcomp,
                                int32_t n, ull *cm)
 {
-    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n;
-         base += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t x = base + threadIdx.x;
+    int64_t r0, r1;
+    wave_rounds(n, r0, r1);
+    SegCarry sc;
+    for (int64_t r = r0; r < r1; ++r) {
+        const int64_t x = (r << 6) + (threadIdx.x & 63);
         const bool valid = x < n;
         long long key = 0;
         ull v = 0;
@@ -287,8 +381,9 @@ __global__ void k_b_vertex_sum(const int64_t *kdeg, const int64_t *loopw, const 
             key = vcomp[x];
             v = (ull)(kdeg[x] - 2 * loopw[x]);
         }
-        seg_atomic_add(valid, key, v, cm);
+        sc.add(valid, key, v, cm);
     }
+    seg_block_flush(sc, cm);
 }
 
 // One level of the flattening: fin holds each level-0 vertex's id at level l
@@ -350,7 +445,7 @@ void launch_b_edge_sum(hipStream_t s, const int64_t *w, const int64_t *ueoff, in
                        int64_t etot, ull *cm)
 {
     if (etot > 0)
-        hipLaunchKernelGGL(k_b_edge_sum, grid_for(etot, 256, 1 << 20), 256, 0, s, w, ueoff, ncomp,
+        hipLaunchKernelGGL(k_b_edge_sum, grid_for(etot, 256, 1024), 256, 0, s, w, ueoff, ncomp,
                            etot, cm);
 }
 
@@ -359,7 +454,7 @@ void launch_b_qagg(hipStream_t s, int32_t n, int64_t *deg, const int64_t *in, in
                    const int32_t *vcomp, bool apply, QAcc *cq)
 {
     if (n > 0)
-        hipLaunchKernelGGL(k_b_qagg, grid_for(n, 256, 1 << 20), 256, 0, s, n, deg, in, size, ddeg,
+        hipLaunchKernelGGL(k_b_qagg, grid_for(n, 256, 1024), 256, 0, s, n, deg, in, size, ddeg,
                            dsize, size_shift, deg32, vcomp, apply ? 1 : 0, cq);
 }
 
@@ -367,7 +462,7 @@ void launch_b_groups(hipStream_t s, const int32_t *gstart, const int32_t *perm,
                      const int32_t *vcomp, int32_t K, ull *kcnt)
 {
     if (K > 0)
-        hipLaunchKernelGGL(k_b_groups, grid_for(K, 256, 1 << 20), 256, 0, s, gstart, perm, vcomp,
+        hipLaunchKernelGGL(k_b_groups, grid_for(K, 256, 1024), 256, 0, s, gstart, perm, vcomp,
                            K, kcnt);
 }
 
@@ -384,7 +479,7 @@ void launch_b_vertex_sum(hipStream_t s, const int64_t *kdeg, const int64_t *loop
                          const int32_t *vcomp, int32_t n, ull *cm)
 {
     if (n > 0)
-        hipLaunchKernelGGL(k_b_vertex_sum, grid_for(n, 256, 1 << 20), 256, 0, s, kdeg, loopw, vcomp,
+        hipLaunchKernelGGL(k_b_vertex_sum, grid_for(n, 256, 1024), 256, 0, s, kdeg, loopw, vcomp,
                            n, cm);
 }
 

@@ -379,18 +379,41 @@ class EdgeUploader {
     ~EdgeUploader();
     static int64_t esc_cap(int64_t e) { return e / 32 + 1024; }
     int threads();  // host threads of the pool (created on first use)
-    // Enqueues the copies on s; encode = false sends every array raw.  The host
-    // buffers must stay unchanged until the stream has finished the copies.
+    // Enqueues the copies on s; encode = false sends every array raw (from
+    // pageable memory through the staging ring when staging is set, else with
+    // plain copies, which allocate nothing).  The host buffers must stay
+    // unchanged until the stream has finished the copies.
     int upload(hipStream_t s, int32_t n, int64_t e, const int32_t *src, const int32_t *dst,
-               const void *w, int wtype, bool encode, const UploadBufs &d, UploadPlan *plan);
+               const void *w, int wtype, bool encode, bool staging, const UploadBufs &d,
+               UploadPlan *plan);
+    // Fills the device array dev from host pieces: piece i (host[i]) is bytes
+    // [off[i], off[i+1]) of it.  The bytes go through two page-locked buffers
+    // of kRingBytes: host threads fill one while the other is on the link (a
+    // buffer is refilled once its copy has finished), so pageable sources move
+    // at page-locked speed.  Returns with the last copies queued on s; the
+    // host pieces may change once the call returns.
+    int upload_concat(hipStream_t s, void *dev, int32_t np, const void *const *host,
+                      const int64_t *off);
+    static constexpr int64_t kRingBytes = 16 << 20;
+    // Copies bytes from dev to host after the work queued on s and returns when
+    // they are there: into pageable host memory of kRingMin bytes or more
+    // through the staging ring (host threads copy each chunk out while the next
+    // one is on the link), else directly.
+    int download(hipStream_t s, void *host, const void *dev, size_t bytes);
+    static constexpr int64_t kRingMin = 256 << 10;
 
   private:
     int stage(void **p, size_t *cap, size_t bytes);
+    int ring();
+    int raw(hipStream_t s, void *dev, const void *host, size_t bytes);
     HostPool *pool_ = nullptr;
     int pool_pid_ = 0;  // the process that started the pool
     void *h_wpk_ = nullptr, *h_off_ = nullptr, *h_esc_ = nullptr;
     size_t wpk_cap_ = 0, off_cap_ = 0, esc_bytes_ = 0;
     std::vector<std::vector<int64_t>> esc_parts_;
+    void *h_ring_[2] = {nullptr, nullptr};
+    hipEvent_t ring_ev_[2] = {nullptr, nullptr};  // the last copy out of each buffer
+    bool stage_ok_ = true;
 };
 void launch_check_sorted_unique(hipStream_t s, const uint32_t *keys, int64_t n, DevFlags *f);
 void launch_map_endpoints(hipStream_t s, const uint32_t *sorted_ids, const int32_t *sorted_pos,

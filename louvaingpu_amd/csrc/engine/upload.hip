@@ -111,6 +111,72 @@ EdgeUploader::~EdgeUploader()
     if (h_wpk_) (void)hipHostFree(h_wpk_);
     if (h_off_) (void)hipHostFree(h_off_);
     if (h_esc_) (void)hipHostFree(h_esc_);
+    for (int b = 0; b < 2; ++b) {
+        if (ring_ev_[b]) {
+            (void)hipEventSynchronize(ring_ev_[b]);  // no copy still reads the buffer
+            (void)hipEventDestroy(ring_ev_[b]);
+        }
+        if (h_ring_[b]) (void)hipHostFree(h_ring_[b]);
+    }
+}
+
+int EdgeUploader::ring()
+{
+    for (int b = 0; b < 2; ++b) {
+        if (!ring_ev_[b] && hipEventCreateWithFlags(&ring_ev_[b], hipEventDisableTiming) != hipSuccess) {
+            ring_ev_[b] = nullptr;
+            (void)hipGetLastError();
+            return LV_ERR_DEVICE;
+        }
+        if (!h_ring_[b] && hipHostMalloc(&h_ring_[b], (size_t)kRingBytes, hipHostMallocDefault) != hipSuccess) {
+            h_ring_[b] = nullptr;
+            (void)hipGetLastError();
+            return LV_ERR_NOMEM;
+        }
+    }
+    return LV_OK;
+}
+
+int EdgeUploader::upload_concat(hipStream_t s, void *dev, int32_t np, const void *const *host,
+                                const int64_t *off)
+{
+    const int64_t total = off[np];
+    if (total <= 0) return LV_OK;
+    LV_TRY(ring());
+    const int T = threads();
+    int64_t k = 0;
+    for (int64_t c0 = 0; c0 < total; c0 += kRingBytes, ++k) {
+        const int64_t len = std::min(kRingBytes, total - c0);
+        const int b = (int)(k & 1);
+        LV_HIP(hipEventSynchronize(ring_ev_[b]));  // its previous copy has left the buffer
+        char *buf = (char *)h_ring_[b];
+        // bytes [a, z) of this chunk, piece by piece (off[i] <= pos < off[i + 1],
+        // so i < np; empty pieces are stepped over)
+        auto fill = [&](int64_t a, int64_t z) {
+            int64_t pos = c0 + a;
+            const int64_t end = c0 + z;
+            int32_t i = (int32_t)(std::upper_bound(off, off + np + 1, pos) - off) - 1;
+            while (pos < end) {
+                const int64_t len_i = std::min(end, off[i + 1]) - pos;
+                if (len_i > 0) {
+                    memcpy(buf + (pos - c0), (const char *)host[i] + (pos - off[i]), (size_t)len_i);
+                    pos += len_i;
+                }
+                ++i;
+            }
+        };
+        // about 1 MiB per thread; small chunks on the calling thread
+        const int use = (int)std::min<int64_t>(T, (len + (1 << 20) - 1) >> 20);
+        if (use <= 1)
+            fill(0, len);
+        else
+            pool_->run([&](int t) {
+                if (t < use) fill(len * t / use, len * (t + 1) / use);
+            });
+        LV_HIP(hipMemcpyAsync((char *)dev + c0, buf, (size_t)len, hipMemcpyHostToDevice, s));
+        LV_HIP(hipEventRecord(ring_ev_[b], s));
+    }
+    return LV_OK;
 }
 
 int EdgeUploader::threads()
@@ -143,18 +209,90 @@ int EdgeUploader::stage(void **p, size_t *cap, size_t bytes)
 }
 
 
+// True for page-locked host memory (hipHostMalloc), which the copy engines
+// read directly; pageable memory (or a pointer the runtime does not know)
+// goes through the staging ring instead of the runtime's pageable path,
+// whose rate varies several-fold between hosts.
+static bool host_pinned(const void *p)
+{
+    hipPointerAttribute_t at{};
+    if (hipPointerGetAttributes(&at, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return at.type == hipMemoryTypeHost;
+}
+
+int EdgeUploader::download(hipStream_t s, void *host, const void *dev, size_t bytes)
+{
+    if (bytes < (size_t)kRingMin || host_pinned(host)) {
+        LV_HIP(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, s));
+        LV_HIP(hipStreamSynchronize(s));
+        return LV_OK;
+    }
+    LV_TRY(ring());
+    const int T = threads();
+    const int64_t total = (int64_t)bytes;
+    const int64_t nch = (total + kRingBytes - 1) / kRingBytes;
+    // chunk k goes to buffer k & 1; chunk k + 1 is on the link while chunk k is
+    // copied out (its buffer's last chunk, k - 1, was copied out already)
+    auto issue = [&](int64_t k) -> int {
+        const int b = (int)(k & 1);
+        const int64_t c0 = k * kRingBytes, len = std::min(kRingBytes, total - c0);
+        LV_HIP(hipEventSynchronize(ring_ev_[b]));  // an earlier upload's copy out of it
+        LV_HIP(hipMemcpyAsync(h_ring_[b], (const char *)dev + c0, (size_t)len,
+                              hipMemcpyDeviceToHost, s));
+        LV_HIP(hipEventRecord(ring_ev_[b], s));
+        return LV_OK;
+    };
+    LV_TRY(issue(0));
+    for (int64_t k = 0; k < nch; ++k) {
+        if (k + 1 < nch) LV_TRY(issue(k + 1));
+        const int b = (int)(k & 1);
+        const int64_t c0 = k * kRingBytes, len = std::min(kRingBytes, total - c0);
+        LV_HIP(hipEventSynchronize(ring_ev_[b]));
+        const char *buf = (const char *)h_ring_[b];
+        char *out = (char *)host + c0;
+        const int use = (int)std::min<int64_t>(T, (len + (1 << 20) - 1) >> 20);
+        if (use <= 1)
+            memcpy(out, buf, (size_t)len);
+        else
+            pool_->run([&](int t) {
+                if (t < use) {
+                    const int64_t a = len * t / use, z = len * (t + 1) / use;
+                    memcpy(out + a, buf + a, (size_t)(z - a));
+                }
+            });
+    }
+    return LV_OK;
+}
+
+// One raw host array to the device: a direct copy from page-locked memory
+// (or when the staging ring may not be allocated), else through the ring.
+int EdgeUploader::raw(hipStream_t s, void *dev, const void *host, size_t bytes)
+{
+    if (!stage_ok_ || host_pinned(host)) {
+        LV_HIP(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, s));
+        return LV_OK;
+    }
+    const int64_t off[2] = {0, (int64_t)bytes};
+    const void *h[1] = {host};
+    return upload_concat(s, dev, 1, h, off);
+}
+
 int EdgeUploader::upload(hipStream_t s, int32_t n, int64_t e, const int32_t *src,
-                         const int32_t *dst, const void *w, int wtype, bool encode,
+                         const int32_t *dst, const void *w, int wtype, bool encode, bool staging,
                          const UploadBufs &d, UploadPlan *plan)
 {
     *plan = UploadPlan{};
+    stage_ok_ = staging;
     if (e <= 0) return LV_OK;
     const size_t eb = 4 * (size_t)e;
-    LV_HIP(hipMemcpyAsync(d.dst, dst, eb, hipMemcpyHostToDevice, s));
+    LV_TRY(raw(s, d.dst, dst, eb));
     plan->bytes = (int64_t)eb;
     if (!encode) {
-        LV_HIP(hipMemcpyAsync(d.w, w, eb, hipMemcpyHostToDevice, s));
-        LV_HIP(hipMemcpyAsync(d.src, src, eb, hipMemcpyHostToDevice, s));
+        LV_TRY(raw(s, d.w, w, eb));
+        LV_TRY(raw(s, d.src, src, eb));
         plan->bytes += 2 * (int64_t)eb;
         return LV_OK;
     }
@@ -209,7 +347,7 @@ int EdgeUploader::upload(hipStream_t s, int32_t n, int64_t e, const int32_t *src
         }
     }
     if (!packed) {
-        LV_HIP(hipMemcpyAsync(d.w, w, eb, hipMemcpyHostToDevice, s));
+        LV_TRY(raw(s, d.w, w, eb));
         plan->bytes += (int64_t)eb;
     }
     // sources: row offsets when sorted and in range
@@ -231,7 +369,7 @@ int EdgeUploader::upload(hipStream_t s, int32_t n, int64_t e, const int32_t *src
         plan->off64 = o64;
         plan->bytes += (int64_t)ob;
     } else {
-        LV_HIP(hipMemcpyAsync(d.src, src, eb, hipMemcpyHostToDevice, s));
+        LV_TRY(raw(s, d.src, src, eb));
         plan->bytes += (int64_t)eb;
     }
     return LV_OK;
