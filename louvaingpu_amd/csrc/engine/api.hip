@@ -348,6 +348,7 @@ void free_sweep_state(lv_ctx *c)
     a.release((void *)h.tmp_c);
     a.release((void *)h.buf);
     a.release((void *)h.buf8);
+    a.release((void *)h.buf4);
     a.release((void *)h.chist);
     a.release((void *)h.hbase);
     a.release((void *)h.pcount);
@@ -715,6 +716,14 @@ int alloc_into(Arena &A, const T *&p, int64_t count)
     return LV_OK;
 }
 
+// Bits that hold every row weight and entry weight w' of level 0 (at most the
+// largest input weight in units, and at least 1): the width of the weight
+// field of a packed partition entry.
+int pack_bits(const lv_ctx *c)
+{
+    return bits_for(c->max_units0 > 1 ? c->max_units0 : 1);
+}
+
 int plan_hubs(lv_ctx *c, int32_t nh)
 {
     Arena &A = c->arena;
@@ -772,10 +781,20 @@ int plan_hubs(lv_ctx *c, int32_t nh)
     h.hbase = hbase;
     LV_TRY(arena_alloc(A, h.chist, nhist));
     LV_TRY(arena_alloc(A, h.tmp_c, E));
-    if (c->narrow)  // a partition's weights sum below the level's total degree, < 2^32
+    h.buf4 = nullptr;
+    h.buf8 = nullptr;
+    h.buf = nullptr;
+    h.pwb = 0;
+    if (c->level == 0 && pack_bits(c) < 32 && bits_for((uint64_t)c->n - 1) + pack_bits(c) <= 32) {
+        // a community id and a row weight of level 0 (at most the largest input
+        // weight) in one word
+        h.pwb = pack_bits(c);
+        LV_TRY(arena_alloc(A, h.buf4, E));
+    } else if (c->narrow) {  // a partition's weights sum below the level's total degree, < 2^32
         LV_TRY(arena_alloc(A, h.buf8, E));
-    else
+    } else {
         LV_TRY(arena_alloc(A, h.buf, E));
+    }
     LV_TRY(arena_alloc(A, h.fbk, 2 * E));
     LV_TRY(arena_alloc(A, h.fbv, 2 * E));
     LV_TRY(arena_alloc(A, h.pcount, P + 1));
@@ -1174,10 +1193,16 @@ int induce(lv_ctx *c, const int32_t *newc, const int32_t *newid_g, int32_t Kn)
         B.chunk_k = chunk_k;
         LV_TRY(arena_alloc(A, B.pcount, nparts + 1));
         LV_TRY(arena_alloc(A, B.poff, nparts + 1));
-        if (nar)
+        B.n32 = nar ? 1 : 0;
+        if (c->level == 0 && pack_bits(c) < 32 && bits_for((uint64_t)Kn - 1) + pack_bits(c) <= 32) {
+            // a next-level id and an entry weight w' of level 0 in one word
+            B.pwb = pack_bits(c);
+            LV_TRY(arena_alloc(A, B.pe4, blen));
+        } else if (nar) {
             LV_TRY(arena_alloc(A, B.pe8, blen));
-        else
+        } else {
             LV_TRY(arena_alloc(A, B.pe, blen));
+        }
         LV_TRY(arena_alloc(A, B.fbk, 2 * blen));
         LV_TRY(arena_alloc(A, B.fbv, 2 * blen));
         LV_TRY(run_cr_big(A, s, ca, B, nbs, nparts, nchunks));
@@ -1205,7 +1230,7 @@ int induce(lv_ctx *c, const int32_t *newc, const int32_t *newid_g, int32_t Kn)
     // release the transient buffers (later launches on this stream may reuse them)
     void *tmp[] = {mdeg, mstart, estart, selfl, ssum, internal, nuniq, ey, ew, ew32, bound, sb, slen,
                    tk1, tk2, sid1, sid2, bP, bP64, bch, pbase, cbase, part_big, chunk_big, chunk_k,
-                   B.pcount, B.poff, B.pe, B.pe8, B.fbk, B.fbv,
+                   B.pcount, B.poff, B.pe, B.pe8, B.pe4, B.fbk, B.fbv,
                    B.chist, bhbase};
     for (void *p : tmp) A.release(p);
     free_level_arrays(c);

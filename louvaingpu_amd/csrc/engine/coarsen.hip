@@ -28,6 +28,10 @@ __device__ __forceinline__ int64_t entry_w(const CoarsenArgs &a, int64_t e)
 // Partition entry e as (community, weight).
 __device__ __forceinline__ PEntry part_entry(const BigSegArgs &B, int64_t e)
 {
+    if (B.pe4) {
+        const uint32_t x = B.pe4[e];
+        return PEntry{(int32_t)(x >> B.pwb), 0, (int64_t)(x & ((1u << B.pwb) - 1u))};
+    }
     if (B.pe8) {
         const PEntry8 x = B.pe8[e];
         return PEntry{x.c, 0, (int64_t)x.w};
@@ -631,7 +635,9 @@ __global__ __launch_bounds__(256) void k_cr_big_scatter(CoarsenArgs a, BigSegArg
         if (y == x) continue;
         int part = hub_part(y, P);
         int64_t pos = lbase[part] + atomicAdd(&hist[part], 1);
-        if (B.pe8) {
+        if (B.pe4) {
+            B.pe4[pos] = ((uint32_t)y << B.pwb) | (uint32_t)entry_w(a, b + t);
+        } else if (B.pe8) {
             PEntry8 en;
             en.c = y;
             en.w = a.ew32[b + t];
@@ -717,6 +723,9 @@ __global__ __launch_bounds__(256) void k_cr_big_agg(CoarsenArgs a, BigSegArgs B)
         s_val[s] = 0;
     }
     __syncthreads();
+    // each thread folds its runs of one neighbour community (a partition can
+    // hold 10^5 entries of one) into one insert
+    KeyRun run;
     for (int64_t e0 = beg + tid; e0 < end; e0 += 4 * 256) {
         PEntry en[4];
 #pragma unroll
@@ -724,8 +733,10 @@ __global__ __launch_bounds__(256) void k_cr_big_agg(CoarsenArgs a, BigSegArgs B)
             if (e0 + u * 256 < end) en[u] = part_entry(B, e0 + u * 256);
 #pragma unroll
         for (int u = 0; u < 4; ++u)
-            if (e0 + u * 256 < end) (void)lds_insert_bounded(s_key, s_val, l2, en[u].c, en[u].w);
+            if (e0 + u * 256 < end && run.add(en[u].c, en[u].w))
+                (void)lds_insert_bounded(s_key, s_val, l2, run.pk, run.pw);
     }
+    if (run.k >= 0) (void)lds_insert_bounded(s_key, s_val, l2, run.k, run.w);
     __syncthreads();
     if (table_to_row<CAP / 256>(s_key, s_val, cap, end - beg, a, g) >= 0) return;
     // An insert that found no slot left the table full, which more than CAP
@@ -789,7 +800,7 @@ int run_cr_big(Arena &A, hipStream_t s, const CoarsenArgs &a, const BigSegArgs &
     hipLaunchKernelGGL(k_cr_big_scatter, (int)nchunks, 256, 0, s, a, B);
     // the partitions add their uniques into nuniq of their segment (zeroed by
     // the setup) and write them into the segment's row
-    if (B.pe8)  // narrow level
+    if (B.n32)  // narrow level
         hipLaunchKernelGGL(k_cr_big_agg<true>, (int)nparts, 256, 0, s, a, B);
     else
         hipLaunchKernelGGL(k_cr_big_agg<false>, (int)nparts, 256, 0, s, a, B);

@@ -455,6 +455,29 @@ __device__ __forceinline__ bool lds_insert_bounded(int32_t *key, V *val, int log
     return false;
 }
 
+// A thread's run of equal keys and its summed weight, so that a run costs
+// one LDS table insert instead of one per entry (the entries of one key meet
+// on one LDS word, where atomics serialise).  add()
+// returns true when the entry ends the previous run, which is then (pk, pw)
+// and must be added by the caller; after the last entry, (k, w) is the run
+// still open (k < 0: none).
+struct KeyRun {
+    int32_t k = -1, pk = -1;
+    int64_t w = 0, pw = 0;
+    __device__ __forceinline__ bool add(int32_t key, int64_t wt)
+    {
+        if (key == k) {
+            w += wt;
+            return false;
+        }
+        pk = k;
+        pw = w;
+        k = key;
+        w = wt;
+        return pk >= 0;
+    }
+};
+
 // Global open-addressing table of `cap` slots (cap > distinct keys).
 __device__ __forceinline__ void glb_insert_mod(int32_t *key, ull *val, int64_t cap, int32_t c,
                                                int64_t w)

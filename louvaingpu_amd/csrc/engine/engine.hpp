@@ -196,7 +196,9 @@ struct HubArgs {
     const int32_t *chunk_k;   // chunk number within its hub
     int32_t *tmp_c;           // community of each hub entry (entry order)
     PEntry *buf;              // partitioned (community, weight) entries, or (narrow
-    PEntry8 *buf8;            // levels) buf8 instead
+    PEntry8 *buf8;            // levels) buf8 instead, or buf4: community << pwb |
+    uint32_t *buf4;           // weight when both fit 32 bits (level 0, small weights)
+    int pwb;
     int32_t *chist;           // per chunk its P partition counts, at hbase[j] + k * P_j
     const int64_t *hbase;     // exclusive scan over hubs of chunks * P
     int64_t *pcount;          // entries per partition: counted up by the count pass,
@@ -243,8 +245,11 @@ struct BigSegArgs {
     const int32_t *chunk_k;
     int64_t *pcount;        // entries per partition (up in the count, down in the scatter)
     int64_t *poff;
-    PEntry *pe;             // partitioned entries, or (narrow levels) pe8 instead
-    PEntry8 *pe8;
+    PEntry *pe;             // partitioned entries, or (narrow levels) pe8 instead, or
+    PEntry8 *pe8;           // pe4: key << pwb | w' when both fit 32 bits (level 0,
+    uint32_t *pe4;          // small weights)
+    int pwb;
+    int n32;                // narrow level: 4-byte LDS sums in the aggregation
     int32_t *chist;         // per chunk its P partition counts, at hbase[j] + k * P_j
     const int64_t *hbase;   // exclusive scan over big segments of chunks * P
     int32_t *fbk;           // fallback tables

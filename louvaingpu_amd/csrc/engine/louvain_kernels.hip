@@ -1983,6 +1983,10 @@ __global__ __launch_bounds__(256) void k_hub_count(SweepArgs a, HubArgs h)
 // Partitioned hub entry e as (community, weight).
 __device__ __forceinline__ PEntry hub_entry(const HubArgs &h, int64_t e)
 {
+    if (h.buf4) {
+        const uint32_t x = h.buf4[e];
+        return PEntry{(int32_t)(x >> h.pwb), 0, (int64_t)(x & ((1u << h.pwb) - 1u))};
+    }
     if (h.buf8) {
         const PEntry8 x = h.buf8[e];
         return PEntry{x.c, 0, (int64_t)x.w};
@@ -2021,7 +2025,9 @@ __global__ __launch_bounds__(256) void k_hub_scatter(SweepArgs a, HubArgs h)
         int part = hub_part(c, P);
         int64_t pos = lbase[part] + atomicAdd(&hist[part], 1);
         const int64_t w = (t < d) ? w_of<W4>(a, rb + t) : 0;
-        if (h.buf8) {
+        if (h.buf4) {
+            h.buf4[pos] = ((uint32_t)c << h.pwb) | (uint32_t)w;
+        } else if (h.buf8) {
             h.buf8[pos] = PEntry8{c, (uint32_t)w};
         } else {
             PEntry en;
@@ -2056,6 +2062,9 @@ __global__ __launch_bounds__(256) void k_hub_agg(SweepArgs a, HubArgs h)
         s_val[s] = 0;
     }
     __syncthreads();
+    // each thread folds its runs of one community (a partition can hold 10^5
+    // entries of one popular community) into one insert
+    KeyRun run;
     for (int64_t e0 = beg + tid; e0 < end; e0 += 4 * 256) {
         PEntry en[4];
 #pragma unroll
@@ -2063,8 +2072,10 @@ __global__ __launch_bounds__(256) void k_hub_agg(SweepArgs a, HubArgs h)
             if (e0 + u * 256 < end) en[u] = hub_entry(h, e0 + u * 256);
 #pragma unroll
         for (int u = 0; u < 4; ++u)
-            if (e0 + u * 256 < end) (void)lds_insert_bounded(s_key, s_val, l2, en[u].c, en[u].w);
+            if (e0 + u * 256 < end && run.add(en[u].c, en[u].w))
+                (void)lds_insert_bounded(s_key, s_val, l2, run.pk, run.pw);
     }
+    if (run.k >= 0) (void)lds_insert_bounded(s_key, s_val, l2, run.k, run.w);
     __syncthreads();
     int cnt = 0;
     double bg = 0.0;
