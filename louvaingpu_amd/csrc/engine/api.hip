@@ -1078,11 +1078,19 @@ int induce(lv_ctx *c, const int32_t *newc, const int32_t *newid_g, int32_t Kn)
     LV_TRY(arena_alloc(A, ssum, K));
     LV_TRY(arena_alloc(A, internal, K));
     LV_TRY(arena_alloc(A, nuniq, K));
-    LV_TRY(arena_alloc(A, ey, nnz));
-    if (nar)
-        LV_TRY(arena_alloc(A, ew32, nnz));
-    else
-        LV_TRY(arena_alloc(A, ew, nnz));
+    // level 0 with small weights: each entry as one word newc << ewb | w'
+    uint32_t *ep = nullptr;
+    const int ewb = pack_bits(c);
+    const bool packed = c->level == 0 && ewb < 32 && bits_for((uint64_t)Kn - 1) + ewb <= 32;
+    if (packed) {
+        LV_TRY(arena_alloc(A, ep, nnz));
+    } else {
+        LV_TRY(arena_alloc(A, ey, nnz));
+        if (nar)
+            LV_TRY(arena_alloc(A, ew32, nnz));
+        else
+            LV_TRY(arena_alloc(A, ew, nnz));
+    }
     // uy/uw become the next level's col/w: segment g's uniques at sb[g] are row
     // newid_g[g] (the total degree bounds every weight and is the same at every
     // level, so a narrow level's coarse rows keep 4-byte weights)
@@ -1099,7 +1107,8 @@ int induce(lv_ctx *c, const int32_t *newc, const int32_t *newid_g, int32_t Kn)
     fill_i64(s, (int64_t *)internal, K, 0);
     fill_i32(s, nuniq, K, 0);
     launch_cr_selfl(s, L.perm, c->selfsum, c->g_head, c->g_hscan, n, selfl);
-    launch_cr_gather(s, estart, bound, mstart, c->col, c->cw, c->cw32, newc, n, nnz, ey, ew, ew32);
+    launch_cr_gather(s, estart, bound, mstart, c->col, c->cw, c->cw32, newc, n, nnz, ey, ew, ew32,
+                     ep, ewb);
     // segments by length
     int64_t *sb = nullptr, *slen = nullptr;
     uint32_t *tk1 = nullptr, *tk2 = nullptr;
@@ -1130,6 +1139,8 @@ int induce(lv_ctx *c, const int32_t *newc, const int32_t *newid_g, int32_t Kn)
     ca.ey = ey;
     ca.ew = ew;
     ca.ew32 = ew32;
+    ca.ep = ep;
+    ca.ewb = ewb;
     ca.sb = sb;
     ca.slen = slen;
     ca.newid_g = newid_g;
@@ -1194,7 +1205,7 @@ int induce(lv_ctx *c, const int32_t *newc, const int32_t *newid_g, int32_t Kn)
         LV_TRY(arena_alloc(A, B.pcount, nparts + 1));
         LV_TRY(arena_alloc(A, B.poff, nparts + 1));
         B.n32 = nar ? 1 : 0;
-        if (c->level == 0 && pack_bits(c) < 32 && bits_for((uint64_t)Kn - 1) + pack_bits(c) <= 32) {
+        if (packed) {
             // a next-level id and an entry weight w' of level 0 in one word
             B.pwb = pack_bits(c);
             LV_TRY(arena_alloc(A, B.pe4, blen));
@@ -1228,7 +1239,7 @@ int induce(lv_ctx *c, const int32_t *newc, const int32_t *newid_g, int32_t Kn)
     LV_HIP(hipGetLastError());
     t_end(c, t0, CAT_COARSEN);
     // release the transient buffers (later launches on this stream may reuse them)
-    void *tmp[] = {mdeg, mstart, estart, selfl, ssum, internal, nuniq, ey, ew, ew32, bound, sb, slen,
+    void *tmp[] = {mdeg, mstart, estart, selfl, ssum, internal, nuniq, ey, ew, ew32, ep, bound, sb, slen,
                    tk1, tk2, sid1, sid2, bP, bP64, bch, pbase, cbase, part_big, chunk_big, chunk_k,
                    B.pcount, B.poff, B.pe, B.pe8, B.pe4, B.fbk, B.fbv,
                    B.chist, bhbase};

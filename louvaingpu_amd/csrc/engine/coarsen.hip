@@ -25,6 +25,19 @@ __device__ __forceinline__ int64_t entry_w(const CoarsenArgs &a, int64_t e)
     return a.ew32 ? (int64_t)a.ew32[e] : a.ew[e];
 }
 
+// Member-order entry e as (next-level id y, weight w').
+__device__ __forceinline__ void entry_of(const CoarsenArgs &a, int64_t e, int32_t &y, int64_t &w)
+{
+    if (a.ep) {
+        const uint32_t x = a.ep[e];
+        y = (int32_t)(x >> a.ewb);
+        w = (int64_t)(x & ((1u << a.ewb) - 1u));
+    } else {
+        y = a.ey[e];
+        w = entry_w(a, e);
+    }
+}
+
 // Partition entry e as (community, weight).
 __device__ __forceinline__ PEntry part_entry(const BigSegArgs &B, int64_t e)
 {
@@ -166,7 +179,7 @@ __global__ __launch_bounds__(256) void k_cr_gather(const int64_t *estart, const 
                                                   const int64_t *cw, const uint32_t *cw32,
                                                   const int32_t *newc,
                                                   int64_t total, int32_t *ey, int64_t *ew,
-                                                  uint32_t *ew32)
+                                                  uint32_t *ew32, uint32_t *ep, int ewb)
 {
     __shared__ int64_t s_est[kGatherLds];
     const int64_t b = blockIdx.x;
@@ -226,8 +239,12 @@ __global__ __launch_bounds__(256) void k_cr_gather(const int64_t *estart, const 
     for (int r = 0; r < R; ++r) {
         const int64_t o = b * kGatherT + r * 256 + threadIdx.x;
         if (ok[r]) {
-            __builtin_nontemporal_store(cc[r], &ey[o]);
             const int64_t wp = uu[r] == 0 ? 1 : uu[r];
+            if (ep) {
+                __builtin_nontemporal_store(((uint32_t)cc[r] << ewb) | (uint32_t)wp, &ep[o]);
+                continue;
+            }
+            __builtin_nontemporal_store(cc[r], &ey[o]);
             if (ew32)
                 __builtin_nontemporal_store((uint32_t)wp, &ew32[o]);
             else
@@ -255,14 +272,15 @@ void launch_cr_selfl(hipStream_t s, const int32_t *perm, const int64_t *selfsum,
 void launch_cr_gather(hipStream_t s, const int64_t *estart, int32_t *bound, const int64_t *mstart,
                       const int32_t *col, const int64_t *cw, const uint32_t *cw32,
                       const int32_t *newc, int32_t n,
-                      int64_t total, int32_t *ey, int64_t *ew, uint32_t *ew32)
+                      int64_t total, int32_t *ey, int64_t *ew, uint32_t *ew32, uint32_t *ep,
+                      int ewb)
 {
     if (total <= 0) return;
     const int64_t nb = (total + kGatherT - 1) / kGatherT;
     hipLaunchKernelGGL(k_lbs_bounds, grid_for(nb + 1, 256), 256, 0, s, estart, n, total, kGatherT,
                        nb, bound);
     hipLaunchKernelGGL(k_cr_gather, (int)nb, 256, 0, s, estart, bound, mstart, col, cw, cw32, newc,
-                       total, ey, ew, ew32);
+                       total, ey, ew, ew32, ep, ewb);
 }
 
 // ---------------------------------------------------------------------------
@@ -397,8 +415,9 @@ __global__ __launch_bounds__(256) void k_cr_agg_group(CoarsenArgs a, const int32
         ull inter = 0, tot = 0;
         if (active) {
             for (int64_t e = b + lg; e < b + L; e += G) {
-                int32_t y = a.ey[e];
-                const int64_t w = entry_w(a, e);
+                int32_t y;
+                int64_t w;
+                entry_of(a, e, y, w);
                 tot += (ull)w;
                 if (y == x)
                     inter += (ull)w;
@@ -471,8 +490,9 @@ __global__ __launch_bounds__(256) void k_cr_agg_block(CoarsenArgs a, const int32
         __syncthreads();
         ull inter = 0, tot = 0;
         for (int64_t e = b + tid; e < b + L; e += 256) {
-            int32_t y = a.ey[e];
-            const int64_t w = entry_w(a, e);
+            int32_t y;
+            int64_t w;
+            entry_of(a, e, y, w);
             tot += (ull)w;
             if (y == x)
                 inter += (ull)w;
@@ -578,8 +598,10 @@ __global__ __launch_bounds__(256) void k_cr_big_count(CoarsenArgs a, BigSegArgs 
     const int64_t t1 = min(L, t0 + cs);
     ull inter = 0, tot = 0;
     for (int64_t t = t0 + threadIdx.x; t < t1; t += 256) {
-        int32_t y = a.ey[b + t];
-        const ull w = (ull)entry_w(a, b + t);
+        int32_t y;
+        int64_t w0;
+        entry_of(a, b + t, y, w0);
+        const ull w = (ull)w0;
         tot += w;
         if (y == x)
             inter += w;
@@ -631,22 +653,24 @@ __global__ __launch_bounds__(256) void k_cr_big_scatter(CoarsenArgs a, BigSegArg
     }
     __syncthreads();
     for (int64_t t = t0 + threadIdx.x; t < t1; t += 256) {
-        int32_t y = a.ey[b + t];
+        int32_t y;
+        int64_t w;
+        entry_of(a, b + t, y, w);
         if (y == x) continue;
         int part = hub_part(y, P);
         int64_t pos = lbase[part] + atomicAdd(&hist[part], 1);
         if (B.pe4) {
-            B.pe4[pos] = ((uint32_t)y << B.pwb) | (uint32_t)entry_w(a, b + t);
+            B.pe4[pos] = ((uint32_t)y << B.pwb) | (uint32_t)w;
         } else if (B.pe8) {
             PEntry8 en;
             en.c = y;
-            en.w = a.ew32[b + t];
+            en.w = (uint32_t)w;
             B.pe8[pos] = en;
         } else {
             PEntry en;
             en.c = y;
             en.pad = 0;
-            en.w = a.ew[b + t];
+            en.w = w;
             B.pe[pos] = en;
         }
     }

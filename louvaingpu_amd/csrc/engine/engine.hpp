@@ -223,6 +223,8 @@ struct CoarsenArgs {
     const int32_t *ey;      // newc[target] of every entry, member order
     const int64_t *ew;      // w' of every entry, or (narrow levels) ew32 instead
     const uint32_t *ew32;
+    const uint32_t *ep;     // or both in one word, newc << ewb | w' (level 0, when the
+    int ewb;                //   next level's ids and the largest weight fit 32 bits)
     const int64_t *sb;      // first entry of group g
     const int64_t *slen;    // entries of group g
     const int32_t *newid_g; // new id of group g
@@ -561,8 +563,9 @@ void launch_cr_selfl(hipStream_t s, const int32_t *perm, const int64_t *selfsum,
                      const int32_t *head, const int32_t *hscan, int32_t n, ull *selfl);
 void launch_cr_gather(hipStream_t s, const int64_t *estart, int32_t *bound, const int64_t *mstart,
                       const int32_t *col, const int64_t *cw, const uint32_t *cw32,
-                      const int32_t *newc, int32_t n, int64_t total, int32_t *ey, int64_t *ew,
-                      uint32_t *ew32);
+                      const int32_t *newc, int32_t n,
+                      int64_t total, int32_t *ey, int64_t *ew, uint32_t *ew32, uint32_t *ep,
+                      int ewb);
 void launch_cr_seg_tier(hipStream_t s, const int32_t *gstart, const int64_t *estart,
                         const int32_t *newid_g, int32_t K, int64_t *sb, int64_t *slen,
                         uint32_t *tkey, int32_t *sid, long long *tier_cnt, int32_t hub_q);
