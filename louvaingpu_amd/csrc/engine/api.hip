@@ -92,6 +92,7 @@ struct lv_ctx {
     Lab8 *lab8 = nullptr;        // narrow levels: 8-byte label records and 32-bit totals
     uint32_t *deg32 = nullptr;
     bool narrow = false;
+    int cpk = 0;  // > 0: the level's CSR entries are col << cpk | weight (no cw/cw32)
     int64_t *deg = nullptr, *in = nullptr;
     int32_t *size = nullptr, *dsize = nullptr;
     ull *ddeg = nullptr;
@@ -378,6 +379,7 @@ void free_level_arrays(lv_ctx *c)
     arena_free(a, c->col);
     arena_free(a, c->cw);
     arena_free(a, c->cw32);
+    c->cpk = 0;
     arena_free(a, c->kdeg);
     arena_free(a, c->loopw);
     arena_free(a, c->selfsum);
@@ -489,6 +491,14 @@ int alloc_csr_w(lv_ctx *c, int64_t nnz, bool w32)
 // row v = its reverse entries then its forward entries.  A canonical, strictly
 // sorted list (canonical) gives rows that list each neighbour once in
 // increasing order; otherwise the fills check.
+// Bits that hold every row weight and entry weight w' of level 0 (at most the
+// largest input weight in units, and at least 1): the width of the weight
+// field of a packed CSR or partition entry.
+int pack_bits(const lv_ctx *c)
+{
+    return bits_for(c->max_units0 > 1 ? c->max_units0 : 1);
+}
+
 int build_level0_csr_sorted(lv_ctx *c, int32_t *tself, int32_t *vself, const int64_t *fs,
                             bool canonical, int wbits, const ull *lastloop)
 {
@@ -536,12 +546,17 @@ int build_level0_csr_sorted(lv_ctx *c, int32_t *tself, int32_t *vself, const int
         launch_rowptr_search(s, sk, nr, n, rs);
     }
     c->nnz = 2 * nr;
+    // one word per entry, col << pk | weight, when the ids and the largest
+    // weight fit 32 bits: every pass over the rows reads half the bytes
+    // (graph sets keep separate columns and weights)
+    const int pk = (!c->batch && pack_bits(c) < 32 && kb + pack_bits(c) <= 32) ? pack_bits(c) : 0;
     LV_TRY(arena_alloc(A, c->col, c->nnz));
-    LV_TRY(alloc_csr_w(c, c->nnz, c->max_units0 < (1ull << 32)));
+    if (!pk) LV_TRY(alloc_csr_w(c, c->nnz, c->max_units0 < (1ull << 32)));
+    c->cpk = pk;
     LV_HIP(hipMemsetAsync(&c->d_flags->err, 0, sizeof(int), s));
     launch_csr_sorted_fill(s, !canonical, c->d_src, c->d_dst, wunits(c), tscan, vscan, e, n, fbits,
                            vshift, kb, fs, sk, sp, nr, rs, c->rowptr, ffwd, c->col, c->cw, c->cw32,
-                           c->kdeg, c->d_flags);
+                           pk, c->kdeg, c->d_flags);
     launch_loop_gather(s, lastloop, wunits(c), n, c->loopw);
     LV_HIP(hipGetLastError());
     if (canonical) {
@@ -570,6 +585,7 @@ int build_level0_csr(lv_ctx *c)
 {
     Arena &A = c->arena;
     hipStream_t s = c->stream;
+    c->cpk = 0;  // (the sorted path may pack the entries)
     const int32_t n = c->n0;
     const int64_t e = c->e0;
     const int64_t total = 2 * e;
@@ -714,14 +730,6 @@ int alloc_into(Arena &A, const T *&p, int64_t count)
     LV_TRY(arena_alloc(A, q, count));
     p = q;
     return LV_OK;
-}
-
-// Bits that hold every row weight and entry weight w' of level 0 (at most the
-// largest input weight in units, and at least 1): the width of the weight
-// field of a packed partition entry.
-int pack_bits(const lv_ctx *c)
-{
-    return bits_for(c->max_units0 > 1 ? c->max_units0 : 1);
 }
 
 int plan_hubs(lv_ctx *c, int32_t nh)
@@ -896,6 +904,7 @@ int one_level(lv_ctx *c, double lower, double *q_out, int32_t *passes_out)
     a.col = c->col;
     a.w = c->cw;
     a.w32 = c->cw32;
+    a.cpk = c->cpk;
     a.kdeg = c->kdeg;
     a.loopw = c->loopw;
     a.selfsum = c->selfsum;
@@ -1108,7 +1117,7 @@ int induce(lv_ctx *c, const int32_t *newc, const int32_t *newid_g, int32_t Kn)
     fill_i32(s, nuniq, K, 0);
     launch_cr_selfl(s, L.perm, c->selfsum, c->g_head, c->g_hscan, n, selfl);
     launch_cr_gather(s, estart, bound, mstart, c->col, c->cw, c->cw32, newc, n, nnz, ey, ew, ew32,
-                     ep, ewb);
+                     ep, ewb, c->cpk);
     // segments by length
     int64_t *sb = nullptr, *slen = nullptr;
     uint32_t *tk1 = nullptr, *tk2 = nullptr;
@@ -1251,6 +1260,7 @@ int induce(lv_ctx *c, const int32_t *newc, const int32_t *newid_g, int32_t Kn)
     c->col = uy;
     c->cw = uw;
     c->cw32 = uw32;
+    c->cpk = 0;
     c->kdeg = kdeg2;
     c->loopw = loop2;
     c->selfsum = self2;
@@ -1477,6 +1487,7 @@ int batch_one_level(lv_ctx *c, const std::vector<double> &lower, const std::vect
     a.col = c->col;
     a.w = c->cw;
     a.w32 = c->cw32;
+    a.cpk = c->cpk;
     a.kdeg = c->kdeg;
     a.loopw = c->loopw;
     a.selfsum = c->selfsum;
@@ -2442,6 +2453,16 @@ int lv_level_csr(lv_ctx *c, int64_t *rowptr, int32_t *col, int64_t *w)
     }
     LV_TRY(d2h(c, rowptr, c->rowptr, (int64_t)c->n + 1));
     LV_TRY(d2h(c, col, c->col, c->nnz));
+    if (c->cpk) {  // one word per entry: split on the host
+        LV_TRY(sync(c));
+        const uint32_t mask = (1u << c->cpk) - 1u;
+        for (int64_t p = 0; p < c->nnz; ++p) {
+            const uint32_t x = (uint32_t)col[p];
+            col[p] = (int32_t)(x >> c->cpk);
+            w[p] = (int64_t)(x & mask);
+        }
+        return LV_OK;
+    }
     if (c->cw32) {  // widened on the host
         std::vector<uint32_t> w32((size_t)c->nnz);
         LV_TRY(d2h(c, w32.data(), c->cw32, c->nnz));

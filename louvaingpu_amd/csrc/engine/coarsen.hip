@@ -179,7 +179,7 @@ __global__ __launch_bounds__(256) void k_cr_gather(const int64_t *estart, const 
                                                   const int64_t *cw, const uint32_t *cw32,
                                                   const int32_t *newc,
                                                   int64_t total, int32_t *ey, int64_t *ew,
-                                                  uint32_t *ew32, uint32_t *ep, int ewb)
+                                                  uint32_t *ew32, uint32_t *ep, int ewb, int cpk)
 {
     __shared__ int64_t s_est[kGatherLds];
     const int64_t b = blockIdx.x;
@@ -228,9 +228,14 @@ __global__ __launch_bounds__(256) void k_cr_gather(const int64_t *estart, const 
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         cc[r] = ok[r] ? __builtin_nontemporal_load(&col[pp[r]]) : 0;
-        uu[r] = !ok[r] ? 0
-                : cw32 ? (int64_t)__builtin_nontemporal_load(&cw32[pp[r]])
-                       : __builtin_nontemporal_load(&cw[pp[r]]);
+        if (cpk) {  // the level's entries are col << cpk | weight
+            uu[r] = (int64_t)((uint32_t)cc[r] & ((1u << cpk) - 1u));
+            cc[r] = (int32_t)((uint32_t)cc[r] >> cpk);
+        } else {
+            uu[r] = !ok[r] ? 0
+                    : cw32 ? (int64_t)__builtin_nontemporal_load(&cw32[pp[r]])
+                           : __builtin_nontemporal_load(&cw[pp[r]]);
+        }
     }
 #pragma unroll
     for (int r = 0; r < R; ++r)
@@ -273,14 +278,14 @@ void launch_cr_gather(hipStream_t s, const int64_t *estart, int32_t *bound, cons
                       const int32_t *col, const int64_t *cw, const uint32_t *cw32,
                       const int32_t *newc, int32_t n,
                       int64_t total, int32_t *ey, int64_t *ew, uint32_t *ew32, uint32_t *ep,
-                      int ewb)
+                      int ewb, int cpk)
 {
     if (total <= 0) return;
     const int64_t nb = (total + kGatherT - 1) / kGatherT;
     hipLaunchKernelGGL(k_lbs_bounds, grid_for(nb + 1, 256), 256, 0, s, estart, n, total, kGatherT,
                        nb, bound);
     hipLaunchKernelGGL(k_cr_gather, (int)nb, 256, 0, s, estart, bound, mstart, col, cw, cw32, newc,
-                       total, ey, ew, ew32, ep, ewb);
+                       total, ey, ew, ew32, ep, ewb, cpk);
 }
 
 // ---------------------------------------------------------------------------

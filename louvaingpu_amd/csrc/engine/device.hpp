@@ -290,6 +290,19 @@ __device__ __forceinline__ void put_w(int64_t *w, uint32_t *w32, int64_t p, int6
         w[p] = u;
 }
 
+// Stores CSR entry p (column cc, weight u): with pk > 0 as one word
+// cc << pk | u in col, else the column and the weight (put_w).
+__device__ __forceinline__ void put_entry(int32_t *col, int64_t *w, uint32_t *w32, int pk,
+                                          int64_t p, int32_t cc, int64_t u)
+{
+    if (pk) {
+        col[p] = (int32_t)(((uint32_t)cc << pk) | (uint32_t)u);
+    } else {
+        col[p] = cc;
+        put_w(w, w32, p, u);
+    }
+}
+
 struct SegCarry {
     long long key = LLONG_MIN;
     ull sum = 0;

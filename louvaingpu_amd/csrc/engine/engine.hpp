@@ -142,9 +142,10 @@ struct SweepArgs {
     const uint32_t *deg32;  // deg[] of narrow levels
     const int64_t *rowptr;  // row v is [rowptr[v], rowend[v]) of col/w (rowend = rowptr + 1
     const int64_t *rowend;  //   when the rows are packed; coarsened levels leave gaps)
-    const int32_t *col;
-    const int64_t *w;       // row entry weights (units), or w32 when every weight fits 32 bits
-    const uint32_t *w32;
+    const int32_t *col;     // columns, or with cpk > 0 one word col << cpk | weight per
+    const int64_t *w;       //   entry (w, w32 null); row entry weights (units), or w32
+    const uint32_t *w32;    //   when every weight fits 32 bits
+    int cpk;
     const int64_t *kdeg;
     const int64_t *loopw;
     const int64_t *selfsum;
@@ -489,7 +490,7 @@ void launch_csr_sorted_fill(hipStream_t s, bool check, const int32_t *src, const
                             int64_t e, int32_t n, int wbits, int vshift, int kb, const int64_t *fs,
                             const uint32_t *skey, const uint64_t *spay, int64_t nr,
                             const int64_t *rs, int64_t *rowptr, int64_t *ffwd, int32_t *col,
-                            int64_t *cw, uint32_t *cw32, int64_t *kdeg, DevFlags *f);
+                            int64_t *cw, uint32_t *cw32, int pk, int64_t *kdeg, DevFlags *f);
 // One-word reverse entries dst << (64 - kb) | src << (64 - 2 kb) | w, and row
 // starts over such words sorted by their top bits (key = word >> kshift).
 // General path as one word per entry (row | col | w, row = n for self-loops);
@@ -565,7 +566,7 @@ void launch_cr_gather(hipStream_t s, const int64_t *estart, int32_t *bound, cons
                       const int32_t *col, const int64_t *cw, const uint32_t *cw32,
                       const int32_t *newc, int32_t n,
                       int64_t total, int32_t *ey, int64_t *ew, uint32_t *ew32, uint32_t *ep,
-                      int ewb);
+                      int ewb, int cpk);
 void launch_cr_seg_tier(hipStream_t s, const int32_t *gstart, const int64_t *estart,
                         const int32_t *newid_g, int32_t K, int64_t *sb, int64_t *slen,
                         uint32_t *tkey, int32_t *sid, long long *tier_cnt, int32_t hub_q);

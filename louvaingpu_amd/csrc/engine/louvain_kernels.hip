@@ -544,8 +544,8 @@ template <bool CHECK>
 __global__ __launch_bounds__(256) void k_fill_rev(const uint32_t *skey, const uint64_t *spay,
                                                  int64_t nr, int wbits, int vshift, uint64_t cmask,
                                                  const int64_t *rs, const int64_t *rowptr,
-                                                 int32_t *col, int64_t *cw, uint32_t *cw32, int64_t *kdeg,
-                                                 DevFlags *f)
+                                                 int32_t *col, int64_t *cw, uint32_t *cw32, int pk,
+                                                 int64_t *kdeg, DevFlags *f)
 {
     int bad = 0;
     constexpr int64_t SPAN = 256 * kSpanRounds;
@@ -581,8 +581,7 @@ __global__ __launch_bounds__(256) void k_fill_rev(const uint32_t *skey, const ui
                 const int64_t u = (int64_t)(x & ((1ull << wbits) - 1ull));
                 const int64_t pos = ro[rr] + r;
                 const int32_t cc = (int32_t)((x >> wbits) & cmask);
-                col[pos] = cc;
-                put_w(cw, cw32, pos, u);
+                put_entry(col, cw, cw32, pk, pos, cc, u);
                 row = v;
                 wp = (ull)(u == 0 ? 1 : u);
                 if constexpr (CHECK) {
@@ -607,7 +606,7 @@ __global__ __launch_bounds__(256) void k_fill_fwd(const int32_t *src, const int3
                                                  const int64_t *fs, const int64_t *rs,
                                                  const uint64_t *spay, int wbits, uint64_t cmask,
                                                  const int64_t *ffwd, int32_t *col, int64_t *cw, uint32_t *cw32,
-                                                 int64_t *kdeg, DevFlags *f)
+                                                 int pk, int64_t *kdeg, DevFlags *f)
 {
     __shared__ int32_t s_wc[4];
     int bad = 0;
@@ -651,8 +650,7 @@ __global__ __launch_bounds__(256) void k_fill_fwd(const int32_t *src, const int3
                     const int64_t sli = off + __popcll(m[rr] & below);  // self-loops before i
                     const int64_t pos = fo[rr] + (i - sli);
                     const int64_t u = uw[rr];
-                    col[pos] = b;
-                    put_w(cw, cw32, pos, u);
+                    put_entry(col, cw, cw32, pk, pos, b, u);
                     wp = (ull)(u == 0 ? 1 : u);
                     if constexpr (CHECK) {
                         const int64_t f0 = fs[v];
@@ -707,7 +705,7 @@ void launch_csr_sorted_fill(hipStream_t s, bool check, const int32_t *src, const
                             int64_t e, int32_t n, int wbits, int vshift, int kb, const int64_t *fs,
                             const uint32_t *skey, const uint64_t *spay, int64_t nr,
                             const int64_t *rs, int64_t *rowptr, int64_t *ffwd, int32_t *col,
-                            int64_t *cw, uint32_t *cw32, int64_t *kdeg, DevFlags *f)
+                            int64_t *cw, uint32_t *cw32, int pk, int64_t *kdeg, DevFlags *f)
 {
     const uint64_t cmask = kb >= 64 ? ~0ull : ((1ull << kb) - 1ull);
     hipLaunchKernelGGL(k_rowptr_sum, grid_for((int64_t)n + 1, 256, 1 << 20), 256, 0, s, rs, fs,
@@ -716,20 +714,20 @@ void launch_csr_sorted_fill(hipStream_t s, bool check, const int32_t *src, const
     if (nr > 0) {
         if (check)
             hipLaunchKernelGGL(k_fill_rev<true>, grid_for(nr, SPAN, 8192), 256, 0, s, skey, spay, nr,
-                               wbits, vshift, cmask, rs, rowptr, col, cw, cw32, kdeg, f);
+                               wbits, vshift, cmask, rs, rowptr, col, cw, cw32, pk, kdeg, f);
         else
             hipLaunchKernelGGL(k_fill_rev<false>, grid_for(nr, SPAN, 1 << 20), 256, 0, s, skey, spay,
-                               nr, wbits, vshift, cmask, rs, rowptr, col, cw, cw32, kdeg, f);
+                               nr, wbits, vshift, cmask, rs, rowptr, col, cw, cw32, pk, kdeg, f);
     }
     if (e > 0) {
         if (check)
             hipLaunchKernelGGL(k_fill_fwd<true>, grid_for(e, kEdgeTile, 8192), 256, 0, s, src, dst,
-                               w, tscan, vscan, e, fs, rs, spay, wbits, cmask, ffwd, col, cw, cw32, kdeg,
-                               f);
+                               w, tscan, vscan, e, fs, rs, spay, wbits, cmask, ffwd, col, cw, cw32, pk,
+                               kdeg, f);
         else
             hipLaunchKernelGGL(k_fill_fwd<false>, grid_for(e, kEdgeTile, 1 << 20), 256, 0, s, src,
                                dst, w, tscan, vscan, e, fs, rs, spay, wbits, cmask, ffwd, col, cw, cw32,
-                               kdeg, f);
+                               pk, kdeg, f);
     }
 }
 
@@ -1142,6 +1140,37 @@ __device__ __forceinline__ int64_t w_of(const SweepArgs &a, int64_t p)
         return __builtin_nontemporal_load(&a.w[p]);
 }
 
+// Row entry p as (column j, weight w) for a level's entry layout WM: 0 col +
+// 8-byte weights, 1 col + 4-byte weights, 2 one word col << cpk | weight (a
+// level 0 whose ids and largest weight fit 32 bits); streamed past the caches.
+template <int WM>
+__device__ __forceinline__ void entry_at(const SweepArgs &a, int64_t p, int32_t &j, int64_t &w)
+{
+    if constexpr (WM == 2) {
+        const uint32_t x = __builtin_nontemporal_load((const uint32_t *)&a.col[p]);
+        j = (int32_t)(x >> a.cpk);
+        w = (int64_t)(x & ((1u << a.cpk) - 1u));
+    } else {
+        j = __builtin_nontemporal_load(&a.col[p]);
+        w = w_of<WM == 1>(a, p);
+    }
+}
+
+template <int WM>
+__device__ __forceinline__ int32_t col_at(const SweepArgs &a, int64_t p)
+{
+    if constexpr (WM == 2)
+        return (int32_t)(__builtin_nontemporal_load((const uint32_t *)&a.col[p]) >> a.cpk);
+    else
+        return __builtin_nontemporal_load(&a.col[p]);
+}
+
+// The entry layout of a level (see entry_at).
+__host__ __device__ __forceinline__ int entry_mode(const SweepArgs &a)
+{
+    return a.cpk ? 2 : (a.w32 ? 1 : 0);
+}
+
 // The vertex's own record, with the size of its community (the wide records
 // are always written; narrow levels gather neighbours from lab8).
 __device__ __forceinline__ Lab own_lab(const SweepArgs &a, int32_t v) { return a.lab[v]; }
@@ -1381,7 +1410,7 @@ void launch_move_deltas(hipStream_t s, const SweepArgs &a, int32_t n, bool own_d
 // With DL the gather is the neighbour's 16-byte label record and the
 // community total goes into the table (dg); otherwise only C[target] is
 // gathered and the totals are read per candidate later.
-template <int S, bool DL, bool W4, typename V, typename D>
+template <int S, bool DL, int WM, typename V, typename D>
 __device__ __forceinline__ void insert_row(const SweepArgs &a, int32_t *key, V *val, D *dg,
                                            int l2, int64_t beg, int64_t end, int lane)
 {
@@ -1391,8 +1420,9 @@ __device__ __forceinline__ void insert_row(const SweepArgs &a, int32_t *key, V *
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int64_t p = p0 + (int64_t)u * S;
-            j[u] = (p < end) ? __builtin_nontemporal_load(&a.col[p]) : -1;
-            w[u] = (p < end) ? w_of<W4>(a, p) : 0;
+            j[u] = -1;
+            w[u] = 0;
+            if (p < end) entry_at<WM>(a, p, j[u], w[u]);
         }
         if constexpr (DL) {
             Lab L[4];
@@ -1416,7 +1446,7 @@ __device__ __forceinline__ void insert_row(const SweepArgs &a, int32_t *key, V *
 
 // G lanes per vertex, 256/G vertices per block, an LDS table of up to
 // 2^LOG2CAP slots per vertex (2^MINLOG2 .. 2^LOG2CAP sized to the row).
-template <int G, int LOG2CAP, int MINLOG2, bool DL, bool B, bool W4, bool N32>
+template <int G, int LOG2CAP, int MINLOG2, bool DL, bool B, int WM, bool N32>
 __global__ __launch_bounds__(256) void k_sweep_group(SweepArgs a, const int32_t *__restrict__ verts,
                                                     int32_t nv)
 {
@@ -1457,7 +1487,7 @@ __global__ __launch_bounds__(256) void k_sweep_group(SweepArgs a, const int32_t 
         }
         __syncthreads();
         if (active) {
-            insert_row<G, DL, W4>(a, key, val, dg, l2, beg, end, lg);
+            insert_row<G, DL, WM>(a, key, val, dg, l2, beg, end, lg);
             if (lg == 0 && loop) {
                 if constexpr (DL)
                     lds_insert_d(key, val, dg, l2, me.c, 0, me.deg);
@@ -1490,7 +1520,7 @@ __global__ __launch_bounds__(256) void k_sweep_group(SweepArgs a, const int32_t 
 
 // One vertex per block of BLOCK threads with an LDS table of up to
 // 2^LOG2CAP slots sized to the row.
-template <int BLOCK, int LOG2CAP, int MINLOG2, bool B, bool W4, bool N32>
+template <int BLOCK, int LOG2CAP, int MINLOG2, bool B, int WM, bool N32>
 __global__ __launch_bounds__(BLOCK) void k_sweep_block(SweepArgs a, const int32_t *__restrict__ verts,
                                                       int32_t nv)
 {
@@ -1514,7 +1544,7 @@ __global__ __launch_bounds__(BLOCK) void k_sweep_block(SweepArgs a, const int32_
             s_val[s] = 0;
         }
         __syncthreads();
-        insert_row<BLOCK, false, W4>(a, s_key, s_val, (TabD<N32> *)nullptr, l2, beg, end, tid);
+        insert_row<BLOCK, false, WM>(a, s_key, s_val, (TabD<N32> *)nullptr, l2, beg, end, tid);
         if (tid == 0 && loop) lds_insert(s_key, s_val, l2, me.c, 0);
         __syncthreads();
         int cnt = 0;
@@ -1538,21 +1568,21 @@ static void launch_grid(KernelT k, int64_t blocks, int block, hipStream_t s, con
     hipLaunchKernelGGL(k, g, block, 0, s, a, verts, nv);
 }
 
-template <bool B, bool W4, bool N32>
+template <bool B, int WM, bool N32>
 static void launch_sweep_tier_t(hipStream_t s, int tier, const SweepArgs &a, const int32_t *verts,
                                 int32_t nv)
 {
     switch (tier) {
-    case 1: launch_grid(k_sweep_group<4, 3, 3, true, B, W4, N32>, (nv + 63) / 64, 256, s, a, verts, nv); break;
-    case 2: launch_grid(k_sweep_group<8, 4, 4, true, B, W4, N32>, (nv + 31) / 32, 256, s, a, verts, nv); break;
-    case 3: launch_grid(k_sweep_group<16, 5, 5, true, B, W4, N32>, (nv + 15) / 16, 256, s, a, verts, nv); break;
-    case 4: launch_grid(k_sweep_group<32, 6, 6, true, B, W4, N32>, (nv + 7) / 8, 256, s, a, verts, nv); break;
-    case 5: launch_grid(k_sweep_group<64, 7, 7, true, B, W4, N32>, (nv + 3) / 4, 256, s, a, verts, nv); break;
-    case 6: launch_grid(k_sweep_group<64, 8, 8, true, B, W4, N32>, (nv + 3) / 4, 256, s, a, verts, nv); break;
-    case 7: launch_grid(k_sweep_group<64, 9, 9, true, B, W4, N32>, (nv + 3) / 4, 256, s, a, verts, nv); break;
-    case 8: launch_grid(k_sweep_group<64, 10, 10, false, B, W4, N32>, (nv + 3) / 4, 256, s, a, verts, nv); break;
-    case 9: launch_grid(k_sweep_block<256, 12, 11, B, W4, N32>, nv, 256, s, a, verts, nv); break;
-    case 10: launch_grid(k_sweep_block<512, 13, 12, B, W4, N32>, nv, 512, s, a, verts, nv); break;
+    case 1: launch_grid(k_sweep_group<4, 3, 3, true, B, WM, N32>, (nv + 63) / 64, 256, s, a, verts, nv); break;
+    case 2: launch_grid(k_sweep_group<8, 4, 4, true, B, WM, N32>, (nv + 31) / 32, 256, s, a, verts, nv); break;
+    case 3: launch_grid(k_sweep_group<16, 5, 5, true, B, WM, N32>, (nv + 15) / 16, 256, s, a, verts, nv); break;
+    case 4: launch_grid(k_sweep_group<32, 6, 6, true, B, WM, N32>, (nv + 7) / 8, 256, s, a, verts, nv); break;
+    case 5: launch_grid(k_sweep_group<64, 7, 7, true, B, WM, N32>, (nv + 3) / 4, 256, s, a, verts, nv); break;
+    case 6: launch_grid(k_sweep_group<64, 8, 8, true, B, WM, N32>, (nv + 3) / 4, 256, s, a, verts, nv); break;
+    case 7: launch_grid(k_sweep_group<64, 9, 9, true, B, WM, N32>, (nv + 3) / 4, 256, s, a, verts, nv); break;
+    case 8: launch_grid(k_sweep_group<64, 10, 10, false, B, WM, N32>, (nv + 3) / 4, 256, s, a, verts, nv); break;
+    case 9: launch_grid(k_sweep_block<256, 12, 11, B, WM, N32>, nv, 256, s, a, verts, nv); break;
+    case 10: launch_grid(k_sweep_block<512, 13, 12, B, WM, N32>, nv, 512, s, a, verts, nv); break;
     default: break;
     }
 }
@@ -1562,22 +1592,27 @@ void launch_sweep_tier(hipStream_t s, int tier, const SweepArgs &a, const int32_
                        bool batch)
 {
     if (nv <= 0) return;
-    // 4-byte tables where the level is narrow and keeps 4-byte weights
-    const bool w4 = a.w32 != nullptr, n32 = w4 && a.lab8 != nullptr;
-    if (batch) {
+    // 4-byte tables where the level is narrow and keeps 4-byte (or packed) weights
+    const int wm = entry_mode(a);
+    const bool n32 = wm != 0 && a.lab8 != nullptr;
+    if (batch) {  // (graph sets keep separate columns and weights)
         if (n32)
-            launch_sweep_tier_t<true, true, true>(s, tier, a, verts, nv);
-        else if (w4)
-            launch_sweep_tier_t<true, true, false>(s, tier, a, verts, nv);
+            launch_sweep_tier_t<true, 1, true>(s, tier, a, verts, nv);
+        else if (wm == 1)
+            launch_sweep_tier_t<true, 1, false>(s, tier, a, verts, nv);
         else
-            launch_sweep_tier_t<true, false, false>(s, tier, a, verts, nv);
+            launch_sweep_tier_t<true, 0, false>(s, tier, a, verts, nv);
     } else {
-        if (n32)
-            launch_sweep_tier_t<false, true, true>(s, tier, a, verts, nv);
-        else if (w4)
-            launch_sweep_tier_t<false, true, false>(s, tier, a, verts, nv);
+        if (wm == 2 && n32)
+            launch_sweep_tier_t<false, 2, true>(s, tier, a, verts, nv);
+        else if (wm == 2)
+            launch_sweep_tier_t<false, 2, false>(s, tier, a, verts, nv);
+        else if (n32)
+            launch_sweep_tier_t<false, 1, true>(s, tier, a, verts, nv);
+        else if (wm == 1)
+            launch_sweep_tier_t<false, 1, false>(s, tier, a, verts, nv);
         else
-            launch_sweep_tier_t<false, false, false>(s, tier, a, verts, nv);
+            launch_sweep_tier_t<false, 0, false>(s, tier, a, verts, nv);
     }
 }
 
@@ -1588,7 +1623,7 @@ void launch_sweep_tier(hipStream_t s, int tier, const SweepArgs &a, const int32_
 // and the only gather per entry is deg[j].  n_v = row length + has_loop (the
 // own community enters through a self-loop with W = 0).
 
-template <int G, bool B, bool W4>
+template <int G, bool B, int WM>
 __global__ __launch_bounds__(256) void k_sweep1_group(SweepArgs a, const int32_t *__restrict__ verts,
                                                      int32_t nv)
 {
@@ -1615,8 +1650,9 @@ __global__ __launch_bounds__(256) void k_sweep1_group(SweepArgs a, const int32_t
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const int64_t p = p0 + (int64_t)u * G;
-                    j[u] = (p < end) ? __builtin_nontemporal_load(&a.col[p]) : -1;
-                    w[u] = (p < end) ? w_of<W4>(a, p) : 0;
+                    j[u] = -1;
+                    w[u] = 0;
+                    if (p < end) entry_at<WM>(a, p, j[u], w[u]);
                 }
 #pragma unroll
                 for (int u = 0; u < 4; ++u) d[u] = (j[u] >= 0) ? deg_of(a, j[u]) : 0;
@@ -1633,7 +1669,7 @@ __global__ __launch_bounds__(256) void k_sweep1_group(SweepArgs a, const int32_t
     }
 }
 
-template <int BLOCK, bool B, bool W4>
+template <int BLOCK, bool B, int WM>
 __global__ __launch_bounds__(BLOCK) void k_sweep1_block(SweepArgs a, const int32_t *__restrict__ verts,
                                                        int32_t nv)
 {
@@ -1659,8 +1695,9 @@ __global__ __launch_bounds__(BLOCK) void k_sweep1_block(SweepArgs a, const int32
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const int64_t p = p0 + (int64_t)u * BLOCK;
-                j[u] = (p < end) ? __builtin_nontemporal_load(&a.col[p]) : -1;
-                w[u] = (p < end) ? w_of<W4>(a, p) : 0;
+                j[u] = -1;
+                w[u] = 0;
+                if (p < end) entry_at<WM>(a, p, j[u], w[u]);
             }
 #pragma unroll
             for (int u = 0; u < 4; ++u) d[u] = (j[u] >= 0) ? deg_of(a, j[u]) : 0;
@@ -1696,21 +1733,21 @@ __global__ __launch_bounds__(BLOCK) void k_sweep1_block(SweepArgs a, const int32
     }
 }
 
-template <bool B, bool W4>
+template <bool B, int WM>
 static void launch_sweep1_tier_t(hipStream_t s, int tier, const SweepArgs &a, const int32_t *verts,
                                  int32_t nv)
 {
     switch (tier) {
-    case 1: launch_grid(k_sweep1_group<4, B, W4>, (nv + 63) / 64, 256, s, a, verts, nv); break;
-    case 2: launch_grid(k_sweep1_group<8, B, W4>, (nv + 31) / 32, 256, s, a, verts, nv); break;
-    case 3: launch_grid(k_sweep1_group<16, B, W4>, (nv + 15) / 16, 256, s, a, verts, nv); break;
-    case 4: launch_grid(k_sweep1_group<32, B, W4>, (nv + 7) / 8, 256, s, a, verts, nv); break;
+    case 1: launch_grid(k_sweep1_group<4, B, WM>, (nv + 63) / 64, 256, s, a, verts, nv); break;
+    case 2: launch_grid(k_sweep1_group<8, B, WM>, (nv + 31) / 32, 256, s, a, verts, nv); break;
+    case 3: launch_grid(k_sweep1_group<16, B, WM>, (nv + 15) / 16, 256, s, a, verts, nv); break;
+    case 4: launch_grid(k_sweep1_group<32, B, WM>, (nv + 7) / 8, 256, s, a, verts, nv); break;
     case 5:
     case 6:
     case 7:
-    case 8: launch_grid(k_sweep1_group<64, B, W4>, (nv + 3) / 4, 256, s, a, verts, nv); break;
-    case 9: launch_grid(k_sweep1_block<256, B, W4>, nv, 256, s, a, verts, nv); break;
-    case 10: launch_grid(k_sweep1_block<512, B, W4>, nv, 512, s, a, verts, nv); break;
+    case 8: launch_grid(k_sweep1_group<64, B, WM>, (nv + 3) / 4, 256, s, a, verts, nv); break;
+    case 9: launch_grid(k_sweep1_block<256, B, WM>, nv, 256, s, a, verts, nv); break;
+    case 10: launch_grid(k_sweep1_block<512, B, WM>, nv, 512, s, a, verts, nv); break;
     default: break;
     }
 }
@@ -1719,18 +1756,20 @@ void launch_sweep1_tier(hipStream_t s, int tier, const SweepArgs &a, const int32
                         int32_t nv, bool batch)
 {
     if (nv <= 0) return;
-    const bool w4 = a.w32 != nullptr;
+    const int wm = entry_mode(a);
     if (batch)
-        w4 ? launch_sweep1_tier_t<true, true>(s, tier, a, verts, nv)
-           : launch_sweep1_tier_t<true, false>(s, tier, a, verts, nv);
+        wm == 1 ? launch_sweep1_tier_t<true, 1>(s, tier, a, verts, nv)
+                : launch_sweep1_tier_t<true, 0>(s, tier, a, verts, nv);
+    else if (wm == 2)
+        launch_sweep1_tier_t<false, 2>(s, tier, a, verts, nv);
     else
-        w4 ? launch_sweep1_tier_t<false, true>(s, tier, a, verts, nv)
-           : launch_sweep1_tier_t<false, false>(s, tier, a, verts, nv);
+        wm == 1 ? launch_sweep1_tier_t<false, 1>(s, tier, a, verts, nv)
+                : launch_sweep1_tier_t<false, 0>(s, tier, a, verts, nv);
 }
 
 // Hubs: one block per chunk of a hub's entries (entry t == row length is the
 // self-loop's own community, W = 0), then one wavefront per hub over its chunks.
-template <bool B, bool W4>
+template <bool B, int WM>
 __global__ __launch_bounds__(256) void k_hub1_chunk(SweepArgs a, HubArgs h)
 {
     __shared__ int r_cnt[4];
@@ -1760,8 +1799,9 @@ __global__ __launch_bounds__(256) void k_hub1_chunk(SweepArgs a, HubArgs h)
         for (int u = 0; u < 4; ++u) {
             const int64_t t = p0 + (int64_t)u * 256;
             own[u] = (t == d);
-            jj[u] = (t < t1 && t < d) ? __builtin_nontemporal_load(&a.col[rb + t]) : -1;
-            w[u] = (jj[u] >= 0) ? w_of<W4>(a, rb + t) : 0;
+            jj[u] = -1;
+            w[u] = 0;
+            if (t < t1 && t < d) entry_at<WM>(a, rb + t, jj[u], w[u]);
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) dg[u] = (jj[u] >= 0) ? deg_of(a, jj[u]) : 0;
@@ -1831,18 +1871,20 @@ int run_hub_sweep1(hipStream_t s, const SweepArgs &a, const HubArgs &h, int32_t 
                    bool batch)
 {
     if (nh <= 0) return LV_OK;
-    const bool w4 = a.w32 != nullptr;
+    const int wm = entry_mode(a);
     if (batch) {
-        if (w4)
-            hipLaunchKernelGGL((k_hub1_chunk<true, true>), (int)nchunks, 256, 0, s, a, h);
+        if (wm == 1)
+            hipLaunchKernelGGL((k_hub1_chunk<true, 1>), (int)nchunks, 256, 0, s, a, h);
         else
-            hipLaunchKernelGGL((k_hub1_chunk<true, false>), (int)nchunks, 256, 0, s, a, h);
+            hipLaunchKernelGGL((k_hub1_chunk<true, 0>), (int)nchunks, 256, 0, s, a, h);
         hipLaunchKernelGGL(k_hub1_final<true>, (int)((nh + 3) / 4), 256, 0, s, a, h, nh);
     } else {
-        if (w4)
-            hipLaunchKernelGGL((k_hub1_chunk<false, true>), (int)nchunks, 256, 0, s, a, h);
+        if (wm == 2)
+            hipLaunchKernelGGL((k_hub1_chunk<false, 2>), (int)nchunks, 256, 0, s, a, h);
+        else if (wm == 1)
+            hipLaunchKernelGGL((k_hub1_chunk<false, 1>), (int)nchunks, 256, 0, s, a, h);
         else
-            hipLaunchKernelGGL((k_hub1_chunk<false, false>), (int)nchunks, 256, 0, s, a, h);
+            hipLaunchKernelGGL((k_hub1_chunk<false, 0>), (int)nchunks, 256, 0, s, a, h);
         hipLaunchKernelGGL(k_hub1_final<false>, (int)((nh + 3) / 4), 256, 0, s, a, h, nh);
     }
     LV_HIP(hipGetLastError());
@@ -1937,6 +1979,7 @@ void launch_hub_setup(hipStream_t s, const int32_t *hv, int32_t nh, const int64_
 
 // Entry t of hub j: t < d is row entry rowptr[v]+t, t == d the virtual
 // own-community entry (weight 0) of a hub with a self-loop.
+template <int WM>
 __global__ __launch_bounds__(256) void k_hub_count(SweepArgs a, HubArgs h)
 {
     __shared__ int hist[kHubMaxParts];
@@ -1959,7 +2002,7 @@ __global__ __launch_bounds__(256) void k_hub_count(SweepArgs a, HubArgs h)
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int64_t t = p0 + (int64_t)u * 256;
-            y[u] = (t < t1 && t < d) ? __builtin_nontemporal_load(&a.col[rb + t]) : -1;
+            y[u] = (t < t1 && t < d) ? col_at<WM>(a, rb + t) : -1;
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) c[u] = y[u] >= 0 ? a.C[y[u]] : own;
@@ -1994,7 +2037,7 @@ __device__ __forceinline__ PEntry hub_entry(const HubArgs &h, int64_t e)
     return h.buf[e];
 }
 
-template <bool W4>
+template <int WM>
 __global__ __launch_bounds__(256) void k_hub_scatter(SweepArgs a, HubArgs h)
 {
     __shared__ int hist[kHubMaxParts];
@@ -2024,7 +2067,11 @@ __global__ __launch_bounds__(256) void k_hub_scatter(SweepArgs a, HubArgs h)
         int32_t c = tc[t];
         int part = hub_part(c, P);
         int64_t pos = lbase[part] + atomicAdd(&hist[part], 1);
-        const int64_t w = (t < d) ? w_of<W4>(a, rb + t) : 0;
+        int64_t w = 0;
+        if (t < d) {
+            int32_t jx;
+            entry_at<WM>(a, rb + t, jx, w);
+        }
         if (h.buf4) {
             h.buf4[pos] = ((uint32_t)c << h.pwb) | (uint32_t)w;
         } else if (h.buf8) {
@@ -2161,12 +2208,18 @@ int run_hub_sweep(Arena &A, hipStream_t s, const SweepArgs &a, const HubArgs &h,
 {
     if (nh <= 0) return LV_OK;
     // pcount is zero here (the plan clears it, every scatter counts it back down)
-    hipLaunchKernelGGL(k_hub_count, (int)nchunks, 256, 0, s, a, h);
-    LV_TRY(scan_exclusive_i64(A, s, h.pcount, (int64_t *)h.poff, nparts, (int64_t *)h.poff + nparts));
-    if (a.w32)
-        hipLaunchKernelGGL(k_hub_scatter<true>, (int)nchunks, 256, 0, s, a, h);
+    const int wm = entry_mode(a);
+    if (wm == 2)
+        hipLaunchKernelGGL(k_hub_count<2>, (int)nchunks, 256, 0, s, a, h);
     else
-        hipLaunchKernelGGL(k_hub_scatter<false>, (int)nchunks, 256, 0, s, a, h);
+        hipLaunchKernelGGL(k_hub_count<0>, (int)nchunks, 256, 0, s, a, h);
+    LV_TRY(scan_exclusive_i64(A, s, h.pcount, (int64_t *)h.poff, nparts, (int64_t *)h.poff + nparts));
+    if (wm == 2)
+        hipLaunchKernelGGL(k_hub_scatter<2>, (int)nchunks, 256, 0, s, a, h);
+    else if (wm == 1)
+        hipLaunchKernelGGL(k_hub_scatter<1>, (int)nchunks, 256, 0, s, a, h);
+    else
+        hipLaunchKernelGGL(k_hub_scatter<0>, (int)nchunks, 256, 0, s, a, h);
     // a partition's sums stay below a narrow level's total degree: 4-byte table
     const bool n32 = a.lab8 != nullptr;
     if (batch) {
