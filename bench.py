@@ -157,17 +157,17 @@ def measured_traffic(workload_key, tier):
     pats, markers = spec
 
     def lookup(pat):
-        hits = [v for k, v in per_kernel.items() if pat in k]
-        return hits[0] if len(hits) == 1 else None
+        # every template instance of the kernel (entry layouts, table widths)
+        return [v for k, v in per_kernel.items() if pat in k]
 
     total, launches = 0.0, 0
     for pat in pats:
-        hit = lookup(pat)
-        if hit is None:
+        hits = lookup(pat)
+        if not hits:
             return None
-        total += hit[0] * hit[1]
+        total += sum(calls * nbytes for calls, nbytes in hits)
     for pat in (markers or pats):
-        launches += lookup(pat)[0]
+        launches += sum(calls for calls, _ in lookup(pat))
     return total / launches if launches else None
 
 

@@ -1150,6 +1150,7 @@ int induce(lv_ctx *c, const int32_t *newc, const int32_t *newid_g, int32_t Kn)
     ca.ew32 = ew32;
     ca.ep = ep;
     ca.ewb = ewb;
+    ca.n32 = nar ? 1 : 0;
     ca.sb = sb;
     ca.slen = slen;
     ca.newid_g = newid_g;

@@ -551,11 +551,11 @@ static void launch_cr_agg_tier_t(hipStream_t s, int tier, const CoarsenArgs &a,
     }
 }
 
-// Narrow levels (the 4-byte entry weights ew32) aggregate in 4-byte sums.
+// Narrow levels (every sum below the total degree, < 2^32) aggregate in 4-byte sums.
 void launch_cr_agg_tier(hipStream_t s, int tier, const CoarsenArgs &a, const int32_t *segs, int32_t ns)
 {
     if (ns <= 0) return;
-    if (a.ew32)
+    if (a.n32)
         launch_cr_agg_tier_t<true>(s, tier, a, segs, ns);
     else
         launch_cr_agg_tier_t<false>(s, tier, a, segs, ns);

@@ -226,6 +226,7 @@ struct CoarsenArgs {
     const uint32_t *ew32;
     const uint32_t *ep;     // or both in one word, newc << ewb | w' (level 0, when the
     int ewb;                //   next level's ids and the largest weight fit 32 bits)
+    int n32;                // narrow level: 4-byte LDS sums in the segment tables
     const int64_t *sb;      // first entry of group g
     const int64_t *slen;    // entries of group g
     const int32_t *newid_g; // new id of group g
