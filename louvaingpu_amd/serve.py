@@ -2,7 +2,7 @@
 
 ``Server`` wraps the C++ request loop of ``liblouvain_host.so``: ``submit``
 queues a graph without copying it (the server keeps references to the arrays
-until ``wait`` returns, since the worker uploads them straight to the device),
+until ``wait`` returns, since the worker reads them when it uploads the set),
 worker threads (one engine context and HIP stream each) admit waiting requests
 by HBM footprint into graph sets and run them while later requests queue, and
 ``wait`` returns a request's final partition with its timings, or raises
