@@ -54,6 +54,9 @@ struct PendingTime {
 
 }  // namespace
 
+// Extra streams a pass's independent sweep tiers are spread over.
+constexpr int kFanStreams = 3;
+
 struct lv_ctx {
     lv_config cfg;
     int device = 0;
@@ -164,6 +167,11 @@ struct lv_ctx {
     PartSink *sink = nullptr;
     hipStream_t copy_stream = nullptr;
     std::vector<hipEvent_t> sink_ev;
+    // the sweep tiers of a pass fanned out over extra streams (created on first
+    // use; not in timing mode or on a borrowed stream)
+    hipStream_t fan[kFanStreams] = {};
+    hipEvent_t fan_fork = nullptr, fan_join[kFanStreams] = {};
+    bool fan_used[kFanStreams] = {};
     // lv_run_static: the device scalars and the arena live in the static
     // scratch array, the stream is the caller's, and the host copies of the
     // scalars are these ordinary (pageable) members
@@ -886,6 +894,63 @@ int init_level(lv_ctx *c)
     return LV_OK;
 }
 
+// The stream of every sweep tier of a pass.  The tiers write next[] of
+// disjoint vertex sets and read only what the pass does not change, so they
+// need no order among themselves: the hubs stay on the context stream, the
+// other tiers go (largest first) to whichever of the fan-out streams or the
+// context stream has the fewest entries so far, after a fork event, so small
+// and tail-end launches overlap the big ones.  fan_join makes the context
+// stream wait for them.  In timing mode (per-tier events) and on a borrowed
+// stream every tier stays on the context stream.
+int fan_tiers(lv_ctx *c, hipStream_t *ts)
+{
+    for (int t = 0; t <= kTiers; ++t) ts[t] = c->stream;
+    if (timing(c) || c->borrowed) return LV_OK;
+    if (!c->fan_fork) {
+        for (int i = 0; i < kFanStreams; ++i) {
+            LV_HIP(hipStreamCreateWithFlags(&c->fan[i], hipStreamNonBlocking));
+            LV_HIP(hipEventCreateWithFlags(&c->fan_join[i], hipEventDisableTiming));
+        }
+        LV_HIP(hipEventCreateWithFlags(&c->fan_fork, hipEventDisableTiming));
+    }
+    int order[kTiers];
+    int no = 0;
+    for (int t = 1; t < kTiers; ++t)
+        if (c->tier_cnt[t] > 0) order[no++] = t;
+    std::sort(order, order + no, [&](int x, int y) { return c->tier_nnz[x] > c->tier_nnz[y]; });
+    int64_t load[kFanStreams + 1] = {};
+    load[kFanStreams] = c->tier_nnz[kTiers];  // the context stream carries the hubs
+    bool forked = false;
+    for (int k = 0; k < no; ++k) {
+        int best = kFanStreams;
+        for (int i = 0; i < kFanStreams; ++i)
+            if (load[i] < load[best]) best = i;
+        load[best] += c->tier_nnz[order[k]] + c->tier_cnt[order[k]];
+        if (best == kFanStreams) continue;
+        if (!forked) {
+            LV_HIP(hipEventRecord(c->fan_fork, c->stream));
+            forked = true;
+        }
+        if (!c->fan_used[best]) {
+            LV_HIP(hipStreamWaitEvent(c->fan[best], c->fan_fork, 0));
+            c->fan_used[best] = true;
+        }
+        ts[order[k]] = c->fan[best];
+    }
+    return LV_OK;
+}
+
+int fan_join(lv_ctx *c)
+{
+    for (int i = 0; i < kFanStreams; ++i) {
+        if (!c->fan_used[i]) continue;
+        LV_HIP(hipEventRecord(c->fan_join[i], c->fan[i]));
+        LV_HIP(hipStreamWaitEvent(c->stream, c->fan_join[i], 0));
+        c->fan_used[i] = false;
+    }
+    return LV_OK;
+}
+
 int one_level(lv_ctx *c, double lower, double *q_out, int32_t *passes_out)
 {
     hipStream_t s = c->stream;
@@ -936,14 +1001,16 @@ int one_level(lv_ctx *c, double lower, double *q_out, int32_t *passes_out)
             t_end(c, tl, CAT_APPLY);
         }
         p_begin(c);
+        hipStream_t ts[kTiers + 1];
+        LV_TRY(fan_tiers(c, ts));
         for (int t = 1; t <= kTiers; ++t) {
             const int64_t nv = c->tier_cnt[t];
             if (nv == 0) continue;
             hipEvent_t t0 = t_begin(c);
             if (t < kTiers && singles)
-                launch_sweep1_tier(s, t, a, c->verts + c->tier_off[t], (int32_t)nv, false);
+                launch_sweep1_tier(ts[t], t, a, c->verts + c->tier_off[t], (int32_t)nv, false);
             else if (t < kTiers)
-                launch_sweep_tier(s, t, a, c->verts + c->tier_off[t], (int32_t)nv, false);
+                launch_sweep_tier(ts[t], t, a, c->verts + c->tier_off[t], (int32_t)nv, false);
             else if (singles)
                 LV_TRY(run_hub_sweep1(s, a, c->hub, c->n_hubs, c->hub_chunks, false));
             else
@@ -953,6 +1020,7 @@ int one_level(lv_ctx *c, double lower, double *q_out, int32_t *passes_out)
                   c->tier_nnz[t]);
             p_tier(c, t, nv);
         }
+        LV_TRY(fan_join(c));
         LV_TRY(p_fetch(c));
         if (!last) {
             hipEvent_t td = t_begin(c);
@@ -1528,14 +1596,16 @@ int batch_one_level(lv_ctx *c, const std::vector<double> &lower, const std::vect
             t_end(c, tl, CAT_APPLY);
         }
         p_begin(c);
+        hipStream_t ts[kTiers + 1];
+        LV_TRY(fan_tiers(c, ts));
         for (int t = 1; t <= kTiers; ++t) {
             const int64_t nv = c->tier_cnt[t];
             if (nv == 0) continue;
             hipEvent_t t0 = t_begin(c);
             if (t < kTiers && singles)
-                launch_sweep1_tier(s, t, a, c->verts + c->tier_off[t], (int32_t)nv, true);
+                launch_sweep1_tier(ts[t], t, a, c->verts + c->tier_off[t], (int32_t)nv, true);
             else if (t < kTiers)
-                launch_sweep_tier(s, t, a, c->verts + c->tier_off[t], (int32_t)nv, true);
+                launch_sweep_tier(ts[t], t, a, c->verts + c->tier_off[t], (int32_t)nv, true);
             else if (singles)
                 LV_TRY(run_hub_sweep1(s, a, c->hub, c->n_hubs, c->hub_chunks, true));
             else
@@ -1545,6 +1615,7 @@ int batch_one_level(lv_ctx *c, const std::vector<double> &lower, const std::vect
                   c->tier_nnz[t]);
             p_tier(c, t, nv);
         }
+        LV_TRY(fan_join(c));
         LV_TRY(p_fetch(c));
         {
             hipEvent_t td = t_begin(c);
@@ -1891,6 +1962,12 @@ int lv_destroy(lv_ctx *c)
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
     for (auto e : c->sink_ev) (void)hipEventDestroy(e);
     if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
+    for (int i = 0; i < kFanStreams; ++i) {
+        if (c->fan[i]) (void)hipStreamSynchronize(c->fan[i]);
+        if (c->fan[i]) (void)hipStreamDestroy(c->fan[i]);
+        if (c->fan_join[i]) (void)hipEventDestroy(c->fan_join[i]);
+    }
+    if (c->fan_fork) (void)hipEventDestroy(c->fan_fork);
     if (c->d_flags) (void)hipFree(c->d_flags);
     if (c->d_qacc) (void)hipFree(c->d_qacc);
     if (c->d_qpart) (void)hipFree(c->d_qpart);
