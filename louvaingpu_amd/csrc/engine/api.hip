@@ -894,18 +894,17 @@ int init_level(lv_ctx *c)
     return LV_OK;
 }
 
-// The stream of every sweep tier of a pass.  The tiers write next[] of
-// disjoint vertex sets and read only what the pass does not change, so they
-// need no order among themselves: the hubs stay on the context stream, the
-// other tiers go (largest first) to whichever of the fan-out streams or the
-// context stream has the fewest entries so far, after a fork event, so small
-// and tail-end launches overlap the big ones.  fan_join makes the context
-// stream wait for them.  In timing mode (per-tier events) and on a borrowed
-// stream every tier stays on the context stream.
-int fan_tiers(lv_ctx *c, hipStream_t *ts)
+// Streams for n independent launches of estimated sizes load[0..n): each
+// (largest first) goes to whichever of the fan-out streams or the context
+// stream (already carrying main_load) has the least so far; a fan-out stream
+// that gets one waits on a fork event recorded on the context stream, and
+// fan_join makes the context stream wait for it again.  Every launch stays on
+// the context stream in timing mode (per-phase events) and on a borrowed
+// stream.
+int fan_assign(lv_ctx *c, int n, const int64_t *load, int64_t main_load, hipStream_t *out)
 {
-    for (int t = 0; t <= kTiers; ++t) ts[t] = c->stream;
-    if (timing(c) || c->borrowed) return LV_OK;
+    for (int k = 0; k < n; ++k) out[k] = c->stream;
+    if (timing(c) || c->borrowed || n == 0) return LV_OK;
     if (!c->fan_fork) {
         for (int i = 0; i < kFanStreams; ++i) {
             LV_HIP(hipStreamCreateWithFlags(&c->fan[i], hipStreamNonBlocking));
@@ -913,19 +912,17 @@ int fan_tiers(lv_ctx *c, hipStream_t *ts)
         }
         LV_HIP(hipEventCreateWithFlags(&c->fan_fork, hipEventDisableTiming));
     }
-    int order[kTiers];
-    int no = 0;
-    for (int t = 1; t < kTiers; ++t)
-        if (c->tier_cnt[t] > 0) order[no++] = t;
-    std::sort(order, order + no, [&](int x, int y) { return c->tier_nnz[x] > c->tier_nnz[y]; });
-    int64_t load[kFanStreams + 1] = {};
-    load[kFanStreams] = c->tier_nnz[kTiers];  // the context stream carries the hubs
+    std::vector<int> order((size_t)n);
+    for (int k = 0; k < n; ++k) order[(size_t)k] = k;
+    std::sort(order.begin(), order.end(), [&](int x, int y) { return load[x] > load[y]; });
+    int64_t acc[kFanStreams + 1] = {};
+    acc[kFanStreams] = main_load;
     bool forked = false;
-    for (int k = 0; k < no; ++k) {
+    for (int k : order) {
         int best = kFanStreams;
         for (int i = 0; i < kFanStreams; ++i)
-            if (load[i] < load[best]) best = i;
-        load[best] += c->tier_nnz[order[k]] + c->tier_cnt[order[k]];
+            if (acc[i] < acc[best]) best = i;
+        acc[best] += load[k];
         if (best == kFanStreams) continue;
         if (!forked) {
             LV_HIP(hipEventRecord(c->fan_fork, c->stream));
@@ -935,9 +932,22 @@ int fan_tiers(lv_ctx *c, hipStream_t *ts)
             LV_HIP(hipStreamWaitEvent(c->fan[best], c->fan_fork, 0));
             c->fan_used[best] = true;
         }
-        ts[order[k]] = c->fan[best];
+        out[k] = c->fan[best];
     }
     return LV_OK;
+}
+
+// The stream of every sweep tier of a pass.  The tiers write next[] of
+// disjoint vertex sets and read only what the pass does not change, so they
+// need no order among themselves: the hubs stay on the context stream and the
+// other tiers are spread by fan_assign, so small and tail-end launches overlap
+// the big ones.
+int fan_tiers(lv_ctx *c, hipStream_t *ts)
+{
+    int64_t load[kTiers - 1];
+    for (int t = 1; t < kTiers; ++t) load[t - 1] = c->tier_cnt[t] ? c->tier_nnz[t] + c->tier_cnt[t] : 0;
+    ts[0] = ts[kTiers] = c->stream;
+    return fan_assign(c, kTiers - 1, load, c->tier_nnz[kTiers], ts + 1);
 }
 
 int fan_join(lv_ctx *c)
@@ -1229,7 +1239,17 @@ int induce(lv_ctx *c, const int32_t *newc, const int32_t *newid_g, int32_t Kn)
     ca.internal = internal;
     ca.ssum = ssum;
     ca.big_q = seg_q;
-    for (int t = 1; t < kTiers; ++t) launch_cr_agg_tier(s, t, ca, segs + toff[t], (int32_t)tcnt[t]);
+    // the segment tiers are independent of each other and of the big segments'
+    // partitions below: spread over the fan-out streams (joined before the
+    // per-vertex pass reads their sums)
+    {
+        int64_t load[kTiers - 1];
+        hipStream_t st[kTiers - 1];
+        for (int t = 1; t < kTiers; ++t) load[t - 1] = tcnt[t] ? tcnt[t] * (int64_t)kTierMaxQ[t - 1] : 0;
+        LV_TRY(fan_assign(c, kTiers - 1, load, seg_q * (int64_t)tcnt[kTiers], st));
+        for (int t = 1; t < kTiers; ++t)
+            launch_cr_agg_tier(st[t - 1], t, ca, segs + toff[t], (int32_t)tcnt[t]);
+    }
     // segments above seg_q entries: hash partitions
     BigSegArgs B{};
     const int32_t nbs = (int32_t)tcnt[kTiers];
@@ -1296,6 +1316,7 @@ int induce(lv_ctx *c, const int32_t *newc, const int32_t *newid_g, int32_t Kn)
         LV_TRY(arena_alloc(A, B.fbv, 2 * blen));
         LV_TRY(run_cr_big(A, s, ca, B, nbs, nparts, nchunks));
     }
+    LV_TRY(fan_join(c));  // the segment tiers' sums and rows
     // next level: row x = newid_g[g] is [sb[g], sb[g] + nuniq[g]) of uy/uw; the
     // rows keep their segments' places (a segment holds at least its uniques),
     // so no copy packs them
