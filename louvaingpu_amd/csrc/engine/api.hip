@@ -507,6 +507,9 @@ int pack_bits(const lv_ctx *c)
     return bits_for(c->max_units0 > 1 ? c->max_units0 : 1);
 }
 
+int fan_assign(lv_ctx *c, int n, const int64_t *load, int64_t main_load, hipStream_t *out);
+int fan_join(lv_ctx *c);
+
 int build_level0_csr_sorted(lv_ctx *c, int32_t *tself, int32_t *vself, const int64_t *fs,
                             bool canonical, int wbits, const ull *lastloop)
 {
@@ -562,10 +565,18 @@ int build_level0_csr_sorted(lv_ctx *c, int32_t *tself, int32_t *vself, const int
     if (!pk) LV_TRY(alloc_csr_w(c, c->nnz, c->max_units0 < (1ull << 32)));
     c->cpk = pk;
     LV_HIP(hipMemsetAsync(&c->d_flags->err, 0, sizeof(int), s));
-    launch_csr_sorted_fill(s, !canonical, c->d_src, c->d_dst, wunits(c), tscan, vscan, e, n, fbits,
-                           vshift, kb, fs, sk, sp, nr, rs, c->rowptr, ffwd, c->col, c->cw, c->cw32,
-                           pk, c->kdeg, c->d_flags);
+    launch_rowptr_sum(s, rs, fs, vscan, n, c->rowptr, ffwd);
+    {
+        // the two fills may overlap (disjoint entries, atomic row sums)
+        const int64_t load[2] = {nr, e};
+        hipStream_t st[2];
+        LV_TRY(fan_assign(c, 2, load, 0, st));
+        launch_csr_sorted_fill(st[0], st[1], !canonical, c->d_src, c->d_dst, wunits(c), tscan,
+                               vscan, e, n, fbits, vshift, kb, fs, sk, sp, nr, rs, c->rowptr, ffwd,
+                               c->col, c->cw, c->cw32, pk, c->kdeg, c->d_flags);
+    }
     launch_loop_gather(s, lastloop, wunits(c), n, c->loopw);
+    LV_TRY(fan_join(c));
     LV_HIP(hipGetLastError());
     if (canonical) {
         c->rows_unique = true;
@@ -1193,9 +1204,16 @@ int induce(lv_ctx *c, const int32_t *newc, const int32_t *newid_g, int32_t Kn)
     fill_i64(s, (int64_t *)ssum, K, 0);
     fill_i64(s, (int64_t *)internal, K, 0);
     fill_i32(s, nuniq, K, 0);
-    launch_cr_selfl(s, L.perm, c->selfsum, c->g_head, c->g_hscan, n, selfl);
-    launch_cr_gather(s, estart, bound, mstart, c->col, c->cw, c->cw32, newc, n, nnz, ey, ew, ew32,
-                     ep, ewb, c->cpk);
+    {
+        // the self-loop sums and the gather run beside the segment sort below
+        // (joined before the segment tiers read the gather's entries)
+        const int64_t load[2] = {n, nnz};
+        hipStream_t st[2];
+        LV_TRY(fan_assign(c, 2, load, (int64_t)K * 8, st));
+        launch_cr_selfl(st[0], L.perm, c->selfsum, c->g_head, c->g_hscan, n, selfl);
+        launch_cr_gather(st[1], estart, bound, mstart, c->col, c->cw, c->cw32, newc, n, nnz, ey, ew,
+                         ew32, ep, ewb, c->cpk);
+    }
     // segments by length
     int64_t *sb = nullptr, *slen = nullptr;
     uint32_t *tk1 = nullptr, *tk2 = nullptr;
@@ -1239,6 +1257,7 @@ int induce(lv_ctx *c, const int32_t *newc, const int32_t *newid_g, int32_t Kn)
     ca.internal = internal;
     ca.ssum = ssum;
     ca.big_q = seg_q;
+    LV_TRY(fan_join(c));  // the gather's entries
     // the segment tiers are independent of each other and of the big segments'
     // partitions below: spread over the fan-out streams (joined before the
     // per-vertex pass reads their sums)

@@ -486,12 +486,15 @@ void launch_src_sorted(hipStream_t s, const int32_t *src, const int32_t *dst, WU
 void launch_rev_entries(hipStream_t s, const int32_t *src, const int32_t *dst, WUnits w,
                         const int32_t *tscan, int64_t e, int wbits, uint32_t *keys,
                         uint64_t *payload);
-void launch_csr_sorted_fill(hipStream_t s, bool check, const int32_t *src, const int32_t *dst,
-                            WUnits w, const int32_t *tscan, const int32_t *vscan,
-                            int64_t e, int32_t n, int wbits, int vshift, int kb, const int64_t *fs,
-                            const uint32_t *skey, const uint64_t *spay, int64_t nr,
-                            const int64_t *rs, int64_t *rowptr, int64_t *ffwd, int32_t *col,
-                            int64_t *cw, uint32_t *cw32, int pk, int64_t *kdeg, DevFlags *f);
+void launch_rowptr_sum(hipStream_t s, const int64_t *rs, const int64_t *fs, const int32_t *vscan,
+                       int32_t n, int64_t *rowptr, int64_t *ffwd);
+void launch_csr_sorted_fill(hipStream_t srev, hipStream_t sfwd, bool check, const int32_t *src,
+                            const int32_t *dst, WUnits w, const int32_t *tscan,
+                            const int32_t *vscan, int64_t e, int32_t n, int wbits, int vshift,
+                            int kb, const int64_t *fs, const uint32_t *skey, const uint64_t *spay,
+                            int64_t nr, const int64_t *rs, const int64_t *rowptr,
+                            const int64_t *ffwd, int32_t *col, int64_t *cw, uint32_t *cw32, int pk,
+                            int64_t *kdeg, DevFlags *f);
 // One-word reverse entries dst << (64 - kb) | src << (64 - 2 kb) | w, and row
 // starts over such words sorted by their top bits (key = word >> kshift).
 // General path as one word per entry (row | col | w, row = n for self-loops);

@@ -700,34 +700,44 @@ void launch_rev_entries(hipStream_t s, const int32_t *src, const int32_t *dst, W
 
 // Reverse entries as (skey, spay = src << wbits | w), or with skey null as one
 // word spay = row << vshift | src << wbits | w; kb = bits of the largest id.
-void launch_csr_sorted_fill(hipStream_t s, bool check, const int32_t *src, const int32_t *dst,
-                            WUnits w, const int32_t *tscan, const int32_t *vscan,
-                            int64_t e, int32_t n, int wbits, int vshift, int kb, const int64_t *fs,
-                            const uint32_t *skey, const uint64_t *spay, int64_t nr,
-                            const int64_t *rs, int64_t *rowptr, int64_t *ffwd, int32_t *col,
-                            int64_t *cw, uint32_t *cw32, int pk, int64_t *kdeg, DevFlags *f)
+void launch_rowptr_sum(hipStream_t s, const int64_t *rs, const int64_t *fs, const int32_t *vscan,
+                       int32_t n, int64_t *rowptr, int64_t *ffwd)
 {
-    const uint64_t cmask = kb >= 64 ? ~0ull : ((1ull << kb) - 1ull);
     hipLaunchKernelGGL(k_rowptr_sum, grid_for((int64_t)n + 1, 256, 1 << 20), 256, 0, s, rs, fs,
                        vscan, n, rowptr, ffwd);
+}
+
+// The reverse fill on srev and the forward fill on sfwd (after rowptr/ffwd of
+// launch_rowptr_sum): they write disjoint entries and meet only in the atomic
+// row sums of kdeg, so they may run concurrently.
+void launch_csr_sorted_fill(hipStream_t srev, hipStream_t sfwd, bool check, const int32_t *src,
+                            const int32_t *dst, WUnits w, const int32_t *tscan,
+                            const int32_t *vscan, int64_t e, int32_t n, int wbits, int vshift,
+                            int kb, const int64_t *fs, const uint32_t *skey, const uint64_t *spay,
+                            int64_t nr, const int64_t *rs, const int64_t *rowptr,
+                            const int64_t *ffwd, int32_t *col, int64_t *cw, uint32_t *cw32, int pk,
+                            int64_t *kdeg, DevFlags *f)
+{
+    const uint64_t cmask = kb >= 64 ? ~0ull : ((1ull << kb) - 1ull);
     constexpr int SPAN = 256 * kSpanRounds;
     if (nr > 0) {
         if (check)
-            hipLaunchKernelGGL(k_fill_rev<true>, grid_for(nr, SPAN, 8192), 256, 0, s, skey, spay, nr,
-                               wbits, vshift, cmask, rs, rowptr, col, cw, cw32, pk, kdeg, f);
-        else
-            hipLaunchKernelGGL(k_fill_rev<false>, grid_for(nr, SPAN, 1 << 20), 256, 0, s, skey, spay,
+            hipLaunchKernelGGL(k_fill_rev<true>, grid_for(nr, SPAN, 8192), 256, 0, srev, skey, spay,
                                nr, wbits, vshift, cmask, rs, rowptr, col, cw, cw32, pk, kdeg, f);
+        else
+            hipLaunchKernelGGL(k_fill_rev<false>, grid_for(nr, SPAN, 1 << 20), 256, 0, srev, skey,
+                               spay, nr, wbits, vshift, cmask, rs, rowptr, col, cw, cw32, pk, kdeg,
+                               f);
     }
     if (e > 0) {
         if (check)
-            hipLaunchKernelGGL(k_fill_fwd<true>, grid_for(e, kEdgeTile, 8192), 256, 0, s, src, dst,
-                               w, tscan, vscan, e, fs, rs, spay, wbits, cmask, ffwd, col, cw, cw32, pk,
-                               kdeg, f);
+            hipLaunchKernelGGL(k_fill_fwd<true>, grid_for(e, kEdgeTile, 8192), 256, 0, sfwd, src,
+                               dst, w, tscan, vscan, e, fs, rs, spay, wbits, cmask, ffwd, col, cw,
+                               cw32, pk, kdeg, f);
         else
-            hipLaunchKernelGGL(k_fill_fwd<false>, grid_for(e, kEdgeTile, 1 << 20), 256, 0, s, src,
-                               dst, w, tscan, vscan, e, fs, rs, spay, wbits, cmask, ffwd, col, cw, cw32,
-                               pk, kdeg, f);
+            hipLaunchKernelGGL(k_fill_fwd<false>, grid_for(e, kEdgeTile, 1 << 20), 256, 0, sfwd,
+                               src, dst, w, tscan, vscan, e, fs, rs, spay, wbits, cmask, ffwd, col,
+                               cw, cw32, pk, kdeg, f);
     }
 }
 
