@@ -59,25 +59,26 @@ HBM_PEAK_GBS = 8000.0
 # Kernels of each sweep tier as named in rocprofv3 summaries: the first pass
 # of a level (no aggregation) and the later passes.  The last kernel of each
 # list runs once per launch of the tier.  Tiers 5-8 share their first-pass
-# kernel, so their traffic is not separable.
+# kernel, and so do tiers 9-10, so their traffic is not separable.
 TIER_KERNELS = {
     1: (["k_sweep1_group<4,", "k_sweep_group<4, 3, 3,"], None),
     2: (["k_sweep1_group<8,", "k_sweep_group<8, 4, 4,"], None),
     3: (["k_sweep1_group<16,", "k_sweep_group<16, 5, 5,"], None),
     4: (["k_sweep1_group<32,", "k_sweep_group<32, 6, 6,"], None),
     5: None, 6: None, 7: None, 8: None,
-    9: (["k_sweep1_block<256,", "k_sweep_block<256, 12, 11,"], None),
-    10: (["k_sweep1_block<512,", "k_sweep_block<512, 13, 12,"], None),
-    11: (["k_hub1_chunk", "k_hub1_final", "k_hub_count", "k_hub_scatter", "k_hub_agg",
+    9: None, 10: None,
+    11: (["k_sweep1_block<512,", "k_sweep_block<512, 13, 12,"], None),
+    12: (["k_hub1_chunk", "k_hub1_final", "k_hub_count", "k_hub_scatter", "k_hub_agg",
           "k_hub_final"], ["k_hub1_final", "k_hub_final"]),
 }
 TIER_NAMES = {1: "k_sweep1_group<4> / k_sweep_group<4,3,3>", 2: "k_sweep1_group<8> / k_sweep_group<8,4,4>",
               3: "k_sweep1_group<16> / k_sweep_group<16,5,5>", 4: "k_sweep1_group<32> / k_sweep_group<32,6,6>",
               5: "k_sweep1_group<64> / k_sweep_group<64,7,7>", 6: "k_sweep1_group<64> / k_sweep_group<64,8,8>",
               7: "k_sweep1_group<64> / k_sweep_group<64,9,9>", 8: "k_sweep1_group<64> / k_sweep_group<64,10,10>",
-              9: "k_sweep1_block<256> / k_sweep_block<256,12,11>",
-              10: "k_sweep1_block<512> / k_sweep_block<512,13,12>",
-              11: "first pass: k_hub1_chunk + k_hub1_final; later passes: k_hub_count + k_hub_scatter "
+              9: "k_sweep1_block<256> / k_sweep_block<256,11,11>",
+              10: "k_sweep1_block<256> / k_sweep_block<256,12,11>",
+              11: "k_sweep1_block<512> / k_sweep_block<512,13,12>",
+              12: "first pass: k_hub1_chunk + k_hub1_final; later passes: k_hub_count + k_hub_scatter "
                   "+ k_hub_agg + k_hub_final"}
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "louvain_%s_hbm_traffic.txt")
 

@@ -96,7 +96,7 @@ typedef struct {
 
 /* Timing counters, filled when LV_FLAG_TIMING is set (milliseconds of device
  * time between HIP events on the context stream). */
-#define LV_SWEEP_TIERS 11
+#define LV_SWEEP_TIERS 12
 typedef struct {
     double csr_ms, init_ms, sweep_ms, apply_ms, renumber_ms, coarsen_ms, flatten_ms;
     double tier_ms[LV_SWEEP_TIERS];

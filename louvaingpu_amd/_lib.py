@@ -31,7 +31,7 @@ LV_WEIGHT_FLOAT32 = 1
 LV_FLAG_TRACE = 1
 LV_FLAG_TIMING = 2
 
-LV_SWEEP_TIERS = 11
+LV_SWEEP_TIERS = 12
 
 # Every function the C header declares (checked by the test suite).
 ENGINE_SYMBOLS = (

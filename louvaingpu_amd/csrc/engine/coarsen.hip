@@ -546,7 +546,8 @@ static void launch_cr_agg_tier_t(hipStream_t s, int tier, const CoarsenArgs &a,
     case 6: launch_seg(k_cr_agg_group<64, 8, 8, N32>, (ns + 3) / 4, s, a, segs, ns); break;
     case 7: launch_seg(k_cr_agg_group<64, 9, 9, N32>, (ns + 3) / 4, s, a, segs, ns); break;
     case 8: launch_seg(k_cr_agg_group<64, 10, 10, N32>, (ns + 3) / 4, s, a, segs, ns); break;
-    case 9: launch_seg(k_cr_agg_block<12, 11, N32>, ns, s, a, segs, ns); break;
+    case 9: launch_seg(k_cr_agg_block<11, 11, N32>, ns, s, a, segs, ns); break;
+    case 10: launch_seg(k_cr_agg_block<12, 11, N32>, ns, s, a, segs, ns); break;
     default: break;
     }
 }

@@ -28,12 +28,14 @@ typedef unsigned long long ull;
 constexpr int64_t kUnitLimit = int64_t(1) << 60;
 
 // Sweep tiers by q = row length + (1 if the vertex has a self-loop).
-// Tiers 1..10 keep an LDS table per row (tier 10: rows of 2049..4096
-// candidates, one 512-lane block with an 8192-slot table, when hub_q is above
-// 2048); tier 11 is the hub partitions.  Coarsening segments use tiers 1..9
-// and the partitions.
+// Tiers 1..11 keep an LDS table per row (tiers 9 and 10: rows of 513..1024 and
+// 1025..2048 candidates, one 256-lane block with a 2048- or 4096-slot table;
+// tier 11: rows of 2049..4096, one 512-lane block with an 8192-slot table, when
+// hub_q is above 2048); tier 12 is the hub partitions.  Coarsening segments use
+// tiers 1..10 and the partitions.
 constexpr int kTiers = LV_SWEEP_TIERS;
-constexpr int kTierMaxQ[kTiers] = {4, 8, 16, 32, 64, 128, 256, 512, 2048, 4096, 0x7fffffff};
+constexpr int kTierMaxQ[kTiers] = {4,    8,    16,   32,   64,   128,
+                                   256,  512,  1024, 2048, 4096, 0x7fffffff};
 constexpr int32_t kHubQDefault = 4096;
 
 #define LV_TRY(expr)                                \

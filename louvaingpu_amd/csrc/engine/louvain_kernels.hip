@@ -1591,13 +1591,14 @@ static void launch_sweep_tier_t(hipStream_t s, int tier, const SweepArgs &a, con
     case 6: launch_grid(k_sweep_group<64, 8, 8, true, B, WM, N32>, (nv + 3) / 4, 256, s, a, verts, nv); break;
     case 7: launch_grid(k_sweep_group<64, 9, 9, true, B, WM, N32>, (nv + 3) / 4, 256, s, a, verts, nv); break;
     case 8: launch_grid(k_sweep_group<64, 10, 10, false, B, WM, N32>, (nv + 3) / 4, 256, s, a, verts, nv); break;
-    case 9: launch_grid(k_sweep_block<256, 12, 11, B, WM, N32>, nv, 256, s, a, verts, nv); break;
-    case 10: launch_grid(k_sweep_block<512, 13, 12, B, WM, N32>, nv, 512, s, a, verts, nv); break;
+    case 9: launch_grid(k_sweep_block<256, 11, 11, B, WM, N32>, nv, 256, s, a, verts, nv); break;
+    case 10: launch_grid(k_sweep_block<256, 12, 11, B, WM, N32>, nv, 256, s, a, verts, nv); break;
+    case 11: launch_grid(k_sweep_block<512, 13, 12, B, WM, N32>, nv, 512, s, a, verts, nv); break;
     default: break;
     }
 }
 
-// Tiers 1..10 (tier 11, the hubs, runs through run_hub_sweep).
+// Tiers 1..11 (tier 12, the hubs, runs through run_hub_sweep).
 void launch_sweep_tier(hipStream_t s, int tier, const SweepArgs &a, const int32_t *verts, int32_t nv,
                        bool batch)
 {
@@ -1756,8 +1757,9 @@ static void launch_sweep1_tier_t(hipStream_t s, int tier, const SweepArgs &a, co
     case 6:
     case 7:
     case 8: launch_grid(k_sweep1_group<64, B, WM>, (nv + 3) / 4, 256, s, a, verts, nv); break;
-    case 9: launch_grid(k_sweep1_block<256, B, WM>, nv, 256, s, a, verts, nv); break;
-    case 10: launch_grid(k_sweep1_block<512, B, WM>, nv, 512, s, a, verts, nv); break;
+    case 9:
+    case 10: launch_grid(k_sweep1_block<256, B, WM>, nv, 256, s, a, verts, nv); break;
+    case 11: launch_grid(k_sweep1_block<512, B, WM>, nv, 512, s, a, verts, nv); break;
     default: break;
     }
 }
