@@ -463,3 +463,39 @@ def test_run_partitions_delivers_the_dendrogram():
         a, b, f = np.zeros(6, np.int32), np.zeros(6, np.int32), np.zeros(6, np.int32)
         assert eng.run_partitions([a], [b], f) == 1
         assert f.tolist() == list(range(6)) and b.tolist() == list(range(6))
+
+
+@pytest.mark.parametrize("wmax", [(1 << 16) - 1, 1 << 16])
+def test_level0_entry_layout_at_the_32_bit_boundary(wmax):
+    """A single graph's level 0 built from a source-sorted list keeps each row
+    entry as one word col << wb | weight when bits(n - 1) + bits(largest
+    weight) <= 32 (the partition entries and the coarsening's member-order
+    entries of level 0 likewise), else columns and weights apart: n = 2^16
+    with weights up to 2^16 - 1 sits exactly on the packed side (16 + 16 bits),
+    2^16 one bit past it.  Both give the oracle's CSR, passes and levels."""
+    gpu_case("level-0 entry layout, n=2^16, largest weight %d" % wmax)
+    rng = np.random.Generator(np.random.PCG64(29))
+    n = 1 << 16
+    # hubs (rows past the block tiers) over a sparse random graph, canonical
+    # (src < dst) and sorted by (src, dst) so the sorted CSR build runs
+    a = np.concatenate([np.repeat(np.arange(8), 6000), rng.integers(0, n, 400_000)])
+    b = np.concatenate([rng.integers(8, n, 48_000), rng.integers(0, n, 400_000)])
+    lo, hi = np.minimum(a, b), np.maximum(a, b)
+    keep = lo != hi
+    key = np.unique(lo[keep].astype(np.int64) * n + hi[keep])
+    src = (key // n).astype(np.int32)
+    dst = (key % n).astype(np.int32)
+    w = rng.integers(1, 50, len(src)).astype(np.int32)
+    w[rng.integers(0, len(w), 64)] = wmax  # the largest weight sets the layout
+    r = oracle.run(n, src, dst, w, trace=True)
+    with Engine(trace=True) as eng:
+        eng.load_edges(n, src, dst, w)
+        eng.CalculateNeighbour()
+        rp, col, cw = eng.level_csr()
+        orp, ocol, ocw = r.csr(0)
+        gk, gw = merged_rows(n, rp, col, cw)
+        ok, ow = merged_rows(n, orp, ocol, ocw)
+        assert np.array_equal(gk, ok) and np.array_equal(gw, ow)
+    eng, r = run_both(n, src, dst, w)
+    with eng:
+        assert_run_matches(eng, r, n)
