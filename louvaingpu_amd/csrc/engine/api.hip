@@ -772,6 +772,7 @@ int plan_hubs(lv_ctx *c, int32_t nh)
     launch_hub_setup(s, hv, nh, c->rowptr, row_end(c), c->selfsum, c->hub_part_target, c->chunk_mul, hq, hP,
                      hP64, hch, hpc);
     h.chunk_mul = c->chunk_mul;
+    h.small_tables = c->hub_part_target <= kHubAggSmallTarget ? 1 : 0;
     LV_TRY(scan_exclusive_i64(A, s, hq, ebase, nh, ebase + nh));
     LV_TRY(scan_exclusive_i64(A, s, hP64, pbase, nh, pbase + nh));
     LV_TRY(scan_exclusive_i64(A, s, hch, cbase, nh, cbase + nh));
@@ -1290,6 +1291,7 @@ int induce(lv_ctx *c, const int32_t *newc, const int32_t *newid_g, int32_t Kn)
         launch_cr_big_setup(s, bsegs, nbs, slen, c->hub_part_target, c->chunk_mul, bP, bP64, bch,
                             blen64, bpc, nuniq);
         B.chunk_mul = c->chunk_mul;
+        B.small_tables = c->hub_part_target <= kHubAggSmallTarget ? 1 : 0;
         LV_TRY(scan_exclusive_i64(A, s, bP64, pbase, nbs, pbase + nbs));
         LV_TRY(scan_exclusive_i64(A, s, bch, cbase, nbs, cbase + nbs));
         LV_TRY(scan_exclusive_i64(A, s, blen64, blscan, nbs, blscan + nbs));

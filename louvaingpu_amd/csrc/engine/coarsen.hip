@@ -736,17 +736,17 @@ __device__ __forceinline__ int table_to_row(int32_t *key, const V *val, int cap,
     return total;
 }
 
-template <bool N32>
+template <bool N32, int L2>
 __global__ __launch_bounds__(256) void k_cr_big_agg(CoarsenArgs a, BigSegArgs B)
 {
-    constexpr int CAP = 1 << kHubAggLog2;
+    constexpr int CAP = 1 << L2;
     __shared__ __align__(16) int32_t s_key[CAP];
     __shared__ TabV<N32> s_val[CAP];
     const int tid = threadIdx.x;
     const int64_t pi = blockIdx.x;
     const int64_t beg = B.poff[pi], end = B.poff[pi + 1];
     const int32_t g = B.bsegs[B.part_big[pi]];  // (big segments are never stopped groups)
-    const int l2 = (end - beg) * 2 >= CAP ? kHubAggLog2 : table_log2(end - beg, 6);
+    const int l2 = (end - beg) * 2 >= CAP ? L2 : table_log2(end - beg, 6);
     const int cap = 1 << l2;
     for (int s = tid; s < cap; s += 256) {
         s_key[s] = -1;
@@ -830,10 +830,12 @@ int run_cr_big(Arena &A, hipStream_t s, const CoarsenArgs &a, const BigSegArgs &
     hipLaunchKernelGGL(k_cr_big_scatter, (int)nchunks, 256, 0, s, a, B);
     // the partitions add their uniques into nuniq of their segment (zeroed by
     // the setup) and write them into the segment's row
-    if (B.n32)  // narrow level
-        hipLaunchKernelGGL(k_cr_big_agg<true>, (int)nparts, 256, 0, s, a, B);
+    if (B.n32 && B.small_tables)  // narrow level, partitions of about 1024 entries
+        hipLaunchKernelGGL((k_cr_big_agg<true, kHubAggLog2Small>), (int)nparts, 256, 0, s, a, B);
+    else if (B.n32)  // narrow level
+        hipLaunchKernelGGL((k_cr_big_agg<true, kHubAggLog2>), (int)nparts, 256, 0, s, a, B);
     else
-        hipLaunchKernelGGL(k_cr_big_agg<false>, (int)nparts, 256, 0, s, a, B);
+        hipLaunchKernelGGL((k_cr_big_agg<false, kHubAggLog2>), (int)nparts, 256, 0, s, a, B);
     LV_HIP(hipGetLastError());
     return LV_OK;
 }

@@ -186,7 +186,9 @@ constexpr int kHubChunk = 4096;
 constexpr int kChunkMul = 32;  // entries per partition of a chunk (0: kHubChunk fixed)
 constexpr int kHubPartTarget = 2048;  // default (lv_config / tuning table override)
 constexpr int kHubMaxParts = 2048;
-constexpr int kHubAggLog2 = 12;  // 4096 LDS slots per partition block
+constexpr int kHubAggLog2 = 12;  // 4096 LDS slots per partition block, or
+constexpr int kHubAggLog2Small = 11;  // 2048 (16 KiB) when the partition target is 1024 or less
+constexpr int kHubAggSmallTarget = 1024;
 
 struct HubArgs {
     const int32_t *hv;        // hub vertex of hub j
@@ -215,6 +217,7 @@ struct HubArgs {
     int32_t *pn;
     const int64_t *cbase;     // first chunk of hub j (exclusive scan of chunks)
     int chunk_mul;            // chunk_span(P, chunk_mul) entries per chunk
+    int small_tables;         // partitions aggregate in 2048-slot tables
     double *cg;               // per-chunk best of the first-pass sweep
     int32_t *cc;
     int64_t *cw;
@@ -261,6 +264,7 @@ struct BigSegArgs {
     int32_t *fbk;           // fallback tables
     ull *fbv;
     int chunk_mul;          // chunk_span(P, chunk_mul) entries per chunk
+    int small_tables;       // partitions aggregate in 2048-slot tables
 };
 
 // ---------------------------------------------------------------------------

@@ -2099,10 +2099,10 @@ __global__ __launch_bounds__(256) void k_hub_scatter(SweepArgs a, HubArgs h)
 }
 
 
-template <bool B, bool N32>
+template <bool B, bool N32, int L2>
 __global__ __launch_bounds__(256) void k_hub_agg(SweepArgs a, HubArgs h)
 {
-    constexpr int CAP = 1 << kHubAggLog2;
+    constexpr int CAP = 1 << L2;
     __shared__ __align__(16) int32_t s_key[CAP];
     __shared__ TabV<N32> s_val[CAP];
     const int tid = threadIdx.x;
@@ -2114,7 +2114,7 @@ __global__ __launch_bounds__(256) void k_hub_agg(SweepArgs a, HubArgs h)
     const double f = (double)a.kdeg[v] / vp.m2;
     // 2 slots per entry up to the LDS capacity; a full table (possible only
     // with more than CAP/2 entries) sends the partition to its global table
-    const int l2 = (end - beg) * 2 >= CAP ? kHubAggLog2 : table_log2(end - beg, 6);
+    const int l2 = (end - beg) * 2 >= CAP ? L2 : table_log2(end - beg, 6);
     const int cap = 1 << l2;
     for (int s = tid; s < cap; s += 256) {
         s_key[s] = -1;
@@ -2234,17 +2234,22 @@ int run_hub_sweep(Arena &A, hipStream_t s, const SweepArgs &a, const HubArgs &h,
         hipLaunchKernelGGL(k_hub_scatter<0>, (int)nchunks, 256, 0, s, a, h);
     // a partition's sums stay below a narrow level's total degree: 4-byte table
     const bool n32 = a.lab8 != nullptr;
+    constexpr int LB = kHubAggLog2, LS = kHubAggLog2Small;
     if (batch) {
-        if (n32)
-            hipLaunchKernelGGL((k_hub_agg<true, true>), (int)nparts, 256, 0, s, a, h);
+        if (n32 && h.small_tables)
+            hipLaunchKernelGGL((k_hub_agg<true, true, LS>), (int)nparts, 256, 0, s, a, h);
+        else if (n32)
+            hipLaunchKernelGGL((k_hub_agg<true, true, LB>), (int)nparts, 256, 0, s, a, h);
         else
-            hipLaunchKernelGGL((k_hub_agg<true, false>), (int)nparts, 256, 0, s, a, h);
+            hipLaunchKernelGGL((k_hub_agg<true, false, LB>), (int)nparts, 256, 0, s, a, h);
         hipLaunchKernelGGL(k_hub_final<true>, (int)((nh + 3) / 4), 256, 0, s, a, h, nh);
     } else {
-        if (n32)
-            hipLaunchKernelGGL((k_hub_agg<false, true>), (int)nparts, 256, 0, s, a, h);
+        if (n32 && h.small_tables)
+            hipLaunchKernelGGL((k_hub_agg<false, true, LS>), (int)nparts, 256, 0, s, a, h);
+        else if (n32)
+            hipLaunchKernelGGL((k_hub_agg<false, true, LB>), (int)nparts, 256, 0, s, a, h);
         else
-            hipLaunchKernelGGL((k_hub_agg<false, false>), (int)nparts, 256, 0, s, a, h);
+            hipLaunchKernelGGL((k_hub_agg<false, false, LB>), (int)nparts, 256, 0, s, a, h);
         hipLaunchKernelGGL(k_hub_final<false>, (int)((nh + 3) / 4), 256, 0, s, a, h, nh);
     }
     LV_HIP(hipGetLastError());
