@@ -5,7 +5,8 @@
  * share one mutable StatusCUDA/CUDAGraph pair (LouvainCUDA/LouvainCPP.cpp:129-134)
  * plus AllocateMemory/DeAllocateMemory (LouvainCUDA/LouvainCPP.cpp:136-205).
  * This header replaces that boundary with plain C: an opaque context owns one
- * HBM arena, one HIP stream and the per-level device state; every function
+ * HBM arena, one HIP stream (plus three it fans independent launches out to,
+ * see lv_config.fanout) and the per-level device state; every function
  * takes plain pointers and sizes and returns an lv_status code.  No exception,
  * abort or host allocation crosses the ABI: host output buffers belong to the
  * caller and the library copies into them.
@@ -80,6 +81,13 @@ typedef struct {
      * what does not encode is copied as it is), 1 = plain copies, 2 = encoded
      * at any size.  Results do not depend on it. */
     int32_t upload;
+    /* Extra HIP streams for the independent launches of one phase (the sweep
+     * tiers of a pass, the coarsening's segment tiers and gathers, the two
+     * halves of the level-0 CSR fill): 0 = when the phase holds at least 2^22
+     * entries, 1 = never, 2 = always.  LV_FLAG_TIMING and a caller's stream
+     * (lv_run_static) keep every launch on one stream.  Results do not depend
+     * on it. */
+    int32_t fanout;
 } lv_config;
 
 /* Per-level summary (the reference keeps only Partition{count,...}). */

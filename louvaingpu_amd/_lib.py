@@ -71,6 +71,7 @@ class LvConfig(ct.Structure):
         ("hub_part_target", ct.c_int32),
         ("chunk_mul", ct.c_int32),
         ("upload", ct.c_int32),
+        ("fanout", ct.c_int32),
     ]
 
 

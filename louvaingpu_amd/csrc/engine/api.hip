@@ -56,6 +56,11 @@ struct PendingTime {
 
 // Extra streams a pass's independent sweep tiers are spread over.
 constexpr int kFanStreams = 3;
+// Fan-out pays from about this many entries in a phase: below it the fork and
+// join events cost more than the overlap gains (device-resident runs, never /
+// always: SBM 3000 2.36 / 3.13 ms, R-MAT 16 2.62 / 2.97, R-MAT 18 2.06 / 2.08,
+// R-MAT 20 4.00 / 3.76, R-MAT 22 12.53 / 12.10).
+constexpr int64_t kFanMinLoad = (int64_t)1 << 22;
 
 struct lv_ctx {
     lv_config cfg;
@@ -911,12 +916,18 @@ int init_level(lv_ctx *c)
 // stream (already carrying main_load) has the least so far; a fan-out stream
 // that gets one waits on a fork event recorded on the context stream, and
 // fan_join makes the context stream wait for it again.  Every launch stays on
-// the context stream in timing mode (per-phase events) and on a borrowed
-// stream.
+// the context stream in timing mode (per-phase events), on a borrowed stream,
+// with lv_config.fanout = 1, and (fanout = 0) when the phase's loads sum to
+// less than kFanMinLoad.
 int fan_assign(lv_ctx *c, int n, const int64_t *load, int64_t main_load, hipStream_t *out)
 {
     for (int k = 0; k < n; ++k) out[k] = c->stream;
-    if (timing(c) || c->borrowed || n == 0) return LV_OK;
+    if (timing(c) || c->borrowed || n == 0 || c->cfg.fanout == 1) return LV_OK;
+    if (c->cfg.fanout == 0) {
+        int64_t total = main_load;
+        for (int k = 0; k < n; ++k) total += load[k];
+        if (total < kFanMinLoad) return LV_OK;
+    }
     if (!c->fan_fork) {
         for (int i = 0; i < kFanStreams; ++i) {
             LV_HIP(hipStreamCreateWithFlags(&c->fan[i], hipStreamNonBlocking));
@@ -1821,7 +1832,7 @@ bool tunables_valid(const lv_config &c)
     return (c.hub_q == 0 || (c.hub_q >= 64 && c.hub_q <= kTierMaxQ[kTiers - 2])) &&
            (c.hub_part_target == 0 || (c.hub_part_target >= 64 && c.hub_part_target <= (1 << 30))) &&
            (c.chunk_mul == 0 || (c.chunk_mul >= 1 && c.chunk_mul <= 1024)) && c.upload >= 0 &&
-           c.upload <= 2;
+           c.upload <= 2 && c.fanout >= 0 && c.fanout <= 2;
 }
 
 struct TuneRow {

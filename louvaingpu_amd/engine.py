@@ -29,9 +29,10 @@ def _p64(a):
 
 def make_config(gamma=1.0, min_gain=MIN, max_passes=0, max_levels=0, device=0, trace=False,
                 timing=False, arena_bytes=0, hub_q=0, hub_part_target=0, chunk_mul=0,
-                upload=0) -> LvConfig:
+                upload=0, fanout=0) -> LvConfig:
     """lv_config; hub_q / hub_part_target / chunk_mul = 0 take the tuning table;
-    upload: 0 encoded from 2^18 edges, 1 plain copies, 2 encoded at any size."""
+    upload: 0 encoded from 2^18 edges, 1 plain copies, 2 encoded at any size;
+    fanout: 0 extra streams for phases of 2^22 entries or more, 1 never, 2 always."""
     cfg = LvConfig()
     _lib.engine().lv_config_default(ct.byref(cfg))
     cfg.gamma = float(gamma)
@@ -45,6 +46,7 @@ def make_config(gamma=1.0, min_gain=MIN, max_passes=0, max_levels=0, device=0, t
     cfg.hub_part_target = int(hub_part_target)
     cfg.chunk_mul = int(chunk_mul)
     cfg.upload = int(upload)
+    cfg.fanout = int(fanout)
     return cfg
 
 

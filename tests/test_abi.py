@@ -59,6 +59,7 @@ def test_strerror_and_defaults():
     cfg = _lib.LvConfig()
     _lib.engine().lv_config_default(ct.byref(cfg))
     assert cfg.gamma == 1.0 and cfg.min_gain == 1e-7 and cfg.max_passes == 0
+    assert cfg.upload == 0 and cfg.fanout == 0
 
 
 def test_null_context_is_rejected():
@@ -76,7 +77,8 @@ def test_bad_config_is_rejected_before_device_use():
     h = ct.c_void_p()
     assert lib.lv_create(ct.byref(h), ct.byref(cfg)) == _lib.LV_ERR_ARG
     for field, bad in (("hub_q", 63), ("hub_q", 4097), ("hub_part_target", 10),
-                       ("chunk_mul", 2000), ("upload", 3), ("upload", -1)):
+                       ("chunk_mul", 2000), ("upload", 3), ("upload", -1), ("fanout", 3),
+                       ("fanout", -1)):
         lib.lv_config_default(ct.byref(cfg))
         assert cfg.hub_q == cfg.hub_part_target == cfg.chunk_mul == 0
         setattr(cfg, field, bad)
