@@ -437,7 +437,7 @@ def e2e_runs(eng, n, hsrc, hdst, hw, steps, warmup, label):
                        "results_ms": parts[2] * 1e3 / k}, levels
 
 
-def measure_graph(wname, graph, steps, warmup, traffic_key):
+def measure_graph(wname, graph, steps, warmup, traffic_key, fanout=0):
     """End-to-end and device-resident timing of one graph, with the dominant
     sweep tier's roofline.  Returns the measurement dict and the engine's final
     partition and modularity."""
@@ -450,7 +450,7 @@ def measure_graph(wname, graph, steps, warmup, traffic_key):
     # the caller's edge list in page-locked host memory (filled outside the steps)
     hsrc, hdst, hw = pinned_empty(e, np.int32), pinned_empty(e, np.int32), pinned_empty(e, w.dtype)
     hsrc[:], hdst[:], hw[:] = src, dst, w
-    with Engine() as eng:
+    with Engine(fanout=fanout) as eng:
         e2e_s, e2e_parts, levels = e2e_runs(eng, n, hsrc, hdst, hw, steps, warmup, wname)
         # the same runs with the input already resident in HBM (the last load's)
         t0 = time.perf_counter()
@@ -490,7 +490,8 @@ def measure_graph(wname, graph, steps, warmup, traffic_key):
     m = {
         "config": {"workload": wname, "n": n, "e": e, "nnz": nnz0, "levels": levels,
                    "passes": [i.passes for i in infos], "level_n": [i.n for i in infos],
-                   "level_nnz": [i.nnz for i in infos], "parallelism": "single-gpu"},
+                   "level_nnz": [i.nnz for i in infos], "parallelism": "single-gpu",
+                   "fanout": fanout},
         "value": nnz0 / e2e_s,
         "ms_per_step": e2e_s * 1e3,
         "e2e_parts_ms": e2e_parts,
@@ -562,7 +563,8 @@ def rmat26f_object(args):
     graph = generators.rmat26_float()
     gen_s = time.perf_counter() - t0
     print("[bench] rmat26f generated in %.1f s" % gen_s, file=sys.stderr, flush=True)
-    m, final, q = measure_graph("rmat26_ef16_float_seed2", graph, args.rmat26_steps, 1, "rmat26f")
+    m, final, q = measure_graph("rmat26_ef16_float_seed2", graph, args.rmat26_steps, 1, "rmat26f",
+                                args.fanout)
     with open(os.path.join(ROOT, "tests", "golden", "rmat26f_oracle.json")) as f:
         fx = json.load(f)
     m["validated"] = (hashlib.sha256(np.ascontiguousarray(final).tobytes()).hexdigest()
@@ -601,6 +603,9 @@ def main():
                     help="graphs of the per-graph loop sample (each at all resolutions)")
     ap.add_argument("--batch-chunk-edges", type=int, default=300_000_000,
                     help="union edges (graph edges x resolutions) per graph set")
+    ap.add_argument("--fanout", type=int, default=0, choices=(0, 1, 2),
+                    help="lv_config.fanout of the end-to-end and device-resident runs "
+                         "(1 = one stream, so a kernel trace shows each kernel's own time)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -619,7 +624,8 @@ def main():
     nnz0 = 2 * len(src)
     print("[bench] workload %s n=%d e=%d nnz=%d" % (wname, n, len(src), nnz0), file=sys.stderr,
           flush=True)
-    m, final, q_gpu = measure_graph(wname, graph, args.steps, args.warmup, args.workload)
+    m, final, q_gpu = measure_graph(wname, graph, args.steps, args.warmup, args.workload,
+                                    args.fanout)
     out = {
         "metric": "Louvain TEPS & end-to-end s at oracle modularity; batched vs per-graph",
         "value": m.pop("value"),
