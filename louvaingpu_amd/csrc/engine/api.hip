@@ -2,6 +2,7 @@
 // the device-resident level loop.  See include/louvain_amd.h.
 #include <algorithm>
 #include <chrono>
+#include <initializer_list>
 #include <climits>
 #include <cstdio>
 #include <cstdlib>
@@ -61,6 +62,11 @@ constexpr int kFanStreams = 3;
 // always: SBM 3000 2.36 / 3.13 ms, R-MAT 16 2.62 / 2.97, R-MAT 18 2.06 / 2.08,
 // R-MAT 20 4.00 / 3.76, R-MAT 22 12.53 / 12.10).
 constexpr int64_t kFanMinLoad = (int64_t)1 << 22;
+// Mailbox of fetch_small: bytes, first data word, and how long the host polls
+// before it waits through the runtime instead.
+constexpr size_t kMboxBytes = 64 << 10;
+constexpr int kMboxData = 16;
+constexpr int kMboxSpinUs = 400;
 
 struct lv_ctx {
     lv_config cfg;
@@ -72,6 +78,11 @@ struct lv_ctx {
     QAcc *d_qacc = nullptr, *h_qacc = nullptr;
     QAcc *d_qpart = nullptr;  // per-block partials of the Q reductions
     long long *d_small = nullptr, *h_small = nullptr;  // 64 slots
+    // small results fetched through k_mailbox: coherent page-locked words,
+    // [0] the flag, data from word kMboxData (h_ and d_ are the host and
+    // device addresses of the same memory)
+    uint32_t *h_mbox = nullptr, *d_mbox = nullptr;
+    uint32_t mbox_seq = 0;
     // input
     bool loaded = false;
     int32_t n0 = 0;
@@ -260,6 +271,78 @@ int sync(lv_ctx *c)
 {
     LV_HIP(hipStreamSynchronize(c->stream));
     if (timing(c)) t_resolve(c);
+    return LV_OK;
+}
+
+// One host buffer filled from device memory by fetch_small.
+struct HostSeg {
+    void *host;
+    const void *dev;
+    size_t bytes;
+};
+
+template <typename T>
+HostSeg seg(T *host, const T *dev, int64_t count)
+{
+    return HostSeg{host, dev, sizeof(T) * (size_t)count};
+}
+
+// Waits for the mailbox flag to read seq: polls for up to kMboxSpinUs, then
+// leaves the wait (and any device error) to the runtime.
+int mbox_wait(lv_ctx *c, uint32_t seq)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int i = 1;; ++i) {
+        if (__atomic_load_n(c->h_mbox, __ATOMIC_ACQUIRE) == seq) return LV_OK;
+        if ((i & 63) == 0 && std::chrono::steady_clock::now() - t0 >
+                                 std::chrono::microseconds(kMboxSpinUs))
+            break;
+        __builtin_ia32_pause();
+    }
+    LV_HIP(hipStreamSynchronize(c->stream));
+    if (__atomic_load_n(c->h_mbox, __ATOMIC_ACQUIRE) != seq) return LV_ERR_DEVICE;
+    return LV_OK;
+}
+
+// Small device values to host buffers after everything queued on the context
+// stream, returning when they are there (the stream has then drained up to
+// the copy).  A mailbox kernel writes them into coherent page-locked memory
+// and raises a flag the host polls: a round trip of about 9 us against 14 us
+// for a runtime copy and a stream synchronisation.  Timing mode (its events
+// are read at the synchronisation), a borrowed stream and larger requests take
+// the runtime copy.
+int fetch_small(lv_ctx *c, std::initializer_list<HostSeg> segs)
+{
+    size_t total = 0;
+    bool words = segs.size() <= (size_t)kMboxSegs;
+    for (const HostSeg &g : segs) {
+        total += g.bytes;
+        words = words && g.bytes % 4 == 0 && ((uintptr_t)g.dev & 3) == 0;
+    }
+    if (timing(c) || c->borrowed || !c->h_mbox || !words ||
+        total > kMboxBytes - 4 * (size_t)kMboxData) {
+        for (const HostSeg &g : segs)
+            if (g.bytes)
+                LV_HIP(hipMemcpyAsync(g.host, g.dev, g.bytes, hipMemcpyDeviceToHost, c->stream));
+        return sync(c);
+    }
+    MboxArgs a{};
+    uint32_t off = 0;
+    for (const HostSeg &g : segs) {
+        a.seg[a.nseg++] = MboxSeg{(const uint32_t *)g.dev, off, (uint32_t)(g.bytes / 4)};
+        off += (uint32_t)(g.bytes / 4);
+    }
+    a.seq = ++c->mbox_seq;
+    a.box = c->d_mbox + kMboxData;
+    a.flag = c->d_mbox;
+    launch_mailbox(c->stream, a);
+    LV_HIP(hipGetLastError());
+    LV_TRY(mbox_wait(c, a.seq));
+    const char *box = (const char *)(c->h_mbox + kMboxData);
+    for (const HostSeg &g : segs) {
+        memcpy(g.host, box, g.bytes);
+        box += g.bytes;
+    }
     return LV_OK;
 }
 
@@ -463,8 +546,7 @@ double current_q(lv_ctx *c, int &rc)
 {
     rc = LV_OK;
     launch_qagg(c->stream, c->n, c->deg, c->in, c->d_qpart, c->d_qacc);
-    rc = d2h(c, c->h_qacc, c->d_qacc, 1);
-    if (rc == LV_OK) rc = sync(c);
+    rc = fetch_small(c, {seg(c->h_qacc, c->d_qacc, 1)});
     if (rc != LV_OK) return 0.0;
     return q_from_aggregates(c->h_qacc->sum_in, c->h_qacc->d2_lo, c->h_qacc->d2_hi, c->m,
                              c->cfg.gamma);
@@ -528,8 +610,7 @@ int build_level0_csr_sorted(lv_ctx *c, int32_t *tself, int32_t *vself, const int
     LV_TRY(scan_exclusive_i32(A, s, tself, tscan, ntiles, tscan + ntiles));
     LV_TRY(scan_exclusive_i32(A, s, vself, vscan, n, vscan + n));
     int32_t nself = 0;
-    LV_TRY(d2h(c, &nself, (const int32_t *)(tscan + ntiles), 1));
-    LV_TRY(sync(c));
+    LV_TRY(fetch_small(c, {seg(&nself, (const int32_t *)(tscan + ntiles), 1)}));
     const int64_t nr = e - nself;
     const int kb = bits_for(n > 0 ? (uint64_t)(n - 1) : 0);
     // one word per reverse entry (dst | src | w) when the weight fits the bits
@@ -586,8 +667,7 @@ int build_level0_csr_sorted(lv_ctx *c, int32_t *tself, int32_t *vself, const int
     if (canonical) {
         c->rows_unique = true;
     } else {
-        LV_TRY(d2h(c, c->h_flags, c->d_flags, 1));
-        LV_TRY(sync(c));
+        LV_TRY(fetch_small(c, {seg(c->h_flags, c->d_flags, 1)}));
         c->rows_unique = (c->h_flags->err & 4) == 0;
     }
     arena_free(A, k1);
@@ -641,8 +721,7 @@ int build_level0_csr(lv_ctx *c)
         launch_src_sorted(s, c->d_src, c->d_dst, wunits(c), e, n, tself, vself, fs, c->kdeg,
                           c->selfsum, lastloop, c->d_flags);
         LV_HIP(hipGetLastError());
-        LV_TRY(d2h(c, c->h_flags, c->d_flags, 1));
-        LV_TRY(sync(c));
+        LV_TRY(fetch_small(c, {seg(c->h_flags, c->d_flags, 1)}));
         const int err = c->h_flags->err;
         if ((err & 8) == 0) {
             const int rc =
@@ -675,8 +754,7 @@ int build_level0_csr(lv_ctx *c)
         launch_csr_entries1(s, c->d_src, c->d_dst, wunits(c), e, n, bits, cb, x1);
         LV_TRY(radix_sort_u64_keys(A, s, x1, x2, total, 64 - bits, &sx));
         launch_row_starts64(s, sx, total, n, 64 - bits, c->rowptr);
-        LV_TRY(d2h(c, &c->h_small[0], (const long long *)(c->rowptr + n), 1));
-        LV_TRY(sync(c));
+        LV_TRY(fetch_small(c, {seg(&c->h_small[0], (const long long *)(c->rowptr + n), 1)}));
         c->nnz = c->h_small[0];
         LV_TRY(arena_alloc(A, c->col, c->nnz));
         LV_TRY(alloc_csr_w(c, c->nnz, c->max_units0 < (1ull << 32)));
@@ -684,8 +762,7 @@ int build_level0_csr(lv_ctx *c)
         launch_csr_unpack1(s, sx, c->nnz, bits, cb, c->col, c->cw, c->cw32, c->kdeg, c->d_flags);
         launch_loop_gather(s, lastloop, wunits(c), n, c->loopw);
         LV_HIP(hipGetLastError());
-        LV_TRY(d2h(c, c->h_flags, c->d_flags, 1));
-        LV_TRY(sync(c));
+        LV_TRY(fetch_small(c, {seg(c->h_flags, c->d_flags, 1)}));
         c->rows_unique = (c->h_flags->err & 4) == 0;
         t_end(c, t0, CAT_CSR);
         arena_free(A, lastloop);
@@ -714,8 +791,7 @@ int build_level0_csr(lv_ctx *c)
         LV_TRY(radix_sort_u32_u32(A, s, k1, v1, k2, v2, total, bits, &sk, &sv));
     }
     launch_rowptr_search(s, sk, total, n, c->rowptr);
-    LV_TRY(d2h(c, &c->h_small[0], (const long long *)(c->rowptr + n), 1));
-    LV_TRY(sync(c));
+    LV_TRY(fetch_small(c, {seg(&c->h_small[0], (const long long *)(c->rowptr + n), 1)}));
     c->nnz = c->h_small[0];
     LV_TRY(arena_alloc(A, c->col, c->nnz));
     LV_TRY(alloc_csr_w(c, c->nnz, c->max_units0 < (1ull << 32)));
@@ -728,8 +804,7 @@ int build_level0_csr(lv_ctx *c)
     LV_HIP(hipMemsetAsync(&c->d_flags->err, 0, sizeof(int), s));
     launch_rows_check(s, sk, c->col, c->nnz, c->d_flags);
     LV_HIP(hipGetLastError());
-    LV_TRY(d2h(c, c->h_flags, c->d_flags, 1));
-    LV_TRY(sync(c));
+    LV_TRY(fetch_small(c, {seg(c->h_flags, c->d_flags, 1)}));
     c->rows_unique = (c->h_flags->err & 4) == 0;
     t_end(c, t0, CAT_CSR);
     arena_free(A, lastloop);
@@ -782,11 +857,10 @@ int plan_hubs(lv_ctx *c, int32_t nh)
     LV_TRY(scan_exclusive_i64(A, s, hP64, pbase, nh, pbase + nh));
     LV_TRY(scan_exclusive_i64(A, s, hch, cbase, nh, cbase + nh));
     LV_TRY(scan_exclusive_i64(A, s, hpc, hbase, nh, hbase + nh));
-    LV_TRY(d2h(c, c->h_small, (const long long *)(ebase + nh), 1));
-    LV_TRY(d2h(c, c->h_small + 1, (const long long *)(pbase + nh), 1));
-    LV_TRY(d2h(c, c->h_small + 2, (const long long *)(cbase + nh), 1));
-    LV_TRY(d2h(c, c->h_small + 3, (const long long *)(hbase + nh), 1));
-    LV_TRY(sync(c));
+    LV_TRY(fetch_small(c, {seg(c->h_small, (const long long *)(ebase + nh), 1),
+                           seg(c->h_small + 1, (const long long *)(pbase + nh), 1),
+                           seg(c->h_small + 2, (const long long *)(cbase + nh), 1),
+                           seg(c->h_small + 3, (const long long *)(hbase + nh), 1)}));
     c->hub_entries = c->h_small[0];
     c->hub_parts = c->h_small[1];
     c->hub_chunks = c->h_small[2];
@@ -873,8 +947,7 @@ int init_level(lv_ctx *c)
     uint32_t *ko = nullptr;
     int32_t *vo = nullptr;
     LV_TRY(radix_sort_u32_i32(A, s, tkey, v1, tkey2, v2, n, 4, &ko, &vo));
-    LV_TRY(d2h(c, c->h_small, c->d_small, 2 * (kTiers + 1) + 1));
-    LV_TRY(sync(c));
+    LV_TRY(fetch_small(c, {seg(c->h_small, c->d_small, 2 * (kTiers + 1) + 1)}));
     {
         // size deltas share the degree word when |delta_size| <= n < 2^(S-1) and
         // |delta_deg| <= total degree leave the packed sum below 2^63
@@ -1072,8 +1145,7 @@ int one_level(lv_ctx *c, double lower, double *q_out, int32_t *passes_out)
                      c->narrow ? c->deg32 : nullptr, c->d_qpart, c->d_qacc);
         t_end(c, t1, CAT_APPLY);
         LV_HIP(hipGetLastError());
-        LV_TRY(d2h(c, c->h_qacc, c->d_qacc, 1));
-        LV_TRY(sync(c));
+        LV_TRY(fetch_small(c, {seg(c->h_qacc, c->d_qacc, 1)}));
         p_resolve(c);
         const double q_new = q_from_aggregates(c->h_qacc->sum_in, c->h_qacc->d2_lo,
                                                c->h_qacc->d2_hi, c->m, c->cfg.gamma);
@@ -1130,8 +1202,7 @@ int renumber(lv_ctx *c, int32_t *k_out)
     launch_cr_groups(s, c->g_head, c->g_hscan, perm, n, c->g_start, lflag);
     LV_TRY(scan_exclusive_i32(A, s, lflag, rank, n, nullptr));
     int32_t kk = 0;
-    LV_TRY(d2h(c, &kk, (const int32_t *)c->d_small, 1));
-    LV_TRY(sync(c));
+    LV_TRY(fetch_small(c, {seg(&kk, (const int32_t *)c->d_small, 1)}));
     LV_TRY(arena_alloc(A, c->g_newid, kk));
     LV_TRY(arena_alloc(A, L.newc, n));
     launch_cr_newid(s, c->g_start, perm, rank, kk, c->g_newid);
@@ -1244,8 +1315,7 @@ int induce(lv_ctx *c, const int32_t *newc, const int32_t *newid_g, int32_t Kn)
     uint32_t *tko = nullptr;
     int32_t *segs = nullptr;
     LV_TRY(radix_sort_u32_i32(A, s, tk1, sid1, tk2, sid2, K, 4, &tko, &segs));
-    LV_TRY(d2h(c, c->h_small, c->d_small, kTiers + 1));
-    LV_TRY(sync(c));
+    LV_TRY(fetch_small(c, {seg(c->h_small, c->d_small, kTiers + 1)}));
     int64_t tcnt[kTiers + 1], toff[kTiers + 1], off = 0;
     for (int t = 0; t <= kTiers; ++t) {
         tcnt[t] = c->h_small[t];
@@ -1307,11 +1377,10 @@ int induce(lv_ctx *c, const int32_t *newc, const int32_t *newid_g, int32_t Kn)
         LV_TRY(scan_exclusive_i64(A, s, bch, cbase, nbs, cbase + nbs));
         LV_TRY(scan_exclusive_i64(A, s, blen64, blscan, nbs, blscan + nbs));
         LV_TRY(scan_exclusive_i64(A, s, bpc, bhbase, nbs, bhbase + nbs));
-        LV_TRY(d2h(c, c->h_small, (const long long *)(pbase + nbs), 1));
-        LV_TRY(d2h(c, c->h_small + 1, (const long long *)(cbase + nbs), 1));
-        LV_TRY(d2h(c, c->h_small + 2, (const long long *)(blscan + nbs), 1));
-        LV_TRY(d2h(c, c->h_small + 3, (const long long *)(bhbase + nbs), 1));
-        LV_TRY(sync(c));
+        LV_TRY(fetch_small(c, {seg(c->h_small, (const long long *)(pbase + nbs), 1),
+                               seg(c->h_small + 1, (const long long *)(cbase + nbs), 1),
+                               seg(c->h_small + 2, (const long long *)(blscan + nbs), 1),
+                               seg(c->h_small + 3, (const long long *)(bhbase + nbs), 1)}));
         nparts = c->h_small[0];
         nchunks = c->h_small[1];
         blen = c->h_small[2];  // entries of the big segments
@@ -1363,8 +1432,7 @@ int induce(lv_ctx *c, const int32_t *newc, const int32_t *newid_g, int32_t Kn)
     fill_i64(s, (int64_t *)c->d_small + 1, 2, 0);
     launch_cr_vertex(s, newid_g, ssum, internal, selfl, sb, nuniq, K, loop2, self2, kdeg2,
                      rowptr2, rowend2, c->d_small + 1, c->d_small + 2);
-    LV_TRY(d2h(c, c->h_small + 1, c->d_small + 1, 2));
-    LV_TRY(sync(c));
+    LV_TRY(fetch_small(c, {seg(c->h_small + 1, c->d_small + 1, 2)}));
     const int64_t m2 = c->h_small[1] / 2;
     const int64_t nnz2 = c->h_small[2];
     LV_HIP(hipGetLastError());
@@ -1547,8 +1615,7 @@ int batch_level_setup(lv_ctx *c, int32_t level)
         fill_i64(s, (int64_t *)c->d_ccnt, nc, 0);
         launch_b_vertex_sum(s, c->kdeg, c->loopw, c->vcomp, c->n, c->d_ccnt);
         LV_HIP(hipGetLastError());
-        LV_TRY(d2h(c, c->h_ccnt, c->d_ccnt, nc));
-        LV_TRY(sync(c));
+        LV_TRY(fetch_small(c, {seg(c->h_ccnt, c->d_ccnt, nc)}));
         for (int32_t k = 0; k < nc; ++k)
             if (c->comp_voff[k + 1] > c->comp_voff[k]) c->comp_m[k] = (int64_t)(c->h_ccnt[k] / 2);
     }
@@ -1571,8 +1638,7 @@ int batch_qagg(lv_ctx *c, bool apply)
     launch_b_qagg(c->stream, c->n, c->deg, c->in, c->size, c->ddeg, c->dsize, c->size_shift,
                   c->narrow ? c->deg32 : nullptr, c->vcomp, apply, c->d_cq);
     LV_HIP(hipGetLastError());
-    LV_TRY(d2h(c, c->h_cq, c->d_cq, c->ncomp));
-    return sync(c);
+    return fetch_small(c, {seg(c->h_cq, c->d_cq, c->ncomp)});
 }
 
 double batch_q(const lv_ctx *c, int32_t k)
@@ -1743,8 +1809,7 @@ int batch_run(lv_ctx *c)
         fill_i64(s, (int64_t *)c->d_ccnt, nc, 0);
         launch_b_groups(s, c->g_start, L.perm, c->vcomp, L.k, c->d_ccnt);
         LV_HIP(hipGetLastError());
-        LV_TRY(d2h(c, c->h_ccnt, c->d_ccnt, nc));
-        LV_TRY(sync(c));
+        LV_TRY(fetch_small(c, {seg(c->h_ccnt, c->d_ccnt, nc)}));
         L.voff_h = c->comp_voff;
         L.koff_h.assign((size_t)nc + 1, 0);
         for (int32_t k = 0; k < nc; ++k) {
@@ -1992,6 +2057,9 @@ int lv_create(lv_ctx **out, const lv_config *cfg)
         if (hipHostMalloc(&c->h_flags, sizeof(DevFlags), hipHostMallocDefault) != hipSuccess) { rc = LV_ERR_NOMEM; break; }
         if (hipHostMalloc(&c->h_qacc, sizeof(QAcc), hipHostMallocDefault) != hipSuccess) { rc = LV_ERR_NOMEM; break; }
         if (hipHostMalloc(&c->h_small, 64 * sizeof(long long), hipHostMallocDefault) != hipSuccess) { rc = LV_ERR_NOMEM; break; }
+        if (hipHostMalloc(&c->h_mbox, kMboxBytes, hipHostMallocCoherent) != hipSuccess) { rc = LV_ERR_NOMEM; break; }
+        memset(c->h_mbox, 0, kMboxBytes);
+        if (hipHostGetDevicePointer((void **)&c->d_mbox, c->h_mbox, 0) != hipSuccess) { rc = LV_ERR_DEVICE; break; }
         if (c->cfg.arena_bytes) rc = c->arena.reserve(c->cfg.arena_bytes);
     } while (0);
     if (rc != LV_OK) {
@@ -2028,6 +2096,7 @@ int lv_destroy(lv_ctx *c)
     if (c->h_flags) (void)hipHostFree(c->h_flags);
     if (c->h_qacc) (void)hipHostFree(c->h_qacc);
     if (c->h_small) (void)hipHostFree(c->h_small);
+    if (c->h_mbox) (void)hipHostFree(c->h_mbox);
     if (c->h_cq) (void)hipHostFree(c->h_cq);
     if (c->h_ccnt) (void)hipHostFree(c->h_ccnt);
     if (c->stream) (void)hipStreamDestroy(c->stream);

@@ -292,6 +292,23 @@ int radix_sort_u32_u64(Arena &a, hipStream_t s, uint32_t *keys, uint64_t *vals, 
                        uint64_t **vals_out);
 void fill_i32(hipStream_t s, int32_t *p, int64_t n, int32_t v);
 void fill_i64(hipStream_t s, int64_t *p, int64_t n, int64_t v);
+// Small results to the host without a runtime copy: k_mailbox copies up to
+// kMboxSegs device word ranges into a page-locked buffer and then sets its
+// flag word to seq (system-scope release).
+constexpr int kMboxSegs = 6;
+struct MboxSeg {
+    const uint32_t *src;
+    uint32_t off;    // first word in box
+    uint32_t words;
+};
+struct MboxArgs {
+    MboxSeg seg[kMboxSegs];
+    int nseg;
+    uint32_t seq;
+    uint32_t *box;   // device address of the data words
+    uint32_t *flag;  // device address of the flag word
+};
+void launch_mailbox(hipStream_t s, const MboxArgs &a);
 void iota_i32(hipStream_t s, int32_t *p, int64_t n);
 void copy_i32(hipStream_t s, int32_t *dst, const int32_t *src, int64_t n);
 void sum_i64(hipStream_t s, const int64_t *p, int64_t n, int64_t *out_dev);  // *out += sum

@@ -215,6 +215,25 @@ void fill_i64(hipStream_t s, int64_t *p, int64_t n, int64_t v)
     if (n > 0) hipLaunchKernelGGL(k_fill<int64_t>, grid_for(n, 256), 256, 0, s, p, n, v);
 }
 
+// Mailbox: the segments' words into page-locked host memory, each thread's
+// stores released at system scope, then the flag word set to seq, which the
+// host polls.  One block; the segments are a few hundred words at most.
+__global__ __launch_bounds__(256) void k_mailbox(MboxArgs a)
+{
+    for (int k = 0; k < a.nseg; ++k) {
+        const MboxSeg g = a.seg[k];
+        for (uint32_t i = threadIdx.x; i < g.words; i += 256) a.box[g.off + i] = g.src[i];
+    }
+    __threadfence_system();
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_store(a.flag, a.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+void launch_mailbox(hipStream_t s, const MboxArgs &a)
+{
+    hipLaunchKernelGGL(k_mailbox, 1, 256, 0, s, a);
+}
+
 void iota_i32(hipStream_t s, int32_t *p, int64_t n)
 {
     if (n > 0) hipLaunchKernelGGL(k_iota, grid_for(n, 256), 256, 0, s, p, n);
