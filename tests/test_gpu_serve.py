@@ -118,11 +118,13 @@ def test_bad_requests_get_their_own_status():
 def test_poisson_stream_of_the_serving_workload():
     """Requests drawn as in the serving workload (SBMs with n log-uniform in
     [100, 50000], seed 4, plus R-MAT scale-20 float graphs) arriving as a
-    Poisson stream at 200 requests/s on two workers: every result equals the
-    request run alone."""
+    Poisson stream at 200 requests/s on two workers, with a burst of four
+    requests at once every 16 (so that sets of several requests form however
+    fast the workers are): every result equals the request run alone."""
     rng = np.random.Generator(np.random.PCG64(4))
     nreq = 80
     gaps = rng.exponential(1.0 / 200.0, nreq)
+    gaps[np.arange(nreq) % 16 >= 13] = 0.0  # requests 13-15 of every 16 follow 12 at once
     params = generators.batch_params(nreq, seed=4)
     rmat = [generators.rmat(20, 16, 0.57, 0.19, 0.19, sd, "float") for sd in (4, 5)]
     reqs = [generators.sbm(n, 100, p, p / 50, sd) for n, p, sd in params]
