@@ -346,6 +346,10 @@ int fetch_small(lv_ctx *c, std::initializer_list<HostSeg> segs)
     return LV_OK;
 }
 
+// Returns when everything queued on the context stream has run (host buffers
+// that queued copies read may then go), through the mailbox like fetch_small.
+int barrier(lv_ctx *c) { return fetch_small(c, {}); }
+
 // Timing: the candidate counts P of a pass's tiers (the 8*P term of the
 // local-moving bytes) are summed per tier into d_small[kPSlot + t] and fetched
 // with the pass's other results.
@@ -1626,7 +1630,7 @@ int batch_level_setup(lv_ctx *c, int32_t level)
     }
     LV_TRY(h2d(c, c->d_cm2, (const double *)m2.data(), nc));
     LV_TRY(h2d(c, c->d_cgam, (const double *)gam.data(), nc));
-    LV_TRY(sync(c));  // the host vectors go out of scope
+    LV_TRY(barrier(c));  // the host vectors go out of scope
     arena_free(A, dvoff);
     return LV_OK;
 }
@@ -1850,7 +1854,7 @@ int batch_run(lv_ctx *c)
         LV_TRY(h2d(c, L.koff, (const int64_t *)L.koff_h.data(), (int64_t)nc + 1));
         LV_TRY(h2d(c, L.vnext, (const int64_t *)vnext.data(), nc));
         if (kn == 0) {
-            LV_TRY(sync(c));
+            LV_TRY(barrier(c));
             break;
         }
         if (kn > INT32_MAX) return LV_ERR_RANGE;
@@ -1864,7 +1868,7 @@ int batch_run(lv_ctx *c)
         arena_free(A, newid_next);
         arena_free(A, newc_next);
         c->comp_voff = voff_next;
-        LV_TRY(sync(c));  // vnext / voff_next host buffers
+        LV_TRY(barrier(c));  // vnext / voff_next host buffers
     }
     // flatten every component through its own levels
     hipEvent_t t0 = t_begin(c);
@@ -1881,7 +1885,7 @@ int batch_run(lv_ctx *c)
                          c->levels[l].koff, c->levels[l].vnext);
     LV_HIP(hipGetLastError());
     t_end(c, t0, CAT_FLATTEN);
-    LV_TRY(sync(c));
+    LV_TRY(barrier(c));
     arena_free(A, dclev);
     c->final_valid = true;
     free_level_arrays(c);
@@ -2389,7 +2393,7 @@ int lv_run(lv_ctx *c, int32_t *levels_out)
         mod = q;
     }
     LV_TRY(flatten_final(c));
-    LV_TRY(sync(c));
+    LV_TRY(barrier(c));
     // the last level's working arrays are not part of the result
     free_level_arrays(c);
     c->state = ST_DONE;  // finished run: only result queries are valid
