@@ -1094,11 +1094,20 @@ int one_level(lv_ctx *c, double lower, double *q_out, int32_t *passes_out)
     double curQ = -1.0;  // LouvainCUDA/OneLevelKernel.cu:528
     int32_t pass = 0;
     double q_level = 0.0;
+    // With min_gain > 0 the second pass is always the last: within a level
+    // sum(in) stays the sum of the self-loops and sum(deg^2) over any
+    // partition is at least its all-singletons value, so Q after the first
+    // pass cannot exceed Q before it and its break test cannot pass.  The first
+    // pass's Q is then fetched with the second's (slot 1 of d_qacc), one host
+    // round trip per level fewer; the break test is still evaluated on it.
+    const bool defer = !c->borrowed && !timing(c) && c->cfg.min_gain > 0 &&
+                       (c->cfg.max_passes == 0 || c->cfg.max_passes >= 2);
+    bool deferred = false;  // pass 1's Q waits in slot 1
     for (;;) {
         ++pass;
         // The break test of pass p compares Q of the state before p's deltas
         // (OneLevelKernel.cu:875-892), known before the sweep runs.
-        const bool last = (q_prev - curQ) < c->cfg.min_gain ||
+        const bool last = deferred || (q_prev - curQ) < c->cfg.min_gain ||
                           (c->cfg.max_passes > 0 && pass >= c->cfg.max_passes);
         a.C = c->C;
         a.next = c->next;
@@ -1144,13 +1153,27 @@ int one_level(lv_ctx *c, double lower, double *q_out, int32_t *passes_out)
             LV_TRY(sync(c));
             c->cur_trace.push_back(std::move(nx));
         }
+        const bool hold = defer && pass == 1 && !last;
         hipEvent_t t1 = t_begin(c);
         launch_apply(s, c->n, c->deg, c->in, c->size, c->ddeg, c->dsize, c->size_shift,
-                     c->narrow ? c->deg32 : nullptr, c->d_qpart, c->d_qacc);
+                     c->narrow ? c->deg32 : nullptr, c->d_qpart, c->d_qacc + (hold ? 1 : 0));
         t_end(c, t1, CAT_APPLY);
         LV_HIP(hipGetLastError());
-        LV_TRY(fetch_small(c, {seg(c->h_qacc, c->d_qacc, 1)}));
+        if (hold) {
+            c->cur_pass_q.push_back(q_prev);
+            std::swap(c->C, c->next);
+            curQ = q_prev;
+            if (curQ < lower) curQ = lower;  // :894-896
+            deferred = true;
+            continue;
+        }
+        LV_TRY(fetch_small(c, {seg(c->h_qacc, c->d_qacc, deferred ? 2 : 1)}));
         p_resolve(c);
+        if (deferred) {
+            q_prev = q_from_aggregates(c->h_qacc[1].sum_in, c->h_qacc[1].d2_lo,
+                                       c->h_qacc[1].d2_hi, c->m, c->cfg.gamma);
+            if (!((q_prev - curQ) < c->cfg.min_gain)) return LV_ERR_DEVICE;  // cannot happen
+        }
         const double q_new = q_from_aggregates(c->h_qacc->sum_in, c->h_qacc->d2_lo,
                                                c->h_qacc->d2_hi, c->m, c->cfg.gamma);
         c->cur_pass_q.push_back(q_prev);
@@ -2055,11 +2078,11 @@ int lv_create(lv_ctx **out, const lv_config *cfg)
         if (hipSetDevice(c->device) != hipSuccess) { rc = LV_ERR_DEVICE; break; }
         if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { rc = LV_ERR_DEVICE; break; }
         if (hipMalloc(&c->d_flags, sizeof(DevFlags)) != hipSuccess) { rc = LV_ERR_NOMEM; break; }
-        if (hipMalloc(&c->d_qacc, sizeof(QAcc)) != hipSuccess) { rc = LV_ERR_NOMEM; break; }
+        if (hipMalloc(&c->d_qacc, 2 * sizeof(QAcc)) != hipSuccess) { rc = LV_ERR_NOMEM; break; }
         if (hipMalloc(&c->d_qpart, sizeof(QAcc) * kQParts) != hipSuccess) { rc = LV_ERR_NOMEM; break; }
         if (hipMalloc(&c->d_small, 64 * sizeof(long long)) != hipSuccess) { rc = LV_ERR_NOMEM; break; }
         if (hipHostMalloc(&c->h_flags, sizeof(DevFlags), hipHostMallocDefault) != hipSuccess) { rc = LV_ERR_NOMEM; break; }
-        if (hipHostMalloc(&c->h_qacc, sizeof(QAcc), hipHostMallocDefault) != hipSuccess) { rc = LV_ERR_NOMEM; break; }
+        if (hipHostMalloc(&c->h_qacc, 2 * sizeof(QAcc), hipHostMallocDefault) != hipSuccess) { rc = LV_ERR_NOMEM; break; }
         if (hipHostMalloc(&c->h_small, 64 * sizeof(long long), hipHostMallocDefault) != hipSuccess) { rc = LV_ERR_NOMEM; break; }
         if (hipHostMalloc(&c->h_mbox, kMboxBytes, hipHostMallocCoherent) != hipSuccess) { rc = LV_ERR_NOMEM; break; }
         memset(c->h_mbox, 0, kMboxBytes);
