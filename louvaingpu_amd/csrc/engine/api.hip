@@ -1659,13 +1659,18 @@ int batch_level_setup(lv_ctx *c, int32_t level)
 }
 
 // Exact Q aggregates per component (apply: add the pass deltas first).
-int batch_qagg(lv_ctx *c, bool apply)
+// Per-component Q terms into d_cq + slot * ncomp; fetch = 1 brings slot 0 to
+// h_cq, fetch = 2 also slot 1 (a deferred first pass) to h_cq + ncomp.
+int batch_qagg(lv_ctx *c, bool apply, int slot = 0, int fetch = 1)
 {
-    fill_i64(c->stream, (int64_t *)c->d_cq, 4 * (int64_t)c->ncomp, 0);
+    QAcc *out = c->d_cq + (int64_t)slot * c->ncomp;
+    fill_i64(c->stream, (int64_t *)out, 4 * (int64_t)c->ncomp, 0);
     launch_b_qagg(c->stream, c->n, c->deg, c->in, c->size, c->ddeg, c->dsize, c->size_shift,
-                  c->narrow ? c->deg32 : nullptr, c->vcomp, apply, c->d_cq);
+                  c->narrow ? c->deg32 : nullptr, c->vcomp, apply, out);
     LV_HIP(hipGetLastError());
-    return fetch_small(c, {seg(c->h_cq, c->d_cq, c->ncomp)});
+    if (fetch == 2) return fetch_small(c, {seg(c->h_cq, c->d_cq, 2 * (int64_t)c->ncomp)});
+    if (fetch == 1) return fetch_small(c, {seg(c->h_cq, c->d_cq, c->ncomp)});
+    return LV_OK;
 }
 
 double batch_q(const lv_ctx *c, int32_t k)
@@ -1685,7 +1690,15 @@ int batch_one_level(lv_ctx *c, const std::vector<double> &lower, const std::vect
     LV_TRY(batch_qagg(c, false));
     std::vector<double> q_prev((size_t)nc, 0.0), curQ((size_t)nc, -1.0);
     std::vector<uint8_t> done((size_t)nc, 1);
-    std::vector<int32_t> mode((size_t)nc, 0);
+    // one mode array per pass parity: a deferred first pass leaves its copy
+    // queued while the second pass fills the other
+    std::vector<int32_t> modes[2] = {std::vector<int32_t>((size_t)nc, 0),
+                                     std::vector<int32_t>((size_t)nc, 0)};
+    // as in one_level: with min_gain > 0 the second pass is every component's
+    // last, so the first pass's Q terms wait on the device (slot 1 of d_cq)
+    const bool defer = !timing(c) && c->cfg.min_gain > 0 &&
+                       (c->cfg.max_passes == 0 || c->cfg.max_passes >= 2);
+    bool deferred = false;
     q_level.assign((size_t)nc, 0.0);
     passes.assign((size_t)nc, 0);
     for (int32_t k = 0; k < nc; ++k) {
@@ -1718,7 +1731,8 @@ int batch_one_level(lv_ctx *c, const std::vector<double> &lower, const std::vect
     a.pcnt = c->pcnt;
     int32_t pass = 0;
     for (;;) {
-        bool any = false;
+        bool any = false, any_last = false;
+        std::vector<int32_t> &mode = modes[pass & 1];
         for (int32_t k = 0; k < nc; ++k) {
             if (done[k]) {
                 mode[k] = 0;
@@ -1726,9 +1740,10 @@ int batch_one_level(lv_ctx *c, const std::vector<double> &lower, const std::vect
             }
             any = true;
             passes[k] += 1;
-            const bool last = (q_prev[k] - curQ[k]) < c->cfg.min_gain ||
+            const bool last = deferred || (q_prev[k] - curQ[k]) < c->cfg.min_gain ||
                               (c->cfg.max_passes > 0 && passes[k] >= c->cfg.max_passes);
             mode[k] = last ? 2 : 1;
+            any_last = any_last || last;
         }
         if (!any) break;
         ++pass;
@@ -1769,13 +1784,30 @@ int batch_one_level(lv_ctx *c, const std::vector<double> &lower, const std::vect
             t_end(c, td, CAT_APPLY);
         }
         LV_HIP(hipGetLastError());
+        const bool hold = defer && pass == 1 && !any_last;
         hipEvent_t t1 = t_begin(c);
-        LV_TRY(batch_qagg(c, true));  // synchronises: mode[] may be rewritten below
+        LV_TRY(batch_qagg(c, true, hold ? 1 : 0, hold ? 0 : deferred ? 2 : 1));
         t_end(c, t1, CAT_APPLY);
+        if (hold) {
+            std::swap(c->C, c->next);
+            for (int32_t k = 0; k < nc; ++k) {
+                if (mode[k] == 0) continue;
+                curQ[k] = q_prev[k];
+                if (curQ[k] < lower[k]) curQ[k] = lower[k];
+            }
+            deferred = true;
+            continue;
+        }
         p_resolve(c);
         std::swap(c->C, c->next);
         for (int32_t k = 0; k < nc; ++k) {
             if (mode[k] == 0) continue;
+            if (deferred) {
+                const QAcc &d = c->h_cq[nc + k];
+                q_prev[k] = q_from_aggregates(d.sum_in, d.d2_lo, d.d2_hi, c->comp_m[k],
+                                              c->comp_gamma[k]);
+                if (!((q_prev[k] - curQ[k]) < c->cfg.min_gain)) return LV_ERR_DEVICE;  // cannot happen
+            }
             const double q_new = batch_q(c, k);
             if (mode[k] == 2) {
                 q_level[k] = q_new;
@@ -1800,11 +1832,11 @@ int batch_run(lv_ctx *c)
     Arena &A = c->arena;
     hipStream_t s = c->stream;
     const int32_t nc = c->ncomp;
-    LV_TRY(pinned_cap(c, nc));
+    LV_TRY(pinned_cap(c, 2 * nc));
     LV_TRY(arena_alloc(A, c->d_cmode, nc));
     LV_TRY(arena_alloc(A, c->d_cm2, nc));
     LV_TRY(arena_alloc(A, c->d_cgam, nc));
-    LV_TRY(arena_alloc(A, c->d_cq, nc));
+    LV_TRY(arena_alloc(A, c->d_cq, 2 * (int64_t)nc));  // slot 1: a deferred first pass
     LV_TRY(arena_alloc(A, c->d_ccnt, nc));
     c->comp_rec.assign((size_t)nc, {});
     c->comp_voff.assign((size_t)nc + 1, 0);
